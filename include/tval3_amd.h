@@ -1,0 +1,215 @@
+/*
+ * tval3_amd.h — C-ABI of the MI355X TVAL3 reconstruction engine.
+ *
+ * Plain C: POD structs, plain pointers and sizes, int status codes, no C++
+ * exceptions and no torch types across the boundary.  Every entry point that
+ * takes vector arguments takes DEVICE pointers and is asynchronous on the
+ * context's HIP stream unless stated otherwise; geometry descriptions are
+ * host arrays (they are turned into device-resident plans once).
+ *
+ * What each entry point replaces in the reference (modk/tval3-cpu):
+ *   tv3_solve / tv3_solve_batch  <- tval3_cpu_2d / tval3_cpu_3d
+ *                                   (2d/tval3_cpu_2d.cpp:752-886, 889-1016;
+ *                                    3d/tval3_cpu_3d.cpp:1172-1378, 1384-1580)
+ *   tv3_plan_dense               <- the mat<T> overloads of gemv
+ *                                   (2d/mat_mult.h:15-36, 3d/mat_mult.h:17-45)
+ *   tv3_plan_sparse              <- the sparse_mat<T> overloads of gemv
+ *                                   (2d/mat_mult.h:38-91, 3d/mat_mult.h:47-118)
+ *   tv3_plan_geometry            <- the geometry overload of gemv
+ *                                   (3d/mat_mult.h:148-595; the 2-D copy at
+ *                                    2d/mat_mult.h:107-194 is commented out)
+ *   tv3_gemv                     <- gemv(CBLAS_TRANSPOSE, const T_A&, x, y)
+ *   tv3_D / tv3_Dt               <- D / Dt (2d/tval3_cpu_2d.cpp:268-293,
+ *                                   3d/tval3_cpu_3d.cpp:365-425)
+ *   tv3_shrink                   <- shrinkage (2d/tval3_cpu_2d.cpp:588-611,
+ *                                   3d/tval3_cpu_3d.cpp:834-871)
+ *   tv3_dot / tv3_nrm2           <- cblas_dotc / cblas_nrm2 wrappers
+ *                                   (2d/blas_wrap.h:71-112)
+ *   tv3_options / tv3_info       <- tval3_options<T> / tval3_info<T> (types
+ *                                   from the reference's tval3_types.h, which
+ *                                   the reference tree does not ship)
+ *
+ * Reduction semantics (both in this library and in the CPU oracle): every
+ * dot product, norm and matrix-vector sum accumulates in FP64 and is rounded
+ * once to the result type the reference uses (float for cblas_sdot/snrm2,
+ * T_scal for the shrinkage sum).  "x + a*y" updates are single-rounding FMAs.
+ */
+#ifndef TVAL3_AMD_H_
+#define TVAL3_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TV3_OK 0
+#define TV3_ERR_ARG 1      /* argument error (reference: tval3_exception) */
+#define TV3_ERR_HIP 2      /* HIP runtime error */
+#define TV3_ERR_ALLOC 3    /* device allocation failed */
+#define TV3_ERR_NOGPU 4    /* no HIP device */
+
+/* ---- element types of a dense A ---------------------------------------- */
+#define TV3_F32 0
+#define TV3_BF16 1
+
+/* ---- gemv transpose flags (CBLAS_TRANSPOSE) ----------------------------- */
+#define TV3_NOTRANS 0
+#define TV3_TRANS 1        /* CblasTrans and CblasConjTrans (real data) */
+
+/* Solver options: the 18 fields of the reference's tval3_options<T>
+ * (used at 2d/tval3_cpu_2d.cpp:163-192, 540-569, 769-852), stored as double.
+ * The last three fields are engine settings with no reference counterpart. */
+typedef struct tv3_options {
+  double mu, beta;            /* final penalty parameters */
+  double mu0, beta0;          /* initial penalty parameters (continuation) */
+  double tol;                 /* outer stop: RelChgOut < tol */
+  double tol_inn;             /* inner stop: RelChg < tol_inn */
+  int maxit;                  /* cap on total inner iterations */
+  int maxcnt;                 /* cap on outer iterations */
+  double gam;                 /* nonmonotone weight */
+  double gamma;               /* backtracking factor */
+  double rate_gam;            /* gam shrink factor on a failed backtrack */
+  double rate_cnt;            /* continuation rate for mu and beta */
+  double c;                   /* Armijo constant */
+  int nonneg;                 /* clamp U >= 0 after each descent step */
+  int isreal;                 /* keep the real part (no-op for real data) */
+  unsigned numInitIterations; /* A^T-skipping state machine */
+  double skipMulRatio;        /* 0 disables A^T skipping */
+  double maxRelativeChange;
+  int scal_double;            /* 0: T_scal = float, 1: T_scal = double */
+  int tv_iso;                 /* 0: anisotropic TV (reference), 1: isotropic */
+  int dt_exact;               /* 3-D only. 0: the reference's Dt (its y-term
+                                 is not the adjoint of D for dim_z >= 2),
+                                 1: the exact adjoint of D */
+} tv3_options;
+
+/* Result record: the reference's tval3_info<T> (2d/tval3_cpu_2d.cpp:865-876). */
+typedef struct tv3_info {
+  double secs;
+  int outer_iters;
+  int total_iters;
+  double rel_chg;
+  double rel_error;
+} tv3_info;
+
+/* One record per inner iteration (no reference counterpart: the reference
+ * keeps no per-iteration history).  Written after the inner stop test. */
+typedef struct tv3_trace_rec {
+  int iter;        /* count_total after this iteration */
+  int outer;       /* count_outer during this iteration */
+  int backtracks;  /* Armijo backtracking steps taken (0..5) */
+  int fallback;    /* 1 if the 5-strike fallback re-descended */
+  double tau, alpha, f, C, rel_chg;
+  double lam1, lam2, lam3, lam4, lam5;
+  double mu, beta;
+} tv3_trace_rec;
+
+/* Ray geometry: the reference's `geometry` type as used by the live
+ * projector (3d/mat_mult.h:164-232).  Host arrays.  Per-MP coordinate arrays
+ * are flattened MP-major: x_emitters[mp * num_emitters + e].
+ * use_path[e * num_receivers + r] = 0 skips the pair, k > 0 maps it to
+ * output row mp * num_paths + k - 1. */
+typedef struct tv3_geometry {
+  int num_mps;
+  int num_emitters;
+  int num_receivers;
+  int num_paths;
+  const int *use_path;
+  const int *x_emitters, *y_emitters, *z_emitters;
+  const int *x_receivers, *y_receivers, *z_receivers;
+  int rv_x, rv_y, rv_z;
+  float scale_factor;
+} tv3_geometry;
+
+typedef struct tv3_ctx tv3_ctx;
+typedef struct tv3_plan tv3_plan;
+
+/* ---- defaults ------------------------------------------------------------ */
+/* Fills the reference's option fields with the public TVAL3 defaults.  The
+ * reference's own defaults live in a header it does not ship. */
+void tv3_options_default(tv3_options *o);
+
+/* ---- context --------------------------------------------------------------- */
+/* stream: a hipStream_t, or NULL for a stream owned by the context. */
+int tv3_ctx_create(int device, void *stream, tv3_ctx **out);
+void tv3_ctx_destroy(tv3_ctx *ctx);
+const char *tv3_ctx_error(const tv3_ctx *ctx);
+int tv3_ctx_sync(tv3_ctx *ctx);
+void *tv3_ctx_stream(const tv3_ctx *ctx);
+/* Per-launch HIP-event timing of the A and A^T products issued by the solver
+ * on the context stream (no reference counterpart: it replaces the
+ * reference's PROFILING gemv slots 10/11, 3d/mat_mult.h:19-44).  enable
+ * resets the totals.  profile_read returns, for [A, A^T], the summed
+ * milliseconds and the number of launches. */
+int tv3_ctx_profile(tv3_ctx *ctx, int enable);
+int tv3_ctx_profile_read(tv3_ctx *ctx, double *ms, long long *count);
+
+/* ---- measurement-operator plans ------------------------------------------ */
+/* Dense A, row-major m x n, on the device (f32 or bf16).  Borrowed. */
+int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,
+                   tv3_plan **out);
+/* CSR and CSC copies of one m x n matrix, zero-based, device pointers,
+ * borrowed (the reference requires format == sparse_mat_both). */
+int tv3_plan_sparse(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                    const int *csr_ind, const float *csr_val,
+                    const int *csc_ptr, const int *csc_ind,
+                    const float *csc_val, tv3_plan **out);
+/* On-the-fly Bresenham projector.  Builds per-ray walk descriptors and the
+ * deterministic voxel-major transpose once. */
+int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out);
+/* Frees the plan's own device arrays; valid after its context is destroyed. */
+void tv3_plan_destroy(tv3_plan *plan);
+/* m = rows (measurements), n = columns (pixels), nnz = stored or walked
+ * entries (m*n for dense). */
+int tv3_plan_shape(const tv3_plan *plan, long long *m, long long *n,
+                   long long *nnz);
+/* Geometry plans: copies the voxel index list of every ray, rays in output
+ * order, voxels in walk order, into host arrays ptr[m+1] and lin[nnz]. */
+int tv3_plan_walk(tv3_ctx *ctx, const tv3_plan *plan, long long *ptr,
+                  int *lin);
+
+/* ---- operators ------------------------------------------------------------ */
+/* y = A x (trans = TV3_NOTRANS) or y = A^T x (TV3_TRANS); overwrites y. */
+int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
+             float *y);
+/* Periodic forward differences of U (dim_x contiguous).  Uz may be NULL
+ * when dim_z == 1. */
+int tv3_D(tv3_ctx *ctx, int dim_x, int dim_y, int dim_z, const float *U,
+          float *Ux, float *Uy, float *Uz);
+/* res = Dt(X, Y, Z); dt_exact as in tv3_options. */
+int tv3_Dt(tv3_ctx *ctx, int dim_x, int dim_y, int dim_z, int dt_exact,
+           const float *X, const float *Y, const float *Z, float *res);
+/* W = shrink(U_dir - sigma_dir / beta, 1 / beta); returns lam1 (host). */
+int tv3_shrink(tv3_ctx *ctx, int dim_x, int dim_y, int dim_z, double beta,
+               int tv_iso, int scal_double, const float *Ux, const float *Uy,
+               const float *Uz, const float *sx, const float *sy,
+               const float *sz, float *Wx, float *Wy, float *Wz,
+               double *lam1);
+/* FP64-accumulated dot product and 2-norm (host results, synchronous). */
+int tv3_dot(tv3_ctx *ctx, long long n, const float *x, const float *y,
+            double *out);
+int tv3_nrm2(tv3_ctx *ctx, long long n, const float *x, double *out);
+
+/* ---- solver ---------------------------------------------------------------- */
+/* One reconstruction (tval3_cpu_3d semantics; dim_z == 1 gives exactly the
+ * tval3_cpu_2d semantics).  b: m floats, U: dim_x*dim_y*dim_z floats (output;
+ * its input values are ignored, as in the reference), Ut: NULL or the ground
+ * truth for info->rel_error.  trace: NULL or trace_cap host records.
+ * Synchronous. */
+int tv3_solve(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+              int dim_x, int dim_y, int dim_z, const float *b, float *U,
+              const float *Ut, tv3_info *info, tv3_trace_rec *trace,
+              int trace_cap);
+/* B independent reconstructions advanced in lockstep by one set of batched
+ * kernels.  A[i] may repeat (shared geometry); sizes may differ.  dims is
+ * B x 3 (dim_x, dim_y, dim_z).  Ut may be NULL or contain NULL entries.
+ * Synchronous. */
+int tv3_solve_batch(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                    const tv3_options *opts, const int *dims,
+                    const float *const *b, float *const *U,
+                    const float *const *Ut, tv3_info *infos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TVAL3_AMD_H_ */

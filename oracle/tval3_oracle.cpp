@@ -1,0 +1,872 @@
+/*
+ * tval3_oracle.cpp — CPU oracle for the TVAL3 inner loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this file; only tests/, the smoke check and the CPU-baseline leg of
+ * the benchmark do, and there only as the checker.
+ *
+ * It restates, step by step, the reference solver of modk/tval3-cpu:
+ *   scaleb        2d/tval3_cpu_2d.cpp:228-266   (3d:311-360)
+ *   D, Dt         2d/tval3_cpu_2d.cpp:268-293   (3d:365-425, incl. the 3-D
+ *                 Dt y-term that is not the adjoint of D for dim_z >= 2)
+ *   get_g_mult    2d/tval3_cpu_2d.cpp:298-394   (3d:433-529)
+ *   get_g         2d/tval3_cpu_2d.cpp:396-423   (3d:535-576)
+ *   get_tau       2d/tval3_cpu_2d.cpp:425-453   (3d:581-619)
+ *   nonneg        2d/tval3_cpu_2d.cpp:455-465
+ *   descend       2d/tval3_cpu_2d.cpp:481-493
+ *   update_g      2d/tval3_cpu_2d.cpp:495-524
+ *   min_u         2d/tval3_cpu_2d.cpp:526-576   (Armijo test on |f|, 5-strike
+ *                 fallback)
+ *   get_gradient  2d/tval3_cpu_2d.cpp:578-585
+ *   shrinkage     2d/tval3_cpu_2d.cpp:587-611   (3d:834-871)
+ *   update_W      2d/tval3_cpu_2d.cpp:651-671
+ *   update_mlp    2d/tval3_cpu_2d.cpp:673-694
+ *   driver        2d/tval3_cpu_2d.cpp:752-886   (3d:1172-1378)
+ *   gemv          2d/mat_mult.h:15-91, 3d/mat_mult.h:148-595 (Bresenham walk
+ *                 restated from its recurrence, 3d/mat_mult.h:206-359)
+ *
+ * A volume with dim_z == 1 takes the 2-D code path (the reference's 2-D
+ * driver); dim_z >= 2 takes the 3-D one.
+ *
+ * Arithmetic conventions (the reference leaves them to MKL and the compiler):
+ *   - dot products and norms accumulate exact FP64 products in FP64 and round
+ *     once to float (cblas_sdot / cblas_snrm2 return float);
+ *   - matrix-vector products accumulate in FP64 and round once to float;
+ *     geometry rows accumulate the float products x * scale_factor;
+ *   - "x + a*y" vector updates (vec_add, cblas_axpy) are single-rounding FMAs
+ *     in the precision the reference computes them in; cblas_axpy and
+ *     cblas_scal take a float alpha (2d/blas_wrap.h:9-45);
+ *   - scalar expressions are evaluated in the reference's types and order
+ *     with no contraction (build with -ffp-contract=off);
+ *   - the shrinkage sum lam1 accumulates in FP64 and rounds to T_scal.
+ * Reductions use fixed 4096-element chunks summed in chunk order, so results
+ * do not depend on the OpenMP thread count.
+ */
+#include "../include/tval3_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef long long i64;
+const i64 kChunk = 4096;
+
+/* ---------------------------------------------------------------------- */
+/* reductions                                                              */
+/* ---------------------------------------------------------------------- */
+template <class F>
+double chunked_sum(i64 n, F term) {
+  const i64 nc = (n + kChunk - 1) / kChunk;
+  std::vector<double> part(nc > 0 ? nc : 1, 0.0);
+#pragma omp parallel for schedule(static)
+  for (i64 c = 0; c < nc; ++c) {
+    const i64 lo = c * kChunk, hi = std::min(n, lo + kChunk);
+    double s = 0.0;
+    for (i64 i = lo; i < hi; ++i) s += term(i);
+    part[c] = s;
+  }
+  double s = 0.0;
+  for (i64 c = 0; c < nc; ++c) s += part[c];
+  return s;
+}
+
+double ddot(i64 n, const float *a, const float *b) {
+  return chunked_sum(n, [&](i64 i) { return (double)a[i] * (double)b[i]; });
+}
+float sdot(i64 n, const float *a, const float *b) { return (float)ddot(n, a, b); }
+float snrm2(i64 n, const float *a) { return (float)std::sqrt(ddot(n, a, a)); }
+
+/* z = x + alpha*y with one rounding, computed in precision TS */
+template <class TS>
+void vec_add(i64 n, const float *x, TS alpha, const float *y, float *z) {
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i)
+    z[i] = (float)std::fma(alpha, (TS)y[i], (TS)x[i]);
+}
+/* z = x - y (vec_add with alpha = -1: exact negation, one rounding) */
+void vec_sub(i64 n, const float *x, const float *y, float *z) {
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) z[i] = x[i] - y[i];
+}
+/* y += alpha*x, float alpha (cblas_saxpy) */
+void axpy(i64 n, float alpha, const float *x, float *y) {
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) y[i] = std::fma(alpha, x[i], y[i]);
+}
+void scal(i64 n, float alpha, float *x) {
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) x[i] = alpha * x[i];
+}
+
+/* ---------------------------------------------------------------------- */
+/* measurement operators                                                   */
+/* ---------------------------------------------------------------------- */
+struct Op {
+  int kind;  // 0 dense, 1 sparse, 2 geometry
+  i64 m, n;
+  // dense
+  const float *A;
+  // sparse (borrowed)
+  const int *csr_ptr, *csr_ind, *csc_ptr, *csc_ind;
+  const float *csr_val, *csc_val;
+  i64 nnz;
+  // geometry: walks in reference loop order
+  float scale;
+  std::vector<int> walk_ray;      // output row of each walk
+  std::vector<i64> walk_ptr;      // start of each walk in walk_lin
+  std::vector<int> walk_lin;      // voxel indices along each walk
+  // row view (output-row order) and voxel-major transpose
+  std::vector<i64> row_ptr;       // m+1
+  std::vector<int> row_walk;      // walks of each row, in loop order
+  std::vector<i64> col_ptr;       // n+1
+  std::vector<int> col_ray;       // rays through each voxel, ascending ray
+};
+
+/* The Bresenham walk of one emitter-receiver pair, restated from the
+ * recurrence at 3d/mat_mult.h:206-359: the major axis is the first of x, y, z
+ * with the largest |d| (ties go to x, then y); start and end voxels are both
+ * visited; the error terms start at 2*m_minor - L and a minor step happens
+ * when the error is > 0. */
+void bresenham(int x, int y, int z, int x1, int y1, int z1, int rvx, int rvy,
+               std::vector<int> &out) {
+  const int dx = x1 - x, dy = y1 - y, dz = z1 - z;
+  const int xi = dx < 0 ? -1 : 1, mx = std::abs(dx);
+  const int yi = dy < 0 ? -1 : 1, my = std::abs(dy);
+  const int zi = dz < 0 ? -1 : 1, mz = std::abs(dz);
+  auto emit = [&]() { out.push_back(x + y * rvx + z * rvx * rvy); };
+  emit();
+  if (mx >= my && mx >= mz) {
+    int e1 = 2 * my - mx, e2 = 2 * mz - mx;
+    for (int j = 1; j < mx + 1; ++j) {
+      if (e1 > 0) { y += yi; e1 -= 2 * mx; }
+      if (e2 > 0) { z += zi; e2 -= 2 * mx; }
+      e1 += 2 * my; e2 += 2 * mz; x += xi;
+      emit();
+    }
+  } else if (my >= mx && my >= mz) {
+    int e1 = 2 * mx - my, e2 = 2 * mz - my;
+    for (int j = 1; j < my + 1; ++j) {
+      if (e1 > 0) { x += xi; e1 -= 2 * my; }
+      if (e2 > 0) { z += zi; e2 -= 2 * my; }
+      e1 += 2 * mx; e2 += 2 * mz; y += yi;
+      emit();
+    }
+  } else {
+    int e1 = 2 * mx - mz, e2 = 2 * my - mz;
+    for (int j = 1; j < mz + 1; ++j) {
+      if (e1 > 0) { x += xi; e1 -= 2 * mz; }
+      if (e2 > 0) { y += yi; e2 -= 2 * mz; }
+      e1 += 2 * mx; e2 += 2 * my; z += zi;
+      emit();
+    }
+  }
+}
+
+void build_geometry(Op &op, const tv3_geometry &g) {
+  op.kind = 2;
+  op.m = (i64)g.num_mps * g.num_paths;
+  op.n = (i64)g.rv_x * g.rv_y * g.rv_z;
+  op.scale = g.scale_factor;
+  op.walk_ptr.push_back(0);
+  const int E = g.num_emitters, R = g.num_receivers;
+  for (int i = 0; i < g.num_mps; ++i)
+    for (int e = 0; e < E; ++e)
+      for (int r = 0; r < R; ++r) {
+        const int up = g.use_path[e * R + r];
+        if (up == 0) continue;
+        const int ray = i * g.num_paths + up - 1;
+        bresenham(g.x_emitters[i * E + e], g.y_emitters[i * E + e],
+                  g.z_emitters[i * E + e], g.x_receivers[i * R + r],
+                  g.y_receivers[i * R + r], g.z_receivers[i * R + r], g.rv_x,
+                  g.rv_y, op.walk_lin);
+        op.walk_ray.push_back(ray);
+        op.walk_ptr.push_back((i64)op.walk_lin.size());
+      }
+  op.nnz = (i64)op.walk_lin.size();
+  const i64 nw = (i64)op.walk_ray.size();
+  // rows
+  op.row_ptr.assign(op.m + 1, 0);
+  for (i64 w = 0; w < nw; ++w) op.row_ptr[op.walk_ray[w] + 1]++;
+  for (i64 r = 0; r < op.m; ++r) op.row_ptr[r + 1] += op.row_ptr[r];
+  op.row_walk.resize(nw);
+  {
+    std::vector<i64> fill(op.row_ptr.begin(), op.row_ptr.end() - 1);
+    for (i64 w = 0; w < nw; ++w) op.row_walk[fill[op.walk_ray[w]]++] = (int)w;
+  }
+  // voxel-major transpose: rays in ascending output row, each walk in order
+  op.col_ptr.assign(op.n + 1, 0);
+  for (i64 k = 0; k < op.nnz; ++k) op.col_ptr[op.walk_lin[k] + 1]++;
+  for (i64 v = 0; v < op.n; ++v) op.col_ptr[v + 1] += op.col_ptr[v];
+  op.col_ray.resize(op.nnz);
+  std::vector<i64> fill(op.col_ptr.begin(), op.col_ptr.end() - 1);
+  for (i64 r = 0; r < op.m; ++r)
+    for (i64 t = op.row_ptr[r]; t < op.row_ptr[r + 1]; ++t) {
+      const int w = op.row_walk[t];
+      for (i64 k = op.walk_ptr[w]; k < op.walk_ptr[w + 1]; ++k)
+        op.col_ray[fill[op.walk_lin[k]]++] = (int)r;
+    }
+}
+
+/* y = A x or A^T x, overwriting y (gemv with beta = 0). */
+void gemv(const Op &op, bool trans, const float *x, float *y) {
+  const i64 m = op.m, n = op.n;
+  if (op.kind == 0) {
+    const float *A = op.A;
+    if (!trans) {
+#pragma omp parallel for schedule(static)
+      for (i64 i = 0; i < m; ++i) {
+        double s = 0.0;
+        const float *row = A + i * n;
+        for (i64 j = 0; j < n; ++j) s += (double)row[j] * (double)x[j];
+        y[i] = (float)s;
+      }
+    } else {
+      const i64 cb = 256;
+#pragma omp parallel for schedule(static)
+      for (i64 j0 = 0; j0 < n; j0 += cb) {
+        const i64 j1 = std::min(n, j0 + cb);
+        double acc[256];
+        for (i64 j = j0; j < j1; ++j) acc[j - j0] = 0.0;
+        for (i64 i = 0; i < m; ++i) {
+          const double xi = (double)x[i];
+          const float *row = A + i * n;
+          for (i64 j = j0; j < j1; ++j) acc[j - j0] += (double)row[j] * xi;
+        }
+        for (i64 j = j0; j < j1; ++j) y[j] = (float)acc[j - j0];
+      }
+    }
+  } else if (op.kind == 1) {
+    const int *ptr = trans ? op.csc_ptr : op.csr_ptr;
+    const int *ind = trans ? op.csc_ind : op.csr_ind;
+    const float *val = trans ? op.csc_val : op.csr_val;
+    const i64 rows = trans ? n : m;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i64 r = 0; r < rows; ++r) {
+      double s = 0.0;
+      for (int k = ptr[r]; k < ptr[r + 1]; ++k)
+        s += (double)val[k] * (double)x[ind[k]];
+      y[r] = (float)s;
+    }
+  } else {
+    const float sc = op.scale;
+    if (!trans) {
+#pragma omp parallel for schedule(dynamic, 64)
+      for (i64 r = 0; r < m; ++r) {
+        double s = 0.0;
+        for (i64 t = op.row_ptr[r]; t < op.row_ptr[r + 1]; ++t) {
+          const int w = op.row_walk[t];
+          for (i64 k = op.walk_ptr[w]; k < op.walk_ptr[w + 1]; ++k)
+            s += (double)(x[op.walk_lin[k]] * sc);
+        }
+        y[r] = (float)s;
+      }
+    } else {
+#pragma omp parallel for schedule(dynamic, 256)
+      for (i64 v = 0; v < n; ++v) {
+        double s = 0.0;
+        for (i64 k = op.col_ptr[v]; k < op.col_ptr[v + 1]; ++k)
+          s += (double)(x[op.col_ray[k]] * sc);
+        y[v] = (float)s;
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* finite differences (x is the contiguous axis; periodic boundaries)      */
+/* ---------------------------------------------------------------------- */
+struct Dims {
+  int q, p, r;  // dim_x, dim_y, dim_z
+  i64 n() const { return (i64)q * p * r; }
+};
+
+/* D: 2d/tval3_cpu_2d.cpp:268-278, 3d/tval3_cpu_3d.cpp:365-389 */
+void Dop(const Dims &d, const float *U, float *Ux, float *Uy, float *Uz) {
+  const i64 q = d.q, p = d.p, r = d.r, s = q * p, n = d.n();
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) {
+    const i64 x = i % q, y = (i / q) % p, z = i / s;
+    Ux[i] = (x < q - 1 ? U[i + 1] : U[i - (q - 1)]) - U[i];
+    Uy[i] = (y < p - 1 ? U[i + q] : U[i - (p - 1) * q]) - U[i];
+    if (r > 1) Uz[i] = (z < r - 1 ? U[i + s] : U[i - (r - 1) * s]) - U[i];
+  }
+}
+
+/* Dt: 2d/tval3_cpu_2d.cpp:280-293, 3d/tval3_cpu_3d.cpp:394-425.  The
+ * reference builds res with a sequence of vector passes; each entry sees the
+ * same additions in the same order here.  For dim_z >= 2 the reference's bulk
+ * y-pass runs over all i >= dim_x, across slice boundaries, and the per-slice
+ * wrap pass then also applies to the first row of every slice: the first row
+ * of slice z >= 1 receives an extra +Y[i - dim_x] - Y[i].  exact = 1 omits
+ * that extra term. */
+void Dtop(const Dims &d, bool exact, const float *X, const float *Y,
+          const float *Z, float *res) {
+  const i64 q = d.q, p = d.p, r = d.r, s = q * p, n = d.n();
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) {
+    const i64 x = i % q, y = (i / q) % p;
+    float t = (x > 0 ? X[i - 1] : X[i + q - 1]) - X[i];
+    if (r == 1) {
+      if (y > 0) {
+        t = t + Y[i - q];
+        t = t - Y[i];
+      } else {
+        t = t + Y[i + (p - 1) * q];
+        t = t - Y[i];
+      }
+    } else {
+      if (i >= q && (y > 0 || !exact)) {  // bulk pass, i >= dim_x
+        t = t + Y[i - q];
+        t = t - Y[i];
+      }
+      if (y == 0) {  // per-slice wrap pass
+        t = t + Y[i + (p - 1) * q];
+        t = t - Y[i];
+      }
+      if (i >= s) {
+        t = t + Z[i - s];
+        t = t - Z[i];
+      } else {
+        t = t + Z[i + (r - 1) * s];
+        t = t - Z[i];
+      }
+    }
+    res[i] = t;
+  }
+}
+
+/* shrinkage: 2d/tval3_cpu_2d.cpp:588-611 (anisotropic, the reference's only
+ * real variant); iso = 1 is the isotropic group soft-threshold of the TVAL3
+ * method, which the reference does not implement.  Returns sum in FP64. */
+template <class TS>
+double shrink(const Dims &d, TS beta, bool iso, const float *Ux,
+              const float *Uy, const float *Uz, const float *sx,
+              const float *sy, const float *sz, float *Wx, float *Wy,
+              float *Wz) {
+  const i64 n = d.n();
+  const bool z3 = d.r > 1;
+  const TS ib = (TS)1 / beta;
+  return chunked_sum(n, [&](i64 i) -> double {
+    const float ubx = (float)((TS)Ux[i] - (TS)sx[i] / beta);
+    const float uby = (float)((TS)Uy[i] - (TS)sy[i] / beta);
+    const float ubz = z3 ? (float)((TS)Uz[i] - (TS)sz[i] / beta) : 0.0f;
+    if (!iso) {
+      TS wx = std::abs((TS)ubx) - ib, wy = std::abs((TS)uby) - ib;
+      wx = (wx >= 0) ? wx : (TS)0;
+      wy = (wy >= 0) ? wy : (TS)0;
+      Wx[i] = (float)((ubx >= 0) ? wx : -wx);
+      Wy[i] = (float)((uby >= 0) ? wy : -wy);
+      TS t = wx + wy;
+      if (z3) {
+        TS wz = std::abs((TS)ubz) - ib;
+        wz = (wz >= 0) ? wz : (TS)0;
+        Wz[i] = (float)((ubz >= 0) ? wz : -wz);
+        t = t + wz;
+      }
+      return (double)t;
+    }
+    TS ss = (TS)ubx * (TS)ubx + (TS)uby * (TS)uby;
+    if (z3) ss = ss + (TS)ubz * (TS)ubz;
+    const TS rr = std::sqrt(ss);
+    TS w = rr - ib;
+    w = (w >= 0) ? w : (TS)0;
+    Wx[i] = (rr > 0) ? (float)(w * (TS)ubx / rr) : 0.0f;
+    Wy[i] = (rr > 0) ? (float)(w * (TS)uby / rr) : 0.0f;
+    if (z3) Wz[i] = (rr > 0) ? (float)(w * (TS)ubz / rr) : 0.0f;
+    return (double)w;
+  });
+}
+
+/* ---------------------------------------------------------------------- */
+/* the solver                                                              */
+/* ---------------------------------------------------------------------- */
+template <class TS>
+struct Solver {
+  typedef std::vector<float> V;
+  const Op &A;
+  const tv3_options &o;
+  Dims dm;
+  i64 n, m;
+  bool z3;
+  // options in T_scal
+  TS o_c, o_gamma, o_rate_gam, o_rate_cnt, o_gam, o_tol, o_tol_inn;
+  // penalty parameters, multipliers, Lagrangian terms (tval3_data)
+  TS mu, beta, muDbeta;
+  V delta, sx, sy, sz;
+  float f;
+  TS lam1, lam2, lam3;
+  float lam4, lam5;
+  V Up, dU, uup, Ux, Uxp, dUx, Uy, Uyp, dUy, Uz, Uzp, dUz, Wx, Wy, Wz, Atb,
+      Au, Aup, dAu, d, g, gp, dg, g2, g2p, dg2;
+  unsigned currInitIteration;
+  TS currSkipValue;
+  V last_Au;
+  // per-iteration trace fields
+  int bt_cnt, bt_fallback;
+  TS last_tau, last_alpha;
+
+  Solver(const Op &A_, const tv3_options &o_, Dims d_)
+      : A(A_), o(o_), dm(d_), n(d_.n()), m(A_.m), z3(d_.r > 1) {
+    o_c = (TS)o.c; o_gamma = (TS)o.gamma; o_rate_gam = (TS)o.rate_gam;
+    o_rate_cnt = (TS)o.rate_cnt; o_gam = (TS)o.gam; o_tol = (TS)o.tol;
+    o_tol_inn = (TS)o.tol_inn;
+    mu = (TS)o.mu0; beta = (TS)o.beta0; muDbeta = 0;
+    f = 0; lam1 = lam2 = lam3 = 0; lam4 = lam5 = 0;
+    const i64 nz = z3 ? n : 0;
+    delta.assign(m, 0.f); last_Au.assign(m, 0.f);
+    Au.assign(m, 0.f); Aup.assign(m, 0.f); dAu.assign(m, 0.f);
+    for (V *v : {&sx, &sy, &Up, &dU, &uup, &Ux, &Uxp, &dUx, &Uy, &Uyp, &dUy,
+                 &Wx, &Wy, &Atb, &d, &g, &gp, &dg, &g2, &g2p, &dg2})
+      v->assign(n, 0.f);
+    for (V *v : {&sz, &Uz, &Uzp, &dUz, &Wz}) v->assign(nz, 0.f);
+    currInitIteration = 0; currSkipValue = 0;
+    bt_cnt = bt_fallback = 0; last_tau = last_alpha = 0;
+  }
+
+  float *P(V &v) { return v.empty() ? nullptr : v.data(); }
+
+  void D_(const V &U, V &X, V &Y, V &Z) { Dop(dm, U.data(), P(X), P(Y), P(Z)); }
+  void Dt_(V &res, const V &X, const V &Y, const V &Z) {
+    Dtop(dm, o.dt_exact != 0, X.data(), Y.data(), z3 ? Z.data() : nullptr,
+         res.data());
+  }
+
+  /* scaleb: 2d/tval3_cpu_2d.cpp:228-266 */
+  TS scaleb(V &b) {
+    TS scl = 1;
+    if (m > 0) {
+      const TS th1 = (TS)0.5, th2 = (TS)1.5;
+      float bmin = b[0], bmax = bmin;
+      TS bmin_abs = std::abs(bmin), bmax_abs = bmin_abs;
+      for (i64 i = 0; i < m; ++i) {
+        const float val = b[i];
+        const TS va = std::abs(val);
+        if (va < bmin_abs) { bmin = val; bmin_abs = va; }
+        if (va > bmax_abs) { bmax = val; bmax_abs = va; }
+      }
+      const TS b_dif = std::abs(bmax - bmin);
+      if (b_dif < th1) {
+        scl = th1 / b_dif;
+        scal(m, (float)scl, b.data());
+      } else if (b_dif > th2) {
+        scl = th2 / b_dif;
+        scal(m, (float)scl, b.data());
+      }
+    }
+    return scl;
+  }
+
+  /* lam2 and lam4 from V = U_dir - W_dir (get_g / update_g / update_W) */
+  void lam24(V &Vx, V &Vy, V &Vz) {
+    vec_sub(n, Ux.data(), Wx.data(), Vx.data());
+    vec_sub(n, Uy.data(), Wy.data(), Vy.data());
+    if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
+    lam2 = (TS)sdot(n, Vx.data(), Vx.data());
+    lam2 += (TS)sdot(n, Vy.data(), Vy.data());
+    if (z3) lam2 += (TS)sdot(n, Vz.data(), Vz.data());
+    lam4 = sdot(n, sx.data(), Vx.data());
+    lam4 += sdot(n, sy.data(), Vy.data());
+    if (z3) lam4 += sdot(n, sz.data(), Vz.data());
+  }
+  void lam35(const V &b) {
+    V Aub(m);
+    vec_sub(m, Au.data(), b.data(), Aub.data());
+    lam3 = (TS)sdot(m, Aub.data(), Aub.data());
+    lam5 = sdot(m, delta.data(), Aub.data());
+  }
+  float f_full() {
+    return (float)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)lam4) -
+                   (TS)lam5);
+  }
+
+  /* get_g_mult: 2d/tval3_cpu_2d.cpp:298-357 (skipping state machine) and
+   * 359-394 (dense: always both products) */
+  void get_g_mult(const V &U, bool force_mul) {
+    gemv(A, false, U.data(), Au.data());
+    if (A.kind == 0 || o.skipMulRatio == 0) {
+      gemv(A, true, Au.data(), g.data());
+      return;
+    }
+    V deltaAu(Au);
+    axpy(m, -1.0f, last_Au.data(), deltaAu.data());
+    const TS nrm_deltaAu = snrm2(m, deltaAu.data());
+    const TS nrm_Au = snrm2(m, Au.data());
+    const TS relChange = nrm_deltaAu / nrm_Au;
+    const TS ratio = (TS)o.skipMulRatio, maxrel = (TS)o.maxRelativeChange;
+    if (force_mul) {
+      last_Au = Au; gemv(A, true, Au.data(), g.data()); currSkipValue = 0;
+    } else if (currInitIteration < o.numInitIterations) {
+      last_Au = Au; gemv(A, true, Au.data(), g.data());
+      currInitIteration++; currSkipValue = 0;
+    } else if (currSkipValue < 1) {
+      last_Au = Au; gemv(A, true, Au.data(), g.data());
+      currSkipValue += ratio;
+    } else if (relChange > maxrel) {
+      last_Au = Au; gemv(A, true, Au.data(), g.data());
+    } else {
+      currSkipValue -= 1;
+    }
+  }
+
+  /* get_g: 2d/tval3_cpu_2d.cpp:396-423 */
+  void get_g(const V &U, const V &b, bool force_mul) {
+    get_g_mult(U, force_mul);
+    axpy(n, -1.0f, Atb.data(), g.data());
+    V Vx(n), Vy(n), Vz(z3 ? n : 0);
+    vec_sub(n, Ux.data(), Wx.data(), Vx.data());
+    vec_sub(n, Uy.data(), Wy.data(), Vy.data());
+    if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
+    Dt_(g2, Vx, Vy, Vz);
+    lam24(Vx, Vy, Vz);
+    lam35(b);
+    f = f_full();
+  }
+
+  /* get_tau: 2d/tval3_cpu_2d.cpp:425-453 */
+  TS get_tau(bool fst_iter) {
+    TS tau;
+    if (fst_iter) {
+      V dx(n), dy(n), dz(z3 ? n : 0);
+      D_(d, dx, dy, dz);
+      float dDd = sdot(n, dx.data(), dx.data()) + sdot(n, dy.data(), dy.data());
+      if (z3) dDd = dDd + sdot(n, dz.data(), dz.data());
+      const float dd = sdot(n, d.data(), d.data());
+      V Ad(m);
+      gemv(A, false, d.data(), Ad.data());
+      const float Add = sdot(m, Ad.data(), Ad.data());
+      tau = std::abs((TS)dd / ((TS)dDd + muDbeta * (TS)Add));
+    } else {
+      vec_sub(n, g.data(), gp.data(), dg.data());
+      vec_sub(n, g2.data(), g2p.data(), dg2.data());
+      const float ss = sdot(n, uup.data(), uup.data());
+      V tmp(dg2);
+      axpy(n, (float)muDbeta, dg.data(), tmp.data());
+      const float sy_ = sdot(n, uup.data(), tmp.data());
+      tau = (TS)std::abs(ss / sy_);
+    }
+    return tau;
+  }
+
+  void nonneg(V &U) {
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < n; ++i) U[i] = (U[i] < 0) ? 0.0f : U[i];
+  }
+
+  /* descend: 2d/tval3_cpu_2d.cpp:481-493 */
+  void descend(V &U, const V &b, TS tau, bool force_mul) {
+    axpy(n, (float)(-1 * tau), d.data(), U.data());
+    if (o.nonneg) nonneg(U);
+    D_(U, Ux, Uy, Uz);
+    get_g(U, b, force_mul);
+  }
+
+  /* update_g: 2d/tval3_cpu_2d.cpp:495-524 */
+  void update_g(TS alpha, V &U, const V &b) {
+    vec_add<TS>(n, gp.data(), alpha, dg.data(), g.data());
+    vec_add<TS>(n, g2p.data(), alpha, dg2.data(), g2.data());
+    vec_add<TS>(n, Up.data(), alpha, dU.data(), U.data());
+    vec_add<TS>(m, Aup.data(), alpha, dAu.data(), Au.data());
+    vec_add<TS>(n, Uxp.data(), alpha, dUx.data(), Ux.data());
+    vec_add<TS>(n, Uyp.data(), alpha, dUy.data(), Uy.data());
+    if (z3) vec_add<TS>(n, Uzp.data(), alpha, dUz.data(), Uz.data());
+    V Vx(n), Vy(n), Vz(z3 ? n : 0);
+    lam24(Vx, Vy, Vz);
+    lam35(b);
+    f = f_full();
+  }
+
+  /* min_u: 2d/tval3_cpu_2d.cpp:526-576 */
+  void min_u(V &U, TS &gam, const V &b, float C, bool fst_iter) {
+    TS tau = get_tau(fst_iter);
+    Up = U; gp = g; g2p = g2; Aup = Au; Uxp = Ux; Uyp = Uy;
+    if (z3) Uzp = Uz;
+    descend(U, b, tau, false);
+    TS alpha = 1;
+    vec_sub(n, U.data(), Up.data(), dU.data());
+    const TS c_armij = (TS)sdot(n, d.data(), d.data()) * tau * o_c * beta;
+    bt_cnt = 0; bt_fallback = 0;
+    if ((TS)std::abs(f) > std::abs((TS)C - alpha * c_armij)) {
+      vec_sub(n, g.data(), gp.data(), dg.data());
+      vec_sub(n, g2.data(), g2p.data(), dg2.data());
+      vec_sub(m, Au.data(), Aup.data(), dAu.data());
+      vec_sub(n, Ux.data(), Uxp.data(), dUx.data());
+      vec_sub(n, Uy.data(), Uyp.data(), dUy.data());
+      if (z3) vec_sub(n, Uz.data(), Uzp.data(), dUz.data());
+      int cnt = 0;
+      while ((TS)std::abs(f) > std::abs((TS)C - alpha * c_armij)) {
+        if (cnt == 5) {
+          gam *= o_rate_gam;
+          tau = get_tau(true);
+          U = Up;
+          descend(U, b, tau, true);
+          bt_fallback = 1;
+          break;
+        }
+        alpha *= o_gamma;
+        update_g(alpha, U, b);
+        cnt++;
+      }
+      bt_cnt = cnt;
+    }
+    last_tau = tau;
+    last_alpha = alpha;
+  }
+
+  /* get_gradient: 2d/tval3_cpu_2d.cpp:578-585 */
+  void get_gradient(const V &DtsAtd) {
+    d = DtsAtd;
+    axpy(n, 1.0f, g2.data(), d.data());
+    axpy(n, (float)muDbeta, g.data(), d.data());
+  }
+
+  void shrinkage() {
+    lam1 = (TS)shrink<TS>(dm, beta, o.tv_iso != 0, Ux.data(), Uy.data(),
+                          P(Uz), sx.data(), sy.data(), P(sz), Wx.data(),
+                          Wy.data(), P(Wz));
+  }
+
+  /* update_W: 2d/tval3_cpu_2d.cpp:651-671 */
+  void update_W() {
+    f = (float)((TS)f - ((lam1 + beta / 2 * lam2) - (TS)lam4));
+    shrinkage();
+    V Vx(n), Vy(n), Vz(z3 ? n : 0);
+    vec_sub(n, Ux.data(), Wx.data(), Vx.data());
+    vec_sub(n, Uy.data(), Wy.data(), Vy.data());
+    if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
+    Dt_(g2, Vx, Vy, Vz);
+    lam24(Vx, Vy, Vz);
+    f = (float)((TS)f + ((lam1 + beta / 2 * lam2) - (TS)lam4));
+  }
+
+  /* update_mlp: 2d/tval3_cpu_2d.cpp:673-694 */
+  void update_mlp(const V &b) {
+    f = f + (lam4 + lam5);
+    V Vx(n), Vy(n), Vz(z3 ? n : 0);
+    vec_sub(n, Ux.data(), Wx.data(), Vx.data());
+    vec_sub(n, Uy.data(), Wy.data(), Vy.data());
+    if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
+    const float nb = (float)(-1 * beta);
+    axpy(n, nb, Vx.data(), sx.data());
+    axpy(n, nb, Vy.data(), sy.data());
+    if (z3) axpy(n, nb, Vz.data(), sz.data());
+    lam4 = sdot(n, sx.data(), Vx.data());
+    lam4 += sdot(n, sy.data(), Vy.data());
+    if (z3) lam4 += sdot(n, sz.data(), Vz.data());
+    V Aub(m);
+    vec_sub(m, Au.data(), b.data(), Aub.data());
+    axpy(m, (float)(-1 * mu), Aub.data(), delta.data());
+    lam5 = sdot(m, delta.data(), Aub.data());
+    f = f - (lam4 + lam5);
+  }
+
+  /* driver: 2d/tval3_cpu_2d.cpp:752-886 */
+  void run(const float *b_in, float *U_out, const float *Ut, tv3_info *info,
+           tv3_trace_rec *trace, int trace_cap) {
+    V bs(b_in, b_in + m);
+    const TS muf = (TS)o.mu, betaf = (TS)o.beta;
+    TS beta0 = 0;
+    V DtsAtd(n, 0.f);
+    TS gam = o_gam;
+    const TS scl = scaleb(bs);
+    gemv(A, true, bs.data(), Atb.data());
+    V U(Atb);
+    if (mu > muf) mu = muf;
+    if (beta > betaf) beta = betaf;
+    muDbeta = mu / beta;
+    V rcdU(U), UrcdU(n);
+    D_(U, Ux, Uy, Uz);
+    shrinkage();
+    get_g(U, bs, true);
+    get_gradient(DtsAtd);
+    TS Q = 1, Qp;
+    float C = f;
+    int count_outer = 0, count_total = 0;
+    bool fst_iter;
+    TS RelChg, RelChgOut = 0, nrmup = 0;
+    while ((count_outer < o.maxcnt) && (count_total < o.maxit)) {
+      fst_iter = true;
+      while (count_total < o.maxit) {
+        min_u(U, gam, bs, C, fst_iter);
+        update_W();
+        Qp = Q, Q = gam * Qp + 1;
+        C = (float)((gam * Qp * (TS)C + (TS)f) / Q);
+        vec_sub(n, U.data(), Up.data(), uup.data());
+        get_gradient(DtsAtd);
+        count_total++;
+        nrmup = (TS)snrm2(n, Up.data());
+        RelChg = (TS)snrm2(n, uup.data()) / nrmup;
+        if (trace && count_total <= trace_cap) {
+          tv3_trace_rec &t = trace[count_total - 1];
+          t.iter = count_total; t.outer = count_outer;
+          t.backtracks = bt_cnt; t.fallback = bt_fallback;
+          t.tau = last_tau; t.alpha = last_alpha; t.f = f; t.C = C;
+          t.rel_chg = RelChg; t.lam1 = lam1; t.lam2 = lam2; t.lam3 = lam3;
+          t.lam4 = lam4; t.lam5 = lam5; t.mu = mu; t.beta = beta;
+        }
+        if ((RelChg < o_tol_inn) && (!fst_iter)) break;
+        fst_iter = false;
+      }
+      count_outer++;
+      vec_sub(n, U.data(), rcdU.data(), UrcdU.data());
+      RelChgOut = (TS)snrm2(n, UrcdU.data()) / nrmup;
+      rcdU = U;
+      if (RelChgOut < o_tol) break;
+      update_mlp(bs);
+      beta0 = beta;
+      beta *= o_rate_cnt;
+      mu *= o_rate_cnt;
+      if (beta > betaf) beta = betaf;
+      if (mu > muf) mu = muf;
+      muDbeta = mu / beta;
+      f = f_full();
+      DtsAtd = d;
+      scal(n, (float)(beta0 / beta), DtsAtd.data());
+      get_gradient(DtsAtd);
+      gam = o_gam; Q = 1; C = f;
+    }
+    for (i64 i = 0; i < n; ++i) U[i] = (float)((TS)U[i] / scl);
+    std::memcpy(U_out, U.data(), sizeof(float) * n);
+    info->secs = 0;
+    info->outer_iters = count_outer;
+    info->total_iters = count_total;
+    info->rel_chg = RelChgOut;
+    if (Ut) {
+      V dlt(Ut, Ut + n);
+      axpy(n, -1.0f, U.data(), dlt.data());
+      const TS a = snrm2(n, dlt.data()), bb = snrm2(n, Ut);
+      info->rel_error = a / bb;
+    } else {
+      info->rel_error = 0;
+    }
+  }
+};
+
+}  // namespace
+
+/* ---------------------------------------------------------------------- */
+/* C interface for the test harness                                        */
+/* ---------------------------------------------------------------------- */
+extern "C" {
+
+typedef struct tv3o_op tv3o_op;
+
+void tv3o_set_threads(int t) {
+#ifdef _OPENMP
+  if (t > 0) omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}
+
+int tv3o_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+tv3o_op *tv3o_op_dense(int m, int n, const float *A) {
+  Op *op = new (std::nothrow) Op();
+  if (!op) return nullptr;
+  op->kind = 0; op->m = m; op->n = n; op->A = A; op->nnz = (i64)m * n;
+  return (tv3o_op *)op;
+}
+
+tv3o_op *tv3o_op_sparse(int m, int n, int nnz, const int *csr_ptr,
+                        const int *csr_ind, const float *csr_val,
+                        const int *csc_ptr, const int *csc_ind,
+                        const float *csc_val) {
+  Op *op = new (std::nothrow) Op();
+  if (!op) return nullptr;
+  op->kind = 1; op->m = m; op->n = n; op->nnz = nnz;
+  op->csr_ptr = csr_ptr; op->csr_ind = csr_ind; op->csr_val = csr_val;
+  op->csc_ptr = csc_ptr; op->csc_ind = csc_ind; op->csc_val = csc_val;
+  return (tv3o_op *)op;
+}
+
+tv3o_op *tv3o_op_geometry(const tv3_geometry *g) {
+  Op *op = new (std::nothrow) Op();
+  if (!op) return nullptr;
+  build_geometry(*op, *g);
+  return (tv3o_op *)op;
+}
+
+void tv3o_op_free(tv3o_op *op) { delete (Op *)op; }
+
+long long tv3o_op_nnz(const tv3o_op *op) { return ((const Op *)op)->nnz; }
+
+/* Walk of every output row (rows in order, walks of one row in loop order). */
+int tv3o_op_walk(const tv3o_op *h, long long *ptr, int *lin) {
+  const Op &op = *(const Op *)h;
+  if (op.kind != 2) return 1;
+  i64 k = 0;
+  ptr[0] = 0;
+  for (i64 r = 0; r < op.m; ++r) {
+    for (i64 t = op.row_ptr[r]; t < op.row_ptr[r + 1]; ++t) {
+      const int w = op.row_walk[t];
+      for (i64 j = op.walk_ptr[w]; j < op.walk_ptr[w + 1]; ++j)
+        lin[k++] = op.walk_lin[j];
+    }
+    ptr[r + 1] = k;
+  }
+  return 0;
+}
+
+int tv3o_gemv(const tv3o_op *h, int trans, const float *x, float *y) {
+  gemv(*(const Op *)h, trans != 0, x, y);
+  return 0;
+}
+
+void tv3o_D(int q, int p, int r, const float *U, float *Ux, float *Uy,
+            float *Uz) {
+  Dop(Dims{q, p, r}, U, Ux, Uy, Uz);
+}
+
+void tv3o_Dt(int q, int p, int r, int exact, const float *X, const float *Y,
+             const float *Z, float *res) {
+  Dtop(Dims{q, p, r}, exact != 0, X, Y, Z, res);
+}
+
+double tv3o_shrink(int q, int p, int r, double beta, int iso, int scal_double,
+                   const float *Ux, const float *Uy, const float *Uz,
+                   const float *sx, const float *sy, const float *sz,
+                   float *Wx, float *Wy, float *Wz) {
+  Dims dm{q, p, r};
+  if (scal_double)
+    return (double)(double)shrink<double>(dm, beta, iso != 0, Ux, Uy, Uz, sx,
+                                          sy, sz, Wx, Wy, Wz);
+  return (double)(float)shrink<float>(dm, (float)beta, iso != 0, Ux, Uy, Uz,
+                                      sx, sy, sz, Wx, Wy, Wz);
+}
+
+double tv3o_dot(long long n, const float *x, const float *y) {
+  return ddot(n, x, y);
+}
+
+int tv3o_solve(const tv3o_op *h, const tv3_options *o, int q, int p, int r,
+               const float *b, float *U, const float *Ut, tv3_info *info,
+               tv3_trace_rec *trace, int trace_cap) {
+  const Op &op = *(const Op *)h;
+  const Dims dm{q, p, r};
+  if (dm.n() != op.n) return 1;
+  if (o->scal_double) {
+    Solver<double> s(op, *o, dm);
+    s.run(b, U, Ut, info, trace, trace_cap);
+  } else {
+    Solver<float> s(op, *o, dm);
+    s.run(b, U, Ut, info, trace, trace_cap);
+  }
+  return 0;
+}
+
+}  // extern "C"

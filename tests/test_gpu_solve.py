@@ -1,0 +1,168 @@
+"""Solver parity: full TVAL3 reconstructions on the GPU against the oracle.
+
+Bar (from the workload definition): the FP32 reconstruction within 1e-4
+relative L2 of the CPU result, with the per-iteration trajectory tracked.  The
+kernels and the oracle share every elementwise rounding; sums differ only in
+FP64 summation order, so the trajectories (iteration counts, backtracking
+counts, fallbacks) are expected to coincide and the objective f to agree to a
+few float ulps.
+"""
+import numpy as np
+import pytest
+
+import tval3_amd
+from tval3_amd import synth
+import tv3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_U = 1e-4
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _compare(ctx, plan, op, b, dims, opts, Ut=None, cap=400, strict_traj=True):
+    U, info, tr = ctx.solve(plan, b, dims, opts, Ut=Ut, trace_cap=cap)
+    Ug = U.cpu().numpy()
+    Uo, info_o, tro = O.solve(op, b, dims, opts, Ut=Ut, trace_cap=cap)
+    rel = _rel(Ug, Uo)
+    same = [t.iter for t, s in zip(tr, tro)
+            if (t.backtracks, t.fallback) != (s.backtracks, s.fallback)]
+    msg = (f"rel={rel:.3e} iters gpu/oracle={info.total_iters}/{info_o.total_iters} "
+           f"outer={info.outer_iters}/{info_o.outer_iters} first traj diff at {same[:3]}")
+    print(msg)
+    assert np.all(np.isfinite(Ug)), msg
+    assert rel <= TOL_U, msg
+    if strict_traj:
+        assert info.total_iters == info_o.total_iters, msg
+        assert info.outer_iters == info_o.outer_iters, msg
+        assert not same, msg
+        if tro:
+            fo = np.array([s.f for s in tro]); fg = np.array([t.f for t in tr])
+            assert np.allclose(fg, fo, rtol=1e-5, atol=1e-6 * np.abs(fo).max()), msg
+    if Ut is not None:
+        assert info.rel_error == pytest.approx(info_o.rel_error, rel=1e-3, abs=1e-6)
+    return Ug, info, tr
+
+
+def _dense_case(q, p, frac, seed):
+    n = q * p
+    m = int(round(frac * n))
+    A = synth.gaussian_A(m, n, seed)
+    x = synth.ellipse_phantom(p, q, 8, seed)
+    b = synth.add_noise(A @ x, 0.01, seed)
+    return A, x, b
+
+
+@pytest.mark.parametrize("sd", [0, 1])
+@pytest.mark.parametrize("iso", [0, 1])
+def test_dense_solve(ctx, sd, iso):
+    A, x, b = _dense_case(32, 32, 0.35, 11)
+    opts = tval3_amd.default_options(maxit=120, scal_double=sd, tv_iso=iso)
+    _compare(ctx, ctx.plan_dense(A), O.Op.dense(A), b, (32, 32, 1), opts, Ut=x)
+
+
+def test_dense_c0_workload(ctx):
+    """64x64 image, m = 1229 (30 %), mu = 2^8, beta = 2^5, tol 1e-4, maxit 300."""
+    A, x, b = _dense_case(64, 64, 0.3, 0)
+    opts = tval3_amd.default_options(maxit=300, tv_iso=1)
+    _compare(ctx, ctx.plan_dense(A), O.Op.dense(A), b, (64, 64, 1), opts, Ut=x)
+
+
+def test_dense_continuation_nonneg(ctx):
+    A, x, b = _dense_case(24, 20, 0.5, 3)
+    opts = tval3_amd.default_options(maxit=150, mu0=4.0, beta0=2.0, nonneg=1,
+                                     tol_inn=1e-2)
+    _compare(ctx, ctx.plan_dense(A), O.Op.dense(A), b, (24, 20, 1), opts, Ut=x)
+
+
+def _ring_case(K, grid, seed):
+    geo = synth.ring_geometry(K, grid)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(grid, seed=seed)
+    b = synth.add_noise(op.gemv(x), 0.005, seed)
+    return geo, op, x, b
+
+
+@pytest.mark.parametrize("sd", [0, 1])
+def test_ring_geometry_solve(ctx, sd):
+    geo, op, x, b = _ring_case(64, 64, 1)
+    opts = tval3_amd.default_options(maxit=100, scal_double=sd)
+    _compare(ctx, ctx.plan_geometry(geo), op, b, (64, 64, 1), opts, Ut=x)
+
+
+def test_ring_sparse_solve(ctx):
+    geo, op, x, b = _ring_case(64, 64, 2)
+    ptr, lin = op.walk()
+    csr = synth.csr_from_walk(ptr, lin, geo.n)
+    sop = O.Op.sparse(geo.m, geo.n, *csr)
+    opts = tval3_amd.default_options(maxit=100)
+    _compare(ctx, ctx.plan_sparse(geo.m, geo.n, *csr), sop, b, (64, 64, 1), opts, Ut=x)
+
+
+def test_ring_skip_mul(ctx):
+    geo, op, x, b = _ring_case(64, 64, 3)
+    opts = tval3_amd.default_options(maxit=80, skipMulRatio=0.5,
+                                     numInitIterations=3, maxRelativeChange=0.05)
+    _compare(ctx, ctx.plan_geometry(geo), op, b, (64, 64, 1), opts, Ut=x)
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+def test_hemisphere_3d_solve(ctx, exact):
+    grid = 20
+    geo = synth.hemisphere_geometry(96, grid)
+    op = O.Op.geometry(geo)
+    x = synth.ellipsoid_phantom(grid, 6, seed=4)
+    b = synth.add_noise(op.gemv(x), 0.005, 4)
+    opts = tval3_amd.default_options(maxit=40, dt_exact=exact)
+    _compare(ctx, ctx.plan_geometry(geo), op, b, (grid, grid, grid), opts, Ut=x)
+
+
+def test_batch_shared_ring(ctx):
+    """70 instances sharing one ring geometry: one 64-lane group + a padded one."""
+    geo, op, x, _ = _ring_case(48, 48, 5)
+    plan = ctx.plan_geometry(geo)
+    B = 70
+    opts = tval3_amd.default_options(maxit=60)
+    bs = [synth.add_noise(op.gemv(x), 0.005, 100 + i) for i in range(B)]
+    Us, infos = ctx.solve_batch([plan] * B, bs, [(48, 48, 1)] * B, opts,
+                                Uts=[x] * B)
+    for i in (0, 1, 63, 64, 69):
+        Uo, info_o, _ = O.solve(op, bs[i], (48, 48, 1), opts, Ut=x)
+        rel = _rel(Us[i].cpu().numpy(), Uo)
+        assert rel <= TOL_U, (i, rel, infos[i].total_iters, info_o.total_iters)
+        assert infos[i].total_iters == info_o.total_iters
+
+
+def test_batch_uneven_dense(ctx):
+    sizes = [(16, 16, 0.3), (24, 20, 0.4), (32, 32, 0.25)]
+    plans, bs, dims, cases = [], [], [], []
+    for k, (q, p, fr) in enumerate(sizes):
+        A, x, b = _dense_case(q, p, fr, 20 + k)
+        plans.append(ctx.plan_dense(A)); bs.append(b); dims.append((q, p, 1))
+        cases.append((A, x, b))
+    opts = tval3_amd.default_options(maxit=80)
+    Us, infos = ctx.solve_batch(plans, bs, dims, opts)
+    for k, (A, x, b) in enumerate(cases):
+        Uo, info_o, _ = O.solve(O.Op.dense(A), b, dims[k], opts)
+        assert _rel(Us[k].cpu().numpy(), Uo) <= TOL_U
+        assert infos[k].total_iters == info_o.total_iters
+
+
+def test_maxit_zero_returns_scaled_Atb(ctx):
+    A, x, b = _dense_case(16, 16, 0.5, 9)
+    opts = tval3_amd.default_options(maxit=0)
+    _compare(ctx, ctx.plan_dense(A), O.Op.dense(A), b, (16, 16, 1), opts)
+
+
+def test_argument_errors(ctx):
+    A, x, b = _dense_case(16, 16, 0.5, 9)
+    plan = ctx.plan_dense(A)
+    with pytest.raises(tval3_amd.Tval3Error, match="length of U"):
+        ctx.solve(plan, b, (15, 16, 1))
+    geo = synth.ring_geometry(16, 16)
+    geo.xe[0] = 99
+    with pytest.raises(tval3_amd.Tval3Error, match="outside"):
+        ctx.plan_geometry(geo)

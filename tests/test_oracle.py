@@ -1,0 +1,339 @@
+"""Pins the CPU oracle (runs without a GPU).
+
+The reference ships no tests or golden vectors and cannot be built here (it
+needs four project headers it does not ship and Intel MKL headers), so each
+oracle function is pinned by closed forms, exact linear-algebra identities,
+brute force on tiny inputs and an independent transcription of the Bresenham
+recurrence (tests/golden).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tval3_amd import synth
+from tval3_amd.abi import default_options
+import tv3_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------------------
+# Bresenham walk (3d/mat_mult.h:241-359)
+# ---------------------------------------------------------------------------
+def test_walk_matches_golden():
+    g = json.load(open(os.path.join(GOLDEN, "bresenham_walks.json")))
+    grid = tuple(g["grid"])
+    segs = g["segments"]
+    K = len(segs)
+    xe = [s["start"][0] for s in segs]; ye = [s["start"][1] for s in segs]
+    ze = [s["start"][2] for s in segs]
+    xr = [s["end"][0] for s in segs]; yr = [s["end"][1] for s in segs]
+    zr = [s["end"][2] for s in segs]
+    # emitter k pairs only with receiver k
+    up = np.zeros((K, K), np.int32)
+    up[np.arange(K), np.arange(K)] = np.arange(1, K + 1)
+    geo = synth.GeometryArrays(1, up.ravel(), xe, ye, ze, xr, yr, zr, grid, K)
+    ptr, lin = O.Op.geometry(geo).walk()
+    for k, s in enumerate(segs):
+        assert list(lin[ptr[k]:ptr[k + 1]]) == s["lin"], k
+
+
+def _coords(lin, grid):
+    q, p, _ = grid
+    return lin % q, (lin // q) % p, lin // (q * p)
+
+
+@pytest.mark.parametrize("geo_fn,grid", [
+    (lambda: synth.ring_geometry(128, 128), (128, 128, 1)),
+    (lambda: synth.hemisphere_geometry(64, 32), (32, 32, 32)),
+])
+def test_walk_invariants(geo_fn, grid):
+    geo = geo_fn()
+    ptr, lin = O.Op.geometry(geo).walk()
+    K = geo.xe.size
+    iu = np.triu_indices(K, 1)
+    for row in range(0, geo.m, max(1, geo.m // 400)):
+        e, r = iu[0][row], iu[1][row]
+        seg = lin[ptr[row]:ptr[row + 1]]
+        x, y, z = _coords(seg, grid)
+        s = np.array([geo.xe[e], geo.ye[e], geo.ze[e]])
+        t = np.array([geo.xr[r], geo.yr[r], geo.zr[r]])
+        d = t - s
+        assert (x[0], y[0], z[0]) == tuple(s) and (x[-1], y[-1], z[-1]) == tuple(t)
+        assert seg.size == np.abs(d).max() + 1
+        assert np.unique(seg).size == seg.size
+        steps = np.stack([np.diff(x), np.diff(y), np.diff(z)])
+        assert np.all(np.abs(steps) <= 1)
+        major = int(np.argmax(np.abs(d)))   # first maximum: ties go to x, then y
+        if d[major] != 0:
+            assert np.all(steps[major] == np.sign(d[major]))
+
+
+def test_ring_nnz_counts():
+    assert O.Op.geometry(synth.ring_geometry(256, 256)).nnz == 4821527
+    assert O.Op.geometry(synth.ring_geometry(128, 128)).nnz == 604831
+
+
+def _explicit(geo):
+    ptr, lin = O.Op.geometry(geo).walk()
+    A = np.zeros((geo.m, geo.n))
+    rows = np.repeat(np.arange(geo.m), np.diff(ptr))
+    np.add.at(A, (rows, lin), np.float64(np.float32(geo.scale)))
+    return A
+
+
+def test_geometry_operator_brute_force():
+    geo = synth.ring_geometry(24, 20, scale=0.5)
+    op = O.Op.geometry(geo)
+    A = _explicit(geo)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(geo.n).astype(np.float32)
+    y = rng.standard_normal(geo.m).astype(np.float32)
+    np.testing.assert_allclose(op.gemv(x), A @ x, rtol=3e-7, atol=1e-6)
+    np.testing.assert_allclose(op.gemv(y, trans=True), A.T @ y, rtol=3e-7, atol=1e-6)
+
+
+def test_dense_and_sparse_operators():
+    A = synth.gaussian_A(50, 70, 1)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(70).astype(np.float32)
+    y = rng.standard_normal(50).astype(np.float32)
+    Ad = A.astype(np.float64)
+    op = O.Op.dense(A)
+    # FP64 accumulation rounded once: within half an ulp of the exact value
+    np.testing.assert_allclose(op.gemv(x), Ad @ x, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(op.gemv(y, True), Ad.T @ y, rtol=1e-6, atol=1e-6)
+    A[np.abs(A) < 0.1] = 0
+    import scipy.sparse as sp
+    S = sp.csr_matrix(A)
+    T = sp.csc_matrix(A)
+    sop = O.Op.sparse(50, 70, S.indptr, S.indices, S.data, T.indptr, T.indices, T.data)
+    np.testing.assert_allclose(sop.gemv(x), A.astype(np.float64) @ x, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(sop.gemv(y, True), A.T.astype(np.float64) @ y, rtol=1e-6, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# D / Dt (2d/tval3_cpu_2d.cpp:268-293, 3d/tval3_cpu_3d.cpp:365-425)
+# ---------------------------------------------------------------------------
+def _D_matrix(dims):
+    n = int(np.prod(dims))
+    z3 = dims[2] > 1
+    M = np.zeros((3 * n if z3 else 2 * n, n))
+    for j in range(n):
+        e = np.zeros(n, np.float32); e[j] = 1
+        Ux, Uy, Uz = O.D(e, dims)
+        M[:n, j] = Ux; M[n:2 * n, j] = Uy
+        if z3:
+            M[2 * n:, j] = Uz
+    return M
+
+
+def _Dt_matrix(dims, exact):
+    n = int(np.prod(dims))
+    z3 = dims[2] > 1
+    k = 3 if z3 else 2
+    M = np.zeros((n, k * n))
+    z = np.zeros(n, np.float32)
+    for j in range(k * n):
+        X, Y, Z = z.copy(), z.copy(), z.copy()
+        [X, Y, Z][j // n][j % n] = 1
+        M[:, j] = O.Dt(X, Y, Z, dims, exact)
+    return M
+
+
+@pytest.mark.parametrize("dims", [(5, 4, 1), (6, 3, 1), (4, 3, 3), (3, 5, 2)])
+def test_D_closed_form(dims):
+    q, p, r = dims
+    U = np.random.default_rng(0).standard_normal(q * p * r).astype(np.float32)
+    V = U.reshape(r, p, q)
+    Ux, Uy, Uz = O.D(U, dims)
+    np.testing.assert_array_equal(Ux, (np.roll(V, -1, 2) - V).ravel())
+    np.testing.assert_array_equal(Uy, (np.roll(V, -1, 1) - V).ravel())
+    if r > 1:
+        np.testing.assert_array_equal(Uz, (np.roll(V, -1, 0) - V).ravel())
+
+
+@pytest.mark.parametrize("dims", [(5, 4, 1), (7, 2, 1), (4, 3, 3), (3, 5, 2)])
+def test_Dt_is_adjoint_except_reference_3d_defect(dims):
+    q, p, r = dims
+    n, s = q * p * r, q * p
+    Dm = _D_matrix(dims)
+    exact = _Dt_matrix(dims, True)
+    np.testing.assert_array_equal(exact, Dm.T)
+    ref = _Dt_matrix(dims, False)
+    if r == 1:
+        np.testing.assert_array_equal(ref, Dm.T)
+        return
+    # the reference's 3-D Dt: row z*s + k (z >= 1, k < dim_x) gains
+    # +Y[(z-1)*s + (dim_y-1)*dim_x + k] - Y[z*s + k]
+    defect = np.zeros_like(ref)
+    for z in range(1, r):
+        for k in range(q):
+            i = z * s + k
+            defect[i, n + (z - 1) * s + (p - 1) * q + k] += 1
+            defect[i, n + i] -= 1
+    np.testing.assert_array_equal(ref - Dm.T, defect)
+
+
+def test_Dt_operation_order_2d():
+    """Each entry is ((X[xp] - X[i]) + Y[yp]) - Y[i] with float rounding at
+    every step, the order of the reference's passes."""
+    q, p = 9, 7
+    rng = np.random.default_rng(3)
+    X = (rng.standard_normal(q * p) * 10 ** rng.uniform(-3, 3, q * p)).astype(np.float32)
+    Y = (rng.standard_normal(q * p) * 10 ** rng.uniform(-3, 3, q * p)).astype(np.float32)
+    Xm, Ym = X.reshape(p, q), Y.reshape(p, q)
+    want = ((np.roll(Xm, 1, 1) - Xm) + np.roll(Ym, 1, 0)) - Ym
+    np.testing.assert_array_equal(O.Dt(X, Y, None, (q, p, 1)), want.ravel())
+
+
+# ---------------------------------------------------------------------------
+# shrinkage (2d/tval3_cpu_2d.cpp:588-611)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("sd", [False, True])
+def test_shrink_anisotropic_closed_form(sd):
+    rng = np.random.default_rng(4)
+    n = 4000
+    Ux, Uy, sx, sy = (rng.standard_normal(n).astype(np.float32) * 0.1 for _ in range(4))
+    beta = 32.0
+    Wx, Wy, _, lam1 = O.shrink(Ux, Uy, None, sx, sy, None, (n, 1, 1), beta, False, sd)
+    T = np.float64 if sd else np.float32
+    for U, s, W in ((Ux, sx, Wx), (Uy, sy, Wy)):
+        ub = (U.astype(T) - s.astype(T) / T(beta)).astype(np.float32)
+        w = np.maximum(np.abs(ub).astype(T) - T(1) / T(beta), 0)
+        np.testing.assert_array_equal(W, np.where(ub >= 0, w, -w).astype(np.float32))
+    ub = [(U.astype(np.float64) - s / beta) for U, s in ((Ux, sx), (Uy, sy))]
+    exact = sum(np.maximum(np.abs(u) - 1 / beta, 0).sum() for u in ub)
+    assert lam1 == pytest.approx(exact, rel=1e-5)
+
+
+def test_shrink_special_cases():
+    rng = np.random.default_rng(5)
+    n = 500
+    Ux, Uy, Uz = (rng.standard_normal(n).astype(np.float32) for _ in range(3))
+    z = np.zeros(n, np.float32)
+    # sigma = 0, beta -> infinity: W -> U
+    Wx, Wy, Wz, _ = O.shrink(Ux, Uy, Uz, z, z, z, (10, 10, 5), 1e12, False, True)
+    np.testing.assert_allclose(Wx, Ux, rtol=1e-6, atol=1e-11)
+    np.testing.assert_allclose(Wz, Uz, rtol=1e-6, atol=1e-11)
+    # isotropic with one non-zero component equals the anisotropic result
+    Wi, Wi2, _, li = O.shrink(Ux, z, None, z, z, None, (n, 1, 1), 2.0, True, True)
+    Wa, _, _, la = O.shrink(Ux, z, None, z, z, None, (n, 1, 1), 2.0, False, True)
+    np.testing.assert_allclose(Wi, Wa, rtol=1e-6, atol=1e-7)
+    assert np.all(Wi2 == 0)
+    assert li == pytest.approx(la, rel=1e-12)
+    # isotropic closed form: w = max(|u| - 1/beta, 0), W = w u / |u|
+    Wx, Wy, _, l1 = O.shrink(Ux, Uy, None, z, z, None, (n, 1, 1), 4.0, True, True)
+    r = np.hypot(Ux.astype(np.float64), Uy.astype(np.float64))
+    w = np.maximum(r - 0.25, 0)
+    np.testing.assert_allclose(Wx, w * Ux / r, rtol=1e-6, atol=1e-7)
+    assert l1 == pytest.approx(w.sum(), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# solver (2d/tval3_cpu_2d.cpp:752-886)
+# ---------------------------------------------------------------------------
+def _dense(q, p, frac, seed, noise=0.01):
+    n = q * p
+    m = int(round(frac * n))
+    A = synth.gaussian_A(m, n, seed)
+    x = synth.ellipse_phantom(p, q, 6, seed)
+    return A, x, synth.add_noise(A @ x, noise, seed)
+
+
+def _scaleb(b):
+    """Closed form of scaleb (2d:228-266): scale to |b_max - b_min| in
+    [0.5, 1.5] where b_min / b_max are the first entries of smallest / largest
+    magnitude."""
+    a = np.abs(b)
+    d = abs(float(b[np.argmax(a)]) - float(b[np.argmin(a)]))
+    return 0.5 / d if d < 0.5 else (1.5 / d if d > 1.5 else 1.0)
+
+
+@pytest.mark.parametrize("amp", [1e-3, 1.0, 50.0])
+def test_maxit_zero_is_scaled_Atb(amp):
+    A, x, b = _dense(12, 10, 0.5, 2)
+    b = (b * amp).astype(np.float32)
+    opts = default_options(maxit=0, scal_double=1)
+    U, info, _ = O.solve(O.Op.dense(A), b, (12, 10, 1), opts)
+    s = _scaleb(b)
+    want = (A.astype(np.float64).T @ (np.float32(s) * b).astype(np.float64)) / s
+    np.testing.assert_allclose(U, want, rtol=2e-6, atol=1e-7 * np.abs(want).max())
+    assert info.total_iters == 0 and info.outer_iters == 0
+
+
+@pytest.mark.parametrize("sd", [0, 1])
+def test_objective_bookkeeping(sd):
+    """f is maintained incrementally (update_W subtracts and re-adds its terms,
+    update_mlp likewise); at the end of every inner iteration it must equal the
+    full Lagrangian lam1 + beta/2 lam2 + mu/2 lam3 - lam4 - lam5."""
+    A, x, b = _dense(20, 16, 0.4, 6)
+    opts = default_options(maxit=120, mu0=8.0, beta0=4.0, scal_double=sd)
+    U, info, tr = O.solve(O.Op.dense(A), b, (20, 16, 1), opts, trace_cap=120)
+    assert len(tr) == info.total_iters > 10
+    for t in tr:
+        full = t.lam1 + t.beta / 2 * t.lam2 + t.mu / 2 * t.lam3 - t.lam4 - t.lam5
+        assert t.f == pytest.approx(full, rel=2e-4, abs=1e-4 * abs(full) + 1e-6)
+    # continuation multiplies mu and beta by rate_cnt per outer iteration
+    mus = sorted({t.mu for t in tr})
+    assert mus[0] == 8.0 and all(b_ / a_ == 2.0 for a_, b_ in zip(mus, mus[1:]))
+
+
+def test_nonmonotone_reference_recurrence():
+    """C_k = (gam Q_{k-1} C_{k-1} + f_k) / Q_k with Q_k = gam Q_{k-1} + 1 inside
+    an outer iteration (2d:802), checked on the trace in FP64."""
+    A, x, b = _dense(16, 16, 0.4, 8)
+    opts = default_options(maxit=60, scal_double=1)
+    _, info, tr = O.solve(O.Op.dense(A), b, (16, 16, 1), opts, trace_cap=60)
+    gam = opts.gam
+    Q, C = 1.0, None
+    prev_outer = tr[0].outer
+    for k, t in enumerate(tr):
+        if t.outer != prev_outer or k == 0:
+            Q, C = 1.0, None
+            prev_outer = t.outer
+        if t.fallback:
+            C = None
+            continue
+        if C is not None:
+            Qn = gam * Q + 1
+            assert t.C == pytest.approx((gam * Q * C + t.f) / Qn, rel=1e-6)
+            Q = Qn
+        else:
+            Q = gam * Q + 1
+        C = t.C
+
+
+def test_float_and_double_scalars_agree():
+    A, x, b = _dense(24, 24, 0.35, 9)
+    op = O.Op.dense(A)
+    Uf, inf_f, _ = O.solve(op, b, (24, 24, 1), default_options(maxit=200, scal_double=0), Ut=x)
+    Ud, inf_d, _ = O.solve(op, b, (24, 24, 1), default_options(maxit=200, scal_double=1), Ut=x)
+    assert np.linalg.norm(Uf - Ud) / np.linalg.norm(Ud) < 2e-2
+    assert inf_f.rel_error < 0.1 and inf_d.rel_error < 0.1
+
+
+def test_noise_free_reconstruction_converges():
+    """Piecewise-constant phantom, 60 % Gaussian measurements, no noise: TV
+    recovery must be accurate."""
+    A, x, b = _dense(24, 24, 0.6, 10, noise=0.0)
+    opts = default_options(maxit=400, tol=1e-6, scal_double=1, tv_iso=1)
+    U, info, _ = O.solve(O.Op.dense(A), b, (24, 24, 1), opts, Ut=x)
+    assert info.rel_error < 0.02
+
+
+def test_thread_count_does_not_change_results():
+    A, x, b = _dense(20, 20, 0.4, 12)
+    op = O.Op.dense(A)
+    opts = default_options(maxit=60)
+    t0 = O.get_threads()
+    try:
+        O.set_threads(1)
+        U1, _, _ = O.solve(op, b, (20, 20, 1), opts)
+        O.set_threads(max(2, t0))
+        U2, _, _ = O.solve(op, b, (20, 20, 1), opts)
+    finally:
+        O.set_threads(t0)
+    np.testing.assert_array_equal(U1, U2)

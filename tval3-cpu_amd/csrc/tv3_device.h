@@ -1,0 +1,177 @@
+// tv3_device.h — device helpers shared by the operator and solver kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tv3_internal.h"
+
+namespace tv3 {
+
+// ---- Bresenham closed form --------------------------------------------------
+// Number of minor-axis steps after j major steps of the reference's error
+// recurrence (3d/mat_mult.h:241-359: error starts at 2*m - L, a minor step
+// happens when it is > 0, then +2*m each step and -2*L per minor step):
+//   k(j) = ceil((2*m*j - L) / (2*L)) = floor((2*m*j + L - 1) / (2*L)).
+// The quotient is taken from a float reciprocal and fixed up by one step;
+// plans keep 2*m*j + L below 2^24, where the estimate is within one.
+__device__ __forceinline__ int bres_k(int m, int L, float inv2L, int j) {
+  if (m == 0) return 0;
+  const int num = 2 * m * j + L - 1;
+  const int den = 2 * L;
+  int q = (int)((float)num * inv2L);
+  if (q * den > num)
+    --q;
+  else if ((q + 1) * den <= num)
+    ++q;
+  return q;
+}
+
+__device__ __forceinline__ int ray_lin(const RayDesc &r, int j) {
+  return r.lin0 + j * r.s_major + bres_k(r.ma, r.L, r.inv2L, j) * r.s_a +
+         bres_k(r.mb, r.L, r.inv2L, j) * r.s_b;
+}
+
+// ---- wavefront / block reductions (fixed order) ------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums NS per-thread values over the threads of one lane class (tid % W) of a
+// 256-thread block and writes the block partial of instance inst0 + b to
+// part[((inst0 + b) * kNslot + slot[s]) * kNblkMax + blk].  Order: butterfly
+// inside each wavefront, then wavefronts 0..3 in order.
+__device__ __forceinline__ bool sel(uint32_t f, uint32_t req, uint32_t forbid) {
+  return (f & req) == req && !(f & forbid);
+}
+
+// Only lanes whose instance passes the (req, forbid) flag test write, so a
+// masked kernel never overwrites the partials another kernel left for the
+// instances it skips.
+template <int W, int NS>
+__device__ __forceinline__ void block_partials(const double (&v)[NS],
+                                               const int (&slot)[NS],
+                                               double *part, int inst0,
+                                               int blk, const InstScal *S,
+                                               uint32_t req, uint32_t forbid) {
+  __shared__ double red[4][NS][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double x = v[s];
+#pragma unroll
+    for (int off = 32; off >= W; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane < W) red[wv][s][lane] = x;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NS * W; t += kBlock) {
+    const int s = t / W, b = t % W;
+    if (!sel(S[inst0 + b].flags, req, forbid)) continue;
+    const double x = ((red[0][s][b] + red[1][s][b]) + red[2][s][b]) + red[3][s][b];
+    part[((long long)(inst0 + b) * kNslot + slot[s]) * kNblkMax + blk] = x;
+  }
+}
+
+// Sum of nblk partials of one slot, by a 256-thread block (fixed order);
+// the result is returned to every thread.
+__device__ __forceinline__ double block_sum_partials(const double *p, int nblk) {
+  __shared__ double red[kBlock];
+  double x = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kBlock) x += p[i];
+  red[threadIdx.x] = x;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// ---- grid geometry of the reconstruction volume -------------------------------
+// x is the contiguous axis; all differences are periodic (2d/tval3_cpu_2d.cpp:
+// 268-278, 3d/tval3_cpu_3d.cpp:365-389).
+struct Vox {
+  int v, x, y, z;
+  int q, p, r, s;
+  __device__ __forceinline__ Vox(int v_, int q_, int p_, int r_)
+      : v(v_), q(q_), p(p_), r(r_), s(q_ * p_) {
+    x = v % q;
+    const int t = v / q;
+    y = t % p;
+    z = t / p;
+  }
+  __device__ __forceinline__ int xn() const { return x < q - 1 ? v + 1 : v - (q - 1); }
+  __device__ __forceinline__ int xp() const { return x > 0 ? v - 1 : v + (q - 1); }
+  __device__ __forceinline__ int yn() const { return y < p - 1 ? v + q : v - (p - 1) * q; }
+  __device__ __forceinline__ int yp() const { return y > 0 ? v - q : v + (p - 1) * q; }
+  __device__ __forceinline__ int zn() const { return z < r - 1 ? v + s : v - (r - 1) * s; }
+  __device__ __forceinline__ int zp() const { return z > 0 ? v - s : v + (r - 1) * s; }
+};
+
+// Dt at one voxel, given accessors VX(j), VY(j), VZ(j) of the three
+// components.  The additions follow the reference's pass order
+// (2d/tval3_cpu_2d.cpp:280-293, 3d/tval3_cpu_3d.cpp:394-425), one rounding
+// each.  For dim_z >= 2 the reference's bulk y-pass covers every i >= dim_x,
+// so the first row of each slice z >= 1 also receives +Y[i - dim_x] - Y[i]
+// (the entry i - dim_x is the last row of the previous slice) before the
+// per-slice wrap pass; exact = true leaves that term out.
+template <class FX, class FY, class FZ>
+__device__ __forceinline__ float dt_at(const Vox &w, bool exact, FX VX, FY VY,
+                                       FZ VZ) {
+  float t = VX(w.xp()) - VX(w.v);
+  if (w.r == 1) {
+    t = t + VY(w.yp());
+    t = t - VY(w.v);
+    return t;
+  }
+  if (w.v >= w.q && (w.y > 0 || !exact)) {
+    t = t + VY(w.v - w.q);
+    t = t - VY(w.v);
+  }
+  if (w.y == 0) {
+    t = t + VY(w.v + (w.p - 1) * w.q);
+    t = t - VY(w.v);
+  }
+  t = t + VZ(w.zp());
+  t = t - VZ(w.v);
+  return t;
+}
+
+// ---- shrinkage (2d/tval3_cpu_2d.cpp:588-611) ---------------------------------
+// Anisotropic soft-threshold of one direction: ubar = u - sigma/beta rounded
+// to float (T_mat), w = max(|ubar| - 1/beta, 0) in T_scal, W = sign+(ubar) w.
+template <class TS>
+__device__ __forceinline__ float shrink1(float u, float sg, TS beta, TS ib,
+                                         TS &w) {
+  const float ub = (float)((TS)u - (TS)sg / beta);
+  TS t = fabs((TS)ub) - ib;
+  t = (t >= (TS)0) ? t : (TS)0;
+  w = t;
+  return (float)((ub >= 0.0f) ? t : -t);
+}
+
+// Isotropic group soft-threshold (no reference counterpart): ubar as above per
+// direction, r = |ubar|_2, w = max(r - 1/beta, 0), W_dir = w * ubar_dir / r.
+template <class TS>
+__device__ __forceinline__ TS shrink_iso(float ux, float uy, float uz, float sx,
+                                         float sy, float sz, bool z3, TS beta,
+                                         TS ib, float &Wx, float &Wy,
+                                         float &Wz) {
+  const float bx = (float)((TS)ux - (TS)sx / beta);
+  const float by = (float)((TS)uy - (TS)sy / beta);
+  const float bz = z3 ? (float)((TS)uz - (TS)sz / beta) : 0.0f;
+  TS ss = (TS)bx * (TS)bx + (TS)by * (TS)by;
+  if (z3) ss = ss + (TS)bz * (TS)bz;
+  const TS rr = sqrt(ss);
+  TS w = rr - ib;
+  w = (w >= (TS)0) ? w : (TS)0;
+  Wx = (rr > 0) ? (float)(w * (TS)bx / rr) : 0.0f;
+  Wy = (rr > 0) ? (float)(w * (TS)by / rr) : 0.0f;
+  Wz = (z3 && rr > 0) ? (float)(w * (TS)bz / rr) : 0.0f;
+  return w;
+}
+
+}  // namespace tv3

@@ -1,0 +1,171 @@
+// tv3_internal.h — shared host/device declarations of the MI355X TVAL3 engine.
+//
+// Memory layout.  A batch of reconstructions is split into groups of W lanes
+// (W = 1 for a single or uneven-size batch, W = 64 for instances that share a
+// measurement operator and image size).  Every vector of a group is stored
+// lane-interleaved: element v of lane b lives at base + v * W + b, so a
+// wavefront that walks one ray for 64 instances issues 256-byte coalesced
+// loads, and a wavefront with W = 1 walks 64 consecutive pixels.  Per-instance
+// scalars (penalties, Lagrangian terms, step sizes, counters, control flags)
+// stay on the device in InstScal; the host reads back only a flag summary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tval3_amd.h"
+
+namespace tv3 {
+
+constexpr int kBlock = 256;       // threads per block of every vector kernel
+constexpr int kNblkMax = 1024;    // max blocks per group of a reduction kernel
+constexpr int kNslot = 16;        // partial-sum slots per instance
+constexpr int kSlotM = 12;        // first slot written by m-vector kernels
+
+// ---- plans ------------------------------------------------------------------
+enum PlanKind { kDense = 0, kSparse = 1, kGeometry = 2 };
+
+// One output row of the on-the-fly projector.  Voxel j (0 <= j <= L) of the
+// walk is lin0 + j*s_major + ka(j)*s_a + kb(j)*s_b with
+// k(j) = floor((2*m*j + L - 1) / (2*L)), the closed form of the Bresenham
+// error recurrence of the reference (3d/mat_mult.h:241-359).  L < 0 marks a
+// row no pair maps to (its value is 0).
+struct RayDesc {
+  int lin0;
+  int L;
+  int s_major, s_a, s_b;
+  int ma, mb;
+  float inv2L;  // 1 / (2L) for the division fix-up
+};
+
+// Device view of a plan, embedded in every group descriptor.
+struct PlanDev {
+  int kind;
+  int m, n;
+  // dense
+  const void *A;
+  int dtype;
+  // sparse (CSR of A and CSR of A^T = CSC of A)
+  const int *r_ptr, *r_ind;
+  const float *r_val;
+  const int *c_ptr, *c_ind;
+  const float *c_val;
+  // geometry
+  const RayDesc *rays;
+  const int *g_cptr;  // voxel-major transpose: rays through voxel v,
+  const int *g_cray;  // ascending, at g_cray[g_cptr[v] .. g_cptr[v+1])
+  float scale;
+};
+
+// ---- batch description ------------------------------------------------------
+struct GroupDesc {
+  int q, p, r;          // dim_x, dim_y, dim_z
+  int n, m;             // pixels, measurements
+  long long off_n;      // float offset of this group's n-vectors
+  long long off_m;      // float offset of this group's m-vectors
+  PlanDev plan;
+};
+
+// Control flags (InstScal::flags)
+enum : uint32_t {
+  F_ACTIVE = 1u << 0,     // instance still iterating
+  F_FST = 1u << 1,        // first inner iteration of an outer iteration
+  F_BT = 1u << 2,         // Armijo failed: backtracking round pending
+  F_FB = 1u << 3,         // 5-strike fallback pending
+  F_BWD = 1u << 4,        // A^T product needed in this descent
+  F_INNER_DONE = 1u << 5, // inner loop ended this step
+  F_MLP = 1u << 6,        // multiplier update pending
+  F_JUST_DONE = 1u << 7,  // finished in this step (write the output)
+  F_DONE = 1u << 8,       // finished
+  F_SKIPG = 1u << 9,      // A^T skipped: g = gp - Atb
+  F_DESC = 1u << 10,      // descent pending (set for all active at a step)
+  F_LIVE = 1u << 11,      // lane holds a real instance (not padding)
+};
+
+// Per-instance scalars.  T_scal quantities are kept in double; for the
+// float instantiation every value is a float widened exactly.  f, C, lam4,
+// lam5 are T_comp (float) in the reference and are stored as float.
+struct InstScal {
+  uint32_t flags;
+  int count_outer, count_total, cnt;  // cnt: backtracking steps this iter
+  int fallback;
+  unsigned currInitIteration;
+  double mu, beta, muDbeta, beta0;
+  double lam1, lam2, lam3;
+  float lam4, lam5, f, C;
+  double gam, Q;
+  double tau, alpha, c_armij;
+  double RelChg, RelChgOut, nrmup;
+  double scl;
+  double currSkipValue;
+  double ratio;        // beta0 / beta after a continuation step
+  double rel_error;
+  int trace_n;
+  int pad;
+};
+
+// Options converted once to device form.
+struct OptDev {
+  double mu, beta, mu0, beta0, tol, tol_inn;
+  int maxit, maxcnt;
+  double gam, gamma, rate_gam, rate_cnt, c;
+  int nonneg, tv_iso, dt_exact, skip;
+  unsigned numInitIterations;
+  double skipMulRatio, maxRelativeChange;
+};
+
+}  // namespace tv3
+
+// ---- context and plan objects (host) ------------------------------------------
+struct tv3_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  // grow-only device workspace for the solver
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  void *pinned = nullptr;  // small pinned host buffer for flag read-back
+  size_t pinned_bytes = 0;
+  // timing events for the benchmark (kernel-level, on this stream)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // optional per-launch timing of the A / A^T products (tv3_ctx_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evkind;     // kind of each recorded pair: 0 = A, 1 = A^T
+  double prof_ms[2] = {0, 0};
+  long long prof_n[2] = {0, 0};
+};
+
+struct tv3_plan {
+  tv3_ctx *ctx = nullptr;
+  tv3::PlanDev dev{};
+  long long nnz = 0;
+  // owned device arrays (geometry)
+  tv3::RayDesc *rays = nullptr;
+  int *cptr = nullptr, *cray = nullptr;
+  std::vector<long long> walk_ptr;  // host: start of each row's walk
+};
+
+namespace tv3 {
+// error helpers (tv3_plan.hip)
+int set_err(tv3_ctx *ctx, int code, const std::string &msg);
+// event pair around one operator launch when profiling is on
+void prof_begin(tv3_ctx *ctx, int kind);
+void prof_end(tv3_ctx *ctx);
+void prof_collect(tv3_ctx *ctx);
+int hip_check(tv3_ctx *ctx, hipError_t e, const char *what);
+
+// operator launches (tv3_plan.hip).  x and y are batch base pointers;
+// flag_mask selects the lanes that compute (0 = all).
+template <int W>
+void launch_fwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                const float *x, float *y, const InstScal *S, uint32_t mask);
+template <int W>
+void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                int kind, const float *x, float *y, const InstScal *S,
+                uint32_t mask, const float *sub, float *scratch);
+}  // namespace tv3

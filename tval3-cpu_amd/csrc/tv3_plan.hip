@@ -1,0 +1,610 @@
+// tv3_plan.hip — context, measurement-operator plans and the A / A^T kernels.
+//
+// Reference counterparts: gemv() for mat<T> (2d/mat_mult.h:15-36),
+// sparse_mat<T> (2d/mat_mult.h:38-91, 3d/mat_mult.h:47-118) and geometry
+// (3d/mat_mult.h:148-595).  Every product accumulates in FP64 and rounds once
+// to float; no kernel uses floating-point atomics, so results are
+// deterministic run to run.  A^T of the on-the-fly projector gathers over a
+// voxel-major transpose built once per plan (rays of each voxel in ascending
+// order), which replaces the reference's per-thread private output buffers
+// and serial reduction (3d/mat_mult.h:367-385, 573-581).
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tv3_internal.h"
+#include "tv3_device.h"
+
+using namespace tv3;
+
+namespace tv3 {
+
+int set_err(tv3_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+void prof_begin(tv3_ctx *ctx, int kind) {
+  if (!ctx->prof) return;
+  const size_t i = ctx->evkind.size() * 2;
+  while (ctx->evpool.size() < i + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { ctx->prof = false; return; }
+    ctx->evpool.push_back(e);
+  }
+  ctx->evkind.push_back(kind);
+  (void)hipEventRecord(ctx->evpool[i], ctx->stream);
+}
+
+void prof_end(tv3_ctx *ctx) {
+  if (!ctx->prof || ctx->evkind.empty()) return;
+  (void)hipEventRecord(ctx->evpool[ctx->evkind.size() * 2 - 1], ctx->stream);
+}
+
+// folds the recorded pairs into the per-kind totals (after a stream sync)
+void prof_collect(tv3_ctx *ctx) {
+  for (size_t k = 0; k < ctx->evkind.size(); ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->evpool[2 * k], ctx->evpool[2 * k + 1]) == hipSuccess) {
+      ctx->prof_ms[ctx->evkind[k]] += ms;
+      ctx->prof_n[ctx->evkind[k]] += 1;
+    }
+  }
+  ctx->evkind.clear();
+}
+
+int hip_check(tv3_ctx *ctx, hipError_t e, const char *what) {
+  if (e == hipSuccess) return TV3_OK;
+  return set_err(ctx, e == hipErrorOutOfMemory ? TV3_ERR_ALLOC : TV3_ERR_HIP,
+                 std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// forward product: y[row] = sum_k A[row, k] x[k]
+// ---------------------------------------------------------------------------
+
+// W = 1: one wavefront per row; lanes take entries lane, lane+64, ...; the 64
+// lane sums are combined by a fixed butterfly.
+__global__ __launch_bounds__(kBlock) void k_fwd_w1(const GroupDesc *G,
+                                                   const float *x, float *y,
+                                                   const InstScal *S,
+                                                   uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= gd.m) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const float *xg = x + gd.off_n;
+  double acc = 0.0;
+  if (P.kind == kGeometry) {
+    const RayDesc rd = P.rays[row];
+    const float sc = P.scale;
+    for (int j = lane; j <= rd.L; j += 64)
+      acc += (double)(xg[ray_lin(rd, j)] * sc);
+  } else if (P.kind == kSparse) {
+    const int k0 = P.r_ptr[row], k1 = P.r_ptr[row + 1];
+    for (int k = k0 + lane; k < k1; k += 64)
+      acc += (double)P.r_val[k] * (double)xg[P.r_ind[k]];
+  } else {
+    const long long base = (long long)row * gd.n;
+    if (P.dtype == TV3_F32) {
+      const float *A = (const float *)P.A + base;
+      for (int j = lane; j < gd.n; j += 64) acc += (double)A[j] * (double)xg[j];
+    } else {
+      const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
+      for (int j = lane; j < gd.n; j += 64)
+        acc += (double)__bfloat162float(A[j]) * (double)xg[j];
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) y[gd.off_m + row] = (float)acc;
+}
+
+// W = 64: one wavefront per row, one lane per instance; entries are walked
+// in order by all lanes together (the walk and the indices are
+// wavefront-uniform), each gather is a 256-byte coalesced load.
+__global__ __launch_bounds__(kBlock) void k_fwd_w64(const GroupDesc *G,
+                                                    const float *x, float *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= gd.m) return;
+  const bool on = !mask || (S[blockIdx.y * 64 + lane].flags & mask) == mask;
+  if (!__any(on)) return;
+  const PlanDev &P = gd.plan;
+  const float *xg = x + gd.off_n + lane;
+  double acc = 0.0;
+  if (P.kind == kGeometry) {
+    const RayDesc rd = P.rays[row];
+    const float sc = P.scale;
+    for (int j = 0; j <= rd.L; ++j)
+      acc += (double)(xg[(long long)ray_lin(rd, j) * 64] * sc);
+  } else if (P.kind == kSparse) {
+    const int k0 = P.r_ptr[row], k1 = P.r_ptr[row + 1];
+    for (int k = k0; k < k1; ++k)
+      acc += (double)P.r_val[k] * (double)xg[(long long)P.r_ind[k] * 64];
+  } else {
+    const long long base = (long long)row * gd.n;
+    if (P.dtype == TV3_F32) {
+      const float *A = (const float *)P.A + base;
+      for (int j = 0; j < gd.n; ++j)
+        acc += (double)A[j] * (double)xg[(long long)j * 64];
+    } else {
+      const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
+      for (int j = 0; j < gd.n; ++j)
+        acc += (double)__bfloat162float(A[j]) * (double)xg[(long long)j * 64];
+    }
+  }
+  if (on) y[gd.off_m + (long long)row * 64 + lane] = (float)acc;
+}
+
+// ---------------------------------------------------------------------------
+// transposed product: y[col] = sum_k A[k, col] x[k] (- sub[col])
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
+                                                   const float *x, float *y,
+                                                   const InstScal *S,
+                                                   uint32_t mask,
+                                                   const float *sub) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const float *xg = x + gd.off_m;
+  double acc = 0.0;
+  if (P.kind == kGeometry) {
+    const int k0 = P.g_cptr[col], k1 = P.g_cptr[col + 1];
+    const float sc = P.scale;
+    for (int k = k0 + lane; k < k1; k += 64) acc += (double)(xg[P.g_cray[k]] * sc);
+  } else {
+    const int k0 = P.c_ptr[col], k1 = P.c_ptr[col + 1];
+    for (int k = k0 + lane; k < k1; k += 64)
+      acc += (double)P.c_val[k] * (double)xg[P.c_ind[k]];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    const long long o = gd.off_n + col;
+    float v = (float)acc;
+    if (sub) v = v - sub[o];
+    y[o] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
+                                                    const float *x, float *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask,
+                                                    const float *sub) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (col >= gd.n) return;
+  const bool on = !mask || (S[blockIdx.y * 64 + lane].flags & mask) == mask;
+  if (!__any(on)) return;
+  const PlanDev &P = gd.plan;
+  const float *xg = x + gd.off_m + lane;
+  double acc = 0.0;
+  if (P.kind == kGeometry) {
+    const int k0 = P.g_cptr[col], k1 = P.g_cptr[col + 1];
+    const float sc = P.scale;
+    for (int k = k0; k < k1; ++k)
+      acc += (double)(xg[(long long)P.g_cray[k] * 64] * sc);
+  } else if (P.kind == kSparse) {
+    const int k0 = P.c_ptr[col], k1 = P.c_ptr[col + 1];
+    for (int k = k0; k < k1; ++k)
+      acc += (double)P.c_val[k] * (double)xg[(long long)P.c_ind[k] * 64];
+  } else {
+    if (P.dtype == TV3_F32) {
+      const float *A = (const float *)P.A + col;
+      for (int i = 0; i < gd.m; ++i)
+        acc += (double)A[(long long)i * gd.n] * (double)xg[(long long)i * 64];
+    } else {
+      const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + col;
+      for (int i = 0; i < gd.m; ++i)
+        acc += (double)__bfloat162float(A[(long long)i * gd.n]) *
+               (double)xg[(long long)i * 64];
+    }
+  }
+  if (on) {
+    const long long o = gd.off_n + (long long)col * 64 + lane;
+    float v = (float)acc;
+    if (sub) v = v - sub[o];
+    y[o] = v;
+  }
+}
+
+// Dense A^T with W = 1: column-parallel over row chunks (split-K).  Chunk c
+// of group g covers rows [c*R, min(m, (c+1)*R)); partial sums go to scratch
+// and are added in chunk order by k_dense_bwd_reduce.
+constexpr int kDenseChunks = 16;
+
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
+    const GroupDesc *G, const float *x, double *scratch, const InstScal *S,
+    uint32_t mask, long long sstride) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (gd.m + kBlock - 1) / kBlock);
+  const int c = blockIdx.y;
+  if (c >= nch) return;
+  const int R = (gd.m + nch - 1) / nch;
+  const int i0 = c * R, i1 = min(gd.m, i0 + R);
+  const float *xg = x + gd.off_m;
+  double acc = 0.0;
+  if (gd.plan.dtype == TV3_F32) {
+    const float *A = (const float *)gd.plan.A + col;
+    for (int i = i0; i < i1; ++i)
+      acc += (double)A[(long long)i * gd.n] * (double)xg[i];
+  } else {
+    const __hip_bfloat16 *A = (const __hip_bfloat16 *)gd.plan.A + col;
+    for (int i = i0; i < i1; ++i)
+      acc += (double)__bfloat162float(A[(long long)i * gd.n]) * (double)xg[i];
+  }
+  scratch[blockIdx.z * sstride + (long long)c * gd.n + col] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce(
+    const GroupDesc *G, const double *scratch, float *y, const InstScal *S,
+    uint32_t mask, const float *sub, long long sstride) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (gd.m + kBlock - 1) / kBlock);
+  const double *sp = scratch + blockIdx.y * sstride + col;
+  double acc = 0.0;
+  for (int c = 0; c < nch; ++c) acc += sp[(long long)c * gd.n];
+  const long long o = gd.off_n + col;
+  float v = (float)acc;
+  if (sub) v = v - sub[o];
+  y[o] = v;
+}
+
+template <>
+void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                   const float *x, float *y, const InstScal *S, uint32_t mask) {
+  dim3 grid((max_m + 3) / 4, G);
+  hipLaunchKernelGGL(k_fwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+}
+
+template <>
+void launch_fwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                    const float *x, float *y, const InstScal *S,
+                    uint32_t mask) {
+  dim3 grid((max_m + 3) / 4, G);
+  hipLaunchKernelGGL(k_fwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
+                     mask);
+}
+
+template <>
+void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                   int kind, const float *x, float *y, const InstScal *S,
+                   uint32_t mask, const float *sub, float *scratch) {
+  if (kind == kDense) {
+    // scratch holds kDenseChunks * max_n doubles per group
+    const long long sstride = (long long)kDenseChunks * max_n;
+    dim3 g1((max_n + kBlock - 1) / kBlock, kDenseChunks, G);
+    hipLaunchKernelGGL(k_dense_bwd_part, g1, dim3(kBlock), 0, s, G_dev, x,
+                       (double *)scratch, S, mask, sstride);
+    dim3 g2((max_n + kBlock - 1) / kBlock, G);
+    hipLaunchKernelGGL(k_dense_bwd_reduce, g2, dim3(kBlock), 0, s, G_dev,
+                       (const double *)scratch, y, S, mask, sub, sstride);
+    return;
+  }
+  dim3 grid((max_n + 3) / 4, G);
+  hipLaunchKernelGGL(k_bwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask,
+                     sub);
+}
+
+template <>
+void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                    int kind, const float *x, float *y, const InstScal *S,
+                    uint32_t mask, const float *sub, float *scratch) {
+  (void)kind;
+  (void)scratch;
+  dim3 grid((max_n + 3) / 4, G);
+  hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
+                     mask, sub);
+}
+
+// ---------------------------------------------------------------------------
+// geometry plan construction
+// ---------------------------------------------------------------------------
+
+// Writes the voxel list of every row (walk order) at lin[ptr[row] ...].
+__global__ void k_walk_export(const RayDesc *rays, int m, const long long *ptr,
+                              int *lin) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= m) return;
+  const RayDesc rd = rays[row];
+  const long long o = ptr[row];
+  for (int j = lane; j <= rd.L; j += 64) lin[o + j] = ray_lin(rd, j);
+}
+
+}  // namespace tv3
+
+// ===========================================================================
+// C-ABI: context, options, plans
+// ===========================================================================
+extern "C" {
+
+void tv3_options_default(tv3_options *o) {
+  // Public TVAL3 defaults (the reference's values sit in tval3_types.h,
+  // which its tree does not ship).
+  o->mu = 256.0; o->beta = 32.0; o->mu0 = 256.0; o->beta0 = 32.0;
+  o->tol = 1e-4; o->tol_inn = 1e-3; o->maxit = 300; o->maxcnt = 10;
+  o->gam = 0.9995; o->gamma = 0.6; o->rate_gam = 0.9; o->rate_cnt = 2.0;
+  o->c = 1e-5; o->nonneg = 0; o->isreal = 1; o->numInitIterations = 0;
+  o->skipMulRatio = 0.0; o->maxRelativeChange = 0.0;
+  o->scal_double = 0; o->tv_iso = 0; o->dt_exact = 0;
+}
+
+int tv3_ctx_create(int device, void *stream, tv3_ctx **out) {
+  if (!out) return TV3_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TV3_ERR_NOGPU;
+  if (device < 0 || device >= ndev) return TV3_ERR_ARG;
+  tv3_ctx *c = new tv3_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return TV3_ERR_HIP; }
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return TV3_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  if (hipHostMalloc(&c->pinned, 1 << 16, hipHostMallocDefault) != hipSuccess) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TV3_ERR_ALLOC;
+  }
+  c->pinned_bytes = 1 << 16;
+  (void)hipEventCreate(&c->ev0);
+  (void)hipEventCreate(&c->ev1);
+  *out = c;
+  return TV3_OK;
+}
+
+void tv3_ctx_destroy(tv3_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *tv3_ctx_error(const tv3_ctx *c) { return c ? c->err.c_str() : ""; }
+
+int tv3_ctx_sync(tv3_ctx *c) {
+  if (!c) return TV3_ERR_ARG;
+  return hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+void *tv3_ctx_stream(const tv3_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int tv3_ctx_profile(tv3_ctx *c, int enable) {
+  if (!c) return TV3_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  c->evkind.clear();
+  c->prof = enable != 0;
+  c->prof_ms[0] = c->prof_ms[1] = 0;
+  c->prof_n[0] = c->prof_n[1] = 0;
+  return TV3_OK;
+}
+
+int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count) {
+  if (!c || !ms || !count) return TV3_ERR_ARG;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "profile sync");
+  if (rc) return rc;
+  prof_collect(c);
+  for (int k = 0; k < 2; ++k) { ms[k] = c->prof_ms[k]; count[k] = c->prof_n[k]; }
+  return TV3_OK;
+}
+
+int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,
+                   tv3_plan **out) {
+  if (!ctx || !out) return TV3_ERR_ARG;
+  if (m <= 0 || n <= 0 || !A || (dtype != TV3_F32 && dtype != TV3_BF16))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: dense plan needs m, n > 0, A and dtype f32/bf16");
+  tv3_plan *p = new tv3_plan();
+  p->ctx = ctx;
+  p->dev.kind = kDense; p->dev.m = m; p->dev.n = n; p->dev.A = A;
+  p->dev.dtype = dtype;
+  p->nnz = (long long)m * n;
+  *out = p;
+  return TV3_OK;
+}
+
+int tv3_plan_sparse(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                    const int *csr_ind, const float *csr_val,
+                    const int *csc_ptr, const int *csc_ind,
+                    const float *csc_val, tv3_plan **out) {
+  if (!ctx || !out) return TV3_ERR_ARG;
+  if (m <= 0 || n <= 0 || nnz < 0)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: sparse plan needs m, n > 0");
+  if (!csr_ptr || !csc_ptr || (nnz > 0 && (!csr_ind || !csr_val || !csc_ind || !csc_val)))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: A must be available in CSR and CSC format!");
+  tv3_plan *p = new tv3_plan();
+  p->ctx = ctx;
+  PlanDev &d = p->dev;
+  d.kind = kSparse; d.m = m; d.n = n;
+  d.r_ptr = csr_ptr; d.r_ind = csr_ind; d.r_val = csr_val;
+  d.c_ptr = csc_ptr; d.c_ind = csc_ind; d.c_val = csc_val;
+  p->nnz = nnz;
+  *out = p;
+  return TV3_OK;
+}
+
+int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
+  if (!ctx || !g || !out) return TV3_ERR_ARG;
+  *out = nullptr;
+  const int E = g->num_emitters, R = g->num_receivers;
+  if (g->num_mps <= 0 || E <= 0 || R <= 0 || g->num_paths <= 0 ||
+      g->rv_x <= 0 || g->rv_y <= 0 || g->rv_z <= 0 || !g->use_path)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: invalid geometry");
+  const long long nvox = (long long)g->rv_x * g->rv_y * g->rv_z;
+  const long long m = (long long)g->num_mps * g->num_paths;
+  if (nvox > 0x7fffffffLL || m > 0x7fffffffLL)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: geometry too large");
+  std::vector<RayDesc> rays(m);
+  for (auto &r : rays) { r = RayDesc{}; r.L = -1; }
+  const int sx = 1, sy = g->rv_x, sz = g->rv_x * g->rv_y;
+  auto inside = [&](int x, int y, int z) {
+    return x >= 0 && x < g->rv_x && y >= 0 && y < g->rv_y && z >= 0 && z < g->rv_z;
+  };
+  for (int i = 0; i < g->num_mps; ++i)
+    for (int e = 0; e < E; ++e)
+      for (int r = 0; r < R; ++r) {
+        const int up = g->use_path[e * R + r];
+        if (up == 0) continue;
+        if (up < 0 || up > g->num_paths)
+          return set_err(ctx, TV3_ERR_ARG, "Argument error: use_path entry out of range");
+        const long long row = (long long)i * g->num_paths + up - 1;
+        if (rays[row].L >= 0)
+          return set_err(ctx, TV3_ERR_ARG, "Argument error: two pairs map to one measurement row");
+        const int x0 = g->x_emitters[i * E + e], y0 = g->y_emitters[i * E + e],
+                  z0 = g->z_emitters[i * E + e];
+        const int x1 = g->x_receivers[i * R + r], y1 = g->y_receivers[i * R + r],
+                  z1 = g->z_receivers[i * R + r];
+        if (!inside(x0, y0, z0) || !inside(x1, y1, z1))
+          return set_err(ctx, TV3_ERR_ARG, "Argument error: transducer outside the volume");
+        const int dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
+        const int mx = std::abs(dx), my = std::abs(dy), mz = std::abs(dz);
+        const int ix = dx < 0 ? -sx : sx, iy = dy < 0 ? -sy : sy, iz = dz < 0 ? -sz : sz;
+        RayDesc d;
+        d.lin0 = x0 * sx + y0 * sy + z0 * sz;
+        // major axis: the first of x, y, z with the largest |d|
+        if (mx >= my && mx >= mz) {
+          d.L = mx; d.s_major = ix; d.s_a = iy; d.s_b = iz; d.ma = my; d.mb = mz;
+        } else if (my >= mx && my >= mz) {
+          d.L = my; d.s_major = iy; d.s_a = ix; d.s_b = iz; d.ma = mx; d.mb = mz;
+        } else {
+          d.L = mz; d.s_major = iz; d.s_a = ix; d.s_b = iy; d.ma = mx; d.mb = my;
+        }
+        // the closed-form quotient is exact while 2*m*j + L < 2^24
+        if (d.L > 2048)
+          return set_err(ctx, TV3_ERR_ARG, "Argument error: ray longer than 2048 voxels");
+        d.inv2L = d.L > 0 ? 1.0f / (float)(2 * d.L) : 0.0f;
+        rays[row] = d;
+      }
+  tv3_plan *p = new tv3_plan();
+  p->ctx = ctx;
+  PlanDev &dv = p->dev;
+  dv.kind = kGeometry; dv.m = (int)m; dv.n = (int)nvox; dv.scale = g->scale_factor;
+  p->walk_ptr.assign(m + 1, 0);
+  for (long long r = 0; r < m; ++r)
+    p->walk_ptr[r + 1] = p->walk_ptr[r] + (rays[r].L >= 0 ? rays[r].L + 1 : 0);
+  const long long nnz = p->walk_ptr[m];
+  p->nnz = nnz;
+  if (nnz > 0x7fffffffLL) { delete p; return set_err(ctx, TV3_ERR_ARG, "Argument error: too many ray voxels"); }
+  hipStream_t s = ctx->stream;
+  int rc = TV3_OK;
+  long long *d_ptr = nullptr;
+  int *d_lin = nullptr;
+  std::vector<int> lin(nnz);
+#define TV3_TRY(x, what) do { rc = hip_check(ctx, (x), what); if (rc) goto fail; } while (0)
+  TV3_TRY(hipMalloc(&p->rays, sizeof(RayDesc) * m), "hipMalloc rays");
+  TV3_TRY(hipMemcpyAsync(p->rays, rays.data(), sizeof(RayDesc) * m, hipMemcpyHostToDevice, s), "copy rays");
+  TV3_TRY(hipMalloc(&d_ptr, sizeof(long long) * (m + 1)), "hipMalloc walk ptr");
+  TV3_TRY(hipMalloc(&d_lin, sizeof(int) * std::max<long long>(nnz, 1)), "hipMalloc walk");
+  TV3_TRY(hipMemcpyAsync(d_ptr, p->walk_ptr.data(), sizeof(long long) * (m + 1), hipMemcpyHostToDevice, s), "copy walk ptr");
+  hipLaunchKernelGGL(k_walk_export, dim3((unsigned)((m + 3) / 4)), dim3(kBlock), 0, s, p->rays, (int)m, d_ptr, d_lin);
+  TV3_TRY(hipGetLastError(), "k_walk_export");
+  TV3_TRY(hipMemcpyAsync(lin.data(), d_lin, sizeof(int) * nnz, hipMemcpyDeviceToHost, s), "copy walk");
+  TV3_TRY(hipStreamSynchronize(s), "walk sync");
+  {
+    // voxel-major transpose, rays ascending within each voxel (counting sort)
+    std::vector<int> cptr(nvox + 1, 0), cray(std::max<long long>(nnz, 1));
+    for (long long k = 0; k < nnz; ++k) cptr[lin[k] + 1]++;
+    for (long long v = 0; v < nvox; ++v) cptr[v + 1] += cptr[v];
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    for (long long r = 0; r < m; ++r)
+      for (long long k = p->walk_ptr[r]; k < p->walk_ptr[r + 1]; ++k)
+        cray[fill[lin[k]]++] = (int)r;
+    TV3_TRY(hipMalloc(&p->cptr, sizeof(int) * (nvox + 1)), "hipMalloc cptr");
+    TV3_TRY(hipMalloc(&p->cray, sizeof(int) * cray.size()), "hipMalloc cray");
+    TV3_TRY(hipMemcpyAsync(p->cptr, cptr.data(), sizeof(int) * (nvox + 1), hipMemcpyHostToDevice, s), "copy cptr");
+    TV3_TRY(hipMemcpyAsync(p->cray, cray.data(), sizeof(int) * cray.size(), hipMemcpyHostToDevice, s), "copy cray");
+    TV3_TRY(hipStreamSynchronize(s), "plan sync");
+  }
+#undef TV3_TRY
+  (void)hipFree(d_ptr);
+  (void)hipFree(d_lin);
+  dv.rays = p->rays; dv.g_cptr = p->cptr; dv.g_cray = p->cray;
+  *out = p;
+  return TV3_OK;
+fail:
+  if (d_ptr) (void)hipFree(d_ptr);
+  if (d_lin) (void)hipFree(d_lin);
+  if (p->rays) (void)hipFree(p->rays);
+  if (p->cptr) (void)hipFree(p->cptr);
+  if (p->cray) (void)hipFree(p->cray);
+  delete p;
+  return rc;
+}
+
+void tv3_plan_destroy(tv3_plan *p) {
+  if (!p) return;
+  // hipFree waits for the device; the plan's context may already be gone
+  if (p->rays) (void)hipFree(p->rays);
+  if (p->cptr) (void)hipFree(p->cptr);
+  if (p->cray) (void)hipFree(p->cray);
+  delete p;
+}
+
+int tv3_plan_shape(const tv3_plan *p, long long *m, long long *n,
+                   long long *nnz) {
+  if (!p) return TV3_ERR_ARG;
+  if (m) *m = p->dev.m;
+  if (n) *n = p->dev.n;
+  if (nnz) *nnz = p->nnz;
+  return TV3_OK;
+}
+
+int tv3_plan_walk(tv3_ctx *ctx, const tv3_plan *p, long long *ptr, int *lin) {
+  if (!ctx || !p || !ptr || !lin) return TV3_ERR_ARG;
+  if (p->dev.kind != kGeometry)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: not a geometry plan");
+  const long long m = p->dev.m, nnz = p->nnz;
+  std::memcpy(ptr, p->walk_ptr.data(), sizeof(long long) * (m + 1));
+  long long *d_ptr = nullptr;
+  int *d_lin = nullptr;
+  int rc = hip_check(ctx, hipMalloc(&d_ptr, sizeof(long long) * (m + 1)), "hipMalloc");
+  if (!rc) rc = hip_check(ctx, hipMalloc(&d_lin, sizeof(int) * std::max<long long>(nnz, 1)), "hipMalloc");
+  if (!rc) rc = hip_check(ctx, hipMemcpyAsync(d_ptr, ptr, sizeof(long long) * (m + 1), hipMemcpyHostToDevice, ctx->stream), "copy");
+  if (!rc) {
+    hipLaunchKernelGGL(k_walk_export, dim3((unsigned)((m + 3) / 4)), dim3(kBlock), 0, ctx->stream, p->dev.rays, (int)m, d_ptr, d_lin);
+    rc = hip_check(ctx, hipGetLastError(), "k_walk_export");
+  }
+  if (!rc) rc = hip_check(ctx, hipMemcpyAsync(lin, d_lin, sizeof(int) * nnz, hipMemcpyDeviceToHost, ctx->stream), "copy");
+  if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (d_ptr) (void)hipFree(d_ptr);
+  if (d_lin) (void)hipFree(d_lin);
+  return rc;
+}
+
+}  // extern "C"

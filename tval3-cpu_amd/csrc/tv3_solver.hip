@@ -1,0 +1,1659 @@
+// tv3_solver.hip — the TVAL3 inner loop on the GPU.
+//
+// Restates the reference solver (2d/tval3_cpu_2d.cpp:97-886,
+// 3d/tval3_cpu_3d.cpp:155-1378) as batched kernels over groups of W
+// lane-interleaved instances.  The reference's ~107 vector passes per inner
+// iteration (2-D) become five fused passes:
+//   k_tau_bb / k_tau_first  get_tau (2d:425-453) dot products
+//   k_descend               descend (2d:481-493): U = Up - tau d, nonneg, D(U),
+//                           dU, and get_g's V, g2 = Dt(V), lam2, lam4
+//                           (2d:396-423) in one stencil pass
+//   k_mpost                 get_g's Aub, lam3, lam5 and the A^T-skip norms
+//   k_bt_*                  update_g (2d:495-524) interpolation + lam2..lam5
+//   k_wstep                 update_W (2d:651-671: shrinkage, V, Dt, lam1,
+//                           lam2, lam4) fused with uup, get_gradient
+//                           (2d:578-585) and the two norms of the stop test
+//   k_outer1, k_mlp_*       RelChgOut and update_mlp (2d:673-694)
+// plus the A / A^T products of tv3_plan.hip.  Every scalar of the reference
+// (mu, beta, lam1..5, f, C, Q, gam, tau, alpha, counters) lives per instance
+// on the device and is updated by one-block-per-instance fin_* kernels in
+// T_scal arithmetic, in the reference's order.  The host reads back only an
+// OR of the instance flags at the three data-dependent branch points
+// (Armijo test, each backtracking round, inner stop).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tv3_device.h"
+#include "tv3_internal.h"
+
+using namespace tv3;
+
+namespace {
+
+struct Arrs {
+  float *U, *Up, *Ux, *Uxp, *Uy, *Uyp, *Uz, *Uzp, *g, *gp, *g2, *g2p;
+  float *dU, *dUx, *dUy, *dUz, *dg, *dg2, *d, *DtsAtd, *Wx, *Wy, *Wz;
+  float *sx, *sy, *sz, *Atb, *uup, *rcdU, *Uout;
+  float *b, *Au, *Aup, *dAu, *delta, *lastAu, *tmpm;
+};
+
+struct KA {
+  const GroupDesc *G;
+  InstScal *S;
+  double *part;
+  int nblk;
+  uint32_t req, forbid;
+};
+
+#define NLOOP_BEGIN(W)                                                       \
+  const GroupDesc &gd = K.G[blockIdx.y];                                    \
+  const int lb = threadIdx.x % (W);                                          \
+  const int inst = blockIdx.y * (W) + lb;                                    \
+  const InstScal &S = K.S[inst];                                             \
+  const bool on = sel(S.flags, K.req, K.forbid);                             \
+  const long long E = (long long)gd.n * (W);                                 \
+  if (on)                                                                    \
+    for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;  \
+         e += (long long)K.nblk * kBlock) {                                  \
+      const int v = (int)(e / (W));                                          \
+      const long long o = gd.off_n + e;                                      \
+      (void)v;
+
+#define MLOOP_BEGIN(W)                                                       \
+  const GroupDesc &gd = K.G[blockIdx.y];                                    \
+  const int lb = threadIdx.x % (W);                                          \
+  const int inst = blockIdx.y * (W) + lb;                                    \
+  const InstScal &S = K.S[inst];                                             \
+  const bool on = sel(S.flags, K.req, K.forbid);                             \
+  const long long E = (long long)gd.m * (W);                                 \
+  if (on)                                                                    \
+    for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;  \
+         e += (long long)K.nblk * kBlock) {                                  \
+      const long long o = gd.off_m + e;
+
+#define LOOP_END }
+
+// element j (a voxel index) of lane lb of this group's n-vector X
+#define AT(X, j) (X)[gd.off_n + (long long)(j) * (W) + lb]
+
+// ---------------------------------------------------------------------------
+// get_tau (2d:425-453)
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_tau_bb(KA K, Arrs a) {
+  double acc[2] = {0.0, 0.0};
+  {
+    NLOOP_BEGIN(W)
+    const float mdb = (float)S.muDbeta;
+    const float dg = a.g[o] - a.gp[o];
+    const float dg2 = a.g2[o] - a.g2p[o];
+    const float tmp = fmaf(mdb, dg, dg2);
+    const float u = a.uup[o];
+    acc[0] += (double)u * (double)u;
+    acc[1] += (double)u * (double)tmp;
+    LOOP_END
+  }
+  const int sl[2] = {0, 1};
+  block_partials<W, 2>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_tau_first(KA K, Arrs a) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  {
+    NLOOP_BEGIN(W)
+    const Vox w(v, gd.q, gd.p, gd.r);
+    const float d0 = a.d[o];
+    const float dx = AT(a.d, w.xn()) - d0;
+    const float dy = AT(a.d, w.yn()) - d0;
+    acc[0] += (double)dx * (double)dx;
+    acc[1] += (double)dy * (double)dy;
+    if (gd.r > 1) {
+      const float dz = AT(a.d, w.zn()) - d0;
+      acc[2] += (double)dz * (double)dz;
+    }
+    acc[3] += (double)d0 * (double)d0;
+    LOOP_END
+  }
+  const int sl[4] = {0, 1, 2, 3};
+  block_partials<W, 4>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_msq(KA K, const float *x) {
+  double acc[1] = {0.0};
+  {
+    MLOOP_BEGIN(W)
+    const float t = x[o];
+    acc[0] += (double)t * (double)t;
+    LOOP_END
+  }
+  const int sl[1] = {kSlotM};
+  block_partials<W, 1>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// ---------------------------------------------------------------------------
+// descend (2d:481-493) + get_g's n-vector part (2d:396-417)
+// ---------------------------------------------------------------------------
+template <int W, class TS>
+__global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
+                                                    int exact) {
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  {
+    NLOOP_BEGIN(W)
+    const float tf = (float)(-1 * (TS)S.tau);
+    auto un = [&](int j) -> float {
+      const long long oj = gd.off_n + (long long)j * W + lb;
+      float u = fmaf(tf, a.d[oj], a.Up[oj]);
+      if (nonneg) u = (u < 0.0f) ? 0.0f : u;
+      return u;
+    };
+    const Vox w(v, gd.q, gd.p, gd.r);
+    const bool z3 = gd.r > 1;
+    const float u0 = un(v);
+    const float ux = un(w.xn()) - u0;
+    const float uy = un(w.yn()) - u0;
+    const float uz = z3 ? un(w.zn()) - u0 : 0.0f;
+    a.U[o] = u0;
+    a.Ux[o] = ux;
+    a.Uy[o] = uy;
+    if (z3) a.Uz[o] = uz;
+    a.dU[o] = u0 - a.Up[o];
+    const float Vx0 = ux - a.Wx[o];
+    const float Vy0 = uy - a.Wy[o];
+    const float Vz0 = z3 ? uz - a.Wz[o] : 0.0f;
+    acc[0] += (double)Vx0 * (double)Vx0;
+    acc[1] += (double)Vy0 * (double)Vy0;
+    acc[3] += (double)a.sx[o] * (double)Vx0;
+    acc[4] += (double)a.sy[o] * (double)Vy0;
+    if (z3) {
+      acc[2] += (double)Vz0 * (double)Vz0;
+      acc[5] += (double)a.sz[o] * (double)Vz0;
+    }
+    const float d0 = a.d[o];
+    acc[6] += (double)d0 * (double)d0;
+    // g2 = Dt(V) at v
+    const int xp = w.xp(), yp = w.yp();
+    const float VXp = (u0 - un(xp)) - AT(a.Wx, xp);
+    const float VYp = (u0 - un(yp)) - AT(a.Wy, yp);
+    float t = VXp - Vx0;
+    if (!z3) {
+      t = t + VYp;
+      t = t - Vy0;
+    } else {
+      const int zp = w.zp();
+      if (w.v >= w.q && (w.y > 0 || !exact)) {
+        float Yb = VYp;
+        if (w.y == 0) {  // last row of the previous slice
+          const int j = v - w.q;
+          Yb = (un(v - w.s) - un(j)) - AT(a.Wy, j);
+        }
+        t = t + Yb;
+        t = t - Vy0;
+      }
+      if (w.y == 0) {
+        t = t + VYp;
+        t = t - Vy0;
+      }
+      const float VZp = (u0 - un(zp)) - AT(a.Wz, zp);
+      t = t + VZp;
+      t = t - Vz0;
+    }
+    a.g2[o] = t;
+    LOOP_END
+  }
+  const int sl[7] = {0, 1, 2, 3, 4, 5, 6};
+  block_partials<W, 7>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// get_g's m-vector part: Aub = Au - b, lam3 = |Aub|^2, lam5 = <delta, Aub>,
+// and the norms of the A^T-skip test (2d:313-316)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a, int skip) {
+  double acc[4] = {0, 0, 0, 0};
+  {
+    MLOOP_BEGIN(W)
+    const float au = a.Au[o];
+    const float aub = au - a.b[o];
+    acc[0] += (double)aub * (double)aub;
+    acc[1] += (double)a.delta[o] * (double)aub;
+    if (skip) {
+      const float da = au - a.lastAu[o];
+      acc[2] += (double)da * (double)da;
+      acc[3] += (double)au * (double)au;
+    }
+    LOOP_END
+  }
+  const int sl[4] = {kSlotM, kSlotM + 1, kSlotM + 2, kSlotM + 3};
+  block_partials<W, 4>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_copy_m(KA K, const float *src,
+                                                   float *dst) {
+  MLOOP_BEGIN(W)
+  dst[o] = src[o];
+  LOOP_END
+}
+
+// g = gp - Atb: get_g with the A^T product skipped keeps the previous g
+// (already holding -Atb) and subtracts Atb again (2d:350-353, 2d:400)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_skipg(KA K, Arrs a) {
+  NLOOP_BEGIN(W)
+  a.g[o] = a.gp[o] - a.Atb[o];
+  LOOP_END
+}
+
+// ---------------------------------------------------------------------------
+// min_u backtracking (2d:550-575)
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_bt_diff_n(KA K, Arrs a) {
+  NLOOP_BEGIN(W)
+  a.dg[o] = a.g[o] - a.gp[o];
+  a.dg2[o] = a.g2[o] - a.g2p[o];
+  a.dUx[o] = a.Ux[o] - a.Uxp[o];
+  a.dUy[o] = a.Uy[o] - a.Uyp[o];
+  if (gd.r > 1) a.dUz[o] = a.Uz[o] - a.Uzp[o];
+  LOOP_END
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_bt_diff_m(KA K, Arrs a) {
+  MLOOP_BEGIN(W)
+  a.dAu[o] = a.Au[o] - a.Aup[o];
+  LOOP_END
+}
+
+template <class TS>
+__device__ __forceinline__ float interp(float x, TS alpha, float y) {
+  return (float)fma(alpha, (TS)y, (TS)x);
+}
+
+template <int W, class TS>
+__global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  {
+    NLOOP_BEGIN(W)
+    const TS al = (TS)S.alpha;
+    a.g[o] = interp<TS>(a.gp[o], al, a.dg[o]);
+    a.g2[o] = interp<TS>(a.g2p[o], al, a.dg2[o]);
+    a.U[o] = interp<TS>(a.Up[o], al, a.dU[o]);
+    const float ux = interp<TS>(a.Uxp[o], al, a.dUx[o]);
+    const float uy = interp<TS>(a.Uyp[o], al, a.dUy[o]);
+    a.Ux[o] = ux;
+    a.Uy[o] = uy;
+    const float Vx = ux - a.Wx[o], Vy = uy - a.Wy[o];
+    acc[0] += (double)Vx * (double)Vx;
+    acc[1] += (double)Vy * (double)Vy;
+    acc[3] += (double)a.sx[o] * (double)Vx;
+    acc[4] += (double)a.sy[o] * (double)Vy;
+    if (gd.r > 1) {
+      const float uz = interp<TS>(a.Uzp[o], al, a.dUz[o]);
+      a.Uz[o] = uz;
+      const float Vz = uz - a.Wz[o];
+      acc[2] += (double)Vz * (double)Vz;
+      acc[5] += (double)a.sz[o] * (double)Vz;
+    }
+    LOOP_END
+  }
+  const int sl[6] = {0, 1, 2, 3, 4, 5};
+  block_partials<W, 6>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W, class TS>
+__global__ __launch_bounds__(kBlock) void k_bt_upd_m(KA K, Arrs a) {
+  double acc[2] = {0, 0};
+  {
+    MLOOP_BEGIN(W)
+    const TS al = (TS)S.alpha;
+    const float au = interp<TS>(a.Aup[o], al, a.dAu[o]);
+    a.Au[o] = au;
+    const float aub = au - a.b[o];
+    acc[0] += (double)aub * (double)aub;
+    acc[1] += (double)a.delta[o] * (double)aub;
+    LOOP_END
+  }
+  const int sl[2] = {kSlotM, kSlotM + 1};
+  block_partials<W, 2>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// ---------------------------------------------------------------------------
+// D (2d:268-278) alone, for the initial U = A^T b
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_D(KA K, const float *U, float *Ux,
+                                              float *Uy, float *Uz) {
+  NLOOP_BEGIN(W)
+  const Vox w(v, gd.q, gd.p, gd.r);
+  const float u0 = U[o];
+  Ux[o] = AT(U, w.xn()) - u0;
+  Uy[o] = AT(U, w.yn()) - u0;
+  if (gd.r > 1) Uz[o] = AT(U, w.zn()) - u0;
+  LOOP_END
+}
+
+// ---------------------------------------------------------------------------
+// update_W (2d:651-671) + uup + get_gradient + stop-test norms
+// ---------------------------------------------------------------------------
+template <int W, class TS, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int iso,
+                                                  int exact) {
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  {
+    NLOOP_BEGIN(W)
+    const TS beta = (TS)S.beta;
+    const TS ib = (TS)1 / beta;
+    const bool z3 = gd.r > 1;
+    const Vox w(v, gd.q, gd.p, gd.r);
+    // shrinkage at voxel j; returns the T_scal lam1 term when asked
+    auto shr = [&](int j, float &wx, float &wy, float &wz) -> TS {
+      const long long oj = gd.off_n + (long long)j * W + lb;
+      if (iso)
+        return shrink_iso<TS>(a.Ux[oj], a.Uy[oj], z3 ? a.Uz[oj] : 0.0f,
+                              a.sx[oj], a.sy[oj], z3 ? a.sz[oj] : 0.0f, z3,
+                              beta, ib, wx, wy, wz);
+      TS tx, ty, tz = 0;
+      wx = shrink1<TS>(a.Ux[oj], a.sx[oj], beta, ib, tx);
+      wy = shrink1<TS>(a.Uy[oj], a.sy[oj], beta, ib, ty);
+      wz = z3 ? shrink1<TS>(a.Uz[oj], a.sz[oj], beta, ib, tz) : 0.0f;
+      TS t = tx + ty;
+      if (z3) t = t + tz;
+      return t;
+    };
+    float wx0, wy0, wz0;
+    const TS l1 = shr(v, wx0, wy0, wz0);
+    acc[0] += (double)l1;
+    a.Wx[o] = wx0;
+    a.Wy[o] = wy0;
+    if (z3) a.Wz[o] = wz0;
+    const float Vx0 = a.Ux[o] - wx0, Vy0 = a.Uy[o] - wy0;
+    const float Vz0 = z3 ? a.Uz[o] - wz0 : 0.0f;
+    acc[1] += (double)Vx0 * (double)Vx0;
+    acc[2] += (double)Vy0 * (double)Vy0;
+    acc[4] += (double)a.sx[o] * (double)Vx0;
+    acc[5] += (double)a.sy[o] * (double)Vy0;
+    if (z3) {
+      acc[3] += (double)Vz0 * (double)Vz0;
+      acc[6] += (double)a.sz[o] * (double)Vz0;
+    }
+    float tx, ty, tz;
+    const int xp = w.xp(), yp = w.yp();
+    shr(xp, tx, ty, tz);
+    const float VXp = AT(a.Ux, xp) - tx;
+    shr(yp, tx, ty, tz);
+    const float VYp = AT(a.Uy, yp) - ty;
+    float t = VXp - Vx0;
+    if (!z3) {
+      t = t + VYp;
+      t = t - Vy0;
+    } else {
+      if (w.v >= w.q && (w.y > 0 || !exact)) {
+        float Yb = VYp;
+        if (w.y == 0) {
+          const int j = v - w.q;
+          shr(j, tx, ty, tz);
+          Yb = AT(a.Uy, j) - ty;
+        }
+        t = t + Yb;
+        t = t - Vy0;
+      }
+      if (w.y == 0) {
+        t = t + VYp;
+        t = t - Vy0;
+      }
+      const int zp = w.zp();
+      shr(zp, tx, ty, tz);
+      const float VZp = AT(a.Uz, zp) - tz;
+      t = t + VZp;
+      t = t - Vz0;
+    }
+    a.g2[o] = t;
+    if (!INIT) {
+      const float up = a.Up[o];
+      const float uu = a.U[o] - up;
+      a.uup[o] = uu;
+      acc[7] += (double)up * (double)up;
+      acc[8] += (double)uu * (double)uu;
+      a.d[o] = fmaf((float)S.muDbeta, a.g[o], a.DtsAtd[o] + t);
+    }
+    LOOP_END
+  }
+  const int sl[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  block_partials<W, 9>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// d = DtsAtd + g2 + muDbeta*g, with DtsAtd = (beta0/beta) d first when
+// scale != 0 (driver, 2d:845-849)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_grad(KA K, Arrs a, int scale) {
+  NLOOP_BEGIN(W)
+  float dts = a.DtsAtd[o];
+  if (scale) {
+    dts = (float)S.ratio * a.d[o];
+    a.DtsAtd[o] = dts;
+  }
+  a.d[o] = fmaf((float)S.muDbeta, a.g[o], dts + a.g2[o]);
+  LOOP_END
+}
+
+// ---------------------------------------------------------------------------
+// outer iteration (2d:820-852)
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_outer1(KA K, Arrs a) {
+  double acc[1] = {0};
+  {
+    NLOOP_BEGIN(W)
+    const float u = a.U[o];
+    const float df = u - a.rcdU[o];
+    acc[0] += (double)df * (double)df;
+    a.rcdU[o] = u;
+    LOOP_END
+  }
+  const int sl[1] = {0};
+  block_partials<W, 1>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_mlp_n(KA K, Arrs a) {
+  double acc[3] = {0, 0, 0};
+  {
+    NLOOP_BEGIN(W)
+    const float nb = (float)(-1 * S.beta);
+    const float Vx = a.Ux[o] - a.Wx[o], Vy = a.Uy[o] - a.Wy[o];
+    const float sx = fmaf(nb, Vx, a.sx[o]), sy = fmaf(nb, Vy, a.sy[o]);
+    a.sx[o] = sx;
+    a.sy[o] = sy;
+    acc[0] += (double)sx * (double)Vx;
+    acc[1] += (double)sy * (double)Vy;
+    if (gd.r > 1) {
+      const float Vz = a.Uz[o] - a.Wz[o];
+      const float sz = fmaf(nb, Vz, a.sz[o]);
+      a.sz[o] = sz;
+      acc[2] += (double)sz * (double)Vz;
+    }
+    LOOP_END
+  }
+  const int sl[3] = {0, 1, 2};
+  block_partials<W, 3>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_mlp_m(KA K, Arrs a) {
+  double acc[1] = {0};
+  {
+    MLOOP_BEGIN(W)
+    const float nm = (float)(-1 * S.mu);
+    const float aub = a.Au[o] - a.b[o];
+    const float dl = fmaf(nm, aub, a.delta[o]);
+    a.delta[o] = dl;
+    acc[0] += (double)dl * (double)aub;
+    LOOP_END
+  }
+  const int sl[1] = {kSlotM};
+  block_partials<W, 1>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// U_out = U / scl (driver, 2d:856-860)
+template <int W, class TS>
+__global__ __launch_bounds__(kBlock) void k_finish(KA K, Arrs a) {
+  NLOOP_BEGIN(W)
+  a.Uout[o] = (float)((TS)a.U[o] / (TS)S.scl);
+  LOOP_END
+}
+
+// |Ut - U|^2 and |Ut|^2 for info.rel_error (2d:869-874)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_relerr(KA K, const float *Ut,
+                                                   const float *U) {
+  double acc[2] = {0, 0};
+  {
+    NLOOP_BEGIN(W)
+    const float t = Ut[o];
+    const float df = t - U[o];
+    acc[0] += (double)df * (double)df;
+    acc[1] += (double)t * (double)t;
+    LOOP_END
+  }
+  const int sl[2] = {0, 1};
+  block_partials<W, 2>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// ---------------------------------------------------------------------------
+// scaleb (2d:228-266): first index of min |b| and of max |b|
+// ---------------------------------------------------------------------------
+struct MinMax {
+  float mn, mx;
+  int imn, imx;
+};
+
+__device__ __forceinline__ void mm_merge(MinMax &a, const MinMax &b) {
+  if (b.mn < a.mn || (b.mn == a.mn && b.imn < a.imn)) { a.mn = b.mn; a.imn = b.imn; }
+  if (b.mx > a.mx || (b.mx == a.mx && b.imx < a.imx)) { a.mx = b.mx; a.imx = b.imx; }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const float *bv,
+                                                      MinMax *mmpart) {
+  const GroupDesc &gd = K.G[blockIdx.y];
+  const int lb = threadIdx.x % W;
+  const int inst = blockIdx.y * W + lb;
+  MinMax r{INFINITY, -INFINITY, 0x7fffffff, 0x7fffffff};
+  const long long E = (long long)gd.m * W;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
+       e += (long long)K.nblk * kBlock) {
+    const float t = fabsf(bv[gd.off_m + e]);
+    const int i = (int)(e / W);
+    MinMax c{t, t, i, i};
+    mm_merge(r, c);
+  }
+  __shared__ MinMax red[kBlock];
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int s = kBlock / 2; s >= W; s >>= 1) {
+    if ((int)threadIdx.x < s) mm_merge(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < W)
+    mmpart[(long long)inst * kNblkMax + blockIdx.x] = red[threadIdx.x];
+}
+
+// b = (float)scl * b (cblas_scal takes a float alpha)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_scale_m(KA K, float *x) {
+  MLOOP_BEGIN(W)
+  x[o] = (float)S.scl * x[o];
+  LOOP_END
+}
+
+// U = rcdU = x (initial guess U = A^T b, 2d:775, 787)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_init_U(KA K, Arrs a) {
+  NLOOP_BEGIN(W)
+  const float t = a.Atb[o];
+  a.U[o] = t;
+  a.rcdU[o] = t;
+  LOOP_END
+}
+
+// pack caller vectors (one pointer per lane, NULL = zeros) into the
+// lane-interleaved layout, and back
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_pack(const GroupDesc *G,
+                                                 const float *const *src,
+                                                 float *dst, int use_m) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const long long len = use_m ? gd.m : gd.n;
+  const long long off = use_m ? gd.off_m : gd.off_n;
+  const long long E = len * W;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
+       e += (long long)gridDim.x * kBlock) {
+    const int lb = (int)(e % W);
+    const float *p = src[blockIdx.y * W + lb];
+    dst[off + e] = p ? p[e / W] : 0.0f;
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_unpack(const GroupDesc *G,
+                                                   const float *src,
+                                                   float *const *dst) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const long long E = (long long)gd.n * W;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
+       e += (long long)gridDim.x * kBlock) {
+    const int lb = (int)(e % W);
+    float *p = dst[blockIdx.y * W + lb];
+    if (p) p[e / W] = src[gd.off_n + e];
+  }
+}
+
+// ===========================================================================
+// per-instance scalar control (one 256-thread block per instance)
+// ===========================================================================
+struct FA {
+  InstScal *S;
+  const GroupDesc *G;
+  const double *part;
+  int nblk_n, nblk_m;
+  int W;
+  uint32_t req, forbid;
+  uint32_t *summary;
+  tv3_trace_rec *trace;
+  int trace_cap;
+  OptDev O;
+};
+
+__device__ __forceinline__ double slot_sum(const FA &F, int inst, int slot) {
+  const int nb = slot >= kSlotM ? F.nblk_m : F.nblk_n;
+  return block_sum_partials(
+      F.part + ((long long)inst * kNslot + slot) * kNblkMax, nb);
+}
+
+// f = lam1 + beta/2*lam2 + mu/2*lam3 - lam4 - lam5 (2d:422, 523, 842)
+template <class TS>
+__device__ __forceinline__ float f_full(const InstScal &s) {
+  const TS lam1 = (TS)s.lam1, lam2 = (TS)s.lam2, lam3 = (TS)s.lam3;
+  const TS beta = (TS)s.beta, mu = (TS)s.mu;
+  return (float)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)s.lam4) -
+                 (TS)s.lam5);
+}
+
+// the Armijo test of min_u (2d:548, 558): |f| > |C - alpha*c_armij|
+template <class TS>
+__device__ __forceinline__ bool armijo_fails(const InstScal &s) {
+  return (TS)fabsf(s.f) > fabs((TS)s.C - (TS)s.alpha * (TS)s.c_armij);
+}
+
+#define FIN_BEGIN                                                      \
+  const int inst = blockIdx.x;                                         \
+  InstScal &s = F.S[inst];                                             \
+  const uint32_t fl0 = s.flags;                                        \
+  const bool on = sel(fl0, F.req, F.forbid);                           \
+  if (!on) {                                                           \
+    if (threadIdx.x == 0 && F.summary) atomicOr(F.summary, fl0);       \
+    return;                                                            \
+  }                                                                    \
+  const GroupDesc &gd = F.G[inst / F.W];                               \
+  (void)gd;
+
+#define FIN_END                                                        \
+  if (threadIdx.x == 0 && F.summary) atomicOr(F.summary, s.flags);
+
+// lam2 and lam4 from three direction dot products (get_g / update_g /
+// update_W: 2d:408-411)
+template <class TS>
+__device__ __forceinline__ void set_lam24(InstScal &s, bool z3, double x2,
+                                          double y2, double z2, double x4,
+                                          double y4, double z4) {
+  TS lam2 = (TS)(float)x2;
+  lam2 += (TS)(float)y2;
+  if (z3) lam2 += (TS)(float)z2;
+  float lam4 = (float)x4;
+  lam4 += (float)y4;
+  if (z3) lam4 += (float)z4;
+  s.lam2 = (double)lam2;
+  s.lam4 = lam4;
+}
+
+// clears per-step flags of every instance and sets tau for active ones
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_tau(FA F) {
+  const int inst = blockIdx.x;
+  InstScal &s = F.S[inst];
+  const uint32_t fl = s.flags;
+  const bool fb = (F.req & F_FB) != 0;
+  if (!fb && !sel(fl, F.req, F.forbid)) {
+    if (threadIdx.x == 0) {
+      s.flags = fl & ~(F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB |
+                       F_BWD | F_SKIPG);
+      if (F.summary) atomicOr(F.summary, s.flags);
+    }
+    return;
+  }
+  if (fb && !(fl & F_FB)) return;
+  const GroupDesc &gd = F.G[inst / F.W];
+  const bool first = fb || (fl & F_FST);
+  double a0, a1, a2 = 0, a3 = 0, am = 0;
+  a0 = slot_sum(F, inst, 0);
+  a1 = slot_sum(F, inst, 1);
+  if (first) {
+    a2 = slot_sum(F, inst, 2);
+    a3 = slot_sum(F, inst, 3);
+    am = slot_sum(F, inst, kSlotM);
+  }
+  if (threadIdx.x == 0) {
+    TS tau;
+    if (first) {
+      float dDd = (float)a0 + (float)a1;
+      if (gd.r > 1) dDd = dDd + (float)a2;
+      const float dd = (float)a3, Add = (float)am;
+      tau = fabs((TS)dd / ((TS)dDd + (TS)s.muDbeta * (TS)Add));
+    } else {
+      const float ss = (float)a0, sy = (float)a1;
+      tau = (TS)fabsf(ss / sy);
+    }
+    s.tau = (double)tau;
+    if (!fb) {
+      s.flags = fl & ~(F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB |
+                       F_BWD | F_SKIPG);
+      s.cnt = 0;
+      s.fallback = 0;
+      s.alpha = 1.0;
+    }
+    if (F.summary) atomicOr(F.summary, s.flags);
+  }
+}
+
+// A^T-skipping state machine of get_g_mult (2d:300-357).  force: the
+// initial get_g and the 5-strike fallback.
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_skip(FA F, int force) {
+  FIN_BEGIN
+  const double dn = slot_sum(F, inst, kSlotM + 2);
+  const double an = slot_sum(F, inst, kSlotM + 3);
+  if (threadIdx.x == 0) {
+    const TS nrm_d = (TS)(float)sqrt(dn), nrm_a = (TS)(float)sqrt(an);
+    const TS relChange = nrm_d / nrm_a;
+    uint32_t fl = s.flags & ~(F_BWD | F_SKIPG);
+    TS csv = (TS)s.currSkipValue;
+    if (force) {
+      fl |= F_BWD; csv = 0;
+    } else if (s.currInitIteration < F.O.numInitIterations) {
+      fl |= F_BWD; s.currInitIteration++; csv = 0;
+    } else if (csv < 1) {
+      fl |= F_BWD; csv += (TS)F.O.skipMulRatio;
+    } else if (relChange > (TS)F.O.maxRelativeChange) {
+      fl |= F_BWD;
+    } else {
+      fl |= F_SKIPG; csv -= 1;
+    }
+    s.currSkipValue = (double)csv;
+    s.flags = fl;
+  }
+  FIN_END
+}
+
+// end of descend/get_g: lam2..5, f; mode 0: Armijo test of min_u
+// (2d:543-549); mode 1: after the fallback; mode 2: initial get_g
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_descend(FA F, int mode) {
+  FIN_BEGIN
+  const bool z3 = gd.r > 1;
+  const double x2 = slot_sum(F, inst, 0), y2 = slot_sum(F, inst, 1);
+  const double z2 = slot_sum(F, inst, 2), x4 = slot_sum(F, inst, 3);
+  const double y4 = slot_sum(F, inst, 4), z4 = slot_sum(F, inst, 5);
+  const double dd = slot_sum(F, inst, 6);
+  const double l3 = slot_sum(F, inst, kSlotM), l5 = slot_sum(F, inst, kSlotM + 1);
+  if (threadIdx.x == 0) {
+    set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
+    s.lam3 = (double)(TS)(float)l3;
+    s.lam5 = (float)l5;
+    s.f = f_full<TS>(s);
+    if (mode == 0) {
+      s.c_armij = (double)((TS)(float)dd * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
+      s.alpha = 1.0;
+      if (armijo_fails<TS>(s)) {
+        s.flags |= F_BT;
+        s.cnt = 0;
+        s.alpha = (double)((TS)s.alpha * (TS)F.O.gamma);
+      }
+    } else if (mode == 1) {
+      s.flags &= ~F_FB;
+      s.fallback = 1;
+    }
+  }
+  FIN_END
+}
+
+// one backtracking round of min_u (2d:555-574)
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_bt(FA F) {
+  FIN_BEGIN
+  const bool z3 = gd.r > 1;
+  const double x2 = slot_sum(F, inst, 0), y2 = slot_sum(F, inst, 1);
+  const double z2 = slot_sum(F, inst, 2), x4 = slot_sum(F, inst, 3);
+  const double y4 = slot_sum(F, inst, 4), z4 = slot_sum(F, inst, 5);
+  const double l3 = slot_sum(F, inst, kSlotM), l5 = slot_sum(F, inst, kSlotM + 1);
+  if (threadIdx.x == 0) {
+    set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
+    s.lam3 = (double)(TS)(float)l3;
+    s.lam5 = (float)l5;
+    s.f = f_full<TS>(s);
+    s.cnt++;
+    if (!armijo_fails<TS>(s)) {
+      s.flags &= ~F_BT;
+    } else if (s.cnt == 5) {
+      s.flags = (s.flags & ~F_BT) | F_FB;
+      s.gam = (double)((TS)s.gam * (TS)F.O.rate_gam);
+    } else {
+      s.alpha = (double)((TS)s.alpha * (TS)F.O.gamma);
+    }
+  }
+  FIN_END
+}
+
+// update_W's scalars, the nonmonotone reference update, the inner stop test
+// (2d:653-670, 799-817).  init: the initial shrinkage + get_g (lam1, lam2,
+// lam4 only).
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_wstep(FA F, int init) {
+  FIN_BEGIN
+  const bool z3 = gd.r > 1;
+  const double l1 = slot_sum(F, inst, 0);
+  const double x2 = slot_sum(F, inst, 1), y2 = slot_sum(F, inst, 2);
+  const double z2 = slot_sum(F, inst, 3), x4 = slot_sum(F, inst, 4);
+  const double y4 = slot_sum(F, inst, 5), z4 = slot_sum(F, inst, 6);
+  double nup = 0, nuu = 0;
+  if (!init) {
+    nup = slot_sum(F, inst, 7);
+    nuu = slot_sum(F, inst, 8);
+  }
+  if (threadIdx.x == 0) {
+    const TS beta = (TS)s.beta;
+    if (init) {
+      s.lam1 = (double)(TS)l1;
+      set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
+    } else {
+      s.f = (float)((TS)s.f - (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
+      s.lam1 = (double)(TS)l1;
+      set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
+      s.f = (float)((TS)s.f + (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
+      const TS gam = (TS)s.gam;
+      const TS Qp = (TS)s.Q;
+      const TS Q = gam * Qp + 1;
+      s.Q = (double)Q;
+      s.C = (float)((gam * Qp * (TS)s.C + (TS)s.f) / Q);
+      s.count_total++;
+      const TS nrmup = (TS)(float)sqrt(nup);
+      const TS relchg = (TS)(float)sqrt(nuu) / nrmup;
+      s.nrmup = (double)nrmup;
+      s.RelChg = (double)relchg;
+      if (F.trace && s.count_total <= F.trace_cap) {
+        tv3_trace_rec &t = F.trace[(long long)inst * F.trace_cap + s.count_total - 1];
+        t.iter = s.count_total; t.outer = s.count_outer;
+        t.backtracks = s.cnt; t.fallback = s.fallback;
+        t.tau = s.tau; t.alpha = s.alpha; t.f = s.f; t.C = s.C;
+        t.rel_chg = s.RelChg; t.lam1 = s.lam1; t.lam2 = s.lam2;
+        t.lam3 = s.lam3; t.lam4 = s.lam4; t.lam5 = s.lam5;
+        t.mu = s.mu; t.beta = s.beta;
+      }
+      const bool fst = (s.flags & F_FST) != 0;
+      if ((relchg < (TS)F.O.tol_inn && !fst) || s.count_total >= F.O.maxit)
+        s.flags |= F_INNER_DONE;
+      else
+        s.flags &= ~F_FST;
+    }
+  }
+  FIN_END
+}
+
+// initial f (end of the first get_g) and the nonmonotone reference values
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_init2(FA F) {
+  FIN_BEGIN
+  const double l3 = slot_sum(F, inst, kSlotM), l5 = slot_sum(F, inst, kSlotM + 1);
+  if (threadIdx.x == 0) {
+    s.lam3 = (double)(TS)(float)l3;
+    s.lam5 = (float)l5;
+    s.f = f_full<TS>(s);
+    s.Q = 1.0;
+    s.C = s.f;
+    if (!(0 < F.O.maxcnt && 0 < F.O.maxit))
+      s.flags = (s.flags & ~F_ACTIVE) | F_DONE | F_JUST_DONE;
+  }
+  FIN_END
+}
+
+// scaleb (2d:228-266) and the penalty clamps (2d:776-778)
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_scaleb(FA F, const MinMax *mmp,
+                                                     const float *bv) {
+  FIN_BEGIN
+  __shared__ MinMax red[kBlock];
+  MinMax r{INFINITY, -INFINITY, 0x7fffffff, 0x7fffffff};
+  for (int i = threadIdx.x; i < F.nblk_m; i += kBlock)
+    mm_merge(r, mmp[(long long)inst * kNblkMax + i]);
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) mm_merge(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    TS scl = 1;
+    const int m = gd.m;
+    if (m > 0) {
+      const MinMax &mm = red[0];
+      const int lbx = inst % F.W;
+      // no finite candidate (all NaN / inf): keep b[0], as the reference's
+      // scan starts from it
+      const int imn = (mm.imn >= 0 && mm.imn < m) ? mm.imn : 0;
+      const int imx = (mm.imx >= 0 && mm.imx < m) ? mm.imx : 0;
+      const float bmin = bv[gd.off_m + (long long)imn * F.W + lbx];
+      const float bmax = bv[gd.off_m + (long long)imx * F.W + lbx];
+      const TS b_dif = fabsf(bmax - bmin);
+      const TS th1 = (TS)0.5, th2 = (TS)1.5;
+      if (b_dif < th1)
+        scl = th1 / b_dif;
+      else if (b_dif > th2)
+        scl = th2 / b_dif;
+    }
+    s.scl = (double)scl;
+    TS mu = (TS)F.O.mu0, beta = (TS)F.O.beta0;
+    const TS muf = (TS)F.O.mu, betaf = (TS)F.O.beta;
+    if (mu > muf) mu = muf;
+    if (beta > betaf) beta = betaf;
+    s.mu = (double)mu;
+    s.beta = (double)beta;
+    s.muDbeta = (double)(mu / beta);
+    s.gam = (double)(TS)F.O.gam;
+  }
+  FIN_END
+}
+
+// RelChgOut and the outer stop test (2d:820-828)
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_outer1(FA F) {
+  FIN_BEGIN
+  const double rc = slot_sum(F, inst, 0);
+  if (threadIdx.x == 0) {
+    s.count_outer++;
+    const TS r = (TS)(float)sqrt(rc) / (TS)s.nrmup;
+    s.RelChgOut = (double)r;
+    uint32_t fl = s.flags & ~F_INNER_DONE;
+    if (r < (TS)F.O.tol)
+      fl = (fl & ~F_ACTIVE) | F_DONE | F_JUST_DONE;
+    else
+      fl |= F_MLP;
+    s.flags = fl;
+  }
+  FIN_END
+}
+
+// update_mlp's scalars and the continuation step (2d:673-694, 831-851)
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_mlp(FA F) {
+  FIN_BEGIN
+  const bool z3 = gd.r > 1;
+  const double x4 = slot_sum(F, inst, 0), y4 = slot_sum(F, inst, 1);
+  const double z4 = slot_sum(F, inst, 2), l5 = slot_sum(F, inst, kSlotM);
+  if (threadIdx.x == 0) {
+    s.f = s.f + (s.lam4 + s.lam5);
+    float lam4 = (float)x4;
+    lam4 += (float)y4;
+    if (z3) lam4 += (float)z4;
+    s.lam4 = lam4;
+    s.lam5 = (float)l5;
+    s.f = s.f - (s.lam4 + s.lam5);
+    const TS beta0 = (TS)s.beta;
+    TS beta = beta0 * (TS)F.O.rate_cnt;
+    TS mu = (TS)s.mu * (TS)F.O.rate_cnt;
+    if (beta > (TS)F.O.beta) beta = (TS)F.O.beta;
+    if (mu > (TS)F.O.mu) mu = (TS)F.O.mu;
+    s.beta0 = (double)beta0;
+    s.beta = (double)beta;
+    s.mu = (double)mu;
+    s.muDbeta = (double)(mu / beta);
+    s.f = f_full<TS>(s);
+    s.ratio = (double)(float)(beta0 / beta);
+    s.gam = (double)(TS)F.O.gam;
+    s.Q = 1.0;
+    s.C = s.f;
+    if (s.count_outer < F.O.maxcnt && s.count_total < F.O.maxit)
+      s.flags |= F_FST;
+    else
+      s.flags = (s.flags & ~F_ACTIVE) | F_DONE | F_JUST_DONE;
+  }
+  FIN_END
+}
+
+// end of a step: drops the per-step flags of every instance and reports the
+// OR of all flags to the host
+__global__ __launch_bounds__(kBlock) void fin_refresh(FA F) {
+  if (threadIdx.x != 0) return;
+  InstScal &s = F.S[blockIdx.x];
+  s.flags &= ~(F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB | F_BWD | F_SKIPG);
+  if (F.summary) atomicOr(F.summary, s.flags);
+}
+
+template <class TS>
+__global__ __launch_bounds__(kBlock) void fin_relerr(FA F) {
+  FIN_BEGIN
+  const double e2 = slot_sum(F, inst, 0), t2 = slot_sum(F, inst, 1);
+  if (threadIdx.x == 0)
+    s.rel_error = (double)((TS)(float)sqrt(e2) / (TS)(float)sqrt(t2));
+  FIN_END
+}
+
+__global__ void k_init_scal(InstScal *S, int B, int Bpad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Bpad) return;
+  InstScal z;
+  memset(&z, 0, sizeof(z));
+  z.flags = i < B ? (F_LIVE | F_ACTIVE | F_FST) : 0u;
+  z.alpha = 1.0;
+  z.Q = 1.0;
+  S[i] = z;
+}
+
+// ===========================================================================
+// host driver
+// ===========================================================================
+struct Batch {
+  int B = 0, G = 0, W = 1, Bpad = 0;
+  std::vector<GroupDesc> groups;
+  int max_n = 0, max_m = 0, kind = 0;
+  long long Nn = 0, Nm = 0;
+  bool z3 = false;
+};
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <int W, class TS>
+class Engine {
+ public:
+  Engine(tv3_ctx *ctx, const Batch &bt, const tv3_options &o)
+      : ctx_(ctx), s_(ctx->stream), bt_(bt) {
+    O_.mu = o.mu; O_.beta = o.beta; O_.mu0 = o.mu0; O_.beta0 = o.beta0;
+    O_.tol = o.tol; O_.tol_inn = o.tol_inn; O_.maxit = o.maxit;
+    O_.maxcnt = o.maxcnt; O_.gam = o.gam; O_.gamma = o.gamma;
+    O_.rate_gam = o.rate_gam; O_.rate_cnt = o.rate_cnt; O_.c = o.c;
+    O_.nonneg = o.nonneg; O_.tv_iso = o.tv_iso; O_.dt_exact = o.dt_exact;
+    O_.numInitIterations = o.numInitIterations;
+    O_.skipMulRatio = o.skipMulRatio;
+    O_.maxRelativeChange = o.maxRelativeChange;
+    O_.skip = (o.skipMulRatio != 0 && bt.kind != kDense) ? 1 : 0;
+    nblk_n_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
+    nblk_m_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
+  }
+
+  int setup() {
+    // 22 common n-vectors + 5 z-direction ones (3-D) + uup, rcdU, Uout
+    const int nn_arrays = bt_.z3 ? 30 : 25;
+    const int nm_arrays = 7;
+    size_t need = 0;
+    need += align_up(sizeof(float) * bt_.Nn) * nn_arrays;
+    need += align_up(sizeof(float) * bt_.Nm) * nm_arrays;
+    need += align_up(sizeof(double) * (size_t)bt_.Bpad * kNslot * kNblkMax);
+    need += align_up(sizeof(MinMax) * (size_t)bt_.Bpad * kNblkMax);
+    need += align_up(sizeof(InstScal) * bt_.Bpad);
+    need += align_up(sizeof(GroupDesc) * bt_.G);
+    need += align_up(sizeof(uint32_t) * 4);
+    need += align_up(sizeof(float *) * bt_.Bpad) * 2;
+    if (bt_.kind == kDense && W == 1)
+      need += align_up(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n);
+    if (ctx_->ws_bytes < need) {
+      if (ctx_->ws) {
+        (void)hipStreamSynchronize(s_);
+        (void)hipFree(ctx_->ws);
+        ctx_->ws = nullptr;
+        ctx_->ws_bytes = 0;
+      }
+      int rc = hip_check(ctx_, hipMalloc(&ctx_->ws, need), "workspace hipMalloc");
+      if (rc) return rc;
+      ctx_->ws_bytes = need;
+    }
+    char *p = (char *)ctx_->ws;
+    auto take = [&](size_t bytes) { char *r = p; p += align_up(bytes); return r; };
+    const size_t fn = sizeof(float) * bt_.Nn, fm = sizeof(float) * bt_.Nm;
+    float **nptrs[] = {&a_.U, &a_.Up, &a_.Ux, &a_.Uxp, &a_.Uy, &a_.Uyp,
+                       &a_.g, &a_.gp, &a_.g2, &a_.g2p, &a_.dU, &a_.dUx,
+                       &a_.dUy, &a_.dg, &a_.dg2, &a_.d, &a_.DtsAtd, &a_.Wx,
+                       &a_.Wy, &a_.sx, &a_.sy, &a_.Atb};
+    for (float **q : nptrs) *q = (float *)take(fn);
+    if (bt_.z3) {
+      float **zp[] = {&a_.Uz, &a_.Uzp, &a_.dUz, &a_.Wz, &a_.sz};
+      for (float **q : zp) *q = (float *)take(fn);
+    } else {
+      a_.Uz = a_.Uzp = a_.dUz = a_.Wz = a_.sz = nullptr;
+    }
+    a_.uup = (float *)take(fn);
+    a_.rcdU = (float *)take(fn);
+    a_.Uout = (float *)take(fn);
+    float **mptrs[] = {&a_.b, &a_.Au, &a_.Aup, &a_.dAu, &a_.delta, &a_.lastAu, &a_.tmpm};
+    for (float **q : mptrs) *q = (float *)take(fm);
+    part_ = (double *)take(sizeof(double) * (size_t)bt_.Bpad * kNslot * kNblkMax);
+    mm_ = (MinMax *)take(sizeof(MinMax) * (size_t)bt_.Bpad * kNblkMax);
+    S_ = (InstScal *)take(sizeof(InstScal) * bt_.Bpad);
+    G_ = (GroupDesc *)take(sizeof(GroupDesc) * bt_.G);
+    summary_ = (uint32_t *)take(sizeof(uint32_t) * 4);
+    ptrs_in_ = (const float **)take(sizeof(float *) * bt_.Bpad);
+    ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
+    scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n) : nullptr;
+    if ((size_t)(p - (char *)ctx_->ws) > need)
+      return set_err(ctx_, TV3_ERR_ALLOC, "internal error: workspace layout exceeds its size");
+    // zero all state (multipliers, W, DtsAtd start at 0; mat<T> zero-fills)
+    int rc = hip_check(ctx_, hipMemsetAsync(ctx_->ws, 0, (size_t)(p - (char *)ctx_->ws), s_), "workspace clear");
+    if (rc) return rc;
+    rc = hip_check(ctx_, hipMemcpyAsync(G_, bt_.groups.data(), sizeof(GroupDesc) * bt_.G, hipMemcpyHostToDevice, s_), "group copy");
+    return rc;
+  }
+
+  int run(const float *const *b, float *const *U, const float *const *Ut,
+          tv3_info *infos, tv3_trace_rec *trace, int trace_cap) {
+    int rc = setup();
+    if (rc) return rc;
+    tv3_trace_rec *d_trace = nullptr;
+    if (trace && trace_cap > 0) {
+      rc = hip_check(ctx_, hipMalloc(&d_trace, sizeof(tv3_trace_rec) * (size_t)bt_.Bpad * trace_cap), "trace hipMalloc");
+      if (rc) return rc;
+      rc = hip_check(ctx_, hipMemsetAsync(d_trace, 0, sizeof(tv3_trace_rec) * (size_t)bt_.Bpad * trace_cap, s_), "trace clear");
+      if (rc) { (void)hipFree(d_trace); return rc; }
+    }
+    trace_ = d_trace;
+    trace_cap_ = trace_cap;
+    rc = solve(b, U, Ut, infos);
+    if (!rc && d_trace)
+      rc = hip_check(ctx_, hipMemcpyAsync(trace, d_trace, sizeof(tv3_trace_rec) * (size_t)bt_.B * trace_cap, hipMemcpyDeviceToHost, s_), "trace copy");
+    if (!rc) rc = hip_check(ctx_, hipStreamSynchronize(s_), "final sync");
+    if (!rc && ctx_->prof) prof_collect(ctx_);
+    if (d_trace) (void)hipFree(d_trace);
+    return rc;
+  }
+
+ private:
+  tv3_ctx *ctx_;
+  hipStream_t s_;
+  const Batch &bt_;
+  OptDev O_{};
+  int nblk_n_ = 1, nblk_m_ = 1;
+  Arrs a_{};
+  double *part_ = nullptr;
+  MinMax *mm_ = nullptr;
+  InstScal *S_ = nullptr;
+  GroupDesc *G_ = nullptr;
+  uint32_t *summary_ = nullptr;
+  const float **ptrs_in_ = nullptr;
+  float **ptrs_out_ = nullptr;
+  float *scratch_ = nullptr;
+  tv3_trace_rec *trace_ = nullptr;
+  int trace_cap_ = 0;
+
+  KA kn(uint32_t req, uint32_t forbid = 0) const { return KA{G_, S_, part_, nblk_n_, req, forbid}; }
+  KA km(uint32_t req, uint32_t forbid = 0) const { return KA{G_, S_, part_, nblk_m_, req, forbid}; }
+  dim3 gn() const { return dim3(nblk_n_, bt_.G); }
+  dim3 gm() const { return dim3(nblk_m_, bt_.G); }
+  FA fa(uint32_t req, uint32_t forbid = 0) const {
+    FA f;
+    f.S = S_; f.G = G_; f.part = part_; f.nblk_n = nblk_n_; f.nblk_m = nblk_m_;
+    f.W = W; f.req = req; f.forbid = forbid; f.summary = summary_;
+    f.trace = trace_; f.trace_cap = trace_cap_; f.O = O_;
+    return f;
+  }
+  dim3 gi() const { return dim3(bt_.Bpad); }
+
+  void fwd(const float *x, float *y, uint32_t mask) {
+    prof_begin(ctx_, 0);
+    launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, x, y, S_, mask);
+    prof_end(ctx_);
+  }
+  void bwd(const float *x, float *y, uint32_t mask, const float *sub) {
+    prof_begin(ctx_, 1);
+    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, x, y, S_, mask, sub, scratch_);
+    prof_end(ctx_);
+  }
+
+  int clear_summary() {
+    return hip_check(ctx_, hipMemsetAsync(summary_, 0, sizeof(uint32_t), s_), "summary clear");
+  }
+  int read_summary(uint32_t &out) {
+    int rc = hip_check(ctx_, hipMemcpyAsync(ctx_->pinned, summary_, sizeof(uint32_t), hipMemcpyDeviceToHost, s_), "summary copy");
+    if (!rc) rc = hip_check(ctx_, hipStreamSynchronize(s_), "summary sync");
+    out = rc ? 0u : *(volatile uint32_t *)ctx_->pinned;
+    return rc;
+  }
+
+  // get_g after U and Ux.. are in place: A U, A^T skip state, A^T, lam3/5
+  void get_g_products(uint32_t req, uint32_t forbid, int force) {
+    fwd(a_.U, a_.Au, req);
+    hipLaunchKernelGGL(k_mpost<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_, O_.skip);
+    if (O_.skip) {
+      hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kBlock), 0, s_, fa(req, forbid), force);
+      hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
+      bwd(a_.Au, a_.g, req | F_BWD, a_.Atb);
+      hipLaunchKernelGGL(k_skipg<W>, gn(), dim3(kBlock), 0, s_, kn(req | F_SKIPG, forbid), a_);
+    } else {
+      bwd(a_.Au, a_.g, req, a_.Atb);
+    }
+  }
+
+  // tau of the first inner iteration / fallback: D(d), |d|^2, A d
+  void tau_first(uint32_t req, uint32_t forbid) {
+    hipLaunchKernelGGL(k_tau_first<W>, gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
+    fwd(a_.d, a_.tmpm, req);
+    hipLaunchKernelGGL(k_msq<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_.tmpm);
+  }
+
+  int solve(const float *const *b, float *const *U, const float *const *Ut,
+            tv3_info *infos) {
+    int rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    // instance scalars and caller pointers
+    hipLaunchKernelGGL(k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad);
+    std::vector<const float *> hin(bt_.Bpad, nullptr);
+    for (int i = 0; i < bt_.B; ++i) hin[i] = b[i];
+    rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hin.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
+    if (rc) return rc;
+    const dim3 gp_(std::max(1, std::min(nblk_m_, 1024)), bt_.G);
+    hipLaunchKernelGGL(k_pack<W>, gp_, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.b, 1);
+    // scaleb
+    hipLaunchKernelGGL(k_absminmax<W>, gm(), dim3(kBlock), 0, s_, km(F_LIVE), a_.b, mm_);
+    hipLaunchKernelGGL(fin_scaleb<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE), mm_, a_.b);
+    hipLaunchKernelGGL(k_scale_m<W>, gm(), dim3(kBlock), 0, s_, km(F_LIVE), a_.b);
+    // Atb = A^T bs; U = Atb
+    bwd(a_.b, a_.Atb, F_LIVE, nullptr);
+    hipLaunchKernelGGL(k_init_U<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_);
+    // D(U), shrinkage, get_g(force), get_gradient
+    hipLaunchKernelGGL(k_D<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.U, a_.Ux, a_.Uy, a_.Uz);
+    hipLaunchKernelGGL((k_wstep<W, TS, true>), gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, O_.tv_iso, O_.dt_exact);
+    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE), 1);
+    get_g_products(F_LIVE, 0, 1);
+    hipLaunchKernelGGL(fin_init2<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE));
+    hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, 0);
+    hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
+    rc = hip_check(ctx_, hipGetLastError(), "init kernels");
+    if (rc) return rc;
+
+    uint32_t sum = (O_.maxcnt > 0 && O_.maxit > 0) ? (F_ACTIVE | F_FST) : 0u;
+    while (sum & F_ACTIVE) {
+      // ---- min_u: get_tau (2d:533) ----
+      hipLaunchKernelGGL(k_tau_bb<W>, gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE, F_FST), a_);
+      if (sum & F_FST) tau_first(F_ACTIVE | F_FST, 0);
+      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE));
+      // keep previous values (2d:536-537) by swapping buffers
+      std::swap(a_.U, a_.Up); std::swap(a_.Ux, a_.Uxp); std::swap(a_.Uy, a_.Uyp);
+      std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
+      std::swap(a_.Au, a_.Aup);
+      // descend (2d:540)
+      hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.nonneg, O_.dt_exact);
+      get_g_products(F_ACTIVE, 0, 0);
+      if ((rc = clear_summary())) return rc;
+      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
+      if ((rc = read_summary(sum))) return rc;
+      if (sum & F_BT) {
+        hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+        hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+        while (sum & F_BT) {
+          hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+          hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+          if ((rc = clear_summary())) return rc;
+          hipLaunchKernelGGL(fin_bt<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT));
+          if ((rc = read_summary(sum))) return rc;
+        }
+        if (sum & F_FB) {
+          // 5-strike fallback (2d:561-567): tau from the first-iteration
+          // formula, U = Up, descend(force)
+          tau_first(F_FB, 0);
+          hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
+          hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_, O_.nonneg, O_.dt_exact);
+          get_g_products(F_FB, 0, 1);
+          hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
+        }
+      }
+      // ---- update_W, reference values, gradient, stop test ----
+      hipLaunchKernelGGL((k_wstep<W, TS, false>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.tv_iso, O_.dt_exact);
+      if ((rc = clear_summary())) return rc;
+      hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
+      if ((rc = read_summary(sum))) return rc;
+      if (sum & F_INNER_DONE) {
+        hipLaunchKernelGGL(k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
+        hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kBlock), 0, s_, fa(F_INNER_DONE));
+        hipLaunchKernelGGL(k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
+        hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
+        hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kBlock), 0, s_, fa(F_MLP));
+        hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
+        hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
+        if ((rc = clear_summary())) return rc;
+        hipLaunchKernelGGL(fin_refresh, gi(), dim3(kBlock), 0, s_, fa(0));
+        if ((rc = read_summary(sum))) return rc;
+      }
+      rc = hip_check(ctx_, hipGetLastError(), "step kernels");
+      if (rc) return rc;
+    }
+    // outputs, rel_error
+    std::vector<float *> hout(bt_.Bpad, nullptr);
+    for (int i = 0; i < bt_.B; ++i) hout[i] = U[i];
+    rc = hip_check(ctx_, hipMemcpyAsync(ptrs_out_, hout.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
+    if (rc) return rc;
+    const dim3 gpn(std::max(1, std::min(nblk_n_, 1024)), bt_.G);
+    hipLaunchKernelGGL(k_unpack<W>, gpn, dim3(kBlock), 0, s_, G_, a_.Uout, ptrs_out_);
+    bool any_ut = false;
+    std::vector<const float *> hut(bt_.Bpad, nullptr);  // alive until the sync below
+    if (Ut) {
+      for (int i = 0; i < bt_.B; ++i) { hut[i] = Ut[i]; any_ut |= Ut[i] != nullptr; }
+      if (any_ut) {
+        rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hut.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack<W>, gpn, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.rcdU, 0);
+        hipLaunchKernelGGL(k_relerr<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.rcdU, a_.Uout);
+        hipLaunchKernelGGL(fin_relerr<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE));
+      }
+    }
+    std::vector<InstScal> hs(bt_.Bpad);
+    rc = hip_check(ctx_, hipMemcpyAsync(hs.data(), S_, sizeof(InstScal) * bt_.Bpad, hipMemcpyDeviceToHost, s_), "scalar copy");
+    if (!rc) rc = hip_check(ctx_, hipStreamSynchronize(s_), "solve sync");
+    if (rc) return rc;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < bt_.B; ++i) {
+      tv3_info &inf = infos[i];
+      inf.secs = secs;
+      inf.outer_iters = hs[i].count_outer;
+      inf.total_iters = hs[i].count_total;
+      inf.rel_chg = hs[i].RelChgOut;
+      inf.rel_error = (Ut && Ut[i]) ? hs[i].rel_error : 0.0;
+    }
+    return TV3_OK;
+  }
+};
+
+int check_opts(tv3_ctx *ctx, const tv3_options *o) {
+  if (!o) return set_err(ctx, TV3_ERR_ARG, "Argument error: options missing");
+  if (o->maxit < 0 || o->maxcnt < 0)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: maxit and maxcnt must be >= 0");
+  return TV3_OK;
+}
+
+int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                     const tv3_options *o, const int *dims,
+                     const float *const *b, float *const *U,
+                     const float *const *Ut, tv3_info *infos,
+                     tv3_trace_rec *trace, int trace_cap) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (B <= 0 || !A || !dims || !b || !U || !infos)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: batch arguments missing");
+  int rc = check_opts(ctx, o);
+  if (rc) return rc;
+  if ((rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
+  Batch bt;
+  bt.B = B;
+  bool shared = true;
+  for (int i = 0; i < B; ++i) {
+    if (!A[i] || !b[i] || !U[i]) return set_err(ctx, TV3_ERR_ARG, "Argument error: null plan or vector");
+    const int q = dims[3 * i], p = dims[3 * i + 1], r = dims[3 * i + 2];
+    if (q <= 0 || p <= 0 || r <= 0)
+      return set_err(ctx, TV3_ERR_ARG, "Argument error: volume dimensions must be positive");
+    if ((long long)q * p * r != A[i]->dev.n)
+      return set_err(ctx, TV3_ERR_ARG, "Argument error: the length of U must be equal to A.dim_x!");
+    if (A[i]->dev.kind != A[0]->dev.kind)
+      return set_err(ctx, TV3_ERR_ARG, "Argument error: one batch needs one operator kind");
+    if (i > 0 && (A[i] != A[0] || q != dims[0] || p != dims[1] || r != dims[2])) shared = false;
+    if (r > 1) bt.z3 = true;
+  }
+  bt.kind = A[0]->dev.kind;
+  bt.W = (shared && B > 1) ? 64 : 1;
+  const int W = bt.W;
+  bt.G = (B + W - 1) / W;
+  bt.Bpad = bt.G * W;
+  long long on = 0, om = 0;
+  for (int g = 0; g < bt.G; ++g) {
+    const int i = g * W;
+    GroupDesc gd{};
+    gd.q = dims[3 * i]; gd.p = dims[3 * i + 1]; gd.r = dims[3 * i + 2];
+    gd.n = A[i]->dev.n; gd.m = A[i]->dev.m;
+    gd.off_n = on; gd.off_m = om;
+    gd.plan = A[i]->dev;
+    on += (long long)gd.n * W;
+    om += (long long)gd.m * W;
+    bt.max_n = std::max(bt.max_n, gd.n);
+    bt.max_m = std::max(bt.max_m, gd.m);
+    bt.groups.push_back(gd);
+  }
+  bt.Nn = std::max<long long>(on, 1);
+  bt.Nm = std::max<long long>(om, 1);
+  if (W == 64) {
+    if (o->scal_double) {
+      Engine<64, double> e(ctx, bt, *o);
+      return e.run(b, U, Ut, infos, trace, trace_cap);
+    }
+    Engine<64, float> e(ctx, bt, *o);
+    return e.run(b, U, Ut, infos, trace, trace_cap);
+  }
+  if (o->scal_double) {
+    Engine<1, double> e(ctx, bt, *o);
+    return e.run(b, U, Ut, infos, trace, trace_cap);
+  }
+  Engine<1, float> e(ctx, bt, *o);
+  return e.run(b, U, Ut, infos, trace, trace_cap);
+}
+
+// ---------------------------------------------------------------------------
+// standalone operators (single volume, W = 1)
+// ---------------------------------------------------------------------------
+template <class TS>
+__global__ __launch_bounds__(kBlock) void k_shrink_only(KA K, Arrs a, int iso) {
+  double acc[1] = {0};
+  {
+    constexpr int W = 1;
+    NLOOP_BEGIN(1)
+    const TS beta = (TS)S.beta, ib = (TS)1 / beta;
+    const bool z3 = gd.r > 1;
+    float wx, wy, wz = 0.0f;
+    TS t;
+    if (iso) {
+      t = shrink_iso<TS>(a.Ux[o], a.Uy[o], z3 ? a.Uz[o] : 0.0f, a.sx[o], a.sy[o],
+                         z3 ? a.sz[o] : 0.0f, z3, beta, ib, wx, wy, wz);
+    } else {
+      TS tx, ty, tz = 0;
+      wx = shrink1<TS>(a.Ux[o], a.sx[o], beta, ib, tx);
+      wy = shrink1<TS>(a.Uy[o], a.sy[o], beta, ib, ty);
+      if (z3) wz = shrink1<TS>(a.Uz[o], a.sz[o], beta, ib, tz);
+      t = tx + ty;
+      if (z3) t = t + tz;
+    }
+    a.Wx[o] = wx;
+    a.Wy[o] = wy;
+    if (z3) a.Wz[o] = wz;
+    acc[0] += (double)t;
+    (void)W;
+    LOOP_END
+  }
+  const int sl[1] = {0};
+  block_partials<1, 1>(acc, sl, K.part, 0, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+__global__ __launch_bounds__(kBlock) void k_Dt_only(KA K, const float *X,
+                                                    const float *Y,
+                                                    const float *Z, float *res,
+                                                    int exact) {
+  constexpr int W = 1;
+  NLOOP_BEGIN(1)
+  const Vox w(v, gd.q, gd.p, gd.r);
+  res[o] = dt_at(w, exact != 0, [&](int j) { return X[j]; },
+                 [&](int j) { return Y[j]; }, [&](int j) { return Z[j]; });
+  (void)W;
+  LOOP_END
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_only(KA K, const float *x,
+                                                     const float *y) {
+  double acc[1] = {0};
+  {
+    constexpr int W = 1;
+    NLOOP_BEGIN(1)
+    acc[0] += (double)x[o] * (double)y[o];
+    (void)W;
+    LOOP_END
+  }
+  const int sl[1] = {0};
+  block_partials<1, 1>(acc, sl, K.part, 0, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// small single-volume scratch: one GroupDesc, one InstScal, partials
+struct OpScratch {
+  GroupDesc *G;
+  InstScal *S;
+  double *part;
+  double *dense;
+};
+
+int op_scratch(tv3_ctx *ctx, const GroupDesc &gd, double beta, OpScratch &sc,
+               long long dense_doubles) {
+  const size_t need = align_up(sizeof(GroupDesc)) + align_up(sizeof(InstScal)) +
+                      align_up(sizeof(double) * kNslot * kNblkMax) +
+                      align_up(sizeof(double) * std::max<long long>(dense_doubles, 1));
+  if (ctx->ws_bytes < need) {
+    if (ctx->ws) { (void)hipStreamSynchronize(ctx->stream); (void)hipFree(ctx->ws); ctx->ws = nullptr; ctx->ws_bytes = 0; }
+    int rc = hip_check(ctx, hipMalloc(&ctx->ws, need), "workspace hipMalloc");
+    if (rc) return rc;
+    ctx->ws_bytes = need;
+  }
+  char *p = (char *)ctx->ws;
+  sc.G = (GroupDesc *)p; p += align_up(sizeof(GroupDesc));
+  sc.S = (InstScal *)p; p += align_up(sizeof(InstScal));
+  sc.part = (double *)p; p += align_up(sizeof(double) * kNslot * kNblkMax);
+  sc.dense = (double *)p;
+  InstScal s{};
+  s.flags = F_LIVE | F_ACTIVE;
+  s.beta = beta;
+  // the sources are stack variables: wait for the copies to finish
+  int rc = hip_check(ctx, hipMemcpyAsync(sc.G, &gd, sizeof(GroupDesc), hipMemcpyHostToDevice, ctx->stream), "copy");
+  if (!rc) rc = hip_check(ctx, hipMemcpyAsync(sc.S, &s, sizeof(InstScal), hipMemcpyHostToDevice, ctx->stream), "copy");
+  if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "copy sync");
+  return rc;
+}
+
+int nblk_for(long long n) {
+  return (int)std::min<long long>(kNblkMax, std::max<long long>(1, (n + kBlock - 1) / kBlock));
+}
+
+int host_partial_sum(tv3_ctx *ctx, const double *part, int slot, int nblk, double *out) {
+  std::vector<double> h(nblk);
+  int rc = hip_check(ctx, hipMemcpyAsync(h.data(), part + (long long)slot * kNblkMax, sizeof(double) * nblk, hipMemcpyDeviceToHost, ctx->stream), "copy");
+  if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  double s = 0;
+  for (int i = 0; i < nblk; ++i) s += h[i];
+  *out = s;
+  return TV3_OK;
+}
+
+GroupDesc vol_desc(int q, int p, int r) {
+  GroupDesc gd{};
+  gd.q = q; gd.p = p; gd.r = r;
+  gd.n = q * p * r;
+  return gd;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int tv3_solve(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+              int dim_x, int dim_y, int dim_z, const float *b, float *U,
+              const float *Ut, tv3_info *info, tv3_trace_rec *trace,
+              int trace_cap) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (!A || !b || !U || !info) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  const int dims[3] = {dim_x, dim_y, dim_z};
+  const tv3_plan *As[1] = {A};
+  const float *bs[1] = {b};
+  float *Us[1] = {U};
+  const float *Uts[1] = {Ut};
+  return solve_batch_impl(ctx, 1, As, opts, dims, bs, Us, Ut ? Uts : nullptr,
+                          info, trace, trace_cap);
+}
+
+int tv3_solve_batch(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                    const tv3_options *opts, const int *dims,
+                    const float *const *b, float *const *U,
+                    const float *const *Ut, tv3_info *infos) {
+  return solve_batch_impl(ctx, B, A, opts, dims, b, U, Ut, infos, nullptr, 0);
+}
+
+int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
+             float *y) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (!A || !x || !y) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  GroupDesc gd = vol_desc(A->dev.n, 1, 1);
+  gd.m = A->dev.m;
+  gd.plan = A->dev;
+  OpScratch sc;
+  const long long dense = A->dev.kind == kDense ? 16LL * A->dev.n : 1;
+  if ((rc = op_scratch(ctx, gd, 0.0, sc, dense))) return rc;
+  if (trans)
+    launch_bwd<1>(ctx->stream, sc.G, 1, gd.n, A->dev.kind, x, y, sc.S, 0, nullptr, (float *)sc.dense);
+  else
+    launch_fwd<1>(ctx->stream, sc.G, 1, gd.m, x, y, sc.S, 0);
+  return hip_check(ctx, hipGetLastError(), "gemv launch");
+}
+
+int tv3_D(tv3_ctx *ctx, int q, int p, int r, const float *U, float *Ux,
+          float *Uy, float *Uz) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (q <= 0 || p <= 0 || r <= 0 || !U || !Ux || !Uy || (r > 1 && !Uz))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: D arguments");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  OpScratch sc;
+  const GroupDesc gd = vol_desc(q, p, r);
+  if ((rc = op_scratch(ctx, gd, 0.0, sc, 1))) return rc;
+  KA K{sc.G, sc.S, sc.part, nblk_for(gd.n), 0, 0};
+  hipLaunchKernelGGL(k_D<1>, dim3(K.nblk, 1), dim3(kBlock), 0, ctx->stream, K, U, Ux, Uy, Uz);
+  return hip_check(ctx, hipGetLastError(), "D launch");
+}
+
+int tv3_Dt(tv3_ctx *ctx, int q, int p, int r, int exact, const float *X,
+           const float *Y, const float *Z, float *res) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (q <= 0 || p <= 0 || r <= 0 || !X || !Y || !res || (r > 1 && !Z))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: Dt arguments");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  OpScratch sc;
+  const GroupDesc gd = vol_desc(q, p, r);
+  if ((rc = op_scratch(ctx, gd, 0.0, sc, 1))) return rc;
+  KA K{sc.G, sc.S, sc.part, nblk_for(gd.n), 0, 0};
+  hipLaunchKernelGGL(k_Dt_only, dim3(K.nblk, 1), dim3(kBlock), 0, ctx->stream, K, X, Y, Z, res, exact);
+  return hip_check(ctx, hipGetLastError(), "Dt launch");
+}
+
+int tv3_shrink(tv3_ctx *ctx, int q, int p, int r, double beta, int iso,
+               int scal_double, const float *Ux, const float *Uy,
+               const float *Uz, const float *sx, const float *sy,
+               const float *sz, float *Wx, float *Wy, float *Wz,
+               double *lam1) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (q <= 0 || p <= 0 || r <= 0 || !Ux || !Uy || !sx || !sy || !Wx || !Wy ||
+      (r > 1 && (!Uz || !sz || !Wz)) || !lam1)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: shrink arguments");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  OpScratch sc;
+  const GroupDesc gd = vol_desc(q, p, r);
+  const double bt = scal_double ? beta : (double)(float)beta;
+  if ((rc = op_scratch(ctx, gd, bt, sc, 1))) return rc;
+  KA K{sc.G, sc.S, sc.part, nblk_for(gd.n), 0, 0};
+  Arrs a{};
+  a.Ux = (float *)Ux; a.Uy = (float *)Uy; a.Uz = (float *)Uz;
+  a.sx = (float *)sx; a.sy = (float *)sy; a.sz = (float *)sz;
+  a.Wx = Wx; a.Wy = Wy; a.Wz = Wz;
+  if (scal_double)
+    hipLaunchKernelGGL(k_shrink_only<double>, dim3(K.nblk, 1), dim3(kBlock), 0, ctx->stream, K, a, iso);
+  else
+    hipLaunchKernelGGL(k_shrink_only<float>, dim3(K.nblk, 1), dim3(kBlock), 0, ctx->stream, K, a, iso);
+  if ((rc = hip_check(ctx, hipGetLastError(), "shrink launch"))) return rc;
+  double s;
+  if ((rc = host_partial_sum(ctx, sc.part, 0, K.nblk, &s))) return rc;
+  *lam1 = scal_double ? s : (double)(float)s;
+  return TV3_OK;
+}
+
+int tv3_dot(tv3_ctx *ctx, long long n, const float *x, const float *y,
+            double *out) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (n < 0 || n > 0x7fffffffLL || !x || !y || !out)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: dot arguments");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  if (n == 0) { *out = 0; return TV3_OK; }
+  OpScratch sc;
+  const GroupDesc gd = vol_desc((int)n, 1, 1);
+  if ((rc = op_scratch(ctx, gd, 0.0, sc, 1))) return rc;
+  KA K{sc.G, sc.S, sc.part, nblk_for(n), 0, 0};
+  hipLaunchKernelGGL(k_dot_only, dim3(K.nblk, 1), dim3(kBlock), 0, ctx->stream, K, x, y);
+  if ((rc = hip_check(ctx, hipGetLastError(), "dot launch"))) return rc;
+  return host_partial_sum(ctx, sc.part, 0, K.nblk, out);
+}
+
+int tv3_nrm2(tv3_ctx *ctx, long long n, const float *x, double *out) {
+  double s = 0;
+  const int rc = tv3_dot(ctx, n, x, x, &s);
+  if (!rc) *out = std::sqrt(s);
+  return rc;
+}
+
+}  // extern "C"

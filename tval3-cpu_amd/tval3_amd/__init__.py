@@ -1,0 +1,317 @@
+"""tval3_amd — Python binding of the MI355X TVAL3 engine (libtval3_amd.so).
+
+A thin ctypes layer over the C-ABI declared in include/tval3_amd.h.  PyTorch is
+used only to own device memory and to supply the HIP stream; every computation
+runs in the hand-written HIP kernels of the native library.  There is no CPU
+fallback: if the library is missing or no GPU is visible, the calls raise.
+
+Reference interface mirrored: tval3_cpu_2d / tval3_cpu_3d
+(2d/tval3_cpu_2d.cpp:889-1016, 3d/tval3_cpu_3d.cpp:1384-1580) and the gemv
+overloads of 2d/mat_mult.h / 3d/mat_mult.h.  Argument errors raise
+Tval3Error with the library's message, as the reference throws
+tval3_exception.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import Options, Info, TraceRec, Geometry, default_options  # noqa: F401
+
+__all__ = ["Tval3Error", "lib", "lib_path", "Context", "Plan", "Options",
+           "Info", "TraceRec", "default_options"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libtval3_amd.so"))
+_lib = None
+
+_VP = C.c_void_p
+_I = C.c_int
+_LL = C.c_longlong
+
+
+class Tval3Error(RuntimeError):
+    """Raised for a non-zero status of the native library (the reference's
+    tval3_exception)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[tval3 status {code}] {msg}")
+        self.code = code
+
+
+def lib_path():
+    return _LIB_PATH
+
+
+def lib():
+    """Load libtval3_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"native library not built: {_LIB_PATH} "
+                           "(run __graft_entry__.build())")
+    L = C.CDLL(_LIB_PATH)
+    sig = {
+        "tv3_options_default": (None, [_VP]),
+        "tv3_ctx_create": (_I, [_I, _VP, C.POINTER(_VP)]),
+        "tv3_ctx_destroy": (None, [_VP]),
+        "tv3_ctx_error": (C.c_char_p, [_VP]),
+        "tv3_ctx_sync": (_I, [_VP]),
+        "tv3_ctx_stream": (_VP, [_VP]),
+        "tv3_ctx_profile": (_I, [_VP, _I]),
+        "tv3_ctx_profile_read": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_LL)]),
+        "tv3_plan_dense": (_I, [_VP, _I, _I, _I, _VP, C.POINTER(_VP)]),
+        "tv3_plan_sparse": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
+        "tv3_plan_geometry": (_I, [_VP, _VP, C.POINTER(_VP)]),
+        "tv3_plan_destroy": (None, [_VP]),
+        "tv3_plan_shape": (_I, [_VP, C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL)]),
+        "tv3_plan_walk": (_I, [_VP, _VP, _VP, _VP]),
+        "tv3_gemv": (_I, [_VP, _VP, _I, _VP, _VP]),
+        "tv3_D": (_I, [_VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
+        "tv3_Dt": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _VP]),
+        "tv3_shrink": (_I, [_VP, _I, _I, _I, C.c_double, _I, _I] + [_VP] * 9
+                       + [C.POINTER(C.c_double)]),
+        "tv3_dot": (_I, [_VP, _LL, _VP, _VP, C.POINTER(C.c_double)]),
+        "tv3_nrm2": (_I, [_VP, _LL, _VP, C.POINTER(C.c_double)]),
+        "tv3_solve": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
+        "tv3_solve_batch": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(t):
+    return None if t is None else _VP(t.data_ptr())
+
+
+class Context:
+    """A device context: HIP stream + device workspace (tv3_ctx)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        self.torch = torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("tval3_amd needs a HIP device")
+        self.device = torch.device("cuda", device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        h = _VP()
+        rc = lib().tv3_ctx_create(device, _VP(stream.cuda_stream), C.byref(h))
+        if rc:
+            raise Tval3Error(rc, "tv3_ctx_create failed")
+        self.h = h
+        import weakref
+        self._plans = weakref.WeakSet()
+
+    def close(self):
+        if getattr(self, "h", None):
+            for p in list(self._plans):
+                p.close()
+            lib().tv3_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise Tval3Error(rc, lib().tv3_ctx_error(self.h).decode())
+
+    def sync(self):
+        self._check(lib().tv3_ctx_sync(self.h))
+
+    def profile(self, enable=True):
+        """Start (and reset) or stop per-launch timing of the A / A^T products."""
+        self._check(lib().tv3_ctx_profile(self.h, int(bool(enable))))
+
+    def profile_read(self):
+        """{'A': (ms_total, launches), 'At': (ms_total, launches)}"""
+        ms = (C.c_double * 2)()
+        cnt = (_LL * 2)()
+        self._check(lib().tv3_ctx_profile_read(self.h, ms, cnt))
+        return {"A": (ms[0], cnt[0]), "At": (ms[1], cnt[1])}
+
+    def tensor(self, a, dtype=None):
+        """Host array -> contiguous device tensor (float32 unless given)."""
+        t = self.torch
+        if isinstance(a, t.Tensor):
+            x = a.to(self.device)
+        else:
+            x = t.from_numpy(np.ascontiguousarray(a))
+            x = x.to(self.device)
+        if dtype is not None:
+            x = x.to(dtype)
+        return x.contiguous()
+
+    # ---- plans -----------------------------------------------------------
+    def plan_dense(self, A, bf16=False):
+        t = self.torch
+        At = self.tensor(A, t.bfloat16 if bf16 else t.float32)
+        m, n = At.shape
+        h = _VP()
+        self._check(lib().tv3_plan_dense(self.h, m, n, 1 if bf16 else 0,
+                                         _ptr(At), C.byref(h)))
+        return Plan(self, h, [At])
+
+    def plan_sparse(self, m, n, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind,
+                    csc_val):
+        t = self.torch
+        arrs = [self.tensor(csr_ptr, t.int32), self.tensor(csr_ind, t.int32),
+                self.tensor(csr_val, t.float32), self.tensor(csc_ptr, t.int32),
+                self.tensor(csc_ind, t.int32), self.tensor(csc_val, t.float32)]
+        h = _VP()
+        self._check(lib().tv3_plan_sparse(self.h, m, n, int(arrs[1].numel()),
+                                          *[_ptr(a) for a in arrs], C.byref(h)))
+        return Plan(self, h, arrs)
+
+    def plan_geometry(self, geo):
+        s = geo.struct()
+        h = _VP()
+        self._check(lib().tv3_plan_geometry(self.h, C.byref(s), C.byref(h)))
+        return Plan(self, h, [])
+
+    # ---- operators ------------------------------------------------------
+    def gemv(self, plan, x, trans=False):
+        t = self.torch
+        x = self.tensor(x, t.float32)
+        y = t.empty(plan.n if trans else plan.m, dtype=t.float32,
+                    device=self.device)
+        self._check(lib().tv3_gemv(self.h, plan.h, int(bool(trans)), _ptr(x),
+                                   _ptr(y)))
+        return y
+
+    def D(self, U, dims):
+        t = self.torch
+        q, p, r = dims
+        U = self.tensor(U, t.float32)
+        out = [t.empty_like(U), t.empty_like(U)] + ([t.empty_like(U)] if r > 1 else [None])
+        self._check(lib().tv3_D(self.h, q, p, r, _ptr(U), *[_ptr(o) for o in out]))
+        return out
+
+    def Dt(self, X, Y, Z, dims, exact=False):
+        t = self.torch
+        q, p, r = dims
+        X, Y = self.tensor(X, t.float32), self.tensor(Y, t.float32)
+        Z = self.tensor(Z, t.float32) if r > 1 else None
+        res = t.empty_like(X)
+        self._check(lib().tv3_Dt(self.h, q, p, r, int(bool(exact)), _ptr(X),
+                                 _ptr(Y), _ptr(Z), _ptr(res)))
+        return res
+
+    def shrink(self, Ux, Uy, Uz, sx, sy, sz, dims, beta, iso=False,
+               scal_double=False):
+        t = self.torch
+        q, p, r = dims
+        ins = [self.tensor(a, t.float32) if a is not None else None
+               for a in (Ux, Uy, Uz, sx, sy, sz)]
+        outs = [t.empty_like(ins[0]), t.empty_like(ins[0]),
+                t.empty_like(ins[0]) if r > 1 else None]
+        lam1 = C.c_double()
+        self._check(lib().tv3_shrink(self.h, q, p, r, float(beta), int(bool(iso)),
+                                     int(bool(scal_double)),
+                                     *[_ptr(a) for a in ins + outs],
+                                     C.byref(lam1)))
+        return outs[0], outs[1], outs[2], lam1.value
+
+    def dot(self, x, y):
+        t = self.torch
+        x, y = self.tensor(x, t.float32), self.tensor(y, t.float32)
+        out = C.c_double()
+        self._check(lib().tv3_dot(self.h, x.numel(), _ptr(x), _ptr(y), C.byref(out)))
+        return out.value
+
+    # ---- solver -----------------------------------------------------------
+    def solve(self, plan, b, dims, opts=None, Ut=None, trace_cap=0):
+        """tval3_cpu_2d/3d on the device: returns (U tensor, Info, [TraceRec])."""
+        t = self.torch
+        if opts is None:
+            opts = default_options()
+        q, p, r = dims
+        b = self.tensor(b, t.float32)
+        if b.numel() != plan.m:
+            raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y!")
+        U = t.empty(q * p * r, dtype=t.float32, device=self.device)
+        Utt = self.tensor(Ut, t.float32) if Ut is not None else None
+        if Utt is not None and Utt.numel() != q * p * r:
+            raise Tval3Error(1, "Argument error: U and Ut must have the same size!")
+        info = Info()
+        trace = (TraceRec * trace_cap)() if trace_cap else None
+        self._check(lib().tv3_solve(self.h, plan.h, C.byref(opts), q, p, r,
+                                    _ptr(b), _ptr(U), _ptr(Utt), C.byref(info),
+                                    trace, int(trace_cap)))
+        n_tr = min(info.total_iters, trace_cap)
+        return U, info, ([trace[i] for i in range(n_tr)] if trace else [])
+
+    def solve_batch(self, plans, bs, dims, opts=None, Uts=None):
+        """B reconstructions in one batched launch sequence.  Returns
+        (list of U tensors, list of Info)."""
+        t = self.torch
+        if opts is None:
+            opts = default_options()
+        B = len(plans)
+        if len(bs) != B or len(dims) != B:
+            raise Tval3Error(1, "Argument error: plans, b and dims differ in length")
+        bts = [self.tensor(b, t.float32) for b in bs]
+        for pl, x in zip(plans, bts):
+            if x.numel() != pl.m:
+                raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y!")
+        if Uts is not None:
+            for d, u in zip(dims, Uts):
+                if u is not None and np.size(u) != d[0] * d[1] * d[2]:
+                    raise Tval3Error(1, "Argument error: U and Ut must have the same size!")
+        Us = [t.empty(d[0] * d[1] * d[2], dtype=t.float32, device=self.device)
+              for d in dims]
+        Utts = ([self.tensor(u, t.float32) if u is not None else None for u in Uts]
+                if Uts is not None else None)
+        parr = (_VP * B)(*[p.h for p in plans])
+        dims_arr = (C.c_int * (3 * B))(*[int(v) for d in dims for v in d])
+        barr = (_VP * B)(*[x.data_ptr() for x in bts])
+        uarr = (_VP * B)(*[x.data_ptr() for x in Us])
+        utarr = ((_VP * B)(*[(x.data_ptr() if x is not None else None) for x in Utts])
+                 if Utts is not None else None)
+        infos = (Info * B)()
+        self._check(lib().tv3_solve_batch(self.h, B, parr, C.byref(opts),
+                                          dims_arr, barr, uarr, utarr, infos))
+        return Us, [infos[i] for i in range(B)]
+
+
+class Plan:
+    """A measurement-operator plan (keeps the borrowed device arrays alive)."""
+
+    def __init__(self, ctx, handle, keep):
+        self.ctx = ctx
+        self.h = handle
+        self._keep = keep
+        ctx._plans.add(self)
+        m, n, nnz = _LL(), _LL(), _LL()
+        lib().tv3_plan_shape(handle, C.byref(m), C.byref(n), C.byref(nnz))
+        self.m, self.n, self.nnz = m.value, n.value, nnz.value
+
+    def walk(self):
+        """Voxel index list of every row as computed by the device walk kernel."""
+        ptr = np.zeros(self.m + 1, np.int64)
+        lin = np.zeros(max(self.nnz, 1), np.int32)
+        self.ctx._check(lib().tv3_plan_walk(self.ctx.h, self.h,
+                                            ptr.ctypes.data_as(_VP),
+                                            lin.ctypes.data_as(_VP)))
+        return ptr, lin[:self.nnz]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tv3_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
