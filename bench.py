@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--maxit", type=int, default=200)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the "
@@ -138,6 +141,13 @@ def main():
                 "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBPS if ach else None, "traffic": None,
                 "bytes_per_launch": Bpad * per_inst}
+
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     cpu = None
     if not args.no_cpu:

@@ -81,6 +81,13 @@ typedef struct tv3_options {
   int dt_exact;               /* 3-D only. 0: the reference's Dt (its y-term
                                  is not the adjoint of D for dim_z >= 2),
                                  1: the exact adjoint of D */
+  int ls_exact;               /* Armijo backtracking.  1: the reference's
+                                 rounds (each step re-interpolates every
+                                 vector and re-evaluates f from rounded
+                                 values); 0: f(alpha) of all five steps from
+                                 the quadratic coefficients gathered by the
+                                 descent pass, then one interpolation pass
+                                 that recomputes the exact lam2..lam5 and f */
 } tv3_options;
 
 /* Result record: the reference's tval3_info<T> (2d/tval3_cpu_2d.cpp:865-876). */

@@ -22,7 +22,7 @@ namespace tv3 {
 
 constexpr int kBlock = 256;       // threads per block of every vector kernel
 constexpr int kNblkMax = 1024;    // max blocks per group of a reduction kernel
-constexpr int kNslot = 16;        // partial-sum slots per instance
+constexpr int kNslot = 24;        // partial-sum slots per instance
 constexpr int kSlotM = 12;        // first slot written by m-vector kernels
 
 // ---- plans ------------------------------------------------------------------
@@ -112,7 +112,7 @@ struct OptDev {
   double mu, beta, mu0, beta0, tol, tol_inn;
   int maxit, maxcnt;
   double gam, gamma, rate_gam, rate_cnt, c;
-  int nonneg, tv_iso, dt_exact, skip;
+  int nonneg, tv_iso, dt_exact, skip, ls_exact;
   unsigned numInitIterations;
   double skipMulRatio, maxRelativeChange;
 };
@@ -128,8 +128,12 @@ struct tv3_ctx {
   // grow-only device workspace for the solver
   void *ws = nullptr;
   size_t ws_bytes = 0;
-  void *pinned = nullptr;  // small pinned host buffer for flag read-back
+  void *pinned = nullptr;  // small pinned host buffer
   size_t pinned_bytes = 0;
+  // per-instance control flags, written by the device straight into mapped
+  // pinned host memory (read after a stream synchronization)
+  uint32_t *hflags = nullptr, *dflags = nullptr;
+  size_t hflags_n = 0;
   // timing events for the benchmark (kernel-level, on this stream)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-launch timing of the A / A^T products (tv3_ctx_profile)

@@ -70,79 +70,164 @@ int hip_check(tv3_ctx *ctx, hipError_t e, const char *what) {
 // forward product: y[row] = sum_k A[row, k] x[k]
 // ---------------------------------------------------------------------------
 
-// W = 1: one wavefront per row; lanes take entries lane, lane+64, ...; the 64
-// lane sums are combined by a fixed butterfly.
+// W = 1: sixteen lanes per row (four rows per wavefront); lane k of a row
+// takes entries k, k+16, ...; the sixteen lane sums are combined by a fixed
+// butterfly.
+constexpr int kSub = 16;
+
+__device__ __forceinline__ double sub_sum(double v) {
+#pragma unroll
+  for (int off = kSub / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fwd_w1(const GroupDesc *G,
                                                    const float *x, float *y,
                                                    const InstScal *S,
                                                    uint32_t mask) {
   const GroupDesc &gd = G[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (row >= gd.m) return;
+  const int sl = threadIdx.x & (kSub - 1);
+  const int row = blockIdx.x * (kBlock / kSub) + (threadIdx.x / kSub);
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const PlanDev &P = gd.plan;
   const float *xg = x + gd.off_n;
   double acc = 0.0;
-  if (P.kind == kGeometry) {
-    const RayDesc rd = P.rays[row];
-    const float sc = P.scale;
-    for (int j = lane; j <= rd.L; j += 64)
-      acc += (double)(xg[ray_lin(rd, j)] * sc);
-  } else if (P.kind == kSparse) {
-    const int k0 = P.r_ptr[row], k1 = P.r_ptr[row + 1];
-    for (int k = k0 + lane; k < k1; k += 64)
-      acc += (double)P.r_val[k] * (double)xg[P.r_ind[k]];
-  } else {
-    const long long base = (long long)row * gd.n;
-    if (P.dtype == TV3_F32) {
-      const float *A = (const float *)P.A + base;
-      for (int j = lane; j < gd.n; j += 64) acc += (double)A[j] * (double)xg[j];
+  if (row < gd.m) {
+    if (P.kind == kGeometry) {
+      const RayDesc rd = P.rays[row];
+      const float sc = P.scale;
+      for (int j = sl; j <= rd.L; j += kSub) acc += (double)(xg[ray_lin(rd, j)] * sc);
+    } else if (P.kind == kSparse) {
+      const int k0 = P.r_ptr[row], k1 = P.r_ptr[row + 1];
+      for (int k = k0 + sl; k < k1; k += kSub)
+        acc += (double)P.r_val[k] * (double)xg[P.r_ind[k]];
     } else {
-      const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
-      for (int j = lane; j < gd.n; j += 64)
-        acc += (double)__bfloat162float(A[j]) * (double)xg[j];
+      const long long base = (long long)row * gd.n;
+      if (P.dtype == TV3_F32) {
+        const float *A = (const float *)P.A + base;
+        for (int j = sl; j < gd.n; j += kSub) acc += (double)A[j] * (double)xg[j];
+      } else {
+        const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
+        for (int j = sl; j < gd.n; j += kSub)
+          acc += (double)__bfloat162float(A[j]) * (double)xg[j];
+      }
     }
   }
-  acc = wave_sum(acc);
-  if (lane == 0) y[gd.off_m + row] = (float)acc;
+  acc = sub_sum(acc);
+  if (sl == 0 && row < gd.m) y[gd.off_m + row] = (float)acc;
 }
 
-// W = 64: one wavefront per row, one lane per instance; entries are walked
-// in order by all lanes together (the walk and the indices are
-// wavefront-uniform), each gather is a 256-byte coalesced load.
+// W = 64: one wavefront per row, one lane per instance.  The walk, the
+// indices and the matrix entries are wavefront-uniform and are kept in scalar
+// registers (readfirstlane / readlane), so the scalar unit runs the walk and
+// the address arithmetic; every gather is one 256-byte coalesced load of the
+// 64 instances' values of a voxel (scalar row base + lane offset).  Entries
+// are consumed in order (the FP64 sum is sequential per lane) but loaded
+// kPipe at a time so that kPipe gathers are in flight per wavefront.
+constexpr int kPipe = 16;
+
+__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __global__ __launch_bounds__(kBlock) void k_fwd_w64(const GroupDesc *G,
                                                     const float *x, float *y,
                                                     const InstScal *S,
                                                     uint32_t mask) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int row = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
   if (row >= gd.m) return;
   const bool on = !mask || (S[blockIdx.y * 64 + lane].flags & mask) == mask;
   if (!__any(on)) return;
   const PlanDev &P = gd.plan;
-  const float *xg = x + gd.off_n + lane;
+  const float *xb = x + gd.off_n;
   double acc = 0.0;
   if (P.kind == kGeometry) {
-    const RayDesc rd = P.rays[row];
+    // the reference's error recurrence (3d/mat_mult.h:244-279), one voxel
+    // per step, identical to the closed form of ray_lin
+    const RayDesc *rp = P.rays + row;
+    const int L = uni_i(rp->L), lin0 = uni_i(rp->lin0);
+    const int sM = uni_i(rp->s_major), sA = uni_i(rp->s_a), sB = uni_i(rp->s_b);
+    const int ta = 2 * uni_i(rp->ma), tb = 2 * uni_i(rp->mb), twoL = 2 * L;
     const float sc = P.scale;
-    for (int j = 0; j <= rd.L; ++j)
-      acc += (double)(xg[(long long)ray_lin(rd, j) * 64] * sc);
+    const bool unit = sc == 1.0f;
+    int lin = lin0, ea = ta - L, eb = tb - L;
+    int j = 0;
+    for (; j + kPipe <= L + 1; j += kPipe) {
+      float v[kPipe];
+#pragma unroll
+      for (int u = 0; u < kPipe; ++u) {
+        v[u] = (xb + (long long)lin * 64)[lane];
+        if (ea > 0) { lin += sA; ea -= twoL; }
+        if (eb > 0) { lin += sB; eb -= twoL; }
+        ea += ta; eb += tb; lin += sM;
+      }
+      if (unit) {
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u) acc += (double)v[u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u) acc += (double)(v[u] * sc);
+      }
+    }
+    for (; j <= L; ++j) {
+      acc += (double)((xb + (long long)lin * 64)[lane] * sc);
+      if (ea > 0) { lin += sA; ea -= twoL; }
+      if (eb > 0) { lin += sB; eb -= twoL; }
+      ea += ta; eb += tb; lin += sM;
+    }
   } else if (P.kind == kSparse) {
-    const int k0 = P.r_ptr[row], k1 = P.r_ptr[row + 1];
-    for (int k = k0; k < k1; ++k)
-      acc += (double)P.r_val[k] * (double)xg[(long long)P.r_ind[k] * 64];
+    const int k0 = uni_i(P.r_ptr[row]), k1 = uni_i(P.r_ptr[row + 1]);
+    for (int k = k0; k < k1; k += kPipe) {
+      // lanes 0..kPipe-1 fetch the next indices and values; readlane makes
+      // them scalar
+      const bool ld = lane < kPipe && k + lane < k1;
+      const int ind = ld ? P.r_ind[k + lane] : 0;
+      const float val = ld ? P.r_val[k + lane] : 0.0f;
+      const int cnt = min(kPipe, k1 - k);
+      float v[kPipe], w[kPipe];
+#pragma unroll
+      for (int u = 0; u < kPipe; ++u) {
+        const int iu = __builtin_amdgcn_readlane(ind, u);
+        w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, val), u));
+        v[u] = u < cnt ? (xb + (long long)iu * 64)[lane] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < kPipe; ++u)
+        if (u < cnt) acc += (double)w[u] * (double)v[u];
+    }
   } else {
     const long long base = (long long)row * gd.n;
+    int j = 0;
     if (P.dtype == TV3_F32) {
       const float *A = (const float *)P.A + base;
-      for (int j = 0; j < gd.n; ++j)
-        acc += (double)A[j] * (double)xg[(long long)j * 64];
+      for (; j < gd.n; j += kPipe) {
+        const int cnt = min(kPipe, gd.n - j);
+        const float a = (lane < cnt) ? A[j + lane] : 0.0f;
+        float v[kPipe], w[kPipe];
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u) {
+          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
+          v[u] = u < cnt ? (xb + (long long)(j + u) * 64)[lane] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)w[u] * (double)v[u];
+      }
     } else {
       const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
-      for (int j = 0; j < gd.n; ++j)
-        acc += (double)__bfloat162float(A[j]) * (double)xg[(long long)j * 64];
+      for (; j < gd.n; j += kPipe) {
+        const int cnt = min(kPipe, gd.n - j);
+        const float a = (lane < cnt) ? __bfloat162float(A[j + lane]) : 0.0f;
+        float v[kPipe], w[kPipe];
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u) {
+          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
+          v[u] = u < cnt ? (xb + (long long)(j + u) * 64)[lane] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)w[u] * (double)v[u];
+      }
     }
   }
   if (on) y[gd.off_m + (long long)row * 64 + lane] = (float)acc;
@@ -158,24 +243,25 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
                                                    uint32_t mask,
                                                    const float *sub) {
   const GroupDesc &gd = G[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (col >= gd.n) return;
+  const int sl = threadIdx.x & (kSub - 1);
+  const int col = blockIdx.x * (kBlock / kSub) + (threadIdx.x / kSub);
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const PlanDev &P = gd.plan;
   const float *xg = x + gd.off_m;
   double acc = 0.0;
-  if (P.kind == kGeometry) {
-    const int k0 = P.g_cptr[col], k1 = P.g_cptr[col + 1];
-    const float sc = P.scale;
-    for (int k = k0 + lane; k < k1; k += 64) acc += (double)(xg[P.g_cray[k]] * sc);
-  } else {
-    const int k0 = P.c_ptr[col], k1 = P.c_ptr[col + 1];
-    for (int k = k0 + lane; k < k1; k += 64)
-      acc += (double)P.c_val[k] * (double)xg[P.c_ind[k]];
+  if (col < gd.n) {
+    if (P.kind == kGeometry) {
+      const int k0 = P.g_cptr[col], k1 = P.g_cptr[col + 1];
+      const float sc = P.scale;
+      for (int k = k0 + sl; k < k1; k += kSub) acc += (double)(xg[P.g_cray[k]] * sc);
+    } else {
+      const int k0 = P.c_ptr[col], k1 = P.c_ptr[col + 1];
+      for (int k = k0 + sl; k < k1; k += kSub)
+        acc += (double)P.c_val[k] * (double)xg[P.c_ind[k]];
+    }
   }
-  acc = wave_sum(acc);
-  if (lane == 0) {
+  acc = sub_sum(acc);
+  if (sl == 0 && col < gd.n) {
     const long long o = gd.off_n + col;
     float v = (float)acc;
     if (sub) v = v - sub[o];
@@ -190,32 +276,78 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
                                                     const float *sub) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int col = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
   if (col >= gd.n) return;
   const bool on = !mask || (S[blockIdx.y * 64 + lane].flags & mask) == mask;
   if (!__any(on)) return;
   const PlanDev &P = gd.plan;
-  const float *xg = x + gd.off_m + lane;
+  const float *xb = x + gd.off_m;
   double acc = 0.0;
-  if (P.kind == kGeometry) {
-    const int k0 = P.g_cptr[col], k1 = P.g_cptr[col + 1];
+  if (P.kind == kGeometry || P.kind == kSparse) {
+    const bool geo = P.kind == kGeometry;
+    const int *ptr = geo ? P.g_cptr : P.c_ptr;
+    const int *ind = geo ? P.g_cray : P.c_ind;
     const float sc = P.scale;
-    for (int k = k0; k < k1; ++k)
-      acc += (double)(xg[(long long)P.g_cray[k] * 64] * sc);
-  } else if (P.kind == kSparse) {
-    const int k0 = P.c_ptr[col], k1 = P.c_ptr[col + 1];
-    for (int k = k0; k < k1; ++k)
-      acc += (double)P.c_val[k] * (double)xg[(long long)P.c_ind[k] * 64];
+    const bool unit = geo && sc == 1.0f;
+    const int k0 = uni_i(ptr[col]), k1 = uni_i(ptr[col + 1]);
+    for (int k = k0; k < k1; k += kPipe) {
+      const int cnt = min(kPipe, k1 - k);
+      const bool ld = lane < cnt;
+      const int id = ld ? ind[k + lane] : 0;
+      const float val = (ld && !geo) ? P.c_val[k + lane] : sc;
+      float v[kPipe], w[kPipe];
+#pragma unroll
+      for (int u = 0; u < kPipe; ++u) {
+        const int iu = __builtin_amdgcn_readlane(id, u);
+        w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, val), u));
+        v[u] = u < cnt ? (xb + (long long)iu * 64)[lane] : 0.0f;
+      }
+      if (unit) {
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)v[u];
+      } else if (geo) {
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)(v[u] * sc);
+      } else {
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)w[u] * (double)v[u];
+      }
+    }
   } else {
+    int i = 0;
     if (P.dtype == TV3_F32) {
       const float *A = (const float *)P.A + col;
-      for (int i = 0; i < gd.m; ++i)
-        acc += (double)A[(long long)i * gd.n] * (double)xg[(long long)i * 64];
+      for (; i < gd.m; i += kPipe) {
+        const int cnt = min(kPipe, gd.m - i);
+        const float a = (lane < cnt) ? A[(long long)(i + lane) * gd.n] : 0.0f;
+        float v[kPipe], w[kPipe];
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u) {
+          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
+          v[u] = u < cnt ? (xb + (long long)(i + u) * 64)[lane] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)w[u] * (double)v[u];
+      }
     } else {
       const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + col;
-      for (int i = 0; i < gd.m; ++i)
-        acc += (double)__bfloat162float(A[(long long)i * gd.n]) *
-               (double)xg[(long long)i * 64];
+      for (; i < gd.m; i += kPipe) {
+        const int cnt = min(kPipe, gd.m - i);
+        const float a = (lane < cnt) ? __bfloat162float(A[(long long)(i + lane) * gd.n]) : 0.0f;
+        float v[kPipe], w[kPipe];
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u) {
+          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
+          v[u] = u < cnt ? (xb + (long long)(i + u) * 64)[lane] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPipe; ++u)
+          if (u < cnt) acc += (double)w[u] * (double)v[u];
+      }
     }
   }
   if (on) {
@@ -277,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce(
 template <>
 void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                    const float *x, float *y, const InstScal *S, uint32_t mask) {
-  dim3 grid((max_m + 3) / 4, G);
+  dim3 grid((max_m + kBlock / kSub - 1) / (kBlock / kSub), G);
   hipLaunchKernelGGL(k_fwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
 }
 
@@ -305,7 +437,7 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                        (const double *)scratch, y, S, mask, sub, sstride);
     return;
   }
-  dim3 grid((max_n + 3) / 4, G);
+  dim3 grid((max_n + kBlock / kSub - 1) / (kBlock / kSub), G);
   hipLaunchKernelGGL(k_bwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask,
                      sub);
 }
@@ -351,7 +483,7 @@ void tv3_options_default(tv3_options *o) {
   o->gam = 0.9995; o->gamma = 0.6; o->rate_gam = 0.9; o->rate_cnt = 2.0;
   o->c = 1e-5; o->nonneg = 0; o->isreal = 1; o->numInitIterations = 0;
   o->skipMulRatio = 0.0; o->maxRelativeChange = 0.0;
-  o->scal_double = 0; o->tv_iso = 0; o->dt_exact = 0;
+  o->scal_double = 0; o->tv_iso = 0; o->dt_exact = 0; o->ls_exact = 0;
 }
 
 int tv3_ctx_create(int device, void *stream, tv3_ctx **out) {
@@ -390,6 +522,7 @@ void tv3_ctx_destroy(tv3_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ws) (void)hipFree(c->ws);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->hflags) (void)hipHostFree(c->hflags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);

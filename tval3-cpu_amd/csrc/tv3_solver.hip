@@ -51,6 +51,14 @@ struct KA {
   uint32_t req, forbid;
 };
 
+// With W = 64 every lane of a wavefront handles the same voxel: make the
+// index wavefront-uniform so the neighbour arithmetic runs on the scalar unit.
+template <int W>
+__device__ __forceinline__ int uni(int v) {
+  if (W == 64) return __builtin_amdgcn_readfirstlane(v);
+  return v;
+}
+
 #define NLOOP_BEGIN(W)                                                       \
   const GroupDesc &gd = K.G[blockIdx.y];                                    \
   const int lb = threadIdx.x % (W);                                          \
@@ -61,7 +69,7 @@ struct KA {
   if (on)                                                                    \
     for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;  \
          e += (long long)K.nblk * kBlock) {                                  \
-      const int v = (int)(e / (W));                                          \
+      const int v = uni<W>((int)(e / (W)));                                  \
       const long long o = gd.off_n + e;                                      \
       (void)v;
 
@@ -141,10 +149,13 @@ __global__ __launch_bounds__(kBlock) void k_msq(KA K, const float *x) {
 // ---------------------------------------------------------------------------
 // descend (2d:481-493) + get_g's n-vector part (2d:396-417)
 // ---------------------------------------------------------------------------
+// With poly != 0 it also gathers, for the backtracking line search, the
+// quadratic coefficients of lam2(alpha) and lam4(alpha): V_dir(alpha) =
+// (U_dir_p - W_dir) + alpha dU_dir with dU_dir = U_dir - U_dir_p (slots 7-11).
 template <int W, class TS>
 __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
-                                                    int exact) {
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+                                                    int exact, int poly) {
+  double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     NLOOP_BEGIN(W)
     const float tf = (float)(-1 * (TS)S.tau);
@@ -178,6 +189,21 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
     }
     const float d0 = a.d[o];
     acc[6] += (double)d0 * (double)d0;
+    if (poly) {
+      const float px = a.Uxp[o], py = a.Uyp[o];
+      const double vx = (double)px - (double)a.Wx[o], dx = (double)(ux - px);
+      const double vy = (double)py - (double)a.Wy[o], dy = (double)(uy - py);
+      double A2 = vx * vx + vy * vy, B2 = vx * dx + vy * dy, C2 = dx * dx + dy * dy;
+      double A4 = (double)a.sx[o] * vx + (double)a.sy[o] * vy;
+      double B4 = (double)a.sx[o] * dx + (double)a.sy[o] * dy;
+      if (z3) {
+        const float pz = a.Uzp[o];
+        const double vz = (double)pz - (double)a.Wz[o], dz = (double)(uz - pz);
+        A2 += vz * vz; B2 += vz * dz; C2 += dz * dz;
+        A4 += (double)a.sz[o] * vz; B4 += (double)a.sz[o] * dz;
+      }
+      acc[7] += A2; acc[8] += B2; acc[9] += C2; acc[10] += A4; acc[11] += B4;
+    }
     // g2 = Dt(V) at v
     const int xp = w.xp(), yp = w.yp();
     const float VXp = (u0 - un(xp)) - AT(a.Wx, xp);
@@ -208,30 +234,45 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
     a.g2[o] = t;
     LOOP_END
   }
-  const int sl[7] = {0, 1, 2, 3, 4, 5, 6};
-  block_partials<W, 7>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+  const int sl[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+  block_partials<W, 12>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
 }
 
 // get_g's m-vector part: Aub = Au - b, lam3 = |Aub|^2, lam5 = <delta, Aub>,
 // and the norms of the A^T-skip test (2d:313-316)
+// poly != 0: also the quadratic coefficients of lam3(alpha), lam5(alpha)
+// with Au(alpha) = Aup + alpha (Au - Aup) (slots kSlotM+4 .. +8).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a, int skip) {
-  double acc[4] = {0, 0, 0, 0};
+__global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a, int skip,
+                                                  int poly) {
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     MLOOP_BEGIN(W)
     const float au = a.Au[o];
-    const float aub = au - a.b[o];
+    const float bb = a.b[o];
+    const float aub = au - bb;
+    const float dl = a.delta[o];
     acc[0] += (double)aub * (double)aub;
-    acc[1] += (double)a.delta[o] * (double)aub;
+    acc[1] += (double)dl * (double)aub;
     if (skip) {
       const float da = au - a.lastAu[o];
       acc[2] += (double)da * (double)da;
       acc[3] += (double)au * (double)au;
     }
+    if (poly) {
+      const float aup = a.Aup[o];
+      const double ap = (double)aup - (double)bb, da = (double)(au - aup);
+      acc[4] += ap * ap;
+      acc[5] += ap * da;
+      acc[6] += da * da;
+      acc[7] += (double)dl * ap;
+      acc[8] += (double)dl * da;
+    }
     LOOP_END
   }
-  const int sl[4] = {kSlotM, kSlotM + 1, kSlotM + 2, kSlotM + 3};
-  block_partials<W, 4>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+  const int sl[9] = {kSlotM, kSlotM + 1, kSlotM + 2, kSlotM + 3, kSlotM + 4,
+                     kSlotM + 5, kSlotM + 6, kSlotM + 7, kSlotM + 8};
+  block_partials<W, 9>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
 }
 
 template <int W>
@@ -315,6 +356,62 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_m(KA K, Arrs a) {
     MLOOP_BEGIN(W)
     const TS al = (TS)S.alpha;
     const float au = interp<TS>(a.Aup[o], al, a.dAu[o]);
+    a.Au[o] = au;
+    const float aub = au - a.b[o];
+    acc[0] += (double)aub * (double)aub;
+    acc[1] += (double)a.delta[o] * (double)aub;
+    LOOP_END
+  }
+  const int sl[2] = {kSlotM, kSlotM + 1};
+  block_partials<W, 2>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+// The state at the step size chosen by the quadratic line search, built as
+// the reference's last backtracking round builds it: dX = X - Xp from the
+// full step, X = Xp + alpha dX (update_g, 2d:500-505), then lam2/lam4 and
+// lam3/lam5 from the rounded vectors.
+template <int W, class TS>
+__global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  {
+    NLOOP_BEGIN(W)
+    const TS al = (TS)S.alpha;
+    const float gp = a.gp[o], g2p = a.g2p[o];
+    a.g[o] = interp<TS>(gp, al, a.g[o] - gp);
+    a.g2[o] = interp<TS>(g2p, al, a.g2[o] - g2p);
+    a.U[o] = interp<TS>(a.Up[o], al, a.dU[o]);
+    const float uxp = a.Uxp[o], uyp = a.Uyp[o];
+    const float ux = interp<TS>(uxp, al, a.Ux[o] - uxp);
+    const float uy = interp<TS>(uyp, al, a.Uy[o] - uyp);
+    a.Ux[o] = ux;
+    a.Uy[o] = uy;
+    const float Vx = ux - a.Wx[o], Vy = uy - a.Wy[o];
+    acc[0] += (double)Vx * (double)Vx;
+    acc[1] += (double)Vy * (double)Vy;
+    acc[3] += (double)a.sx[o] * (double)Vx;
+    acc[4] += (double)a.sy[o] * (double)Vy;
+    if (gd.r > 1) {
+      const float uzp = a.Uzp[o];
+      const float uz = interp<TS>(uzp, al, a.Uz[o] - uzp);
+      a.Uz[o] = uz;
+      const float Vz = uz - a.Wz[o];
+      acc[2] += (double)Vz * (double)Vz;
+      acc[5] += (double)a.sz[o] * (double)Vz;
+    }
+    LOOP_END
+  }
+  const int sl[6] = {0, 1, 2, 3, 4, 5};
+  block_partials<W, 6>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+}
+
+template <int W, class TS>
+__global__ __launch_bounds__(kBlock) void k_interp_m(KA K, Arrs a) {
+  double acc[2] = {0, 0};
+  {
+    MLOOP_BEGIN(W)
+    const TS al = (TS)S.alpha;
+    const float aup = a.Aup[o];
+    const float au = interp<TS>(aup, al, a.Au[o] - aup);
     a.Au[o] = au;
     const float aub = au - a.b[o];
     acc[0] += (double)aub * (double)aub;
@@ -626,16 +723,49 @@ struct FA {
   int nblk_n, nblk_m;
   int W;
   uint32_t req, forbid;
-  uint32_t *summary;
+  uint32_t *fout;  // device view of the mapped host flag array (or NULL)
   tv3_trace_rec *trace;
   int trace_cap;
   OptDev O;
 };
 
-__device__ __forceinline__ double slot_sum(const FA &F, int inst, int slot) {
-  const int nb = slot >= kSlotM ? F.nblk_m : F.nblk_n;
-  return block_sum_partials(
-      F.part + ((long long)inst * kNslot + slot) * kNblkMax, nb);
+// Sums of NS partial-sum slots of one instance by a 256-thread block, in one
+// pass: thread t adds partials t, t+256, ...; wavefront butterfly; the four
+// wavefronts in order.  Every thread receives the results.
+template <int NS>
+__device__ __forceinline__ void fin_sums(const FA &F, int inst,
+                                         const int (&slots)[NS],
+                                         double (&out)[NS]) {
+  __shared__ double red[4][NS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double x[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) x[k] = 0.0;
+  // all slots' loads are issued before any reduction
+  for (int i = threadIdx.x; i < kNblkMax; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int nb = slots[k] >= kSlotM ? F.nblk_m : F.nblk_n;
+      if (i < nb) x[k] += F.part[((long long)inst * kNslot + slots[k]) * kNblkMax + i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const double t = wave_sum(x[k]);
+    if (lane == 0) red[wv][k] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+    out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  __syncthreads();
+}
+
+__device__ __forceinline__ void put_flags(const FA &F, int inst, uint32_t f) {
+  if (F.fout) {
+    F.fout[inst] = f;
+    __threadfence_system();
+  }
 }
 
 // f = lam1 + beta/2*lam2 + mu/2*lam3 - lam4 - lam5 (2d:422, 523, 842)
@@ -659,14 +789,14 @@ __device__ __forceinline__ bool armijo_fails(const InstScal &s) {
   const uint32_t fl0 = s.flags;                                        \
   const bool on = sel(fl0, F.req, F.forbid);                           \
   if (!on) {                                                           \
-    if (threadIdx.x == 0 && F.summary) atomicOr(F.summary, fl0);       \
+    if (threadIdx.x == 0) put_flags(F, inst, fl0);                     \
     return;                                                            \
   }                                                                    \
   const GroupDesc &gd = F.G[inst / F.W];                               \
   (void)gd;
 
 #define FIN_END                                                        \
-  if (threadIdx.x == 0 && F.summary) atomicOr(F.summary, s.flags);
+  if (threadIdx.x == 0) put_flags(F, inst, s.flags);
 
 // lam2 and lam4 from three direction dot products (get_g / update_g /
 // update_W: 2d:408-411)
@@ -684,52 +814,57 @@ __device__ __forceinline__ void set_lam24(InstScal &s, bool z3, double x2,
   s.lam4 = lam4;
 }
 
-// clears per-step flags of every instance and sets tau for active ones
+constexpr uint32_t kStepFlags =
+    F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB | F_BWD | F_SKIPG;
+
+// tau of get_tau (2d:425-453).  Normal mode (req without F_FB): also clears
+// the per-step flags of every instance.  Fallback mode (req = F_FB): the
+// first-iteration formula for the instances in the fallback.
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_tau(FA F) {
   const int inst = blockIdx.x;
   InstScal &s = F.S[inst];
   const uint32_t fl = s.flags;
   const bool fb = (F.req & F_FB) != 0;
-  if (!fb && !sel(fl, F.req, F.forbid)) {
-    if (threadIdx.x == 0) {
-      s.flags = fl & ~(F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB |
-                       F_BWD | F_SKIPG);
-      if (F.summary) atomicOr(F.summary, s.flags);
+  if (fb ? !(fl & F_FB) : !sel(fl, F.req, F.forbid)) {
+    if (threadIdx.x == 0 && !fb) {
+      s.flags = fl & ~kStepFlags;
+      put_flags(F, inst, s.flags);
     }
     return;
   }
-  if (fb && !(fl & F_FB)) return;
   const GroupDesc &gd = F.G[inst / F.W];
   const bool first = fb || (fl & F_FST);
-  double a0, a1, a2 = 0, a3 = 0, am = 0;
-  a0 = slot_sum(F, inst, 0);
-  a1 = slot_sum(F, inst, 1);
+  double r[5] = {0, 0, 0, 0, 0};
   if (first) {
-    a2 = slot_sum(F, inst, 2);
-    a3 = slot_sum(F, inst, 3);
-    am = slot_sum(F, inst, kSlotM);
+    const int sl[5] = {0, 1, 2, 3, kSlotM};
+    fin_sums<5>(F, inst, sl, r);
+  } else {
+    const int sl[2] = {0, 1};
+    double r2[2];
+    fin_sums<2>(F, inst, sl, r2);
+    r[0] = r2[0];
+    r[1] = r2[1];
   }
   if (threadIdx.x == 0) {
     TS tau;
     if (first) {
-      float dDd = (float)a0 + (float)a1;
-      if (gd.r > 1) dDd = dDd + (float)a2;
-      const float dd = (float)a3, Add = (float)am;
+      float dDd = (float)r[0] + (float)r[1];
+      if (gd.r > 1) dDd = dDd + (float)r[2];
+      const float dd = (float)r[3], Add = (float)r[4];
       tau = fabs((TS)dd / ((TS)dDd + (TS)s.muDbeta * (TS)Add));
     } else {
-      const float ss = (float)a0, sy = (float)a1;
+      const float ss = (float)r[0], sy = (float)r[1];
       tau = (TS)fabsf(ss / sy);
     }
     s.tau = (double)tau;
     if (!fb) {
-      s.flags = fl & ~(F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB |
-                       F_BWD | F_SKIPG);
+      s.flags = fl & ~kStepFlags;
       s.cnt = 0;
       s.fallback = 0;
       s.alpha = 1.0;
     }
-    if (F.summary) atomicOr(F.summary, s.flags);
+    put_flags(F, inst, s.flags);
   }
 }
 
@@ -738,10 +873,11 @@ __global__ __launch_bounds__(kBlock) void fin_tau(FA F) {
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_skip(FA F, int force) {
   FIN_BEGIN
-  const double dn = slot_sum(F, inst, kSlotM + 2);
-  const double an = slot_sum(F, inst, kSlotM + 3);
+  const int sl[2] = {kSlotM + 2, kSlotM + 3};
+  double r[2];
+  fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    const TS nrm_d = (TS)(float)sqrt(dn), nrm_a = (TS)(float)sqrt(an);
+    const TS nrm_d = (TS)(float)sqrt(r[0]), nrm_a = (TS)(float)sqrt(r[1]);
     const TS relChange = nrm_d / nrm_a;
     uint32_t fl = s.flags & ~(F_BWD | F_SKIPG);
     TS csv = (TS)s.currSkipValue;
@@ -762,29 +898,70 @@ __global__ __launch_bounds__(kBlock) void fin_skip(FA F, int force) {
   FIN_END
 }
 
-// end of descend/get_g: lam2..5, f; mode 0: Armijo test of min_u
-// (2d:543-549); mode 1: after the fallback; mode 2: initial get_g
+// End of descend/get_g: lam2..5 and f.  mode 0: the Armijo test of min_u
+// (2d:543-549) and, if it fails, the backtracking step size: with
+// ls_exact = 0 the five candidate steps alpha = gamma^k are evaluated from
+// the quadratic f(alpha) whose coefficients the descent pass gathered (the
+// state for the chosen alpha is then rebuilt exactly by k_interp_*), with
+// ls_exact = 1 one round at a time by k_bt_upd_* and fin_lams.  mode 1:
+// after the 5-strike fallback.  mode 2: the initial get_g.
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_descend(FA F, int mode) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
-  const double x2 = slot_sum(F, inst, 0), y2 = slot_sum(F, inst, 1);
-  const double z2 = slot_sum(F, inst, 2), x4 = slot_sum(F, inst, 3);
-  const double y4 = slot_sum(F, inst, 4), z4 = slot_sum(F, inst, 5);
-  const double dd = slot_sum(F, inst, 6);
-  const double l3 = slot_sum(F, inst, kSlotM), l5 = slot_sum(F, inst, kSlotM + 1);
+  const bool poly = mode == 0 && !F.O.ls_exact;
+  double r[19];
+  if (poly) {
+    const int sl[19] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, kSlotM, kSlotM + 1,
+                        kSlotM + 4, kSlotM + 5, kSlotM + 6, kSlotM + 7, kSlotM + 8};
+    fin_sums<19>(F, inst, sl, r);
+  } else {
+    const int sl[9] = {0, 1, 2, 3, 4, 5, 6, kSlotM, kSlotM + 1};
+    double q[9];
+    fin_sums<9>(F, inst, sl, q);
+    for (int k = 0; k < 7; ++k) r[k] = q[k];
+    r[12] = q[7];
+    r[13] = q[8];
+  }
   if (threadIdx.x == 0) {
-    set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
-    s.lam3 = (double)(TS)(float)l3;
-    s.lam5 = (float)l5;
+    set_lam24<TS>(s, z3, r[0], r[1], r[2], r[3], r[4], r[5]);
+    s.lam3 = (double)(TS)(float)r[12];
+    s.lam5 = (float)r[13];
     s.f = f_full<TS>(s);
     if (mode == 0) {
-      s.c_armij = (double)((TS)(float)dd * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
+      s.c_armij = (double)((TS)(float)r[6] * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
       s.alpha = 1.0;
       if (armijo_fails<TS>(s)) {
-        s.flags |= F_BT;
         s.cnt = 0;
-        s.alpha = (double)((TS)s.alpha * (TS)F.O.gamma);
+        if (!poly) {
+          s.flags |= F_BT;
+          s.alpha = (double)((TS)s.alpha * (TS)F.O.gamma);
+        } else {
+          // lam2(a) = A2 + 2a B2 + a^2 C2, lam4(a) = A4 + a B4, same for
+          // lam3 / lam5 (U_dir(a) = U_dir_p + a dU_dir, Au(a) likewise)
+          const double A2 = r[7], B2 = r[8], C2 = r[9], A4 = r[10], B4 = r[11];
+          const double A3 = r[14], B3 = r[15], C3 = r[16], A5 = r[17], B5 = r[18];
+          const double lam1 = s.lam1, hb = (double)((TS)s.beta / 2),
+                       hm = (double)((TS)s.mu / 2);
+          TS al = (TS)s.alpha;
+          bool ok = false;
+          for (int k = 1; k <= 5 && !ok; ++k) {
+            al = al * (TS)F.O.gamma;
+            const double a = (double)al;
+            const double f = lam1 + hb * (A2 + a * (2 * B2 + a * C2)) +
+                             hm * (A3 + a * (2 * B3 + a * C3)) -
+                             (A4 + a * B4) - (A5 + a * B5);
+            s.cnt = k;
+            ok = !(fabs(f) > fabs((double)s.C - a * s.c_armij));
+          }
+          s.alpha = (double)al;
+          if (ok) {
+            s.flags |= F_BT;  // rebuild the state at alpha (k_interp_*)
+          } else {
+            s.flags |= F_FB;
+            s.gam = (double)((TS)s.gam * (TS)F.O.rate_gam);
+          }
+        }
       }
     } else if (mode == 1) {
       s.flags &= ~F_FB;
@@ -794,28 +971,33 @@ __global__ __launch_bounds__(kBlock) void fin_descend(FA F, int mode) {
   FIN_END
 }
 
-// one backtracking round of min_u (2d:555-574)
+// lam2..lam5 and f after an interpolation of the state (update_g,
+// 2d:495-524).  mode 0: one reference backtracking round (2d:555-574);
+// mode 1: the final interpolation of the quadratic line search.
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_bt(FA F) {
+__global__ __launch_bounds__(kBlock) void fin_lams(FA F, int mode) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
-  const double x2 = slot_sum(F, inst, 0), y2 = slot_sum(F, inst, 1);
-  const double z2 = slot_sum(F, inst, 2), x4 = slot_sum(F, inst, 3);
-  const double y4 = slot_sum(F, inst, 4), z4 = slot_sum(F, inst, 5);
-  const double l3 = slot_sum(F, inst, kSlotM), l5 = slot_sum(F, inst, kSlotM + 1);
+  const int sl[8] = {0, 1, 2, 3, 4, 5, kSlotM, kSlotM + 1};
+  double r[8];
+  fin_sums<8>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
-    s.lam3 = (double)(TS)(float)l3;
-    s.lam5 = (float)l5;
+    set_lam24<TS>(s, z3, r[0], r[1], r[2], r[3], r[4], r[5]);
+    s.lam3 = (double)(TS)(float)r[6];
+    s.lam5 = (float)r[7];
     s.f = f_full<TS>(s);
-    s.cnt++;
-    if (!armijo_fails<TS>(s)) {
+    if (mode == 1) {
       s.flags &= ~F_BT;
-    } else if (s.cnt == 5) {
-      s.flags = (s.flags & ~F_BT) | F_FB;
-      s.gam = (double)((TS)s.gam * (TS)F.O.rate_gam);
     } else {
-      s.alpha = (double)((TS)s.alpha * (TS)F.O.gamma);
+      s.cnt++;
+      if (!armijo_fails<TS>(s)) {
+        s.flags &= ~F_BT;
+      } else if (s.cnt == 5) {
+        s.flags = (s.flags & ~F_BT) | F_FB;
+        s.gam = (double)((TS)s.gam * (TS)F.O.rate_gam);
+      } else {
+        s.alpha = (double)((TS)s.alpha * (TS)F.O.gamma);
+      }
     }
   }
   FIN_END
@@ -828,24 +1010,18 @@ template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_wstep(FA F, int init) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
-  const double l1 = slot_sum(F, inst, 0);
-  const double x2 = slot_sum(F, inst, 1), y2 = slot_sum(F, inst, 2);
-  const double z2 = slot_sum(F, inst, 3), x4 = slot_sum(F, inst, 4);
-  const double y4 = slot_sum(F, inst, 5), z4 = slot_sum(F, inst, 6);
-  double nup = 0, nuu = 0;
-  if (!init) {
-    nup = slot_sum(F, inst, 7);
-    nuu = slot_sum(F, inst, 8);
-  }
+  const int sl[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  double r[9];
+  fin_sums<9>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     const TS beta = (TS)s.beta;
     if (init) {
-      s.lam1 = (double)(TS)l1;
-      set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
+      s.lam1 = (double)(TS)r[0];
+      set_lam24<TS>(s, z3, r[1], r[2], r[3], r[4], r[5], r[6]);
     } else {
       s.f = (float)((TS)s.f - (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
-      s.lam1 = (double)(TS)l1;
-      set_lam24<TS>(s, z3, x2, y2, z2, x4, y4, z4);
+      s.lam1 = (double)(TS)r[0];
+      set_lam24<TS>(s, z3, r[1], r[2], r[3], r[4], r[5], r[6]);
       s.f = (float)((TS)s.f + (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
       const TS gam = (TS)s.gam;
       const TS Qp = (TS)s.Q;
@@ -853,8 +1029,8 @@ __global__ __launch_bounds__(kBlock) void fin_wstep(FA F, int init) {
       s.Q = (double)Q;
       s.C = (float)((gam * Qp * (TS)s.C + (TS)s.f) / Q);
       s.count_total++;
-      const TS nrmup = (TS)(float)sqrt(nup);
-      const TS relchg = (TS)(float)sqrt(nuu) / nrmup;
+      const TS nrmup = (TS)(float)sqrt(r[7]);
+      const TS relchg = (TS)(float)sqrt(r[8]) / nrmup;
       s.nrmup = (double)nrmup;
       s.RelChg = (double)relchg;
       if (F.trace && s.count_total <= F.trace_cap) {
@@ -880,10 +1056,12 @@ __global__ __launch_bounds__(kBlock) void fin_wstep(FA F, int init) {
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_init2(FA F) {
   FIN_BEGIN
-  const double l3 = slot_sum(F, inst, kSlotM), l5 = slot_sum(F, inst, kSlotM + 1);
+  const int sl[2] = {kSlotM, kSlotM + 1};
+  double r[2];
+  fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    s.lam3 = (double)(TS)(float)l3;
-    s.lam5 = (float)l5;
+    s.lam3 = (double)(TS)(float)r[0];
+    s.lam5 = (float)r[1];
     s.f = f_full<TS>(s);
     s.Q = 1.0;
     s.C = s.f;
@@ -944,13 +1122,15 @@ __global__ __launch_bounds__(kBlock) void fin_scaleb(FA F, const MinMax *mmp,
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_outer1(FA F) {
   FIN_BEGIN
-  const double rc = slot_sum(F, inst, 0);
+  const int sl[1] = {0};
+  double r[1];
+  fin_sums<1>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     s.count_outer++;
-    const TS r = (TS)(float)sqrt(rc) / (TS)s.nrmup;
-    s.RelChgOut = (double)r;
+    const TS rr = (TS)(float)sqrt(r[0]) / (TS)s.nrmup;
+    s.RelChgOut = (double)rr;
     uint32_t fl = s.flags & ~F_INNER_DONE;
-    if (r < (TS)F.O.tol)
+    if (rr < (TS)F.O.tol)
       fl = (fl & ~F_ACTIVE) | F_DONE | F_JUST_DONE;
     else
       fl |= F_MLP;
@@ -964,15 +1144,16 @@ template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_mlp(FA F) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
-  const double x4 = slot_sum(F, inst, 0), y4 = slot_sum(F, inst, 1);
-  const double z4 = slot_sum(F, inst, 2), l5 = slot_sum(F, inst, kSlotM);
+  const int sl[4] = {0, 1, 2, kSlotM};
+  double r[4];
+  fin_sums<4>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     s.f = s.f + (s.lam4 + s.lam5);
-    float lam4 = (float)x4;
-    lam4 += (float)y4;
-    if (z3) lam4 += (float)z4;
+    float lam4 = (float)r[0];
+    lam4 += (float)r[1];
+    if (z3) lam4 += (float)r[2];
     s.lam4 = lam4;
-    s.lam5 = (float)l5;
+    s.lam5 = (float)r[3];
     s.f = s.f - (s.lam4 + s.lam5);
     const TS beta0 = (TS)s.beta;
     TS beta = beta0 * (TS)F.O.rate_cnt;
@@ -996,25 +1177,27 @@ __global__ __launch_bounds__(kBlock) void fin_mlp(FA F) {
   FIN_END
 }
 
-// end of a step: drops the per-step flags of every instance and reports the
-// OR of all flags to the host
+// end of a step: drops the per-step flags of every instance and publishes
+// the flags
 __global__ __launch_bounds__(kBlock) void fin_refresh(FA F) {
   if (threadIdx.x != 0) return;
   InstScal &s = F.S[blockIdx.x];
-  s.flags &= ~(F_MLP | F_JUST_DONE | F_INNER_DONE | F_BT | F_FB | F_BWD | F_SKIPG);
-  if (F.summary) atomicOr(F.summary, s.flags);
+  s.flags &= ~kStepFlags;
+  put_flags(F, blockIdx.x, s.flags);
 }
 
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_relerr(FA F) {
   FIN_BEGIN
-  const double e2 = slot_sum(F, inst, 0), t2 = slot_sum(F, inst, 1);
+  const int sl[2] = {0, 1};
+  double r[2];
+  fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0)
-    s.rel_error = (double)((TS)(float)sqrt(e2) / (TS)(float)sqrt(t2));
+    s.rel_error = (double)((TS)(float)sqrt(r[0]) / (TS)(float)sqrt(r[1]));
   FIN_END
 }
 
-__global__ void k_init_scal(InstScal *S, int B, int Bpad) {
+__global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Bpad) return;
   InstScal z;
@@ -1023,6 +1206,10 @@ __global__ void k_init_scal(InstScal *S, int B, int Bpad) {
   z.alpha = 1.0;
   z.Q = 1.0;
   S[i] = z;
+  if (fout) {
+    fout[i] = z.flags;
+    __threadfence_system();
+  }
 }
 
 // ===========================================================================
@@ -1052,6 +1239,7 @@ class Engine {
     O_.skipMulRatio = o.skipMulRatio;
     O_.maxRelativeChange = o.maxRelativeChange;
     O_.skip = (o.skipMulRatio != 0 && bt.kind != kDense) ? 1 : 0;
+    O_.ls_exact = o.ls_exact;
     nblk_n_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
     nblk_m_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
   }
@@ -1067,7 +1255,6 @@ class Engine {
     need += align_up(sizeof(MinMax) * (size_t)bt_.Bpad * kNblkMax);
     need += align_up(sizeof(InstScal) * bt_.Bpad);
     need += align_up(sizeof(GroupDesc) * bt_.G);
-    need += align_up(sizeof(uint32_t) * 4);
     need += align_up(sizeof(float *) * bt_.Bpad) * 2;
     if (bt_.kind == kDense && W == 1)
       need += align_up(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n);
@@ -1105,10 +1292,26 @@ class Engine {
     mm_ = (MinMax *)take(sizeof(MinMax) * (size_t)bt_.Bpad * kNblkMax);
     S_ = (InstScal *)take(sizeof(InstScal) * bt_.Bpad);
     G_ = (GroupDesc *)take(sizeof(GroupDesc) * bt_.G);
-    summary_ = (uint32_t *)take(sizeof(uint32_t) * 4);
     ptrs_in_ = (const float **)take(sizeof(float *) * bt_.Bpad);
     ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
     scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n) : nullptr;
+    if (ctx_->hflags_n < (size_t)bt_.Bpad) {
+      if (ctx_->hflags) {
+        (void)hipStreamSynchronize(s_);
+        (void)hipHostFree(ctx_->hflags);
+        ctx_->hflags = ctx_->dflags = nullptr;
+        ctx_->hflags_n = 0;
+      }
+      void *h = nullptr;
+      int rc0 = hip_check(ctx_, hipHostMalloc(&h, sizeof(uint32_t) * bt_.Bpad, hipHostMallocMapped | hipHostMallocCoherent), "flag hipHostMalloc");
+      if (rc0) return rc0;
+      void *d = nullptr;
+      rc0 = hip_check(ctx_, hipHostGetDevicePointer(&d, h, 0), "flag device pointer");
+      if (rc0) { (void)hipHostFree(h); return rc0; }
+      ctx_->hflags = (uint32_t *)h;
+      ctx_->dflags = (uint32_t *)d;
+      ctx_->hflags_n = bt_.Bpad;
+    }
     if ((size_t)(p - (char *)ctx_->ws) > need)
       return set_err(ctx_, TV3_ERR_ALLOC, "internal error: workspace layout exceeds its size");
     // zero all state (multipliers, W, DtsAtd start at 0; mat<T> zero-fills)
@@ -1151,7 +1354,6 @@ class Engine {
   MinMax *mm_ = nullptr;
   InstScal *S_ = nullptr;
   GroupDesc *G_ = nullptr;
-  uint32_t *summary_ = nullptr;
   const float **ptrs_in_ = nullptr;
   float **ptrs_out_ = nullptr;
   float *scratch_ = nullptr;
@@ -1165,7 +1367,7 @@ class Engine {
   FA fa(uint32_t req, uint32_t forbid = 0) const {
     FA f;
     f.S = S_; f.G = G_; f.part = part_; f.nblk_n = nblk_n_; f.nblk_m = nblk_m_;
-    f.W = W; f.req = req; f.forbid = forbid; f.summary = summary_;
+    f.W = W; f.req = req; f.forbid = forbid; f.fout = ctx_->dflags;
     f.trace = trace_; f.trace_cap = trace_cap_; f.O = O_;
     return f;
   }
@@ -1182,20 +1384,21 @@ class Engine {
     prof_end(ctx_);
   }
 
-  int clear_summary() {
-    return hip_check(ctx_, hipMemsetAsync(summary_, 0, sizeof(uint32_t), s_), "summary clear");
-  }
+  // OR of the instance flags the fin kernels published to mapped host memory
   int read_summary(uint32_t &out) {
-    int rc = hip_check(ctx_, hipMemcpyAsync(ctx_->pinned, summary_, sizeof(uint32_t), hipMemcpyDeviceToHost, s_), "summary copy");
-    if (!rc) rc = hip_check(ctx_, hipStreamSynchronize(s_), "summary sync");
-    out = rc ? 0u : *(volatile uint32_t *)ctx_->pinned;
-    return rc;
+    int rc = hip_check(ctx_, hipStreamSynchronize(s_), "flag sync");
+    out = 0;
+    if (rc) return rc;
+    const volatile uint32_t *h = ctx_->hflags;
+    for (int i = 0; i < bt_.Bpad; ++i) out |= h[i];
+    return TV3_OK;
   }
 
   // get_g after U and Ux.. are in place: A U, A^T skip state, A^T, lam3/5
   void get_g_products(uint32_t req, uint32_t forbid, int force) {
     fwd(a_.U, a_.Au, req);
-    hipLaunchKernelGGL(k_mpost<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_, O_.skip);
+    hipLaunchKernelGGL(k_mpost<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_, O_.skip,
+                       (!force && !O_.ls_exact) ? 1 : 0);
     if (O_.skip) {
       hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kBlock), 0, s_, fa(req, forbid), force);
       hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
@@ -1218,7 +1421,7 @@ class Engine {
     int rc;
     const auto t0 = std::chrono::steady_clock::now();
     // instance scalars and caller pointers
-    hipLaunchKernelGGL(k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad);
+    hipLaunchKernelGGL(k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad, ctx_->dflags);
     std::vector<const float *> hin(bt_.Bpad, nullptr);
     for (int i = 0; i < bt_.B; ++i) hin[i] = b[i];
     rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hin.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
@@ -1254,34 +1457,41 @@ class Engine {
       std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
       std::swap(a_.Au, a_.Aup);
       // descend (2d:540)
-      hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.nonneg, O_.dt_exact);
+      hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_,
+                         O_.nonneg, O_.dt_exact, O_.ls_exact ? 0 : 1);
       get_g_products(F_ACTIVE, 0, 0);
-      if ((rc = clear_summary())) return rc;
       hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
       if ((rc = read_summary(sum))) return rc;
+      // Armijo backtracking (2d:548-575)
       if (sum & F_BT) {
-        hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-        hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-        while (sum & F_BT) {
-          hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-          hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-          if ((rc = clear_summary())) return rc;
-          hipLaunchKernelGGL(fin_bt<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT));
-          if ((rc = read_summary(sum))) return rc;
+        if (!O_.ls_exact) {
+          // step size already chosen on the device: rebuild the state once
+          hipLaunchKernelGGL((k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+          hipLaunchKernelGGL((k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+          hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 1);
+        } else {
+          hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+          hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+          while (sum & F_BT) {
+            hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+            hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+            hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 0);
+            if ((rc = read_summary(sum))) return rc;
+          }
         }
-        if (sum & F_FB) {
-          // 5-strike fallback (2d:561-567): tau from the first-iteration
-          // formula, U = Up, descend(force)
-          tau_first(F_FB, 0);
-          hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
-          hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_, O_.nonneg, O_.dt_exact);
-          get_g_products(F_FB, 0, 1);
-          hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
-        }
+      }
+      if (sum & F_FB) {
+        // 5-strike fallback (2d:561-567): tau from the first-iteration
+        // formula, U = Up, descend(force)
+        tau_first(F_FB, 0);
+        hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
+        hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_,
+                           O_.nonneg, O_.dt_exact, 0);
+        get_g_products(F_FB, 0, 1);
+        hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
       }
       // ---- update_W, reference values, gradient, stop test ----
       hipLaunchKernelGGL((k_wstep<W, TS, false>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.tv_iso, O_.dt_exact);
-      if ((rc = clear_summary())) return rc;
       hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
       if ((rc = read_summary(sum))) return rc;
       if (sum & F_INNER_DONE) {
@@ -1292,7 +1502,6 @@ class Engine {
         hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kBlock), 0, s_, fa(F_MLP));
         hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
         hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
-        if ((rc = clear_summary())) return rc;
         hipLaunchKernelGGL(fin_refresh, gi(), dim3(kBlock), 0, s_, fa(0));
         if ((rc = read_summary(sum))) return rc;
       }

@@ -99,7 +99,10 @@ class Context:
             raise RuntimeError("tval3_amd needs a HIP device")
         self.device = torch.device("cuda", device)
         if stream is None:
-            stream = torch.cuda.current_stream(self.device)
+            # a stream of our own (default priority); every native call is
+            # ordered after torch's current stream and torch's current stream
+            # after it (_enter/_exit), so tensors move freely between the two
+            stream = torch.cuda.Stream(self.device)
         self.stream = stream
         h = _VP()
         rc = lib().tv3_ctx_create(device, _VP(stream.cuda_stream), C.byref(h))
@@ -125,6 +128,19 @@ class Context:
     def _check(self, rc):
         if rc:
             raise Tval3Error(rc, lib().tv3_ctx_error(self.h).decode())
+
+    def _enter(self):
+        self.stream.wait_stream(self.torch.cuda.current_stream(self.device))
+
+    def _exit(self):
+        self.torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+    def _call(self, fn, *args):
+        """Runs one native entry point ordered with torch's current stream."""
+        self._enter()
+        rc = fn(*args)
+        self._exit()
+        self._check(rc)
 
     def sync(self):
         self._check(lib().tv3_ctx_sync(self.h))
@@ -158,8 +174,8 @@ class Context:
         At = self.tensor(A, t.bfloat16 if bf16 else t.float32)
         m, n = At.shape
         h = _VP()
-        self._check(lib().tv3_plan_dense(self.h, m, n, 1 if bf16 else 0,
-                                         _ptr(At), C.byref(h)))
+        self._call(lib().tv3_plan_dense, self.h, m, n, 1 if bf16 else 0,
+                                         _ptr(At), C.byref(h))
         return Plan(self, h, [At])
 
     def plan_sparse(self, m, n, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind,
@@ -169,14 +185,14 @@ class Context:
                 self.tensor(csr_val, t.float32), self.tensor(csc_ptr, t.int32),
                 self.tensor(csc_ind, t.int32), self.tensor(csc_val, t.float32)]
         h = _VP()
-        self._check(lib().tv3_plan_sparse(self.h, m, n, int(arrs[1].numel()),
-                                          *[_ptr(a) for a in arrs], C.byref(h)))
+        self._call(lib().tv3_plan_sparse, self.h, m, n, int(arrs[1].numel()),
+                                          *[_ptr(a) for a in arrs], C.byref(h))
         return Plan(self, h, arrs)
 
     def plan_geometry(self, geo):
         s = geo.struct()
         h = _VP()
-        self._check(lib().tv3_plan_geometry(self.h, C.byref(s), C.byref(h)))
+        self._call(lib().tv3_plan_geometry, self.h, C.byref(s), C.byref(h))
         return Plan(self, h, [])
 
     # ---- operators ------------------------------------------------------
@@ -185,8 +201,8 @@ class Context:
         x = self.tensor(x, t.float32)
         y = t.empty(plan.n if trans else plan.m, dtype=t.float32,
                     device=self.device)
-        self._check(lib().tv3_gemv(self.h, plan.h, int(bool(trans)), _ptr(x),
-                                   _ptr(y)))
+        self._call(lib().tv3_gemv, self.h, plan.h, int(bool(trans)), _ptr(x),
+                                   _ptr(y))
         return y
 
     def D(self, U, dims):
@@ -194,7 +210,7 @@ class Context:
         q, p, r = dims
         U = self.tensor(U, t.float32)
         out = [t.empty_like(U), t.empty_like(U)] + ([t.empty_like(U)] if r > 1 else [None])
-        self._check(lib().tv3_D(self.h, q, p, r, _ptr(U), *[_ptr(o) for o in out]))
+        self._call(lib().tv3_D, self.h, q, p, r, _ptr(U), *[_ptr(o) for o in out])
         return out
 
     def Dt(self, X, Y, Z, dims, exact=False):
@@ -203,8 +219,8 @@ class Context:
         X, Y = self.tensor(X, t.float32), self.tensor(Y, t.float32)
         Z = self.tensor(Z, t.float32) if r > 1 else None
         res = t.empty_like(X)
-        self._check(lib().tv3_Dt(self.h, q, p, r, int(bool(exact)), _ptr(X),
-                                 _ptr(Y), _ptr(Z), _ptr(res)))
+        self._call(lib().tv3_Dt, self.h, q, p, r, int(bool(exact)), _ptr(X),
+                                 _ptr(Y), _ptr(Z), _ptr(res))
         return res
 
     def shrink(self, Ux, Uy, Uz, sx, sy, sz, dims, beta, iso=False,
@@ -216,17 +232,17 @@ class Context:
         outs = [t.empty_like(ins[0]), t.empty_like(ins[0]),
                 t.empty_like(ins[0]) if r > 1 else None]
         lam1 = C.c_double()
-        self._check(lib().tv3_shrink(self.h, q, p, r, float(beta), int(bool(iso)),
+        self._call(lib().tv3_shrink, self.h, q, p, r, float(beta), int(bool(iso)),
                                      int(bool(scal_double)),
                                      *[_ptr(a) for a in ins + outs],
-                                     C.byref(lam1)))
+                                     C.byref(lam1))
         return outs[0], outs[1], outs[2], lam1.value
 
     def dot(self, x, y):
         t = self.torch
         x, y = self.tensor(x, t.float32), self.tensor(y, t.float32)
         out = C.c_double()
-        self._check(lib().tv3_dot(self.h, x.numel(), _ptr(x), _ptr(y), C.byref(out)))
+        self._call(lib().tv3_dot, self.h, x.numel(), _ptr(x), _ptr(y), C.byref(out))
         return out.value
 
     # ---- solver -----------------------------------------------------------
@@ -245,9 +261,9 @@ class Context:
             raise Tval3Error(1, "Argument error: U and Ut must have the same size!")
         info = Info()
         trace = (TraceRec * trace_cap)() if trace_cap else None
-        self._check(lib().tv3_solve(self.h, plan.h, C.byref(opts), q, p, r,
+        self._call(lib().tv3_solve, self.h, plan.h, C.byref(opts), q, p, r,
                                     _ptr(b), _ptr(U), _ptr(Utt), C.byref(info),
-                                    trace, int(trace_cap)))
+                                    trace, int(trace_cap))
         n_tr = min(info.total_iters, trace_cap)
         return U, info, ([trace[i] for i in range(n_tr)] if trace else [])
 
@@ -279,8 +295,8 @@ class Context:
         utarr = ((_VP * B)(*[(x.data_ptr() if x is not None else None) for x in Utts])
                  if Utts is not None else None)
         infos = (Info * B)()
-        self._check(lib().tv3_solve_batch(self.h, B, parr, C.byref(opts),
-                                          dims_arr, barr, uarr, utarr, infos))
+        self._call(lib().tv3_solve_batch, self.h, B, parr, C.byref(opts),
+                                          dims_arr, barr, uarr, utarr, infos)
         return Us, [infos[i] for i in range(B)]
 
 
@@ -300,9 +316,9 @@ class Plan:
         """Voxel index list of every row as computed by the device walk kernel."""
         ptr = np.zeros(self.m + 1, np.int64)
         lin = np.zeros(max(self.nnz, 1), np.int32)
-        self.ctx._check(lib().tv3_plan_walk(self.ctx.h, self.h,
+        self.ctx._call(lib().tv3_plan_walk, self.ctx.h, self.h,
                                             ptr.ctypes.data_as(_VP),
-                                            lin.ctypes.data_as(_VP)))
+                                            lin.ctypes.data_as(_VP))
         return ptr, lin[:self.nnz]
 
     def close(self):
