@@ -141,20 +141,25 @@ def test_hemisphere_3d_solve(ctx, exact):
     _compare(ctx, ctx.plan_geometry(geo), op, b, (grid, grid, grid), opts, Ut=x)
 
 
-def test_batch_shared_ring(ctx):
-    """70 instances sharing one ring geometry: one 64-lane group + a padded one."""
+@pytest.mark.parametrize("B,width", [(70, None), (130, None), (70, "64")])
+def test_batch_shared_ring(ctx, B, width, monkeypatch):
+    """Instances sharing one ring geometry, lane-interleaved groups: 16-lane
+    groups (70: five groups, the last padded; 130: nine groups, dealt to the
+    XCDs) and 64-lane groups."""
+    if width:
+        monkeypatch.setenv("TV3_GROUP_WIDTH", width)
     geo, op, x, _ = _ring_case(48, 48, 5)
     plan = ctx.plan_geometry(geo)
-    B = 70
-    opts = tval3_amd.default_options(maxit=60)
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1)
     bs = [synth.add_noise(op.gemv(x), 0.005, 100 + i) for i in range(B)]
     Us, infos = ctx.solve_batch([plan] * B, bs, [(48, 48, 1)] * B, opts,
                                 Uts=[x] * B)
-    for i in (0, 1, 63, 64, 69):
+    for i in sorted({0, 1, 15, 16, 63, 64, B - 1}):
         Uo, info_o, _ = O.solve(op, bs[i], (48, 48, 1), opts, Ut=x)
         rel = _rel(Us[i].cpu().numpy(), Uo)
         assert rel <= TOL_U, (i, rel, infos[i].total_iters, info_o.total_iters)
         assert infos[i].total_iters == info_o.total_iters
+        assert infos[i].rel_error == pytest.approx(info_o.rel_error, rel=1e-3)
 
 
 def test_batch_uneven_dense(ctx):

@@ -142,38 +142,33 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64(const GroupDesc *G,
   const float *xb = x + gd.off_n;
   double acc = 0.0;
   if (P.kind == kGeometry) {
-    // the reference's error recurrence (3d/mat_mult.h:244-279), one voxel
-    // per step, identical to the closed form of ray_lin
-    const RayDesc *rp = P.rays + row;
-    const int L = uni_i(rp->L), lin0 = uni_i(rp->lin0);
-    const int sM = uni_i(rp->s_major), sA = uni_i(rp->s_a), sB = uni_i(rp->s_b);
-    const int ta = 2 * uni_i(rp->ma), tb = 2 * uni_i(rp->mb), twoL = 2 * L;
+    // voxel indices of 64 consecutive walk steps, one per lane (closed form of
+    // the reference's recurrence), handed to the whole wavefront by readlane
+    const RayDesc rd = P.rays[row];
+    const int L = uni_i(rd.L);
     const float sc = P.scale;
     const bool unit = sc == 1.0f;
-    int lin = lin0, ea = ta - L, eb = tb - L;
-    int j = 0;
-    for (; j + kPipe <= L + 1; j += kPipe) {
-      float v[kPipe];
+    for (int j0 = 0; j0 <= L; j0 += 64) {
+      const int jl = j0 + lane;
+      const int myidx = jl <= L ? ray_lin(rd, jl) : 0;
+      const int cnt = min(64, L + 1 - j0);
+      for (int u0 = 0; u0 < cnt; u0 += kPipe) {
+        float v[kPipe];
 #pragma unroll
-      for (int u = 0; u < kPipe; ++u) {
-        v[u] = (xb + (long long)lin * 64)[lane];
-        if (ea > 0) { lin += sA; ea -= twoL; }
-        if (eb > 0) { lin += sB; eb -= twoL; }
-        ea += ta; eb += tb; lin += sM;
+        for (int u = 0; u < kPipe; ++u) {
+          const int iu = __builtin_amdgcn_readlane(myidx, u0 + u);
+          v[u] = (u0 + u < cnt) ? (xb + (long long)iu * 64)[lane] : 0.0f;
+        }
+        if (unit) {
+#pragma unroll
+          for (int u = 0; u < kPipe; ++u)
+            if (u0 + u < cnt) acc += (double)v[u];
+        } else {
+#pragma unroll
+          for (int u = 0; u < kPipe; ++u)
+            if (u0 + u < cnt) acc += (double)(v[u] * sc);
+        }
       }
-      if (unit) {
-#pragma unroll
-        for (int u = 0; u < kPipe; ++u) acc += (double)v[u];
-      } else {
-#pragma unroll
-        for (int u = 0; u < kPipe; ++u) acc += (double)(v[u] * sc);
-      }
-    }
-    for (; j <= L; ++j) {
-      acc += (double)((xb + (long long)lin * 64)[lane] * sc);
-      if (ea > 0) { lin += sA; ea -= twoL; }
-      if (eb > 0) { lin += sB; eb -= twoL; }
-      ea += ta; eb += tb; lin += sM;
     }
   } else if (P.kind == kSparse) {
     const int k0 = uni_i(P.r_ptr[row]), k1 = uni_i(P.r_ptr[row + 1]);
@@ -290,30 +285,32 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
     const float sc = P.scale;
     const bool unit = geo && sc == 1.0f;
     const int k0 = uni_i(ptr[col]), k1 = uni_i(ptr[col + 1]);
-    for (int k = k0; k < k1; k += kPipe) {
-      const int cnt = min(kPipe, k1 - k);
-      const bool ld = lane < cnt;
-      const int id = ld ? ind[k + lane] : 0;
-      const float val = (ld && !geo) ? P.c_val[k + lane] : sc;
-      float v[kPipe], w[kPipe];
+    for (int kb = k0; kb < k1; kb += 64) {
+      const int cnt64 = min(64, k1 - kb);
+      const bool ld = lane < cnt64;
+      const int id = ld ? ind[kb + lane] : 0;
+      const float val = (ld && !geo) ? P.c_val[kb + lane] : sc;
+      for (int u0 = 0; u0 < cnt64; u0 += kPipe) {
+        float v[kPipe], w[kPipe];
 #pragma unroll
-      for (int u = 0; u < kPipe; ++u) {
-        const int iu = __builtin_amdgcn_readlane(id, u);
-        w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, val), u));
-        v[u] = u < cnt ? (xb + (long long)iu * 64)[lane] : 0.0f;
-      }
-      if (unit) {
+        for (int u = 0; u < kPipe; ++u) {
+          const int iu = __builtin_amdgcn_readlane(id, u0 + u);
+          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, val), u0 + u));
+          v[u] = (u0 + u < cnt64) ? (xb + (long long)iu * 64)[lane] : 0.0f;
+        }
+        if (unit) {
 #pragma unroll
-        for (int u = 0; u < kPipe; ++u)
-          if (u < cnt) acc += (double)v[u];
-      } else if (geo) {
+          for (int u = 0; u < kPipe; ++u)
+            if (u0 + u < cnt64) acc += (double)v[u];
+        } else if (geo) {
 #pragma unroll
-        for (int u = 0; u < kPipe; ++u)
-          if (u < cnt) acc += (double)(v[u] * sc);
-      } else {
+          for (int u = 0; u < kPipe; ++u)
+            if (u0 + u < cnt64) acc += (double)(v[u] * sc);
+        } else {
 #pragma unroll
-        for (int u = 0; u < kPipe; ++u)
-          if (u < cnt) acc += (double)w[u] * (double)v[u];
+          for (int u = 0; u < kPipe; ++u)
+            if (u0 + u < cnt64) acc += (double)w[u] * (double)v[u];
+        }
       }
     }
   } else {
@@ -355,6 +352,146 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
     float v = (float)acc;
     if (sub) v = v - sub[o];
     y[o] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// W = 16, XCD-affine (shared-geometry batches)
+// ---------------------------------------------------------------------------
+// A group of 16 instances keeps its image (n x 16 floats) and measurements
+// (m x 16 floats) in one XCD's L2: with at least eight groups, the flat block
+// index b runs group (b % 8) + 8 * slot, so every block of a group lands on
+// the same XCD (workgroups are dealt to the eight XCDs round robin).  A
+// wavefront handles one row; its four 16-lane quarters take four consecutive
+// entries per load (4 x 64 bytes), kQuad loads in flight; the quarter sums
+// are combined by a fixed butterfly.
+constexpr int kXcd = 8;
+constexpr int kQuad = 4;
+
+__device__ __forceinline__ bool grp_decode(int nG, int RB, int affine, int &g,
+                                           int &rb) {
+  if (!affine) {
+    g = blockIdx.y;
+    rb = blockIdx.x;
+    return true;
+  }
+  const int b = blockIdx.x;
+  const int t = b / kXcd;
+  g = (b % kXcd) + kXcd * (t / RB);
+  rb = t % RB;
+  return g < nG;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fwd_w16(const GroupDesc *G, int nG,
+                                                    int RB, int affine,
+                                                    const float *x, float *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask) {
+  int g, rb;
+  if (!grp_decode(nG, RB, affine, g, rb)) return;
+  const GroupDesc &gd = G[g];
+  const int lane = threadIdx.x & 63, q4 = lane >> 4, bl = lane & 15;
+  const int row = uni_i(rb * (kBlock / 64) + (threadIdx.x >> 6));
+  if (row >= gd.m) return;
+  const bool on = !mask || (S[g * 16 + bl].flags & mask) == mask;
+  if (!__any(on)) return;
+  const PlanDev &P = gd.plan;
+  const float *xb = x + gd.off_n + bl;
+  double acc = 0.0;
+  if (P.kind == kGeometry) {
+    // 64 walk steps per round, one index per lane (closed form); quarter q4
+    // of the wavefront takes steps j0 + 4u + q4 through a lane shuffle
+    const RayDesc rd = P.rays[row];
+    const int L = uni_i(rd.L);
+    const float sc = P.scale;
+    for (int j0 = 0; j0 <= L; j0 += 64) {
+      const int jl = j0 + lane;
+      const int myidx = jl <= L ? ray_lin(rd, jl) : 0;
+      for (int u0 = 0; u0 < 16; u0 += kQuad) {
+        if (j0 + 4 * u0 > L) break;
+        float v[kQuad];
+#pragma unroll
+        for (int q = 0; q < kQuad; ++q) {
+          const int t = 4 * (u0 + q) + q4;
+          const int iu = __shfl(myidx, t, 64);
+          v[q] = (j0 + t <= L) ? xb[(long long)iu * 16] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kQuad; ++q)
+          if (j0 + 4 * (u0 + q) + q4 <= L) acc += (double)(v[q] * sc);
+      }
+    }
+  } else {
+    const int k0 = uni_i(P.r_ptr[row]), k1 = uni_i(P.r_ptr[row + 1]);
+    for (int k = k0; k < k1; k += 4 * kQuad) {
+      float v[kQuad], w[kQuad];
+#pragma unroll
+      for (int q = 0; q < kQuad; ++q) {
+        const int kk = k + 4 * q + q4;
+        const bool ok = kk < k1;
+        w[q] = ok ? P.r_val[kk] : 0.0f;
+        v[q] = ok ? xb[(long long)P.r_ind[kk] * 16] : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < kQuad; ++q)
+        if (k + 4 * q + q4 < k1) acc += (double)w[q] * (double)v[q];
+    }
+  }
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  if (on && q4 == 0) y[gd.off_m + (long long)row * 16 + bl] = (float)acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwd_w16(const GroupDesc *G, int nG,
+                                                    int RB, int affine,
+                                                    const float *x, float *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask,
+                                                    const float *sub) {
+  int g, rb;
+  if (!grp_decode(nG, RB, affine, g, rb)) return;
+  const GroupDesc &gd = G[g];
+  const int lane = threadIdx.x & 63, q4 = lane >> 4, bl = lane & 15;
+  const int col = uni_i(rb * (kBlock / 64) + (threadIdx.x >> 6));
+  if (col >= gd.n) return;
+  const bool on = !mask || (S[g * 16 + bl].flags & mask) == mask;
+  if (!__any(on)) return;
+  const PlanDev &P = gd.plan;
+  const bool geo = P.kind == kGeometry;
+  const int *ptr = geo ? P.g_cptr : P.c_ptr;
+  const int *ind = geo ? P.g_cray : P.c_ind;
+  const float sc = P.scale;
+  const float *xb = x + gd.off_m + bl;
+  double acc = 0.0;
+  const int k0 = uni_i(ptr[col]), k1 = uni_i(ptr[col + 1]);
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int cnt64 = min(64, k1 - kb);
+    const bool ld = lane < cnt64;
+    const int id = ld ? ind[kb + lane] : 0;
+    const float val = (ld && !geo) ? P.c_val[kb + lane] : 0.0f;
+    for (int u0 = 0; u0 < 16; u0 += kQuad) {
+      if (4 * u0 >= cnt64) break;
+      float v[kQuad], w[kQuad];
+#pragma unroll
+      for (int q = 0; q < kQuad; ++q) {
+        const int t = 4 * (u0 + q) + q4;
+        const int iu = __shfl(id, t, 64);
+        w[q] = __shfl(val, t, 64);
+        v[q] = (t < cnt64) ? xb[(long long)iu * 16] : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < kQuad; ++q)
+        if (4 * (u0 + q) + q4 < cnt64)
+          acc += geo ? (double)(v[q] * sc) : (double)w[q] * (double)v[q];
+    }
+  }
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  if (on && q4 == 0) {
+    const long long o = gd.off_n + (long long)col * 16 + bl;
+    float r = (float)acc;
+    if (sub) r = r - sub[o];
+    y[o] = r;
   }
 }
 
@@ -451,6 +588,38 @@ void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
   dim3 grid((max_n + 3) / 4, G);
   hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
                      mask, sub);
+}
+
+template <>
+void launch_fwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                    const float *x, float *y, const InstScal *S,
+                    uint32_t mask) {
+  const int RB = (max_m + 3) / 4;
+  if (G >= kXcd) {
+    const int nb = kXcd * ((G + kXcd - 1) / kXcd) * RB;
+    hipLaunchKernelGGL(k_fwd_w16, dim3(nb), dim3(kBlock), 0, s, G_dev, G, RB, 1,
+                       x, y, S, mask);
+  } else {
+    hipLaunchKernelGGL(k_fwd_w16, dim3(RB, G), dim3(kBlock), 0, s, G_dev, G, RB,
+                       0, x, y, S, mask);
+  }
+}
+
+template <>
+void launch_bwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                    int kind, const float *x, float *y, const InstScal *S,
+                    uint32_t mask, const float *sub, float *scratch) {
+  (void)kind;
+  (void)scratch;
+  const int RB = (max_n + 3) / 4;
+  if (G >= kXcd) {
+    const int nb = kXcd * ((G + kXcd - 1) / kXcd) * RB;
+    hipLaunchKernelGGL(k_bwd_w16, dim3(nb), dim3(kBlock), 0, s, G_dev, G, RB, 1,
+                       x, y, S, mask, sub);
+  } else {
+    hipLaunchKernelGGL(k_bwd_w16, dim3(RB, G), dim3(kBlock), 0, s, G_dev, G, RB,
+                       0, x, y, S, mask, sub);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1578,7 +1579,14 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
     if (r > 1) bt.z3 = true;
   }
   bt.kind = A[0]->dev.kind;
+  // shared operator and size: 64-lane groups (one walk and one 256-byte
+  // gather per voxel for 64 instances).  TV3_GROUP_WIDTH=16 selects 16-lane
+  // groups dealt one per XCD (measured slower on the 256x256 ring).
   bt.W = (shared && B > 1) ? 64 : 1;
+  if (const char *ev = getenv("TV3_GROUP_WIDTH")) {
+    const int w = atoi(ev);
+    if (shared && B > 1 && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense)) bt.W = w;
+  }
   const int W = bt.W;
   bt.G = (B + W - 1) / W;
   bt.Bpad = bt.G * W;
@@ -1604,6 +1612,14 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
       return e.run(b, U, Ut, infos, trace, trace_cap);
     }
     Engine<64, float> e(ctx, bt, *o);
+    return e.run(b, U, Ut, infos, trace, trace_cap);
+  }
+  if (W == 16) {
+    if (o->scal_double) {
+      Engine<16, double> e(ctx, bt, *o);
+      return e.run(b, U, Ut, infos, trace, trace_cap);
+    }
+    Engine<16, float> e(ctx, bt, *o);
     return e.run(b, U, Ut, infos, trace, trace_cap);
   }
   if (o->scal_double) {
