@@ -1,16 +1,33 @@
 #!/usr/bin/env python
-"""TVAL3 reconstructions per second on the 2-D ultrasound ring, one MI355X.
+"""TVAL3 reconstructions per second on one MI355X (default: the 2-D ultrasound
+ring, 256x256).
 
-Workload: 256 transducers on the circle inscribed in a 256x256 grid, all 32,640
-emitter-receiver pairs (i < j), on-the-fly Bresenham projector with weight 1 per
-traversed pixel, disk phantom with seeded inclusions, y = Ax + 0.5 % noise,
-float32, maxit 200, default TVAL3 options (mu = 2^8, beta = 2^5).
+Workloads (--workload):
+  ring         256 transducers on the circle inscribed in a 256x256 grid, all
+               32,640 emitter-receiver pairs (i < j), on-the-fly Bresenham
+               projector (weight 1 per traversed pixel), disk phantom with
+               seeded inclusions, y = Ax + 0.5 % noise, float32, maxit 200,
+               default TVAL3 options (mu = 2^8, beta = 2^5).  A step is one
+               batched solve of B reconstructions sharing the geometry (64-lane
+               kernels); the plain single-reconstruction path (W = 1) is timed
+               beside it.  [default; the headline metric]
+  dense-batch  64 independent dense reconstructions of uneven size (grids from
+               {64..256}^2, m ~ U[20 %, 40 %] n), A ~ N(0, 1/m) stored bf16, one
+               batched solve; per-instance error against the FP64 solve of the
+               same (bf16-rounded) matrices on a sample of instances.
+  hemisphere   1024 Fibonacci transducers on the lower hemisphere of a 128^3
+               volume, 523,776 rays, 3-D anisotropic TV, maxit 100, one solve.
+  dense        64x64 image, m = 1229, dense FP32 A, isotropic TV, maxit 300.
 
-A "step" is one batched solve: B independent reconstructions (one measurement
-vector each, shared geometry) advanced together by the engine's 64-lane kernels.
-value = B * steps / wall time of the timed steps (inputs already in HBM).  The
-plain single-instance FP32 path (W = 1 kernels) is timed beside it, and the CPU
-oracle (the C++/OpenMP restatement of the reference) on the host cores.
+value = reconstructions per second of the timed steps (inputs already in HBM).
+The CPU oracle (the C++/OpenMP restatement of the reference) is timed on the
+host cores on a bounded sample of the same workload.  The roofline object
+reports the dominant measurement-operator kernel: algorithmic bytes per launch
+(CSR-equivalent for the projector: 4-byte index + 4-byte value per entry, row
+pointers, x and y; dense: the stored A, x and y) over its mean launch time,
+taken live with HIP events on the solver's stream.  "traffic" is the HBM bytes
+per launch measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE, calibrated on a
+copy kernel) when a committed profile summary for the workload exists.
 
 Only one GPU is used; --gpus above 1 exits with a message.
 """
@@ -33,57 +50,54 @@ def log(*a):
 
 
 def csr_equiv_bytes(nnz, m, n):
-    """Algorithmic bytes of one A or A^T product, CSR-equivalent (4-byte value +
-    4-byte index per entry, row pointers, x and y)."""
     return 8 * nnz + 4 * (m + 1) + 4 * n + 4 * m
 
 
-def cpu_baseline(geo, b, opts, budget_s):
-    import numpy as np  # noqa: F401
+def traffic_from_profiles(workload, kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic_<workload>.json
+    (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_time(fn, budget_s, max_runs=20):
     import tv3_oracle as O
-    threads = len(os.sched_getaffinity(0))
-    O.set_threads(threads)
-    op = O.Op.geometry(geo)
-    dims = (geo.rv[0], geo.rv[1], 1)
+    O.set_threads(len(os.sched_getaffinity(0)))
     runs, t0 = 0, time.perf_counter()
     while True:
-        O.solve(op, b, dims, opts)
+        fn()
         runs += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or runs >= 20:
-            break
-    return {"value": runs / el, "unit": "recons/s", "cores": O.get_threads(),
-            "kind": "port",
-            "sample": f"{runs} full 200-iteration reconstructions of the 256x256 "
-                      f"ring workload (C++/OpenMP oracle, {el:.1f} s)"}
+        if el >= budget_s or runs >= max_runs:
+            return runs, el, O.get_threads()
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=128)
-    ap.add_argument("--plain-runs", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--maxit", type=int, default=200)
-    ap.add_argument("--copy-mb", type=int, default=0,
-                    help="also run a device-to-device copy of this many MiB "
-                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the "
-              "multi-GPU sharding of this path is design-only.", file=sys.stderr)
-        sys.exit(2)
+def roofline(prof, bytes_per_launch, kernels, workload):
+    ops = {}
+    for k, (ms, cnt) in prof.items():
+        avg = ms / max(cnt, 1)
+        ops[k] = {"launches": cnt, "ms_per_launch": avg,
+                  "GBps": bytes_per_launch / (avg * 1e-3) / 1e9 if cnt else None}
+    dom = max(ops, key=lambda k: prof[k][0])
+    ach = ops[dom]["GBps"]
+    kern = kernels[dom]
+    return ops, {"bound": "hbm", "kernel": kern, "achieved": ach,
+                 "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                 "frac": ach / HBM_PEAK_GBPS if ach else None,
+                 "traffic": traffic_from_profiles(workload, kern),
+                 "bytes_per_launch": bytes_per_launch}
 
-    import numpy as np
+
+# ---------------------------------------------------------------------------
+def run_ring(args, ctx):
     import torch
     import tval3_amd
     from tval3_amd import synth
-
-    ctx = tval3_amd.Context(0)
     G = args.grid
     geo = synth.ring_geometry(G, G)
     t0 = time.perf_counter()
@@ -98,7 +112,7 @@ def main():
     opts = tval3_amd.default_options(maxit=args.maxit)
     torch.cuda.synchronize()
 
-    # ---- plain FP32 path: one reconstruction at a time (W = 1 kernels)
+    # plain FP32 path: one reconstruction at a time (W = 1 kernels)
     U0, info0, _ = ctx.solve(plan, bs[0], dims, opts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -108,9 +122,8 @@ def main():
     t_plain = (time.perf_counter() - t0) / args.plain_runs
     log(f"plain: {1e3 * t_plain:.1f} ms/recon, iters={info0.total_iters}")
 
-    # ---- batched path: B reconstructions per step (64-lane kernels)
-    plans = [plan] * B
-    dl = [dims] * B
+    # batched path: B reconstructions per step (64-lane kernels)
+    plans, dl = [plan] * B, [dims] * B
     for _ in range(args.warmup):
         Us, infos = ctx.solve_batch(plans, bs, dl, opts)
     torch.cuda.synchronize()
@@ -126,39 +139,22 @@ def main():
     value = B * args.steps / t_steps
     rel_b0 = float(torch.linalg.norm(Us[0] - U0) / torch.linalg.norm(U0))
     iters = [inf.total_iters for inf in infos]
-
-    # ---- roofline of the projector (A and A^T, CSR-equivalent bytes)
     Bpad = ((B + 63) // 64) * 64
-    per_inst = csr_equiv_bytes(plan.nnz, plan.m, plan.n)
-    ops = {}
-    for k, (ms, cnt) in prof.items():
-        avg = ms / max(cnt, 1)
-        ops[k] = {"launches": cnt, "ms_per_launch": avg,
-                  "GBps": Bpad * per_inst / (avg * 1e-3) / 1e9 if cnt else None}
-    dom = max(ops, key=lambda k: prof[k][0])
-    ach = ops[dom]["GBps"]
-    roofline = {"bound": "hbm", "kernel": "k_fwd_w64" if dom == "A" else "k_bwd_w64",
-                "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBPS if ach else None, "traffic": None,
-                "bytes_per_launch": Bpad * per_inst}
-
-    if args.copy_mb:
-        src = torch.ones(args.copy_mb * 262144, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-
+    ops, roof = roofline(prof, Bpad * csr_equiv_bytes(plan.nnz, plan.m, plan.n),
+                         {"A": "k_fwd_w64", "At": "k_bwd_w64"}, "ring")
+    roof["cache_resident"] = True
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(geo, bs_host[0], opts, args.cpu_seconds)
-        log(f"cpu: {cpu['value']:.3f} recons/s on {cpu['cores']} threads")
-
+        import tv3_oracle as O
+        op = O.Op.geometry(geo)
+        runs, el, thr = cpu_time(lambda: O.solve(op, bs_host[0], dims, opts), args.cpu_seconds)
+        cpu = {"value": runs / el, "unit": "recons/s", "cores": thr, "kind": "port",
+               "sample": f"{runs} full {args.maxit}-iteration reconstructions of the "
+                         f"{G}x{G} ring workload (C++/OpenMP oracle, {el:.1f} s)"}
+        log(f"cpu: {cpu['value']:.3f} recons/s on {thr} threads")
     out = {
-        "metric": METRIC, "value": value, "unit": "recons/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_steps / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "value": value, "unit": "recons/s", "steps": args.steps,
+        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "f32",
         "data": "synthetic: seeded disk phantom with inclusions, 0.5% noise",
         "config": {"workload": f"2D USCT ring {G}x{G}, {plan.m} rays, "
                                f"on-the-fly Bresenham, maxit {args.maxit}",
@@ -168,13 +164,216 @@ def main():
         "batched_over_plain": value * t_plain,
         "batch_vs_plain_rel_l2": rel_b0,
         "iters_min_max": [min(iters), max(iters)],
-        "operators": ops,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "operators": ops, "roofline": roof, "cpu_baseline": cpu,
     }
+    return out
+
+
+def run_dense_batch(args, ctx):
+    import numpy as np
+    import torch
+    import tval3_amd
+    from tval3_amd import synth
+    B = args.batch if args.batch_set else 64
+    sizes = synth.batched_dense_sizes(B, seed=0)
+    gen = torch.Generator(device="cuda")
+    plans, bs, dims, keep = [], [], [], []
+    tot_elems = 0
+    for k, (g, m) in enumerate(sizes):
+        n = g * g
+        gen.manual_seed(1000 + k)
+        Ab = torch.empty((m, n), dtype=torch.bfloat16, device="cuda")
+        step = max(1, (1 << 28) // n)
+        for r0 in range(0, m, step):
+            r1 = min(m, r0 + step)
+            Ab[r0:r1] = (torch.randn((r1 - r0, n), generator=gen, device="cuda")
+                         * (1.0 / np.sqrt(m))).to(torch.bfloat16)
+        x = torch.from_numpy(synth.ellipse_phantom(g, g, 8, seed=k)).cuda()
+        y = (Ab.float() @ x) if m * n < (1 << 31) else torch.cat(
+            [Ab[r0:r0 + 4096].float() @ x for r0 in range(0, m, 4096)])
+        b = torch.from_numpy(synth.add_noise(y.cpu().numpy(), 0.01, seed=k)).cuda()
+        plans.append(ctx.plan_dense(Ab, bf16=True))
+        bs.append(b); dims.append((g, g, 1)); keep.append((Ab, x))
+        tot_elems += m * n
+    torch.cuda.synchronize()
+    log(f"dense-batch: {B} instances, {tot_elems / 1e9:.2f} G bf16 entries "
+        f"({2 * tot_elems / 2**30:.1f} GiB)")
+    opts = tval3_amd.default_options(maxit=args.maxit)
+    for _ in range(args.warmup):
+        ctx.solve_batch(plans, bs, dims, opts)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        Us, infos = ctx.solve_batch(plans, bs, dims, opts)
+        log(f"step {k + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    value = B * args.steps / t_steps
+    bytes_launch = sum(2 * m * g * g + 4 * g * g + 4 * m for g, m in sizes)
+    ops, roof = roofline(prof, bytes_launch,
+                         {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}, "dense-batch")
+    # error against the FP64 solve of the same bf16-rounded matrices, on the
+    # smallest instances (CPU oracle, double/double)
+    import tv3_oracle as O
+    order = sorted(range(B), key=lambda k: sizes[k][0] ** 2 * sizes[k][1])[:args.f64_sample]
+    errs = {}
+    for k in order:
+        Ab, x = keep[k]
+        A_r = Ab.float().cpu().numpy()
+        op = O.Op.dense(A_r)
+        U64, _, _ = O.solve_f64(op, bs[k].cpu().numpy(), dims[k], opts)
+        Ug = Us[k].cpu().numpy().astype(np.float64)
+        errs[k] = float(np.linalg.norm(Ug - U64) / np.linalg.norm(U64))
+    log(f"rel L2 vs FP64 on {len(errs)} instances: max {max(errs.values()):.2e}")
+    cpu = None
+    if not args.no_cpu:
+        k = order[0]
+        Ab, x = keep[k]
+        op = O.Op.dense(Ab.float().cpu().numpy())
+        bk = bs[k].cpu().numpy()
+        runs, el, thr = cpu_time(lambda: O.solve(op, bk, dims[k], opts), args.cpu_seconds)
+        # scale by the batch's entry count (the solve is A-bound on the CPU)
+        m0, g0 = sizes[k][1], sizes[k][0]
+        frac = (m0 * g0 * g0) / tot_elems * B
+        cpu = {"value": runs / el * frac, "unit": "recons/s (batch-average size)",
+               "cores": thr, "kind": "port",
+               "sample": f"{runs} reconstructions of instance {k} ({g0}^2, m={m0}) "
+                         f"in {el:.1f} s, scaled by entry count to the batch's "
+                         f"average instance"}
+    return {
+        "value": value, "unit": "recons/s", "steps": args.steps,
+        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "bf16 A, f32 vectors",
+        "data": "synthetic: A ~ N(0,1/m) rounded to bf16, 8-ellipse phantoms, 1% noise",
+        "config": {"workload": f"{B} uneven dense 2D reconstructions (bf16 A), maxit {args.maxit}",
+                   "batch": B, "bf16_entries": tot_elems, "maxit": args.maxit},
+        "rel_l2_vs_fp64": errs, "iters": [i.total_iters for i in infos],
+        "operators": ops, "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+def run_single(args, ctx, workload):
+    import numpy as np
+    import torch
+    import tval3_amd
+    from tval3_amd import synth
+    import tv3_oracle as O
+    if workload == "hemisphere":
+        grid = args.grid if args.grid_set else 128
+        geo = synth.hemisphere_geometry(1024, grid)
+        plan = ctx.plan_geometry(geo)
+        x = synth.ellipsoid_phantom(grid, 20, seed=0)
+        y = ctx.gemv(plan, x).cpu().numpy()
+        b = synth.add_noise(y, 0.005, seed=0)
+        dims = (grid, grid, grid)
+        maxit = args.maxit if args.maxit_set else 100
+        opts = tval3_amd.default_options(maxit=maxit)
+        bytes_launch = csr_equiv_bytes(plan.nnz, plan.m, plan.n)
+        kern = {"A": "k_fwd_w1", "At": "k_bwd_w1"}
+        op_fn = lambda: O.Op.geometry(geo)  # noqa: E731
+        desc = f"3D USCT hemisphere {grid}^3, {plan.m} rays, maxit {maxit}"
+    else:
+        q = p = 64
+        m = 1229
+        A = synth.gaussian_A(m, q * p, 0)
+        x = synth.ellipse_phantom(p, q, 8, 0)
+        b = synth.add_noise(A @ x, 0.01, 0)
+        plan = ctx.plan_dense(A)
+        dims = (q, p, 1)
+        maxit = args.maxit if args.maxit_set else 300
+        opts = tval3_amd.default_options(maxit=maxit, tv_iso=1)
+        bytes_launch = 4 * m * q * p + 4 * q * p + 4 * m
+        kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}
+        op_fn = lambda: O.Op.dense(A)  # noqa: E731
+        desc = f"2D dense 64x64, m={m}, isotropic TV, maxit {maxit}"
+    bt = ctx.tensor(b)
+    for _ in range(args.warmup):
+        ctx.solve(plan, bt, dims, opts)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        U, info, _ = ctx.solve(plan, bt, dims, opts, Ut=x)
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ops, roof = roofline(prof, bytes_launch, kern, workload)
+    cpu = None
+    if not args.no_cpu:
+        op = op_fn()
+        runs, el, thr = cpu_time(lambda: O.solve(op, b, dims, opts), args.cpu_seconds, 5)
+        cpu = {"value": runs / el, "unit": "recons/s", "cores": thr, "kind": "port",
+               "sample": f"{runs} full reconstructions ({el:.1f} s, C++/OpenMP oracle)"}
+    return {
+        "value": args.steps / t_steps, "unit": "recons/s", "steps": args.steps,
+        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "f32",
+        "data": "synthetic (seeded phantom, noise)",
+        "config": {"workload": desc, "batch": 1, "maxit": maxit, "nnz": plan.nnz},
+        "iters": info.total_iters, "rel_error_vs_truth": info.rel_error,
+        "operators": ops, "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ring",
+                    choices=["ring", "dense-batch", "hemisphere", "dense"])
+    ap.add_argument("--batch", type=int, default=None)
+    ap.add_argument("--plain-runs", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid", type=int, default=None)
+    ap.add_argument("--maxit", type=int, default=None)
+    ap.add_argument("--f64-sample", type=int, default=4)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the "
+              "multi-GPU sharding of this path is design-only.", file=sys.stderr)
+        sys.exit(2)
+    args.batch_set = args.batch is not None
+    args.grid_set = args.grid is not None
+    args.maxit_set = args.maxit is not None
+    if args.batch is None:
+        args.batch = 128
+    if args.grid is None:
+        args.grid = 256
+    if args.maxit is None:
+        args.maxit = 200 if args.workload != "dense-batch" else 300
+
+    import torch
+    import tval3_amd
+    ctx = tval3_amd.Context(0)
+    if args.workload == "ring":
+        out = run_ring(args, ctx)
+    elif args.workload == "dense-batch":
+        out = run_dense_batch(args, ctx)
+    else:
+        out = run_single(args, ctx, args.workload)
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+    line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"),
+            "n_gpus": 1, "steps": out.pop("steps"), "warmup": args.warmup,
+            "ms_per_step": out.pop("ms_per_step"), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": out.pop("dtype"),
+            "data": out.pop("data"), "config": out.pop("config")}
+    line.update(out)
+    cpu = line.get("cpu_baseline")
     if cpu:
-        out["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(out))
+        line["gpu_over_cpu"] = line["value"] / cpu["value"]
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -50,6 +50,11 @@ def lib():
         L.tv3o_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                  C.c_int, _FP, _FP, _FP, C.c_void_p,
                                  C.c_void_p, C.c_int]
+        _DP = C.POINTER(C.c_double)
+        L.tv3o_solve_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                     C.c_int, _DP, _DP, _DP, C.c_void_p,
+                                     C.c_void_p, C.c_int]
+        L.tv3o_gemv_f64.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
         L.tv3o_set_threads.argtypes = [C.c_int]
         L.tv3o_get_threads.restype = C.c_int
         _lib = L
@@ -123,6 +128,14 @@ class Op:
             raise ValueError("not a geometry operator")
         return ptr, lin
 
+    def gemv_f64(self, x, trans=False):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n if trans else self.m, np.float64)
+        dp = C.POINTER(C.c_double)
+        lib().tv3o_gemv_f64(self.h, int(bool(trans)), x.ctypes.data_as(dp),
+                            y.ctypes.data_as(dp))
+        return y
+
     def gemv(self, x, trans=False):
         x = _c32(x)
         y = np.zeros(self.n if trans else self.m, np.float32)
@@ -177,6 +190,27 @@ def solve(op, b, dims, opts, Ut=None, trace_cap=0):
     Utc = _c32(Ut) if Ut is not None else None
     rc = lib().tv3o_solve(op.h, C.byref(opts), q, p, r, _f(b), _f(U), _f(Utc),
                           C.byref(info), trace, int(trace_cap))
+    if rc:
+        raise ValueError("oracle solve: dimension mismatch")
+    n_tr = min(info.total_iters, trace_cap)
+    return U, info, ([trace[i] for i in range(n_tr)] if trace else [])
+
+
+def solve_f64(op, b, dims, opts, Ut=None, trace_cap=0):
+    """The double/double instantiation of the restated solver (FP64 vectors
+    and scalars): the FP64 reference for reduced-precision results."""
+    from tval3_amd.abi import Info, TraceRec
+    q, p, r = dims
+    dp = C.POINTER(C.c_double)
+    b = np.ascontiguousarray(b, np.float64)
+    U = np.zeros(q * p * r, np.float64)
+    info = Info()
+    trace = (TraceRec * trace_cap)() if trace_cap else None
+    Utc = np.ascontiguousarray(Ut, np.float64) if Ut is not None else None
+    rc = lib().tv3o_solve_f64(op.h, C.byref(opts), q, p, r, b.ctypes.data_as(dp),
+                              U.ctypes.data_as(dp),
+                              Utc.ctypes.data_as(dp) if Utc is not None else None,
+                              C.byref(info), trace, int(trace_cap))
     if rc:
         raise ValueError("oracle solve: dimension mismatch")
     n_tr = min(info.total_iters, trace_cap)

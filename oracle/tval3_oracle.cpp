@@ -79,30 +79,37 @@ double chunked_sum(i64 n, F term) {
   return s;
 }
 
-double ddot(i64 n, const float *a, const float *b) {
+template <class T>
+double ddot(i64 n, const T *a, const T *b) {
   return chunked_sum(n, [&](i64 i) { return (double)a[i] * (double)b[i]; });
 }
-float sdot(i64 n, const float *a, const float *b) { return (float)ddot(n, a, b); }
-float snrm2(i64 n, const float *a) { return (float)std::sqrt(ddot(n, a, a)); }
+/* cblas_?dot / cblas_?nrm2: result in the vector type */
+template <class T>
+T sdot(i64 n, const T *a, const T *b) { return (T)ddot(n, a, b); }
+template <class T>
+T snrm2(i64 n, const T *a) { return (T)std::sqrt(ddot(n, a, a)); }
 
 /* z = x + alpha*y with one rounding, computed in precision TS */
-template <class TS>
-void vec_add(i64 n, const float *x, TS alpha, const float *y, float *z) {
+template <class T, class TS>
+void vec_add(i64 n, const T *x, TS alpha, const T *y, T *z) {
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i)
-    z[i] = (float)std::fma(alpha, (TS)y[i], (TS)x[i]);
+    z[i] = (T)std::fma(alpha, (TS)y[i], (TS)x[i]);
 }
 /* z = x - y (vec_add with alpha = -1: exact negation, one rounding) */
-void vec_sub(i64 n, const float *x, const float *y, float *z) {
+template <class T>
+void vec_sub(i64 n, const T *x, const T *y, T *z) {
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i) z[i] = x[i] - y[i];
 }
-/* y += alpha*x, float alpha (cblas_saxpy) */
-void axpy(i64 n, float alpha, const float *x, float *y) {
+/* y += alpha*x, alpha in the vector type (cblas_saxpy / cblas_daxpy) */
+template <class T>
+void axpy(i64 n, T alpha, const T *x, T *y) {
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i) y[i] = std::fma(alpha, x[i], y[i]);
 }
-void scal(i64 n, float alpha, float *x) {
+template <class T>
+void scal(i64 n, T alpha, T *x) {
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i) x[i] = alpha * x[i];
 }
@@ -216,8 +223,11 @@ void build_geometry(Op &op, const tv3_geometry &g) {
     }
 }
 
-/* y = A x or A^T x, overwriting y (gemv with beta = 0). */
-void gemv(const Op &op, bool trans, const float *x, float *y) {
+/* y = A x or A^T x, overwriting y (gemv with beta = 0).  T is the vector
+ * type; geometry products are formed in it (mult_elementN/T, 3d/mat_mult.h:
+ * 121-139), all sums in FP64. */
+template <class T>
+void gemv(const Op &op, bool trans, const T *x, T *y) {
   const i64 m = op.m, n = op.n;
   if (op.kind == 0) {
     const float *A = op.A;
@@ -227,7 +237,7 @@ void gemv(const Op &op, bool trans, const float *x, float *y) {
         double s = 0.0;
         const float *row = A + i * n;
         for (i64 j = 0; j < n; ++j) s += (double)row[j] * (double)x[j];
-        y[i] = (float)s;
+        y[i] = (T)s;
       }
     } else {
       const i64 cb = 256;
@@ -241,7 +251,7 @@ void gemv(const Op &op, bool trans, const float *x, float *y) {
           const float *row = A + i * n;
           for (i64 j = j0; j < j1; ++j) acc[j - j0] += (double)row[j] * xi;
         }
-        for (i64 j = j0; j < j1; ++j) y[j] = (float)acc[j - j0];
+        for (i64 j = j0; j < j1; ++j) y[j] = (T)acc[j - j0];
       }
     }
   } else if (op.kind == 1) {
@@ -254,10 +264,10 @@ void gemv(const Op &op, bool trans, const float *x, float *y) {
       double s = 0.0;
       for (int k = ptr[r]; k < ptr[r + 1]; ++k)
         s += (double)val[k] * (double)x[ind[k]];
-      y[r] = (float)s;
+      y[r] = (T)s;
     }
   } else {
-    const float sc = op.scale;
+    const T sc = (T)op.scale;
     if (!trans) {
 #pragma omp parallel for schedule(dynamic, 64)
       for (i64 r = 0; r < m; ++r) {
@@ -267,7 +277,7 @@ void gemv(const Op &op, bool trans, const float *x, float *y) {
           for (i64 k = op.walk_ptr[w]; k < op.walk_ptr[w + 1]; ++k)
             s += (double)(x[op.walk_lin[k]] * sc);
         }
-        y[r] = (float)s;
+        y[r] = (T)s;
       }
     } else {
 #pragma omp parallel for schedule(dynamic, 256)
@@ -275,7 +285,7 @@ void gemv(const Op &op, bool trans, const float *x, float *y) {
         double s = 0.0;
         for (i64 k = op.col_ptr[v]; k < op.col_ptr[v + 1]; ++k)
           s += (double)(x[op.col_ray[k]] * sc);
-        y[v] = (float)s;
+        y[v] = (T)s;
       }
     }
   }
@@ -290,7 +300,8 @@ struct Dims {
 };
 
 /* D: 2d/tval3_cpu_2d.cpp:268-278, 3d/tval3_cpu_3d.cpp:365-389 */
-void Dop(const Dims &d, const float *U, float *Ux, float *Uy, float *Uz) {
+template <class T>
+void Dop(const Dims &d, const T *U, T *Ux, T *Uy, T *Uz) {
   const i64 q = d.q, p = d.p, r = d.r, s = q * p, n = d.n();
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i) {
@@ -308,13 +319,14 @@ void Dop(const Dims &d, const float *U, float *Ux, float *Uy, float *Uz) {
  * wrap pass then also applies to the first row of every slice: the first row
  * of slice z >= 1 receives an extra +Y[i - dim_x] - Y[i].  exact = 1 omits
  * that extra term. */
-void Dtop(const Dims &d, bool exact, const float *X, const float *Y,
-          const float *Z, float *res) {
+template <class T>
+void Dtop(const Dims &d, bool exact, const T *X, const T *Y, const T *Z,
+          T *res) {
   const i64 q = d.q, p = d.p, r = d.r, s = q * p, n = d.n();
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i) {
     const i64 x = i % q, y = (i / q) % p;
-    float t = (x > 0 ? X[i - 1] : X[i + q - 1]) - X[i];
+    T t = (x > 0 ? X[i - 1] : X[i + q - 1]) - X[i];
     if (r == 1) {
       if (y > 0) {
         t = t + Y[i - q];
@@ -347,29 +359,28 @@ void Dtop(const Dims &d, bool exact, const float *X, const float *Y,
 /* shrinkage: 2d/tval3_cpu_2d.cpp:588-611 (anisotropic, the reference's only
  * real variant); iso = 1 is the isotropic group soft-threshold of the TVAL3
  * method, which the reference does not implement.  Returns sum in FP64. */
-template <class TS>
-double shrink(const Dims &d, TS beta, bool iso, const float *Ux,
-              const float *Uy, const float *Uz, const float *sx,
-              const float *sy, const float *sz, float *Wx, float *Wy,
-              float *Wz) {
+template <class T, class TS>
+double shrink(const Dims &d, TS beta, bool iso, const T *Ux, const T *Uy,
+              const T *Uz, const T *sx, const T *sy, const T *sz, T *Wx,
+              T *Wy, T *Wz) {
   const i64 n = d.n();
   const bool z3 = d.r > 1;
   const TS ib = (TS)1 / beta;
   return chunked_sum(n, [&](i64 i) -> double {
-    const float ubx = (float)((TS)Ux[i] - (TS)sx[i] / beta);
-    const float uby = (float)((TS)Uy[i] - (TS)sy[i] / beta);
-    const float ubz = z3 ? (float)((TS)Uz[i] - (TS)sz[i] / beta) : 0.0f;
+    const T ubx = (T)((TS)Ux[i] - (TS)sx[i] / beta);
+    const T uby = (T)((TS)Uy[i] - (TS)sy[i] / beta);
+    const T ubz = z3 ? (T)((TS)Uz[i] - (TS)sz[i] / beta) : (T)0;
     if (!iso) {
       TS wx = std::abs((TS)ubx) - ib, wy = std::abs((TS)uby) - ib;
       wx = (wx >= 0) ? wx : (TS)0;
       wy = (wy >= 0) ? wy : (TS)0;
-      Wx[i] = (float)((ubx >= 0) ? wx : -wx);
-      Wy[i] = (float)((uby >= 0) ? wy : -wy);
+      Wx[i] = (T)((ubx >= 0) ? wx : -wx);
+      Wy[i] = (T)((uby >= 0) ? wy : -wy);
       TS t = wx + wy;
       if (z3) {
         TS wz = std::abs((TS)ubz) - ib;
         wz = (wz >= 0) ? wz : (TS)0;
-        Wz[i] = (float)((ubz >= 0) ? wz : -wz);
+        Wz[i] = (T)((ubz >= 0) ? wz : -wz);
         t = t + wz;
       }
       return (double)t;
@@ -379,9 +390,9 @@ double shrink(const Dims &d, TS beta, bool iso, const float *Ux,
     const TS rr = std::sqrt(ss);
     TS w = rr - ib;
     w = (w >= 0) ? w : (TS)0;
-    Wx[i] = (rr > 0) ? (float)(w * (TS)ubx / rr) : 0.0f;
-    Wy[i] = (rr > 0) ? (float)(w * (TS)uby / rr) : 0.0f;
-    if (z3) Wz[i] = (rr > 0) ? (float)(w * (TS)ubz / rr) : 0.0f;
+    Wx[i] = (rr > 0) ? (T)(w * (TS)ubx / rr) : (T)0;
+    Wy[i] = (rr > 0) ? (T)(w * (TS)uby / rr) : (T)0;
+    if (z3) Wz[i] = (rr > 0) ? (T)(w * (TS)ubz / rr) : (T)0;
     return (double)w;
   });
 }
@@ -389,9 +400,9 @@ double shrink(const Dims &d, TS beta, bool iso, const float *Ux,
 /* ---------------------------------------------------------------------- */
 /* the solver                                                              */
 /* ---------------------------------------------------------------------- */
-template <class TS>
+template <class TC, class TS>
 struct Solver {
-  typedef std::vector<float> V;
+  typedef std::vector<TC> V;
   const Op &A;
   const tv3_options &o;
   Dims dm;
@@ -402,9 +413,9 @@ struct Solver {
   // penalty parameters, multipliers, Lagrangian terms (tval3_data)
   TS mu, beta, muDbeta;
   V delta, sx, sy, sz;
-  float f;
+  TC f;
   TS lam1, lam2, lam3;
-  float lam4, lam5;
+  TC lam4, lam5;
   V Up, dU, uup, Ux, Uxp, dUx, Uy, Uyp, dUy, Uz, Uzp, dUz, Wx, Wy, Wz, Atb,
       Au, Aup, dAu, d, g, gp, dg, g2, g2p, dg2;
   unsigned currInitIteration;
@@ -422,17 +433,17 @@ struct Solver {
     mu = (TS)o.mu0; beta = (TS)o.beta0; muDbeta = 0;
     f = 0; lam1 = lam2 = lam3 = 0; lam4 = lam5 = 0;
     const i64 nz = z3 ? n : 0;
-    delta.assign(m, 0.f); last_Au.assign(m, 0.f);
-    Au.assign(m, 0.f); Aup.assign(m, 0.f); dAu.assign(m, 0.f);
+    delta.assign(m, (TC)0); last_Au.assign(m, (TC)0);
+    Au.assign(m, (TC)0); Aup.assign(m, (TC)0); dAu.assign(m, (TC)0);
     for (V *v : {&sx, &sy, &Up, &dU, &uup, &Ux, &Uxp, &dUx, &Uy, &Uyp, &dUy,
                  &Wx, &Wy, &Atb, &d, &g, &gp, &dg, &g2, &g2p, &dg2})
-      v->assign(n, 0.f);
-    for (V *v : {&sz, &Uz, &Uzp, &dUz, &Wz}) v->assign(nz, 0.f);
+      v->assign(n, (TC)0);
+    for (V *v : {&sz, &Uz, &Uzp, &dUz, &Wz}) v->assign(nz, (TC)0);
     currInitIteration = 0; currSkipValue = 0;
     bt_cnt = bt_fallback = 0; last_tau = last_alpha = 0;
   }
 
-  float *P(V &v) { return v.empty() ? nullptr : v.data(); }
+  TC *P(V &v) { return v.empty() ? nullptr : v.data(); }
 
   void D_(const V &U, V &X, V &Y, V &Z) { Dop(dm, U.data(), P(X), P(Y), P(Z)); }
   void Dt_(V &res, const V &X, const V &Y, const V &Z) {
@@ -445,10 +456,10 @@ struct Solver {
     TS scl = 1;
     if (m > 0) {
       const TS th1 = (TS)0.5, th2 = (TS)1.5;
-      float bmin = b[0], bmax = bmin;
+      TC bmin = b[0], bmax = bmin;
       TS bmin_abs = std::abs(bmin), bmax_abs = bmin_abs;
       for (i64 i = 0; i < m; ++i) {
-        const float val = b[i];
+        const TC val = b[i];
         const TS va = std::abs(val);
         if (va < bmin_abs) { bmin = val; bmin_abs = va; }
         if (va > bmax_abs) { bmax = val; bmax_abs = va; }
@@ -456,10 +467,10 @@ struct Solver {
       const TS b_dif = std::abs(bmax - bmin);
       if (b_dif < th1) {
         scl = th1 / b_dif;
-        scal(m, (float)scl, b.data());
+        scal(m, (TC)scl, b.data());
       } else if (b_dif > th2) {
         scl = th2 / b_dif;
-        scal(m, (float)scl, b.data());
+        scal(m, (TC)scl, b.data());
       }
     }
     return scl;
@@ -483,8 +494,8 @@ struct Solver {
     lam3 = (TS)sdot(m, Aub.data(), Aub.data());
     lam5 = sdot(m, delta.data(), Aub.data());
   }
-  float f_full() {
-    return (float)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)lam4) -
+  TC f_full() {
+    return (TC)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)lam4) -
                    (TS)lam5);
   }
 
@@ -497,7 +508,7 @@ struct Solver {
       return;
     }
     V deltaAu(Au);
-    axpy(m, -1.0f, last_Au.data(), deltaAu.data());
+    axpy(m, (TC)-1, last_Au.data(), deltaAu.data());
     const TS nrm_deltaAu = snrm2(m, deltaAu.data());
     const TS nrm_Au = snrm2(m, Au.data());
     const TS relChange = nrm_deltaAu / nrm_Au;
@@ -520,7 +531,7 @@ struct Solver {
   /* get_g: 2d/tval3_cpu_2d.cpp:396-423 */
   void get_g(const V &U, const V &b, bool force_mul) {
     get_g_mult(U, force_mul);
-    axpy(n, -1.0f, Atb.data(), g.data());
+    axpy(n, (TC)-1, Atb.data(), g.data());
     V Vx(n), Vy(n), Vz(z3 ? n : 0);
     vec_sub(n, Ux.data(), Wx.data(), Vx.data());
     vec_sub(n, Uy.data(), Wy.data(), Vy.data());
@@ -537,20 +548,20 @@ struct Solver {
     if (fst_iter) {
       V dx(n), dy(n), dz(z3 ? n : 0);
       D_(d, dx, dy, dz);
-      float dDd = sdot(n, dx.data(), dx.data()) + sdot(n, dy.data(), dy.data());
+      TC dDd = sdot(n, dx.data(), dx.data()) + sdot(n, dy.data(), dy.data());
       if (z3) dDd = dDd + sdot(n, dz.data(), dz.data());
-      const float dd = sdot(n, d.data(), d.data());
+      const TC dd = sdot(n, d.data(), d.data());
       V Ad(m);
       gemv(A, false, d.data(), Ad.data());
-      const float Add = sdot(m, Ad.data(), Ad.data());
+      const TC Add = sdot(m, Ad.data(), Ad.data());
       tau = std::abs((TS)dd / ((TS)dDd + muDbeta * (TS)Add));
     } else {
       vec_sub(n, g.data(), gp.data(), dg.data());
       vec_sub(n, g2.data(), g2p.data(), dg2.data());
-      const float ss = sdot(n, uup.data(), uup.data());
+      const TC ss = sdot(n, uup.data(), uup.data());
       V tmp(dg2);
-      axpy(n, (float)muDbeta, dg.data(), tmp.data());
-      const float sy_ = sdot(n, uup.data(), tmp.data());
+      axpy(n, (TC)muDbeta, dg.data(), tmp.data());
+      const TC sy_ = sdot(n, uup.data(), tmp.data());
       tau = (TS)std::abs(ss / sy_);
     }
     return tau;
@@ -563,7 +574,7 @@ struct Solver {
 
   /* descend: 2d/tval3_cpu_2d.cpp:481-493 */
   void descend(V &U, const V &b, TS tau, bool force_mul) {
-    axpy(n, (float)(-1 * tau), d.data(), U.data());
+    axpy(n, (TC)(-1 * tau), d.data(), U.data());
     if (o.nonneg) nonneg(U);
     D_(U, Ux, Uy, Uz);
     get_g(U, b, force_mul);
@@ -571,13 +582,13 @@ struct Solver {
 
   /* update_g: 2d/tval3_cpu_2d.cpp:495-524 */
   void update_g(TS alpha, V &U, const V &b) {
-    vec_add<TS>(n, gp.data(), alpha, dg.data(), g.data());
-    vec_add<TS>(n, g2p.data(), alpha, dg2.data(), g2.data());
-    vec_add<TS>(n, Up.data(), alpha, dU.data(), U.data());
-    vec_add<TS>(m, Aup.data(), alpha, dAu.data(), Au.data());
-    vec_add<TS>(n, Uxp.data(), alpha, dUx.data(), Ux.data());
-    vec_add<TS>(n, Uyp.data(), alpha, dUy.data(), Uy.data());
-    if (z3) vec_add<TS>(n, Uzp.data(), alpha, dUz.data(), Uz.data());
+    vec_add<TC, TS>(n, gp.data(), alpha, dg.data(), g.data());
+    vec_add<TC, TS>(n, g2p.data(), alpha, dg2.data(), g2.data());
+    vec_add<TC, TS>(n, Up.data(), alpha, dU.data(), U.data());
+    vec_add<TC, TS>(m, Aup.data(), alpha, dAu.data(), Au.data());
+    vec_add<TC, TS>(n, Uxp.data(), alpha, dUx.data(), Ux.data());
+    vec_add<TC, TS>(n, Uyp.data(), alpha, dUy.data(), Uy.data());
+    if (z3) vec_add<TC, TS>(n, Uzp.data(), alpha, dUz.data(), Uz.data());
     V Vx(n), Vy(n), Vz(z3 ? n : 0);
     lam24(Vx, Vy, Vz);
     lam35(b);
@@ -585,7 +596,7 @@ struct Solver {
   }
 
   /* min_u: 2d/tval3_cpu_2d.cpp:526-576 */
-  void min_u(V &U, TS &gam, const V &b, float C, bool fst_iter) {
+  void min_u(V &U, TS &gam, const V &b, TC C, bool fst_iter) {
     TS tau = get_tau(fst_iter);
     Up = U; gp = g; g2p = g2; Aup = Au; Uxp = Ux; Uyp = Uy;
     if (z3) Uzp = Uz;
@@ -624,19 +635,19 @@ struct Solver {
   /* get_gradient: 2d/tval3_cpu_2d.cpp:578-585 */
   void get_gradient(const V &DtsAtd) {
     d = DtsAtd;
-    axpy(n, 1.0f, g2.data(), d.data());
-    axpy(n, (float)muDbeta, g.data(), d.data());
+    axpy(n, (TC)1, g2.data(), d.data());
+    axpy(n, (TC)muDbeta, g.data(), d.data());
   }
 
   void shrinkage() {
-    lam1 = (TS)shrink<TS>(dm, beta, o.tv_iso != 0, Ux.data(), Uy.data(),
+    lam1 = (TS)shrink<TC, TS>(dm, beta, o.tv_iso != 0, Ux.data(), Uy.data(),
                           P(Uz), sx.data(), sy.data(), P(sz), Wx.data(),
                           Wy.data(), P(Wz));
   }
 
   /* update_W: 2d/tval3_cpu_2d.cpp:651-671 */
   void update_W() {
-    f = (float)((TS)f - ((lam1 + beta / 2 * lam2) - (TS)lam4));
+    f = (TC)((TS)f - ((lam1 + beta / 2 * lam2) - (TS)lam4));
     shrinkage();
     V Vx(n), Vy(n), Vz(z3 ? n : 0);
     vec_sub(n, Ux.data(), Wx.data(), Vx.data());
@@ -644,7 +655,7 @@ struct Solver {
     if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
     Dt_(g2, Vx, Vy, Vz);
     lam24(Vx, Vy, Vz);
-    f = (float)((TS)f + ((lam1 + beta / 2 * lam2) - (TS)lam4));
+    f = (TC)((TS)f + ((lam1 + beta / 2 * lam2) - (TS)lam4));
   }
 
   /* update_mlp: 2d/tval3_cpu_2d.cpp:673-694 */
@@ -654,7 +665,7 @@ struct Solver {
     vec_sub(n, Ux.data(), Wx.data(), Vx.data());
     vec_sub(n, Uy.data(), Wy.data(), Vy.data());
     if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
-    const float nb = (float)(-1 * beta);
+    const TC nb = (TC)(-1 * beta);
     axpy(n, nb, Vx.data(), sx.data());
     axpy(n, nb, Vy.data(), sy.data());
     if (z3) axpy(n, nb, Vz.data(), sz.data());
@@ -663,18 +674,18 @@ struct Solver {
     if (z3) lam4 += sdot(n, sz.data(), Vz.data());
     V Aub(m);
     vec_sub(m, Au.data(), b.data(), Aub.data());
-    axpy(m, (float)(-1 * mu), Aub.data(), delta.data());
+    axpy(m, (TC)(-1 * mu), Aub.data(), delta.data());
     lam5 = sdot(m, delta.data(), Aub.data());
     f = f - (lam4 + lam5);
   }
 
   /* driver: 2d/tval3_cpu_2d.cpp:752-886 */
-  void run(const float *b_in, float *U_out, const float *Ut, tv3_info *info,
+  void run(const TC *b_in, TC *U_out, const TC *Ut, tv3_info *info,
            tv3_trace_rec *trace, int trace_cap) {
     V bs(b_in, b_in + m);
     const TS muf = (TS)o.mu, betaf = (TS)o.beta;
     TS beta0 = 0;
-    V DtsAtd(n, 0.f);
+    V DtsAtd(n, (TC)0);
     TS gam = o_gam;
     const TS scl = scaleb(bs);
     gemv(A, true, bs.data(), Atb.data());
@@ -688,7 +699,7 @@ struct Solver {
     get_g(U, bs, true);
     get_gradient(DtsAtd);
     TS Q = 1, Qp;
-    float C = f;
+    TC C = f;
     int count_outer = 0, count_total = 0;
     bool fst_iter;
     TS RelChg, RelChgOut = 0, nrmup = 0;
@@ -698,7 +709,7 @@ struct Solver {
         min_u(U, gam, bs, C, fst_iter);
         update_W();
         Qp = Q, Q = gam * Qp + 1;
-        C = (float)((gam * Qp * (TS)C + (TS)f) / Q);
+        C = (TC)((gam * Qp * (TS)C + (TS)f) / Q);
         vec_sub(n, U.data(), Up.data(), uup.data());
         get_gradient(DtsAtd);
         count_total++;
@@ -729,19 +740,19 @@ struct Solver {
       muDbeta = mu / beta;
       f = f_full();
       DtsAtd = d;
-      scal(n, (float)(beta0 / beta), DtsAtd.data());
+      scal(n, (TC)(beta0 / beta), DtsAtd.data());
       get_gradient(DtsAtd);
       gam = o_gam; Q = 1; C = f;
     }
-    for (i64 i = 0; i < n; ++i) U[i] = (float)((TS)U[i] / scl);
-    std::memcpy(U_out, U.data(), sizeof(float) * n);
+    for (i64 i = 0; i < n; ++i) U[i] = (TC)((TS)U[i] / scl);
+    std::memcpy(U_out, U.data(), sizeof(TC) * n);
     info->secs = 0;
     info->outer_iters = count_outer;
     info->total_iters = count_total;
     info->rel_chg = RelChgOut;
     if (Ut) {
       V dlt(Ut, Ut + n);
-      axpy(n, -1.0f, U.data(), dlt.data());
+      axpy(n, (TC)-1, U.data(), dlt.data());
       const TS a = snrm2(n, dlt.data()), bb = snrm2(n, Ut);
       info->rel_error = a / bb;
     } else {
@@ -843,10 +854,10 @@ double tv3o_shrink(int q, int p, int r, double beta, int iso, int scal_double,
                    float *Wx, float *Wy, float *Wz) {
   Dims dm{q, p, r};
   if (scal_double)
-    return (double)(double)shrink<double>(dm, beta, iso != 0, Ux, Uy, Uz, sx,
-                                          sy, sz, Wx, Wy, Wz);
-  return (double)(float)shrink<float>(dm, (float)beta, iso != 0, Ux, Uy, Uz,
-                                      sx, sy, sz, Wx, Wy, Wz);
+    return (double)(double)shrink<float, double>(dm, beta, iso != 0, Ux, Uy,
+                                                 Uz, sx, sy, sz, Wx, Wy, Wz);
+  return (double)(float)shrink<float, float>(dm, (float)beta, iso != 0, Ux,
+                                             Uy, Uz, sx, sy, sz, Wx, Wy, Wz);
 }
 
 double tv3o_dot(long long n, const float *x, const float *y) {
@@ -860,12 +871,31 @@ int tv3o_solve(const tv3o_op *h, const tv3_options *o, int q, int p, int r,
   const Dims dm{q, p, r};
   if (dm.n() != op.n) return 1;
   if (o->scal_double) {
-    Solver<double> s(op, *o, dm);
+    Solver<float, double> s(op, *o, dm);
     s.run(b, U, Ut, info, trace, trace_cap);
   } else {
-    Solver<float> s(op, *o, dm);
+    Solver<float, float> s(op, *o, dm);
     s.run(b, U, Ut, info, trace, trace_cap);
   }
+  return 0;
+}
+
+/* The reference's double/double instantiation (tval3_cpu_2d(mat<double>&,
+ * ...), 2d/tval3_cpu_2d.cpp:26-29): FP64 vectors and scalars, the FP64
+ * solve against which reduced-precision GPU results are measured. */
+int tv3o_solve_f64(const tv3o_op *h, const tv3_options *o, int q, int p,
+                   int r, const double *b, double *U, const double *Ut,
+                   tv3_info *info, tv3_trace_rec *trace, int trace_cap) {
+  const Op &op = *(const Op *)h;
+  const Dims dm{q, p, r};
+  if (dm.n() != op.n) return 1;
+  Solver<double, double> s(op, *o, dm);
+  s.run(b, U, Ut, info, trace, trace_cap);
+  return 0;
+}
+
+int tv3o_gemv_f64(const tv3o_op *h, int trans, const double *x, double *y) {
+  gemv(*(const Op *)h, trans != 0, x, y);
   return 0;
 }
 

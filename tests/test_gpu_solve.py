@@ -192,3 +192,31 @@ def test_argument_errors(ctx):
     geo.xe[0] = 99
     with pytest.raises(tval3_amd.Tval3Error, match="outside"):
         ctx.plan_geometry(geo)
+
+
+def test_batch_bf16_dense_uneven(ctx):
+    """Uneven-size batch with A stored in bf16 (the batched dense workload at
+    small sizes).  The kernels widen bf16 exactly, so the oracle run on the
+    bf16-rounded A (as float32) is the parity reference; the FP64 solve of the
+    same matrices gives the reported error."""
+    import torch
+    sizes = [(24, 24, 0.3), (32, 32, 0.25), (20, 28, 0.4), (40, 40, 0.2)]
+    plans, bs, dims, cases = [], [], [], []
+    for k, (q, p, fr) in enumerate(sizes):
+        A, x, b = _dense_case(q, p, fr, 40 + k)
+        Ab = torch.from_numpy(A).to(torch.bfloat16)
+        A_r = Ab.to(torch.float32).numpy()
+        b = synth.add_noise(A_r @ x, 0.01, 40 + k)
+        plans.append(ctx.plan_dense(Ab, bf16=True)); bs.append(b); dims.append((q, p, 1))
+        cases.append((A_r, x, b))
+    opts = tval3_amd.default_options(maxit=100, ls_exact=1)
+    Us, infos = ctx.solve_batch(plans, bs, dims, opts, Uts=[c[1] for c in cases])
+    for k, (A_r, x, b) in enumerate(cases):
+        op = O.Op.dense(A_r)
+        Uo, info_o, _ = O.solve(op, b, dims[k], opts, Ut=x)
+        Ug = Us[k].cpu().numpy()
+        assert _rel(Ug, Uo) <= TOL_U, k
+        assert infos[k].total_iters == info_o.total_iters
+        U64, _, _ = O.solve_f64(op, b, dims[k], opts, Ut=x)
+        print(f"instance {k}: rel L2 vs FP64 solve {_rel(Ug, U64):.2e}")
+        assert infos[k].rel_error == pytest.approx(info_o.rel_error, rel=1e-3)

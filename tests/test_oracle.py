@@ -337,3 +337,23 @@ def test_thread_count_does_not_change_results():
     finally:
         O.set_threads(t0)
     np.testing.assert_array_equal(U1, U2)
+
+
+def test_fp64_solve_close_to_fp32():
+    """The double/double solve (FP64 reference) and the float/double solve of
+    the same problem follow the same trajectory until a backtracking decision
+    first differs (float rounding at an Armijo near-tie); over the first 20
+    iterations they agree to float accuracy.  The FP64 gemv is the exact
+    product."""
+    A, x, b = _dense(20, 20, 0.5, 13)
+    op = O.Op.dense(A)
+    xd = np.random.default_rng(0).standard_normal(400)
+    np.testing.assert_allclose(op.gemv_f64(xd), A.astype(np.float64) @ xd, rtol=1e-12, atol=1e-13)
+    opts = default_options(maxit=20, scal_double=1)
+    U64, i64, t64 = O.solve_f64(op, b, (20, 20, 1), opts, Ut=x, trace_cap=20)
+    U32, i32, t32 = O.solve(op, b, (20, 20, 1), opts, Ut=x, trace_cap=20)
+    assert [t.backtracks for t in t64] == [t.backtracks for t in t32]
+    assert np.linalg.norm(U32 - U64) / np.linalg.norm(U64) < 1e-4
+    opts = default_options(maxit=300, scal_double=1)
+    U64, i64, _ = O.solve_f64(op, b, (20, 20, 1), opts, Ut=x)
+    assert i64.rel_error < 0.05

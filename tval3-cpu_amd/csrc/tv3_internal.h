@@ -48,6 +48,7 @@ struct PlanDev {
   // dense
   const void *A;
   int dtype;
+  int a_vec;  // A is 16-byte aligned and n % 8 == 0: rows take vector loads
   // sparse (CSR of A and CSR of A^T = CSC of A)
   const int *r_ptr, *r_ind;
   const float *r_val;
@@ -165,11 +166,15 @@ int hip_check(tv3_ctx *ctx, hipError_t e, const char *what);
 
 // operator launches (tv3_plan.hip).  x and y are batch base pointers;
 // flag_mask selects the lanes that compute (0 = all).
+// kind and dtype are those of every group of the launch (one operator kind
+// per batch; dense batches of mixed element type are split by the caller).
 template <int W>
 void launch_fwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
-                const float *x, float *y, const InstScal *S, uint32_t mask);
+                int kind, int dtype, const float *x, float *y,
+                const InstScal *S, uint32_t mask);
 template <int W>
 void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
-                int kind, const float *x, float *y, const InstScal *S,
-                uint32_t mask, const float *sub, float *scratch);
+                int kind, int dtype, const float *x, float *y,
+                const InstScal *S, uint32_t mask, const float *sub,
+                float *scratch);
 }  // namespace tv3

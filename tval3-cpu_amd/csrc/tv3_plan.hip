@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -495,17 +496,79 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w16(const GroupDesc *G, int nG,
   }
 }
 
-// Dense A^T with W = 1: column-parallel over row chunks (split-K).  Chunk c
-// of group g covers rows [c*R, min(m, (c+1)*R)); partial sums go to scratch
-// and are added in chunk order by k_dense_bwd_reduce.
+// ---------------------------------------------------------------------------
+// dense A with W = 1
+// ---------------------------------------------------------------------------
+// A entry -> float (bf16 widens exactly)
+__device__ __forceinline__ float a_val(const float *A, long long i) { return A[i]; }
+__device__ __forceinline__ float a_val(const __hip_bfloat16 *A, long long i) {
+  return __bfloat162float(A[i]);
+}
+// eight consecutive entries from a 16-byte (bf16) / 32-byte (f32) aligned
+// address as one or two vector loads
+__device__ __forceinline__ void load8(const float *p, float (&v)[8]) {
+  const float4 a = *(const float4 *)p, b = *(const float4 *)(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void load8(const __hip_bfloat16 *p, float (&v)[8]) {
+  const uint4 q = *(const uint4 *)p;
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = __uint_as_float(w[k] << 16);
+    v[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
+}
+
+// y = A x: one wavefront per row; with n a multiple of 8 every lane loads 8
+// consecutive entries (16 bytes of bf16, 32 of f32) and the matching 8 x
+// values per step; FP64 lane sums, fixed butterfly.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
+                                                         const float *x,
+                                                         float *y,
+                                                         const InstScal *S,
+                                                         uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= gd.m) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int n = gd.n;
+  const T *A = (const T *)gd.plan.A + (long long)row * n;
+  const float *xg = x + gd.off_n;
+  double acc = 0.0;
+  // rows start 16-byte aligned when n % 8 == 0; x starts 32-byte aligned
+  // (group offsets are multiples of 64 floats)
+  if (gd.plan.a_vec) {
+    for (int j = lane * 8; j < n; j += 512) {
+      float a[8], v[8];
+      load8(A + j, a);
+      load8(xg + j, v);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += (double)a[u] * (double)v[u];
+    }
+  } else {
+    for (int j = lane; j < n; j += 64) acc += (double)a_val(A, j) * (double)xg[j];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) y[gd.off_m + row] = (float)acc;
+}
+
+// A^T x, split-K: chunk c of a group covers rows [c*R, min(m, (c+1)*R)); a
+// thread owns 8 consecutive columns (one 16/32-byte load per row) and writes
+// FP64 partial sums; k_dense_bwd_reduce adds the chunks in order.
 constexpr int kDenseChunks = 16;
 
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
     const GroupDesc *G, const float *x, double *scratch, const InstScal *S,
     uint32_t mask, long long sstride) {
   const GroupDesc &gd = G[blockIdx.z];
-  const int col = blockIdx.x * kBlock + threadIdx.x;
-  if (col >= gd.n) return;
+  const int n = gd.n;
+  const int c0 = (blockIdx.x * kBlock + threadIdx.x) * 8;
+  if (c0 >= n) return;
   if (mask && (S[blockIdx.z].flags & mask) != mask) return;
   const int nch = min(kDenseChunks, (gd.m + kBlock - 1) / kBlock);
   const int c = blockIdx.y;
@@ -513,17 +576,25 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
   const int R = (gd.m + nch - 1) / nch;
   const int i0 = c * R, i1 = min(gd.m, i0 + R);
   const float *xg = x + gd.off_m;
-  double acc = 0.0;
-  if (gd.plan.dtype == TV3_F32) {
-    const float *A = (const float *)gd.plan.A + col;
-    for (int i = i0; i < i1; ++i)
-      acc += (double)A[(long long)i * gd.n] * (double)xg[i];
+  const T *A = (const T *)gd.plan.A;
+  const int w = min(8, n - c0);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (w == 8 && gd.plan.a_vec) {
+    for (int i = i0; i < i1; ++i) {
+      const double xi = (double)xg[i];
+      float a[8];
+      load8(A + (long long)i * n + c0, a);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += (double)a[u] * xi;
+    }
   } else {
-    const __hip_bfloat16 *A = (const __hip_bfloat16 *)gd.plan.A + col;
-    for (int i = i0; i < i1; ++i)
-      acc += (double)__bfloat162float(A[(long long)i * gd.n]) * (double)xg[i];
+    for (int i = i0; i < i1; ++i) {
+      const double xi = (double)xg[i];
+      for (int u = 0; u < w; ++u) acc[u] += (double)a_val(A, (long long)i * n + c0 + u) * xi;
+    }
   }
-  scratch[blockIdx.z * sstride + (long long)c * gd.n + col] = acc;
+  double *out = scratch + blockIdx.z * sstride + (long long)c * n + c0;
+  for (int u = 0; u < w; ++u) out[u] = acc[u];
 }
 
 __global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce(
@@ -545,15 +616,26 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce(
 
 template <>
 void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
-                   const float *x, float *y, const InstScal *S, uint32_t mask) {
+                   int kind, int dtype, const float *x, float *y,
+                   const InstScal *S, uint32_t mask) {
+  if (kind == kDense) {
+    dim3 g((max_m + 3) / 4, G);
+    if (dtype == TV3_BF16)
+      hipLaunchKernelGGL(k_dense_fwd_w1<__hip_bfloat16>, g, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+    else
+      hipLaunchKernelGGL(k_dense_fwd_w1<float>, g, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+    return;
+  }
   dim3 grid((max_m + kBlock / kSub - 1) / (kBlock / kSub), G);
   hipLaunchKernelGGL(k_fwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
 }
 
 template <>
 void launch_fwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
-                    const float *x, float *y, const InstScal *S,
-                    uint32_t mask) {
+                    int kind, int dtype, const float *x, float *y,
+                    const InstScal *S, uint32_t mask) {
+  (void)kind;
+  (void)dtype;
   dim3 grid((max_m + 3) / 4, G);
   hipLaunchKernelGGL(k_fwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
                      mask);
@@ -561,14 +643,19 @@ void launch_fwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
 
 template <>
 void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
-                   int kind, const float *x, float *y, const InstScal *S,
-                   uint32_t mask, const float *sub, float *scratch) {
+                   int kind, int dtype, const float *x, float *y,
+                   const InstScal *S, uint32_t mask, const float *sub,
+                   float *scratch) {
   if (kind == kDense) {
     // scratch holds kDenseChunks * max_n doubles per group
     const long long sstride = (long long)kDenseChunks * max_n;
-    dim3 g1((max_n + kBlock - 1) / kBlock, kDenseChunks, G);
-    hipLaunchKernelGGL(k_dense_bwd_part, g1, dim3(kBlock), 0, s, G_dev, x,
-                       (double *)scratch, S, mask, sstride);
+    dim3 g1((max_n + 8 * kBlock - 1) / (8 * kBlock), kDenseChunks, G);
+    if (dtype == TV3_BF16)
+      hipLaunchKernelGGL(k_dense_bwd_part<__hip_bfloat16>, g1, dim3(kBlock), 0, s, G_dev, x,
+                         (double *)scratch, S, mask, sstride);
+    else
+      hipLaunchKernelGGL(k_dense_bwd_part<float>, g1, dim3(kBlock), 0, s, G_dev, x,
+                         (double *)scratch, S, mask, sstride);
     dim3 g2((max_n + kBlock - 1) / kBlock, G);
     hipLaunchKernelGGL(k_dense_bwd_reduce, g2, dim3(kBlock), 0, s, G_dev,
                        (const double *)scratch, y, S, mask, sub, sstride);
@@ -581,9 +668,11 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
 
 template <>
 void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
-                    int kind, const float *x, float *y, const InstScal *S,
-                    uint32_t mask, const float *sub, float *scratch) {
+                    int kind, int dtype, const float *x, float *y,
+                    const InstScal *S, uint32_t mask, const float *sub,
+                    float *scratch) {
   (void)kind;
+  (void)dtype;
   (void)scratch;
   dim3 grid((max_n + 3) / 4, G);
   hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
@@ -592,8 +681,10 @@ void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
 
 template <>
 void launch_fwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
-                    const float *x, float *y, const InstScal *S,
-                    uint32_t mask) {
+                    int kind, int dtype, const float *x, float *y,
+                    const InstScal *S, uint32_t mask) {
+  (void)kind;
+  (void)dtype;
   const int RB = (max_m + 3) / 4;
   if (G >= kXcd) {
     const int nb = kXcd * ((G + kXcd - 1) / kXcd) * RB;
@@ -607,9 +698,11 @@ void launch_fwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
 
 template <>
 void launch_bwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
-                    int kind, const float *x, float *y, const InstScal *S,
-                    uint32_t mask, const float *sub, float *scratch) {
+                    int kind, int dtype, const float *x, float *y,
+                    const InstScal *S, uint32_t mask, const float *sub,
+                    float *scratch) {
   (void)kind;
+  (void)dtype;
   (void)scratch;
   const int RB = (max_n + 3) / 4;
   if (G >= kXcd) {
@@ -736,6 +829,7 @@ int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,
   p->ctx = ctx;
   p->dev.kind = kDense; p->dev.m = m; p->dev.n = n; p->dev.A = A;
   p->dev.dtype = dtype;
+  p->dev.a_vec = ((uintptr_t)A % 16 == 0) && (n % 8 == 0);
   p->nnz = (long long)m * n;
   *out = p;
   return TV3_OK;

@@ -1219,7 +1219,7 @@ __global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
 struct Batch {
   int B = 0, G = 0, W = 1, Bpad = 0;
   std::vector<GroupDesc> groups;
-  int max_n = 0, max_m = 0, kind = 0;
+  int max_n = 0, max_m = 0, kind = 0, dtype = 0;
   long long Nn = 0, Nm = 0;
   bool z3 = false;
 };
@@ -1376,12 +1376,12 @@ class Engine {
 
   void fwd(const float *x, float *y, uint32_t mask) {
     prof_begin(ctx_, 0);
-    launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, x, y, S_, mask);
+    launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
     prof_end(ctx_);
   }
   void bwd(const float *x, float *y, uint32_t mask, const float *sub) {
     prof_begin(ctx_, 1);
-    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, x, y, S_, mask, sub, scratch_);
+    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_);
     prof_end(ctx_);
   }
 
@@ -1573,12 +1573,14 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
       return set_err(ctx, TV3_ERR_ARG, "Argument error: volume dimensions must be positive");
     if ((long long)q * p * r != A[i]->dev.n)
       return set_err(ctx, TV3_ERR_ARG, "Argument error: the length of U must be equal to A.dim_x!");
-    if (A[i]->dev.kind != A[0]->dev.kind)
-      return set_err(ctx, TV3_ERR_ARG, "Argument error: one batch needs one operator kind");
+    if (A[i]->dev.kind != A[0]->dev.kind ||
+        (A[i]->dev.kind == kDense && A[i]->dev.dtype != A[0]->dev.dtype))
+      return set_err(ctx, TV3_ERR_ARG, "Argument error: one batch needs one operator kind and element type");
     if (i > 0 && (A[i] != A[0] || q != dims[0] || p != dims[1] || r != dims[2])) shared = false;
     if (r > 1) bt.z3 = true;
   }
   bt.kind = A[0]->dev.kind;
+  bt.dtype = A[0]->dev.dtype;
   // shared operator and size: 64-lane groups (one walk and one 256-byte
   // gather per voxel for 64 instances).  TV3_GROUP_WIDTH=16 selects 16-lane
   // groups dealt one per XCD (measured slower on the 256x256 ring).
@@ -1598,8 +1600,9 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
     gd.n = A[i]->dev.n; gd.m = A[i]->dev.m;
     gd.off_n = on; gd.off_m = om;
     gd.plan = A[i]->dev;
-    on += (long long)gd.n * W;
-    om += (long long)gd.m * W;
+    // offsets stay multiples of 64 floats (256-byte aligned group vectors)
+    on += ((long long)gd.n * W + 63) / 64 * 64;
+    om += ((long long)gd.m * W + 63) / 64 * 64;
     bt.max_n = std::max(bt.max_n, gd.n);
     bt.max_m = std::max(bt.max_m, gd.m);
     bt.groups.push_back(gd);
@@ -1790,9 +1793,9 @@ int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
   const long long dense = A->dev.kind == kDense ? 16LL * A->dev.n : 1;
   if ((rc = op_scratch(ctx, gd, 0.0, sc, dense))) return rc;
   if (trans)
-    launch_bwd<1>(ctx->stream, sc.G, 1, gd.n, A->dev.kind, x, y, sc.S, 0, nullptr, (float *)sc.dense);
+    launch_bwd<1>(ctx->stream, sc.G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, sc.S, 0, nullptr, (float *)sc.dense);
   else
-    launch_fwd<1>(ctx->stream, sc.G, 1, gd.m, x, y, sc.S, 0);
+    launch_fwd<1>(ctx->stream, sc.G, 1, gd.m, A->dev.kind, A->dev.dtype, x, y, sc.S, 0);
   return hip_check(ctx, hipGetLastError(), "gemv launch");
 }
 
