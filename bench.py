@@ -77,12 +77,15 @@ def cpu_time(fn, budget_s, max_runs=20):
             return runs, el, O.get_threads()
 
 
-def roofline(prof, bytes_per_launch, kernels, workload):
+def roofline(prof, kernels, workload):
+    """Per-operator figures from the library's launch profile: each launch is
+    charged the algorithmic bytes of the instances it computed (masked-off
+    instances of a batch read nothing), so achieved = bytes / time."""
     ops = {}
-    for k, (ms, cnt) in prof.items():
-        avg = ms / max(cnt, 1)
-        ops[k] = {"launches": cnt, "ms_per_launch": avg,
-                  "GBps": bytes_per_launch / (avg * 1e-3) / 1e9 if cnt else None}
+    for k, (ms, cnt, by) in prof.items():
+        ops[k] = {"launches": cnt, "ms_per_launch": ms / max(cnt, 1),
+                  "bytes_per_launch": by / max(cnt, 1),
+                  "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
     dom = max(ops, key=lambda k: prof[k][0])
     ach = ops[dom]["GBps"]
     kern = kernels[dom]
@@ -90,7 +93,8 @@ def roofline(prof, bytes_per_launch, kernels, workload):
                  "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                  "frac": ach / HBM_PEAK_GBPS if ach else None,
                  "traffic": traffic_from_profiles(workload, kern),
-                 "bytes_per_launch": bytes_per_launch}
+                 "bytes_per_launch": ops[dom]["bytes_per_launch"],
+                 "ms_per_launch": ops[dom]["ms_per_launch"]}
 
 
 # ---------------------------------------------------------------------------
@@ -139,14 +143,35 @@ def run_ring(args, ctx):
     value = B * args.steps / t_steps
     rel_b0 = float(torch.linalg.norm(Us[0] - U0) / torch.linalg.norm(U0))
     iters = [inf.total_iters for inf in infos]
-    Bpad = ((B + 63) // 64) * 64
-    ops, roof = roofline(prof, Bpad * csr_equiv_bytes(plan.nnz, plan.m, plan.n),
-                         {"A": "k_fwd_w64", "At": "k_bwd_w64"}, "ring")
+    ops, roof = roofline(prof, {"A": "k_fwd_w64", "At": "k_bwd_w64"}, "ring")
+    # the projector walks its indices on the fly: its bytes are the CSR
+    # equivalent (8 B per entry), and x / y stay cache-resident
     roof["cache_resident"] = True
+    roof["csr_equivalent"] = True
+    roof["voxel_visits_per_s"] = (ops["A"]["bytes_per_launch"] / csr_equiv_bytes(
+        plan.nnz, plan.m, plan.n) * plan.nnz / (ops["A"]["ms_per_launch"] * 1e-3))
+    import numpy as np
+    import tv3_oracle as O
+    op = O.Op.geometry(geo)
+    fp64 = None
+    if not args.no_fp64:
+        U64, _, _ = O.solve_f64(op, bs_host[0], dims, opts)
+        U32, _, _ = O.solve(op, bs_host[0], dims, opts)
+        nrm = np.linalg.norm(U64)
+        fp64 = {"instance": 0,
+                "batched_vs_fp64": float(np.linalg.norm(Us[0].cpu().numpy() - U64) / nrm),
+                "plain_vs_fp64": float(np.linalg.norm(U0.cpu().numpy() - U64) / nrm),
+                "oracle_fp32_vs_fp64": float(np.linalg.norm(U32 - U64) / nrm),
+                "batched_vs_oracle_fp32": float(np.linalg.norm(Us[0].cpu().numpy() - U32)
+                                                / np.linalg.norm(U32))}
+        # the FP64 solve's own sensitivity: b perturbed by 1e-12 relative
+        rng = np.random.default_rng(5)
+        bp = bs_host[0].astype(np.float64) * (1 + 1e-12 * rng.standard_normal(plan.m))
+        Up, _, _ = O.solve_f64(op, bp, dims, opts)
+        fp64["fp64_vs_fp64_b_perturbed_1e-12"] = float(np.linalg.norm(Up - U64) / nrm)
+        log(f"fp64: {fp64}")
     cpu = None
     if not args.no_cpu:
-        import tv3_oracle as O
-        op = O.Op.geometry(geo)
         runs, el, thr = cpu_time(lambda: O.solve(op, bs_host[0], dims, opts), args.cpu_seconds)
         cpu = {"value": runs / el, "unit": "recons/s", "cores": thr, "kind": "port",
                "sample": f"{runs} full {args.maxit}-iteration reconstructions of the "
@@ -164,6 +189,7 @@ def run_ring(args, ctx):
         "batched_over_plain": value * t_plain,
         "batch_vs_plain_rel_l2": rel_b0,
         "iters_min_max": [min(iters), max(iters)],
+        "rel_l2_vs_fp64": fp64,
         "operators": ops, "roofline": roof, "cpu_baseline": cpu,
     }
     return out
@@ -198,7 +224,8 @@ def run_dense_batch(args, ctx):
     torch.cuda.synchronize()
     log(f"dense-batch: {B} instances, {tot_elems / 1e9:.2f} G bf16 entries "
         f"({2 * tot_elems / 2**30:.1f} GiB)")
-    opts = tval3_amd.default_options(maxit=args.maxit)
+    # FP32 vectors, FP64 step-size / line-search scalars
+    opts = tval3_amd.default_options(maxit=args.maxit, scal_double=1)
     for _ in range(args.warmup):
         ctx.solve_batch(plans, bs, dims, opts)
     torch.cuda.synchronize()
@@ -212,22 +239,29 @@ def run_dense_batch(args, ctx):
     prof = ctx.profile_read()
     ctx.profile(False)
     value = B * args.steps / t_steps
-    bytes_launch = sum(2 * m * g * g + 4 * g * g + 4 * m for g, m in sizes)
-    ops, roof = roofline(prof, bytes_launch,
-                         {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}, "dense-batch")
+    ops, roof = roofline(prof, {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"},
+                         "dense-batch")
     # error against the FP64 solve of the same bf16-rounded matrices, on the
-    # smallest instances (CPU oracle, double/double)
+    # smallest instances (CPU oracle, double/double), beside the FP32 oracle's
+    # own distance to that FP64 solve
     import tv3_oracle as O
     order = sorted(range(B), key=lambda k: sizes[k][0] ** 2 * sizes[k][1])[:args.f64_sample]
     errs = {}
     for k in order:
         Ab, x = keep[k]
-        A_r = Ab.float().cpu().numpy()
-        op = O.Op.dense(A_r)
-        U64, _, _ = O.solve_f64(op, bs[k].cpu().numpy(), dims[k], opts)
+        op = O.Op.dense(Ab.float().cpu().numpy())
+        bk = bs[k].cpu().numpy()
+        U64, _, _ = O.solve_f64(op, bk, dims[k], opts)
+        U32, _, _ = O.solve(op, bk, dims[k], opts)
         Ug = Us[k].cpu().numpy().astype(np.float64)
-        errs[k] = float(np.linalg.norm(Ug - U64) / np.linalg.norm(U64))
-    log(f"rel L2 vs FP64 on {len(errs)} instances: max {max(errs.values()):.2e}")
+        nrm = np.linalg.norm(U64)
+        errs[k] = {"gpu_vs_fp64": float(np.linalg.norm(Ug - U64) / nrm),
+                   "oracle_fp32_vs_fp64": float(np.linalg.norm(U32 - U64) / nrm),
+                   "gpu_vs_oracle_fp32": float(np.linalg.norm(Ug - U32) / np.linalg.norm(U32)),
+                   "grid": sizes[k][0], "m": sizes[k][1]}
+    log("rel L2 vs FP64: " + ", ".join(f"{k}: {v['gpu_vs_fp64']:.2e} "
+                                      f"(oracle {v['oracle_fp32_vs_fp64']:.2e})"
+                                      for k, v in errs.items()))
     cpu = None
     if not args.no_cpu:
         k = order[0]
@@ -270,7 +304,6 @@ def run_single(args, ctx, workload):
         dims = (grid, grid, grid)
         maxit = args.maxit if args.maxit_set else 100
         opts = tval3_amd.default_options(maxit=maxit)
-        bytes_launch = csr_equiv_bytes(plan.nnz, plan.m, plan.n)
         kern = {"A": "k_fwd_w1", "At": "k_bwd_w1"}
         op_fn = lambda: O.Op.geometry(geo)  # noqa: E731
         desc = f"3D USCT hemisphere {grid}^3, {plan.m} rays, maxit {maxit}"
@@ -284,7 +317,6 @@ def run_single(args, ctx, workload):
         dims = (q, p, 1)
         maxit = args.maxit if args.maxit_set else 300
         opts = tval3_amd.default_options(maxit=maxit, tv_iso=1)
-        bytes_launch = 4 * m * q * p + 4 * q * p + 4 * m
         kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}
         op_fn = lambda: O.Op.dense(A)  # noqa: E731
         desc = f"2D dense 64x64, m={m}, isotropic TV, maxit {maxit}"
@@ -300,7 +332,11 @@ def run_single(args, ctx, workload):
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
-    ops, roof = roofline(prof, bytes_launch, kern, workload)
+    ops, roof = roofline(prof, kern, workload)
+    if workload == "hemisphere":
+        roof["csr_equivalent"] = True
+    else:
+        roof["cache_resident"] = True  # 20 MB A stays in the 256 MB last-level cache
     cpu = None
     if not args.no_cpu:
         op = op_fn()
@@ -328,6 +364,8 @@ def main():
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 oracle solve used for the error report")
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--maxit", type=int, default=None)
     ap.add_argument("--f64-sample", type=int, default=4)
@@ -347,7 +385,7 @@ def main():
     if args.grid is None:
         args.grid = 256
     if args.maxit is None:
-        args.maxit = 200 if args.workload != "dense-batch" else 300
+        args.maxit = 300 if args.workload == "dense-batch" else 200
 
     import torch
     import tval3_amd

@@ -37,6 +37,8 @@
 #ifndef TVAL3_AMD_H_
 #define TVAL3_AMD_H_
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -143,13 +145,24 @@ void tv3_ctx_destroy(tv3_ctx *ctx);
 const char *tv3_ctx_error(const tv3_ctx *ctx);
 int tv3_ctx_sync(tv3_ctx *ctx);
 void *tv3_ctx_stream(const tv3_ctx *ctx);
+/* Device memory for host callers (no reference counterpart: the reference
+ * works on host arrays).  The C++ overload layer uses these to stage host
+ * mat<T> data, so that it needs no HIP headers.  upload / download are
+ * synchronous on the context stream. */
+int tv3_mem_alloc(tv3_ctx *ctx, size_t bytes, void **out);
+int tv3_mem_free(tv3_ctx *ctx, void *p);
+int tv3_mem_upload(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
+int tv3_mem_download(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Per-launch HIP-event timing of the A and A^T products issued by the solver
  * on the context stream (no reference counterpart: it replaces the
  * reference's PROFILING gemv slots 10/11, 3d/mat_mult.h:19-44).  enable
  * resets the totals.  profile_read returns, for [A, A^T], the summed
- * milliseconds and the number of launches. */
+ * milliseconds, the number of launches and (bytes, may be NULL) the summed
+ * algorithmic bytes of the instances those launches computed: dense
+ * s_A*m*n + 4n + 4m, CSR / CSC 8*nnz + 4(rows+1) + 4n + 4m, geometry the
+ * CSR-equivalent of its walk. */
 int tv3_ctx_profile(tv3_ctx *ctx, int enable);
-int tv3_ctx_profile_read(tv3_ctx *ctx, double *ms, long long *count);
+int tv3_ctx_profile_read(tv3_ctx *ctx, double *ms, long long *count, double *bytes);
 
 /* ---- measurement-operator plans ------------------------------------------ */
 /* Dense A, row-major m x n, on the device (f32 or bf16).  Borrowed. */

@@ -1,0 +1,110 @@
+// Test driver of tval3_gpu_2d (include/tval3_gpu_2d.hpp).
+//   shim_2d args                 argument checks only (no GPU use)
+//   shim_2d dense|sparse|geometry DIR   solve the problem stored in DIR with
+//                                float and double scalars; writes U_f.bin,
+//                                U_d.bin, info_f.txt, info_d.txt into DIR
+#include "ref_types_2d.h"
+#include "tval3_gpu_2d.hpp"
+#include "shim_io.h"
+
+#include <cstring>
+#include <memory>
+
+static int args_mode() {
+  mat<float> A(6, 16), b(6), U(4, 4), Ut(4, 4);
+  A.format = mat_row_major;
+  U.format = Ut.format = mat_col_major;
+  tval3_options<float> of;
+  mat<float> U_row(4, 4);
+  expect_error("dense format", [&] { tval3_gpu_2d(U_row, A, b, of, Ut); });
+  mat<float> Ut_bad(3, 3);
+  Ut_bad.format = mat_col_major;
+  expect_error("dense Ut size", [&] { tval3_gpu_2d(U, A, b, of, Ut_bad); });
+  mat<float> A_bad(6, 15);
+  expect_error("dense U length", [&] { tval3_gpu_2d(U, A_bad, b, of, Ut); });
+  mat<float> b_bad(7);
+  expect_error("dense b length", [&] { tval3_gpu_2d(U, A, b_bad, of, Ut); });
+  mat<float> U2(4, 8), Ut2(4, 8), b2(12);
+  U2.format = Ut2.format = mat_col_major;
+  expect_error("dense layers", [&] { tval3_gpu_2d(U2, A, b2, of, Ut2); });
+  sparse_mat<float> S;
+  S.dim_x = 16; S.dim_y = 6; S.format = sparse_mat_csr;
+  expect_error("sparse format", [&] { tval3_gpu_2d(U, S, b, of, Ut); });
+  geometry g;
+  g.num_emitters = 2; g.num_receivers = 2;
+  expect_error("geometry format", [&] { tval3_gpu_2d(U, g, b, of, Ut); });
+  mat<float> Ug(4, 4), Utg(4, 4);
+  expect_error("geometry b length", [&] { tval3_gpu_2d(Ug, g, b, of, Utg); });
+  mat<double> Ud(4, 4), Ad(6, 16), bd(6), Utd(4, 4);
+  tval3_options<double> od;
+  expect_error("double data", [&] { tval3_gpu_2d(Ud, Ad, bd, od, Utd); });
+  mat<std::complex<float> > Uc(4, 4), Ac(6, 16), bc(6), Utc(4, 4);
+  expect_error("complex data", [&] { tval3_gpu_2d(Uc, Ac, bc, of, Utc); });
+  return 0;
+}
+
+template <class Ts, class A_t>
+static void run(const std::string &dir, const char *tag, const A_t &A, const mat<float> &b,
+                int dx, int dy, bool col_major, int maxit) {
+  mat<float> U(dy, dx), Ut(0);
+  U.format = Ut.format = col_major ? mat_col_major : mat_row_major;
+  tval3_options<Ts> o;
+  o.maxit = maxit;
+  const tval3_info<Ts> info = tval3_gpu_2d(U, A, b, o, Ut);
+  save(dir + "/U_" + tag + ".bin", U.data(), (size_t)U.len);
+  save_info(dir + "/info_" + tag + ".txt", info);
+}
+
+int main(int argc, char **argv) {
+  try {
+    if (argc >= 2 && !std::strcmp(argv[1], "args")) return args_mode();
+    if (argc < 3) {
+      std::fprintf(stderr, "usage: shim_2d args | dense|sparse|geometry DIR\n");
+      return 2;
+    }
+    const std::string kind = argv[1], dir = argv[2];
+    auto meta = load_meta(dir + "/meta.txt");
+    const int dx = (int)meta["dx"], dy = (int)meta["dy"], maxit = (int)meta["maxit"];
+    std::vector<float> bv = load<float>(dir + "/b.bin");
+    mat<float> b((int)bv.size());
+    std::copy(bv.begin(), bv.end(), b.data());
+    if (kind == "dense") {
+      std::vector<float> Av = load<float>(dir + "/A.bin");
+      mat<float> A((int)meta["m"], (int)meta["n"]);
+      std::copy(Av.begin(), Av.end(), A.data());
+      run<float>(dir, "f", A, b, dx, dy, true, maxit);
+      run<double>(dir, "d", A, b, dx, dy, true, maxit);
+    } else if (kind == "sparse") {
+      sparse_mat<float> S;
+      S.dim_y = (int)meta["m"]; S.dim_x = (int)meta["n"];
+      S.csr_ptr = load<int>(dir + "/csr_ptr.bin"); S.csr_ind = load<int>(dir + "/csr_ind.bin");
+      S.csr_val = load<float>(dir + "/csr_val.bin"); S.csc_ptr = load<int>(dir + "/csc_ptr.bin");
+      S.csc_ind = load<int>(dir + "/csc_ind.bin"); S.csc_val = load<float>(dir + "/csc_val.bin");
+      run<float>(dir, "f", S, b, dx, dy, true, maxit);
+      run<double>(dir, "d", S, b, dx, dy, true, maxit);
+    } else if (kind == "geometry") {
+      std::vector<int> up = load<int>(dir + "/use_path.bin");
+      std::vector<int> xe = load<int>(dir + "/xe.bin"), ye = load<int>(dir + "/ye.bin");
+      std::vector<int> xr = load<int>(dir + "/xr.bin"), yr = load<int>(dir + "/yr.bin");
+      std::unique_ptr<bool[]> upb(new bool[up.size()]);
+      for (size_t k = 0; k < up.size(); ++k) upb[k] = up[k] != 0;
+      geometry g;
+      g.num_emitters = (int)xe.size(); g.num_receivers = (int)xr.size();
+      g.use_path = upb.get();
+      g.x_emitters = xe.data(); g.y_emitters = ye.data();
+      g.x_receivers = xr.data(); g.y_receivers = yr.data();
+      g.ld = (int)meta["ld"];
+      g.scale_factor = (float)meta["scale"];
+      run<float>(dir, "f", g, b, dx, dy, false, maxit);
+      run<double>(dir, "d", g, b, dx, dy, false, maxit);
+    } else {
+      std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
+      return 2;
+    }
+    std::printf("ok\n");
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+}

@@ -1,0 +1,107 @@
+// Test driver of tval3_gpu_3d (include/tval3_gpu_3d.hpp).
+//   shim_3d args                 argument checks only (no GPU use)
+//   shim_3d dense|sparse|geometry DIR   as shim_2d, with dims dx dy dz
+#include "ref_types_3d.h"
+#include "tval3_gpu_3d.hpp"
+#include "shim_io.h"
+
+#include <cstring>
+
+static int args_mode() {
+  mat<float> A(6, 64), b(6), U(4, 4, 4), Ut(4, 4, 4);
+  tval3_options<float> of;
+  mat<float> U_col(4, 4, 4);
+  U_col.format = mat_col_major;
+  expect_error("dense format", [&] { tval3_gpu_3d(U_col, A, b, of, Ut); });
+  mat<float> Ut_bad(3, 3, 3);
+  expect_error("dense Ut size", [&] { tval3_gpu_3d(U, A, b, of, Ut_bad); });
+  mat<float> A_bad(6, 63);
+  expect_error("dense U length", [&] { tval3_gpu_3d(U, A_bad, b, of, Ut); });
+  mat<float> b_bad(7);
+  expect_error("dense b length", [&] { tval3_gpu_3d(U, A, b_bad, of, Ut); });
+  sparse_mat<float> S;
+  S.dim_x = 64; S.dim_y = 6; S.format = sparse_mat_csc;
+  mat<float> Uc(4, 4, 4), Utc(4, 4, 4);
+  Uc.format = Utc.format = mat_col_major;
+  expect_error("sparse format", [&] { tval3_gpu_3d(Uc, S, b, of, Utc); });
+  geometry g;
+  g.numMPs = 1; g.numPaths = 5;
+  expect_error("geometry b length", [&] { tval3_gpu_3d(U, g, b, of, Ut); });
+  mat<double> Ud(4, 4, 4), Ad(6, 64), bd(6), Utd(4, 4, 4);
+  tval3_options<double> od;
+  expect_error("double data", [&] { tval3_gpu_3d(Ud, Ad, bd, od, Utd); });
+  return 0;
+}
+
+template <class Ts, class A_t>
+static void run(const std::string &dir, const char *tag, const A_t &A, const mat<float> &b,
+                int dx, int dy, int dz, bool col_major, int maxit) {
+  mat<float> U(dy, dx, dz), Ut(0);
+  U.format = Ut.format = col_major ? mat_col_major : mat_row_major;
+  tval3_options<Ts> o;
+  o.maxit = maxit;
+  const tval3_info<Ts> info = tval3_gpu_3d(U, A, b, o, Ut);
+  save(dir + "/U_" + tag + ".bin", U.data(), (size_t)U.len);
+  save_info(dir + "/info_" + tag + ".txt", info);
+}
+
+int main(int argc, char **argv) {
+  try {
+    if (argc >= 2 && !std::strcmp(argv[1], "args")) return args_mode();
+    if (argc < 3) {
+      std::fprintf(stderr, "usage: shim_3d args | dense|sparse|geometry DIR\n");
+      return 2;
+    }
+    const std::string kind = argv[1], dir = argv[2];
+    auto meta = load_meta(dir + "/meta.txt");
+    const int dx = (int)meta["dx"], dy = (int)meta["dy"], dz = (int)meta["dz"];
+    const int maxit = (int)meta["maxit"];
+    std::vector<float> bv = load<float>(dir + "/b.bin");
+    mat<float> b((int)bv.size());
+    std::copy(bv.begin(), bv.end(), b.data());
+    if (kind == "dense") {
+      std::vector<float> Av = load<float>(dir + "/A.bin");
+      mat<float> A((int)meta["m"], (int)meta["n"]);
+      std::copy(Av.begin(), Av.end(), A.data());
+      run<float>(dir, "f", A, b, dx, dy, dz, false, maxit);
+      run<double>(dir, "d", A, b, dx, dy, dz, false, maxit);
+    } else if (kind == "sparse") {
+      sparse_mat<float> S;
+      S.dim_y = (int)meta["m"]; S.dim_x = (int)meta["n"];
+      S.csr_ptr = load<int>(dir + "/csr_ptr.bin"); S.csr_ind = load<int>(dir + "/csr_ind.bin");
+      S.csr_val = load<float>(dir + "/csr_val.bin"); S.csc_ptr = load<int>(dir + "/csc_ptr.bin");
+      S.csc_ind = load<int>(dir + "/csc_ind.bin"); S.csc_val = load<float>(dir + "/csc_val.bin");
+      run<float>(dir, "f", S, b, dx, dy, dz, true, maxit);
+      run<double>(dir, "d", S, b, dx, dy, dz, true, maxit);
+    } else if (kind == "geometry") {
+      const int P = (int)meta["mps"], E = (int)meta["E"], R = (int)meta["R"];
+      std::vector<int> up = load<int>(dir + "/use_path.bin");
+      std::vector<int> c[6] = {load<int>(dir + "/xe.bin"), load<int>(dir + "/ye.bin"),
+                               load<int>(dir + "/ze.bin"), load<int>(dir + "/xr.bin"),
+                               load<int>(dir + "/yr.bin"), load<int>(dir + "/zr.bin")};
+      std::vector<int *> rows[6];
+      for (int k = 0; k < 6; ++k) {
+        const int w = k < 3 ? E : R;
+        for (int i = 0; i < P; ++i) rows[k].push_back(c[k].data() + (size_t)i * w);
+      }
+      geometry g;
+      g.numMPs = P; g.numPaths = (int)meta["paths"];
+      g.num_emitters = E; g.num_receivers = R;
+      g.use_path = up.data();
+      g.x_emitters = rows[0].data(); g.y_emitters = rows[1].data(); g.z_emitters = rows[2].data();
+      g.x_receivers = rows[3].data(); g.y_receivers = rows[4].data(); g.z_receivers = rows[5].data();
+      g.rv_x = (int)meta["rv_x"]; g.rv_y = (int)meta["rv_y"]; g.rv_z = (int)meta["rv_z"];
+      g.scale_factor = (float)meta["scale"];
+      run<float>(dir, "f", g, b, dx, dy, dz, false, maxit);
+      run<double>(dir, "d", g, b, dx, dy, dz, false, maxit);
+    } else {
+      std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
+      return 2;
+    }
+    std::printf("ok\n");
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+}

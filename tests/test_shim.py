@@ -1,0 +1,166 @@
+"""The C++ overload layer (include/tval3_gpu_2d.hpp, tval3_gpu_3d.hpp): the
+reference's tval3_cpu_2d / tval3_cpu_3d call surface over the C-ABI.
+
+The drivers under tests/cpp are built by build().  The argument checks run
+without a GPU and must give the reference's messages (check_params,
+2d/tval3_cpu_2d.cpp:696-750, 3d/tval3_cpu_3d.cpp:1028-1165).  The solves run
+float and double scalars and are compared with the oracle given the same
+options (the layer selects the reference's round-by-round backtracking).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import tv3_oracle as O
+import tval3_amd
+from tval3_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CPP = os.path.join(HERE, "cpp")
+
+
+def _driver(name):
+    path = os.path.join(CPP, name)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (run __graft_entry__.build())")
+    return path
+
+
+EXPECT_2D = {
+    "dense format": "Argument error: Matrices must be in row major format!",
+    "dense Ut size": "Argument error: U and Ut must have the same size!",
+    "dense U length": "Argument error: the length of U must be equal to A.dim_x * layers!",
+    "dense b length": "Argument error: b.len must be equal to A.dim_y * layers!",
+    "sparse format": "Argument error: A must be available in CSR and CSC format!",
+    "geometry format": "Argument error: Matrices must be in row major format!",
+    "geometry b length": "Argument error: b.len must be equal to A.num_emitters * A.num_receivers!",
+}
+EXPECT_3D = {
+    "dense format": "Argument error: Matrices must be in row major format!",
+    "dense Ut size": "Argument error: U and Ut must have the same size!",
+    "dense U length": "Argument error: the length of U must be equal to A.dim_x!",
+    "dense b length": "Argument error: b.len must be equal to A.dim_y!",
+    "sparse format": "Argument error: A must be available in csr and csc format!",
+    "geometry b length": "Argument error: b.len must be equal to A.num_emitters * A.num_receivers!",
+}
+
+
+@pytest.mark.parametrize("name,expect", [("shim_2d", EXPECT_2D), ("shim_3d", EXPECT_3D)])
+def test_argument_checks(name, expect):
+    out = subprocess.run([_driver(name), "args"], capture_output=True, text=True,
+                         timeout=60, check=True).stdout
+    got = dict(line.split(": ", 1) for line in out.strip().splitlines())
+    for case, msg in expect.items():
+        assert got[case] == msg, case
+    # the engine's own limits are argument or internal errors, never silent
+    assert got["double data"].startswith("Internal error: ")
+    if name == "shim_2d":
+        assert got["dense layers"].startswith("Argument error: multilayer")
+        assert got["complex data"].startswith("Internal error: ")
+
+
+# ---- GPU solves through the overload layer ------------------------------------
+def _write(d, meta, **arrays):
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "meta.txt"), "w") as f:
+        for k, v in meta.items():
+            f.write(f"{k} {v}\n")
+    for k, a in arrays.items():
+        a.tofile(os.path.join(d, k + ".bin"))
+
+
+def _run_and_compare(name, kind, d, op, b, dims, maxit):
+    print(f"shim case {name} {kind} dims={dims} m={b.size}", flush=True)
+    r = subprocess.run([_driver(name), kind, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = int(np.prod(dims))
+    for tag, sd in (("f", 0), ("d", 1)):
+        Ug = np.fromfile(os.path.join(d, f"U_{tag}.bin"), np.float32)
+        assert Ug.size == n
+        outer, total, _, _ = open(os.path.join(d, f"info_{tag}.txt")).read().split()
+        opts = tval3_amd.default_options(maxit=maxit, ls_exact=1, scal_double=sd)
+        Uo, info_o, _ = O.solve(op, b, dims, opts)
+        rel = float(np.linalg.norm(Ug - Uo) / np.linalg.norm(Uo))
+        assert int(total) == info_o.total_iters and int(outer) == info_o.outer_iters
+        assert rel <= 1e-6, (tag, rel)
+
+
+@pytest.mark.gpu
+def test_shim_2d_dense(tmp_path):
+    q = p = 24
+    m = 230
+    A = synth.gaussian_A(m, q * p, seed=3)
+    x = synth.ellipse_phantom(p, q, 6, seed=3)
+    b = synth.add_noise(A @ x, 0.01, seed=3)
+    d = str(tmp_path / "dense2d")
+    _write(d, {"dx": q, "dy": p, "m": m, "n": q * p, "maxit": 80}, A=A, b=b)
+    _run_and_compare("shim_2d", "dense", d, O.Op.dense(A), b, (q, p, 1), 80)
+
+
+@pytest.mark.gpu
+def test_shim_2d_sparse(tmp_path):
+    G = 32
+    geo = synth.ring_geometry(G, G)
+    ptr, lin = O.Op.geometry(geo).walk()
+    m, n = geo.m, geo.n
+    rp, ri, rv, cp, ci, cv = synth.csr_from_walk(ptr, lin, n)
+    op = O.Op.sparse(m, n, rp, ri, rv, cp, ci, cv)
+    x = synth.disk_phantom(G, seed=1)
+    b = synth.add_noise(op.gemv(x), 0.005, seed=1)
+    d = str(tmp_path / "sparse2d")
+    _write(d, {"dx": G, "dy": G, "m": m, "n": n, "maxit": 60},
+           b=b, csr_ptr=rp.astype(np.int32), csr_ind=ri.astype(np.int32),
+           csr_val=rv.astype(np.float32), csc_ptr=cp.astype(np.int32),
+           csc_ind=ci.astype(np.int32), csc_val=cv.astype(np.float32))
+    _run_and_compare("shim_2d", "sparse", d, op, b, (G, G, 1), 60)
+
+
+@pytest.mark.gpu
+def test_shim_2d_geometry(tmp_path):
+    # the 2-D geometry layout: ray e * R + r for every pair (no compaction)
+    G, K = 32, 32
+    xe, ye = synth.ring_transducers(K, G)
+    E = R = K
+    used = np.array([[1 if e < r else 0 for r in range(R)] for e in range(E)], np.int32).ravel()
+    up = np.where(used != 0, np.arange(E * R, dtype=np.int32) + 1, 0).astype(np.int32)
+    z = np.zeros(K, np.int32)
+    geo = synth.GeometryArrays(1, up, xe, ye, z, xe, ye, z, (G, G, 1), E * R)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(G, seed=2)
+    b = synth.add_noise(op.gemv(x), 0.005, seed=2) * (used != 0)
+    b = b.astype(np.float32)
+    d = str(tmp_path / "geo2d")
+    _write(d, {"dx": G, "dy": G, "ld": G, "scale": 1.0, "maxit": 60},
+           b=b, use_path=used, xe=xe, ye=ye, xr=xe, yr=ye)
+    _run_and_compare("shim_2d", "geometry", d, op, b, (G, G, 1), 60)
+
+
+@pytest.mark.gpu
+def test_shim_3d_geometry(tmp_path):
+    grid, K = 16, 40
+    geo = synth.hemisphere_geometry(K, grid)
+    op = O.Op.geometry(geo)
+    x = synth.ellipsoid_phantom(grid, 4, seed=4)
+    b = synth.add_noise(op.gemv(x), 0.005, seed=4)
+    d = str(tmp_path / "geo3d")
+    _write(d, {"dx": grid, "dy": grid, "dz": grid, "mps": geo.num_mps, "E": geo.E,
+               "R": geo.R, "paths": geo.num_paths, "rv_x": geo.rv[0], "rv_y": geo.rv[1],
+               "rv_z": geo.rv[2], "scale": geo.scale, "maxit": 40},
+           b=b, use_path=geo.use_path, xe=geo.xe, ye=geo.ye, ze=geo.ze,
+           xr=geo.xr, yr=geo.yr, zr=geo.zr)
+    _run_and_compare("shim_3d", "geometry", d, op, b, (grid, grid, grid), 40)
+
+
+@pytest.mark.gpu
+def test_shim_3d_dense(tmp_path):
+    q, p, r = 8, 8, 4
+    n = q * p * r
+    m = 100
+    A = synth.gaussian_A(m, n, seed=5)
+    x = np.random.default_rng(5).random(n).astype(np.float32)
+    b = synth.add_noise(A @ x, 0.01, seed=5)
+    d = str(tmp_path / "dense3d")
+    _write(d, {"dx": q, "dy": p, "dz": r, "m": m, "n": n, "maxit": 50}, A=A, b=b)
+    _run_and_compare("shim_3d", "dense", d, O.Op.dense(A), b, (q, p, r), 50)

@@ -49,6 +49,7 @@ struct PlanDev {
   const void *A;
   int dtype;
   int a_vec;  // A is 16-byte aligned and n % 8 == 0: rows take vector loads
+  long long nnz;  // stored entries (dense: m*n; geometry: walk length)
   // sparse (CSR of A and CSR of A^T = CSC of A)
   const int *r_ptr, *r_ind;
   const float *r_val;
@@ -143,6 +144,8 @@ struct tv3_ctx {
   std::vector<int> evkind;     // kind of each recorded pair: 0 = A, 1 = A^T
   double prof_ms[2] = {0, 0};
   long long prof_n[2] = {0, 0};
+  // device: algorithmic bytes of the instances each profiled launch computed
+  unsigned long long *prof_bytes = nullptr;
 };
 
 struct tv3_plan {
@@ -162,6 +165,10 @@ int set_err(tv3_ctx *ctx, int code, const std::string &msg);
 void prof_begin(tv3_ctx *ctx, int kind);
 void prof_end(tv3_ctx *ctx);
 void prof_collect(tv3_ctx *ctx);
+// adds the algorithmic bytes of the lanes a masked A (trans 0) / A^T (trans 1)
+// launch computed to ctx->prof_bytes[trans]; W lanes per group, Bpad lanes
+void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
+                const InstScal *S, uint32_t mask, int trans);
 int hip_check(tv3_ctx *ctx, hipError_t e, const char *what);
 
 // operator launches (tv3_plan.hip).  x and y are batch base pointers;

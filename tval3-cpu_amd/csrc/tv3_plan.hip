@@ -61,6 +61,30 @@ void prof_collect(tv3_ctx *ctx) {
   ctx->evkind.clear();
 }
 
+__global__ void k_prof_bytes(const GroupDesc *G, int W, int Bpad, const InstScal *S,
+                             uint32_t mask, int trans, unsigned long long *acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Bpad) return;
+  const uint32_t f = S[i].flags;
+  if (!(f & F_LIVE) || (mask && (f & mask) != mask)) return;
+  const PlanDev &pd = G[i / W].plan;
+  const unsigned long long m = pd.m, n = pd.n, nnz = pd.nnz;
+  unsigned long long b;
+  if (pd.kind == kDense) {
+    b = (pd.dtype == TV3_BF16 ? 2ull : 4ull) * m * n + 4 * n + 4 * m;
+  } else {  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
+    b = 8 * nnz + 4 * ((trans ? n : m) + 1) + 4 * n + 4 * m;
+  }
+  atomicAdd(&acc[trans], b);
+}
+
+void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
+                const InstScal *S, uint32_t mask, int trans) {
+  if (!ctx->prof || !ctx->prof_bytes) return;
+  hipLaunchKernelGGL(k_prof_bytes, dim3((Bpad + 255) / 256), dim3(256), 0, ctx->stream,
+                     G_dev, W, Bpad, S, mask, trans, ctx->prof_bytes);
+}
+
 int hip_check(tv3_ctx *ctx, hipError_t e, const char *what) {
   if (e == hipSuccess) return TV3_OK;
   return set_err(ctx, e == hipErrorOutOfMemory ? TV3_ERR_ALLOC : TV3_ERR_HIP,
@@ -783,6 +807,7 @@ void tv3_ctx_destroy(tv3_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ws) (void)hipFree(c->ws);
+  if (c->prof_bytes) (void)hipFree(c->prof_bytes);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->hflags) (void)hipHostFree(c->hflags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -801,6 +826,44 @@ int tv3_ctx_sync(tv3_ctx *c) {
 
 void *tv3_ctx_stream(const tv3_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int tv3_mem_alloc(tv3_ctx *c, size_t bytes, void **out) {
+  if (!c || !out) return TV3_ERR_ARG;
+  *out = nullptr;
+  if (!bytes) return TV3_OK;
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  if (hipMalloc(out, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return set_err(c, TV3_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return TV3_OK;
+}
+
+int tv3_mem_free(tv3_ctx *c, void *p) {
+  if (!c) return TV3_ERR_ARG;
+  if (!p) return TV3_OK;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (rc) return rc;
+  return hip_check(c, hipFree(p), "hipFree");
+}
+
+int tv3_mem_upload(tv3_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return TV3_ERR_ARG;
+  if (!bytes) return TV3_OK;
+  int rc = hip_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
+}
+
+int tv3_mem_download(tv3_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return TV3_ERR_ARG;
+  if (!bytes) return TV3_OK;
+  int rc = hip_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "download");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "download sync");
+}
+
 int tv3_ctx_profile(tv3_ctx *c, int enable) {
   if (!c) return TV3_ERR_ARG;
   (void)hipStreamSynchronize(c->stream);
@@ -808,15 +871,32 @@ int tv3_ctx_profile(tv3_ctx *c, int enable) {
   c->prof = enable != 0;
   c->prof_ms[0] = c->prof_ms[1] = 0;
   c->prof_n[0] = c->prof_n[1] = 0;
+  if (c->prof) {
+    if (!c->prof_bytes) {
+      int rc = hip_check(c, hipMalloc(&c->prof_bytes, 2 * sizeof(unsigned long long)), "profile hipMalloc");
+      if (rc) { c->prof = false; c->prof_bytes = nullptr; return rc; }
+    }
+    int rc = hip_check(c, hipMemsetAsync(c->prof_bytes, 0, 2 * sizeof(unsigned long long), c->stream), "profile clear");
+    if (rc) { c->prof = false; return rc; }
+  }
   return TV3_OK;
 }
 
-int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count) {
+int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes) {
   if (!c || !ms || !count) return TV3_ERR_ARG;
   int rc = hip_check(c, hipStreamSynchronize(c->stream), "profile sync");
   if (rc) return rc;
   prof_collect(c);
   for (int k = 0; k < 2; ++k) { ms[k] = c->prof_ms[k]; count[k] = c->prof_n[k]; }
+  if (bytes) {
+    unsigned long long hb[2] = {0, 0};
+    if (c->prof_bytes) {
+      rc = hip_check(c, hipMemcpy(hb, c->prof_bytes, sizeof(hb), hipMemcpyDeviceToHost), "profile read");
+      if (rc) return rc;
+    }
+    bytes[0] = (double)hb[0];
+    bytes[1] = (double)hb[1];
+  }
   return TV3_OK;
 }
 
@@ -831,6 +911,7 @@ int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,
   p->dev.dtype = dtype;
   p->dev.a_vec = ((uintptr_t)A % 16 == 0) && (n % 8 == 0);
   p->nnz = (long long)m * n;
+  p->dev.nnz = p->nnz;
   *out = p;
   return TV3_OK;
 }
@@ -851,6 +932,7 @@ int tv3_plan_sparse(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
   d.r_ptr = csr_ptr; d.r_ind = csr_ind; d.r_val = csr_val;
   d.c_ptr = csc_ptr; d.c_ind = csc_ind; d.c_val = csc_val;
   p->nnz = nnz;
+  p->dev.nnz = nnz;
   *out = p;
   return TV3_OK;
 }
@@ -916,6 +998,7 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
     p->walk_ptr[r + 1] = p->walk_ptr[r] + (rays[r].L >= 0 ? rays[r].L + 1 : 0);
   const long long nnz = p->walk_ptr[m];
   p->nnz = nnz;
+  p->dev.nnz = nnz;
   if (nnz > 0x7fffffffLL) { delete p; return set_err(ctx, TV3_ERR_ARG, "Argument error: too many ray voxels"); }
   hipStream_t s = ctx->stream;
   int rc = TV3_OK;

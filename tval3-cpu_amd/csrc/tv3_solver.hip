@@ -1378,11 +1378,13 @@ class Engine {
     prof_begin(ctx_, 0);
     launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
     prof_end(ctx_);
+    prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 0);
   }
   void bwd(const float *x, float *y, uint32_t mask, const float *sub) {
     prof_begin(ctx_, 1);
     launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_);
     prof_end(ctx_);
+    prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 1);
   }
 
   // OR of the instance flags the fin kernels published to mapped host memory

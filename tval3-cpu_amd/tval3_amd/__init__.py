@@ -60,7 +60,8 @@ def lib():
         "tv3_ctx_sync": (_I, [_VP]),
         "tv3_ctx_stream": (_VP, [_VP]),
         "tv3_ctx_profile": (_I, [_VP, _I]),
-        "tv3_ctx_profile_read": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_LL)]),
+        "tv3_ctx_profile_read": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_LL),
+                                      C.POINTER(C.c_double)]),
         "tv3_plan_dense": (_I, [_VP, _I, _I, _I, _VP, C.POINTER(_VP)]),
         "tv3_plan_sparse": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
         "tv3_plan_geometry": (_I, [_VP, _VP, C.POINTER(_VP)]),
@@ -150,11 +151,14 @@ class Context:
         self._check(lib().tv3_ctx_profile(self.h, int(bool(enable))))
 
     def profile_read(self):
-        """{'A': (ms_total, launches), 'At': (ms_total, launches)}"""
+        """{'A': (ms_total, launches, bytes), 'At': (...)}: summed launch
+        milliseconds, launch count and summed algorithmic bytes of the
+        instances the launches computed."""
         ms = (C.c_double * 2)()
         cnt = (_LL * 2)()
-        self._check(lib().tv3_ctx_profile_read(self.h, ms, cnt))
-        return {"A": (ms[0], cnt[0]), "At": (ms[1], cnt[1])}
+        by = (C.c_double * 2)()
+        self._check(lib().tv3_ctx_profile_read(self.h, ms, cnt, by))
+        return {"A": (ms[0], cnt[0], by[0]), "At": (ms[1], cnt[1], by[1])}
 
     def tensor(self, a, dtype=None):
         """Host array -> contiguous device tensor (float32 unless given)."""
