@@ -41,6 +41,29 @@
  *   - the shrinkage sum lam1 accumulates in FP64 and rounds to T_scal.
  * Reductions use fixed 4096-element chunks summed in chunk order, so results
  * do not depend on the OpenMP thread count.
+ *
+ * How each part is pinned (tests/test_oracle.py, tests/test_walk_closed_form.py):
+ *   Bresenham walk   golden walks from an independent transcription of the
+ *                    recurrence, walk invariants, the integer closed form,
+ *                    nnz counts of the ring geometries
+ *   gemv             FP64 brute force (dense, CSR/CSC, walk-built matrix)
+ *   D, Dt            closed form; adjoint identity; the 3-D y-term defect
+ *                    equals its closed form
+ *   shrinkage        soft-threshold closed forms and limits
+ *   scaleb           closed form (maxit = 0 returns the scaled A^T b)
+ *   get_tau, descend, get_gradient, update_W, min_u (Armijo rule and the
+ *   5-strike fallback), update_mlp, continuation, nonneg
+ *                    an FP64 model of the augmented Lagrangian: first steps
+ *                    are exact line minimisers along grad/beta, later steps
+ *                    are Barzilai-Borwein, accepted steps satisfy the
+ *                    nonmonotone Armijo rule and rejected ones do not, f
+ *                    equals the Lagrangian, multipliers follow the ALM update
+ *   ls_exact = 0     same steps as the round-by-round search (f(alpha) is
+ *                    an exact quadratic)
+ * Parity unpinned: the A^T-skipping state machine of get_g_mult
+ * (2d/tval3_cpu_2d.cpp:300-357, skipMulRatio > 0).  No closed form or
+ * invariant fixes its skip decisions; it follows the reference's control
+ * flow line by line and is checked only as the GPU path's reference.
  */
 #include "../include/tval3_amd.h"
 

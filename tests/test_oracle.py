@@ -357,3 +357,195 @@ def test_fp64_solve_close_to_fp32():
     opts = default_options(maxit=300, scal_double=1)
     U64, i64, _ = O.solve_f64(op, b, (20, 20, 1), opts, Ut=x)
     assert i64.rel_error < 0.05
+
+
+# ---------------------------------------------------------------------------
+# step size, line search and multipliers, pinned by the mathematics of the
+# augmented Lagrangian (no reference formula retyped):
+#   L(u, w; s, dl) = sum|w| - s.(Du - w) + beta/2 |Du - w|^2
+#                    - dl.(Au - b) + mu/2 |Au - b|^2
+# The solver's d is grad_u L / beta; its first step of every outer iteration
+# is the exact minimiser of L along d; later steps are Barzilai-Borwein
+# (s's / s'y on consecutive iterates and gradients); accepted steps satisfy
+# the nonmonotone Armijo rule L_new <= C - c alpha tau beta |d|^2 and the
+# rejected alpha / gamma did not; multipliers follow s -= beta (Du - w),
+# dl -= mu (Au - b).
+# ---------------------------------------------------------------------------
+class _ALM:
+    def __init__(self, A, bs, q, p, mu, beta):
+        self.A = A.astype(np.float64)
+        self.bs, self.q, self.p, self.mu, self.beta = bs, q, p, mu, beta
+
+    def D(self, U):
+        U2 = U.reshape(self.p, self.q)
+        return np.concatenate([(np.roll(U2, -1, 1) - U2).ravel(),
+                               (np.roll(U2, -1, 0) - U2).ravel()])
+
+    def Dt(self, X):
+        n = self.p * self.q
+        X2, Y2 = X[:n].reshape(self.p, self.q), X[n:].reshape(self.p, self.q)
+        return ((np.roll(X2, 1, 1) - X2) + (np.roll(Y2, 1, 0) - Y2)).ravel()
+
+    def W(self, U, s):
+        v = self.D(U) - s / self.beta
+        return np.sign(v) * np.maximum(np.abs(v) - 1.0 / self.beta, 0.0)
+
+    def L(self, U, W, s, dl):
+        V, r = self.D(U) - W, self.A @ U - self.bs
+        return (np.abs(W).sum() - s @ V + self.beta / 2 * V @ V
+                - dl @ r + self.mu / 2 * r @ r)
+
+    def d(self, U, W, s, dl):
+        V, r = self.D(U) - W, self.A @ U - self.bs
+        return (self.Dt(-s + self.beta * V) + self.A.T @ (-dl + self.mu * r)) / self.beta
+
+    def line_min(self, U, W, s, dl, d):
+        """exact minimiser of the quadratic t -> L(U - t d) from three values"""
+        h = 1e-3 * np.linalg.norm(U) / max(np.linalg.norm(d), 1e-300)
+        f0, f1, f2 = (self.L(U - t * d, W, s, dl) for t in (0.0, h, 2 * h))
+        return h * (3 * f0 - 4 * f1 + f2) / (2 * (f0 - 2 * f1 + f2))
+
+
+def _alm_setup(seed, c=1e-5):
+    q, p = 12, 10
+    A, x, b = _dense(q, p, 0.5, seed)
+    s = _scaleb(b)
+    bs = s * b.astype(np.float64)
+    opts = default_options(scal_double=1, ls_exact=1, c=c)
+    M = _ALM(A, bs, q, p, opts.mu, opts.beta)
+    return A, b, s, opts, M, (q, p, 1)
+
+
+def _run_f64(A, b, dims, opts, maxit, s, trace=0):
+    opts.maxit = maxit
+    U, info, tr = O.solve_f64(O.Op.dense(A), b.astype(np.float64), dims, opts,
+                              trace_cap=trace)
+    return U * s, info, tr
+
+
+# c = 0.6: at the exact line minimum of a quadratic the decrease is half the
+# linear slope, so alpha = 1 fails the rule and alpha = gamma passes
+@pytest.mark.parametrize("seed,c", [(21, 1e-5), (22, 1e-5), (23, 1e-5), (21, 0.6), (23, 0.6)])
+def test_first_step_bb_step_and_armijo(seed, c):
+    A, b, s, opts, M, dims = _alm_setup(seed, c)
+    n = dims[0] * dims[1]
+    z2, zm = np.zeros(2 * n), np.zeros(A.shape[0])
+    U0 = M.A.T @ M.bs
+    W0 = M.W(U0, z2)
+    d0 = M.d(U0, W0, z2, zm)
+    U1, _, tr = _run_f64(A, b, dims, opts, 1, s, trace=1)
+    t1 = tr[0]
+    assert not t1.fallback
+    if c > 0.5:
+        assert t1.backtracks == 1
+    # first step: exact minimiser along d
+    assert t1.tau == pytest.approx(M.line_min(U0, W0, z2, zm, d0), rel=1e-7)
+    # the accepted step is U0 - alpha tau d with alpha = gamma^k
+    assert t1.alpha == pytest.approx(opts.gamma ** t1.backtracks, rel=1e-12)
+    np.testing.assert_allclose(U1, U0 - t1.alpha * t1.tau * d0, rtol=1e-9,
+                               atol=1e-9 * np.abs(U0).max())
+    # nonmonotone Armijo acceptance (C = L at the start)
+    C0 = M.L(U0, W0, z2, zm)
+    slope = opts.c * t1.tau * opts.beta * (d0 @ d0)
+
+    def ok(al):
+        return abs(M.L(U0 - al * t1.tau * d0, W0, z2, zm)) <= abs(C0 - al * slope)
+    assert ok(t1.alpha)
+    if t1.backtracks:
+        assert not ok(t1.alpha / opts.gamma)
+    # f after the w-update is L at the new (u, w)
+    W1 = M.W(U1, z2)
+    assert t1.f == pytest.approx(M.L(U1, W1, z2, zm), rel=1e-9)
+    # second step: Barzilai-Borwein s's / s'y on the gradients d
+    d1 = M.d(U1, W1, z2, zm)
+    U2, _, tr2 = _run_f64(A, b, dims, opts, 2, s, trace=2)
+    sv, yv = U1 - U0, d1 - d0
+    assert tr2[1].tau == pytest.approx(abs(sv @ sv / (sv @ yv)), rel=1e-7)
+    if not tr2[1].fallback:
+        np.testing.assert_allclose(U2, U1 - tr2[1].alpha * tr2[1].tau * d1, rtol=1e-9,
+                                   atol=1e-9 * np.abs(U1).max())
+
+
+@pytest.mark.parametrize("seed", [24, 25])
+def test_multiplier_update_sets_next_outer_gradient(seed):
+    """After the first outer iteration s = -beta (DU - W), dl = -mu (AU - b);
+    the next outer iteration's first step is the exact minimiser of L with
+    those multipliers along d = grad_u L / beta."""
+    A, b, s, opts, M, dims = _alm_setup(seed)
+    _, info, tr = _run_f64(A, b, dims, opts, 80, s, trace=80)
+    k = next(i for i, t in enumerate(tr) if t.outer == 1)
+    assert k >= 2
+    Uk, _, _ = _run_f64(A, b, dims, opts, k, s)
+    n = dims[0] * dims[1]
+    Wk = M.W(Uk, np.zeros(2 * n))
+    sg = -M.beta * (M.D(Uk) - Wk)
+    dl = -M.mu * (M.A @ Uk - M.bs)
+    d = M.d(Uk, Wk, sg, dl)
+    assert tr[k].tau == pytest.approx(M.line_min(Uk, Wk, sg, dl, d), rel=1e-6)
+    assert tr[k - 1].outer == 0
+    # the step is taken along that d, and the w-update shrinks DU - s / beta:
+    # f after it is L(U, W; s, dl)
+    Uk1, _, _ = _run_f64(A, b, dims, opts, k + 1, s)
+    t = tr[k]
+    if not t.fallback:
+        np.testing.assert_allclose(Uk1, Uk - t.alpha * t.tau * d, rtol=1e-8,
+                                   atol=1e-8 * np.abs(Uk).max())
+    Wk1 = M.W(Uk1, sg)
+    assert t.f == pytest.approx(M.L(Uk1, Wk1, sg, dl), rel=1e-8)
+
+
+def test_nonneg_clamps_every_iterate():
+    A, x, b = _dense(16, 12, 0.4, 26)
+    opts = default_options(maxit=40, nonneg=1, scal_double=1)
+    U, info, _ = O.solve(O.Op.dense(A), b, (16, 12, 1), opts)
+    assert info.total_iters > 0 and U.min() >= 0.0
+    opts.nonneg = 0
+    U2, _, _ = O.solve(O.Op.dense(A), b, (16, 12, 1), opts)
+    assert U2.min() < 0.0  # the clamp is doing something on this problem
+
+
+@pytest.mark.parametrize("seed", [27, 28])
+def test_quadratic_model_line_search_matches_rounds(seed):
+    """With W fixed during the line search, f(alpha) is an exact quadratic in
+    alpha, so deciding all backtracking rounds from its coefficients
+    (ls_exact = 0) must take the same steps as evaluating every round
+    (ls_exact = 1); in FP64 the trajectories agree to rounding."""
+    A, x, b = _dense(16, 16, 0.4, seed)
+    op = O.Op.dense(A)
+    res = []
+    for ls in (1, 0):
+        opts = default_options(maxit=60, scal_double=1, ls_exact=ls, c=0.6)
+        U, info, tr = O.solve_f64(op, b.astype(np.float64), (16, 16, 1), opts, trace_cap=60)
+        res.append((U, info, tr))
+    (U1, i1, t1), (U0, i0, t0) = res
+    assert sum(t.backtracks for t in t1) > 5
+    assert [t.backtracks for t in t1] == [t.backtracks for t in t0]
+    assert i1.total_iters == i0.total_iters
+    assert np.linalg.norm(U1 - U0) / np.linalg.norm(U1) < 1e-10
+
+
+@pytest.mark.parametrize("seed", [23, 24])
+def test_five_strike_fallback(seed):
+    """c = 0.97 on these problems: the first step of the second outer
+    iteration fails the rule five times (the rule needs alpha <= 2 (1 - c) =
+    0.06 at the exact line minimum; gamma^5 = 0.078).  The fallback takes the
+    full first-step tau along d, and the nonmonotone weight gam is multiplied
+    by rate_gam in that step's C average (Q restarts at 1, C at L)."""
+    A, b, s, opts, M, dims = _alm_setup(seed, c=0.97)
+    _, _, tr = _run_f64(A, b, dims, opts, 80, s, trace=80)
+    k = next(i for i, t in enumerate(tr) if t.outer == 1)
+    t = tr[k]
+    assert t.fallback == 1
+    Uk, _, _ = _run_f64(A, b, dims, opts, k, s)
+    n = dims[0] * dims[1]
+    Wk = M.W(Uk, np.zeros(2 * n))
+    sg = -M.beta * (M.D(Uk) - Wk)
+    dl = -M.mu * (M.A @ Uk - M.bs)
+    d = M.d(Uk, Wk, sg, dl)
+    tau = M.line_min(Uk, Wk, sg, dl, d)
+    assert t.tau == pytest.approx(tau, rel=1e-6)
+    Uk1, _, _ = _run_f64(A, b, dims, opts, k + 1, s)
+    np.testing.assert_allclose(Uk1, Uk - tau * d, rtol=1e-8, atol=1e-8 * np.abs(Uk).max())
+    C0 = M.L(Uk, Wk, sg, dl)
+    g = opts.gam * opts.rate_gam
+    assert t.C == pytest.approx((g * C0 + t.f) / (g + 1), rel=1e-8)
