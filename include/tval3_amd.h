@@ -229,6 +229,25 @@ int tv3_solve_batch(tv3_ctx *ctx, int B, const tv3_plan *const *A,
                     const float *const *b, float *const *U,
                     const float *const *Ut, tv3_info *infos);
 
+/* ---- differentiation: K unrolled inner iterations and their adjoint ------- */
+/* No single reference counterpart: the map is K inner iterations of the first
+ * outer iteration of tval3_cpu_2d (2d/tval3_cpu_2d.cpp:752-830; min_u 526-576,
+ * update_W 651-671, get_gradient 578-585) with no stopping test, sigma = delta
+ * = 0, and the line-search decisions held fixed: alpha[k] is the step factor
+ * of step k+1 and first_step[k] = 1 where that step used the first-step tau
+ * (step 1, or a 5-strike fallback with alpha = 1).  A forward solve with
+ * maxit = K, tol_inn = 0 and a trace gives both.  2-D, anisotropic TV,
+ * without nonneg.  U = u_K / scl (device, dim_x * dim_y floats).  The tape
+ * keeps checkpoints every ckpt steps; backward returns dL/db (device, m
+ * floats) and dL/dmu (host) for the cotangent dU = dL/dU. */
+typedef struct tv3_tape tv3_tape;
+int tv3_unrolled_forward(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+                         int dim_x, int dim_y, int K, int ckpt, const double *alpha,
+                         const int *first_step, const float *b, float *U, tv3_tape **tape);
+int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *tape, const float *dU, float *db,
+                          double *dmu);
+void tv3_tape_destroy(tv3_tape *tape);
+
 #ifdef __cplusplus
 }
 #endif

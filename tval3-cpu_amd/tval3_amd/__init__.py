@@ -77,6 +77,14 @@ def lib():
         "tv3_nrm2": (_I, [_VP, _LL, _VP, C.POINTER(C.c_double)]),
         "tv3_solve": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
         "tv3_solve_batch": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+        "tv3_mem_alloc": (_I, [_VP, C.c_size_t, C.POINTER(_VP)]),
+        "tv3_mem_free": (_I, [_VP, _VP]),
+        "tv3_mem_upload": (_I, [_VP, _VP, _VP, C.c_size_t]),
+        "tv3_mem_download": (_I, [_VP, _VP, _VP, C.c_size_t]),
+        "tv3_unrolled_forward": (_I, [_VP, _VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP, _VP,
+                                      C.POINTER(_VP)]),
+        "tv3_unrolled_backward": (_I, [_VP, _VP, _VP, _VP, C.POINTER(C.c_double)]),
+        "tv3_tape_destroy": (None, [_VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -271,6 +279,36 @@ class Context:
         n_tr = min(info.total_iters, trace_cap)
         return U, info, ([trace[i] for i in range(n_tr)] if trace else [])
 
+    # ---- K unrolled iterations and their adjoint (differentiation harness) ----
+    def unrolled_forward(self, plan, b, dims, opts, K, ckpt, alpha, first):
+        """U = u_K / scl of the unrolled map (tv3_unrolled_forward) and its
+        tape.  alpha / first: the K step factors and first-step flags."""
+        t = self.torch
+        q, p = dims[0], dims[1]
+        b = self.tensor(b, t.float32)
+        if b.numel() != plan.m:
+            raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y!")
+        al = np.ascontiguousarray(alpha, np.float64)
+        fs = np.ascontiguousarray(first, np.int32)
+        if al.size != K or fs.size != K:
+            raise Tval3Error(1, "Argument error: alpha and first_step need K entries")
+        U = t.empty(q * p, dtype=t.float32, device=self.device)
+        h = _VP()
+        self._call(lib().tv3_unrolled_forward, self.h, plan.h, C.byref(opts), q, p,
+                   int(K), int(ckpt), al.ctypes.data_as(_VP), fs.ctypes.data_as(_VP),
+                   _ptr(b), _ptr(U), C.byref(h))
+        return U, Tape(self, h, plan.m)
+
+    def unrolled_backward(self, tape, dU):
+        """(dL/db tensor, dL/dmu float) for the cotangent dU."""
+        t = self.torch
+        dU = self.tensor(dU, t.float32)
+        db = t.empty(tape.m, dtype=t.float32, device=self.device)
+        dmu = C.c_double()
+        self._call(lib().tv3_unrolled_backward, self.h, tape.h, _ptr(dU), _ptr(db),
+                   C.byref(dmu))
+        return db, dmu.value
+
     def solve_batch(self, plans, bs, dims, opts=None, Uts=None):
         """B reconstructions in one batched launch sequence.  Returns
         (list of U tensors, list of Info)."""
@@ -328,6 +366,24 @@ class Plan:
     def close(self):
         if getattr(self, "h", None):
             lib().tv3_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tape:
+    """Checkpoints of one unrolled forward pass (tv3_tape)."""
+
+    def __init__(self, ctx, h, m):
+        self.ctx, self.h, self.m = ctx, h, m
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tv3_tape_destroy(self.h)
             self.h = None
 
     def __del__(self):
