@@ -18,6 +18,8 @@ Workloads (--workload):
   hemisphere   1024 Fibonacci transducers on the lower hemisphere of a 128^3
                volume, 523,776 rays, 3-D anisotropic TV, maxit 100, one solve.
   dense        64x64 image, m = 1229, dense FP32 A, isotropic TV, maxit 300.
+  diff         gradients of |U_K - x|^2 in b and mu through K = 20 unrolled
+               iterations (checkpoints every 5), 128-transducer ring on 128x128.
 
 value = reconstructions per second of the timed steps (inputs already in HBM).
 The CPU oracle (the C++/OpenMP restatement of the reference) is timed on the
@@ -288,6 +290,81 @@ def run_dense_batch(args, ctx):
     }
 
 
+def run_diff(args, ctx):
+    """Gradient of |U_K - x|^2 with respect to b and mu through K = 20 unrolled
+    iterations (checkpoints every 5) on the 128-transducer ring, 128x128."""
+    import numpy as np
+    import torch
+    from tval3_amd import diff, synth
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    G, Kt, K, ck = 128, 128, 20, 5
+    geo = synth.ring_geometry(Kt, G)
+    plan = ctx.plan_geometry(geo)
+    x = synth.disk_phantom(G, seed=0)
+    y = ctx.gemv(plan, x).cpu().numpy()
+    b = ctx.tensor(synth.add_noise(y, 0.005, seed=0))
+    xt = ctx.tensor(x)
+    mu = 256.0
+    alpha, first = diff.record(ctx, plan, b, (G, G), mu, K)
+
+    def grad_once():
+        bt = b.clone().requires_grad_(True)
+        mt = torch.tensor(mu, dtype=torch.float64, device=b.device, requires_grad=True)
+        U = diff.unrolled(ctx, plan, bt, mt, (G, G), K=K, ckpt=ck, alpha=alpha, first=first)
+        L = ((U - xt) ** 2).sum()
+        L.backward()
+        return bt.grad, mt.grad
+
+    for _ in range(args.warmup):
+        grad_once()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grad_once()
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ops, roof = roofline(prof, {"A": "k_fwd_w1", "At": "k_bwd_w1"}, "diff")
+    roof["cache_resident"] = True
+    roof["csr_equivalent"] = True
+    cpu = None
+    if not args.no_cpu:
+        import diff_ref as R
+        import tv3_oracle as O
+        op = O.Op.geometry(geo)
+        ptr, lin = op.walk()
+        A, AT = R.sparse_from_walk(ptr, lin, geo.m, geo.n)
+        bh = b.cpu().numpy().astype(np.float64)
+        torch.set_num_threads(len(os.sched_getaffinity(0)))
+
+        def cpu_grad():
+            bt = torch.tensor(bh, requires_grad=True)
+            mt = torch.tensor(mu, dtype=torch.float64, requires_grad=True)
+            U = R.unrolled(A, AT, bt, mt, 32.0, G, G, K, alpha, first)
+            ((U - torch.from_numpy(x.astype(np.float64))) ** 2).sum().backward()
+
+        runs, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < args.cpu_seconds and runs < 50:
+            cpu_grad()
+            runs += 1
+        el = time.perf_counter() - t1
+        cpu = {"value": runs / el, "unit": "gradients/s", "cores": torch.get_num_threads(),
+               "kind": "port",
+               "sample": f"{runs} FP64 torch-autograd gradients of the same unrolled map "
+                         f"on the CPU ({el:.1f} s)"}
+    return {
+        "value": args.steps / t_steps, "unit": "gradients/s", "steps": args.steps,
+        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "f32",
+        "data": "synthetic: disk phantom, 0.5% noise",
+        "config": {"workload": f"d|U_K - x|^2 / d(b, mu), K={K} unrolled iterations, "
+                               f"checkpoints every {ck}, ring {Kt} transducers on {G}x{G}",
+                   "rays": plan.m, "nnz": plan.nnz, "K": K, "ckpt": ck},
+        "operators": ops, "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
 def run_single(args, ctx, workload):
     import numpy as np
     import torch
@@ -359,7 +436,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ring",
-                    choices=["ring", "dense-batch", "hemisphere", "dense"])
+                    choices=["ring", "dense-batch", "hemisphere", "dense", "diff"])
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -394,6 +471,8 @@ def main():
         out = run_ring(args, ctx)
     elif args.workload == "dense-batch":
         out = run_dense_batch(args, ctx)
+    elif args.workload == "diff":
+        out = run_diff(args, ctx)
     else:
         out = run_single(args, ctx, args.workload)
     if args.copy_mb:

@@ -201,10 +201,16 @@ struct tv3_tape {
   int err() { return hip_check(ctx, hipGetLastError(), "adjoint launch"); }
 
   void fwdA(const float *x, float *y) {
+    prof_begin(ctx, 0);
     launch_fwd<1>(st, G, 1, m, A->dev.kind, A->dev.dtype, x, y, S, 0);
+    prof_end(ctx);
+    prof_count(ctx, G, 1, 1, S, 0, 0);
   }
   void bwdA(const float *x, float *y) {
+    prof_begin(ctx, 1);
     launch_bwd<1>(st, G, 1, n, A->dev.kind, A->dev.dtype, x, y, S, 0, nullptr, (float *)dense);
+    prof_end(ctx);
+    prof_count(ctx, G, 1, 1, S, 0, 1);
   }
   int host_sum(int slots, double *out) {
     std::vector<double> h((size_t)slots * kNblkMax);

@@ -253,6 +253,167 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64(const GroupDesc *G,
   if (on) y[gd.off_m + (long long)row * 64 + lane] = (float)acc;
 }
 
+// W = 64, on-the-fly projector, 16-byte gathers.  One wavefront per ray;
+// lane l owns instances 4c .. 4c+3 (c = l % 16) and quarter q = l / 16 takes
+// the walk steps s = 4t + q of every 64-step chunk, so one load instruction
+// fetches four voxels' 256-byte rows (a 64-lane 4-byte gather fetches one).
+// Each lane computes the voxel of one step of the chunk by the closed form;
+// quarters read theirs with a lane shuffle.  The four quarter sums are
+// combined in a fixed order; FP64 sums of float values, as in k_fwd_w64.
+constexpr int kPipeQ = 4;
+
+// TV3_PROJ_VEC=0 selects the 4-byte-gather kernels (k_fwd_w64 / k_bwd_w64)
+// for the projector; both give the same results
+bool proj_vec() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_PROJ_VEC");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return v != 0;
+}
+
+__device__ __forceinline__ double quarter_sum(double v) {
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
+                                                     const float *x, float *y,
+                                                     const InstScal *S,
+                                                     uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, c = lane & 15;
+  const int row = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+  if (row >= gd.m) return;
+  const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
+  bool on[4];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    on[k] = !mask || (Sg[k].flags & mask) == mask;
+    any = any || on[k];
+  }
+  if (!__any(any)) return;
+  const PlanDev &P = gd.plan;
+  const RayDesc rd = P.rays[row];
+  const int L = uni_i(rd.L);
+  const float sc = P.scale;
+  const bool unit = sc == 1.0f;
+  const float *xb = x + gd.off_n + 4 * c;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int j0 = 0; j0 <= L; j0 += 64) {
+    const int jl = j0 + lane;
+    const int myidx = jl <= L ? ray_lin(rd, jl) : 0;
+    const int cnt = min(64, L + 1 - j0);
+    const int tmax = (cnt + 3) >> 2;
+    for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
+      float4 v[kPipeQ];
+#pragma unroll
+      for (int u = 0; u < kPipeQ; ++u) {
+        const int st = 4 * (t0 + u) + q;
+        const int iu = __shfl(myidx, st & 63, 64);
+        v[u] = st < cnt ? *(const float4 *)(xb + (long long)iu * 64) : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kPipeQ; ++u) {
+        if (4 * (t0 + u) + q < cnt) {
+          if (unit) {
+            a0 += (double)v[u].x; a1 += (double)v[u].y;
+            a2 += (double)v[u].z; a3 += (double)v[u].w;
+          } else {
+            a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
+            a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
+          }
+        }
+      }
+    }
+  }
+  a0 = quarter_sum(a0); a1 = quarter_sum(a1);
+  a2 = quarter_sum(a2); a3 = quarter_sum(a3);
+  if (q == 0) {
+    float *yo = y + gd.off_m + (long long)row * 64 + 4 * c;
+    if (on[0] && on[1] && on[2] && on[3]) {
+      *(float4 *)yo = make_float4((float)a0, (float)a1, (float)a2, (float)a3);
+    } else {
+      if (on[0]) yo[0] = (float)a0;
+      if (on[1]) yo[1] = (float)a1;
+      if (on[2]) yo[2] = (float)a2;
+      if (on[3]) yo[3] = (float)a3;
+    }
+  }
+}
+
+// A^T of the on-the-fly projector with 16-byte gathers: one wavefront per
+// voxel over its ascending ray list, quarters take list entries 4t + q of
+// each 64-entry chunk, lanes own four instances (as k_fwd_w64q).
+__global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
+                                                     const float *x, float *y,
+                                                     const InstScal *S,
+                                                     uint32_t mask,
+                                                     const float *sub) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, c = lane & 15;
+  const int col = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+  if (col >= gd.n) return;
+  const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
+  bool on[4];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    on[k] = !mask || (Sg[k].flags & mask) == mask;
+    any = any || on[k];
+  }
+  if (!__any(any)) return;
+  const PlanDev &P = gd.plan;
+  const float sc = P.scale;
+  const bool unit = sc == 1.0f;
+  const float *xb = x + gd.off_m + 4 * c;
+  const int k0 = uni_i(P.g_cptr[col]), k1 = uni_i(P.g_cptr[col + 1]);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int cnt = min(64, k1 - kb);
+    const int id = lane < cnt ? P.g_cray[kb + lane] : 0;
+    const int tmax = (cnt + 3) >> 2;
+    for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
+      float4 v[kPipeQ];
+#pragma unroll
+      for (int u = 0; u < kPipeQ; ++u) {
+        const int st = 4 * (t0 + u) + q;
+        const int iu = __shfl(id, st & 63, 64);
+        v[u] = st < cnt ? *(const float4 *)(xb + (long long)iu * 64) : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kPipeQ; ++u) {
+        if (4 * (t0 + u) + q < cnt) {
+          if (unit) {
+            a0 += (double)v[u].x; a1 += (double)v[u].y;
+            a2 += (double)v[u].z; a3 += (double)v[u].w;
+          } else {
+            a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
+            a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
+          }
+        }
+      }
+    }
+  }
+  a0 = quarter_sum(a0); a1 = quarter_sum(a1);
+  a2 = quarter_sum(a2); a3 = quarter_sum(a3);
+  if (q == 0) {
+    const long long o = gd.off_n + (long long)col * 64 + 4 * c;
+    float r[4] = {(float)a0, (float)a1, (float)a2, (float)a3};
+    if (sub) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[k] = r[k] - sub[o + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (on[k]) y[o + k] = r[k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // transposed product: y[col] = sum_k A[k, col] x[k] (- sub[col])
 // ---------------------------------------------------------------------------
@@ -658,11 +819,12 @@ template <>
 void launch_fwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                     int kind, int dtype, const float *x, float *y,
                     const InstScal *S, uint32_t mask) {
-  (void)kind;
   (void)dtype;
   dim3 grid((max_m + 3) / 4, G);
-  hipLaunchKernelGGL(k_fwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
-                     mask);
+  if (kind == kGeometry && proj_vec())
+    hipLaunchKernelGGL(k_fwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+  else
+    hipLaunchKernelGGL(k_fwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
 }
 
 template <>
@@ -695,12 +857,13 @@ void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                     int kind, int dtype, const float *x, float *y,
                     const InstScal *S, uint32_t mask, const float *sub,
                     float *scratch) {
-  (void)kind;
   (void)dtype;
   (void)scratch;
   dim3 grid((max_n + 3) / 4, G);
-  hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S,
-                     mask, sub);
+  if (kind == kGeometry && proj_vec())
+    hipLaunchKernelGGL(k_bwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
+  else
+    hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
 }
 
 template <>

@@ -1216,6 +1216,52 @@ __global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
 // ===========================================================================
 // host driver
 // ===========================================================================
+// ---- lane compaction for sparse masked operator launches -------------------
+// A 64-lane group pays for a whole walk even when one lane computes.  When a
+// masked A / A^T product selects only a few lanes (the 5-strike fallback
+// usually does), their vectors are gathered out of the interleaved layout,
+// multiplied by the W = 1 kernels and scattered back.  Every walk sum is an
+// FP64 sum of float values in both layouts, so the results are the same.
+constexpr int kLaneMax = 16;
+struct LaneList {
+  int n;
+  int inst[kLaneMax];
+};
+
+// xc[k * pitch + e] = x[off + e * 64 + lane] for the k-th listed instance
+__global__ __launch_bounds__(kBlock) void k_lane_gather(const GroupDesc *G, const float *x,
+                                                        float *xc, LaneList L, int len,
+                                                        int use_m, long long pitch) {
+  const int k = blockIdx.y;
+  if (k >= L.n) return;
+  const int inst = L.inst[k];
+  const GroupDesc &gd = G[inst / 64];
+  const int b = inst % 64;
+  const long long off = use_m ? gd.off_m : gd.off_n;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < len; e += gridDim.x * kBlock)
+    xc[k * pitch + e] = x[off + (long long)e * 64 + b];
+}
+
+// y[off + e * 64 + lane] = yc[k * pitch + e] (- sub) for listed instances
+// whose flags pass the launch's mask
+__global__ __launch_bounds__(kBlock) void k_lane_scatter(const GroupDesc *G, const InstScal *S,
+                                                         const float *yc, float *y, LaneList L,
+                                                         int len, int use_m, long long pitch,
+                                                         uint32_t mask, const float *sub) {
+  const int k = blockIdx.y;
+  if (k >= L.n) return;
+  const int inst = L.inst[k];
+  if (mask && (S[inst].flags & mask) != mask) return;
+  const GroupDesc &gd = G[inst / 64];
+  const int b = inst % 64;
+  const long long off = use_m ? gd.off_m : gd.off_n;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < len; e += gridDim.x * kBlock) {
+    const long long o = off + (long long)e * 64 + b;
+    const float v = yc[k * pitch + e];
+    y[o] = sub ? v - sub[o] : v;
+  }
+}
+
 struct Batch {
   int B = 0, G = 0, W = 1, Bpad = 0;
   std::vector<GroupDesc> groups;
@@ -1259,6 +1305,12 @@ class Engine {
     need += align_up(sizeof(float *) * bt_.Bpad) * 2;
     if (bt_.kind == kDense && W == 1)
       need += align_up(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n);
+    compact_ok_ = (W == 64 && bt_.kind == kGeometry);
+    pitch_ = ((long long)std::max(bt_.max_n, bt_.max_m) + 63) / 64 * 64;
+    if (compact_ok_) {
+      need += align_up(sizeof(GroupDesc) * kLaneMax);
+      need += align_up(sizeof(float) * (size_t)kLaneMax * pitch_) * 2;
+    }
     if (ctx_->ws_bytes < need) {
       if (ctx_->ws) {
         (void)hipStreamSynchronize(s_);
@@ -1296,6 +1348,16 @@ class Engine {
     ptrs_in_ = (const float **)take(sizeof(float *) * bt_.Bpad);
     ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
     scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n) : nullptr;
+    if (compact_ok_) {
+      Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * kLaneMax);
+      xc_ = (float *)take(sizeof(float) * (size_t)kLaneMax * pitch_);
+      yc_ = (float *)take(sizeof(float) * (size_t)kLaneMax * pitch_);
+      gc_host_.assign(kLaneMax, bt_.groups[0]);
+      for (int k = 0; k < kLaneMax; ++k) {
+        gc_host_[k].off_n = k * pitch_;
+        gc_host_[k].off_m = k * pitch_;
+      }
+    }
     if (ctx_->hflags_n < (size_t)bt_.Bpad) {
       if (ctx_->hflags) {
         (void)hipStreamSynchronize(s_);
@@ -1319,6 +1381,8 @@ class Engine {
     int rc = hip_check(ctx_, hipMemsetAsync(ctx_->ws, 0, (size_t)(p - (char *)ctx_->ws), s_), "workspace clear");
     if (rc) return rc;
     rc = hip_check(ctx_, hipMemcpyAsync(G_, bt_.groups.data(), sizeof(GroupDesc) * bt_.G, hipMemcpyHostToDevice, s_), "group copy");
+    if (!rc && compact_ok_)
+      rc = hip_check(ctx_, hipMemcpyAsync(Gc_, gc_host_.data(), sizeof(GroupDesc) * kLaneMax, hipMemcpyHostToDevice, s_), "group copy");
     return rc;
   }
 
@@ -1358,6 +1422,13 @@ class Engine {
   const float **ptrs_in_ = nullptr;
   float **ptrs_out_ = nullptr;
   float *scratch_ = nullptr;
+  // lane compaction (W = 64, on-the-fly projector)
+  bool compact_ok_ = false, compact_ = false;
+  LaneList lanes_{};
+  GroupDesc *Gc_ = nullptr;
+  float *xc_ = nullptr, *yc_ = nullptr;
+  long long pitch_ = 0;
+  std::vector<GroupDesc> gc_host_;
   tv3_trace_rec *trace_ = nullptr;
   int trace_cap_ = 0;
 
@@ -1375,16 +1446,45 @@ class Engine {
   dim3 gi() const { return dim3(bt_.Bpad); }
 
   void fwd(const float *x, float *y, uint32_t mask) {
+    if (compact_) {
+      const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
+      const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
+      hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
+      launch_fwd<1>(s_, Gc_, lanes_.n, bt_.max_m, bt_.kind, bt_.dtype, xc_, yc_, S_, 0);
+      hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const float *)nullptr);
+      return;
+    }
     prof_begin(ctx_, 0);
     launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
     prof_end(ctx_);
     prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 0);
   }
   void bwd(const float *x, float *y, uint32_t mask, const float *sub) {
+    if (compact_) {
+      const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
+      const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
+      hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
+      launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, S_, 0, nullptr, nullptr);
+      hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
+      return;
+    }
     prof_begin(ctx_, 1);
     launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_);
     prof_end(ctx_);
     prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 1);
+  }
+
+  // instances whose published flags (last read_summary) contain all bits of
+  // req, into lanes_; false when there are more than kLaneMax
+  bool collect_lanes(uint32_t req) {
+    const volatile uint32_t *h = ctx_->hflags;
+    lanes_.n = 0;
+    for (int i = 0; i < bt_.Bpad; ++i) {
+      if ((h[i] & req) != req) continue;
+      if (lanes_.n == kLaneMax) return false;
+      lanes_.inst[lanes_.n++] = i;
+    }
+    return lanes_.n > 0;
   }
 
   // OR of the instance flags the fin kernels published to mapped host memory
@@ -1486,11 +1586,13 @@ class Engine {
       if (sum & F_FB) {
         // 5-strike fallback (2d:561-567): tau from the first-iteration
         // formula, U = Up, descend(force)
+        compact_ = compact_ok_ && collect_lanes(F_FB);
         tau_first(F_FB, 0);
         hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
         hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_,
                            O_.nonneg, O_.dt_exact, 0);
         get_g_products(F_FB, 0, 1);
+        compact_ = false;
         hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
       }
       // ---- update_W, reference values, gradient, stop test ----
