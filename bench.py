@@ -115,7 +115,7 @@ def run_ring(args, ctx):
     bs_host = [synth.add_noise(y, 0.005, seed=i) for i in range(B)]
     bs = [ctx.tensor(b) for b in bs_host]
     dims = (G, G, 1)
-    opts = tval3_amd.default_options(maxit=args.maxit)
+    opts = tval3_amd.default_options(maxit=args.maxit, graph=args.graph)
     torch.cuda.synchronize()
 
     # plain FP32 path: one reconstruction at a time (W = 1 kernels)
@@ -227,7 +227,7 @@ def run_dense_batch(args, ctx):
     log(f"dense-batch: {B} instances, {tot_elems / 1e9:.2f} G bf16 entries "
         f"({2 * tot_elems / 2**30:.1f} GiB)")
     # FP32 vectors, FP64 step-size / line-search scalars
-    opts = tval3_amd.default_options(maxit=args.maxit, scal_double=1)
+    opts = tval3_amd.default_options(maxit=args.maxit, scal_double=1, graph=args.graph)
     for _ in range(args.warmup):
         ctx.solve_batch(plans, bs, dims, opts)
     torch.cuda.synchronize()
@@ -380,7 +380,7 @@ def run_single(args, ctx, workload):
         b = synth.add_noise(y, 0.005, seed=0)
         dims = (grid, grid, grid)
         maxit = args.maxit if args.maxit_set else 100
-        opts = tval3_amd.default_options(maxit=maxit)
+        opts = tval3_amd.default_options(maxit=maxit, graph=args.graph)
         kern = {"A": "k_fwd_w1", "At": "k_bwd_w1"}
         op_fn = lambda: O.Op.geometry(geo)  # noqa: E731
         desc = f"3D USCT hemisphere {grid}^3, {plan.m} rays, maxit {maxit}"
@@ -393,7 +393,7 @@ def run_single(args, ctx, workload):
         plan = ctx.plan_dense(A)
         dims = (q, p, 1)
         maxit = args.maxit if args.maxit_set else 300
-        opts = tval3_amd.default_options(maxit=maxit, tv_iso=1)
+        opts = tval3_amd.default_options(maxit=maxit, tv_iso=1, graph=args.graph)
         kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}
         op_fn = lambda: O.Op.dense(A)  # noqa: E731
         desc = f"2D dense 64x64, m={m}, isotropic TV, maxit {maxit}"
@@ -446,6 +446,8 @@ def main():
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--maxit", type=int, default=None)
     ap.add_argument("--f64-sample", type=int, default=4)
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="1: replay each pair of solver steps from a hipGraph")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")

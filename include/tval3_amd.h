@@ -60,7 +60,7 @@ extern "C" {
 
 /* Solver options: the 18 fields of the reference's tval3_options<T>
  * (used at 2d/tval3_cpu_2d.cpp:163-192, 540-569, 769-852), stored as double.
- * The last three fields are engine settings with no reference counterpart. */
+ * The last five fields are engine settings with no reference counterpart. */
 typedef struct tv3_options {
   double mu, beta;            /* final penalty parameters */
   double mu0, beta0;          /* initial penalty parameters (continuation) */
@@ -90,6 +90,12 @@ typedef struct tv3_options {
                                  the quadratic coefficients gathered by the
                                  descent pass, then one interpolation pass
                                  that recomputes the exact lam2..lam5 and f */
+  int graph;                  /* 0: the host launches each step's kernels and
+                                 skips branches no instance takes; 1: every
+                                 step issues all (flag-masked) launches, two
+                                 steps are captured into a hipGraph once and
+                                 replayed, and the host only checks for
+                                 termination.  Same results. */
 } tv3_options;
 
 /* Result record: the reference's tval3_info<T> (2d/tval3_cpu_2d.cpp:865-876). */

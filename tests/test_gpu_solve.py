@@ -220,3 +220,61 @@ def test_batch_bf16_dense_uneven(ctx):
         U64, _, _ = O.solve_f64(op, b, dims[k], opts, Ut=x)
         print(f"instance {k}: rel L2 vs FP64 solve {_rel(Ug, U64):.2e}")
         assert infos[k].rel_error == pytest.approx(info_o.rel_error, rel=1e-3)
+
+
+# ---- graph mode: the same launches, captured once per two steps --------------
+def _same(a, b):
+    return np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+@pytest.mark.parametrize("ls", [0, 1])
+def test_graph_mode_plain_ring(ctx, ls):
+    """Uniform masked steps replayed from a hipGraph give the host-driven
+    result bit for bit (every replay feeds the next, so the final iterate
+    checks them all) and match the oracle's trajectory."""
+    geo, op, x, b = _ring_case(48, 48, 5)
+    plan = ctx.plan_geometry(geo)
+    o_host = tval3_amd.default_options(maxit=120, ls_exact=ls)
+    o_graph = tval3_amd.default_options(maxit=120, ls_exact=ls, graph=1)
+    U0, i0, t0 = ctx.solve(plan, b, (48, 48, 1), o_host, Ut=x, trace_cap=120)
+    for rep in range(2):  # two solves in a row: a fresh capture each time
+        U1, i1, t1 = ctx.solve(plan, b, (48, 48, 1), o_graph, Ut=x, trace_cap=120)
+        assert _same(U0, U1), rep
+        assert (i1.total_iters, i1.outer_iters) == (i0.total_iters, i0.outer_iters)
+        assert [(t.backtracks, t.fallback, t.tau) for t in t1] == \
+               [(t.backtracks, t.fallback, t.tau) for t in t0]
+    _compare(ctx, plan, op, b, (48, 48, 1), o_graph, Ut=x, strict_traj=(ls == 1))
+
+
+def test_graph_mode_shared_batch(ctx):
+    geo, op, x, _ = _ring_case(48, 48, 5)
+    plan = ctx.plan_geometry(geo)
+    B = 70
+    bs = [synth.add_noise(op.gemv(x), 0.005, 300 + i) for i in range(B)]
+    res = []
+    for g in (0, 1):
+        opts = tval3_amd.default_options(maxit=60, ls_exact=1, graph=g)
+        res.append(ctx.solve_batch([plan] * B, bs, [(48, 48, 1)] * B, opts))
+    (Ua, ia), (Ub, ib) = res
+    for k in range(B):
+        assert _same(Ua[k], Ub[k]), k
+        assert ia[k].total_iters == ib[k].total_iters
+    Uo, info_o, _ = O.solve(op, bs[5], (48, 48, 1), tval3_amd.default_options(maxit=60, ls_exact=1))
+    assert _rel(Ub[5].cpu().numpy(), Uo) <= TOL_U
+
+
+def test_graph_mode_dense_bf16_batch_and_continuation(ctx):
+    import torch
+    sizes = [(24, 24, 0.3), (20, 28, 0.4), (32, 32, 0.25)]
+    plans, bs, dims = [], [], []
+    for k, (q, p, fr) in enumerate(sizes):
+        A, x, b = _dense_case(q, p, fr, 60 + k)
+        plans.append(ctx.plan_dense(torch.from_numpy(A).to(torch.bfloat16), bf16=True))
+        bs.append(b)
+        dims.append((q, p, 1))
+    kw = dict(maxit=150, mu0=16.0, beta0=4.0, nonneg=1, scal_double=1)
+    Ua, ia = ctx.solve_batch(plans, bs, dims, tval3_amd.default_options(**kw))
+    Ub, ib = ctx.solve_batch(plans, bs, dims, tval3_amd.default_options(graph=1, **kw))
+    for k in range(len(sizes)):
+        assert _same(Ua[k], Ub[k]), k
+        assert (ia[k].total_iters, ia[k].outer_iters) == (ib[k].total_iters, ib[k].outer_iters)

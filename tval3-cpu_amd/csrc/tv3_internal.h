@@ -165,6 +165,17 @@ int set_err(tv3_ctx *ctx, int code, const std::string &msg);
 void prof_begin(tv3_ctx *ctx, int kind);
 void prof_end(tv3_ctx *ctx);
 void prof_collect(tv3_ctx *ctx);
+// dense W = 1 launch shape: rows per wavefront of A x (1, 2, 4) and rows
+// loaded ahead in A^T x (1, 2, 4).  Neither changes any summation order.
+// The current setting is per host thread; the solver sets it from its
+// autotuner for the duration of a solve.
+struct DenseTune {
+  int rows;
+  int unroll;
+};
+void dense_tune_set(DenseTune t);
+DenseTune dense_tune_cur();
+
 // adds the algorithmic bytes of the lanes a masked A (trans 0) / A^T (trans 1)
 // launch computed to ctx->prof_bytes[trans]; W lanes per group, Bpad lanes
 void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,

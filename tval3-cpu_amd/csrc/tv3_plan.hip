@@ -262,6 +262,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64(const GroupDesc *G,
 // combined in a fixed order; FP64 sums of float values, as in k_fwd_w64.
 constexpr int kPipeQ = 4;
 
+namespace {
+thread_local DenseTune g_dense_tune{2, 2};
+}
+void dense_tune_set(DenseTune t) { g_dense_tune = t; }
+DenseTune dense_tune_cur() { return g_dense_tune; }
+
 // TV3_PROJ_VEC=0 selects the 4-byte-gather kernels (k_fwd_w64 / k_bwd_w64)
 // for the projector; both give the same results
 bool proj_vec() {
@@ -706,10 +712,12 @@ __device__ __forceinline__ void load8(const __hip_bfloat16 *p, float (&v)[8]) {
   }
 }
 
-// y = A x: one wavefront per row; with n a multiple of 8 every lane loads 8
-// consecutive entries (16 bytes of bf16, 32 of f32) and the matching 8 x
-// values per step; FP64 lane sums, fixed butterfly.
-template <class T>
+// y = A x: one wavefront per R consecutive rows; with n a multiple of 8 every
+// lane loads 8 consecutive entries of each row (16 bytes of bf16, 32 of f32)
+// and the matching 8 x values once for all R rows per step; FP64 lane sums,
+// fixed butterfly.  Each row's summation order is the same for every R, so
+// R is a pure speed choice (picked by the autotuner).
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
                                                          const float *x,
                                                          float *y,
@@ -717,28 +725,45 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
                                                          uint32_t mask) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (row >= gd.m) return;
+  const int row0 = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * R;
+  if (row0 >= gd.m) return;
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const int n = gd.n;
-  const T *A = (const T *)gd.plan.A + (long long)row * n;
+  const int nr = min(R, gd.m - row0);
+  const T *A = (const T *)gd.plan.A + (long long)row0 * n;
   const float *xg = x + gd.off_n;
-  double acc = 0.0;
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
   // rows start 16-byte aligned when n % 8 == 0; x starts 32-byte aligned
   // (group offsets are multiples of 64 floats)
   if (gd.plan.a_vec) {
     for (int j = lane * 8; j < n; j += 512) {
-      float a[8], v[8];
-      load8(A + j, a);
+      float v[8];
       load8(xg + j, v);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc += (double)a[u] * (double)v[u];
+      for (int r = 0; r < R; ++r) {
+        if (r < nr) {
+          float a[8];
+          load8(A + (long long)r * n + j, a);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc[r] += (double)a[u] * (double)v[u];
+        }
+      }
     }
   } else {
-    for (int j = lane; j < n; j += 64) acc += (double)a_val(A, j) * (double)xg[j];
+    for (int j = lane; j < n; j += 64) {
+      const double xv = (double)xg[j];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r < nr) acc[r] += (double)a_val(A + (long long)r * n, j) * xv;
+    }
   }
-  acc = wave_sum(acc);
-  if (lane == 0) y[gd.off_m + row] = (float)acc;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double t = wave_sum(acc[r]);
+    if (lane == 0 && r < nr) y[gd.off_m + row0 + r] = (float)t;
+  }
 }
 
 // A^T x, split-K: chunk c of a group covers rows [c*R, min(m, (c+1)*R)); a
@@ -746,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
 // FP64 partial sums; k_dense_bwd_reduce adds the chunks in order.
 constexpr int kDenseChunks = 16;
 
-template <class T>
+template <class T, int U>
 __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
     const GroupDesc *G, const float *x, double *scratch, const InstScal *S,
     uint32_t mask, long long sstride) {
@@ -765,7 +790,23 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
   const int w = min(8, n - c0);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (w == 8 && gd.plan.a_vec) {
-    for (int i = i0; i < i1; ++i) {
+    // U rows loaded before they are consumed, in row order: the per-column
+    // summation order does not depend on U
+    int i = i0;
+    for (; i + U <= i1; i += U) {
+      float a[U][8];
+      double xi[U];
+#pragma unroll
+      for (int r = 0; r < U; ++r) {
+        load8(A + (long long)(i + r) * n + c0, a[r]);
+        xi[r] = (double)xg[i + r];
+      }
+#pragma unroll
+      for (int r = 0; r < U; ++r)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+    }
+    for (; i < i1; ++i) {
       const double xi = (double)xg[i];
       float a[8];
       load8(A + (long long)i * n + c0, a);
@@ -804,11 +845,19 @@ void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                    int kind, int dtype, const float *x, float *y,
                    const InstScal *S, uint32_t mask) {
   if (kind == kDense) {
-    dim3 g((max_m + 3) / 4, G);
-    if (dtype == TV3_BF16)
-      hipLaunchKernelGGL(k_dense_fwd_w1<__hip_bfloat16>, g, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
-    else
-      hipLaunchKernelGGL(k_dense_fwd_w1<float>, g, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+    const int R = dense_tune_cur().rows;
+    dim3 g((max_m + 4 * R - 1) / (4 * R), G);
+#define TV3_DFWD(TT, RR) hipLaunchKernelGGL((k_dense_fwd_w1<TT, RR>), g, dim3(kBlock), 0, s, G_dev, x, y, S, mask)
+    if (dtype == TV3_BF16) {
+      if (R == 4) TV3_DFWD(__hip_bfloat16, 4);
+      else if (R == 2) TV3_DFWD(__hip_bfloat16, 2);
+      else TV3_DFWD(__hip_bfloat16, 1);
+    } else {
+      if (R == 4) TV3_DFWD(float, 4);
+      else if (R == 2) TV3_DFWD(float, 2);
+      else TV3_DFWD(float, 1);
+    }
+#undef TV3_DFWD
     return;
   }
   dim3 grid((max_m + kBlock / kSub - 1) / (kBlock / kSub), G);
@@ -836,12 +885,19 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
     // scratch holds kDenseChunks * max_n doubles per group
     const long long sstride = (long long)kDenseChunks * max_n;
     dim3 g1((max_n + 8 * kBlock - 1) / (8 * kBlock), kDenseChunks, G);
-    if (dtype == TV3_BF16)
-      hipLaunchKernelGGL(k_dense_bwd_part<__hip_bfloat16>, g1, dim3(kBlock), 0, s, G_dev, x,
-                         (double *)scratch, S, mask, sstride);
-    else
-      hipLaunchKernelGGL(k_dense_bwd_part<float>, g1, dim3(kBlock), 0, s, G_dev, x,
-                         (double *)scratch, S, mask, sstride);
+    const int Ur = dense_tune_cur().unroll;
+#define TV3_DBWD(TT, UU) hipLaunchKernelGGL((k_dense_bwd_part<TT, UU>), g1, dim3(kBlock), 0, s, G_dev, x, \
+                                           (double *)scratch, S, mask, sstride)
+    if (dtype == TV3_BF16) {
+      if (Ur == 4) TV3_DBWD(__hip_bfloat16, 4);
+      else if (Ur == 2) TV3_DBWD(__hip_bfloat16, 2);
+      else TV3_DBWD(__hip_bfloat16, 1);
+    } else {
+      if (Ur == 4) TV3_DBWD(float, 4);
+      else if (Ur == 2) TV3_DBWD(float, 2);
+      else TV3_DBWD(float, 1);
+    }
+#undef TV3_DBWD
     dim3 g2((max_n + kBlock - 1) / kBlock, G);
     hipLaunchKernelGGL(k_dense_bwd_reduce, g2, dim3(kBlock), 0, s, G_dev,
                        (const double *)scratch, y, S, mask, sub, sstride);
@@ -933,6 +989,7 @@ void tv3_options_default(tv3_options *o) {
   o->c = 1e-5; o->nonneg = 0; o->isreal = 1; o->numInitIterations = 0;
   o->skipMulRatio = 0.0; o->maxRelativeChange = 0.0;
   o->scal_double = 0; o->tv_iso = 0; o->dt_exact = 0; o->ls_exact = 0;
+  o->graph = 0;
 }
 
 int tv3_ctx_create(int device, void *stream, tv3_ctx **out) {

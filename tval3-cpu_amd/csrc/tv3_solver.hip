@@ -27,7 +27,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "tv3_device.h"
@@ -1272,6 +1275,20 @@ struct Batch {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// ---- autotuner for the dense W = 1 launch shape ------------------------------
+// Each candidate is timed with HIP events on the batch's own operators (two
+// launches after one warm-up); the fastest is cached per device and batch
+// signature.  Candidates never change results (see DenseTune).
+// TV3_AUTOTUNE=0 keeps the default shape.
+using TuneKey = std::tuple<int, int, int, int, int, long long>;
+std::mutex g_tune_mu;
+std::map<TuneKey, DenseTune> g_tune_cache;
+
+bool autotune_enabled() {
+  const char *e = std::getenv("TV3_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
 template <int W, class TS>
 class Engine {
  public:
@@ -1287,6 +1304,7 @@ class Engine {
     O_.maxRelativeChange = o.maxRelativeChange;
     O_.skip = (o.skipMulRatio != 0 && bt.kind != kDense) ? 1 : 0;
     O_.ls_exact = o.ls_exact;
+    graph_ = o.graph != 0;
     nblk_n_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
     nblk_m_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
   }
@@ -1386,10 +1404,66 @@ class Engine {
     return rc;
   }
 
+  // times fn() (two launches after a warm-up) on the solver stream, in ms
+  template <class F>
+  float time_launches(F fn) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1e30f;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 1e30f; }
+    fn();
+    (void)hipEventRecord(e0, s_);
+    fn();
+    fn();
+    (void)hipEventRecord(e1, s_);
+    float ms = 1e30f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+  }
+
+  // picks the dense launch shape for this batch (dense, W = 1 only)
+  int tune_dense() {
+    if (!(W == 1 && bt_.kind == kDense)) return TV3_OK;
+    DenseTune best{2, 2};
+    if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
+    long long work = 0;
+    for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
+    const TuneKey key{ctx_->device, bt_.dtype, bt_.G, bt_.max_n, bt_.max_m, work};
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) { dense_tune_set(it->second); return TV3_OK; }
+    }
+    float tbest = 1e30f;
+    for (int r : {1, 2, 4}) {
+      dense_tune_set(DenseTune{r, best.unroll});
+      const float t = time_launches([&] {
+        launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, a_.Atb, a_.tmpm, S_, 0);
+      });
+      if (t < tbest) { tbest = t; best.rows = r; }
+    }
+    tbest = 1e30f;
+    for (int u : {1, 2, 4}) {
+      dense_tune_set(DenseTune{best.rows, u});
+      const float t = time_launches([&] {
+        launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.uup, S_, 0, nullptr, scratch_);
+      });
+      if (t < tbest) { tbest = t; best.unroll = u; }
+    }
+    dense_tune_set(best);
+    int rc = hip_check(ctx_, hipGetLastError(), "autotune launches");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+    return TV3_OK;
+  }
+
   int run(const float *const *b, float *const *U, const float *const *Ut,
           tv3_info *infos, tv3_trace_rec *trace, int trace_cap) {
     int rc = setup();
     if (rc) return rc;
+    if ((rc = tune_dense())) return rc;
     tv3_trace_rec *d_trace = nullptr;
     if (trace && trace_cap > 0) {
       rc = hip_check(ctx_, hipMalloc(&d_trace, sizeof(tv3_trace_rec) * (size_t)bt_.Bpad * trace_cap), "trace hipMalloc");
@@ -1422,6 +1496,8 @@ class Engine {
   const float **ptrs_in_ = nullptr;
   float **ptrs_out_ = nullptr;
   float *scratch_ = nullptr;
+  // graph mode (tv3_options::graph) and capture state
+  bool graph_ = false, capturing_ = false;
   // lane compaction (W = 64, on-the-fly projector)
   bool compact_ok_ = false, compact_ = false;
   LaneList lanes_{};
@@ -1454,6 +1530,10 @@ class Engine {
       hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const float *)nullptr);
       return;
     }
+    if (capturing_) {
+      launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
+      return;
+    }
     prof_begin(ctx_, 0);
     launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
     prof_end(ctx_);
@@ -1466,6 +1546,10 @@ class Engine {
       hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
       launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, S_, 0, nullptr, nullptr);
       hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
+      return;
+    }
+    if (capturing_) {
+      launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_);
       return;
     }
     prof_begin(ctx_, 1);
@@ -1519,6 +1603,112 @@ class Engine {
     hipLaunchKernelGGL(k_msq<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_.tmpm);
   }
 
+  // One inner iteration of every active instance (min_u, update_W, the stop
+  // tests and, for instances whose inner loop ended, the outer update).  With
+  // uniform = false the host skips the launches of branches no instance takes
+  // (it reads the OR of the flags at each branch point); with uniform = true
+  // every launch is issued, masked on the device by the same flags, and the
+  // host reads nothing: the launch sequence is then the same every iteration
+  // and is captured into a graph.
+  int step(uint32_t &sum, bool uniform) {
+    int rc = TV3_OK;
+    // ---- min_u: get_tau (2d:533) ----
+    hipLaunchKernelGGL(k_tau_bb<W>, gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE, F_FST), a_);
+    if (uniform || (sum & F_FST)) tau_first(F_ACTIVE | F_FST, 0);
+    hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE));
+    // keep previous values (2d:536-537) by swapping buffers
+    std::swap(a_.U, a_.Up); std::swap(a_.Ux, a_.Uxp); std::swap(a_.Uy, a_.Uyp);
+    std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
+    std::swap(a_.Au, a_.Aup);
+    // descend (2d:540)
+    hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_,
+                       O_.nonneg, O_.dt_exact, O_.ls_exact ? 0 : 1);
+    get_g_products(F_ACTIVE, 0, 0);
+    hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
+    if (!uniform && (rc = read_summary(sum))) return rc;
+    // Armijo backtracking (2d:548-575)
+    if (uniform || (sum & F_BT)) {
+      if (!O_.ls_exact) {
+        // step size already chosen on the device: rebuild the state once
+        hipLaunchKernelGGL((k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+        hipLaunchKernelGGL((k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+        hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 1);
+      } else {
+        hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+        hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+        // at most five rounds: the fifth failure sets the fallback
+        for (int round = 0; round < 5 && (uniform || (sum & F_BT)); ++round) {
+          hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+          hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+          hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 0);
+          if (!uniform && (rc = read_summary(sum))) return rc;
+        }
+      }
+    }
+    if (uniform || (sum & F_FB)) {
+      // 5-strike fallback (2d:561-567): tau from the first-iteration
+      // formula, U = Up, descend(force)
+      compact_ = !uniform && compact_ok_ && collect_lanes(F_FB);
+      tau_first(F_FB, 0);
+      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
+      hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_,
+                         O_.nonneg, O_.dt_exact, 0);
+      get_g_products(F_FB, 0, 1);
+      compact_ = false;
+      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
+    }
+    // ---- update_W, reference values, gradient, stop test ----
+    hipLaunchKernelGGL((k_wstep<W, TS, false>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.tv_iso, O_.dt_exact);
+    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
+    if (!uniform && (rc = read_summary(sum))) return rc;
+    if (uniform || (sum & F_INNER_DONE)) {
+      hipLaunchKernelGGL(k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
+      hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kBlock), 0, s_, fa(F_INNER_DONE));
+      hipLaunchKernelGGL(k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
+      hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
+      hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kBlock), 0, s_, fa(F_MLP));
+      hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
+      hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
+      hipLaunchKernelGGL(fin_refresh, gi(), dim3(kBlock), 0, s_, fa(0));
+      if (!uniform && (rc = read_summary(sum))) return rc;
+    }
+    rc = hip_check(ctx_, hipGetLastError(), "step kernels");
+    if (rc) return rc;
+    return TV3_OK;
+  }
+
+  // Graph mode: two uniform iterations (the buffer swaps of one iteration
+  // undo those of the other) captured once and replayed; the host reads the
+  // flags after every `per_check` replays and stops when no instance is
+  // active.  Replays after the last instance finished are fully masked.
+  int run_graph(uint32_t &sum) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = hip_check(ctx_, hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    capturing_ = true;
+    uint32_t dummy = ~0u;
+    int rc1 = step(dummy, true);
+    if (!rc1) rc1 = step(dummy, true);
+    capturing_ = false;
+    rc = hip_check(ctx_, hipStreamEndCapture(s_, &graph), "end capture");
+    if (rc1 || rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc1 ? rc1 : rc;
+    }
+    rc = hip_check(ctx_, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    const int per_check = bt_.W == 1 && bt_.B == 1 ? 4 : 1;
+    while (!rc && (sum & F_ACTIVE)) {
+      for (int k = 0; k < per_check && !rc; ++k)
+        rc = hip_check(ctx_, hipGraphLaunch(exec, s_), "graph launch");
+      if (!rc) rc = read_summary(sum);
+    }
+    (void)hipGraphExecDestroy(exec);
+    return rc;
+  }
+
   int solve(const float *const *b, float *const *U, const float *const *Ut,
             tv3_info *infos) {
     int rc;
@@ -1550,68 +1740,11 @@ class Engine {
     if (rc) return rc;
 
     uint32_t sum = (O_.maxcnt > 0 && O_.maxit > 0) ? (F_ACTIVE | F_FST) : 0u;
-    while (sum & F_ACTIVE) {
-      // ---- min_u: get_tau (2d:533) ----
-      hipLaunchKernelGGL(k_tau_bb<W>, gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE, F_FST), a_);
-      if (sum & F_FST) tau_first(F_ACTIVE | F_FST, 0);
-      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE));
-      // keep previous values (2d:536-537) by swapping buffers
-      std::swap(a_.U, a_.Up); std::swap(a_.Ux, a_.Uxp); std::swap(a_.Uy, a_.Uyp);
-      std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
-      std::swap(a_.Au, a_.Aup);
-      // descend (2d:540)
-      hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_,
-                         O_.nonneg, O_.dt_exact, O_.ls_exact ? 0 : 1);
-      get_g_products(F_ACTIVE, 0, 0);
-      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
-      if ((rc = read_summary(sum))) return rc;
-      // Armijo backtracking (2d:548-575)
-      if (sum & F_BT) {
-        if (!O_.ls_exact) {
-          // step size already chosen on the device: rebuild the state once
-          hipLaunchKernelGGL((k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-          hipLaunchKernelGGL((k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-          hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 1);
-        } else {
-          hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-          hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-          while (sum & F_BT) {
-            hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-            hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-            hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 0);
-            if ((rc = read_summary(sum))) return rc;
-          }
-        }
-      }
-      if (sum & F_FB) {
-        // 5-strike fallback (2d:561-567): tau from the first-iteration
-        // formula, U = Up, descend(force)
-        compact_ = compact_ok_ && collect_lanes(F_FB);
-        tau_first(F_FB, 0);
-        hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
-        hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_,
-                           O_.nonneg, O_.dt_exact, 0);
-        get_g_products(F_FB, 0, 1);
-        compact_ = false;
-        hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
-      }
-      // ---- update_W, reference values, gradient, stop test ----
-      hipLaunchKernelGGL((k_wstep<W, TS, false>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.tv_iso, O_.dt_exact);
-      hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
-      if ((rc = read_summary(sum))) return rc;
-      if (sum & F_INNER_DONE) {
-        hipLaunchKernelGGL(k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
-        hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kBlock), 0, s_, fa(F_INNER_DONE));
-        hipLaunchKernelGGL(k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
-        hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
-        hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kBlock), 0, s_, fa(F_MLP));
-        hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
-        hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
-        hipLaunchKernelGGL(fin_refresh, gi(), dim3(kBlock), 0, s_, fa(0));
-        if ((rc = read_summary(sum))) return rc;
-      }
-      rc = hip_check(ctx_, hipGetLastError(), "step kernels");
-      if (rc) return rc;
+    if (!graph_) {
+      while (sum & F_ACTIVE)
+        if ((rc = step(sum, false))) return rc;
+    } else if (sum & F_ACTIVE) {
+      if ((rc = run_graph(sum))) return rc;
     }
     // outputs, rel_error
     std::vector<float *> hout(bt_.Bpad, nullptr);
