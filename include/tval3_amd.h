@@ -60,7 +60,8 @@ extern "C" {
 
 /* Solver options: the 18 fields of the reference's tval3_options<T>
  * (used at 2d/tval3_cpu_2d.cpp:163-192, 540-569, 769-852), stored as double.
- * The last five fields are engine settings with no reference counterpart. */
+ * scal_double .. graph are engine settings with no field in tval3_options<T>;
+ * submat_mb is the reference's submat_mb function argument. */
 typedef struct tv3_options {
   double mu, beta;            /* final penalty parameters */
   double mu0, beta0;          /* initial penalty parameters (continuation) */
@@ -96,6 +97,10 @@ typedef struct tv3_options {
                                  steps are captured into a hipGraph once and
                                  replayed, and the host only checks for
                                  termination.  Same results. */
+  int submat_mb;              /* dense A: row-block size (MiB of float rows)
+                                 of the blocked A^T A product of get_g, the
+                                 reference's submat_mb argument (default 5;
+                                 2d/tval3_cpu_2d.cpp:361-394) */
 } tv3_options;
 
 /* Result record: the reference's tval3_info<T> (2d/tval3_cpu_2d.cpp:865-876). */

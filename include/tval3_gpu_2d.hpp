@@ -3,9 +3,9 @@
 //
 // Include after the reference's 2-D type header.  Same arguments, same
 // argument checks and messages as tval3_cpu_2d (check_params,
-// 2d/tval3_cpu_2d.cpp:696-750), same result record.  submat_mb and
-// virtual_procs_per_core are accepted and ignored (they tune the CPU path:
-// row-block size of the blocked A^T A product and MKL threads).
+// 2d/tval3_cpu_2d.cpp:696-750), same result record.  submat_mb sets
+// the row blocks of the dense A^T A product (as in the reference);
+// virtual_procs_per_core (MKL threads) is accepted and ignored.
 //
 // Supported element types: float vectors with float or double scalars
 // (T_comp = float, T_scal = float | double).  The double/double and complex
@@ -73,12 +73,13 @@ inline void no_layers(int layers) {
 
 template <class Ts>
 inline tval3_info<Ts> run_2d(mat<float> &U, const mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut) {
+                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+                             int submat_mb) {
   const int layers = A.dim_y ? (int)(b.len / A.dim_y) : 0;
   check_2d(U, A, b, Ut, layers);
   no_layers(layers);
   return solve_host<tval3_exception, tval3_info<Ts>>(
-      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8,
+      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &keep) {
         dense_plan(c, ph, keep, &A.data()[0], (int)A.dim_y, (int)A.dim_x);
       });
@@ -86,12 +87,13 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const mat<float> &A, const mat<float
 
 template <class Ts>
 inline tval3_info<Ts> run_2d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut) {
+                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+                             int submat_mb) {
   const int layers = A.dim_y ? (int)(b.len / A.dim_y) : 0;
   check_2d(U, A, b, Ut, layers);
   no_layers(layers);
   return solve_host<tval3_exception, tval3_info<Ts>>(
-      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8,
+      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &keep) {
         sparse_plan(c, ph, keep, A);
       });
@@ -99,7 +101,8 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const sparse_mat<float> &A, const ma
 
 template <class Ts>
 inline tval3_info<Ts> run_2d(mat<float> &U, const geometry &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut) {
+                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+                             int submat_mb) {
   check_2d(U, A, b, Ut, 1);
   const int E = (int)A.num_emitters, R = (int)A.num_receivers;
   std::vector<int> up((size_t)E * R), xe(E), ye(E), ze(E, 0), xr(R), yr(R), zr(R, 0);
@@ -129,7 +132,7 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const geometry &A, const mat<float> 
   g.rv_z = 1;
   g.scale_factor = (float)A.scale_factor;
   return solve_host<tval3_exception, tval3_info<Ts>>(
-      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8,
+      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &) {
         geometry_plan<tval3_exception>(c, ph, g);
       });
@@ -148,39 +151,39 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const geometry &A, const mat<float> 
 inline const tval3_info<float> tval3_gpu_2d(mat<float> &U, const mat<float> &A, const mat<float> &b,
                                             const tval3_options<float> &opts, const mat<float> &Ut,
                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<float> tval3_gpu_2d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
                                             const tval3_options<float> &opts, const mat<float> &Ut,
                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<float> tval3_gpu_2d(mat<float> &U, const geometry &A, const mat<float> &b,
                                             const tval3_options<float> &opts, const mat<float> &Ut,
                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 // float / double
 inline const tval3_info<double> tval3_gpu_2d(mat<float> &U, const mat<float> &A, const mat<float> &b,
                                              const tval3_options<double> &opts, const mat<float> &Ut,
                                              int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<double> tval3_gpu_2d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
                                              const tval3_options<double> &opts, const mat<float> &Ut,
                                              int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<double> tval3_gpu_2d(mat<float> &U, const geometry &A, const mat<float> &b,
                                              const tval3_options<double> &opts, const mat<float> &Ut,
                                              int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 // double / double, complex<float> / float, complex<double> / double
 template <class A_t>

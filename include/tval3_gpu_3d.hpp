@@ -3,8 +3,9 @@
 //
 // Include after the reference's 3-D type header.  Same arguments, same
 // argument checks and messages as tval3_cpu_3d (check_params,
-// 3d/tval3_cpu_3d.cpp:1028-1165), same result record.  submat_mb and
-// virtual_procs_per_core are accepted and ignored (they tune the CPU path).
+// 3d/tval3_cpu_3d.cpp:1028-1165), same result record.  submat_mb sets
+// the row blocks of the dense A^T A product (as in the reference);
+// virtual_procs_per_core (MKL threads) is accepted and ignored.
 // U's dim_z selects the 3-D stencil; dim_z == 1 is the 2-D one.  The Dt
 // y-term of the reference (not the adjoint of D for dim_z >= 2) is kept.
 //
@@ -56,10 +57,11 @@ inline void check_3d(const mat<Tc> &U, const geometry &A, const mat<Tc> &b, cons
 
 template <class Ts>
 inline tval3_info<Ts> run_3d(mat<float> &U, const mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut) {
+                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+                             int submat_mb) {
   check_3d(U, A, b, Ut);
   return solve_host<tval3_exception, tval3_info<Ts>>(
-      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, (int)U.dim_z, sizeof(Ts) == 8,
+      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, (int)U.dim_z, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &keep) {
         dense_plan(c, ph, keep, &A.data()[0], (int)A.dim_y, (int)A.dim_x);
       });
@@ -67,10 +69,11 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const mat<float> &A, const mat<float
 
 template <class Ts>
 inline tval3_info<Ts> run_3d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut) {
+                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+                             int submat_mb) {
   check_3d(U, A, b, Ut);
   return solve_host<tval3_exception, tval3_info<Ts>>(
-      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, (int)U.dim_z, sizeof(Ts) == 8,
+      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, (int)U.dim_z, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &keep) {
         sparse_plan(c, ph, keep, A);
       });
@@ -78,7 +81,8 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const sparse_mat<float> &A, const ma
 
 template <class Ts>
 inline tval3_info<Ts> run_3d(mat<float> &U, const geometry &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut) {
+                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+                             int submat_mb) {
   check_3d(U, A, b, Ut);
   const int P = (int)A.numMPs, E = (int)A.num_emitters, R = (int)A.num_receivers;
   std::vector<int> up((size_t)E * R), xe((size_t)P * E), ye(xe.size()), ze(xe.size()),
@@ -113,7 +117,7 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const geometry &A, const mat<float> 
   g.rv_z = (int)A.rv_z;
   g.scale_factor = (float)A.scale_factor;
   return solve_host<tval3_exception, tval3_info<Ts>>(
-      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, (int)U.dim_z, sizeof(Ts) == 8,
+      U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, (int)U.dim_z, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &) {
         geometry_plan<tval3_exception>(c, ph, g);
       });
@@ -131,38 +135,38 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const geometry &A, const mat<float> 
 inline const tval3_info<float> tval3_gpu_3d(mat<float> &U, const mat<float> &A, const mat<float> &b,
                                             const tval3_options<float> &opts, const mat<float> &Ut,
                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<float> tval3_gpu_3d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
                                             const tval3_options<float> &opts, const mat<float> &Ut,
                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<float> tval3_gpu_3d(mat<float> &U, const geometry &A, const mat<float> &b,
                                             const tval3_options<float> &opts, const mat<float> &Ut,
                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<double> tval3_gpu_3d(mat<float> &U, const mat<float> &A, const mat<float> &b,
                                              const tval3_options<double> &opts, const mat<float> &Ut,
                                              int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<double> tval3_gpu_3d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
                                              const tval3_options<double> &opts, const mat<float> &Ut,
                                              int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 inline const tval3_info<double> tval3_gpu_3d(mat<float> &U, const geometry &A, const mat<float> &b,
                                              const tval3_options<double> &opts, const mat<float> &Ut,
                                              int submat_mb = 5, int virtual_procs_per_core = 2) {
-  (void)submat_mb; (void)virtual_procs_per_core;
-  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut);
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 template <class A_t>
 inline const tval3_info<double> tval3_gpu_3d(mat<double> &, const A_t &, const mat<double> &,

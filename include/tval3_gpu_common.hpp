@@ -110,7 +110,7 @@ class PlanHolder {
 
 // tval3_options<T> (18 reference fields) -> tv3_options
 template <class Opts>
-inline tv3_options to_c(const Opts &o, int scal_double, int ls_poly) {
+inline tv3_options to_c(const Opts &o, int scal_double, int ls_poly, int submat_mb) {
   tv3_options c;
   tv3_options_default(&c);
   c.mu = (double)o.mu;
@@ -135,6 +135,7 @@ inline tv3_options to_c(const Opts &o, int scal_double, int ls_poly) {
   c.tv_iso = 0;
   c.dt_exact = 0;
   c.ls_exact = ls_poly ? 0 : 1;
+  c.submat_mb = submat_mb > 0 ? submat_mb : 5;
   return c;
 }
 
@@ -168,7 +169,7 @@ struct KeepAll {
 // are U's (dim_x contiguous).
 template <class Exc, class Info, class Opts, class MatU, class MakePlan>
 inline Info solve_host(MatU &U, const MatU &b, const Opts &opts, const MatU &Ut,
-                       int dx, int dy, int dz, int scal_double,
+                       int dx, int dy, int dz, int scal_double, int submat_mb,
                        MakePlan make_plan) {
   const auto t0 = std::chrono::steady_clock::now();
   Session &s = session();
@@ -182,7 +183,7 @@ inline Info solve_host(MatU &U, const MatU &b, const Opts &opts, const MatU &Ut,
   DevBuf<Exc> dU(c, n * sizeof(float));
   const bool has_ut = Ut.len > 0 && (size_t)Ut.len == n;
   DevBuf<Exc> *dUt = has_ut ? keep.put(c, &Ut.data()[0], n * sizeof(float)) : nullptr;
-  const tv3_options co = to_c(opts, scal_double, s.ls_poly);
+  const tv3_options co = to_c(opts, scal_double, s.ls_poly, submat_mb);
   tv3_info ci{};
   const int rc = tv3_solve(c, plan.p, &co, dx, dy, dz, db->template as<float>(),
                            dU.template as<float>(),

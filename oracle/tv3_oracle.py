@@ -55,6 +55,7 @@ def lib():
                                      C.c_int, _DP, _DP, _DP, C.c_void_p,
                                      C.c_void_p, C.c_int]
         L.tv3o_gemv_f64.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
+        L.tv3o_AtA_blocked.argtypes = [C.c_void_p, _FP, C.c_int, _FP, _FP]
         L.tv3o_set_threads.argtypes = [C.c_int]
         L.tv3o_get_threads.restype = C.c_int
         _lib = L
@@ -135,6 +136,15 @@ class Op:
         lib().tv3o_gemv_f64(self.h, int(bool(trans)), x.ctypes.data_as(dp),
                             y.ctypes.data_as(dp))
         return y
+
+    def AtA_blocked(self, U, submat_mb=5):
+        """(Au, g) of get_g_mult's blocked dense product."""
+        U = _c32(U)
+        Au = np.zeros(self.m, np.float32)
+        g = np.zeros(self.n, np.float32)
+        if lib().tv3o_AtA_blocked(self.h, _f(U), int(submat_mb), _f(Au), _f(g)):
+            raise ValueError("not a dense operator")
+        return Au, g
 
     def gemv(self, x, trans=False):
         x = _c32(x)

@@ -246,6 +246,45 @@ void build_geometry(Op &op, const tv3_geometry &g) {
     }
 }
 
+/* get_g_mult for a dense A (2d/tval3_cpu_2d.cpp:361-394, single layer;
+ * 3d/tval3_cpu_3d.cpp:498-529): row blocks of r = max(submat_mb MiB /
+ * sizeof(T) / n, 1) rows; per block Au_blk = A_blk U and g_tmp = A_blk^T
+ * Au_blk (each a gemv: FP64 sums rounded to T), then g += g_tmp (cblas_axpy
+ * with alpha 1: one rounding per block, in block order). */
+template <class T>
+void dense_AtA_blocked(const Op &op, const T *U, T *Au, T *g, int submat_mb) {
+  const i64 m = op.m, n = op.n;
+  const i64 elems = (i64)submat_mb * 1024 * 1024 / (i64)sizeof(T);
+  const i64 r = std::max<i64>(elems / n, 1);
+  const float *A = op.A;
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < m; ++i) {
+    double s = 0.0;
+    const float *row = A + i * n;
+    for (i64 j = 0; j < n; ++j) s += (double)row[j] * (double)U[j];
+    Au[i] = (T)s;
+  }
+  const i64 cb = 256;
+#pragma omp parallel for schedule(static)
+  for (i64 j0 = 0; j0 < n; j0 += cb) {
+    const i64 j1 = std::min(n, j0 + cb);
+    T gacc[256];
+    double acc[256];
+    for (i64 j = j0; j < j1; ++j) gacc[j - j0] = (T)0;
+    for (i64 y0 = 0; y0 < m; y0 += r) {
+      const i64 y1 = std::min(m, y0 + r);
+      for (i64 j = j0; j < j1; ++j) acc[j - j0] = 0.0;
+      for (i64 i = y0; i < y1; ++i) {
+        const double ai = (double)Au[i];
+        const float *row = A + i * n;
+        for (i64 j = j0; j < j1; ++j) acc[j - j0] += (double)row[j] * ai;
+      }
+      for (i64 j = j0; j < j1; ++j) gacc[j - j0] = gacc[j - j0] + (T)acc[j - j0];
+    }
+    for (i64 j = j0; j < j1; ++j) g[j] = gacc[j - j0];
+  }
+}
+
 /* y = A x or A^T x, overwriting y (gemv with beta = 0).  T is the vector
  * type; geometry products are formed in it (mult_elementN/T, 3d/mat_mult.h:
  * 121-139), all sums in FP64. */
@@ -525,8 +564,12 @@ struct Solver {
   /* get_g_mult: 2d/tval3_cpu_2d.cpp:298-357 (skipping state machine) and
    * 359-394 (dense: always both products) */
   void get_g_mult(const V &U, bool force_mul) {
+    if (A.kind == 0) {
+      dense_AtA_blocked<TC>(A, U.data(), Au.data(), g.data(), o.submat_mb > 0 ? o.submat_mb : 5);
+      return;
+    }
     gemv(A, false, U.data(), Au.data());
-    if (A.kind == 0 || o.skipMulRatio == 0) {
+    if (o.skipMulRatio == 0) {
       gemv(A, true, Au.data(), g.data());
       return;
     }
@@ -914,6 +957,15 @@ int tv3o_solve_f64(const tv3o_op *h, const tv3_options *o, int q, int p,
   if (dm.n() != op.n) return 1;
   Solver<double, double> s(op, *o, dm);
   s.run(b, U, Ut, info, trace, trace_cap);
+  return 0;
+}
+
+/* get_g_mult's blocked dense product (test entry point): Au = A U, g = the
+ * block-accumulated A^T Au. */
+int tv3o_AtA_blocked(const tv3o_op *h, const float *U, int submat_mb, float *Au, float *g) {
+  const Op &op = *(const Op *)h;
+  if (op.kind != 0) return 1;
+  dense_AtA_blocked<float>(op, U, Au, g, submat_mb);
   return 0;
 }
 

@@ -549,3 +549,30 @@ def test_five_strike_fallback(seed):
     C0 = M.L(Uk, Wk, sg, dl)
     g = opts.gam * opts.rate_gam
     assert t.C == pytest.approx((g * C0 + t.f) / (g + 1), rel=1e-8)
+
+
+@pytest.mark.parametrize("n,m,smb", [(4096, 1229, 5), (1024, 700, 1), (512, 300, 0)])
+def test_dense_get_g_blocking(n, m, smb):
+    """get_g_mult's dense product (2d/tval3_cpu_2d.cpp:361-394): row blocks of
+    max(submat_mb MiB / 4 / n, 1) rows; each block's A_blk^T Au_blk is an FP64
+    sum rounded to float, the blocks are added in float in order.  Checked
+    against that formula in numpy, and against the unblocked product, which
+    it must differ from in the last bits when there are several blocks."""
+    rng = np.random.default_rng(n + m)
+    A = (rng.standard_normal((m, n)) / np.sqrt(m)).astype(np.float32)
+    U = rng.standard_normal(n).astype(np.float32)
+    op = O.Op.dense(A)
+    Au, g = op.AtA_blocked(U, smb)
+    A64 = A.astype(np.float64)
+    Au_ref = (A64 @ U.astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(Au, Au_ref)
+    r = max((max(smb, 0) * 1024 * 1024 // 4) // n, 1)
+    gref = np.zeros(n, np.float32)
+    for y0 in range(0, m, r):
+        blk = (A64[y0:y0 + r].T @ Au_ref[y0:y0 + r].astype(np.float64)).astype(np.float32)
+        gref = (gref + blk).astype(np.float32)
+    assert np.max(np.abs(g.astype(np.float64) - gref)) <= 2e-7 * np.abs(gref).max()
+    nblk = (m + r - 1) // r
+    plain = op.gemv(Au, trans=True)
+    if nblk > 2:
+        assert np.any(plain != g)

@@ -208,7 +208,7 @@ struct tv3_tape {
   }
   void bwdA(const float *x, float *y) {
     prof_begin(ctx, 1);
-    launch_bwd<1>(st, G, 1, n, A->dev.kind, A->dev.dtype, x, y, S, 0, nullptr, (float *)dense);
+    launch_bwd<1>(st, G, 1, n, A->dev.kind, A->dev.dtype, x, y, S, 0, nullptr, (float *)dense, 0);
     prof_end(ctx);
     prof_count(ctx, G, 1, 1, S, 0, 1);
   }

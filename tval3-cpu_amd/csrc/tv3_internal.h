@@ -69,6 +69,11 @@ struct GroupDesc {
   long long off_n;      // float offset of this group's n-vectors
   long long off_m;      // float offset of this group's m-vectors
   PlanDev plan;
+  // blocked dense product of get_g (W = 1): rows per block, slices per block
+  // (0: one thread walks all rows of its columns) and the double offset of
+  // this group's slice partials in the solver scratch
+  int dblk_rows, dblk_S;
+  long long dblk_off;
 };
 
 // Control flags (InstScal::flags)
@@ -165,13 +170,15 @@ int set_err(tv3_ctx *ctx, int code, const std::string &msg);
 void prof_begin(tv3_ctx *ctx, int kind);
 void prof_end(tv3_ctx *ctx);
 void prof_collect(tv3_ctx *ctx);
-// dense W = 1 launch shape: rows per wavefront of A x (1, 2, 4) and rows
-// loaded ahead in A^T x (1, 2, 4).  Neither changes any summation order.
+// dense W = 1 launch shape: rows per wavefront of A x (1, 2, 4), rows
+// loaded ahead in A^T x (1, 2, 4, 8) and columns per thread of the blocked
+// A^T x (4, 8).  Neither changes any summation order.
 // The current setting is per host thread; the solver sets it from its
 // autotuner for the duration of a solve.
 struct DenseTune {
   int rows;
-  int unroll;
+  int unroll;  // 1, 2, 4, 8
+  int cols;    // columns per thread of the blocked A^T product: 4 or 8
 };
 void dense_tune_set(DenseTune t);
 DenseTune dense_tune_cur();
@@ -190,9 +197,19 @@ template <int W>
 void launch_fwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                 int kind, int dtype, const float *x, float *y,
                 const InstScal *S, uint32_t mask);
+// submat_mb > 0 (dense A only): the blocked product of get_g, row blocks of
+// max(submat_mb MiB / 4 / n, 1) rows, each block's sum rounded to float and
+// the blocks added in float in order; 0: one FP64 sum over all rows.
 template <int W>
 void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                 int kind, int dtype, const float *x, float *y,
                 const InstScal *S, uint32_t mask, const float *sub,
-                float *scratch);
+                float *scratch, int submat_mb);
+
+// rows per block of the blocked dense product (2d/tval3_cpu_2d.cpp:366-369)
+__host__ __device__ inline int dense_block_rows(int submat_mb, int n) {
+  const long long e = (long long)submat_mb * 1024 * 1024 / 4;
+  const long long r = e / (n > 0 ? n : 1);
+  return r < 1 ? 1 : (int)(r > 0x7fffffff ? 0x7fffffff : r);
+}
 }  // namespace tv3

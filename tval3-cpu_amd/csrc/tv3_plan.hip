@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64(const GroupDesc *G,
 constexpr int kPipeQ = 4;
 
 namespace {
-thread_local DenseTune g_dense_tune{2, 2};
+thread_local DenseTune g_dense_tune{2, 2, 8};
 }
 void dense_tune_set(DenseTune t) { g_dense_tune = t; }
 DenseTune dense_tune_cur() { return g_dense_tune; }
@@ -460,7 +460,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
                                                     const float *x, float *y,
                                                     const InstScal *S,
                                                     uint32_t mask,
-                                                    const float *sub) {
+                                                    const float *sub,
+                                                    int submat_mb) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int col = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
@@ -506,37 +507,40 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
       }
     }
   } else {
-    int i = 0;
-    if (P.dtype == TV3_F32) {
-      const float *A = (const float *)P.A + col;
-      for (; i < gd.m; i += kPipe) {
-        const int cnt = min(kPipe, gd.m - i);
-        const float a = (lane < cnt) ? A[(long long)(i + lane) * gd.n] : 0.0f;
-        float v[kPipe], w[kPipe];
-#pragma unroll
-        for (int u = 0; u < kPipe; ++u) {
-          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
-          v[u] = u < cnt ? (xb + (long long)(i + u) * 64)[lane] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < kPipe; ++u)
-          if (u < cnt) acc += (double)w[u] * (double)v[u];
+    // dense: rows in order; with submat_mb the FP64 sum restarts at every
+    // block boundary and the rounded block sums are added in float
+    const int rb = submat_mb > 0 ? dense_block_rows(submat_mb, gd.n) : 0x7fffffff;
+    int next_b = rb;
+    float gf = 0.0f;
+    for (int i = 0; i < gd.m; i += kPipe) {
+      const int cnt = min(kPipe, gd.m - i);
+      float a = 0.0f;
+      if (lane < cnt) {
+        const long long ia = (long long)(i + lane) * gd.n + col;
+        a = P.dtype == TV3_F32 ? ((const float *)P.A)[ia]
+                               : __bfloat162float(((const __hip_bfloat16 *)P.A)[ia]);
       }
-    } else {
-      const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + col;
-      for (; i < gd.m; i += kPipe) {
-        const int cnt = min(kPipe, gd.m - i);
-        const float a = (lane < cnt) ? __bfloat162float(A[(long long)(i + lane) * gd.n]) : 0.0f;
-        float v[kPipe], w[kPipe];
+      float v[kPipe], w[kPipe];
 #pragma unroll
-        for (int u = 0; u < kPipe; ++u) {
-          w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
-          v[u] = u < cnt ? (xb + (long long)(i + u) * 64)[lane] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < kPipe; ++u)
-          if (u < cnt) acc += (double)w[u] * (double)v[u];
+      for (int u = 0; u < kPipe; ++u) {
+        w[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a), u));
+        v[u] = u < cnt ? (xb + (long long)(i + u) * 64)[lane] : 0.0f;
       }
+#pragma unroll
+      for (int u = 0; u < kPipe; ++u) {
+        if (u < cnt) {
+          if (i + u == next_b) {
+            gf = gf + (float)acc;
+            acc = 0.0;
+            next_b += rb;
+          }
+          acc += (double)w[u] * (double)v[u];
+        }
+      }
+    }
+    if (submat_mb > 0) {
+      gf = gf + (float)acc;
+      acc = (double)gf;  // exact: gf is a float
     }
   }
   if (on) {
@@ -712,6 +716,23 @@ __device__ __forceinline__ void load8(const __hip_bfloat16 *p, float (&v)[8]) {
   }
 }
 
+// four consecutive entries (8 bytes of bf16, 16 of f32)
+__device__ __forceinline__ void load4(const float *p, float (&v)[4]) {
+  const float4 a = *(const float4 *)p;
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void load4(const __hip_bfloat16 *p, float (&v)[4]) {
+  const uint2 q = *(const uint2 *)p;
+  v[0] = __uint_as_float(q.x << 16);
+  v[1] = __uint_as_float(q.x & 0xffff0000u);
+  v[2] = __uint_as_float(q.y << 16);
+  v[3] = __uint_as_float(q.y & 0xffff0000u);
+}
+template <int C, class T>
+__device__ __forceinline__ void loadc(const T *p, float (&v)[C]) {
+  if constexpr (C == 8) load8(p, v); else load4(p, v);
+}
+
 // y = A x: one wavefront per R consecutive rows; with n a multiple of 8 every
 // lane loads 8 consecutive entries of each row (16 bytes of bf16, 32 of f32)
 // and the matching 8 x values once for all R rows per step; FP64 lane sums,
@@ -840,6 +861,152 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce(
   y[o] = v;
 }
 
+// The blocked A^T x of get_g (dense, W = 1): a thread owns C (4 or 8)
+// consecutive columns and walks all rows in order (no split over rows), FP64
+// within a block of rows, float across blocks.  y = g - sub.
+template <class T, int U, int C>
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_blk(
+    const GroupDesc *G, const float *x, float *y, const InstScal *S,
+    uint32_t mask, const float *sub, int submat_mb) {
+  const GroupDesc &gd = G[blockIdx.y];
+  if (gd.dblk_S > 0) return;  // k_dense_bwd_bs / _bred handle this group
+  const int n = gd.n;
+  const int c0 = (blockIdx.x * kBlock + threadIdx.x) * C;
+  if (c0 >= n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int m = gd.m;
+  const int rb = dense_block_rows(submat_mb, n);
+  const float *xg = x + gd.off_m;
+  const T *A = (const T *)gd.plan.A;
+  const int w = min(C, n - c0);
+  float gf[C];
+#pragma unroll
+  for (int u = 0; u < C; ++u) gf[u] = 0.0f;
+  for (int y0 = 0; y0 < m; y0 += rb) {
+    const int y1 = min(m, y0 + rb);
+    double acc[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u) acc[u] = 0.0;
+    if (w == C && gd.plan.a_vec) {
+      int i = y0;
+      for (; i + U <= y1; i += U) {
+        float a[U][C];
+        double xi[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+          loadc<C>(A + (long long)(i + r) * n + c0, a[r]);
+          xi[r] = (double)xg[i + r];
+        }
+#pragma unroll
+        for (int r = 0; r < U; ++r)
+#pragma unroll
+          for (int u = 0; u < C; ++u) acc[u] += (double)a[r][u] * xi[r];
+      }
+      for (; i < y1; ++i) {
+        const double xi = (double)xg[i];
+        float a[C];
+        loadc<C>(A + (long long)i * n + c0, a);
+#pragma unroll
+        for (int u = 0; u < C; ++u) acc[u] += (double)a[u] * xi;
+      }
+    } else {
+      for (int i = y0; i < y1; ++i) {
+        const double xi = (double)xg[i];
+        for (int u = 0; u < w; ++u) acc[u] += (double)a_val(A, (long long)i * n + c0 + u) * xi;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < C; ++u) gf[u] = gf[u] + (float)acc[u];
+  }
+  const long long o = gd.off_n + c0;
+  for (int u = 0; u < w; ++u) {
+    float v = gf[u];
+    if (sub) v = v - sub[o + u];
+    y[o + u] = v;
+  }
+}
+
+// Blocked A^T x with row slices (groups with few blocks): chunk c = b * S + s
+// covers slice s of row block b; each thread sums 8 columns over the slice's
+// rows in FP64 and writes the partials; k_dense_bwd_bred adds the S slices of
+// a block in FP64, rounds once per block and adds the blocks in float.
+constexpr int kDenseBlkChunks = 64;
+
+template <class T, int U>
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_bs(
+    const GroupDesc *G, const float *x, double *scratch, const InstScal *S,
+    uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int Sl = gd.dblk_S;
+  if (Sl <= 0) return;
+  const int n = gd.n, m = gd.m, rb = gd.dblk_rows;
+  const int c0 = (blockIdx.x * kBlock + threadIdx.x) * 8;
+  if (c0 >= n) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int nb = (m + rb - 1) / rb;
+  const int c = blockIdx.y;
+  if (c >= nb * Sl) return;
+  const int b = c / Sl, sl = c % Sl;
+  const int y0 = b * rb, y1 = min(m, y0 + rb);
+  const int rps = (y1 - y0 + Sl - 1) / Sl;
+  const int i0 = min(y1, y0 + sl * rps), i1 = min(y1, i0 + rps);
+  const float *xg = x + gd.off_m;
+  const T *A = (const T *)gd.plan.A;
+  const int w = min(8, n - c0);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (w == 8 && gd.plan.a_vec) {
+    int i = i0;
+    for (; i + U <= i1; i += U) {
+      float a[U][8];
+      double xi[U];
+#pragma unroll
+      for (int r = 0; r < U; ++r) {
+        load8(A + (long long)(i + r) * n + c0, a[r]);
+        xi[r] = (double)xg[i + r];
+      }
+#pragma unroll
+      for (int r = 0; r < U; ++r)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+    }
+    for (; i < i1; ++i) {
+      const double xi = (double)xg[i];
+      float a[8];
+      load8(A + (long long)i * n + c0, a);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += (double)a[u] * xi;
+    }
+  } else {
+    for (int i = i0; i < i1; ++i) {
+      const double xi = (double)xg[i];
+      for (int u = 0; u < w; ++u) acc[u] += (double)a_val(A, (long long)i * n + c0 + u) * xi;
+    }
+  }
+  double *out = scratch + gd.dblk_off + (long long)c * n + c0;
+  for (int u = 0; u < w; ++u) out[u] = acc[u];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_bred(
+    const GroupDesc *G, const double *scratch, float *y, const InstScal *S,
+    uint32_t mask, const float *sub) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int Sl = gd.dblk_S;
+  if (Sl <= 0) return;
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int nb = (gd.m + gd.dblk_rows - 1) / gd.dblk_rows;
+  const double *sp = scratch + gd.dblk_off + col;
+  float gf = 0.0f;
+  for (int b = 0; b < nb; ++b) {
+    double t = 0.0;
+    for (int s2 = 0; s2 < Sl; ++s2) t += sp[(long long)(b * Sl + s2) * gd.n];
+    gf = gf + (float)t;
+  }
+  const long long o = gd.off_n + col;
+  y[o] = sub ? gf - sub[o] : gf;
+}
+
 template <>
 void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                    int kind, int dtype, const float *x, float *y,
@@ -880,7 +1047,44 @@ template <>
 void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                    int kind, int dtype, const float *x, float *y,
                    const InstScal *S, uint32_t mask, const float *sub,
-                   float *scratch) {
+                   float *scratch, int submat_mb) {
+  if (kind == kDense && submat_mb > 0) {
+    const DenseTune tn = dense_tune_cur();
+    const int Ur = tn.unroll, Cc = tn.cols;
+    dim3 g((max_n + Cc * kBlock - 1) / (Cc * kBlock), G);
+#define TV3_DBLK(TT, UU, CC) hipLaunchKernelGGL((k_dense_bwd_blk<TT, UU, CC>), g, dim3(kBlock), 0, s, G_dev, x, y, \
+                                               S, mask, sub, submat_mb)
+#define TV3_DBLK_U(TT, CC)              \
+  if (Ur == 8) TV3_DBLK(TT, 8, CC);     \
+  else if (Ur == 4) TV3_DBLK(TT, 4, CC); \
+  else if (Ur == 2) TV3_DBLK(TT, 2, CC); \
+  else TV3_DBLK(TT, 1, CC);
+    if (dtype == TV3_BF16) {
+      if (Cc == 4) { TV3_DBLK_U(__hip_bfloat16, 4) } else { TV3_DBLK_U(__hip_bfloat16, 8) }
+    } else {
+      if (Cc == 4) { TV3_DBLK_U(float, 4) } else { TV3_DBLK_U(float, 8) }
+    }
+#undef TV3_DBLK_U
+#undef TV3_DBLK
+    // groups in slice mode
+    dim3 g1((max_n + 8 * kBlock - 1) / (8 * kBlock), kDenseBlkChunks, G);
+#define TV3_DBS(TT, UU) hipLaunchKernelGGL((k_dense_bwd_bs<TT, UU>), g1, dim3(kBlock), 0, s, G_dev, x, \
+                                          (double *)scratch, S, mask)
+    if (dtype == TV3_BF16) {
+      if (Ur >= 4) TV3_DBS(__hip_bfloat16, 4);
+      else if (Ur == 2) TV3_DBS(__hip_bfloat16, 2);
+      else TV3_DBS(__hip_bfloat16, 1);
+    } else {
+      if (Ur >= 4) TV3_DBS(float, 4);
+      else if (Ur == 2) TV3_DBS(float, 2);
+      else TV3_DBS(float, 1);
+    }
+#undef TV3_DBS
+    dim3 g2((max_n + kBlock - 1) / kBlock, G);
+    hipLaunchKernelGGL(k_dense_bwd_bred, g2, dim3(kBlock), 0, s, G_dev, (const double *)scratch, y, S,
+                       mask, sub);
+    return;
+  }
   if (kind == kDense) {
     // scratch holds kDenseChunks * max_n doubles per group
     const long long sstride = (long long)kDenseChunks * max_n;
@@ -889,11 +1093,11 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
 #define TV3_DBWD(TT, UU) hipLaunchKernelGGL((k_dense_bwd_part<TT, UU>), g1, dim3(kBlock), 0, s, G_dev, x, \
                                            (double *)scratch, S, mask, sstride)
     if (dtype == TV3_BF16) {
-      if (Ur == 4) TV3_DBWD(__hip_bfloat16, 4);
+      if (Ur >= 4) TV3_DBWD(__hip_bfloat16, 4);
       else if (Ur == 2) TV3_DBWD(__hip_bfloat16, 2);
       else TV3_DBWD(__hip_bfloat16, 1);
     } else {
-      if (Ur == 4) TV3_DBWD(float, 4);
+      if (Ur >= 4) TV3_DBWD(float, 4);
       else if (Ur == 2) TV3_DBWD(float, 2);
       else TV3_DBWD(float, 1);
     }
@@ -912,14 +1116,15 @@ template <>
 void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                     int kind, int dtype, const float *x, float *y,
                     const InstScal *S, uint32_t mask, const float *sub,
-                    float *scratch) {
+                    float *scratch, int submat_mb) {
   (void)dtype;
   (void)scratch;
   dim3 grid((max_n + 3) / 4, G);
   if (kind == kGeometry && proj_vec())
     hipLaunchKernelGGL(k_bwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
   else
-    hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
+    hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub,
+                       kind == kDense ? submat_mb : 0);
 }
 
 template <>
@@ -943,7 +1148,7 @@ template <>
 void launch_bwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                     int kind, int dtype, const float *x, float *y,
                     const InstScal *S, uint32_t mask, const float *sub,
-                    float *scratch) {
+                    float *scratch, int submat_mb) {
   (void)kind;
   (void)dtype;
   (void)scratch;
@@ -990,6 +1195,7 @@ void tv3_options_default(tv3_options *o) {
   o->skipMulRatio = 0.0; o->maxRelativeChange = 0.0;
   o->scal_double = 0; o->tv_iso = 0; o->dt_exact = 0; o->ls_exact = 0;
   o->graph = 0;
+  o->submat_mb = 5;
 }
 
 int tv3_ctx_create(int device, void *stream, tv3_ctx **out) {

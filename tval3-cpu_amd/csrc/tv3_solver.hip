@@ -1270,6 +1270,7 @@ struct Batch {
   std::vector<GroupDesc> groups;
   int max_n = 0, max_m = 0, kind = 0, dtype = 0;
   long long Nn = 0, Nm = 0;
+  long long dblk_doubles = 0;  // slice partials of the blocked dense product
   bool z3 = false;
 };
 
@@ -1305,8 +1306,15 @@ class Engine {
     O_.skip = (o.skipMulRatio != 0 && bt.kind != kDense) ? 1 : 0;
     O_.ls_exact = o.ls_exact;
     graph_ = o.graph != 0;
+    submat_mb_ = o.submat_mb > 0 ? o.submat_mb : 5;
     nblk_n_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
     nblk_m_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
+  }
+
+  // dense W = 1 scratch: split-K partials of the plain A^T product or slice
+  // partials of the blocked one (never live at the same time)
+  size_t scratch_doubles() const {
+    return (size_t)std::max<long long>((long long)bt_.G * 16 * bt_.max_n, bt_.dblk_doubles);
   }
 
   int setup() {
@@ -1322,7 +1330,7 @@ class Engine {
     need += align_up(sizeof(GroupDesc) * bt_.G);
     need += align_up(sizeof(float *) * bt_.Bpad) * 2;
     if (bt_.kind == kDense && W == 1)
-      need += align_up(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n);
+      need += align_up(sizeof(double) * scratch_doubles());
     compact_ok_ = (W == 64 && bt_.kind == kGeometry);
     pitch_ = ((long long)std::max(bt_.max_n, bt_.max_m) + 63) / 64 * 64;
     if (compact_ok_) {
@@ -1365,7 +1373,7 @@ class Engine {
     G_ = (GroupDesc *)take(sizeof(GroupDesc) * bt_.G);
     ptrs_in_ = (const float **)take(sizeof(float *) * bt_.Bpad);
     ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
-    scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * (size_t)bt_.G * 16 * bt_.max_n) : nullptr;
+    scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * scratch_doubles()) : nullptr;
     if (compact_ok_) {
       Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * kLaneMax);
       xc_ = (float *)take(sizeof(float) * (size_t)kLaneMax * pitch_);
@@ -1425,7 +1433,7 @@ class Engine {
   // picks the dense launch shape for this batch (dense, W = 1 only)
   int tune_dense() {
     if (!(W == 1 && bt_.kind == kDense)) return TV3_OK;
-    DenseTune best{2, 2};
+    DenseTune best{2, 2, 8};
     if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
     long long work = 0;
     for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
@@ -1437,19 +1445,21 @@ class Engine {
     }
     float tbest = 1e30f;
     for (int r : {1, 2, 4}) {
-      dense_tune_set(DenseTune{r, best.unroll});
+      dense_tune_set(DenseTune{r, best.unroll, best.cols});
       const float t = time_launches([&] {
         launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, a_.Atb, a_.tmpm, S_, 0);
       });
       if (t < tbest) { tbest = t; best.rows = r; }
     }
     tbest = 1e30f;
-    for (int u : {1, 2, 4}) {
-      dense_tune_set(DenseTune{best.rows, u});
-      const float t = time_launches([&] {
-        launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.uup, S_, 0, nullptr, scratch_);
-      });
-      if (t < tbest) { tbest = t; best.unroll = u; }
+    for (int c : {8, 4}) {
+      for (int u : {1, 2, 4, 8}) {
+        dense_tune_set(DenseTune{best.rows, u, c});
+        const float t = time_launches([&] {
+          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.uup, S_, 0, nullptr, scratch_, submat_mb_);
+        });
+        if (t < tbest) { tbest = t; best.unroll = u; best.cols = c; }
+      }
     }
     dense_tune_set(best);
     int rc = hip_check(ctx_, hipGetLastError(), "autotune launches");
@@ -1498,6 +1508,7 @@ class Engine {
   float *scratch_ = nullptr;
   // graph mode (tv3_options::graph) and capture state
   bool graph_ = false, capturing_ = false;
+  int submat_mb_ = 5;  // row blocks of the dense get_g product
   // lane compaction (W = 64, on-the-fly projector)
   bool compact_ok_ = false, compact_ = false;
   LaneList lanes_{};
@@ -1539,21 +1550,23 @@ class Engine {
     prof_end(ctx_);
     prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 0);
   }
-  void bwd(const float *x, float *y, uint32_t mask, const float *sub) {
+  // blocked: the get_g product (dense A: row blocks of submat_mb)
+  void bwd(const float *x, float *y, uint32_t mask, const float *sub, bool blocked = false) {
+    const int smb = blocked ? submat_mb_ : 0;
     if (compact_) {
       const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
       const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
       hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
-      launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, S_, 0, nullptr, nullptr);
+      launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, S_, 0, nullptr, nullptr, 0);
       hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
       return;
     }
     if (capturing_) {
-      launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_);
+      launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_, smb);
       return;
     }
     prof_begin(ctx_, 1);
-    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_);
+    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_, smb);
     prof_end(ctx_);
     prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 1);
   }
@@ -1589,10 +1602,10 @@ class Engine {
     if (O_.skip) {
       hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kBlock), 0, s_, fa(req, forbid), force);
       hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
-      bwd(a_.Au, a_.g, req | F_BWD, a_.Atb);
+      bwd(a_.Au, a_.g, req | F_BWD, a_.Atb, true);
       hipLaunchKernelGGL(k_skipg<W>, gn(), dim3(kBlock), 0, s_, kn(req | F_SKIPG, forbid), a_);
     } else {
-      bwd(a_.Au, a_.g, req, a_.Atb);
+      bwd(a_.Au, a_.g, req, a_.Atb, true);
     }
   }
 
@@ -1842,6 +1855,18 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
     om += ((long long)gd.m * W + 63) / 64 * 64;
     bt.max_n = std::max(bt.max_n, gd.n);
     bt.max_m = std::max(bt.max_m, gd.m);
+    if (W == 1 && gd.plan.kind == kDense) {
+      // blocked get_g product: groups with up to 64 row blocks split each
+      // block into row slices (about 16 or more chunks in all)
+      const int smb = o->submat_mb > 0 ? o->submat_mb : 5;
+      gd.dblk_rows = dense_block_rows(smb, gd.n);
+      const long long nb = ((long long)gd.m + gd.dblk_rows - 1) / gd.dblk_rows;
+      if (nb <= 64) {
+        gd.dblk_S = (int)std::max<long long>(1, 16 / std::max<long long>(nb, 1));
+        gd.dblk_off = bt.dblk_doubles;
+        bt.dblk_doubles += nb * gd.dblk_S * (long long)gd.n;
+      }
+    }
     bt.groups.push_back(gd);
   }
   bt.Nn = std::max<long long>(on, 1);
@@ -2030,7 +2055,7 @@ int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
   const long long dense = A->dev.kind == kDense ? 16LL * A->dev.n : 1;
   if ((rc = op_scratch(ctx, gd, 0.0, sc, dense))) return rc;
   if (trans)
-    launch_bwd<1>(ctx->stream, sc.G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, sc.S, 0, nullptr, (float *)sc.dense);
+    launch_bwd<1>(ctx->stream, sc.G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, sc.S, 0, nullptr, (float *)sc.dense, 0);
   else
     launch_fwd<1>(ctx->stream, sc.G, 1, gd.m, A->dev.kind, A->dev.dtype, x, y, sc.S, 0);
   return hip_check(ctx, hipGetLastError(), "gemv launch");

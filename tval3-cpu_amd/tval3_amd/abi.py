@@ -22,7 +22,7 @@ class Options(C.Structure):
         ("numInitIterations", C.c_uint),
         ("skipMulRatio", C.c_double), ("maxRelativeChange", C.c_double),
         ("scal_double", C.c_int), ("tv_iso", C.c_int), ("dt_exact", C.c_int),
-        ("ls_exact", C.c_int), ("graph", C.c_int),
+        ("ls_exact", C.c_int), ("graph", C.c_int), ("submat_mb", C.c_int),
     ]
 
     def as_dict(self):
@@ -72,7 +72,7 @@ def default_options(**kw):
                 gamma=0.6, rate_gam=0.9, rate_cnt=2.0, c=1e-5, nonneg=0,
                 isreal=1, numInitIterations=0, skipMulRatio=0.0,
                 maxRelativeChange=0.0, scal_double=0, tv_iso=0, dt_exact=0,
-                ls_exact=0, graph=0)
+                ls_exact=0, graph=0, submat_mb=5)
     vals.update(kw)
     for k, v in vals.items():
         setattr(o, k, v)
