@@ -145,7 +145,7 @@ def run_ring(args, ctx):
     value = B * args.steps / t_steps
     rel_b0 = float(torch.linalg.norm(Us[0] - U0) / torch.linalg.norm(U0))
     iters = [inf.total_iters for inf in infos]
-    ops, roof = roofline(prof, {"A": "k_fwd_w64", "At": "k_bwd_w64"}, "ring")
+    ops, roof = roofline(prof, {"A": "k_fwd_w64q", "At": "k_bwd_w64q"}, "ring")
     # the projector walks its indices on the fly: its bytes are the CSR
     # equivalent (8 B per entry), and x / y stay cache-resident
     roof["cache_resident"] = True
@@ -241,7 +241,7 @@ def run_dense_batch(args, ctx):
     prof = ctx.profile_read()
     ctx.profile(False)
     value = B * args.steps / t_steps
-    ops, roof = roofline(prof, {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"},
+    ops, roof = roofline(prof, {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_blk"},
                          "dense-batch")
     # error against the FP64 solve of the same bf16-rounded matrices, on the
     # smallest instances (CPU oracle, double/double), beside the FP32 oracle's
@@ -394,7 +394,7 @@ def run_single(args, ctx, workload):
         dims = (q, p, 1)
         maxit = args.maxit if args.maxit_set else 300
         opts = tval3_amd.default_options(maxit=maxit, tv_iso=1, graph=args.graph)
-        kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}
+        kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_bs"}
         op_fn = lambda: O.Op.dense(A)  # noqa: E731
         desc = f"2D dense 64x64, m={m}, isotropic TV, maxit {maxit}"
     bt = ctx.tensor(b)
