@@ -198,7 +198,23 @@ struct tv3_tape {
 
   dim3 grid() const { return dim3(nblk); }
 
-  int err() { return hip_check(ctx, hipGetLastError(), "adjoint launch"); }
+  // device-to-device copies and fills on the tape's stream; the first failure
+  // is kept for err()
+  hipError_t async_err = hipSuccess;
+  void keep(hipError_t e) {
+    if (e != hipSuccess && async_err == hipSuccess) async_err = e;
+  }
+  void cp(float *dst, const float *src, size_t count) {
+    keep(hipMemcpyAsync(dst, src, sizeof(float) * count, hipMemcpyDeviceToDevice, st));
+  }
+  void zero(float *dst, size_t count) { keep(hipMemsetAsync(dst, 0, sizeof(float) * count, st)); }
+
+  int err() {
+    const hipError_t e = async_err;
+    async_err = hipSuccess;
+    if (e != hipSuccess) return hip_check(ctx, e, "adjoint copy");
+    return hip_check(ctx, hipGetLastError(), "adjoint launch");
+  }
 
   void fwdA(const float *x, float *y) {
     prof_begin(ctx, 0);
@@ -380,10 +396,10 @@ int tv3_unrolled_forward(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opt
   // rolling states in the segment buffers: slots 0, 1, 2 reused
   int slot = 0;
   auto save_ck = [&](int c, const float *u1, const float *d1, const float *u0, const float *d0) {
-    hipMemcpyAsync(t->ck_vec(c, 0), u1, sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
-    hipMemcpyAsync(t->ck_vec(c, 1), d1, sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
-    if (u0) hipMemcpyAsync(t->ck_vec(c, 2), u0, sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
-    if (d0) hipMemcpyAsync(t->ck_vec(c, 3), d0, sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
+    t->cp(t->ck_vec(c, 0), u1, n);
+    t->cp(t->ck_vec(c, 1), d1, n);
+    if (u0) t->cp(t->ck_vec(c, 2), u0, n);
+    if (d0) t->cp(t->ck_vec(c, 3), d0, n);
   };
   save_ck(0, u_cur, d_cur, nullptr, nullptr);
   for (int k = 1; k <= K; ++k) {
@@ -400,8 +416,11 @@ int tv3_unrolled_forward(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opt
   hipLaunchKernelGGL(k_ad_axpby, t->grid(), dim3(kBlock), 0, t->st, n, (float)(1.0 / t->scl), u_cur,
                      0.0f, nullptr, U);
   // keep u_K for the adjoint of the output scaling
-  hipMemcpyAsync(t->tn2, u_cur, sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
-  if ((rc = hip_check(ctx, hipStreamSynchronize(t->st), "forward sync"))) { tv3_tape_destroy(t); return rc; }
+  t->cp(t->tn2, u_cur, n);
+  if ((rc = t->err()) || (rc = hip_check(ctx, hipStreamSynchronize(t->st), "forward sync"))) {
+    tv3_tape_destroy(t);
+    return rc;
+  }
   *tape = t;
   return TV3_OK;
 }
@@ -416,13 +435,14 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
   const float ib = (float)(1.0 / t->beta);
   const float mr = (float)t->mr;
   double mbar = 0;
+  float hv[2];  // host staging of two entries of db; alive until the final sync
   // U = u_K / scl: u_K-bar = dU / scl, scl-bar = -<dU, u_K> / scl^2
   double sclbar;
   if ((rc = t->dot(dU, t->tn2, &sclbar))) return rc;
   sclbar = -sclbar / (t->scl * t->scl);
-  hipMemsetAsync(t->ub, 0, sizeof(float) * n * 3, t->st);
-  hipMemsetAsync(t->dbar, 0, sizeof(float) * n * 3, t->st);
-  hipMemsetAsync(t->bsbar, 0, sizeof(float) * m, t->st);
+  t->zero(t->ub, n * 3);
+  t->zero(t->dbar, n * 3);
+  t->zero(t->bsbar, m);
   hipLaunchKernelGGL(k_ad_axpby, t->grid(), dim3(kBlock), 0, t->st, n, (float)(1.0 / t->scl), dU,
                      0.0f, nullptr, t->ubar(K));
   // adjoint of d_k = F(u_k): u-bar += J d-bar, bs-bar -= m A d-bar, m-bar += <r_k, A d-bar>
@@ -437,18 +457,18 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
     t->bwdA(t->tm1, t->tn1);
     hipLaunchKernelGGL(k_ad_jac, t->grid(), dim3(kBlock), 0, t->st, t->q, t->p, u, ib, dbk,
                        t->tn1, mr, 1.0f, 0.0f, nullptr, 1.0f, t->ubar(k));
-    hipMemsetAsync(dbk, 0, sizeof(float) * n, t->st);
+    t->zero(dbk, n);
     return t->err();
   };
   const int nseg = (K + ck - 1) / ck;
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int k0 = sg * ck, k1 = std::min(K, k0 + ck);
     // recompute u_{k0-1} .. u_{k1}: segment index j = k - k0 + 1
-    hipMemcpyAsync(t->seg_u(1), t->ck_vec(sg, 0), sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
-    hipMemcpyAsync(t->seg_d(1), t->ck_vec(sg, 1), sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
+    t->cp(t->seg_u(1), t->ck_vec(sg, 0), n);
+    t->cp(t->seg_d(1), t->ck_vec(sg, 1), n);
     if (k0 > 0) {
-      hipMemcpyAsync(t->seg_u(0), t->ck_vec(sg, 2), sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
-      hipMemcpyAsync(t->seg_d(0), t->ck_vec(sg, 3), sizeof(float) * n, hipMemcpyDeviceToDevice, t->st);
+      t->cp(t->seg_u(0), t->ck_vec(sg, 2), n);
+      t->cp(t->seg_d(0), t->ck_vec(sg, 3), n);
     }
     // r_{k0} = A u_{k0} - bs
     t->fwdA(t->seg_u(1), t->seg_r(1));
@@ -474,7 +494,7 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
                          t->ubar(k - 1), t->ubar(k - 1));
       hipLaunchKernelGGL(k_ad_axpby, t->grid(), dim3(kBlock), 0, t->st, n, (float)(-al * tau),
                          t->ubar(k), 1.0f, t->dbr(k - 1), t->dbr(k - 1));
-      hipMemsetAsync(t->ubar(k), 0, sizeof(float) * n, t->st);
+      t->zero(t->ubar(k), n);
       if (t->first[i]) {
         // tau = N / Dn: d-bar += tbar (2/Dn) (d - tau (Dt D d + m A^T A d)); m-bar -= tbar tau |Ad|^2 / Dn
         const double Dn = t->tDn[i];
@@ -510,7 +530,6 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
     // scl = th / |b[imax] - b[imin]|: d scl / d b[imax] = -scl sgn / bdif, d/d b[imin] = +scl sgn / bdif
     const double sgn = t->bdif_signed < 0 ? -1.0 : 1.0, bdif = std::fabs(t->bdif_signed);
     const double gmax = -t->scl * sgn / bdif * sclbar;
-    float hv[2];
     if ((rc = hip_check(ctx, hipMemcpyAsync(&hv[0], db + t->imax, sizeof(float), hipMemcpyDeviceToHost, t->st), "copy")) ||
         (rc = hip_check(ctx, hipMemcpyAsync(&hv[1], db + t->imin, sizeof(float), hipMemcpyDeviceToHost, t->st), "copy")) ||
         (rc = hip_check(ctx, hipStreamSynchronize(t->st), "sync")))
@@ -523,7 +542,8 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
         (rc = hip_check(ctx, hipMemcpyAsync(db + t->imin, &hv[1], sizeof(float), hipMemcpyHostToDevice, t->st), "copy")))
       return rc;
   }
-  if ((rc = hip_check(ctx, hipStreamSynchronize(t->st), "backward sync"))) return rc;
+  if ((rc = t->err()) || (rc = hip_check(ctx, hipStreamSynchronize(t->st), "backward sync")))
+    return rc;
   *dmu = mbar / t->beta;
   return TV3_OK;
 }

@@ -4,15 +4,18 @@
 // 3d/tval3_cpu_3d.cpp:155-1378) as batched kernels over groups of W
 // lane-interleaved instances.  The reference's ~107 vector passes per inner
 // iteration (2-D) become five fused passes:
-//   k_tau_bb / k_tau_first  get_tau (2d:425-453) dot products
+//   k_tau_first             get_tau's first-iteration dot products
+//                           (2d:430-442); the Barzilai-Borwein ones
+//                           (2d:443-450) come from k_wstep
 //   k_descend               descend (2d:481-493): U = Up - tau d, nonneg, D(U),
 //                           dU, and get_g's V, g2 = Dt(V), lam2, lam4
 //                           (2d:396-423) in one stencil pass
 //   k_mpost                 get_g's Aub, lam3, lam5 and the A^T-skip norms
 //   k_bt_*                  update_g (2d:495-524) interpolation + lam2..lam5
 //   k_wstep                 update_W (2d:651-671: shrinkage, V, Dt, lam1,
-//                           lam2, lam4) fused with uup, get_gradient
-//                           (2d:578-585) and the two norms of the stop test
+//                           lam2, lam4) fused with s = U - Up, get_gradient
+//                           (2d:578-585), the two norms of the stop test
+//                           and the next get_tau's <s, s>, <s, y>
 //   k_outer1, k_mlp_*       RelChgOut and update_mlp (2d:673-694)
 // plus the A / A^T products of tv3_plan.hip.  Every scalar of the reference
 // (mu, beta, lam1..5, f, C, Q, gam, tau, alpha, counters) lives per instance
@@ -43,9 +46,12 @@ namespace {
 struct Arrs {
   float *U, *Up, *Ux, *Uxp, *Uy, *Uyp, *Uz, *Uzp, *g, *gp, *g2, *g2p;
   float *dU, *dUx, *dUy, *dUz, *dg, *dg2, *d, *DtsAtd, *Wx, *Wy, *Wz;
-  float *sx, *sy, *sz, *Atb, *uup, *rcdU, *Uout;
+  float *sx, *sy, *sz, *Atb, *tmpn, *rcdU, *Uout;
   float *b, *Au, *Aup, *dAu, *delta, *lastAu, *tmpm;
 };
+
+// threads per block of the one-block-per-instance fin_* kernels
+constexpr int kFin = 512;
 
 struct KA {
   const GroupDesc *G;
@@ -98,24 +104,6 @@ __device__ __forceinline__ int uni(int v) {
 // get_tau (2d:425-453)
 // ---------------------------------------------------------------------------
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_tau_bb(KA K, Arrs a) {
-  double acc[2] = {0.0, 0.0};
-  {
-    NLOOP_BEGIN(W)
-    const float mdb = (float)S.muDbeta;
-    const float dg = a.g[o] - a.gp[o];
-    const float dg2 = a.g2[o] - a.g2p[o];
-    const float tmp = fmaf(mdb, dg, dg2);
-    const float u = a.uup[o];
-    acc[0] += (double)u * (double)u;
-    acc[1] += (double)u * (double)tmp;
-    LOOP_END
-  }
-  const int sl[2] = {0, 1};
-  block_partials<W, 2>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
-}
-
-template <int W>
 __global__ __launch_bounds__(kBlock) void k_tau_first(KA K, Arrs a) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   {
@@ -153,65 +141,84 @@ __global__ __launch_bounds__(kBlock) void k_msq(KA K, const float *x) {
 // ---------------------------------------------------------------------------
 // descend (2d:481-493) + get_g's n-vector part (2d:396-417)
 // ---------------------------------------------------------------------------
-// With poly != 0 it also gathers, for the backtracking line search, the
+// With POLY it also gathers, for the backtracking line search, the
 // quadratic coefficients of lam2(alpha) and lam4(alpha): V_dir(alpha) =
 // (U_dir_p - W_dir) + alpha dU_dir with dU_dir = U_dir - U_dir_p (slots 7-11).
-template <int W, class TS>
+// D3: the batch holds 3-D volumes (each group still tests its own dim_z).
+//
+// The vector kernels read every value of an element before they write any:
+// the arrays are distinct, but the compiler cannot prove it and would keep
+// each load behind the preceding stores, one memory latency at a time.
+template <int W, class TS, bool D3, bool POLY>
 __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
-                                                    int exact, int poly) {
+                                                    int exact) {
   double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     NLOOP_BEGIN(W)
     const float tf = (float)(-1 * (TS)S.tau);
-    auto un = [&](int j) -> float {
-      const long long oj = gd.off_n + (long long)j * W + lb;
-      float u = fmaf(tf, a.d[oj], a.Up[oj]);
+    auto un = [&](float dd, float pp) -> float {
+      float u = fmaf(tf, dd, pp);
       if (nonneg) u = (u < 0.0f) ? 0.0f : u;
       return u;
     };
+    auto unj = [&](int j) -> float { return un(AT(a.d, j), AT(a.Up, j)); };
     const Vox w(v, gd.q, gd.p, gd.r);
-    const bool z3 = gd.r > 1;
-    const float u0 = un(v);
-    const float ux = un(w.xn()) - u0;
-    const float uy = un(w.yn()) - u0;
-    const float uz = z3 ? un(w.zn()) - u0 : 0.0f;
-    a.U[o] = u0;
-    a.Ux[o] = ux;
-    a.Uy[o] = uy;
-    if (z3) a.Uz[o] = uz;
-    a.dU[o] = u0 - a.Up[o];
-    const float Vx0 = ux - a.Wx[o];
-    const float Vy0 = uy - a.Wy[o];
-    const float Vz0 = z3 ? uz - a.Wz[o] : 0.0f;
+    const bool z3 = D3 && gd.r > 1;
+    const int xn = w.xn(), yn = w.yn(), xp = w.xp(), yp = w.yp();
+    // ---- loads
+    const float d0 = a.d[o], p0 = a.Up[o];
+    const float dxn = AT(a.d, xn), pxn = AT(a.Up, xn);
+    const float dyn = AT(a.d, yn), pyn = AT(a.Up, yn);
+    const float dxp = AT(a.d, xp), pxp = AT(a.Up, xp);
+    const float dyp = AT(a.d, yp), pyp = AT(a.Up, yp);
+    const float wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
+    const float wxp = AT(a.Wx, xp), wyp = AT(a.Wy, yp);
+    float qx = 0.0f, qy = 0.0f, qz = 0.0f;  // Ux, Uy, Uz of the previous iterate
+    if (POLY) {
+      qx = a.Uxp[o];
+      qy = a.Uyp[o];
+    }
+    float dzn = 0.0f, pzn = 0.0f, wz = 0.0f, sz = 0.0f;
+    if (z3) {
+      dzn = AT(a.d, w.zn());
+      pzn = AT(a.Up, w.zn());
+      wz = a.Wz[o];
+      sz = a.sz[o];
+      if (POLY) qz = a.Uzp[o];
+    }
+    // ---- U = Up - tau d, D(U), V = D(U) - W
+    const float u0 = un(d0, p0);
+    const float ux = un(dxn, pxn) - u0;
+    const float uy = un(dyn, pyn) - u0;
+    const float uz = z3 ? un(dzn, pzn) - u0 : 0.0f;
+    const float Vx0 = ux - wx;
+    const float Vy0 = uy - wy;
+    const float Vz0 = z3 ? uz - wz : 0.0f;
     acc[0] += (double)Vx0 * (double)Vx0;
     acc[1] += (double)Vy0 * (double)Vy0;
-    acc[3] += (double)a.sx[o] * (double)Vx0;
-    acc[4] += (double)a.sy[o] * (double)Vy0;
+    acc[3] += (double)sx * (double)Vx0;
+    acc[4] += (double)sy * (double)Vy0;
     if (z3) {
       acc[2] += (double)Vz0 * (double)Vz0;
-      acc[5] += (double)a.sz[o] * (double)Vz0;
+      acc[5] += (double)sz * (double)Vz0;
     }
-    const float d0 = a.d[o];
     acc[6] += (double)d0 * (double)d0;
-    if (poly) {
-      const float px = a.Uxp[o], py = a.Uyp[o];
-      const double vx = (double)px - (double)a.Wx[o], dx = (double)(ux - px);
-      const double vy = (double)py - (double)a.Wy[o], dy = (double)(uy - py);
+    if (POLY) {
+      const double vx = (double)qx - (double)wx, dx = (double)(ux - qx);
+      const double vy = (double)qy - (double)wy, dy = (double)(uy - qy);
       double A2 = vx * vx + vy * vy, B2 = vx * dx + vy * dy, C2 = dx * dx + dy * dy;
-      double A4 = (double)a.sx[o] * vx + (double)a.sy[o] * vy;
-      double B4 = (double)a.sx[o] * dx + (double)a.sy[o] * dy;
+      double A4 = (double)sx * vx + (double)sy * vy;
+      double B4 = (double)sx * dx + (double)sy * dy;
       if (z3) {
-        const float pz = a.Uzp[o];
-        const double vz = (double)pz - (double)a.Wz[o], dz = (double)(uz - pz);
+        const double vz = (double)qz - (double)wz, dz = (double)(uz - qz);
         A2 += vz * vz; B2 += vz * dz; C2 += dz * dz;
-        A4 += (double)a.sz[o] * vz; B4 += (double)a.sz[o] * dz;
+        A4 += (double)sz * vz; B4 += (double)sz * dz;
       }
       acc[7] += A2; acc[8] += B2; acc[9] += C2; acc[10] += A4; acc[11] += B4;
     }
-    // g2 = Dt(V) at v
-    const int xp = w.xp(), yp = w.yp();
-    const float VXp = (u0 - un(xp)) - AT(a.Wx, xp);
-    const float VYp = (u0 - un(yp)) - AT(a.Wy, yp);
+    // ---- g2 = Dt(V) at v
+    const float VXp = (u0 - un(dxp, pxp)) - wxp;
+    const float VYp = (u0 - un(dyp, pyp)) - wyp;
     float t = VXp - Vx0;
     if (!z3) {
       t = t + VYp;
@@ -222,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
         float Yb = VYp;
         if (w.y == 0) {  // last row of the previous slice
           const int j = v - w.q;
-          Yb = (un(v - w.s) - un(j)) - AT(a.Wy, j);
+          Yb = (unj(v - w.s) - unj(j)) - AT(a.Wy, j);
         }
         t = t + Yb;
         t = t - Vy0;
@@ -231,10 +238,16 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
         t = t + VYp;
         t = t - Vy0;
       }
-      const float VZp = (u0 - un(zp)) - AT(a.Wz, zp);
+      const float VZp = (u0 - unj(zp)) - AT(a.Wz, zp);
       t = t + VZp;
       t = t - Vz0;
     }
+    // ---- stores
+    a.U[o] = u0;
+    a.Ux[o] = ux;
+    a.Uy[o] = uy;
+    if (z3) a.Uz[o] = uz;
+    a.dU[o] = u0 - p0;
     a.g2[o] = t;
     LOOP_END
   }
@@ -302,11 +315,16 @@ __global__ __launch_bounds__(kBlock) void k_skipg(KA K, Arrs a) {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bt_diff_n(KA K, Arrs a) {
   NLOOP_BEGIN(W)
-  a.dg[o] = a.g[o] - a.gp[o];
-  a.dg2[o] = a.g2[o] - a.g2p[o];
-  a.dUx[o] = a.Ux[o] - a.Uxp[o];
-  a.dUy[o] = a.Uy[o] - a.Uyp[o];
-  if (gd.r > 1) a.dUz[o] = a.Uz[o] - a.Uzp[o];
+  const float dg = a.g[o] - a.gp[o];
+  const float dg2 = a.g2[o] - a.g2p[o];
+  const float dux = a.Ux[o] - a.Uxp[o];
+  const float duy = a.Uy[o] - a.Uyp[o];
+  const float duz = gd.r > 1 ? a.Uz[o] - a.Uzp[o] : 0.0f;
+  a.dg[o] = dg;
+  a.dg2[o] = dg2;
+  a.dUx[o] = dux;
+  a.dUy[o] = duy;
+  if (gd.r > 1) a.dUz[o] = duz;
   LOOP_END
 }
 
@@ -328,25 +346,41 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
   {
     NLOOP_BEGIN(W)
     const TS al = (TS)S.alpha;
-    a.g[o] = interp<TS>(a.gp[o], al, a.dg[o]);
-    a.g2[o] = interp<TS>(a.g2p[o], al, a.dg2[o]);
-    a.U[o] = interp<TS>(a.Up[o], al, a.dU[o]);
-    const float ux = interp<TS>(a.Uxp[o], al, a.dUx[o]);
-    const float uy = interp<TS>(a.Uyp[o], al, a.dUy[o]);
-    a.Ux[o] = ux;
-    a.Uy[o] = uy;
-    const float Vx = ux - a.Wx[o], Vy = uy - a.Wy[o];
+    const bool z3 = gd.r > 1;
+    // ---- loads
+    const float gp = a.gp[o], dg = a.dg[o], g2p = a.g2p[o], dg2 = a.dg2[o];
+    const float up = a.Up[o], du = a.dU[o];
+    const float uxp = a.Uxp[o], dux = a.dUx[o], uyp = a.Uyp[o], duy = a.dUy[o];
+    const float wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
+    float uzp = 0.0f, duz = 0.0f, wz = 0.0f, sz = 0.0f;
+    if (z3) {
+      uzp = a.Uzp[o];
+      duz = a.dUz[o];
+      wz = a.Wz[o];
+      sz = a.sz[o];
+    }
+    // ---- the state at alpha, lam2, lam4
+    const float ux = interp<TS>(uxp, al, dux);
+    const float uy = interp<TS>(uyp, al, duy);
+    const float Vx = ux - wx, Vy = uy - wy;
     acc[0] += (double)Vx * (double)Vx;
     acc[1] += (double)Vy * (double)Vy;
-    acc[3] += (double)a.sx[o] * (double)Vx;
-    acc[4] += (double)a.sy[o] * (double)Vy;
-    if (gd.r > 1) {
-      const float uz = interp<TS>(a.Uzp[o], al, a.dUz[o]);
-      a.Uz[o] = uz;
-      const float Vz = uz - a.Wz[o];
+    acc[3] += (double)sx * (double)Vx;
+    acc[4] += (double)sy * (double)Vy;
+    float uz = 0.0f;
+    if (z3) {
+      uz = interp<TS>(uzp, al, duz);
+      const float Vz = uz - wz;
       acc[2] += (double)Vz * (double)Vz;
-      acc[5] += (double)a.sz[o] * (double)Vz;
+      acc[5] += (double)sz * (double)Vz;
     }
+    // ---- stores
+    a.g[o] = interp<TS>(gp, al, dg);
+    a.g2[o] = interp<TS>(g2p, al, dg2);
+    a.U[o] = interp<TS>(up, al, du);
+    a.Ux[o] = ux;
+    a.Uy[o] = uy;
+    if (z3) a.Uz[o] = uz;
     LOOP_END
   }
   const int sl[6] = {0, 1, 2, 3, 4, 5};
@@ -380,28 +414,41 @@ __global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
   {
     NLOOP_BEGIN(W)
     const TS al = (TS)S.alpha;
-    const float gp = a.gp[o], g2p = a.g2p[o];
-    a.g[o] = interp<TS>(gp, al, a.g[o] - gp);
-    a.g2[o] = interp<TS>(g2p, al, a.g2[o] - g2p);
-    a.U[o] = interp<TS>(a.Up[o], al, a.dU[o]);
-    const float uxp = a.Uxp[o], uyp = a.Uyp[o];
-    const float ux = interp<TS>(uxp, al, a.Ux[o] - uxp);
-    const float uy = interp<TS>(uyp, al, a.Uy[o] - uyp);
-    a.Ux[o] = ux;
-    a.Uy[o] = uy;
-    const float Vx = ux - a.Wx[o], Vy = uy - a.Wy[o];
+    const bool z3 = gd.r > 1;
+    // ---- loads
+    const float gp = a.gp[o], g2p = a.g2p[o], gc = a.g[o], g2c = a.g2[o];
+    const float up = a.Up[o], du = a.dU[o];
+    const float uxp = a.Uxp[o], uyp = a.Uyp[o], uxc = a.Ux[o], uyc = a.Uy[o];
+    const float wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
+    float uzp = 0.0f, uzc = 0.0f, wz = 0.0f, sz = 0.0f;
+    if (z3) {
+      uzp = a.Uzp[o];
+      uzc = a.Uz[o];
+      wz = a.Wz[o];
+      sz = a.sz[o];
+    }
+    // ---- the state at alpha, lam2, lam4
+    const float ux = interp<TS>(uxp, al, uxc - uxp);
+    const float uy = interp<TS>(uyp, al, uyc - uyp);
+    const float Vx = ux - wx, Vy = uy - wy;
     acc[0] += (double)Vx * (double)Vx;
     acc[1] += (double)Vy * (double)Vy;
-    acc[3] += (double)a.sx[o] * (double)Vx;
-    acc[4] += (double)a.sy[o] * (double)Vy;
-    if (gd.r > 1) {
-      const float uzp = a.Uzp[o];
-      const float uz = interp<TS>(uzp, al, a.Uz[o] - uzp);
-      a.Uz[o] = uz;
-      const float Vz = uz - a.Wz[o];
+    acc[3] += (double)sx * (double)Vx;
+    acc[4] += (double)sy * (double)Vy;
+    float uz = 0.0f;
+    if (z3) {
+      uz = interp<TS>(uzp, al, uzc - uzp);
+      const float Vz = uz - wz;
       acc[2] += (double)Vz * (double)Vz;
-      acc[5] += (double)a.sz[o] * (double)Vz;
+      acc[5] += (double)sz * (double)Vz;
     }
+    // ---- stores
+    a.g[o] = interp<TS>(gp, al, gc - gp);
+    a.g2[o] = interp<TS>(g2p, al, g2c - g2p);
+    a.U[o] = interp<TS>(up, al, du);
+    a.Ux[o] = ux;
+    a.Uy[o] = uy;
+    if (z3) a.Uz[o] = uz;
     LOOP_END
   }
   const int sl[6] = {0, 1, 2, 3, 4, 5};
@@ -442,55 +489,83 @@ __global__ __launch_bounds__(kBlock) void k_D(KA K, const float *U, float *Ux,
 }
 
 // ---------------------------------------------------------------------------
-// update_W (2d:651-671) + uup + get_gradient + stop-test norms
+// update_W (2d:651-671) + get_gradient + stop-test norms + get_tau's
+// <s, s>, <s, y> of the next iteration
 // ---------------------------------------------------------------------------
-template <int W, class TS, bool INIT>
-__global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int iso,
-                                                  int exact) {
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+template <int W, class TS, bool INIT, bool D3, bool ISO>
+__global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
+  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     NLOOP_BEGIN(W)
     const TS beta = (TS)S.beta;
     const TS ib = (TS)1 / beta;
-    const bool z3 = gd.r > 1;
+    const bool z3 = D3 && gd.r > 1;
     const Vox w(v, gd.q, gd.p, gd.r);
-    // shrinkage at voxel j; returns the T_scal lam1 term when asked
-    auto shr = [&](int j, float &wx, float &wy, float &wz) -> TS {
+    // D(U) and sigma at one voxel; the anisotropic shrinkage of a neighbour
+    // needs one direction only (dir 0, 1, 2; -1: all)
+    struct Cell {
+      float ux, uy, uz, sx, sy, sz;
+    };
+    auto cell = [&](int j, int dir) -> Cell {
       const long long oj = gd.off_n + (long long)j * W + lb;
-      if (iso)
-        return shrink_iso<TS>(a.Ux[oj], a.Uy[oj], z3 ? a.Uz[oj] : 0.0f,
-                              a.sx[oj], a.sy[oj], z3 ? a.sz[oj] : 0.0f, z3,
-                              beta, ib, wx, wy, wz);
+      Cell c{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      if (ISO || dir <= 0) { c.ux = a.Ux[oj]; c.sx = a.sx[oj]; }
+      if (ISO || dir < 0 || dir == 1) { c.uy = a.Uy[oj]; c.sy = a.sy[oj]; }
+      if (z3 && (ISO || dir < 0 || dir == 2)) { c.uz = a.Uz[oj]; c.sz = a.sz[oj]; }
+      return c;
+    };
+    // shrinkage of one voxel; returns the T_scal lam1 term
+    auto shr = [&](const Cell &c, float &wx, float &wy, float &wz) -> TS {
+      if (ISO)
+        return shrink_iso<TS>(c.ux, c.uy, c.uz, c.sx, c.sy, c.sz, z3, beta, ib, wx, wy, wz);
       TS tx, ty, tz = 0;
-      wx = shrink1<TS>(a.Ux[oj], a.sx[oj], beta, ib, tx);
-      wy = shrink1<TS>(a.Uy[oj], a.sy[oj], beta, ib, ty);
-      wz = z3 ? shrink1<TS>(a.Uz[oj], a.sz[oj], beta, ib, tz) : 0.0f;
+      wx = shrink1<TS>(c.ux, c.sx, beta, ib, tx);
+      wy = shrink1<TS>(c.uy, c.sy, beta, ib, ty);
+      wz = z3 ? shrink1<TS>(c.uz, c.sz, beta, ib, tz) : 0.0f;
       TS t = tx + ty;
       if (z3) t = t + tz;
       return t;
     };
+    // ---- loads
+    const int xp = w.xp(), yp = w.yp();
+    const Cell c0 = cell(v, -1);
+    const Cell cx = cell(xp, 0);
+    const Cell cy = cell(yp, 1);
+    Cell cz{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, cj = cz;
+    if (z3) {
+      cz = cell(w.zp(), 2);
+      // the y-term from the last row of the previous slice
+      if (w.v >= w.q && w.y == 0 && !exact) cj = cell(v - w.q, 1);
+    }
+    float up = 0.0f, uc = 0.0f, gc = 0.0f, gpv = 0.0f, g2pv = 0.0f, dts = 0.0f;
+    if (!INIT) {
+      up = a.Up[o];
+      uc = a.U[o];
+      gc = a.g[o];
+      gpv = a.gp[o];
+      g2pv = a.g2p[o];
+      dts = a.DtsAtd[o];
+    }
+    // ---- W = shrink(D(U) + sigma/beta), V = D(U) - W, lam1, lam2, lam4
     float wx0, wy0, wz0;
-    const TS l1 = shr(v, wx0, wy0, wz0);
+    const TS l1 = shr(c0, wx0, wy0, wz0);
     acc[0] += (double)l1;
-    a.Wx[o] = wx0;
-    a.Wy[o] = wy0;
-    if (z3) a.Wz[o] = wz0;
-    const float Vx0 = a.Ux[o] - wx0, Vy0 = a.Uy[o] - wy0;
-    const float Vz0 = z3 ? a.Uz[o] - wz0 : 0.0f;
+    const float Vx0 = c0.ux - wx0, Vy0 = c0.uy - wy0;
+    const float Vz0 = z3 ? c0.uz - wz0 : 0.0f;
     acc[1] += (double)Vx0 * (double)Vx0;
     acc[2] += (double)Vy0 * (double)Vy0;
-    acc[4] += (double)a.sx[o] * (double)Vx0;
-    acc[5] += (double)a.sy[o] * (double)Vy0;
+    acc[4] += (double)c0.sx * (double)Vx0;
+    acc[5] += (double)c0.sy * (double)Vy0;
     if (z3) {
       acc[3] += (double)Vz0 * (double)Vz0;
-      acc[6] += (double)a.sz[o] * (double)Vz0;
+      acc[6] += (double)c0.sz * (double)Vz0;
     }
+    // ---- g2 = Dt(V) at v
     float tx, ty, tz;
-    const int xp = w.xp(), yp = w.yp();
-    shr(xp, tx, ty, tz);
-    const float VXp = AT(a.Ux, xp) - tx;
-    shr(yp, tx, ty, tz);
-    const float VYp = AT(a.Uy, yp) - ty;
+    shr(cx, tx, ty, tz);
+    const float VXp = cx.ux - tx;
+    shr(cy, tx, ty, tz);
+    const float VYp = cy.uy - ty;
     float t = VXp - Vx0;
     if (!z3) {
       t = t + VYp;
@@ -499,9 +574,8 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int iso,
       if (w.v >= w.q && (w.y > 0 || !exact)) {
         float Yb = VYp;
         if (w.y == 0) {
-          const int j = v - w.q;
-          shr(j, tx, ty, tz);
-          Yb = AT(a.Uy, j) - ty;
+          shr(cj, tx, ty, tz);
+          Yb = cj.uy - ty;
         }
         t = t + Yb;
         t = t - Vy0;
@@ -510,25 +584,37 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int iso,
         t = t + VYp;
         t = t - Vy0;
       }
-      const int zp = w.zp();
-      shr(zp, tx, ty, tz);
-      const float VZp = AT(a.Uz, zp) - tz;
+      shr(cz, tx, ty, tz);
+      const float VZp = cz.uz - tz;
       t = t + VZp;
       t = t - Vz0;
     }
-    a.g2[o] = t;
+    float dn = 0.0f;
     if (!INIT) {
-      const float up = a.Up[o];
-      const float uu = a.U[o] - up;
-      a.uup[o] = uu;
+      const float uu = uc - up;
       acc[7] += (double)up * (double)up;
       acc[8] += (double)uu * (double)uu;
-      a.d[o] = fmaf((float)S.muDbeta, a.g[o], a.DtsAtd[o] + t);
+      const float mdb = (float)S.muDbeta;
+      dn = fmaf(mdb, gc, dts + t);
+      // the next iteration's Barzilai-Borwein step (get_tau, 2d:443-450):
+      // s = U - Up, y = (g2 - g2p) + (mu/beta) (g - gp).  No kernel between
+      // here and the next get_tau changes these vectors or mu/beta of an
+      // instance that is not restarting its inner loop (which takes the
+      // first-iteration formula instead), so the dot products are formed
+      // here; slot 8 holds <s, s>.
+      const float y = fmaf(mdb, gc - gpv, t - g2pv);
+      acc[9] += (double)uu * (double)y;
     }
+    // ---- stores
+    a.Wx[o] = wx0;
+    a.Wy[o] = wy0;
+    if (z3) a.Wz[o] = wz0;
+    a.g2[o] = t;
+    if (!INIT) a.d[o] = dn;
     LOOP_END
   }
-  const int sl[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
-  block_partials<W, 9>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+  const int sl[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+  block_partials<W, 10>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
 }
 
 // d = DtsAtd + g2 + muDbeta*g, with DtsAtd = (beta0/beta) d first when
@@ -537,11 +623,13 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_grad(KA K, Arrs a, int scale) {
   NLOOP_BEGIN(W)
   float dts = a.DtsAtd[o];
+  const float dc = scale ? a.d[o] : 0.0f;
+  const float gc = a.g[o], g2c = a.g2[o];
   if (scale) {
-    dts = (float)S.ratio * a.d[o];
+    dts = (float)S.ratio * dc;
     a.DtsAtd[o] = dts;
   }
-  a.d[o] = fmaf((float)S.muDbeta, a.g[o], dts + a.g2[o]);
+  a.d[o] = fmaf((float)S.muDbeta, gc, dts + g2c);
   LOOP_END
 }
 
@@ -569,15 +657,19 @@ __global__ __launch_bounds__(kBlock) void k_mlp_n(KA K, Arrs a) {
   {
     NLOOP_BEGIN(W)
     const float nb = (float)(-1 * S.beta);
+    const bool z3 = gd.r > 1;
     const float Vx = a.Ux[o] - a.Wx[o], Vy = a.Uy[o] - a.Wy[o];
     const float sx = fmaf(nb, Vx, a.sx[o]), sy = fmaf(nb, Vy, a.sy[o]);
+    float Vz = 0.0f, sz = 0.0f;
+    if (z3) {
+      Vz = a.Uz[o] - a.Wz[o];
+      sz = fmaf(nb, Vz, a.sz[o]);
+    }
     a.sx[o] = sx;
     a.sy[o] = sy;
     acc[0] += (double)sx * (double)Vx;
     acc[1] += (double)sy * (double)Vy;
-    if (gd.r > 1) {
-      const float Vz = a.Uz[o] - a.Wz[o];
-      const float sz = fmaf(nb, Vz, a.sz[o]);
+    if (z3) {
       a.sz[o] = sz;
       acc[2] += (double)sz * (double)Vz;
     }
@@ -736,17 +828,18 @@ struct FA {
 // Sums of NS partial-sum slots of one instance by a 256-thread block, in one
 // pass: thread t adds partials t, t+256, ...; wavefront butterfly; the four
 // wavefronts in order.  Every thread receives the results.
+// (one-block-per-instance kernels run with kFin threads)
 template <int NS>
 __device__ __forceinline__ void fin_sums(const FA &F, int inst,
                                          const int (&slots)[NS],
                                          double (&out)[NS]) {
-  __shared__ double red[4][NS];
+  __shared__ double red[kFin / 64][NS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double x[NS];
 #pragma unroll
   for (int k = 0; k < NS; ++k) x[k] = 0.0;
-  // all slots' loads are issued before any reduction
-  for (int i = threadIdx.x; i < kNblkMax; i += kBlock) {
+  // one partial per thread and slot: all loads are issued before any reduction
+  for (int i = threadIdx.x; i < kNblkMax; i += kFin) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       const int nb = slots[k] >= kSlotM ? F.nblk_m : F.nblk_n;
@@ -760,8 +853,11 @@ __device__ __forceinline__ void fin_sums(const FA &F, int inst,
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < NS; ++k)
-    out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+  for (int k = 0; k < NS; ++k) {
+    double t = red[0][k];
+    for (int w = 1; w < kFin / 64; ++w) t += red[w][k];
+    out[k] = t;
+  }
   __syncthreads();
 }
 
@@ -825,7 +921,7 @@ constexpr uint32_t kStepFlags =
 // the per-step flags of every instance.  Fallback mode (req = F_FB): the
 // first-iteration formula for the instances in the fallback.
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_tau(FA F) {
+__global__ __launch_bounds__(kFin) void fin_tau(FA F) {
   const int inst = blockIdx.x;
   InstScal &s = F.S[inst];
   const uint32_t fl = s.flags;
@@ -844,7 +940,8 @@ __global__ __launch_bounds__(kBlock) void fin_tau(FA F) {
     const int sl[5] = {0, 1, 2, 3, kSlotM};
     fin_sums<5>(F, inst, sl, r);
   } else {
-    const int sl[2] = {0, 1};
+    // <s, s> and <s, y>, formed by the previous update_W pass
+    const int sl[2] = {8, 9};
     double r2[2];
     fin_sums<2>(F, inst, sl, r2);
     r[0] = r2[0];
@@ -875,7 +972,7 @@ __global__ __launch_bounds__(kBlock) void fin_tau(FA F) {
 // A^T-skipping state machine of get_g_mult (2d:300-357).  force: the
 // initial get_g and the 5-strike fallback.
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_skip(FA F, int force) {
+__global__ __launch_bounds__(kFin) void fin_skip(FA F, int force) {
   FIN_BEGIN
   const int sl[2] = {kSlotM + 2, kSlotM + 3};
   double r[2];
@@ -910,7 +1007,7 @@ __global__ __launch_bounds__(kBlock) void fin_skip(FA F, int force) {
 // ls_exact = 1 one round at a time by k_bt_upd_* and fin_lams.  mode 1:
 // after the 5-strike fallback.  mode 2: the initial get_g.
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_descend(FA F, int mode) {
+__global__ __launch_bounds__(kFin) void fin_descend(FA F, int mode) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
   const bool poly = mode == 0 && !F.O.ls_exact;
@@ -979,7 +1076,7 @@ __global__ __launch_bounds__(kBlock) void fin_descend(FA F, int mode) {
 // 2d:495-524).  mode 0: one reference backtracking round (2d:555-574);
 // mode 1: the final interpolation of the quadratic line search.
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_lams(FA F, int mode) {
+__global__ __launch_bounds__(kFin) void fin_lams(FA F, int mode) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
   const int sl[8] = {0, 1, 2, 3, 4, 5, kSlotM, kSlotM + 1};
@@ -1011,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void fin_lams(FA F, int mode) {
 // (2d:653-670, 799-817).  init: the initial shrinkage + get_g (lam1, lam2,
 // lam4 only).
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_wstep(FA F, int init) {
+__global__ __launch_bounds__(kFin) void fin_wstep(FA F, int init) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
   const int sl[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
@@ -1058,7 +1155,7 @@ __global__ __launch_bounds__(kBlock) void fin_wstep(FA F, int init) {
 
 // initial f (end of the first get_g) and the nonmonotone reference values
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_init2(FA F) {
+__global__ __launch_bounds__(kFin) void fin_init2(FA F) {
   FIN_BEGIN
   const int sl[2] = {kSlotM, kSlotM + 1};
   double r[2];
@@ -1124,7 +1221,7 @@ __global__ __launch_bounds__(kBlock) void fin_scaleb(FA F, const MinMax *mmp,
 
 // RelChgOut and the outer stop test (2d:820-828)
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_outer1(FA F) {
+__global__ __launch_bounds__(kFin) void fin_outer1(FA F) {
   FIN_BEGIN
   const int sl[1] = {0};
   double r[1];
@@ -1145,7 +1242,7 @@ __global__ __launch_bounds__(kBlock) void fin_outer1(FA F) {
 
 // update_mlp's scalars and the continuation step (2d:673-694, 831-851)
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_mlp(FA F) {
+__global__ __launch_bounds__(kFin) void fin_mlp(FA F) {
   FIN_BEGIN
   const bool z3 = gd.r > 1;
   const int sl[4] = {0, 1, 2, kSlotM};
@@ -1183,7 +1280,7 @@ __global__ __launch_bounds__(kBlock) void fin_mlp(FA F) {
 
 // end of a step: drops the per-step flags of every instance and publishes
 // the flags
-__global__ __launch_bounds__(kBlock) void fin_refresh(FA F) {
+__global__ __launch_bounds__(kFin) void fin_refresh(FA F) {
   if (threadIdx.x != 0) return;
   InstScal &s = F.S[blockIdx.x];
   s.flags &= ~kStepFlags;
@@ -1191,7 +1288,7 @@ __global__ __launch_bounds__(kBlock) void fin_refresh(FA F) {
 }
 
 template <class TS>
-__global__ __launch_bounds__(kBlock) void fin_relerr(FA F) {
+__global__ __launch_bounds__(kFin) void fin_relerr(FA F) {
   FIN_BEGIN
   const int sl[2] = {0, 1};
   double r[2];
@@ -1225,11 +1322,22 @@ __global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
 // usually does), their vectors are gathered out of the interleaved layout,
 // multiplied by the W = 1 kernels and scattered back.  Every walk sum is an
 // FP64 sum of float values in both layouts, so the results are the same.
-constexpr int kLaneMax = 16;
+// At most kLaneCap lanes; the limit in use is lane_limit() (default 16,
+// TV3_LANE_MAX=k sets another, 0 turns compaction off).
+constexpr int kLaneCap = 64;
 struct LaneList {
   int n;
-  int inst[kLaneMax];
+  int inst[kLaneCap];
 };
+
+int lane_limit() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_LANE_MAX");
+    if (!e || !e[0]) return 16;
+    return std::max(0, std::min(kLaneCap, std::atoi(e)));
+  }();
+  return v;
+}
 
 // xc[k * pitch + e] = x[off + e * 64 + lane] for the k-th listed instance
 __global__ __launch_bounds__(kBlock) void k_lane_gather(const GroupDesc *G, const float *x,
@@ -1318,7 +1426,7 @@ class Engine {
   }
 
   int setup() {
-    // 22 common n-vectors + 5 z-direction ones (3-D) + uup, rcdU, Uout
+    // 22 common n-vectors + 5 z-direction ones (3-D) + a work vector, rcdU, Uout
     const int nn_arrays = bt_.z3 ? 30 : 25;
     const int nm_arrays = 7;
     size_t need = 0;
@@ -1331,11 +1439,12 @@ class Engine {
     need += align_up(sizeof(float *) * bt_.Bpad) * 2;
     if (bt_.kind == kDense && W == 1)
       need += align_up(sizeof(double) * scratch_doubles());
-    compact_ok_ = (W == 64 && bt_.kind == kGeometry);
+    lane_max_ = lane_limit();
+    compact_ok_ = (W == 64 && bt_.kind == kGeometry && lane_max_ > 0);
     pitch_ = ((long long)std::max(bt_.max_n, bt_.max_m) + 63) / 64 * 64;
     if (compact_ok_) {
-      need += align_up(sizeof(GroupDesc) * kLaneMax);
-      need += align_up(sizeof(float) * (size_t)kLaneMax * pitch_) * 2;
+      need += align_up(sizeof(GroupDesc) * lane_max_);
+      need += align_up(sizeof(float) * (size_t)lane_max_ * pitch_) * 2;
     }
     if (ctx_->ws_bytes < need) {
       if (ctx_->ws) {
@@ -1362,7 +1471,7 @@ class Engine {
     } else {
       a_.Uz = a_.Uzp = a_.dUz = a_.Wz = a_.sz = nullptr;
     }
-    a_.uup = (float *)take(fn);
+    a_.tmpn = (float *)take(fn);
     a_.rcdU = (float *)take(fn);
     a_.Uout = (float *)take(fn);
     float **mptrs[] = {&a_.b, &a_.Au, &a_.Aup, &a_.dAu, &a_.delta, &a_.lastAu, &a_.tmpm};
@@ -1375,11 +1484,11 @@ class Engine {
     ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
     scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * scratch_doubles()) : nullptr;
     if (compact_ok_) {
-      Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * kLaneMax);
-      xc_ = (float *)take(sizeof(float) * (size_t)kLaneMax * pitch_);
-      yc_ = (float *)take(sizeof(float) * (size_t)kLaneMax * pitch_);
-      gc_host_.assign(kLaneMax, bt_.groups[0]);
-      for (int k = 0; k < kLaneMax; ++k) {
+      Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * lane_max_);
+      xc_ = (float *)take(sizeof(float) * (size_t)lane_max_ * pitch_);
+      yc_ = (float *)take(sizeof(float) * (size_t)lane_max_ * pitch_);
+      gc_host_.assign(lane_max_, bt_.groups[0]);
+      for (int k = 0; k < lane_max_; ++k) {
         gc_host_[k].off_n = k * pitch_;
         gc_host_[k].off_m = k * pitch_;
       }
@@ -1408,7 +1517,7 @@ class Engine {
     if (rc) return rc;
     rc = hip_check(ctx_, hipMemcpyAsync(G_, bt_.groups.data(), sizeof(GroupDesc) * bt_.G, hipMemcpyHostToDevice, s_), "group copy");
     if (!rc && compact_ok_)
-      rc = hip_check(ctx_, hipMemcpyAsync(Gc_, gc_host_.data(), sizeof(GroupDesc) * kLaneMax, hipMemcpyHostToDevice, s_), "group copy");
+      rc = hip_check(ctx_, hipMemcpyAsync(Gc_, gc_host_.data(), sizeof(GroupDesc) * lane_max_, hipMemcpyHostToDevice, s_), "group copy");
     return rc;
   }
 
@@ -1456,7 +1565,7 @@ class Engine {
       for (int u : {1, 2, 4, 8}) {
         dense_tune_set(DenseTune{best.rows, u, c});
         const float t = time_launches([&] {
-          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.uup, S_, 0, nullptr, scratch_, submat_mb_);
+          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, S_, 0, nullptr, scratch_, submat_mb_);
         });
         if (t < tbest) { tbest = t; best.unroll = u; best.cols = c; }
       }
@@ -1511,6 +1620,10 @@ class Engine {
   int submat_mb_ = 5;  // row blocks of the dense get_g product
   // lane compaction (W = 64, on-the-fly projector)
   bool compact_ok_ = false, compact_ = false;
+  int lane_max_ = 0;
+  // the host copy of the flags is current: read_summary ran after the last
+  // fin_* launch (true at the top of every step but the first)
+  bool flags_fresh_ = false;
   LaneList lanes_{};
   GroupDesc *Gc_ = nullptr;
   float *xc_ = nullptr, *yc_ = nullptr;
@@ -1572,13 +1685,13 @@ class Engine {
   }
 
   // instances whose published flags (last read_summary) contain all bits of
-  // req, into lanes_; false when there are more than kLaneMax
+  // req, into lanes_; false when there are more than lane_max_
   bool collect_lanes(uint32_t req) {
     const volatile uint32_t *h = ctx_->hflags;
     lanes_.n = 0;
     for (int i = 0; i < bt_.Bpad; ++i) {
       if ((h[i] & req) != req) continue;
-      if (lanes_.n == kLaneMax) return false;
+      if (lanes_.n == lane_max_) return false;
       lanes_.inst[lanes_.n++] = i;
     }
     return lanes_.n > 0;
@@ -1589,6 +1702,7 @@ class Engine {
     int rc = hip_check(ctx_, hipStreamSynchronize(s_), "flag sync");
     out = 0;
     if (rc) return rc;
+    flags_fresh_ = true;
     const volatile uint32_t *h = ctx_->hflags;
     for (int i = 0; i < bt_.Bpad; ++i) out |= h[i];
     return TV3_OK;
@@ -1600,13 +1714,38 @@ class Engine {
     hipLaunchKernelGGL(k_mpost<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_, O_.skip,
                        (!force && !O_.ls_exact) ? 1 : 0);
     if (O_.skip) {
-      hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kBlock), 0, s_, fa(req, forbid), force);
+      hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kFin), 0, s_, fa(req, forbid), force);
       hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
       bwd(a_.Au, a_.g, req | F_BWD, a_.Atb, true);
       hipLaunchKernelGGL(k_skipg<W>, gn(), dim3(kBlock), 0, s_, kn(req | F_SKIPG, forbid), a_);
     } else {
       bwd(a_.Au, a_.g, req, a_.Atb, true);
     }
+  }
+
+  // descend + get_g's n-vector part, the instantiation for the batch's
+  // dimensionality and line-search mode
+  void descend(uint32_t req, bool poly) {
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg, O_.dt_exact);
+    };
+    if (bt_.z3)
+      poly ? go(k_descend<W, TS, true, true>) : go(k_descend<W, TS, true, false>);
+    else
+      poly ? go(k_descend<W, TS, false, true>) : go(k_descend<W, TS, false, false>);
+  }
+
+  // update_W pass (INIT: the initial shrinkage, without the gradient)
+  template <bool INIT>
+  void wstep(uint32_t req) {
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.dt_exact);
+    };
+    const bool iso = O_.tv_iso != 0;
+    if (bt_.z3)
+      iso ? go(k_wstep<W, TS, INIT, true, true>) : go(k_wstep<W, TS, INIT, true, false>);
+    else
+      iso ? go(k_wstep<W, TS, INIT, false, true>) : go(k_wstep<W, TS, INIT, false, false>);
   }
 
   // tau of the first inner iteration / fallback: D(d), |d|^2, A d
@@ -1625,19 +1764,24 @@ class Engine {
   // and is captured into a graph.
   int step(uint32_t &sum, bool uniform) {
     int rc = TV3_OK;
+    const bool fresh = flags_fresh_;
+    flags_fresh_ = false;
     // ---- min_u: get_tau (2d:533) ----
-    hipLaunchKernelGGL(k_tau_bb<W>, gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE, F_FST), a_);
-    if (uniform || (sum & F_FST)) tau_first(F_ACTIVE | F_FST, 0);
-    hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE));
+    if (uniform || (sum & F_FST)) {
+      // instances restarting their inner loop: A d of a few lanes
+      compact_ = !uniform && fresh && compact_ok_ && collect_lanes(F_ACTIVE | F_FST);
+      tau_first(F_ACTIVE | F_FST, 0);
+      compact_ = false;
+    }
+    hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE));
     // keep previous values (2d:536-537) by swapping buffers
     std::swap(a_.U, a_.Up); std::swap(a_.Ux, a_.Uxp); std::swap(a_.Uy, a_.Uyp);
     std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
     std::swap(a_.Au, a_.Aup);
     // descend (2d:540)
-    hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_,
-                       O_.nonneg, O_.dt_exact, O_.ls_exact ? 0 : 1);
+    descend(F_ACTIVE, !O_.ls_exact);
     get_g_products(F_ACTIVE, 0, 0);
-    hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
+    hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE), 0);
     if (!uniform && (rc = read_summary(sum))) return rc;
     // Armijo backtracking (2d:548-575)
     if (uniform || (sum & F_BT)) {
@@ -1645,7 +1789,7 @@ class Engine {
         // step size already chosen on the device: rebuild the state once
         hipLaunchKernelGGL((k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
         hipLaunchKernelGGL((k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-        hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 1);
+        hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa(F_BT), 1);
       } else {
         hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
         hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
@@ -1653,7 +1797,7 @@ class Engine {
         for (int round = 0; round < 5 && (uniform || (sum & F_BT)); ++round) {
           hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
           hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-          hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kBlock), 0, s_, fa(F_BT), 0);
+          hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa(F_BT), 0);
           if (!uniform && (rc = read_summary(sum))) return rc;
         }
       }
@@ -1663,26 +1807,25 @@ class Engine {
       // formula, U = Up, descend(force)
       compact_ = !uniform && compact_ok_ && collect_lanes(F_FB);
       tau_first(F_FB, 0);
-      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB));
-      hipLaunchKernelGGL((k_descend<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_FB), a_,
-                         O_.nonneg, O_.dt_exact, 0);
+      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa(F_FB));
+      descend(F_FB, false);
       get_g_products(F_FB, 0, 1);
       compact_ = false;
-      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kBlock), 0, s_, fa(F_FB), 1);
+      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa(F_FB), 1);
     }
     // ---- update_W, reference values, gradient, stop test ----
-    hipLaunchKernelGGL((k_wstep<W, TS, false>), gn(), dim3(kBlock), 0, s_, kn(F_ACTIVE), a_, O_.tv_iso, O_.dt_exact);
-    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_ACTIVE), 0);
+    wstep<false>(F_ACTIVE);
+    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE), 0);
     if (!uniform && (rc = read_summary(sum))) return rc;
     if (uniform || (sum & F_INNER_DONE)) {
       hipLaunchKernelGGL(k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
-      hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kBlock), 0, s_, fa(F_INNER_DONE));
+      hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kFin), 0, s_, fa(F_INNER_DONE));
       hipLaunchKernelGGL(k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
       hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
-      hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kBlock), 0, s_, fa(F_MLP));
+      hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kFin), 0, s_, fa(F_MLP));
       hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
       hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
-      hipLaunchKernelGGL(fin_refresh, gi(), dim3(kBlock), 0, s_, fa(0));
+      hipLaunchKernelGGL(fin_refresh, gi(), dim3(kFin), 0, s_, fa(0));
       if (!uniform && (rc = read_summary(sum))) return rc;
     }
     rc = hip_check(ctx_, hipGetLastError(), "step kernels");
@@ -1726,6 +1869,7 @@ class Engine {
             tv3_info *infos) {
     int rc;
     const auto t0 = std::chrono::steady_clock::now();
+    flags_fresh_ = false;
     // instance scalars and caller pointers
     hipLaunchKernelGGL(k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad, ctx_->dflags);
     std::vector<const float *> hin(bt_.Bpad, nullptr);
@@ -1743,10 +1887,10 @@ class Engine {
     hipLaunchKernelGGL(k_init_U<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_);
     // D(U), shrinkage, get_g(force), get_gradient
     hipLaunchKernelGGL(k_D<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.U, a_.Ux, a_.Uy, a_.Uz);
-    hipLaunchKernelGGL((k_wstep<W, TS, true>), gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, O_.tv_iso, O_.dt_exact);
-    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE), 1);
+    wstep<true>(F_LIVE);
+    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE), 1);
     get_g_products(F_LIVE, 0, 1);
-    hipLaunchKernelGGL(fin_init2<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE));
+    hipLaunchKernelGGL(fin_init2<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
     hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, 0);
     hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
     rc = hip_check(ctx_, hipGetLastError(), "init kernels");
@@ -1775,7 +1919,7 @@ class Engine {
         if (rc) return rc;
         hipLaunchKernelGGL(k_pack<W>, gpn, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.rcdU, 0);
         hipLaunchKernelGGL(k_relerr<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.rcdU, a_.Uout);
-        hipLaunchKernelGGL(fin_relerr<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE));
+        hipLaunchKernelGGL(fin_relerr<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
       }
     }
     std::vector<InstScal> hs(bt_.Bpad);
