@@ -59,6 +59,10 @@ struct PlanDev {
   const RayDesc *rays;
   const int *g_cptr;  // voxel-major transpose: rays through voxel v,
   const int *g_cray;  // ascending, at g_cray[g_cptr[v] .. g_cptr[v+1])
+  // order in which the 64-lane forward walks the rays: rays between
+  // neighbouring emitters and neighbouring receivers come together
+  const int *ray_order;
+  const int *col_order;  // order in which the 64-lane transpose walks voxels
   float scale;
 };
 
@@ -159,7 +163,7 @@ struct tv3_plan {
   long long nnz = 0;
   // owned device arrays (geometry)
   tv3::RayDesc *rays = nullptr;
-  int *cptr = nullptr, *cray = nullptr;
+  int *cptr = nullptr, *cray = nullptr, *order = nullptr, *corder = nullptr;
   std::vector<long long> walk_ptr;  // host: start of each row's walk
 };
 

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "tv3_internal.h"
@@ -278,6 +279,52 @@ bool proj_vec() {
   return v != 0;
 }
 
+// Workgroups go to the eight XCDs round robin by linear id.  With xcd != 0
+// the grid's x extent is a multiple of 8 and XCD k takes the k-th contiguous
+// eighth of the rows (columns), so the rays in flight on one XCD are
+// neighbours in the plan's walk order and share that XCD's L2.
+// TV3_XCD_MAP: bit 0 forward, bit 1 transpose (default 0: measured slower,
+// the XCDs then share fewer lines in the memory-side cache).
+int xcd_map() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_XCD_MAP");
+    return e ? std::atoi(e) & 3 : 0;
+  }();
+  return v;
+}
+
+// Voxel tile of the transpose's column order (TV3_VOX_TILE=t: t x t squares
+// of each slice, 0 = raster order).  Default 32, the best of a sweep on the
+// 256-transducer ring (transpose 0.194 -> 0.186 ms for two 64-lane groups).
+int vox_tile() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_VOX_TILE");
+    return e ? std::max(0, std::atoi(e)) : 32;
+  }();
+  return v;
+}
+
+// Tile of the forward walk order: rays from te consecutive emitters to tr
+// consecutive receivers (TV3_RAY_TILE=te,tr; 0,0 keeps the measurement
+// order).  Default 16 x 32, the best of a sweep on the 256-transducer ring
+// (forward 0.161 -> 0.135 ms for two 64-lane groups): the rays in flight
+// then cross fewer distinct voxels.
+void ray_tile(int &te, int &tr) {
+  static const std::pair<int, int> v = [] {
+    int a = 16, b = 32;
+    if (const char *e = std::getenv("TV3_RAY_TILE")) {
+      if (std::sscanf(e, "%d,%d", &a, &b) != 2) a = b = 16;
+    }
+    return std::make_pair(std::max(0, a), std::max(0, b));
+  }();
+  te = v.first;
+  tr = v.second;
+}
+
+__device__ __forceinline__ int xcd_block(int bx, int nbx, int xcd) {
+  return xcd ? (bx & 7) * (nbx >> 3) + (bx >> 3) : bx;
+}
+
 __device__ __forceinline__ double quarter_sum(double v) {
   v += __shfl_xor(v, 16, 64);
   v += __shfl_xor(v, 32, 64);
@@ -287,12 +334,14 @@ __device__ __forceinline__ double quarter_sum(double v) {
 __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
                                                      const float *x, float *y,
                                                      const InstScal *S,
-                                                     uint32_t mask) {
+                                                     uint32_t mask, int xcd) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, c = lane & 15;
-  const int row = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
-  if (row >= gd.m) return;
+  const int bx = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int slot = uni_i(bx * (kBlock / 64) + (threadIdx.x >> 6));
+  if (slot >= gd.m) return;
+  const int row = gd.plan.ray_order ? uni_i(gd.plan.ray_order[slot]) : slot;
   const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
   bool on[4];
   bool any = false;
@@ -358,12 +407,14 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
                                                      const float *x, float *y,
                                                      const InstScal *S,
                                                      uint32_t mask,
-                                                     const float *sub) {
+                                                     const float *sub, int xcd) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, c = lane & 15;
-  const int col = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
-  if (col >= gd.n) return;
+  const int bx = xcd_block(blockIdx.x, gridDim.x, xcd);
+  const int slot = uni_i(bx * (kBlock / 64) + (threadIdx.x >> 6));
+  if (slot >= gd.n) return;
+  const int col = gd.plan.col_order ? uni_i(gd.plan.col_order[slot]) : slot;
   const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
   bool on[4];
   bool any = false;
@@ -1037,10 +1088,13 @@ void launch_fwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                     const InstScal *S, uint32_t mask) {
   (void)dtype;
   dim3 grid((max_m + 3) / 4, G);
-  if (kind == kGeometry && proj_vec())
-    hipLaunchKernelGGL(k_fwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
-  else
+  if (kind == kGeometry && proj_vec()) {
+    const int xcd = xcd_map() & 1;
+    if (xcd) grid.x = (grid.x + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_fwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, xcd);
+  } else {
     hipLaunchKernelGGL(k_fwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+  }
 }
 
 template <>
@@ -1120,11 +1174,14 @@ void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
   (void)dtype;
   (void)scratch;
   dim3 grid((max_n + 3) / 4, G);
-  if (kind == kGeometry && proj_vec())
-    hipLaunchKernelGGL(k_bwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
-  else
+  if (kind == kGeometry && proj_vec()) {
+    const int xcd = (xcd_map() >> 1) & 1;
+    if (xcd) grid.x = (grid.x + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_bwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub, xcd);
+  } else {
     hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub,
                        kind == kDense ? submat_mb : 0);
+  }
 }
 
 template <>
@@ -1415,6 +1472,40 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
         d.inv2L = d.L > 0 ? 1.0f / (float)(2 * d.L) : 0.0f;
         rays[row] = d;
       }
+  // forward walk order: tiles of te emitters x tr receivers per measurement
+  // plane, each tile in pair order
+  std::vector<int> order;
+  {
+    int te, tr;
+    ray_tile(te, tr);
+    if (te > 0 && tr > 0) {
+      order.reserve(m);
+      for (int i = 0; i < g->num_mps; ++i)
+        for (int e0 = 0; e0 < E; e0 += te)
+          for (int r0 = 0; r0 < R; r0 += tr)
+            for (int e = e0; e < std::min(E, e0 + te); ++e)
+              for (int r = r0; r < std::min(R, r0 + tr); ++r) {
+                const int up = g->use_path[e * R + r];
+                if (up != 0) order.push_back((int)((long long)i * g->num_paths + up - 1));
+              }
+      // rows no pair maps to (L = -1) keep their place at the end
+      std::vector<char> seen(m, 0);
+      for (int r : order) seen[r] = 1;
+      for (long long r = 0; r < m; ++r)
+        if (!seen[r]) order.push_back((int)r);
+    }
+  }
+  // transpose column order: t x t tiles of each slice
+  std::vector<int> corder;
+  if (const int t = vox_tile()) {
+    corder.reserve(nvox);
+    for (int z = 0; z < g->rv_z; ++z)
+      for (int y0 = 0; y0 < g->rv_y; y0 += t)
+        for (int x0 = 0; x0 < g->rv_x; x0 += t)
+          for (int y = y0; y < std::min(g->rv_y, y0 + t); ++y)
+            for (int x = x0; x < std::min(g->rv_x, x0 + t); ++x)
+              corder.push_back(x * sx + y * sy + z * sz);
+  }
   tv3_plan *p = new tv3_plan();
   p->ctx = ctx;
   PlanDev &dv = p->dev;
@@ -1434,6 +1525,14 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
 #define TV3_TRY(x, what) do { rc = hip_check(ctx, (x), what); if (rc) goto fail; } while (0)
   TV3_TRY(hipMalloc(&p->rays, sizeof(RayDesc) * m), "hipMalloc rays");
   TV3_TRY(hipMemcpyAsync(p->rays, rays.data(), sizeof(RayDesc) * m, hipMemcpyHostToDevice, s), "copy rays");
+  if (!order.empty()) {
+    TV3_TRY(hipMalloc(&p->order, sizeof(int) * m), "hipMalloc ray order");
+    TV3_TRY(hipMemcpyAsync(p->order, order.data(), sizeof(int) * m, hipMemcpyHostToDevice, s), "copy ray order");
+  }
+  if (!corder.empty()) {
+    TV3_TRY(hipMalloc(&p->corder, sizeof(int) * nvox), "hipMalloc column order");
+    TV3_TRY(hipMemcpyAsync(p->corder, corder.data(), sizeof(int) * nvox, hipMemcpyHostToDevice, s), "copy column order");
+  }
   TV3_TRY(hipMalloc(&d_ptr, sizeof(long long) * (m + 1)), "hipMalloc walk ptr");
   TV3_TRY(hipMalloc(&d_lin, sizeof(int) * std::max<long long>(nnz, 1)), "hipMalloc walk");
   TV3_TRY(hipMemcpyAsync(d_ptr, p->walk_ptr.data(), sizeof(long long) * (m + 1), hipMemcpyHostToDevice, s), "copy walk ptr");
@@ -1459,13 +1558,15 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
 #undef TV3_TRY
   (void)hipFree(d_ptr);
   (void)hipFree(d_lin);
-  dv.rays = p->rays; dv.g_cptr = p->cptr; dv.g_cray = p->cray;
+  dv.rays = p->rays; dv.g_cptr = p->cptr; dv.g_cray = p->cray; dv.ray_order = p->order; dv.col_order = p->corder;
   *out = p;
   return TV3_OK;
 fail:
   if (d_ptr) (void)hipFree(d_ptr);
   if (d_lin) (void)hipFree(d_lin);
   if (p->rays) (void)hipFree(p->rays);
+  if (p->order) (void)hipFree(p->order);
+  if (p->corder) (void)hipFree(p->corder);
   if (p->cptr) (void)hipFree(p->cptr);
   if (p->cray) (void)hipFree(p->cray);
   delete p;
@@ -1476,6 +1577,8 @@ void tv3_plan_destroy(tv3_plan *p) {
   if (!p) return;
   // hipFree waits for the device; the plan's context may already be gone
   if (p->rays) (void)hipFree(p->rays);
+  if (p->order) (void)hipFree(p->order);
+  if (p->corder) (void)hipFree(p->corder);
   if (p->cptr) (void)hipFree(p->cptr);
   if (p->cray) (void)hipFree(p->cray);
   delete p;
