@@ -62,7 +62,9 @@ struct PlanDev {
   // order in which the 64-lane forward walks the rays: rays between
   // neighbouring emitters and neighbouring receivers come together
   const int *ray_order;
-  const int *col_order;  // order in which the 64-lane transpose walks voxels
+  // the transpose lists again in the 64-lane transpose's walk order: slot s
+  // is voxel t_col[s], its rays t_ray[t_ptr[s] .. t_ptr[s+1])
+  const int *t_ptr, *t_ray, *t_col;
   float scale;
 };
 
@@ -163,7 +165,8 @@ struct tv3_plan {
   long long nnz = 0;
   // owned device arrays (geometry)
   tv3::RayDesc *rays = nullptr;
-  int *cptr = nullptr, *cray = nullptr, *order = nullptr, *corder = nullptr;
+  int *cptr = nullptr, *cray = nullptr, *order = nullptr;
+  int *tptr = nullptr, *tray = nullptr, *tcol = nullptr;
   std::vector<long long> walk_ptr;  // host: start of each row's walk
 };
 

@@ -279,20 +279,6 @@ bool proj_vec() {
   return v != 0;
 }
 
-// Workgroups go to the eight XCDs round robin by linear id.  With xcd != 0
-// the grid's x extent is a multiple of 8 and XCD k takes the k-th contiguous
-// eighth of the rows (columns), so the rays in flight on one XCD are
-// neighbours in the plan's walk order and share that XCD's L2.
-// TV3_XCD_MAP: bit 0 forward, bit 1 transpose (default 0: measured slower,
-// the XCDs then share fewer lines in the memory-side cache).
-int xcd_map() {
-  static const int v = [] {
-    const char *e = std::getenv("TV3_XCD_MAP");
-    return e ? std::atoi(e) & 3 : 0;
-  }();
-  return v;
-}
-
 // Voxel tile of the transpose's column order (TV3_VOX_TILE=t: t x t squares
 // of each slice, 0 = raster order).  Default 32, the best of a sweep on the
 // 256-transducer ring (transpose 0.194 -> 0.186 ms for two 64-lane groups).
@@ -321,10 +307,6 @@ void ray_tile(int &te, int &tr) {
   tr = v.second;
 }
 
-__device__ __forceinline__ int xcd_block(int bx, int nbx, int xcd) {
-  return xcd ? (bx & 7) * (nbx >> 3) + (bx >> 3) : bx;
-}
-
 __device__ __forceinline__ double quarter_sum(double v) {
   v += __shfl_xor(v, 16, 64);
   v += __shfl_xor(v, 32, 64);
@@ -334,12 +316,11 @@ __device__ __forceinline__ double quarter_sum(double v) {
 __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
                                                      const float *x, float *y,
                                                      const InstScal *S,
-                                                     uint32_t mask, int xcd) {
+                                                     uint32_t mask) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, c = lane & 15;
-  const int bx = xcd_block(blockIdx.x, gridDim.x, xcd);
-  const int slot = uni_i(bx * (kBlock / 64) + (threadIdx.x >> 6));
+  const int slot = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
   if (slot >= gd.m) return;
   const int row = gd.plan.ray_order ? uni_i(gd.plan.ray_order[slot]) : slot;
   const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
@@ -400,21 +381,26 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
   }
 }
 
-// A^T of the on-the-fly projector with 16-byte gathers: one wavefront per
-// voxel over its ascending ray list, quarters take list entries 4t + q of
-// each 64-entry chunk, lanes own four instances (as k_fwd_w64q).
+// A^T of the on-the-fly projector with 16-byte gathers.  A wavefront walks
+// kBwdCols consecutive voxels of the plan's transpose order (t_col; the ray
+// lists t_ray[t_ptr[s] .. t_ptr[s+1]) ascending); quarters take list entries
+// 4t + q of each 64-entry chunk, lanes own four instances (as k_fwd_w64q).
+// The list bounds of the voxel after next and the first index chunk of the
+// next voxel are loaded while the current voxel's gathers are in flight
+// (two voxels per wavefront measured best: 0.186 -> 0.173 ms for two 64-lane
+// groups of the 256 ring; four 0.186, eight 0.237).
+constexpr int kBwdCols = 2;
 __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
                                                      const float *x, float *y,
                                                      const InstScal *S,
                                                      uint32_t mask,
-                                                     const float *sub, int xcd) {
+                                                     const float *sub) {
   const GroupDesc &gd = G[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, c = lane & 15;
-  const int bx = xcd_block(blockIdx.x, gridDim.x, xcd);
-  const int slot = uni_i(bx * (kBlock / 64) + (threadIdx.x >> 6));
-  if (slot >= gd.n) return;
-  const int col = gd.plan.col_order ? uni_i(gd.plan.col_order[slot]) : slot;
+  const int s0 = uni_i((blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * kBwdCols);
+  if (s0 >= gd.n) return;
+  const int ns = min(kBwdCols, gd.n - s0);
   const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
   bool on[4];
   bool any = false;
@@ -428,46 +414,66 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
   const float sc = P.scale;
   const bool unit = sc == 1.0f;
   const float *xb = x + gd.off_m + 4 * c;
-  const int k0 = uni_i(P.g_cptr[col]), k1 = uni_i(P.g_cptr[col + 1]);
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int cnt = min(64, k1 - kb);
-    const int id = lane < cnt ? P.g_cray[kb + lane] : 0;
-    const int tmax = (cnt + 3) >> 2;
-    for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
-      float4 v[kPipeQ];
+  // voxel A: current, B: next (bounds known), C: after next
+  int kA0 = uni_i(P.t_ptr[s0]), kA1 = uni_i(P.t_ptr[s0 + 1]);
+  int colA = uni_i(P.t_col[s0]);
+  int idA = lane < min(64, kA1 - kA0) ? P.t_ray[kA0 + lane] : 0;
+  int kB0 = 0, kB1 = 0, colB = 0;
+  if (ns > 1) {
+    kB0 = uni_i(P.t_ptr[s0 + 1]);
+    kB1 = uni_i(P.t_ptr[s0 + 2]);
+    colB = uni_i(P.t_col[s0 + 1]);
+  }
+  for (int i = 0; i < ns; ++i) {
+    const int idB = (i + 1 < ns && lane < min(64, kB1 - kB0)) ? P.t_ray[kB0 + lane] : 0;
+    int kC0 = 0, kC1 = 0, colC = 0;
+    if (i + 2 < ns) {
+      kC0 = uni_i(P.t_ptr[s0 + i + 2]);
+      kC1 = uni_i(P.t_ptr[s0 + i + 3]);
+      colC = uni_i(P.t_col[s0 + i + 2]);
+    }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int kb = kA0; kb < kA1; kb += 64) {
+      const int cnt = min(64, kA1 - kb);
+      const int id = kb == kA0 ? idA : (lane < cnt ? P.t_ray[kb + lane] : 0);
+      const int tmax = (cnt + 3) >> 2;
+      for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
+        float4 v[kPipeQ];
 #pragma unroll
-      for (int u = 0; u < kPipeQ; ++u) {
-        const int st = 4 * (t0 + u) + q;
-        const int iu = __shfl(id, st & 63, 64);
-        v[u] = st < cnt ? *(const float4 *)(xb + (long long)iu * 64) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+        for (int u = 0; u < kPipeQ; ++u) {
+          const int st = 4 * (t0 + u) + q;
+          const int iu = __shfl(id, st & 63, 64);
+          v[u] = st < cnt ? *(const float4 *)(xb + (long long)iu * 64) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
 #pragma unroll
-      for (int u = 0; u < kPipeQ; ++u) {
-        if (4 * (t0 + u) + q < cnt) {
-          if (unit) {
-            a0 += (double)v[u].x; a1 += (double)v[u].y;
-            a2 += (double)v[u].z; a3 += (double)v[u].w;
-          } else {
-            a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
-            a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
+        for (int u = 0; u < kPipeQ; ++u) {
+          if (4 * (t0 + u) + q < cnt) {
+            if (unit) {
+              a0 += (double)v[u].x; a1 += (double)v[u].y;
+              a2 += (double)v[u].z; a3 += (double)v[u].w;
+            } else {
+              a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
+              a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
+            }
           }
         }
       }
     }
-  }
-  a0 = quarter_sum(a0); a1 = quarter_sum(a1);
-  a2 = quarter_sum(a2); a3 = quarter_sum(a3);
-  if (q == 0) {
-    const long long o = gd.off_n + (long long)col * 64 + 4 * c;
-    float r[4] = {(float)a0, (float)a1, (float)a2, (float)a3};
-    if (sub) {
+    a0 = quarter_sum(a0); a1 = quarter_sum(a1);
+    a2 = quarter_sum(a2); a3 = quarter_sum(a3);
+    if (q == 0) {
+      const long long o = gd.off_n + (long long)colA * 64 + 4 * c;
+      float r[4] = {(float)a0, (float)a1, (float)a2, (float)a3};
+      if (sub) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) r[k] = r[k] - sub[o + k];
+        for (int k = 0; k < 4; ++k) r[k] = r[k] - sub[o + k];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (on[k]) y[o + k] = r[k];
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (on[k]) y[o + k] = r[k];
+    kA0 = kB0; kA1 = kB1; colA = colB; idA = idB;
+    kB0 = kC0; kB1 = kC1; colB = colC;
   }
 }
 
@@ -1089,9 +1095,7 @@ void launch_fwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
   (void)dtype;
   dim3 grid((max_m + 3) / 4, G);
   if (kind == kGeometry && proj_vec()) {
-    const int xcd = xcd_map() & 1;
-    if (xcd) grid.x = (grid.x + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_fwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, xcd);
+    hipLaunchKernelGGL(k_fwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
   } else {
     hipLaunchKernelGGL(k_fwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
   }
@@ -1173,12 +1177,12 @@ void launch_bwd<64>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                     float *scratch, int submat_mb) {
   (void)dtype;
   (void)scratch;
-  dim3 grid((max_n + 3) / 4, G);
   if (kind == kGeometry && proj_vec()) {
-    const int xcd = (xcd_map() >> 1) & 1;
-    if (xcd) grid.x = (grid.x + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_bwd_w64q, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub, xcd);
+    const int per = (kBlock / 64) * kBwdCols;
+    hipLaunchKernelGGL(k_bwd_w64q, dim3((max_n + per - 1) / per, G), dim3(kBlock), 0, s, G_dev,
+                       x, y, S, mask, sub);
   } else {
+    dim3 grid((max_n + 3) / 4, G);
     hipLaunchKernelGGL(k_bwd_w64, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub,
                        kind == kDense ? submat_mb : 0);
   }
@@ -1495,10 +1499,11 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
         if (!seen[r]) order.push_back((int)r);
     }
   }
-  // transpose column order: t x t tiles of each slice
+  // transpose walk order: t x t tiles of each slice (raster order for t = 0)
   std::vector<int> corder;
-  if (const int t = vox_tile()) {
-    corder.reserve(nvox);
+  corder.reserve(nvox);
+  {
+    const int t = vox_tile() > 0 ? vox_tile() : std::max(g->rv_x, g->rv_y);
     for (int z = 0; z < g->rv_z; ++z)
       for (int y0 = 0; y0 < g->rv_y; y0 += t)
         for (int x0 = 0; x0 < g->rv_x; x0 += t)
@@ -1529,10 +1534,7 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
     TV3_TRY(hipMalloc(&p->order, sizeof(int) * m), "hipMalloc ray order");
     TV3_TRY(hipMemcpyAsync(p->order, order.data(), sizeof(int) * m, hipMemcpyHostToDevice, s), "copy ray order");
   }
-  if (!corder.empty()) {
-    TV3_TRY(hipMalloc(&p->corder, sizeof(int) * nvox), "hipMalloc column order");
-    TV3_TRY(hipMemcpyAsync(p->corder, corder.data(), sizeof(int) * nvox, hipMemcpyHostToDevice, s), "copy column order");
-  }
+
   TV3_TRY(hipMalloc(&d_ptr, sizeof(long long) * (m + 1)), "hipMalloc walk ptr");
   TV3_TRY(hipMalloc(&d_lin, sizeof(int) * std::max<long long>(nnz, 1)), "hipMalloc walk");
   TV3_TRY(hipMemcpyAsync(d_ptr, p->walk_ptr.data(), sizeof(long long) * (m + 1), hipMemcpyHostToDevice, s), "copy walk ptr");
@@ -1553,12 +1555,26 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
     TV3_TRY(hipMalloc(&p->cray, sizeof(int) * cray.size()), "hipMalloc cray");
     TV3_TRY(hipMemcpyAsync(p->cptr, cptr.data(), sizeof(int) * (nvox + 1), hipMemcpyHostToDevice, s), "copy cptr");
     TV3_TRY(hipMemcpyAsync(p->cray, cray.data(), sizeof(int) * cray.size(), hipMemcpyHostToDevice, s), "copy cray");
+    // the same lists in the walk order of the 64-lane transpose
+    std::vector<int> tptr(nvox + 1, 0), tray(cray.size());
+    for (long long k = 0; k < nvox; ++k) {
+      const int v = corder[k];
+      tptr[k + 1] = tptr[k] + (cptr[v + 1] - cptr[v]);
+      std::copy(cray.begin() + cptr[v], cray.begin() + cptr[v + 1], tray.begin() + tptr[k]);
+    }
+    TV3_TRY(hipMalloc(&p->tptr, sizeof(int) * (nvox + 1)), "hipMalloc tptr");
+    TV3_TRY(hipMalloc(&p->tray, sizeof(int) * tray.size()), "hipMalloc tray");
+    TV3_TRY(hipMalloc(&p->tcol, sizeof(int) * nvox), "hipMalloc tcol");
+    TV3_TRY(hipMemcpyAsync(p->tptr, tptr.data(), sizeof(int) * (nvox + 1), hipMemcpyHostToDevice, s), "copy tptr");
+    TV3_TRY(hipMemcpyAsync(p->tray, tray.data(), sizeof(int) * tray.size(), hipMemcpyHostToDevice, s), "copy tray");
+    TV3_TRY(hipMemcpyAsync(p->tcol, corder.data(), sizeof(int) * nvox, hipMemcpyHostToDevice, s), "copy tcol");
     TV3_TRY(hipStreamSynchronize(s), "plan sync");
   }
 #undef TV3_TRY
   (void)hipFree(d_ptr);
   (void)hipFree(d_lin);
-  dv.rays = p->rays; dv.g_cptr = p->cptr; dv.g_cray = p->cray; dv.ray_order = p->order; dv.col_order = p->corder;
+  dv.rays = p->rays; dv.g_cptr = p->cptr; dv.g_cray = p->cray; dv.ray_order = p->order;
+  dv.t_ptr = p->tptr; dv.t_ray = p->tray; dv.t_col = p->tcol;
   *out = p;
   return TV3_OK;
 fail:
@@ -1566,7 +1582,9 @@ fail:
   if (d_lin) (void)hipFree(d_lin);
   if (p->rays) (void)hipFree(p->rays);
   if (p->order) (void)hipFree(p->order);
-  if (p->corder) (void)hipFree(p->corder);
+  if (p->tptr) (void)hipFree(p->tptr);
+  if (p->tray) (void)hipFree(p->tray);
+  if (p->tcol) (void)hipFree(p->tcol);
   if (p->cptr) (void)hipFree(p->cptr);
   if (p->cray) (void)hipFree(p->cray);
   delete p;
@@ -1578,7 +1596,9 @@ void tv3_plan_destroy(tv3_plan *p) {
   // hipFree waits for the device; the plan's context may already be gone
   if (p->rays) (void)hipFree(p->rays);
   if (p->order) (void)hipFree(p->order);
-  if (p->corder) (void)hipFree(p->corder);
+  if (p->tptr) (void)hipFree(p->tptr);
+  if (p->tray) (void)hipFree(p->tray);
+  if (p->tcol) (void)hipFree(p->tcol);
   if (p->cptr) (void)hipFree(p->cptr);
   if (p->cray) (void)hipFree(p->cray);
   delete p;

@@ -1643,6 +1643,13 @@ class Engine {
     f.trace = trace_; f.trace_cap = trace_cap_; f.O = O_;
     return f;
   }
+  // fin_* kernels after which the host reads no flags need not publish them
+  // to host memory
+  FA fa_quiet(uint32_t req, uint32_t forbid = 0) const {
+    FA f = fa(req, forbid);
+    f.fout = nullptr;
+    return f;
+  }
   dim3 gi() const { return dim3(bt_.Bpad); }
 
   void fwd(const float *x, float *y, uint32_t mask) {
@@ -1714,7 +1721,7 @@ class Engine {
     hipLaunchKernelGGL(k_mpost<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_, O_.skip,
                        (!force && !O_.ls_exact) ? 1 : 0);
     if (O_.skip) {
-      hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kFin), 0, s_, fa(req, forbid), force);
+      hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(req, forbid), force);
       hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
       bwd(a_.Au, a_.g, req | F_BWD, a_.Atb, true);
       hipLaunchKernelGGL(k_skipg<W>, gn(), dim3(kBlock), 0, s_, kn(req | F_SKIPG, forbid), a_);
@@ -1773,7 +1780,7 @@ class Engine {
       tau_first(F_ACTIVE | F_FST, 0);
       compact_ = false;
     }
-    hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE));
+    hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_ACTIVE));
     // keep previous values (2d:536-537) by swapping buffers
     std::swap(a_.U, a_.Up); std::swap(a_.Ux, a_.Uxp); std::swap(a_.Uy, a_.Uyp);
     std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
@@ -1789,7 +1796,7 @@ class Engine {
         // step size already chosen on the device: rebuild the state once
         hipLaunchKernelGGL((k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
         hipLaunchKernelGGL((k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-        hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa(F_BT), 1);
+        hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_BT), 1);
       } else {
         hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
         hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
@@ -1807,11 +1814,11 @@ class Engine {
       // formula, U = Up, descend(force)
       compact_ = !uniform && compact_ok_ && collect_lanes(F_FB);
       tau_first(F_FB, 0);
-      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa(F_FB));
+      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB));
       descend(F_FB, false);
       get_g_products(F_FB, 0, 1);
       compact_ = false;
-      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa(F_FB), 1);
+      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB), 1);
     }
     // ---- update_W, reference values, gradient, stop test ----
     wstep<false>(F_ACTIVE);
@@ -1819,10 +1826,10 @@ class Engine {
     if (!uniform && (rc = read_summary(sum))) return rc;
     if (uniform || (sum & F_INNER_DONE)) {
       hipLaunchKernelGGL(k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
-      hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kFin), 0, s_, fa(F_INNER_DONE));
+      hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_INNER_DONE));
       hipLaunchKernelGGL(k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
       hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
-      hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kFin), 0, s_, fa(F_MLP));
+      hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_MLP));
       hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
       hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
       hipLaunchKernelGGL(fin_refresh, gi(), dim3(kFin), 0, s_, fa(0));
