@@ -1322,21 +1322,28 @@ __global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
 // usually does), their vectors are gathered out of the interleaved layout,
 // multiplied by the W = 1 kernels and scattered back.  Every walk sum is an
 // FP64 sum of float values in both layouts, so the results are the same.
-// At most kLaneCap lanes; the limit in use is lane_limit() (default 16,
-// TV3_LANE_MAX=k sets another, 0 turns compaction off).
-constexpr int kLaneCap = 64;
+// Up to lane_limit() lanes (default 16, TV3_LANE_MAX=k sets another up to
+// 64, 0 turns it off) go to the W = 1 kernels.  More lanes, when they fit in
+// fewer 64-lane groups than the batch has, are packed into compact 64-lane
+// groups (listed instance k in group k / 64, lane k % 64) and multiplied by
+// the 64-lane kernels (TV3_COMPACT64=0 turns this off).  Lists hold at most
+// kLaneCap instances.
+constexpr int kLaneCap = 256;
 struct LaneList {
   int n;
   int inst[kLaneCap];
 };
 
+// (read at every batch set-up)
 int lane_limit() {
-  static const int v = [] {
-    const char *e = std::getenv("TV3_LANE_MAX");
-    if (!e || !e[0]) return 16;
-    return std::max(0, std::min(kLaneCap, std::atoi(e)));
-  }();
-  return v;
+  const char *e = std::getenv("TV3_LANE_MAX");
+  if (!e || !e[0]) return 16;
+  return std::max(0, std::min(64, std::atoi(e)));
+}
+
+bool compact64_on() {
+  const char *e = std::getenv("TV3_COMPACT64");
+  return !(e && e[0] == '0');
 }
 
 // xc[k * pitch + e] = x[off + e * 64 + lane] for the k-th listed instance
@@ -1369,6 +1376,46 @@ __global__ __launch_bounds__(kBlock) void k_lane_scatter(const GroupDesc *G, con
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < len; e += gridDim.x * kBlock) {
     const long long o = off + (long long)e * 64 + b;
     const float v = yc[k * pitch + e];
+    y[o] = sub ? v - sub[o] : v;
+  }
+}
+
+// compact 64-lane layout: xc[g * pitch + v * 64 + l] = x of listed instance
+// 64 g + l at voxel (row) v; unlisted lanes of the last group get 0
+__global__ __launch_bounds__(kBlock) void k_lane_gather64(const GroupDesc *G, const float *x,
+                                                          float *xc, LaneList L, int len,
+                                                          int use_m, long long pitch) {
+  const int g = blockIdx.y;
+  const long long E = (long long)len * 64;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
+       e += (long long)gridDim.x * kBlock) {
+    const int k = g * 64 + (int)(e & 63);
+    float v = 0.0f;
+    if (k < L.n) {
+      const int inst = L.inst[k];
+      const GroupDesc &gd = G[inst / 64];
+      v = x[(use_m ? gd.off_m : gd.off_n) + (e & ~63LL) + inst % 64];
+    }
+    xc[g * pitch + e] = v;
+  }
+}
+
+// y of listed instances whose flags pass the mask = compact result (- sub)
+__global__ __launch_bounds__(kBlock) void k_lane_scatter64(const GroupDesc *G, const InstScal *S,
+                                                           const float *yc, float *y, LaneList L,
+                                                           int len, int use_m, long long pitch,
+                                                           uint32_t mask, const float *sub) {
+  const int g = blockIdx.y;
+  const long long E = (long long)len * 64;
+  for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
+       e += (long long)gridDim.x * kBlock) {
+    const int k = g * 64 + (int)(e & 63);
+    if (k >= L.n) continue;
+    const int inst = L.inst[k];
+    if (mask && (S[inst].flags & mask) != mask) continue;
+    const GroupDesc &gd = G[inst / 64];
+    const long long o = (use_m ? gd.off_m : gd.off_n) + (e & ~63LL) + inst % 64;
+    const float v = yc[g * pitch + e];
     y[o] = sub ? v - sub[o] : v;
   }
 }
@@ -1440,11 +1487,15 @@ class Engine {
     if (bt_.kind == kDense && W == 1)
       need += align_up(sizeof(double) * scratch_doubles());
     lane_max_ = lane_limit();
-    compact_ok_ = (W == 64 && bt_.kind == kGeometry && lane_max_ > 0);
+    compact_ok_ = (W == 64 && bt_.kind == kGeometry);
     pitch_ = ((long long)std::max(bt_.max_n, bt_.max_m) + 63) / 64 * 64;
+    gc64_max_ = (compact_ok_ && compact64_on()) ? std::min(bt_.G - 1, kLaneCap / 64) : 0;
+    pitch64_ = (long long)std::max(bt_.max_n, bt_.max_m) * 64;
     if (compact_ok_) {
-      need += align_up(sizeof(GroupDesc) * lane_max_);
+      need += align_up(sizeof(GroupDesc) * std::max(lane_max_, 1));
       need += align_up(sizeof(float) * (size_t)lane_max_ * pitch_) * 2;
+      need += align_up(sizeof(GroupDesc) * std::max(gc64_max_, 1));
+      need += align_up(sizeof(float) * (size_t)gc64_max_ * pitch64_) * 2;
     }
     if (ctx_->ws_bytes < need) {
       if (ctx_->ws) {
@@ -1484,13 +1535,21 @@ class Engine {
     ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
     scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * scratch_doubles()) : nullptr;
     if (compact_ok_) {
-      Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * lane_max_);
+      Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * std::max(lane_max_, 1));
       xc_ = (float *)take(sizeof(float) * (size_t)lane_max_ * pitch_);
       yc_ = (float *)take(sizeof(float) * (size_t)lane_max_ * pitch_);
       gc_host_.assign(lane_max_, bt_.groups[0]);
       for (int k = 0; k < lane_max_; ++k) {
         gc_host_[k].off_n = k * pitch_;
         gc_host_[k].off_m = k * pitch_;
+      }
+      G64c_ = (GroupDesc *)take(sizeof(GroupDesc) * std::max(gc64_max_, 1));
+      x64c_ = (float *)take(sizeof(float) * (size_t)gc64_max_ * pitch64_);
+      y64c_ = (float *)take(sizeof(float) * (size_t)gc64_max_ * pitch64_);
+      g64_host_.assign(gc64_max_, bt_.groups[0]);
+      for (int k = 0; k < gc64_max_; ++k) {
+        g64_host_[k].off_n = k * pitch64_;
+        g64_host_[k].off_m = k * pitch64_;
       }
     }
     if (ctx_->hflags_n < (size_t)bt_.Bpad) {
@@ -1516,8 +1575,10 @@ class Engine {
     int rc = hip_check(ctx_, hipMemsetAsync(ctx_->ws, 0, (size_t)(p - (char *)ctx_->ws), s_), "workspace clear");
     if (rc) return rc;
     rc = hip_check(ctx_, hipMemcpyAsync(G_, bt_.groups.data(), sizeof(GroupDesc) * bt_.G, hipMemcpyHostToDevice, s_), "group copy");
-    if (!rc && compact_ok_)
+    if (!rc && compact_ok_ && lane_max_ > 0)
       rc = hip_check(ctx_, hipMemcpyAsync(Gc_, gc_host_.data(), sizeof(GroupDesc) * lane_max_, hipMemcpyHostToDevice, s_), "group copy");
+    if (!rc && gc64_max_ > 0)
+      rc = hip_check(ctx_, hipMemcpyAsync(G64c_, g64_host_.data(), sizeof(GroupDesc) * gc64_max_, hipMemcpyHostToDevice, s_), "group copy");
     return rc;
   }
 
@@ -1619,7 +1680,13 @@ class Engine {
   bool graph_ = false, capturing_ = false;
   int submat_mb_ = 5;  // row blocks of the dense get_g product
   // lane compaction (W = 64, on-the-fly projector)
-  bool compact_ok_ = false, compact_ = false;
+  bool compact_ok_ = false;
+  int cmode_ = 0;  // lane compaction of the next products: 0 off, 1 W = 1, 2 compact W = 64
+  int ncg_ = 0, gc64_max_ = 0;  // compact 64-lane groups in use / available
+  long long pitch64_ = 0;
+  GroupDesc *G64c_ = nullptr;
+  float *x64c_ = nullptr, *y64c_ = nullptr;
+  std::vector<GroupDesc> g64_host_;
   int lane_max_ = 0;
   // the host copy of the flags is current: read_summary ran after the last
   // fin_* launch (true at the top of every step but the first)
@@ -1653,7 +1720,18 @@ class Engine {
   dim3 gi() const { return dim3(bt_.Bpad); }
 
   void fwd(const float *x, float *y, uint32_t mask) {
-    if (compact_) {
+    if (cmode_ == 2) {
+      const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
+      const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
+      hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_n, 0, pitch64_);
+      prof_begin(ctx_, 0);
+      launch_fwd<64>(s_, G64c_, ncg_, bt_.max_m, bt_.kind, bt_.dtype, x64c_, y64c_, S_, 0);
+      prof_end(ctx_);
+      prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 0);
+      hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const float *)nullptr);
+      return;
+    }
+    if (cmode_ == 1) {
       const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
       const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
       hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
@@ -1673,7 +1751,18 @@ class Engine {
   // blocked: the get_g product (dense A: row blocks of submat_mb)
   void bwd(const float *x, float *y, uint32_t mask, const float *sub, bool blocked = false) {
     const int smb = blocked ? submat_mb_ : 0;
-    if (compact_) {
+    if (cmode_ == 2) {
+      const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
+      const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
+      hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_m, 1, pitch64_);
+      prof_begin(ctx_, 1);
+      launch_bwd<64>(s_, G64c_, ncg_, bt_.max_n, bt_.kind, bt_.dtype, x64c_, y64c_, S_, 0, nullptr, nullptr, 0);
+      prof_end(ctx_);
+      prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 1);
+      hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
+      return;
+    }
+    if (cmode_ == 1) {
       const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
       const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
       hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
@@ -1692,16 +1781,21 @@ class Engine {
   }
 
   // instances whose published flags (last read_summary) contain all bits of
-  // req, into lanes_; false when there are more than lane_max_
-  bool collect_lanes(uint32_t req) {
+  // req, into lanes_, and the compaction mode for them: 1 (W = 1 kernels) up
+  // to lane_max_ lanes, 2 (compact 64-lane groups) when they fill fewer
+  // groups than the batch, else 0
+  int pick_compact(uint32_t req) {
     const volatile uint32_t *h = ctx_->hflags;
     lanes_.n = 0;
     for (int i = 0; i < bt_.Bpad; ++i) {
       if ((h[i] & req) != req) continue;
-      if (lanes_.n == lane_max_) return false;
+      if (lanes_.n == kLaneCap) return 0;
       lanes_.inst[lanes_.n++] = i;
     }
-    return lanes_.n > 0;
+    if (lanes_.n == 0) return 0;
+    if (lanes_.n <= lane_max_) return 1;
+    ncg_ = (lanes_.n + 63) / 64;
+    return ncg_ <= gc64_max_ ? 2 : 0;
   }
 
   // OR of the instance flags the fin kernels published to mapped host memory
@@ -1776,9 +1870,9 @@ class Engine {
     // ---- min_u: get_tau (2d:533) ----
     if (uniform || (sum & F_FST)) {
       // instances restarting their inner loop: A d of a few lanes
-      compact_ = !uniform && fresh && compact_ok_ && collect_lanes(F_ACTIVE | F_FST);
+      cmode_ = (!uniform && fresh && compact_ok_) ? pick_compact(F_ACTIVE | F_FST) : 0;
       tau_first(F_ACTIVE | F_FST, 0);
-      compact_ = false;
+      cmode_ = 0;
     }
     hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_ACTIVE));
     // keep previous values (2d:536-537) by swapping buffers
@@ -1812,12 +1906,12 @@ class Engine {
     if (uniform || (sum & F_FB)) {
       // 5-strike fallback (2d:561-567): tau from the first-iteration
       // formula, U = Up, descend(force)
-      compact_ = !uniform && compact_ok_ && collect_lanes(F_FB);
+      cmode_ = (!uniform && compact_ok_) ? pick_compact(F_FB) : 0;
       tau_first(F_FB, 0);
       hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB));
       descend(F_FB, false);
       get_g_products(F_FB, 0, 1);
-      compact_ = false;
+      cmode_ = 0;
       hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB), 1);
     }
     // ---- update_W, reference values, gradient, stop test ----
