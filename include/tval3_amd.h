@@ -226,7 +226,12 @@ int tv3_nrm2(tv3_ctx *ctx, long long n, const float *x, double *out);
  * tval3_cpu_2d semantics).  b: m floats, U: dim_x*dim_y*dim_z floats (output;
  * its input values are ignored, as in the reference), Ut: NULL or the ground
  * truth for info->rel_error.  trace: NULL or trace_cap host records.
- * Synchronous. */
+ * Synchronous.
+ * 2-D multi-layer mode (tval3_cpu_2d with b.len = layers * A.dim_y,
+ * 2d/tval3_cpu_2d.cpp:889-1016, 2d/mat_mult.h:15-91): with dim_z == 1 and a
+ * dense or sparse A, dim_x*dim_y may be `layers` times A's column count; U
+ * then holds the layers' images one after the other, b holds layers * m
+ * floats, and every product applies A to each layer. */
 int tv3_solve(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
               int dim_x, int dim_y, int dim_z, const float *b, float *U,
               const float *Ut, tv3_info *info, tv3_trace_rec *trace,

@@ -10,7 +10,10 @@
 // Supported element types: float vectors with float or double scalars
 // (T_comp = float, T_scal = float | double).  The double/double and complex
 // overloads exist so that callers link, and throw tval3_exception.
-// Multi-layer calls (b.len = layers * A.dim_y with layers > 1) throw as well.
+// Multi-layer calls (b.len = layers * A.dim_y, U.len = layers * A.dim_x:
+// U's pixels hold the layers one after the other, every product applies A
+// to each layer, get_g takes two plain products) run for dense and sparse A,
+// as in the reference.
 //
 // The reference's 2-D geometry projector is compiled out
 // (2d/mat_mult.h:111-193: its gemv body is a comment, so A x = 0).  This
@@ -65,19 +68,12 @@ inline void check_2d(const mat<Tc> &U, const geometry &A, const mat<Tc> &b,
     throw tval3_exception(std::string("Argument error: b.len must be equal to A.num_emitters * A.num_receivers!"));
 }
 
-inline void no_layers(int layers) {
-  if (layers != 1)
-    throw tval3_exception(std::string(
-        "Argument error: multilayer reconstruction (layers > 1) is not supported by the GPU engine!"));
-}
-
 template <class Ts>
 inline tval3_info<Ts> run_2d(mat<float> &U, const mat<float> &A, const mat<float> &b,
                              const tval3_options<Ts> &opts, const mat<float> &Ut,
                              int submat_mb) {
   const int layers = A.dim_y ? (int)(b.len / A.dim_y) : 0;
   check_2d(U, A, b, Ut, layers);
-  no_layers(layers);
   return solve_host<tval3_exception, tval3_info<Ts>>(
       U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &keep) {
@@ -91,7 +87,6 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const sparse_mat<float> &A, const ma
                              int submat_mb) {
   const int layers = A.dim_y ? (int)(b.len / A.dim_y) : 0;
   check_2d(U, A, b, Ut, layers);
-  no_layers(layers);
   return solve_host<tval3_exception, tval3_info<Ts>>(
       U, b, opts, Ut, (int)U.dim_x, (int)U.dim_y, 1, sizeof(Ts) == 8, submat_mb,
       [&](tv3_ctx *c, PlanHolder &ph, KeepAll<tval3_exception> &keep) {

@@ -189,11 +189,20 @@ def dot(x, y):
     return lib().tv3o_dot(x.size, _f(x), _f(y))
 
 
+def _check_layers(op, n, blen):
+    """U holds n / op.n layers (2-D dense or sparse multi-layer mode), and b
+    as many measurement vectors."""
+    if op.n <= 0 or n % op.n or blen != op.m * (n // op.n):
+        raise ValueError(f"oracle solve: U has {n} pixels and b {blen} entries "
+                         f"for an operator of {op.m} x {op.n}")
+
+
 def solve(op, b, dims, opts, Ut=None, trace_cap=0):
     """Run the restated reference solver.  Returns (U, info, trace list)."""
     from tval3_amd.abi import Info, TraceRec
     q, p, r = dims
     b = _c32(b)
+    _check_layers(op, q * p * r, b.size)
     U = np.zeros(q * p * r, np.float32)
     info = Info()
     trace = (TraceRec * trace_cap)() if trace_cap else None
@@ -213,6 +222,7 @@ def solve_f64(op, b, dims, opts, Ut=None, trace_cap=0):
     q, p, r = dims
     dp = C.POINTER(C.c_double)
     b = np.ascontiguousarray(b, np.float64)
+    _check_layers(op, q * p * r, b.size)
     U = np.zeros(q * p * r, np.float64)
     info = Info()
     trace = (TraceRec * trace_cap)() if trace_cap else None

@@ -470,6 +470,11 @@ struct Solver {
   Dims dm;
   i64 n, m;
   bool z3;
+  // 2-D multi-layer mode (2d/tval3_cpu_2d.cpp:889-1016 pass layers =
+  // b.len / A.dim_y): U holds `layers` images of A.n pixels one after the
+  // other and b `layers` measurement vectors of A.m; every product applies A
+  // to each layer (2d/mat_mult.h:22-35, 68-90)
+  int layers;
   // options in T_scal
   TS o_c, o_gamma, o_rate_gam, o_rate_cnt, o_gam, o_tol, o_tol_inn;
   // penalty parameters, multipliers, Lagrangian terms (tval3_data)
@@ -488,7 +493,8 @@ struct Solver {
   TS last_tau, last_alpha;
 
   Solver(const Op &A_, const tv3_options &o_, Dims d_)
-      : A(A_), o(o_), dm(d_), n(d_.n()), m(A_.m), z3(d_.r > 1) {
+      : A(A_), o(o_), dm(d_), n(d_.n()), m(A_.m * (d_.n() / A_.n)), z3(d_.r > 1),
+        layers((int)(d_.n() / A_.n)) {
     o_c = (TS)o.c; o_gamma = (TS)o.gamma; o_rate_gam = (TS)o.rate_gam;
     o_rate_cnt = (TS)o.rate_cnt; o_gam = (TS)o.gam; o_tol = (TS)o.tol;
     o_tol_inn = (TS)o.tol_inn;
@@ -561,16 +567,30 @@ struct Solver {
                    (TS)lam5);
   }
 
+  /* gemv with layers: A applied to each layer (the reference's gemm /
+   * cscmm computes the same products; each output is an FP64 sum rounded
+   * once, as for one layer) */
+  void gemvL(bool trans, const TC *x, TC *y) {
+    const i64 in = trans ? A.m : A.n, out = trans ? A.n : A.m;
+    for (int l = 0; l < layers; ++l) gemv(A, trans, x + l * in, y + l * out);
+  }
+
   /* get_g_mult: 2d/tval3_cpu_2d.cpp:298-357 (skipping state machine) and
-   * 359-394 (dense: always both products) */
+   * 359-394 (dense: always both products; blocked for one layer, two plain
+   * products for several) */
   void get_g_mult(const V &U, bool force_mul) {
     if (A.kind == 0) {
-      dense_AtA_blocked<TC>(A, U.data(), Au.data(), g.data(), o.submat_mb > 0 ? o.submat_mb : 5);
+      if (layers == 1) {
+        dense_AtA_blocked<TC>(A, U.data(), Au.data(), g.data(), o.submat_mb > 0 ? o.submat_mb : 5);
+      } else {
+        gemvL(false, U.data(), Au.data());
+        gemvL(true, Au.data(), g.data());
+      }
       return;
     }
-    gemv(A, false, U.data(), Au.data());
+    gemvL(false, U.data(), Au.data());
     if (o.skipMulRatio == 0) {
-      gemv(A, true, Au.data(), g.data());
+      gemvL(true, Au.data(), g.data());
       return;
     }
     V deltaAu(Au);
@@ -580,15 +600,15 @@ struct Solver {
     const TS relChange = nrm_deltaAu / nrm_Au;
     const TS ratio = (TS)o.skipMulRatio, maxrel = (TS)o.maxRelativeChange;
     if (force_mul) {
-      last_Au = Au; gemv(A, true, Au.data(), g.data()); currSkipValue = 0;
+      last_Au = Au; gemvL(true, Au.data(), g.data()); currSkipValue = 0;
     } else if (currInitIteration < o.numInitIterations) {
-      last_Au = Au; gemv(A, true, Au.data(), g.data());
+      last_Au = Au; gemvL(true, Au.data(), g.data());
       currInitIteration++; currSkipValue = 0;
     } else if (currSkipValue < 1) {
-      last_Au = Au; gemv(A, true, Au.data(), g.data());
+      last_Au = Au; gemvL(true, Au.data(), g.data());
       currSkipValue += ratio;
     } else if (relChange > maxrel) {
-      last_Au = Au; gemv(A, true, Au.data(), g.data());
+      last_Au = Au; gemvL(true, Au.data(), g.data());
     } else {
       currSkipValue -= 1;
     }
@@ -618,7 +638,7 @@ struct Solver {
       if (z3) dDd = dDd + sdot(n, dz.data(), dz.data());
       const TC dd = sdot(n, d.data(), d.data());
       V Ad(m);
-      gemv(A, false, d.data(), Ad.data());
+      gemvL(false, d.data(), Ad.data());
       const TC Add = sdot(m, Ad.data(), Ad.data());
       tau = std::abs((TS)dd / ((TS)dDd + muDbeta * (TS)Add));
     } else {
@@ -754,7 +774,7 @@ struct Solver {
     V DtsAtd(n, (TC)0);
     TS gam = o_gam;
     const TS scl = scaleb(bs);
-    gemv(A, true, bs.data(), Atb.data());
+    gemvL(true, bs.data(), Atb.data());
     V U(Atb);
     if (mu > muf) mu = muf;
     if (beta > betaf) beta = betaf;
@@ -930,12 +950,21 @@ double tv3o_dot(long long n, const float *x, const float *y) {
   return ddot(n, x, y);
 }
 
+/* U's pixel count is A.n times the number of layers; several layers only
+ * for 2-D dense and sparse operators (2d/tval3_cpu_2d.cpp:737-748 forbids
+ * them for geometry) */
+bool layers_ok(const Op &op, const Dims &dm) {
+  if (op.n <= 0 || dm.n() % op.n != 0) return false;
+  const i64 L = dm.n() / op.n;
+  return L == 1 || (L > 1 && dm.r == 1 && op.kind != 2);
+}
+
 int tv3o_solve(const tv3o_op *h, const tv3_options *o, int q, int p, int r,
                const float *b, float *U, const float *Ut, tv3_info *info,
                tv3_trace_rec *trace, int trace_cap) {
   const Op &op = *(const Op *)h;
   const Dims dm{q, p, r};
-  if (dm.n() != op.n) return 1;
+  if (!layers_ok(op, dm)) return 1;
   if (o->scal_double) {
     Solver<float, double> s(op, *o, dm);
     s.run(b, U, Ut, info, trace, trace_cap);
@@ -954,7 +983,7 @@ int tv3o_solve_f64(const tv3o_op *h, const tv3_options *o, int q, int p,
                    tv3_info *info, tv3_trace_rec *trace, int trace_cap) {
   const Op &op = *(const Op *)h;
   const Dims dm{q, p, r};
-  if (dm.n() != op.n) return 1;
+  if (!layers_ok(op, dm)) return 1;
   Solver<double, double> s(op, *o, dm);
   s.run(b, U, Ut, info, trace, trace_cap);
   return 0;

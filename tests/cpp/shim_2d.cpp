@@ -24,9 +24,10 @@ static int args_mode() {
   expect_error("dense U length", [&] { tval3_gpu_2d(U, A_bad, b, of, Ut); });
   mat<float> b_bad(7);
   expect_error("dense b length", [&] { tval3_gpu_2d(U, A, b_bad, of, Ut); });
-  mat<float> U2(4, 8), Ut2(4, 8), b2(12);
+  // two layers by b.len / A.dim_y, one measurement too many for them
+  mat<float> U2(4, 8), Ut2(4, 8), b2(13);
   U2.format = Ut2.format = mat_col_major;
-  expect_error("dense layers", [&] { tval3_gpu_2d(U2, A, b2, of, Ut2); });
+  expect_error("dense layers b length", [&] { tval3_gpu_2d(U2, A, b2, of, Ut2); });
   sparse_mat<float> S;
   S.dim_x = 16; S.dim_y = 6; S.format = sparse_mat_csr;
   expect_error("sparse format", [&] { tval3_gpu_2d(U, S, b, of, Ut); });

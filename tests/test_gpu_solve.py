@@ -87,6 +87,69 @@ def test_dense_continuation_nonneg(ctx):
     _compare(ctx, ctx.plan_dense(A), O.Op.dense(A), b, (24, 20, 1), opts, Ut=x)
 
 
+def _layered(L, q, p_layer, frac, seed, sparse=False, n8=True):
+    """L images of q x p_layer stacked along y and one A for all of them
+    (the reference's 2-D multi-layer mode)."""
+    N = q * p_layer
+    M = int(round(frac * N))
+    A = synth.gaussian_A(M, N, seed)
+    if sparse:
+        A[np.abs(A) < 0.8 * np.abs(A).mean()] = 0
+    x = np.concatenate([synth.ellipse_phantom(p_layer, q, 4, seed + l).ravel() for l in range(L)])
+    b = np.concatenate([synth.add_noise(A @ x[l * N:(l + 1) * N], 0.01, seed + l)
+                        for l in range(L)]).astype(np.float32)
+    return A, x, b, (q, L * p_layer, 1)
+
+
+@pytest.mark.parametrize("ls", [1, 0])
+@pytest.mark.parametrize("shape", [(3, 16, 16, 0.4), (2, 15, 13, 0.3026)])
+def test_dense_layers(ctx, shape, ls, monkeypatch):
+    """Multi-layer dense solve (layers = U pixels / A.n) against the oracle.
+    15 x 13 layers have rows of 195 entries (scalar loads) and 59 rows, so
+    with the untuned shape of two rows per wavefront one wavefront's rows
+    cross the layer boundary."""
+    monkeypatch.setenv("TV3_AUTOTUNE", "0")
+    L, q, p, frac = shape
+    A, x, b, dims = _layered(L, q, p, frac, 21)
+    assert shape[0] != 2 or A.shape[0] == 59
+    opts = tval3_amd.default_options(maxit=80, ls_exact=ls)
+    _compare(ctx, ctx.plan_dense(A), O.Op.dense(A), b, dims, opts, Ut=x, strict_traj=bool(ls))
+
+
+def test_dense_layers_bf16_and_sparse(ctx):
+    import scipy.sparse as sp
+    A, x, b, dims = _layered(2, 16, 12, 0.5, 31)
+    A16 = A.astype(np.float32)
+    Ab = ctx.torch.tensor(A16).to(ctx.torch.bfloat16).float().numpy()
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1, scal_double=1)
+    _compare(ctx, ctx.plan_dense(A16, bf16=True), O.Op.dense(Ab), b, dims, opts, Ut=x)
+    A, x, b, dims = _layered(3, 12, 10, 0.5, 41, sparse=True)
+    S, T = sp.csr_matrix(A), sp.csc_matrix(A)
+    args = (A.shape[0], A.shape[1], S.indptr, S.indices, S.data, T.indptr, T.indices, T.data)
+    _compare(ctx, ctx.plan_sparse(*args), O.Op.sparse(*args), b, dims, opts, Ut=x)
+
+
+def test_layers_batch_and_errors(ctx):
+    """Multi-layer instances of different layer counts in one batch (W = 1
+    groups); a dense batch cannot mix them with single-layer instances, and
+    the projector takes no layers."""
+    A, x, b, dims = _layered(3, 16, 16, 0.4, 51)
+    A2, x2, b2, dims2 = _layered(2, 16, 16, 0.4, 52)
+    plan = ctx.plan_dense(A)
+    opts = tval3_amd.default_options(maxit=50, ls_exact=1)
+    Us, infos = ctx.solve_batch([plan, plan], [b, b[: 2 * plan.m]], [dims, (16, 32, 1)], opts)
+    for U, bb, d in ((Us[0], b, dims), (Us[1], b[: 2 * plan.m], (16, 32, 1))):
+        Uo, _, _ = O.solve(O.Op.dense(A), bb, d, opts)
+        assert _rel(U.cpu().numpy(), Uo) == 0.0
+    b1 = b[: plan.m]
+    with pytest.raises(tval3_amd.Tval3Error, match="mix"):
+        ctx.solve_batch([plan, plan], [b, b1], [dims, (16, 16, 1)], opts)
+    geo = synth.ring_geometry(16, 16)
+    gplan = ctx.plan_geometry(geo)
+    with pytest.raises(tval3_amd.Tval3Error, match="layers"):
+        ctx.solve(gplan, np.zeros(2 * gplan.m, np.float32), (16, 32, 1), opts)
+
+
 def _ring_case(K, grid, seed):
     geo = synth.ring_geometry(K, grid)
     op = O.Op.geometry(geo)

@@ -576,3 +576,67 @@ def test_dense_get_g_blocking(n, m, smb):
     plain = op.gemv(Au, trans=True)
     if nblk > 2:
         assert np.any(plain != g)
+
+
+def _layered_case(L, q, p_layer, frac, seed, sparse):
+    """L layers of a q x p_layer image stacked along y, one A for all of them;
+    returns the layered operator, the explicit block-diagonal operator
+    I_L (x) A, b and the image dims."""
+    import scipy.sparse as sp
+    N = q * p_layer
+    M = int(round(frac * N))
+    A = synth.gaussian_A(M, N, seed)
+    if sparse:
+        A[np.abs(A) < 0.8 * np.abs(A).mean()] = 0
+    x = np.concatenate([synth.ellipse_phantom(p_layer, q, 4, seed + l).ravel() for l in range(L)])
+    Abig = np.kron(np.eye(L, dtype=np.float32), A).astype(np.float32)
+    b = synth.add_noise(Abig.astype(np.float64) @ x, 0.01, seed).astype(np.float32)
+    if sparse:
+        S, T = sp.csr_matrix(A), sp.csc_matrix(A)
+        Sb, Tb = sp.csr_matrix(Abig), sp.csc_matrix(Abig)
+        op = O.Op.sparse(M, N, S.indptr, S.indices, S.data, T.indptr, T.indices, T.data)
+        big = O.Op.sparse(L * M, L * N, Sb.indptr, Sb.indices, Sb.data, Tb.indptr, Tb.indices, Tb.data)
+    else:
+        op, big = O.Op.dense(A), O.Op.dense(Abig)
+    return op, big, b, (q, L * p_layer, 1)
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+@pytest.mark.parametrize("sd", [0, 1])
+def test_layers_equal_block_diagonal_operator(sparse, sd):
+    """The 2-D multi-layer mode (layers = b.len / A.dim_y; 2d/mat_mult.h:22-35,
+    68-90 multiply every layer by A; 2d:365 and 390-393 take two plain
+    products in get_g) is the single-layer solve with the block-diagonal
+    operator I_L (x) A.  With one row block (submat_mb large) the blocked get_g
+    of that solve is the plain pair of products too, and the zero blocks add
+    exact zeros in order, so the two solves agree bit for bit."""
+    op, big, b, dims = _layered_case(3, 12, 10, 0.4, 7, sparse)
+    opts = default_options(maxit=60, scal_double=sd, submat_mb=4096)
+    U1, i1, t1 = O.solve(op, b, dims, opts, trace_cap=100)
+    U2, i2, t2 = O.solve(big, b, dims, opts, trace_cap=100)
+    np.testing.assert_array_equal(U1, U2)
+    assert i1.total_iters == i2.total_iters and i1.outer_iters == i2.outer_iters
+    assert [t.f for t in t1] == [t.f for t in t2]
+
+
+def test_layers_take_plain_products():
+    """Several layers never take the blocked get_g product (2d:365): 1 MiB row
+    blocks change nothing there, while they do change the single-layer solve
+    of the block-diagonal operator (720 rows in blocks of 145)."""
+    op, big, b, dims = _layered_case(2, 30, 30, 0.4, 5, False)
+    o1, o2 = default_options(maxit=20, submat_mb=1), default_options(maxit=20, submat_mb=4096)
+    np.testing.assert_array_equal(O.solve(op, b, dims, o1)[0], O.solve(op, b, dims, o2)[0])
+    assert np.any(O.solve(big, b, dims, o1)[0] != O.solve(big, b, dims, o2)[0])
+
+
+def test_layers_argument_checks():
+    op, _, b, (q, p, r) = _layered_case(2, 8, 6, 0.5, 3, False)
+    opts = default_options(maxit=5)
+    with pytest.raises(ValueError):       # b is one layer short
+        O.solve(op, b[: op.m], (q, p, r), opts)
+    with pytest.raises(ValueError):       # U is not a whole number of layers
+        O.solve(op, b, (q, p + 1, r), opts)
+    geo = synth.ring_geometry(16, 16)
+    gop = O.Op.geometry(geo)
+    with pytest.raises(ValueError):       # no layers for the projector
+        O.solve(gop, np.zeros(2 * gop.m, np.float32), (16, 32, 1), opts)

@@ -57,7 +57,7 @@ def test_argument_checks(name, expect):
     # the engine's own limits are argument or internal errors, never silent
     assert got["double data"].startswith("Internal error: ")
     if name == "shim_2d":
-        assert got["dense layers"].startswith("Argument error: multilayer")
+        assert got["dense layers b length"] == "Argument error: b.len must be equal to A.dim_y * layers!"
         assert got["complex data"].startswith("Internal error: ")
 
 
@@ -97,6 +97,21 @@ def test_shim_2d_dense(tmp_path):
     d = str(tmp_path / "dense2d")
     _write(d, {"dx": q, "dy": p, "m": m, "n": q * p, "maxit": 80}, A=A, b=b)
     _run_and_compare("shim_2d", "dense", d, O.Op.dense(A), b, (q, p, 1), 80)
+
+
+@pytest.mark.gpu
+def test_shim_2d_dense_layers(tmp_path):
+    """Multi-layer call: b holds three measurement vectors of one A, U the
+    three images stacked along y (layers = b.len / A.dim_y)."""
+    q = p = 16
+    L, m = 3, 100
+    A = synth.gaussian_A(m, q * p, seed=4)
+    x = np.concatenate([synth.ellipse_phantom(p, q, 4, seed=4 + l).ravel() for l in range(L)])
+    b = np.concatenate([synth.add_noise(A @ x[l * q * p:(l + 1) * q * p], 0.01, seed=4 + l)
+                        for l in range(L)]).astype(np.float32)
+    d = str(tmp_path / "dense2d_layers")
+    _write(d, {"dx": q, "dy": L * p, "m": m, "n": q * p, "maxit": 60}, A=A, b=b)
+    _run_and_compare("shim_2d", "dense", d, O.Op.dense(A), b, (q, L * p, 1), 60)
 
 
 @pytest.mark.gpu

@@ -71,7 +71,11 @@ struct PlanDev {
 // ---- batch description ------------------------------------------------------
 struct GroupDesc {
   int q, p, r;          // dim_x, dim_y, dim_z
-  int n, m;             // pixels, measurements
+  int n, m;             // pixels, measurements (plan.n, plan.m times layers)
+  // 2-D multi-layer mode: the n-vectors hold `layers` images of plan.n
+  // pixels one after the other and the m-vectors `layers` blocks of plan.m;
+  // the operator applies A to each (W = 1 batches only)
+  int layers;
   long long off_n;      // float offset of this group's n-vectors
   long long off_m;      // float offset of this group's m-vectors
   PlanDev plan;

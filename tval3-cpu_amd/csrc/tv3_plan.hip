@@ -68,13 +68,16 @@ __global__ void k_prof_bytes(const GroupDesc *G, int W, int Bpad, const InstScal
   if (i >= Bpad) return;
   const uint32_t f = S[i].flags;
   if (!(f & F_LIVE) || (mask && (f & mask) != mask)) return;
-  const PlanDev &pd = G[i / W].plan;
+  const GroupDesc &gd = G[i / W];
+  const PlanDev &pd = gd.plan;
   const unsigned long long m = pd.m, n = pd.n, nnz = pd.nnz;
+  // the operator once, the vectors of every layer
+  const unsigned long long vec = 4ull * gd.n + 4ull * gd.m;
   unsigned long long b;
   if (pd.kind == kDense) {
-    b = (pd.dtype == TV3_BF16 ? 2ull : 4ull) * m * n + 4 * n + 4 * m;
+    b = (pd.dtype == TV3_BF16 ? 2ull : 4ull) * m * n + vec;
   } else {  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
-    b = 8 * nnz + 4 * ((trans ? n : m) + 1) + 4 * n + 4 * m;
+    b = 8 * nnz + 4 * ((trans ? n : m) + 1) + vec;
   }
   atomicAdd(&acc[trans], b);
 }
@@ -116,25 +119,27 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w1(const GroupDesc *G,
   const int row = blockIdx.x * (kBlock / kSub) + (threadIdx.x / kSub);
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const PlanDev &P = gd.plan;
-  const float *xg = x + gd.off_n;
+  // row of A and layer of this output row (one layer: the row itself)
+  const int ar = row % P.m;
+  const float *xg = x + gd.off_n + (long long)(row / P.m) * P.n;
   double acc = 0.0;
   if (row < gd.m) {
     if (P.kind == kGeometry) {
-      const RayDesc rd = P.rays[row];
+      const RayDesc rd = P.rays[ar];
       const float sc = P.scale;
       for (int j = sl; j <= rd.L; j += kSub) acc += (double)(xg[ray_lin(rd, j)] * sc);
     } else if (P.kind == kSparse) {
-      const int k0 = P.r_ptr[row], k1 = P.r_ptr[row + 1];
+      const int k0 = P.r_ptr[ar], k1 = P.r_ptr[ar + 1];
       for (int k = k0 + sl; k < k1; k += kSub)
         acc += (double)P.r_val[k] * (double)xg[P.r_ind[k]];
     } else {
-      const long long base = (long long)row * gd.n;
+      const long long base = (long long)ar * P.n;
       if (P.dtype == TV3_F32) {
         const float *A = (const float *)P.A + base;
-        for (int j = sl; j < gd.n; j += kSub) acc += (double)A[j] * (double)xg[j];
+        for (int j = sl; j < P.n; j += kSub) acc += (double)A[j] * (double)xg[j];
       } else {
         const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
-        for (int j = sl; j < gd.n; j += kSub)
+        for (int j = sl; j < P.n; j += kSub)
           acc += (double)__bfloat162float(A[j]) * (double)xg[j];
       }
     }
@@ -491,15 +496,17 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
   const int col = blockIdx.x * (kBlock / kSub) + (threadIdx.x / kSub);
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const PlanDev &P = gd.plan;
-  const float *xg = x + gd.off_m;
+  // column of A and layer of this output entry (one layer: the column itself)
+  const int ac = col % P.n;
+  const float *xg = x + gd.off_m + (long long)(col / P.n) * P.m;
   double acc = 0.0;
   if (col < gd.n) {
     if (P.kind == kGeometry) {
-      const int k0 = P.g_cptr[col], k1 = P.g_cptr[col + 1];
+      const int k0 = P.g_cptr[ac], k1 = P.g_cptr[ac + 1];
       const float sc = P.scale;
       for (int k = k0 + sl; k < k1; k += kSub) acc += (double)(xg[P.g_cray[k]] * sc);
     } else {
-      const int k0 = P.c_ptr[col], k1 = P.c_ptr[col + 1];
+      const int k0 = P.c_ptr[ac], k1 = P.c_ptr[ac + 1];
       for (int k = k0 + sl; k < k1; k += kSub)
         acc += (double)P.c_val[k] * (double)xg[P.c_ind[k]];
     }
@@ -794,7 +801,9 @@ __device__ __forceinline__ void loadc(const T *p, float (&v)[C]) {
 // lane loads 8 consecutive entries of each row (16 bytes of bf16, 32 of f32)
 // and the matching 8 x values once for all R rows per step; FP64 lane sums,
 // fixed butterfly.  Each row's summation order is the same for every R, so
-// R is a pure speed choice (picked by the autotuner).
+// R is a pure speed choice (picked by the autotuner).  With several layers,
+// output row l * plan.m + i is row i of A times layer l of x; R rows that
+// cross a layer boundary are taken one at a time.
 template <class T, int R>
 __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
                                                          const float *x,
@@ -806,10 +815,22 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
   const int row0 = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * R;
   if (row0 >= gd.m) return;
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
-  const int n = gd.n;
+  const int n = gd.plan.n, pm = gd.plan.m;
   const int nr = min(R, gd.m - row0);
-  const T *A = (const T *)gd.plan.A + (long long)row0 * n;
-  const float *xg = x + gd.off_n;
+  if (R > 1 && row0 / pm != (row0 + nr - 1) / pm) {
+    for (int r = 0; r < nr; ++r) {
+      const int row = row0 + r;
+      const T *Ar = (const T *)gd.plan.A + (long long)(row % pm) * n;
+      const float *xr = x + gd.off_n + (long long)(row / pm) * n;
+      double a = 0.0;
+      for (int j = lane; j < n; j += 64) a += (double)a_val(Ar, j) * (double)xr[j];
+      a = wave_sum(a);
+      if (lane == 0) y[gd.off_m + row] = (float)a;
+    }
+    return;
+  }
+  const T *A = (const T *)gd.plan.A + (long long)(row0 % pm) * n;
+  const float *xg = x + gd.off_n + (long long)(row0 / pm) * n;
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -854,20 +875,23 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
     const GroupDesc *G, const float *x, double *scratch, const InstScal *S,
     uint32_t mask, long long sstride) {
   const GroupDesc &gd = G[blockIdx.z];
-  const int n = gd.n;
+  // c0 indexes the layered output (gd.n = layers * n entries); with n % 8 ==
+  // 0 (a_vec) a thread's 8 columns lie in one layer
+  const int n = gd.plan.n, pm = gd.plan.m;
   const int c0 = (blockIdx.x * kBlock + threadIdx.x) * 8;
-  if (c0 >= n) return;
+  if (c0 >= gd.n) return;
   if (mask && (S[blockIdx.z].flags & mask) != mask) return;
-  const int nch = min(kDenseChunks, (gd.m + kBlock - 1) / kBlock);
+  const int nch = min(kDenseChunks, (pm + kBlock - 1) / kBlock);
   const int c = blockIdx.y;
   if (c >= nch) return;
-  const int R = (gd.m + nch - 1) / nch;
-  const int i0 = c * R, i1 = min(gd.m, i0 + R);
-  const float *xg = x + gd.off_m;
+  const int R = (pm + nch - 1) / nch;
+  const int i0 = c * R, i1 = min(pm, i0 + R);
   const T *A = (const T *)gd.plan.A;
-  const int w = min(8, n - c0);
+  const int w = min(8, gd.n - c0);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (w == 8 && gd.plan.a_vec) {
+    const float *xg = x + gd.off_m + (long long)(c0 / n) * pm;
+    const int j0 = c0 % n;
     // U rows loaded before they are consumed, in row order: the per-column
     // summation order does not depend on U
     int i = i0;
@@ -876,7 +900,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
       double xi[U];
 #pragma unroll
       for (int r = 0; r < U; ++r) {
-        load8(A + (long long)(i + r) * n + c0, a[r]);
+        load8(A + (long long)(i + r) * n + j0, a[r]);
         xi[r] = (double)xg[i + r];
       }
 #pragma unroll
@@ -887,17 +911,19 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
     for (; i < i1; ++i) {
       const double xi = (double)xg[i];
       float a[8];
-      load8(A + (long long)i * n + c0, a);
+      load8(A + (long long)i * n + j0, a);
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc[u] += (double)a[u] * xi;
     }
   } else {
-    for (int i = i0; i < i1; ++i) {
-      const double xi = (double)xg[i];
-      for (int u = 0; u < w; ++u) acc[u] += (double)a_val(A, (long long)i * n + c0 + u) * xi;
+    for (int u = 0; u < w; ++u) {
+      const int col = c0 + u;
+      const float *xg = x + gd.off_m + (long long)(col / n) * pm;
+      const int j = col % n;
+      for (int i = i0; i < i1; ++i) acc[u] += (double)a_val(A, (long long)i * n + j) * (double)xg[i];
     }
   }
-  double *out = scratch + blockIdx.z * sstride + (long long)c * n + c0;
+  double *out = scratch + blockIdx.z * sstride + (long long)c * gd.n + c0;
   for (int u = 0; u < w; ++u) out[u] = acc[u];
 }
 
@@ -908,7 +934,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce(
   const int col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= gd.n) return;
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
-  const int nch = min(kDenseChunks, (gd.m + kBlock - 1) / kBlock);
+  const int nch = min(kDenseChunks, (gd.plan.m + kBlock - 1) / kBlock);
   const double *sp = scratch + blockIdx.y * sstride + col;
   double acc = 0.0;
   for (int c = 0; c < nch; ++c) acc += sp[(long long)c * gd.n];

@@ -1427,6 +1427,7 @@ struct Batch {
   long long Nn = 0, Nm = 0;
   long long dblk_doubles = 0;  // slice partials of the blocked dense product
   bool z3 = false;
+  bool layered = false;  // multi-layer instances: get_g takes two plain products
 };
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1436,7 +1437,7 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // launches after one warm-up); the fastest is cached per device and batch
 // signature.  Candidates never change results (see DenseTune).
 // TV3_AUTOTUNE=0 keeps the default shape.
-using TuneKey = std::tuple<int, int, int, int, int, long long>;
+using TuneKey = std::tuple<int, int, int, int, int, long long, int>;
 std::mutex g_tune_mu;
 std::map<TuneKey, DenseTune> g_tune_cache;
 
@@ -1607,7 +1608,7 @@ class Engine {
     if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
     long long work = 0;
     for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
-    const TuneKey key{ctx_->device, bt_.dtype, bt_.G, bt_.max_n, bt_.max_m, work};
+    const TuneKey key{ctx_->device, bt_.dtype, bt_.G, bt_.max_n, bt_.max_m, work, bt_.layered ? 1 : 0};
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
@@ -1626,7 +1627,8 @@ class Engine {
       for (int u : {1, 2, 4, 8}) {
         dense_tune_set(DenseTune{best.rows, u, c});
         const float t = time_launches([&] {
-          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, S_, 0, nullptr, scratch_, submat_mb_);
+          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, S_, 0, nullptr, scratch_,
+                        bt_.layered ? 0 : submat_mb_);
         });
         if (t < tbest) { tbest = t; best.unroll = u; best.cols = c; }
       }
@@ -1750,7 +1752,8 @@ class Engine {
   }
   // blocked: the get_g product (dense A: row blocks of submat_mb)
   void bwd(const float *x, float *y, uint32_t mask, const float *sub, bool blocked = false) {
-    const int smb = blocked ? submat_mb_ : 0;
+    // several layers: two plain products (2d/tval3_cpu_2d.cpp:365, 390-393)
+    const int smb = (blocked && !bt_.layered) ? submat_mb_ : 0;
     if (cmode_ == 2) {
       const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
       const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
@@ -2061,28 +2064,45 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
   Batch bt;
   bt.B = B;
   bool shared = true;
+  std::vector<int> layers(B, 1);
   for (int i = 0; i < B; ++i) {
     if (!A[i] || !b[i] || !U[i]) return set_err(ctx, TV3_ERR_ARG, "Argument error: null plan or vector");
     const int q = dims[3 * i], p = dims[3 * i + 1], r = dims[3 * i + 2];
     if (q <= 0 || p <= 0 || r <= 0)
       return set_err(ctx, TV3_ERR_ARG, "Argument error: volume dimensions must be positive");
-    if ((long long)q * p * r != A[i]->dev.n)
-      return set_err(ctx, TV3_ERR_ARG, "Argument error: the length of U must be equal to A.dim_x!");
+    // 2-D multi-layer mode (2d/tval3_cpu_2d.cpp:889-1016: layers = b.len /
+    // A.dim_y): U holds a whole number of A.dim_x-pixel layers
+    const long long nu = (long long)q * p * r, na = A[i]->dev.n;
+    if (nu != na) {
+      if (r != 1 || A[i]->dev.kind == kGeometry || na <= 0 || nu % na != 0)
+        return set_err(ctx, TV3_ERR_ARG, "Argument error: the length of U must be equal to A.dim_x * layers!");
+      if ((nu / na) * (long long)A[i]->dev.m > 0x7fffffffLL)
+        return set_err(ctx, TV3_ERR_ARG, "Argument error: too many layers");
+      layers[i] = (int)(nu / na);
+      bt.layered = true;
+    }
     if (A[i]->dev.kind != A[0]->dev.kind ||
         (A[i]->dev.kind == kDense && A[i]->dev.dtype != A[0]->dev.dtype))
       return set_err(ctx, TV3_ERR_ARG, "Argument error: one batch needs one operator kind and element type");
     if (i > 0 && (A[i] != A[0] || q != dims[0] || p != dims[1] || r != dims[2])) shared = false;
     if (r > 1) bt.z3 = true;
   }
+  if (bt.layered && A[0]->dev.kind == kDense)
+    for (int i = 0; i < B; ++i)
+      if (layers[i] == 1)
+        return set_err(ctx, TV3_ERR_ARG,
+                       "Argument error: a dense batch cannot mix multi-layer and single-layer instances");
   bt.kind = A[0]->dev.kind;
   bt.dtype = A[0]->dev.dtype;
   // shared operator and size: 64-lane groups (one walk and one 256-byte
   // gather per voxel for 64 instances).  TV3_GROUP_WIDTH=16 selects 16-lane
   // groups dealt one per XCD (measured slower on the 256x256 ring).
-  bt.W = (shared && B > 1) ? 64 : 1;
+  // Multi-layer instances run in W = 1 groups.
+  bt.W = (shared && B > 1 && !bt.layered) ? 64 : 1;
   if (const char *ev = getenv("TV3_GROUP_WIDTH")) {
     const int w = atoi(ev);
-    if (shared && B > 1 && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense)) bt.W = w;
+    if (shared && B > 1 && !bt.layered && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense))
+      bt.W = w;
   }
   const int W = bt.W;
   bt.G = (B + W - 1) / W;
@@ -2092,7 +2112,8 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
     const int i = g * W;
     GroupDesc gd{};
     gd.q = dims[3 * i]; gd.p = dims[3 * i + 1]; gd.r = dims[3 * i + 2];
-    gd.n = A[i]->dev.n; gd.m = A[i]->dev.m;
+    gd.layers = layers[i];
+    gd.n = A[i]->dev.n * gd.layers; gd.m = A[i]->dev.m * gd.layers;
     gd.off_n = on; gd.off_m = om;
     gd.plan = A[i]->dev;
     // offsets stay multiples of 64 floats (256-byte aligned group vectors)
@@ -2100,7 +2121,7 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
     om += ((long long)gd.m * W + 63) / 64 * 64;
     bt.max_n = std::max(bt.max_n, gd.n);
     bt.max_m = std::max(bt.max_m, gd.m);
-    if (W == 1 && gd.plan.kind == kDense) {
+    if (W == 1 && gd.plan.kind == kDense && gd.layers == 1) {
       // blocked get_g product: groups with up to 64 row blocks split each
       // block into row slices (about 16 or more chunks in all)
       const int smb = o->submat_mb > 0 ? o->submat_mb : 5;

@@ -265,8 +265,8 @@ class Context:
             opts = default_options()
         q, p, r = dims
         b = self.tensor(b, t.float32)
-        if b.numel() != plan.m:
-            raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y!")
+        if b.numel() != plan.m * _layers(plan, dims):
+            raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y * layers!")
         U = t.empty(q * p * r, dtype=t.float32, device=self.device)
         Utt = self.tensor(Ut, t.float32) if Ut is not None else None
         if Utt is not None and Utt.numel() != q * p * r:
@@ -319,9 +319,9 @@ class Context:
         if len(bs) != B or len(dims) != B:
             raise Tval3Error(1, "Argument error: plans, b and dims differ in length")
         bts = [self.tensor(b, t.float32) for b in bs]
-        for pl, x in zip(plans, bts):
-            if x.numel() != pl.m:
-                raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y!")
+        for pl, x, d in zip(plans, bts, dims):
+            if x.numel() != pl.m * _layers(pl, d):
+                raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y * layers!")
         if Uts is not None:
             for d, u in zip(dims, Uts):
                 if u is not None and np.size(u) != d[0] * d[1] * d[2]:
@@ -340,6 +340,13 @@ class Context:
         self._call(lib().tv3_solve_batch, self.h, B, parr, C.byref(opts),
                                           dims_arr, barr, uarr, utarr, infos)
         return Us, [infos[i] for i in range(B)]
+
+
+def _layers(plan, dims):
+    """Layers of a 2-D multi-layer solve: U's pixel count over A's columns
+    (1 when they are equal; the library rejects other mismatches)."""
+    n = int(dims[0]) * int(dims[1]) * int(dims[2])
+    return n // plan.n if plan.n > 0 and n % plan.n == 0 and n > plan.n else 1
 
 
 class Plan:
