@@ -60,6 +60,9 @@
  *                    equals the Lagrangian, multipliers follow the ALM update
  *   ls_exact = 0     same steps as the round-by-round search (f(alpha) is
  *                    an exact quadratic)
+ *   layers > 1       bit-identical to the single-layer solve with the
+ *                    block-diagonal operator I (x) A (dense and sparse), and
+ *                    independent of submat_mb (two plain products, 2d:365)
  * Parity unpinned: the A^T-skipping state machine of get_g_mult
  * (2d/tval3_cpu_2d.cpp:300-357, skipMulRatio > 0).  No closed form or
  * invariant fixes its skip decisions; it follows the reference's control
