@@ -25,6 +25,8 @@
  *                                   3d/tval3_cpu_3d.cpp:834-871)
  *   tv3_dot / tv3_nrm2           <- cblas_dotc / cblas_nrm2 wrappers
  *                                   (2d/blas_wrap.h:71-112)
+ *   tv3_geometry_save/load,      <- no counterpart: an on-disk form of the
+ *   tv3_sparse_save/load            caller-filled geometry / sparse_mat<T>
  *   tv3_options / tv3_info       <- tval3_options<T> / tval3_info<T> (types
  *                                   from the reference's tval3_types.h, which
  *                                   the reference tree does not ship)
@@ -198,6 +200,32 @@ int tv3_plan_shape(const tv3_plan *plan, long long *m, long long *n,
  * order, voxels in walk order, into host arrays ptr[m+1] and lin[nnz]. */
 int tv3_plan_walk(tv3_ctx *ctx, const tv3_plan *plan, long long *ptr,
                   int *lin);
+
+/* ---- operator files (host only; no GPU use; ctx may be NULL) -------------- */
+/* The reference fills geometry and sparse_mat<T> in the caller
+ * (3d/mat_mult.h:148-202, 2d/mat_mult.h:38-91) and has no file format; these
+ * write and read one.  Little-endian and packed: "TV3GEOM1", int32 num_mps,
+ * num_emitters, num_receivers, num_paths, rv_x, rv_y, rv_z, float32
+ * scale_factor, int32 use_path[E*R], x/y/z_emitters[P*E], x/y/z_receivers[P*R];
+ * "TV3SPAR1", int32 m, n, nnz, csr_ptr[m+1], csr_ind[nnz], float32
+ * csr_val[nnz], int32 csc_ptr[n+1], csc_ind[nnz], float32 csc_val[nnz].
+ * A load checks the magic, the exact length and every count and index, and
+ * points the description into *storage, released with tv3_storage_free. */
+int tv3_geometry_save(tv3_ctx *ctx, const char *path, const tv3_geometry *g);
+int tv3_geometry_load(tv3_ctx *ctx, const char *path, tv3_geometry *g,
+                      void **storage);
+int tv3_sparse_save(tv3_ctx *ctx, const char *path, int m, int n, int nnz,
+                    const int *csr_ptr, const int *csr_ind,
+                    const float *csr_val, const int *csc_ptr,
+                    const int *csc_ind, const float *csc_val);
+int tv3_sparse_load(tv3_ctx *ctx, const char *path, int *m, int *n, int *nnz,
+                    const int **csr_ptr, const int **csr_ind,
+                    const float **csr_val, const int **csc_ptr,
+                    const int **csc_ind, const float **csc_val,
+                    void **storage);
+void tv3_storage_free(void *storage);
+/* Text of the calling thread's last failed operator-file call. */
+const char *tv3_io_error(void);
 
 /* ---- operators ------------------------------------------------------------ */
 /* y = A x (trans = TV3_NOTRANS) or y = A^T x (TV3_TRANS); overwrites y. */

@@ -167,6 +167,30 @@ def test_ring_geometry_solve(ctx, sd, ls):
              strict_traj=bool(ls))
 
 
+def test_multi_mp_geometry(ctx):
+    """Three measurement positions (numMPs > 1) with skipped pairs, a shuffled
+    output order and rows no pair maps to: products, one solve and a 70-lane
+    batch (two 64-lane groups, tiled walk orders over all positions) against
+    the oracle."""
+    geo = synth.multi_mp_ring_geometry(32, 40, 3, skip_every=5, extra_rows=9, seed=3)
+    op = O.Op.geometry(geo)
+    plan = ctx.plan_geometry(geo)
+    x = synth.disk_phantom(40, seed=4)
+    y = ctx.gemv(plan, x).cpu().numpy()
+    np.testing.assert_array_equal(y, op.gemv(x))
+    yb = synth.add_noise(y, 0.01, 4)
+    np.testing.assert_array_equal(ctx.gemv(plan, yb, trans=True).cpu().numpy(), op.gemv(yb, trans=True))
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1)
+    _compare(ctx, plan, op, yb, (40, 40, 1), opts, Ut=x)
+    B = 70
+    bs = [synth.add_noise(y, 0.01, 10 + i) for i in range(B)]
+    Us, infos = ctx.solve_batch([plan] * B, bs, [(40, 40, 1)] * B, opts)
+    for i in (0, 33, 64, 69):
+        Uo, info_o, _ = O.solve(op, bs[i], (40, 40, 1), opts)
+        assert _rel(Us[i].cpu().numpy(), Uo) == 0.0, i
+        assert infos[i].total_iters == info_o.total_iters
+
+
 @pytest.mark.parametrize("ls", [1, 0])
 def test_ring256_workload(ctx, ls):
     """The benchmark workload itself (256 transducers, 256x256, maxit 200)."""

@@ -640,3 +640,39 @@ def test_layers_argument_checks():
     gop = O.Op.geometry(geo)
     with pytest.raises(ValueError):       # no layers for the projector
         O.solve(gop, np.zeros(2 * gop.m, np.float32), (16, 32, 1), opts)
+
+
+@pytest.mark.parametrize("skip,extra", [(0, 0), (3, 7)])
+def test_multi_mp_rows_are_per_position_operators(skip, extra):
+    """numMPs > 1 (3d/mat_mult.h:183-202): row i * num_paths + use_path - 1 is
+    the walk of the pair at measurement position i, so the multi-position
+    operator is the per-position single-position operators stacked, row for
+    row (walks and products bit for bit); rows no pair maps to are 0."""
+    K, grid, P = 24, 32, 3
+    geo = synth.multi_mp_ring_geometry(K, grid, P, skip_every=skip, extra_rows=extra, seed=5)
+    op = O.Op.geometry(geo)
+    ptr, lin = op.walk()
+    x = synth.disk_phantom(grid, seed=2)
+    y = op.gemv(x)
+    npaths = geo.num_paths
+    for i in range(P):
+        sl = slice(i * K, (i + 1) * K)
+        one = synth.GeometryArrays(1, geo.use_path, geo.xe[sl], geo.ye[sl], geo.ze[sl],
+                                   geo.xr[sl], geo.yr[sl], geo.zr[sl], geo.rv, npaths)
+        op1 = O.Op.geometry(one)
+        p1, l1 = op1.walk()
+        base = i * npaths
+        for r in range(npaths):
+            np.testing.assert_array_equal(lin[ptr[base + r]:ptr[base + r + 1]], l1[p1[r]:p1[r + 1]])
+        np.testing.assert_array_equal(y[base:base + npaths], op1.gemv(x))
+        used = np.zeros(npaths, bool)
+        used[geo.use_path[geo.use_path > 0] - 1] = True
+        assert np.count_nonzero(~used) == extra and np.all(y[base:base + npaths][~used] == 0)
+    # the transpose gathers the same rays
+    yb = synth.add_noise(y, 0.1, 1)
+    At = op.gemv(yb, trans=True)
+    ref = sum(O.Op.geometry(synth.GeometryArrays(
+        1, geo.use_path, geo.xe[i * K:(i + 1) * K], geo.ye[i * K:(i + 1) * K], geo.ze[i * K:(i + 1) * K],
+        geo.xr[i * K:(i + 1) * K], geo.yr[i * K:(i + 1) * K], geo.zr[i * K:(i + 1) * K], geo.rv,
+        npaths)).gemv(yb[i * npaths:(i + 1) * npaths], trans=True).astype(np.float64) for i in range(P))
+    np.testing.assert_allclose(At, ref, rtol=1e-6, atol=1e-5)

@@ -85,6 +85,13 @@ def lib():
                                       C.POINTER(_VP)]),
         "tv3_unrolled_backward": (_I, [_VP, _VP, _VP, _VP, C.POINTER(C.c_double)]),
         "tv3_tape_destroy": (None, [_VP]),
+        "tv3_geometry_save": (_I, [_VP, C.c_char_p, _VP]),
+        "tv3_geometry_load": (_I, [_VP, C.c_char_p, _VP, C.POINTER(_VP)]),
+        "tv3_sparse_save": (_I, [_VP, C.c_char_p, _I, _I, _I] + [_VP] * 6),
+        "tv3_sparse_load": (_I, [_VP, C.c_char_p] + [C.POINTER(_I)] * 3
+                            + [C.POINTER(_VP)] * 7),
+        "tv3_storage_free": (None, [_VP]),
+        "tv3_io_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -19,7 +19,7 @@ __all__ = [
     "gaussian_A", "ellipse_phantom", "ring_transducers", "ring_geometry",
     "disk_phantom", "hemisphere_transducers", "hemisphere_geometry",
     "ellipsoid_phantom", "add_noise", "GeometryArrays", "csr_from_walk",
-    "batched_dense_sizes",
+    "batched_dense_sizes", "multi_mp_ring_geometry",
 ]
 
 
@@ -112,6 +112,35 @@ def ring_geometry(K=256, grid=256, scale=1.0):
     z = np.zeros(K, np.int32)
     up, npaths = _pairs_use_path(K)
     return GeometryArrays(1, up, x, y, z, x, y, z, (grid, grid, 1), npaths, scale)
+
+
+def multi_mp_ring_geometry(K, grid, n_mps, skip_every=0, extra_rows=0, seed=0, scale=1.0):
+    """Several measurement positions (numMPs > 1; 3d/mat_mult.h:183-202): at
+    position i the ring of K transducers is rotated by 2 pi i / (K n_mps).
+    use_path (shared by all positions) gives the pairs i < j a seeded random
+    output order, 1-based, and leaves out every skip_every-th pair (0: none);
+    row i * num_paths + use_path - 1 belongs to position i.  extra_rows rows
+    per position get no pair (their measurements are 0)."""
+    c = (grid - 1) / 2.0
+    xs, ys = [], []
+    for i in range(n_mps):
+        t = 2 * np.pi * np.arange(K) / K + 2 * np.pi * i / (K * n_mps)
+        xs.append(np.rint(c + c * np.cos(t)))
+        ys.append(np.rint(c + c * np.sin(t)))
+    x = np.concatenate(xs).astype(np.int32)
+    y = np.concatenate(ys).astype(np.int32)
+    z = np.zeros_like(x)
+    iu = np.triu_indices(K, 1)
+    keep = np.ones(iu[0].size, bool)
+    if skip_every:
+        keep[::skip_every] = False
+    kept = int(keep.sum())
+    num_paths = kept + int(extra_rows)
+    rank = np.random.default_rng(seed).permutation(num_paths)[:kept].astype(np.int32) + 1
+    up = np.zeros((K, K), np.int32)
+    up[iu[0][keep], iu[1][keep]] = rank
+    return GeometryArrays(n_mps, up.ravel(), x, y, z, x, y, z, (grid, grid, 1),
+                          num_paths, scale)
 
 
 def disk_phantom(grid=256, radius=None, seed=0):
