@@ -21,7 +21,7 @@
 namespace tv3 {
 
 constexpr int kBlock = 256;       // threads per block of every vector kernel
-constexpr int kNblkMax = 1024;    // max blocks per group of a reduction kernel
+constexpr int kNblkMax = 512;     // max blocks per group of a reduction kernel
 constexpr int kNslot = 24;        // partial-sum slots per instance
 constexpr int kSlotM = 12;        // first slot written by m-vector kernels
 

@@ -103,19 +103,22 @@ __device__ __forceinline__ int uni(int v) {
 // ---------------------------------------------------------------------------
 // get_tau (2d:425-453)
 // ---------------------------------------------------------------------------
-template <int W>
+template <int W, bool D3>
 __global__ __launch_bounds__(kBlock) void k_tau_first(KA K, Arrs a) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   {
     NLOOP_BEGIN(W)
     const Vox w(v, gd.q, gd.p, gd.r);
+    const bool z3 = D3 && gd.r > 1;
     const float d0 = a.d[o];
-    const float dx = AT(a.d, w.xn()) - d0;
-    const float dy = AT(a.d, w.yn()) - d0;
+    const float dxn = AT(a.d, w.xn()), dyn = AT(a.d, w.yn());
+    const float dzn = z3 ? AT(a.d, w.zn()) : 0.0f;
+    const float dx = dxn - d0;
+    const float dy = dyn - d0;
     acc[0] += (double)dx * (double)dx;
     acc[1] += (double)dy * (double)dy;
-    if (gd.r > 1) {
-      const float dz = AT(a.d, w.zn()) - d0;
+    if (z3) {
+      const float dz = dzn - d0;
       acc[2] += (double)dz * (double)dz;
     }
     acc[3] += (double)d0 * (double)d0;
@@ -259,25 +262,25 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
 // and the norms of the A^T-skip test (2d:313-316)
 // poly != 0: also the quadratic coefficients of lam3(alpha), lam5(alpha)
 // with Au(alpha) = Aup + alpha (Au - Aup) (slots kSlotM+4 .. +8).
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a, int skip,
-                                                  int poly) {
+template <int W, bool SKIP, bool POLY>
+__global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a) {
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     MLOOP_BEGIN(W)
     const float au = a.Au[o];
     const float bb = a.b[o];
-    const float aub = au - bb;
     const float dl = a.delta[o];
+    const float la = SKIP ? a.lastAu[o] : 0.0f;
+    const float aup = POLY ? a.Aup[o] : 0.0f;
+    const float aub = au - bb;
     acc[0] += (double)aub * (double)aub;
     acc[1] += (double)dl * (double)aub;
-    if (skip) {
-      const float da = au - a.lastAu[o];
+    if (SKIP) {
+      const float da = au - la;
       acc[2] += (double)da * (double)da;
       acc[3] += (double)au * (double)au;
     }
-    if (poly) {
-      const float aup = a.Aup[o];
+    if (POLY) {
       const double ap = (double)aup - (double)bb, da = (double)(au - aup);
       acc[4] += ap * ap;
       acc[5] += ap * da;
@@ -1441,6 +1444,13 @@ using TuneKey = std::tuple<int, int, int, int, int, long long, int>;
 std::mutex g_tune_mu;
 std::map<TuneKey, DenseTune> g_tune_cache;
 
+// blocks of a vector pass over the whole batch (TV3_VEC_BLOCKS overrides)
+long long vec_blocks() {
+  const char *e = std::getenv("TV3_VEC_BLOCKS");
+  const long long v = e ? std::atoll(e) : 0;
+  return v > 0 ? v : 2048;
+}
+
 bool autotune_enabled() {
   const char *e = std::getenv("TV3_AUTOTUNE");
   return !(e && e[0] == '0');
@@ -1463,8 +1473,15 @@ class Engine {
     O_.ls_exact = o.ls_exact;
     graph_ = o.graph != 0;
     submat_mb_ = o.submat_mb > 0 ? o.submat_mb : 5;
-    nblk_n_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
-    nblk_m_ = (int)std::min<long long>(kNblkMax, std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
+    // blocks per group of the vector passes: at most kNblkMax (512 measured
+    // best on the 256 ring at B = 128 and 256: 1024 gave 503 / 559 recons/s,
+    // 512 gave 511 / 579), and about vec_blocks() over the whole batch, at
+    // least 128 per group
+    const long long per = std::max<long long>(128, vec_blocks() / std::max(bt.G, 1));
+    nblk_n_ = (int)std::min<long long>(std::min<long long>(kNblkMax, per),
+                                       std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
+    nblk_m_ = (int)std::min<long long>(std::min<long long>(kNblkMax, per),
+                                       std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
   }
 
   // dense W = 1 scratch: split-K partials of the plain A^T product or slice
@@ -1815,8 +1832,14 @@ class Engine {
   // get_g after U and Ux.. are in place: A U, A^T skip state, A^T, lam3/5
   void get_g_products(uint32_t req, uint32_t forbid, int force) {
     fwd(a_.U, a_.Au, req);
-    hipLaunchKernelGGL(k_mpost<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_, O_.skip,
-                       (!force && !O_.ls_exact) ? 1 : 0);
+    {
+      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_); };
+      const bool poly = !force && !O_.ls_exact;
+      if (O_.skip)
+        poly ? go(k_mpost<W, true, true>) : go(k_mpost<W, true, false>);
+      else
+        poly ? go(k_mpost<W, false, true>) : go(k_mpost<W, false, false>);
+    }
     if (O_.skip) {
       hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(req, forbid), force);
       hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
@@ -1854,7 +1877,10 @@ class Engine {
 
   // tau of the first inner iteration / fallback: D(d), |d|^2, A d
   void tau_first(uint32_t req, uint32_t forbid) {
-    hipLaunchKernelGGL(k_tau_first<W>, gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
+    if (bt_.z3)
+      hipLaunchKernelGGL((k_tau_first<W, true>), gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
+    else
+      hipLaunchKernelGGL((k_tau_first<W, false>), gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
     fwd(a_.d, a_.tmpm, req);
     hipLaunchKernelGGL(k_msq<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_.tmpm);
   }
