@@ -99,53 +99,91 @@ int hip_check(tv3_ctx *ctx, hipError_t e, const char *what) {
 // forward product: y[row] = sum_k A[row, k] x[k]
 // ---------------------------------------------------------------------------
 
-// W = 1: sixteen lanes per row (four rows per wavefront); lane k of a row
-// takes entries k, k+16, ...; the sixteen lane sums are combined by a fixed
-// butterfly.
-constexpr int kSub = 16;
+// W = 1: L lanes per row (64 / L rows per wavefront); lane k of a row takes
+// entries k, k+L, ... in order; the L lane sums are combined by a fixed
+// butterfly.  Geometry and CSR rows issue kW1Unroll index and gather loads
+// before they add.  kSubF / kSubT: lanes per row of A x / A^T y (8 and 8
+// measured best on the 128^3 hemisphere and the 256 ring's plain path;
+// four loads ahead took A^T y there from 0.250 to 0.231 ms).
+constexpr int kSubF = 8;
+constexpr int kSubT = 8;
+// geometry columns of A^T y in the plan's tiled voxel order (t_*): 0.224 ->
+// 0.196 ms on the 128^3 hemisphere; the tiled ray order made A x slower
+// there (0.091 -> 0.098 ms), so A x keeps the measurement order
+constexpr bool kW1TiledF = false;
+constexpr bool kW1TiledT = true;
+constexpr int kW1Unroll = 4;
 
-__device__ __forceinline__ double sub_sum(double v) {
+template <int L>
+__device__ __forceinline__ double lane_sum(double v) {
 #pragma unroll
-  for (int off = kSub / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
+template <int L>
 __global__ __launch_bounds__(kBlock) void k_fwd_w1(const GroupDesc *G,
                                                    const float *x, float *y,
                                                    const InstScal *S,
                                                    uint32_t mask) {
   const GroupDesc &gd = G[blockIdx.y];
-  const int sl = threadIdx.x & (kSub - 1);
-  const int row = blockIdx.x * (kBlock / kSub) + (threadIdx.x / kSub);
+  const int sl = threadIdx.x & (L - 1);
+  const int row = blockIdx.x * (kBlock / L) + (threadIdx.x / L);
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const PlanDev &P = gd.plan;
-  // row of A and layer of this output row (one layer: the row itself)
-  const int ar = row % P.m;
+  // row of A and layer of this output row (one layer: the row itself; a
+  // geometry row slot walks the plan's tiled ray order)
+  int ar = row % P.m;
+  if (kW1TiledF && P.kind == kGeometry && P.ray_order && row < gd.m) ar = P.ray_order[row];
   const float *xg = x + gd.off_n + (long long)(row / P.m) * P.n;
   double acc = 0.0;
   if (row < gd.m) {
+    constexpr int U = kW1Unroll;
     if (P.kind == kGeometry) {
       const RayDesc rd = P.rays[ar];
       const float sc = P.scale;
-      for (int j = sl; j <= rd.L; j += kSub) acc += (double)(xg[ray_lin(rd, j)] * sc);
+      for (int j = sl; j <= rd.L; j += L * U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int jj = j + u * L;
+          v[u] = jj <= rd.L ? xg[ray_lin(rd, jj)] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j + u * L <= rd.L) acc += (double)(v[u] * sc);
+      }
     } else if (P.kind == kSparse) {
       const int k0 = P.r_ptr[ar], k1 = P.r_ptr[ar + 1];
-      for (int k = k0 + sl; k < k1; k += kSub)
-        acc += (double)P.r_val[k] * (double)xg[P.r_ind[k]];
+      for (int k = k0 + sl; k < k1; k += L * U) {
+        int id[U];
+        float w[U], v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kk = k + u * L;
+          id[u] = kk < k1 ? P.r_ind[kk] : -1;
+          w[u] = kk < k1 ? P.r_val[kk] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? xg[id[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (id[u] >= 0) acc += (double)w[u] * (double)v[u];
+      }
     } else {
       const long long base = (long long)ar * P.n;
       if (P.dtype == TV3_F32) {
         const float *A = (const float *)P.A + base;
-        for (int j = sl; j < P.n; j += kSub) acc += (double)A[j] * (double)xg[j];
+        for (int j = sl; j < P.n; j += L) acc += (double)A[j] * (double)xg[j];
       } else {
         const __hip_bfloat16 *A = (const __hip_bfloat16 *)P.A + base;
-        for (int j = sl; j < P.n; j += kSub)
+        for (int j = sl; j < P.n; j += L)
           acc += (double)__bfloat162float(A[j]) * (double)xg[j];
       }
     }
   }
-  acc = sub_sum(acc);
-  if (sl == 0 && row < gd.m) y[gd.off_m + row] = (float)acc;
+  acc = lane_sum<L>(acc);
+  if (sl == 0 && row < gd.m) y[gd.off_m + (long long)(row / P.m) * P.m + ar] = (float)acc;
 }
 
 // W = 64: one wavefront per row, one lane per instance.  The walk, the
@@ -486,34 +524,50 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
 // transposed product: y[col] = sum_k A[k, col] x[k] (- sub[col])
 // ---------------------------------------------------------------------------
 
+template <int L>
 __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
                                                    const float *x, float *y,
                                                    const InstScal *S,
                                                    uint32_t mask,
                                                    const float *sub) {
   const GroupDesc &gd = G[blockIdx.y];
-  const int sl = threadIdx.x & (kSub - 1);
-  const int col = blockIdx.x * (kBlock / kSub) + (threadIdx.x / kSub);
+  const int sl = threadIdx.x & (L - 1);
+  const int col = blockIdx.x * (kBlock / L) + (threadIdx.x / L);
   if (mask && (S[blockIdx.y].flags & mask) != mask) return;
   const PlanDev &P = gd.plan;
-  // column of A and layer of this output entry (one layer: the column itself)
-  const int ac = col % P.n;
+  // column of A and layer of this output entry (one layer: the column
+  // itself; a geometry column slot walks the plan's tiled voxel order)
+  const bool geo = P.kind == kGeometry;
+  const bool tiled = kW1TiledT && geo && P.t_col;
+  int ac = col % P.n;
   const float *xg = x + gd.off_m + (long long)(col / P.n) * P.m;
   double acc = 0.0;
   if (col < gd.n) {
-    if (P.kind == kGeometry) {
-      const int k0 = P.g_cptr[ac], k1 = P.g_cptr[ac + 1];
-      const float sc = P.scale;
-      for (int k = k0 + sl; k < k1; k += kSub) acc += (double)(xg[P.g_cray[k]] * sc);
-    } else {
-      const int k0 = P.c_ptr[ac], k1 = P.c_ptr[ac + 1];
-      for (int k = k0 + sl; k < k1; k += kSub)
-        acc += (double)P.c_val[k] * (double)xg[P.c_ind[k]];
+    constexpr int U = kW1Unroll;
+    const int *ptr = tiled ? P.t_ptr : geo ? P.g_cptr : P.c_ptr;
+    const int *ind = tiled ? P.t_ray : geo ? P.g_cray : P.c_ind;
+    const int k0 = ptr[ac], k1 = ptr[ac + 1];
+    if (tiled) ac = P.t_col[col];
+    const float sc = P.scale;
+    for (int k = k0 + sl; k < k1; k += L * U) {
+      int id[U];
+      float w[U], v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = k + u * L;
+        id[u] = kk < k1 ? ind[kk] : -1;
+        w[u] = (!geo && kk < k1) ? P.c_val[kk] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? xg[id[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (id[u] >= 0) acc += geo ? (double)(v[u] * sc) : (double)w[u] * (double)v[u];
     }
   }
-  acc = sub_sum(acc);
+  acc = lane_sum<L>(acc);
   if (sl == 0 && col < gd.n) {
-    const long long o = gd.off_n + col;
+    const long long o = gd.off_n + (long long)(col / P.n) * P.n + ac;
     float v = (float)acc;
     if (sub) v = v - sub[o];
     y[o] = v;
@@ -1110,8 +1164,8 @@ void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
 #undef TV3_DFWD
     return;
   }
-  dim3 grid((max_m + kBlock / kSub - 1) / (kBlock / kSub), G);
-  hipLaunchKernelGGL(k_fwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+  dim3 grid((max_m + kBlock / kSubF - 1) / (kBlock / kSubF), G);
+  hipLaunchKernelGGL(k_fwd_w1<kSubF>, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask);
 }
 
 template <>
@@ -1191,8 +1245,8 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                        (const double *)scratch, y, S, mask, sub, sstride);
     return;
   }
-  dim3 grid((max_n + kBlock / kSub - 1) / (kBlock / kSub), G);
-  hipLaunchKernelGGL(k_bwd_w1, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask,
+  dim3 grid((max_n + kBlock / kSubT - 1) / (kBlock / kSubT), G);
+  hipLaunchKernelGGL(k_bwd_w1<kSubT>, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask,
                      sub);
 }
 
