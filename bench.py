@@ -18,6 +18,12 @@ Workloads (--workload):
   hemisphere   1024 Fibonacci transducers on the lower hemisphere of a 128^3
                volume, 523,776 rays, 3-D anisotropic TV, maxit 100, one solve.
   dense        64x64 image, m = 1229, dense FP32 A, isotropic TV, maxit 300.
+  dense-layers the reference's 2-D multi-layer mode: 8 layers of 64x64 stacked
+               along y sharing one A (m = 1229 per layer), isotropic TV,
+               maxit 300; value = layered solves/s (images/s = 8x).
+  ring-f64     the 256x256 ring with the double/double solver (FP64 vectors
+               and scalars, tv3_solve_f64), one reconstruction at a time,
+               maxit 200; CPU baseline: the oracle's double/double solve.
   diff         gradients of |U_K - x|^2 in b and mu through K = 20 unrolled
                iterations (checkpoints every 5), 128-transducer ring on 128x128.
 
@@ -384,6 +390,33 @@ def run_single(args, ctx, workload):
         kern = {"A": "k_fwd_w1", "At": "k_bwd_w1"}
         op_fn = lambda: O.Op.geometry(geo)  # noqa: E731
         desc = f"3D USCT hemisphere {grid}^3, {plan.m} rays, maxit {maxit}"
+    elif workload == "ring-f64":
+        grid = args.grid if args.grid_set else 256
+        geo = synth.ring_geometry(256, grid)
+        plan = ctx.plan_geometry(geo)
+        x = synth.disk_phantom(grid, seed=0)
+        y = ctx.gemv(plan, x).cpu().numpy()
+        b = synth.add_noise(y, 0.005, seed=0).astype(np.float64)
+        dims = (grid, grid, 1)
+        maxit = args.maxit if args.maxit_set else 200
+        opts = tval3_amd.default_options(maxit=maxit, graph=args.graph)
+        kern = {"A": "k_fwd_w1d", "At": "k_bwd_w1d"}
+        op_fn = lambda: O.Op.geometry(geo)  # noqa: E731
+        desc = f"2D USCT ring 256 transducers on {grid}x{grid}, FP64 vectors, maxit {maxit}"
+    elif workload == "dense-layers":
+        q = p = 64
+        m, L = 1229, 8
+        A = synth.gaussian_A(m, q * p, 0)
+        x = np.concatenate([synth.ellipse_phantom(p, q, 8, l) for l in range(L)])
+        b = np.concatenate([synth.add_noise(A @ x[l * q * p:(l + 1) * q * p], 0.01, l)
+                            for l in range(L)])
+        plan = ctx.plan_dense(A)
+        dims = (q, L * p, 1)
+        maxit = args.maxit if args.maxit_set else 300
+        opts = tval3_amd.default_options(maxit=maxit, tv_iso=1, graph=args.graph)
+        kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}
+        op_fn = lambda: O.Op.dense(A)  # noqa: E731
+        desc = f"2D dense multi-layer: {L} layers of 64x64, m={m} per layer, isotropic TV, maxit {maxit}"
     else:
         q = p = 64
         m = 1229
@@ -397,32 +430,36 @@ def run_single(args, ctx, workload):
         kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_bs"}
         op_fn = lambda: O.Op.dense(A)  # noqa: E731
         desc = f"2D dense 64x64, m={m}, isotropic TV, maxit {maxit}"
-    bt = ctx.tensor(b)
+    f64 = workload == "ring-f64"
+    bt = ctx.tensor(b, torch.float64 if f64 else torch.float32)
     for _ in range(args.warmup):
-        ctx.solve(plan, bt, dims, opts)
+        ctx.solve(plan, bt, dims, opts, f64=f64)
     torch.cuda.synchronize()
     ctx.profile(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        U, info, _ = ctx.solve(plan, bt, dims, opts, Ut=x)
+        U, info, _ = ctx.solve(plan, bt, dims, opts, Ut=x, f64=f64)
     torch.cuda.synchronize()
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
     ops, roof = roofline(prof, kern, workload)
-    if workload == "hemisphere":
+    if workload in ("hemisphere", "ring-f64"):
         roof["csr_equivalent"] = True
     else:
         roof["cache_resident"] = True  # 20 MB A stays in the 256 MB last-level cache
     cpu = None
     if not args.no_cpu:
         op = op_fn()
-        runs, el, thr = cpu_time(lambda: O.solve(op, b, dims, opts), args.cpu_seconds, 5)
+        solve = O.solve_f64 if f64 else O.solve
+        runs, el, thr = cpu_time(lambda: solve(op, b, dims, opts), args.cpu_seconds, 5)
         cpu = {"value": runs / el, "unit": "recons/s", "cores": thr, "kind": "port",
                "sample": f"{runs} full reconstructions ({el:.1f} s, C++/OpenMP oracle)"}
+    extra = {"images_per_s": L * args.steps / t_steps} if workload == "dense-layers" else {}
     return {
+        **extra,
         "value": args.steps / t_steps, "unit": "recons/s", "steps": args.steps,
-        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "f32",
+        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "f64" if f64 else "f32",
         "data": "synthetic (seeded phantom, noise)",
         "config": {"workload": desc, "batch": 1, "maxit": maxit, "nnz": plan.nnz},
         "iters": info.total_iters, "rel_error_vs_truth": info.rel_error,
@@ -436,7 +473,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ring",
-                    choices=["ring", "dense-batch", "hemisphere", "dense", "diff"])
+                    choices=["ring", "dense-batch", "hemisphere", "dense", "dense-layers", "ring-f64",
+                             "diff"])
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)

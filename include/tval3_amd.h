@@ -19,6 +19,11 @@
  *                                   (3d/mat_mult.h:148-595; the 2-D copy at
  *                                    2d/mat_mult.h:107-194 is commented out)
  *   tv3_gemv                     <- gemv(CBLAS_TRANSPOSE, const T_A&, x, y)
+ *   tv3_solve_f64,               <- the double/double overloads: mat<double>
+ *   tv3_solve_batch_f64,            U with mat<double>, sparse_mat<double> or
+ *   tv3_gemv_f64,                   geometry A (3d/tval3_cpu_3d.cpp:47, 88,
+ *   tv3_plan_sparse_f64,            124; 2d/tval3_cpu_2d.cpp:26-29) and gemv
+ *   tv3_plan_dense(TV3_F64)         on double vectors (3d/mat_mult.h:17-139)
  *   tv3_D / tv3_Dt               <- D / Dt (2d/tval3_cpu_2d.cpp:268-293,
  *                                   3d/tval3_cpu_3d.cpp:365-425)
  *   tv3_shrink                   <- shrinkage (2d/tval3_cpu_2d.cpp:588-611,
@@ -55,6 +60,7 @@ extern "C" {
 /* ---- element types of a dense A ---------------------------------------- */
 #define TV3_F32 0
 #define TV3_BF16 1
+#define TV3_F64 2 /* mat<double>: for tv3_solve_f64 / tv3_gemv_f64 only */
 
 /* ---- gemv transpose flags (CBLAS_TRANSPOSE) ----------------------------- */
 #define TV3_NOTRANS 0
@@ -190,6 +196,13 @@ int tv3_plan_sparse(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
 /* On-the-fly Bresenham projector.  Builds per-ray walk descriptors and the
  * deterministic voxel-major transpose once. */
 int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out);
+/* sparse_mat<double>: as tv3_plan_sparse with double values; for the _f64
+ * entry points only (the float ones reject it, as they reject TV3_F64 dense
+ * plans).  Geometry plans serve both. */
+int tv3_plan_sparse_f64(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                        const int *csr_ind, const double *csr_val,
+                        const int *csc_ptr, const int *csc_ind,
+                        const double *csc_val, tv3_plan **out);
 /* Frees the plan's own device arrays; valid after its context is destroyed. */
 void tv3_plan_destroy(tv3_plan *plan);
 /* m = rows (measurements), n = columns (pixels), nnz = stored or walked
@@ -272,6 +285,23 @@ int tv3_solve_batch(tv3_ctx *ctx, int B, const tv3_plan *const *A,
                     const tv3_options *opts, const int *dims,
                     const float *const *b, float *const *U,
                     const float *const *Ut, tv3_info *infos);
+/* The same with FP64 vectors and scalars (T_comp = T_scal = double;
+ * opts->scal_double is ignored): A is a TV3_F64 dense, a double sparse or a
+ * geometry plan.  Every product, dot and norm is an FP64 sum, in a different
+ * order than the reference's loops, so results agree with the reference's
+ * double solve to FP64 rounding.  Instances run one per group (no 64-lane
+ * interleaving); multi-layer mode as in tv3_solve. */
+int tv3_solve_f64(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+                  int dim_x, int dim_y, int dim_z, const double *b, double *U,
+                  const double *Ut, tv3_info *info, tv3_trace_rec *trace,
+                  int trace_cap);
+int tv3_solve_batch_f64(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                        const tv3_options *opts, const int *dims,
+                        const double *const *b, double *const *U,
+                        const double *const *Ut, tv3_info *infos);
+/* y = A x or A^T x on double vectors (plans as for tv3_solve_f64). */
+int tv3_gemv_f64(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
+                 double *y);
 
 /* ---- differentiation: K unrolled inner iterations and their adjoint ------- */
 /* No single reference counterpart: the map is K inner iterations of the first

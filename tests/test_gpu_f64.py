@@ -1,0 +1,214 @@
+"""The double/double path (tv3_solve_f64, tv3_gemv_f64): FP64 vectors and
+scalars, the reference's mat<double> / sparse_mat<double> / geometry
+overloads with T_comp = double.
+
+Bar: products within 1e-13 relative of an FP64 evaluation (numpy's for
+double-valued matrices, the oracle's for the projector).  For whole solves
+both sides sum in FP64 but in different orders, and the TVAL3 iteration
+amplifies rounding differences about tenfold every five inner iterations
+(the oracle's own result moves by up to 1e-5 on the 64x64 ring when b is
+perturbed by one part in 1e16).  So a solve is held to:
+  * the same trajectory (iteration, outer and backtracking counts);
+  * the first five iterations' f and tau within 1e-12 relative (there the
+    two runs differ by rounding only, so a wrong term shows at once);
+  * the final U within 10x the oracle's own spread under 1e-16 / 1e-15
+    relative perturbations of b (floor 1e-12).
+"""
+import numpy as np
+import pytest
+
+import tval3_amd
+from tval3_amd import synth
+import tv3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 1e-13
+TOL_EARLY = 1e-12
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _spread(op, b, dims, opts, Uo):
+    """The oracle's own sensitivity: max relative change of U under two
+    rounding-level perturbations of b."""
+    rng = np.random.default_rng(1234)
+    s = 0.0
+    for eps in (1e-16, 1e-15):
+        bp = b * (1 + eps * rng.standard_normal(b.size))
+        Up, _, _ = O.solve_f64(op, bp, dims, opts)
+        s = max(s, _rel(Up, Uo))
+    return s
+
+
+def _compare(ctx, plan, op, b, dims, opts, Ut=None, cap=400):
+    U, info, tr = ctx.solve(plan, b, dims, opts, Ut=Ut, trace_cap=cap, f64=True)
+    assert U.dtype == ctx.torch.float64
+    Ug = U.cpu().numpy()
+    Uo, info_o, tro = O.solve_f64(op, b, dims, opts, Ut=Ut, trace_cap=cap)
+    rel = _rel(Ug, Uo)
+    spread = _spread(op, b, dims, opts, Uo)
+    diff = [t.iter for t, s in zip(tr, tro)
+            if (t.backtracks, t.fallback) != (s.backtracks, s.fallback)]
+    early = max([abs(t.f - s.f) / max(abs(s.f), 1e-300) for t, s in zip(tr[:5], tro[:5])] +
+                [abs(t.tau - s.tau) / max(abs(s.tau), 1e-300) for t, s in zip(tr[:5], tro[:5])])
+    msg = (f"rel={rel:.3e} oracle spread={spread:.3e} early={early:.2e} "
+           f"iters gpu/oracle={info.total_iters}/{info_o.total_iters} "
+           f"outer={info.outer_iters}/{info_o.outer_iters} first traj diff at {diff[:3]}")
+    print(msg)
+    assert np.all(np.isfinite(Ug)), msg
+    assert info.total_iters == info_o.total_iters, msg
+    assert info.outer_iters == info_o.outer_iters, msg
+    assert not diff, msg
+    assert len(tr) >= min(5, info_o.total_iters), msg
+    assert early <= TOL_EARLY, msg
+    assert rel <= max(10 * spread, 1e-12), msg
+    return Ug, info
+
+
+def _dense_case(q, p, frac, seed):
+    n = q * p
+    m = int(round(frac * n))
+    A = synth.gaussian_A(m, n, seed)
+    x = synth.ellipse_phantom(p, q, 8, seed)
+    b = synth.add_noise(A @ x, 0.01, seed).astype(np.float64)
+    return A, x, b
+
+
+def test_gemv_f64_double_valued(ctx):
+    """Dense and sparse matrices with entries that are not floats: y = A x
+    and A^T y against numpy's FP64 products (the oracle holds float entries
+    only, so these two are pinned by numpy)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    for m, n in ((37, 53), (300, 1000), (1, 7)):
+        A = rng.standard_normal((m, n))
+        x = rng.standard_normal(n)
+        y = rng.standard_normal(m)
+        plan = ctx.plan_dense(A, f64=True)
+        assert _rel(ctx.gemv(plan, x, f64=True).cpu().numpy(), A @ x) <= TOL_OP
+        assert _rel(ctx.gemv(plan, y, trans=True, f64=True).cpu().numpy(), A.T @ y) <= TOL_OP
+        As = A * (rng.random((m, n)) < 0.2)
+        S, T = sp.csr_matrix(As), sp.csc_matrix(As)
+        splan = ctx.plan_sparse(m, n, S.indptr, S.indices, S.data, T.indptr, T.indices,
+                                T.data, f64=True)
+        assert _rel(ctx.gemv(splan, x, f64=True).cpu().numpy(), As @ x) <= TOL_OP
+        assert _rel(ctx.gemv(splan, y, trans=True, f64=True).cpu().numpy(), As.T @ y) <= TOL_OP
+
+
+def test_gemv_f64_geometry(ctx):
+    """The projector on double vectors against the oracle's gemv<double>
+    (products x * (double)scale, FP64 sums): 2-D ring, numMPs > 1 with
+    skipped pairs, 3-D hemisphere."""
+    rng = np.random.default_rng(6)
+    for geo in (synth.ring_geometry(64, 64),
+                synth.multi_mp_ring_geometry(32, 40, 3, skip_every=5, extra_rows=9, seed=3),
+                synth.hemisphere_geometry(64, 16)):
+        op = O.Op.geometry(geo)
+        plan = ctx.plan_geometry(geo)
+        x = rng.standard_normal(geo.n)
+        y = rng.standard_normal(geo.m)
+        assert _rel(ctx.gemv(plan, x, f64=True).cpu().numpy(), op.gemv_f64(x)) <= TOL_OP
+        assert _rel(ctx.gemv(plan, y, trans=True, f64=True).cpu().numpy(),
+                    op.gemv_f64(y, trans=True)) <= TOL_OP
+
+
+@pytest.mark.parametrize("ls", [1, 0])
+@pytest.mark.parametrize("iso", [0, 1])
+def test_dense_solve_f64(ctx, iso, ls):
+    A, x, b = _dense_case(32, 32, 0.35, 11)
+    opts = tval3_amd.default_options(maxit=120, tv_iso=iso, ls_exact=ls)
+    _compare(ctx, ctx.plan_dense(A, f64=True), O.Op.dense(A), b, (32, 32, 1), opts, Ut=x)
+
+
+def test_dense_continuation_nonneg_f64(ctx):
+    A, x, b = _dense_case(24, 20, 0.5, 3)
+    opts = tval3_amd.default_options(maxit=150, mu0=4.0, beta0=2.0, nonneg=1,
+                                     tol_inn=1e-2, ls_exact=1)
+    _compare(ctx, ctx.plan_dense(A, f64=True), O.Op.dense(A), b, (24, 20, 1), opts, Ut=x)
+
+
+def test_dense_layers_f64(ctx):
+    """2-D multi-layer mode with double vectors."""
+    L, q, p = 3, 16, 12
+    N = q * p
+    A = synth.gaussian_A(int(0.4 * N), N, 21)
+    x = np.concatenate([synth.ellipse_phantom(p, q, 4, 21 + l).ravel() for l in range(L)])
+    b = np.concatenate([synth.add_noise(A @ x[l * N:(l + 1) * N], 0.01, 21 + l)
+                        for l in range(L)]).astype(np.float64)
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1)
+    _compare(ctx, ctx.plan_dense(A, f64=True), O.Op.dense(A), b, (q, L * p, 1), opts, Ut=x)
+
+
+@pytest.mark.parametrize("ls", [1, 0])
+def test_ring_geometry_solve_f64(ctx, ls):
+    geo = synth.ring_geometry(64, 64)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(64, seed=1)
+    b = synth.add_noise(op.gemv(x), 0.005, 1).astype(np.float64)
+    opts = tval3_amd.default_options(maxit=100, ls_exact=ls)
+    _compare(ctx, ctx.plan_geometry(geo), op, b, (64, 64, 1), opts, Ut=x)
+
+
+def test_ring_sparse_and_skip_f64(ctx):
+    """sparse_mat<double> from the ring's walk, and A^T skipping (the skip
+    policy) on the projector."""
+    geo = synth.ring_geometry(48, 48)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(48, seed=2)
+    b = synth.add_noise(op.gemv(x), 0.005, 2).astype(np.float64)
+    ptr, lin = op.walk()
+    csr = synth.csr_from_walk(ptr, lin, geo.n)
+    sop = O.Op.sparse(geo.m, geo.n, *csr)
+    opts = tval3_amd.default_options(maxit=80, ls_exact=1)
+    _compare(ctx, ctx.plan_sparse(geo.m, geo.n, *csr, f64=True), sop, b, (48, 48, 1), opts, Ut=x)
+    opts = tval3_amd.default_options(maxit=80, skipMulRatio=0.5, numInitIterations=3,
+                                     maxRelativeChange=0.05, ls_exact=1)
+    _compare(ctx, ctx.plan_geometry(geo), op, b, (48, 48, 1), opts, Ut=x)
+
+
+def test_hemisphere_3d_solve_f64(ctx):
+    grid = 16
+    geo = synth.hemisphere_geometry(64, grid)
+    op = O.Op.geometry(geo)
+    x = synth.ellipsoid_phantom(grid, 6, seed=4)
+    b = synth.add_noise(op.gemv(x), 0.005, 4).astype(np.float64)
+    opts = tval3_amd.default_options(maxit=40, ls_exact=1)
+    _compare(ctx, ctx.plan_geometry(geo), op, b, (grid, grid, grid), opts, Ut=x)
+
+
+def test_batch_f64(ctx):
+    """Three instances of different operators and sizes in one batch (one
+    instance per group) against separate oracle solves."""
+    A1, x1, b1 = _dense_case(16, 16, 0.4, 7)
+    A2, x2, b2 = _dense_case(20, 12, 0.5, 8)
+    p1, p2 = ctx.plan_dense(A1, f64=True), ctx.plan_dense(A2, f64=True)
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1)
+    dims = [(16, 16, 1), (20, 12, 1), (16, 16, 1)]
+    bs = [b1, b2, b1 * 0.5]
+    Us, infos = ctx.solve_batch([p1, p2, p1], bs, dims, opts, f64=True)
+    for U, info, A, b, d in zip(Us, infos, (A1, A2, A1), bs, dims):
+        op = O.Op.dense(A)
+        Uo, info_o, _ = O.solve_f64(op, b, d, opts)
+        assert _rel(U.cpu().numpy(), Uo) <= max(10 * _spread(op, b, d, opts, Uo), 1e-12)
+        assert info.total_iters == info_o.total_iters
+
+
+def test_f64_plan_errors(ctx):
+    """Float entry points reject double-valued plans and the double ones
+    float-valued plans; the projector serves both."""
+    A, x, b = _dense_case(16, 16, 0.4, 9)
+    opts = tval3_amd.default_options(maxit=5)
+    p32, p64 = ctx.plan_dense(A), ctx.plan_dense(A, f64=True)
+    with pytest.raises(tval3_amd.Tval3Error, match="tv3_solve_f64"):
+        ctx.solve(p64, b, (16, 16, 1), opts)
+    with pytest.raises(tval3_amd.Tval3Error, match="tv3_solve_f64"):
+        ctx.solve(p32, b, (16, 16, 1), opts, f64=True)
+    with pytest.raises(tval3_amd.Tval3Error, match="tv3_gemv_f64"):
+        ctx.gemv(p64, x.ravel())
+    with pytest.raises(tval3_amd.Tval3Error, match="tv3_gemv_f64"):
+        ctx.gemv(p32, x.ravel(), f64=True)

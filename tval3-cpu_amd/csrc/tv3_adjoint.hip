@@ -355,6 +355,8 @@ int tv3_unrolled_forward(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opt
   if (!ctx) return TV3_ERR_ARG;
   if (!A || !opts || !alpha || !first_step || !b || !U || !tape)
     return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  if (!plan_takes_f32(A->dev))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: the unrolled solver takes float-valued plans");
   if (dim_x <= 0 || dim_y <= 0 || (long long)dim_x * dim_y != A->dev.n)
     return set_err(ctx, TV3_ERR_ARG, "Argument error: dim_x * dim_y must equal the plan's columns");
   if (K < 1 || ckpt < 1 || ckpt > K)

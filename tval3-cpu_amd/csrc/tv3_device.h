@@ -49,11 +49,11 @@ __device__ __forceinline__ bool sel(uint32_t f, uint32_t req, uint32_t forbid) {
 // Only lanes whose instance passes the (req, forbid) flag test write, so a
 // masked kernel never overwrites the partials another kernel left for the
 // instances it skips.
-template <int W, int NS>
+template <int W, int NS, class SC>
 __device__ __forceinline__ void block_partials(const double (&v)[NS],
                                                const int (&slot)[NS],
                                                double *part, int inst0,
-                                               int blk, const InstScal *S,
+                                               int blk, const SC *S,
                                                uint32_t req, uint32_t forbid) {
   __shared__ double red[4][NS][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -119,9 +119,9 @@ struct Vox {
 // (the entry i - dim_x is the last row of the previous slice) before the
 // per-slice wrap pass; exact = true leaves that term out.
 template <class FX, class FY, class FZ>
-__device__ __forceinline__ float dt_at(const Vox &w, bool exact, FX VX, FY VY,
-                                       FZ VZ) {
-  float t = VX(w.xp()) - VX(w.v);
+__device__ __forceinline__ auto dt_at(const Vox &w, bool exact, FX VX, FY VY,
+                                      FZ VZ) -> decltype(VX(0)) {
+  decltype(VX(0)) t = VX(w.xp()) - VX(w.v);
   if (w.r == 1) {
     t = t + VY(w.yp());
     t = t - VY(w.v);
@@ -142,35 +142,33 @@ __device__ __forceinline__ float dt_at(const Vox &w, bool exact, FX VX, FY VY,
 
 // ---- shrinkage (2d/tval3_cpu_2d.cpp:588-611) ---------------------------------
 // Anisotropic soft-threshold of one direction: ubar = u - sigma/beta rounded
-// to float (T_mat), w = max(|ubar| - 1/beta, 0) in T_scal, W = sign+(ubar) w.
-template <class TS>
-__device__ __forceinline__ float shrink1(float u, float sg, TS beta, TS ib,
-                                         TS &w) {
-  const float ub = (float)((TS)u - (TS)sg / beta);
+// to the vector type (T_mat), w = max(|ubar| - 1/beta, 0) in T_scal,
+// W = sign+(ubar) w.
+template <class TS, class VC>
+__device__ __forceinline__ VC shrink1(VC u, VC sg, TS beta, TS ib, TS &w) {
+  const VC ub = (VC)((TS)u - (TS)sg / beta);
   TS t = fabs((TS)ub) - ib;
   t = (t >= (TS)0) ? t : (TS)0;
   w = t;
-  return (float)((ub >= 0.0f) ? t : -t);
+  return (VC)((ub >= (VC)0) ? t : -t);
 }
 
 // Isotropic group soft-threshold (no reference counterpart): ubar as above per
 // direction, r = |ubar|_2, w = max(r - 1/beta, 0), W_dir = w * ubar_dir / r.
-template <class TS>
-__device__ __forceinline__ TS shrink_iso(float ux, float uy, float uz, float sx,
-                                         float sy, float sz, bool z3, TS beta,
-                                         TS ib, float &Wx, float &Wy,
-                                         float &Wz) {
-  const float bx = (float)((TS)ux - (TS)sx / beta);
-  const float by = (float)((TS)uy - (TS)sy / beta);
-  const float bz = z3 ? (float)((TS)uz - (TS)sz / beta) : 0.0f;
+template <class TS, class VC>
+__device__ __forceinline__ TS shrink_iso(VC ux, VC uy, VC uz, VC sx, VC sy, VC sz, bool z3,
+                                         TS beta, TS ib, VC &Wx, VC &Wy, VC &Wz) {
+  const VC bx = (VC)((TS)ux - (TS)sx / beta);
+  const VC by = (VC)((TS)uy - (TS)sy / beta);
+  const VC bz = z3 ? (VC)((TS)uz - (TS)sz / beta) : (VC)0;
   TS ss = (TS)bx * (TS)bx + (TS)by * (TS)by;
   if (z3) ss = ss + (TS)bz * (TS)bz;
   const TS rr = sqrt(ss);
   TS w = rr - ib;
   w = (w >= (TS)0) ? w : (TS)0;
-  Wx = (rr > 0) ? (float)(w * (TS)bx / rr) : 0.0f;
-  Wy = (rr > 0) ? (float)(w * (TS)by / rr) : 0.0f;
-  Wz = (z3 && rr > 0) ? (float)(w * (TS)bz / rr) : 0.0f;
+  Wx = (rr > 0) ? (VC)(w * (TS)bx / rr) : (VC)0;
+  Wy = (rr > 0) ? (VC)(w * (TS)by / rr) : (VC)0;
+  Wz = (z3 && rr > 0) ? (VC)(w * (TS)bz / rr) : (VC)0;
   return w;
 }
 

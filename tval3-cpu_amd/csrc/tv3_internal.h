@@ -55,6 +55,8 @@ struct PlanDev {
   const float *r_val;
   const int *c_ptr, *c_ind;
   const float *c_val;
+  // double values of a sparse_mat<double> plan (then r_val / c_val are null)
+  const double *r_vald, *c_vald;
   // geometry
   const RayDesc *rays;
   const int *g_cptr;  // voxel-major transpose: rays through voxel v,
@@ -67,6 +69,16 @@ struct PlanDev {
   const int *t_ptr, *t_ray, *t_col;
   float scale;
 };
+
+// Vector types a plan serves: float-valued operators (f32 / bf16 dense, float
+// sparse) multiply float vectors, double-valued ones (f64 dense, double
+// sparse) double vectors; the geometry projector serves both.
+__host__ __device__ inline bool plan_takes_f32(const PlanDev &p) {
+  return p.kind == kDense ? p.dtype != TV3_F64 : p.kind == kSparse ? p.r_val != nullptr || p.nnz == 0 : true;
+}
+__host__ __device__ inline bool plan_takes_f64(const PlanDev &p) {
+  return p.kind == kDense ? p.dtype == TV3_F64 : p.kind == kSparse ? p.r_vald != nullptr || p.nnz == 0 : true;
+}
 
 // ---- batch description ------------------------------------------------------
 struct GroupDesc {
@@ -104,15 +116,30 @@ enum : uint32_t {
 
 // Per-instance scalars.  T_scal quantities are kept in double; for the
 // float instantiation every value is a float widened exactly.  f, C, lam4,
-// lam5 are T_comp (float) in the reference and are stored as float.
-struct InstScal {
+// lam5 are T_comp in the reference and are stored in the vector type VC
+// (float, or double for the FP64-vector solver) in a 32-byte base that
+// comes first, so both instantiations have the same size and every other
+// field (flags included) the same offset: the operator launches, which read
+// only `flags`, take either through InstScal.
+template <class VC>
+struct InstScalVC;
+template <>
+struct InstScalVC<float> {
+  float lam4, lam5, f, C;
+  float pad_vc[4];
+};
+template <>
+struct InstScalVC<double> {
+  double lam4, lam5, f, C;
+};
+template <class VC>
+struct InstScalT : InstScalVC<VC> {
   uint32_t flags;
   int count_outer, count_total, cnt;  // cnt: backtracking steps this iter
   int fallback;
   unsigned currInitIteration;
   double mu, beta, muDbeta, beta0;
   double lam1, lam2, lam3;
-  float lam4, lam5, f, C;
   double gam, Q;
   double tau, alpha, c_armij;
   double RelChg, RelChgOut, nrmup;
@@ -123,6 +150,11 @@ struct InstScal {
   int trace_n;
   int pad;
 };
+typedef InstScalT<float> InstScal;
+typedef InstScalT<double> InstScalD;
+static_assert(sizeof(InstScalVC<float>) == sizeof(InstScalVC<double>) &&
+                  sizeof(InstScal) == sizeof(InstScalD),
+              "instance scalar layouts differ");
 
 // Options converted once to device form.
 struct OptDev {
@@ -217,9 +249,22 @@ void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                 const InstScal *S, uint32_t mask, const float *sub,
                 float *scratch, int submat_mb);
 
-// rows per block of the blocked dense product (2d/tval3_cpu_2d.cpp:366-369)
-__host__ __device__ inline int dense_block_rows(int submat_mb, int n) {
-  const long long e = (long long)submat_mb * 1024 * 1024 / 4;
+// FP64 vectors (the double/double solver): W = 1 only.  submat_mb is
+// accepted and ignored: the transposed product is one split FP64 sum.
+template <int W>
+void launch_fwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                int kind, int dtype, const double *x, double *y,
+                const InstScal *S, uint32_t mask);
+template <int W>
+void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                int kind, int dtype, const double *x, double *y,
+                const InstScal *S, uint32_t mask, const double *sub,
+                double *scratch, int submat_mb);
+
+// rows per block of the blocked dense product (2d/tval3_cpu_2d.cpp:366-369):
+// submat_mb MiB of elem-byte entries
+__host__ __device__ inline int dense_block_rows(int submat_mb, int n, int elem = 4) {
+  const long long e = (long long)submat_mb * 1024 * 1024 / elem;
   const long long r = e / (n > 0 ? n : 1);
   return r < 1 ? 1 : (int)(r > 0x7fffffff ? 0x7fffffff : r);
 }

@@ -75,7 +75,7 @@ __global__ void k_prof_bytes(const GroupDesc *G, int W, int Bpad, const InstScal
   const unsigned long long vec = 4ull * gd.n + 4ull * gd.m;
   unsigned long long b;
   if (pd.kind == kDense) {
-    b = (pd.dtype == TV3_BF16 ? 2ull : 4ull) * m * n + vec;
+    b = (pd.dtype == TV3_BF16 ? 2ull : pd.dtype == TV3_F64 ? 8ull : 4ull) * m * n + vec;
   } else {  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
     b = 8 * nnz + 4 * ((trans ? n : m) + 1) + vec;
   }
@@ -1305,6 +1305,200 @@ void launch_bwd<16>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
 }
 
 // ---------------------------------------------------------------------------
+// FP64 vectors, W = 1: the reference's overloads with T_comp = double
+// (mat<double>, sparse_mat<double> and geometry with mat<double> U,
+// 3d/tval3_cpu_3d.cpp:47, 88, 124; 2d/tval3_cpu_2d.cpp:26-29).  Entries,
+// vector values and every product are double (a geometry entry is
+// x * (double)scale, mult_elementN with T = double, 3d/mat_mult.h:121-139);
+// the sums are FP64 like the reference's but run in a different order (lane
+// sums, row chunks), so results agree to FP64 rounding.  Layers as in the
+// float W = 1 kernels.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_fwd_w1d(const GroupDesc *G,
+                                                    const double *x, double *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int sl = threadIdx.x & (L - 1);
+  const int row = blockIdx.x * (kBlock / L) + (threadIdx.x / L);
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const int ar = row % P.m;
+  const double *xg = x + gd.off_n + (long long)(row / P.m) * P.n;
+  double acc = 0.0;
+  if (row < gd.m) {
+    constexpr int U = kW1Unroll;
+    if (P.kind == kGeometry) {
+      const RayDesc rd = P.rays[ar];
+      const double sc = (double)P.scale;
+      for (int j = sl; j <= rd.L; j += L * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int jj = j + u * L;
+          v[u] = jj <= rd.L ? xg[ray_lin(rd, jj)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j + u * L <= rd.L) acc += v[u] * sc;
+      }
+    } else if (P.kind == kSparse) {
+      const int k0 = P.r_ptr[ar], k1 = P.r_ptr[ar + 1];
+      for (int k = k0 + sl; k < k1; k += L * U) {
+        int id[U];
+        double w[U], v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kk = k + u * L;
+          id[u] = kk < k1 ? P.r_ind[kk] : -1;
+          w[u] = kk < k1 ? P.r_vald[kk] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? xg[id[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (id[u] >= 0) acc += w[u] * v[u];
+      }
+    } else {
+      const double *A = (const double *)P.A + (long long)ar * P.n;
+      for (int j = sl; j < P.n; j += L) acc += A[j] * xg[j];
+    }
+  }
+  acc = lane_sum<L>(acc);
+  if (sl == 0 && row < gd.m) y[gd.off_m + (long long)(row / P.m) * P.m + ar] = acc;
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_bwd_w1d(const GroupDesc *G,
+                                                    const double *x, double *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask,
+                                                    const double *sub) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int sl = threadIdx.x & (L - 1);
+  const int col = blockIdx.x * (kBlock / L) + (threadIdx.x / L);
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const bool geo = P.kind == kGeometry;
+  const bool tiled = geo && P.t_col;
+  int ac = col % P.n;
+  const double *xg = x + gd.off_m + (long long)(col / P.n) * P.m;
+  double acc = 0.0;
+  if (col < gd.n) {
+    constexpr int U = kW1Unroll;
+    const int *ptr = tiled ? P.t_ptr : geo ? P.g_cptr : P.c_ptr;
+    const int *ind = tiled ? P.t_ray : geo ? P.g_cray : P.c_ind;
+    const int k0 = ptr[ac], k1 = ptr[ac + 1];
+    if (tiled) ac = P.t_col[col];
+    const double sc = (double)P.scale;
+    for (int k = k0 + sl; k < k1; k += L * U) {
+      int id[U];
+      double w[U], v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = k + u * L;
+        id[u] = kk < k1 ? ind[kk] : -1;
+        w[u] = (!geo && kk < k1) ? P.c_vald[kk] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? xg[id[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (id[u] >= 0) acc += geo ? v[u] * sc : w[u] * v[u];
+    }
+  }
+  acc = lane_sum<L>(acc);
+  if (sl == 0 && col < gd.n) {
+    const long long o = gd.off_n + (long long)(col / P.n) * P.n + ac;
+    y[o] = sub ? acc - sub[o] : acc;
+  }
+}
+
+// dense A^T x: chunk c of a group covers rows [c*R, min(m, (c+1)*R)); a
+// thread owns one column (a wavefront reads 512 contiguous bytes of a row)
+// and writes its FP64 chunk sum; k_dense_bwd_reduce_d adds the chunks.
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_part_d(
+    const GroupDesc *G, const double *x, double *scratch, const InstScal *S,
+    uint32_t mask, long long sstride) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m;
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (pm + 63) / 64);
+  const int c = blockIdx.y;
+  if (c >= nch) return;
+  const int R = (pm + nch - 1) / nch;
+  const int i0 = c * R, i1 = min(pm, i0 + R);
+  const double *A = (const double *)gd.plan.A + col % n;
+  const double *xg = x + gd.off_m + (long long)(col / n) * pm;
+  double acc = 0.0;
+  int i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    double a[4], xi[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a[r] = A[(long long)(i + r) * n];
+      xi[r] = xg[i + r];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc += a[r] * xi[r];
+  }
+  for (; i < i1; ++i) acc += A[(long long)i * n] * xg[i];
+  scratch[blockIdx.z * sstride + (long long)c * gd.n + col] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce_d(
+    const GroupDesc *G, const double *scratch, double *y, const InstScal *S,
+    uint32_t mask, const double *sub, long long sstride) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (gd.plan.m + 63) / 64);
+  const double *sp = scratch + blockIdx.y * sstride + col;
+  double acc = 0.0;
+  for (int c = 0; c < nch; ++c) acc += sp[(long long)c * gd.n];
+  const long long o = gd.off_n + col;
+  y[o] = sub ? acc - sub[o] : acc;
+}
+
+template <>
+void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                   int kind, int dtype, const double *x, double *y,
+                   const InstScal *S, uint32_t mask) {
+  (void)dtype;
+  if (kind == kDense) {
+    hipLaunchKernelGGL(k_fwd_w1d<64>, dim3((max_m + kBlock / 64 - 1) / (kBlock / 64), G), dim3(kBlock), 0, s,
+                       G_dev, x, y, S, mask);
+    return;
+  }
+  hipLaunchKernelGGL(k_fwd_w1d<kSubF>, dim3((max_m + kBlock / kSubF - 1) / (kBlock / kSubF), G), dim3(kBlock), 0,
+                     s, G_dev, x, y, S, mask);
+}
+
+template <>
+void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                   int kind, int dtype, const double *x, double *y,
+                   const InstScal *S, uint32_t mask, const double *sub,
+                   double *scratch, int submat_mb) {
+  (void)dtype;
+  (void)submat_mb;
+  if (kind == kDense) {
+    // scratch holds kDenseChunks * max_n doubles per group
+    const long long sstride = (long long)kDenseChunks * max_n;
+    hipLaunchKernelGGL(k_dense_bwd_part_d, dim3((max_n + kBlock - 1) / kBlock, kDenseChunks, G), dim3(kBlock), 0,
+                       s, G_dev, x, scratch, S, mask, sstride);
+    hipLaunchKernelGGL(k_dense_bwd_reduce_d, dim3((max_n + kBlock - 1) / kBlock, G), dim3(kBlock), 0, s, G_dev,
+                       (const double *)scratch, y, S, mask, sub, sstride);
+    return;
+  }
+  hipLaunchKernelGGL(k_bwd_w1d<kSubT>, dim3((max_n + kBlock / kSubT - 1) / (kBlock / kSubT), G), dim3(kBlock), 0,
+                     s, G_dev, x, y, S, mask, sub);
+}
+
+// ---------------------------------------------------------------------------
 // geometry plan construction
 // ---------------------------------------------------------------------------
 
@@ -1470,8 +1664,8 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
 int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,
                    tv3_plan **out) {
   if (!ctx || !out) return TV3_ERR_ARG;
-  if (m <= 0 || n <= 0 || !A || (dtype != TV3_F32 && dtype != TV3_BF16))
-    return set_err(ctx, TV3_ERR_ARG, "Argument error: dense plan needs m, n > 0, A and dtype f32/bf16");
+  if (m <= 0 || n <= 0 || !A || (dtype != TV3_F32 && dtype != TV3_BF16 && dtype != TV3_F64))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: dense plan needs m, n > 0, A and dtype f32/bf16/f64");
   tv3_plan *p = new tv3_plan();
   p->ctx = ctx;
   p->dev.kind = kDense; p->dev.m = m; p->dev.n = n; p->dev.A = A;
@@ -1498,6 +1692,27 @@ int tv3_plan_sparse(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
   d.kind = kSparse; d.m = m; d.n = n;
   d.r_ptr = csr_ptr; d.r_ind = csr_ind; d.r_val = csr_val;
   d.c_ptr = csc_ptr; d.c_ind = csc_ind; d.c_val = csc_val;
+  p->nnz = nnz;
+  p->dev.nnz = nnz;
+  *out = p;
+  return TV3_OK;
+}
+
+int tv3_plan_sparse_f64(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                        const int *csr_ind, const double *csr_val,
+                        const int *csc_ptr, const int *csc_ind,
+                        const double *csc_val, tv3_plan **out) {
+  if (!ctx || !out) return TV3_ERR_ARG;
+  if (m <= 0 || n <= 0 || nnz < 0)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: sparse plan needs m, n > 0");
+  if (!csr_ptr || !csc_ptr || (nnz > 0 && (!csr_ind || !csr_val || !csc_ind || !csc_val)))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: A must be available in CSR and CSC format!");
+  tv3_plan *p = new tv3_plan();
+  p->ctx = ctx;
+  PlanDev &d = p->dev;
+  d.kind = kSparse; d.m = m; d.n = n;
+  d.r_ptr = csr_ptr; d.r_ind = csr_ind; d.r_vald = csr_val;
+  d.c_ptr = csc_ptr; d.c_ind = csc_ind; d.c_vald = csc_val;
   p->nnz = nnz;
   p->dev.nnz = nnz;
   *out = p;

@@ -41,13 +41,30 @@
 
 using namespace tv3;
 
+#ifndef TV3_VEC
+#define TV3_VEC float
+#endif
+
 namespace {
 
+// Vector (T_comp) type of this translation unit: float here; the FP64
+// engine (tv3_solver_f64.hip) compiles this file again with double.
+typedef TV3_VEC vec_t;
+__device__ __forceinline__ float vfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double vfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float vabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double vabs(double a) { return fabs(a); }
+// per-instance scalars with T_comp fields of vec_t; the operator launches
+// read only their flags and take them through InstScal (same layout)
+constexpr bool kVecF64 = sizeof(vec_t) == sizeof(double);
+typedef InstScalT<vec_t> Scal;
+inline const InstScal *flags_of(const Scal *s) { return reinterpret_cast<const InstScal *>(s); }
+
 struct Arrs {
-  float *U, *Up, *Ux, *Uxp, *Uy, *Uyp, *Uz, *Uzp, *g, *gp, *g2, *g2p;
-  float *dU, *dUx, *dUy, *dUz, *dg, *dg2, *d, *DtsAtd, *Wx, *Wy, *Wz;
-  float *sx, *sy, *sz, *Atb, *tmpn, *rcdU, *Uout;
-  float *b, *Au, *Aup, *dAu, *delta, *lastAu, *tmpm;
+  vec_t *U, *Up, *Ux, *Uxp, *Uy, *Uyp, *Uz, *Uzp, *g, *gp, *g2, *g2p;
+  vec_t *dU, *dUx, *dUy, *dUz, *dg, *dg2, *d, *DtsAtd, *Wx, *Wy, *Wz;
+  vec_t *sx, *sy, *sz, *Atb, *tmpn, *rcdU, *Uout;
+  vec_t *b, *Au, *Aup, *dAu, *delta, *lastAu, *tmpm;
 };
 
 // threads per block of the one-block-per-instance fin_* kernels
@@ -55,7 +72,7 @@ constexpr int kFin = 512;
 
 struct KA {
   const GroupDesc *G;
-  InstScal *S;
+  Scal *S;
   double *part;
   int nblk;
   uint32_t req, forbid;
@@ -73,7 +90,7 @@ __device__ __forceinline__ int uni(int v) {
   const GroupDesc &gd = K.G[blockIdx.y];                                    \
   const int lb = threadIdx.x % (W);                                          \
   const int inst = blockIdx.y * (W) + lb;                                    \
-  const InstScal &S = K.S[inst];                                             \
+  const Scal &S = K.S[inst];                                             \
   const bool on = sel(S.flags, K.req, K.forbid);                             \
   const long long E = (long long)gd.n * (W);                                 \
   if (on)                                                                    \
@@ -87,7 +104,7 @@ __device__ __forceinline__ int uni(int v) {
   const GroupDesc &gd = K.G[blockIdx.y];                                    \
   const int lb = threadIdx.x % (W);                                          \
   const int inst = blockIdx.y * (W) + lb;                                    \
-  const InstScal &S = K.S[inst];                                             \
+  const Scal &S = K.S[inst];                                             \
   const bool on = sel(S.flags, K.req, K.forbid);                             \
   const long long E = (long long)gd.m * (W);                                 \
   if (on)                                                                    \
@@ -110,15 +127,15 @@ __global__ __launch_bounds__(kBlock) void k_tau_first(KA K, Arrs a) {
     NLOOP_BEGIN(W)
     const Vox w(v, gd.q, gd.p, gd.r);
     const bool z3 = D3 && gd.r > 1;
-    const float d0 = a.d[o];
-    const float dxn = AT(a.d, w.xn()), dyn = AT(a.d, w.yn());
-    const float dzn = z3 ? AT(a.d, w.zn()) : 0.0f;
-    const float dx = dxn - d0;
-    const float dy = dyn - d0;
+    const vec_t d0 = a.d[o];
+    const vec_t dxn = AT(a.d, w.xn()), dyn = AT(a.d, w.yn());
+    const vec_t dzn = z3 ? AT(a.d, w.zn()) : 0.0f;
+    const vec_t dx = dxn - d0;
+    const vec_t dy = dyn - d0;
     acc[0] += (double)dx * (double)dx;
     acc[1] += (double)dy * (double)dy;
     if (z3) {
-      const float dz = dzn - d0;
+      const vec_t dz = dzn - d0;
       acc[2] += (double)dz * (double)dz;
     }
     acc[3] += (double)d0 * (double)d0;
@@ -129,11 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_tau_first(KA K, Arrs a) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_msq(KA K, const float *x) {
+__global__ __launch_bounds__(kBlock) void k_msq(KA K, const vec_t *x) {
   double acc[1] = {0.0};
   {
     MLOOP_BEGIN(W)
-    const float t = x[o];
+    const vec_t t = x[o];
     acc[0] += (double)t * (double)t;
     LOOP_END
   }
@@ -158,30 +175,30 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
   double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     NLOOP_BEGIN(W)
-    const float tf = (float)(-1 * (TS)S.tau);
-    auto un = [&](float dd, float pp) -> float {
-      float u = fmaf(tf, dd, pp);
+    const vec_t tf = (vec_t)(-1 * (TS)S.tau);
+    auto un = [&](vec_t dd, vec_t pp) -> vec_t {
+      vec_t u = vfma(tf, dd, pp);
       if (nonneg) u = (u < 0.0f) ? 0.0f : u;
       return u;
     };
-    auto unj = [&](int j) -> float { return un(AT(a.d, j), AT(a.Up, j)); };
+    auto unj = [&](int j) -> vec_t { return un(AT(a.d, j), AT(a.Up, j)); };
     const Vox w(v, gd.q, gd.p, gd.r);
     const bool z3 = D3 && gd.r > 1;
     const int xn = w.xn(), yn = w.yn(), xp = w.xp(), yp = w.yp();
     // ---- loads
-    const float d0 = a.d[o], p0 = a.Up[o];
-    const float dxn = AT(a.d, xn), pxn = AT(a.Up, xn);
-    const float dyn = AT(a.d, yn), pyn = AT(a.Up, yn);
-    const float dxp = AT(a.d, xp), pxp = AT(a.Up, xp);
-    const float dyp = AT(a.d, yp), pyp = AT(a.Up, yp);
-    const float wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
-    const float wxp = AT(a.Wx, xp), wyp = AT(a.Wy, yp);
-    float qx = 0.0f, qy = 0.0f, qz = 0.0f;  // Ux, Uy, Uz of the previous iterate
+    const vec_t d0 = a.d[o], p0 = a.Up[o];
+    const vec_t dxn = AT(a.d, xn), pxn = AT(a.Up, xn);
+    const vec_t dyn = AT(a.d, yn), pyn = AT(a.Up, yn);
+    const vec_t dxp = AT(a.d, xp), pxp = AT(a.Up, xp);
+    const vec_t dyp = AT(a.d, yp), pyp = AT(a.Up, yp);
+    const vec_t wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
+    const vec_t wxp = AT(a.Wx, xp), wyp = AT(a.Wy, yp);
+    vec_t qx = 0.0f, qy = 0.0f, qz = 0.0f;  // Ux, Uy, Uz of the previous iterate
     if (POLY) {
       qx = a.Uxp[o];
       qy = a.Uyp[o];
     }
-    float dzn = 0.0f, pzn = 0.0f, wz = 0.0f, sz = 0.0f;
+    vec_t dzn = 0.0f, pzn = 0.0f, wz = 0.0f, sz = 0.0f;
     if (z3) {
       dzn = AT(a.d, w.zn());
       pzn = AT(a.Up, w.zn());
@@ -190,13 +207,13 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
       if (POLY) qz = a.Uzp[o];
     }
     // ---- U = Up - tau d, D(U), V = D(U) - W
-    const float u0 = un(d0, p0);
-    const float ux = un(dxn, pxn) - u0;
-    const float uy = un(dyn, pyn) - u0;
-    const float uz = z3 ? un(dzn, pzn) - u0 : 0.0f;
-    const float Vx0 = ux - wx;
-    const float Vy0 = uy - wy;
-    const float Vz0 = z3 ? uz - wz : 0.0f;
+    const vec_t u0 = un(d0, p0);
+    const vec_t ux = un(dxn, pxn) - u0;
+    const vec_t uy = un(dyn, pyn) - u0;
+    const vec_t uz = z3 ? un(dzn, pzn) - u0 : 0.0f;
+    const vec_t Vx0 = ux - wx;
+    const vec_t Vy0 = uy - wy;
+    const vec_t Vz0 = z3 ? uz - wz : 0.0f;
     acc[0] += (double)Vx0 * (double)Vx0;
     acc[1] += (double)Vy0 * (double)Vy0;
     acc[3] += (double)sx * (double)Vx0;
@@ -220,16 +237,16 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
       acc[7] += A2; acc[8] += B2; acc[9] += C2; acc[10] += A4; acc[11] += B4;
     }
     // ---- g2 = Dt(V) at v
-    const float VXp = (u0 - un(dxp, pxp)) - wxp;
-    const float VYp = (u0 - un(dyp, pyp)) - wyp;
-    float t = VXp - Vx0;
+    const vec_t VXp = (u0 - un(dxp, pxp)) - wxp;
+    const vec_t VYp = (u0 - un(dyp, pyp)) - wyp;
+    vec_t t = VXp - Vx0;
     if (!z3) {
       t = t + VYp;
       t = t - Vy0;
     } else {
       const int zp = w.zp();
       if (w.v >= w.q && (w.y > 0 || !exact)) {
-        float Yb = VYp;
+        vec_t Yb = VYp;
         if (w.y == 0) {  // last row of the previous slice
           const int j = v - w.q;
           Yb = (unj(v - w.s) - unj(j)) - AT(a.Wy, j);
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
         t = t + VYp;
         t = t - Vy0;
       }
-      const float VZp = (u0 - unj(zp)) - AT(a.Wz, zp);
+      const vec_t VZp = (u0 - unj(zp)) - AT(a.Wz, zp);
       t = t + VZp;
       t = t - Vz0;
     }
@@ -267,16 +284,16 @@ __global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a) {
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     MLOOP_BEGIN(W)
-    const float au = a.Au[o];
-    const float bb = a.b[o];
-    const float dl = a.delta[o];
-    const float la = SKIP ? a.lastAu[o] : 0.0f;
-    const float aup = POLY ? a.Aup[o] : 0.0f;
-    const float aub = au - bb;
+    const vec_t au = a.Au[o];
+    const vec_t bb = a.b[o];
+    const vec_t dl = a.delta[o];
+    const vec_t la = SKIP ? a.lastAu[o] : 0.0f;
+    const vec_t aup = POLY ? a.Aup[o] : 0.0f;
+    const vec_t aub = au - bb;
     acc[0] += (double)aub * (double)aub;
     acc[1] += (double)dl * (double)aub;
     if (SKIP) {
-      const float da = au - la;
+      const vec_t da = au - la;
       acc[2] += (double)da * (double)da;
       acc[3] += (double)au * (double)au;
     }
@@ -296,8 +313,8 @@ __global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_copy_m(KA K, const float *src,
-                                                   float *dst) {
+__global__ __launch_bounds__(kBlock) void k_copy_m(KA K, const vec_t *src,
+                                                   vec_t *dst) {
   MLOOP_BEGIN(W)
   dst[o] = src[o];
   LOOP_END
@@ -318,11 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_skipg(KA K, Arrs a) {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bt_diff_n(KA K, Arrs a) {
   NLOOP_BEGIN(W)
-  const float dg = a.g[o] - a.gp[o];
-  const float dg2 = a.g2[o] - a.g2p[o];
-  const float dux = a.Ux[o] - a.Uxp[o];
-  const float duy = a.Uy[o] - a.Uyp[o];
-  const float duz = gd.r > 1 ? a.Uz[o] - a.Uzp[o] : 0.0f;
+  const vec_t dg = a.g[o] - a.gp[o];
+  const vec_t dg2 = a.g2[o] - a.g2p[o];
+  const vec_t dux = a.Ux[o] - a.Uxp[o];
+  const vec_t duy = a.Uy[o] - a.Uyp[o];
+  const vec_t duz = gd.r > 1 ? a.Uz[o] - a.Uzp[o] : 0.0f;
   a.dg[o] = dg;
   a.dg2[o] = dg2;
   a.dUx[o] = dux;
@@ -339,8 +356,8 @@ __global__ __launch_bounds__(kBlock) void k_bt_diff_m(KA K, Arrs a) {
 }
 
 template <class TS>
-__device__ __forceinline__ float interp(float x, TS alpha, float y) {
-  return (float)fma(alpha, (TS)y, (TS)x);
+__device__ __forceinline__ vec_t interp(vec_t x, TS alpha, vec_t y) {
+  return (vec_t)fma(alpha, (TS)y, (TS)x);
 }
 
 template <int W, class TS>
@@ -351,11 +368,11 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
     const TS al = (TS)S.alpha;
     const bool z3 = gd.r > 1;
     // ---- loads
-    const float gp = a.gp[o], dg = a.dg[o], g2p = a.g2p[o], dg2 = a.dg2[o];
-    const float up = a.Up[o], du = a.dU[o];
-    const float uxp = a.Uxp[o], dux = a.dUx[o], uyp = a.Uyp[o], duy = a.dUy[o];
-    const float wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
-    float uzp = 0.0f, duz = 0.0f, wz = 0.0f, sz = 0.0f;
+    const vec_t gp = a.gp[o], dg = a.dg[o], g2p = a.g2p[o], dg2 = a.dg2[o];
+    const vec_t up = a.Up[o], du = a.dU[o];
+    const vec_t uxp = a.Uxp[o], dux = a.dUx[o], uyp = a.Uyp[o], duy = a.dUy[o];
+    const vec_t wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
+    vec_t uzp = 0.0f, duz = 0.0f, wz = 0.0f, sz = 0.0f;
     if (z3) {
       uzp = a.Uzp[o];
       duz = a.dUz[o];
@@ -363,17 +380,17 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
       sz = a.sz[o];
     }
     // ---- the state at alpha, lam2, lam4
-    const float ux = interp<TS>(uxp, al, dux);
-    const float uy = interp<TS>(uyp, al, duy);
-    const float Vx = ux - wx, Vy = uy - wy;
+    const vec_t ux = interp<TS>(uxp, al, dux);
+    const vec_t uy = interp<TS>(uyp, al, duy);
+    const vec_t Vx = ux - wx, Vy = uy - wy;
     acc[0] += (double)Vx * (double)Vx;
     acc[1] += (double)Vy * (double)Vy;
     acc[3] += (double)sx * (double)Vx;
     acc[4] += (double)sy * (double)Vy;
-    float uz = 0.0f;
+    vec_t uz = 0.0f;
     if (z3) {
       uz = interp<TS>(uzp, al, duz);
-      const float Vz = uz - wz;
+      const vec_t Vz = uz - wz;
       acc[2] += (double)Vz * (double)Vz;
       acc[5] += (double)sz * (double)Vz;
     }
@@ -396,9 +413,9 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_m(KA K, Arrs a) {
   {
     MLOOP_BEGIN(W)
     const TS al = (TS)S.alpha;
-    const float au = interp<TS>(a.Aup[o], al, a.dAu[o]);
+    const vec_t au = interp<TS>(a.Aup[o], al, a.dAu[o]);
     a.Au[o] = au;
-    const float aub = au - a.b[o];
+    const vec_t aub = au - a.b[o];
     acc[0] += (double)aub * (double)aub;
     acc[1] += (double)a.delta[o] * (double)aub;
     LOOP_END
@@ -419,11 +436,11 @@ __global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
     const TS al = (TS)S.alpha;
     const bool z3 = gd.r > 1;
     // ---- loads
-    const float gp = a.gp[o], g2p = a.g2p[o], gc = a.g[o], g2c = a.g2[o];
-    const float up = a.Up[o], du = a.dU[o];
-    const float uxp = a.Uxp[o], uyp = a.Uyp[o], uxc = a.Ux[o], uyc = a.Uy[o];
-    const float wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
-    float uzp = 0.0f, uzc = 0.0f, wz = 0.0f, sz = 0.0f;
+    const vec_t gp = a.gp[o], g2p = a.g2p[o], gc = a.g[o], g2c = a.g2[o];
+    const vec_t up = a.Up[o], du = a.dU[o];
+    const vec_t uxp = a.Uxp[o], uyp = a.Uyp[o], uxc = a.Ux[o], uyc = a.Uy[o];
+    const vec_t wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
+    vec_t uzp = 0.0f, uzc = 0.0f, wz = 0.0f, sz = 0.0f;
     if (z3) {
       uzp = a.Uzp[o];
       uzc = a.Uz[o];
@@ -431,17 +448,17 @@ __global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
       sz = a.sz[o];
     }
     // ---- the state at alpha, lam2, lam4
-    const float ux = interp<TS>(uxp, al, uxc - uxp);
-    const float uy = interp<TS>(uyp, al, uyc - uyp);
-    const float Vx = ux - wx, Vy = uy - wy;
+    const vec_t ux = interp<TS>(uxp, al, uxc - uxp);
+    const vec_t uy = interp<TS>(uyp, al, uyc - uyp);
+    const vec_t Vx = ux - wx, Vy = uy - wy;
     acc[0] += (double)Vx * (double)Vx;
     acc[1] += (double)Vy * (double)Vy;
     acc[3] += (double)sx * (double)Vx;
     acc[4] += (double)sy * (double)Vy;
-    float uz = 0.0f;
+    vec_t uz = 0.0f;
     if (z3) {
       uz = interp<TS>(uzp, al, uzc - uzp);
-      const float Vz = uz - wz;
+      const vec_t Vz = uz - wz;
       acc[2] += (double)Vz * (double)Vz;
       acc[5] += (double)sz * (double)Vz;
     }
@@ -464,10 +481,10 @@ __global__ __launch_bounds__(kBlock) void k_interp_m(KA K, Arrs a) {
   {
     MLOOP_BEGIN(W)
     const TS al = (TS)S.alpha;
-    const float aup = a.Aup[o];
-    const float au = interp<TS>(aup, al, a.Au[o] - aup);
+    const vec_t aup = a.Aup[o];
+    const vec_t au = interp<TS>(aup, al, a.Au[o] - aup);
     a.Au[o] = au;
-    const float aub = au - a.b[o];
+    const vec_t aub = au - a.b[o];
     acc[0] += (double)aub * (double)aub;
     acc[1] += (double)a.delta[o] * (double)aub;
     LOOP_END
@@ -480,11 +497,11 @@ __global__ __launch_bounds__(kBlock) void k_interp_m(KA K, Arrs a) {
 // D (2d:268-278) alone, for the initial U = A^T b
 // ---------------------------------------------------------------------------
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_D(KA K, const float *U, float *Ux,
-                                              float *Uy, float *Uz) {
+__global__ __launch_bounds__(kBlock) void k_D(KA K, const vec_t *U, vec_t *Ux,
+                                              vec_t *Uy, vec_t *Uz) {
   NLOOP_BEGIN(W)
   const Vox w(v, gd.q, gd.p, gd.r);
-  const float u0 = U[o];
+  const vec_t u0 = U[o];
   Ux[o] = AT(U, w.xn()) - u0;
   Uy[o] = AT(U, w.yn()) - u0;
   if (gd.r > 1) Uz[o] = AT(U, w.zn()) - u0;
@@ -507,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
     // D(U) and sigma at one voxel; the anisotropic shrinkage of a neighbour
     // needs one direction only (dir 0, 1, 2; -1: all)
     struct Cell {
-      float ux, uy, uz, sx, sy, sz;
+      vec_t ux, uy, uz, sx, sy, sz;
     };
     auto cell = [&](int j, int dir) -> Cell {
       const long long oj = gd.off_n + (long long)j * W + lb;
@@ -518,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
       return c;
     };
     // shrinkage of one voxel; returns the T_scal lam1 term
-    auto shr = [&](const Cell &c, float &wx, float &wy, float &wz) -> TS {
+    auto shr = [&](const Cell &c, vec_t &wx, vec_t &wy, vec_t &wz) -> TS {
       if (ISO)
         return shrink_iso<TS>(c.ux, c.uy, c.uz, c.sx, c.sy, c.sz, z3, beta, ib, wx, wy, wz);
       TS tx, ty, tz = 0;
@@ -540,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
       // the y-term from the last row of the previous slice
       if (w.v >= w.q && w.y == 0 && !exact) cj = cell(v - w.q, 1);
     }
-    float up = 0.0f, uc = 0.0f, gc = 0.0f, gpv = 0.0f, g2pv = 0.0f, dts = 0.0f;
+    vec_t up = 0.0f, uc = 0.0f, gc = 0.0f, gpv = 0.0f, g2pv = 0.0f, dts = 0.0f;
     if (!INIT) {
       up = a.Up[o];
       uc = a.U[o];
@@ -550,11 +567,11 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
       dts = a.DtsAtd[o];
     }
     // ---- W = shrink(D(U) + sigma/beta), V = D(U) - W, lam1, lam2, lam4
-    float wx0, wy0, wz0;
+    vec_t wx0, wy0, wz0;
     const TS l1 = shr(c0, wx0, wy0, wz0);
     acc[0] += (double)l1;
-    const float Vx0 = c0.ux - wx0, Vy0 = c0.uy - wy0;
-    const float Vz0 = z3 ? c0.uz - wz0 : 0.0f;
+    const vec_t Vx0 = c0.ux - wx0, Vy0 = c0.uy - wy0;
+    const vec_t Vz0 = z3 ? c0.uz - wz0 : 0.0f;
     acc[1] += (double)Vx0 * (double)Vx0;
     acc[2] += (double)Vy0 * (double)Vy0;
     acc[4] += (double)c0.sx * (double)Vx0;
@@ -564,18 +581,18 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
       acc[6] += (double)c0.sz * (double)Vz0;
     }
     // ---- g2 = Dt(V) at v
-    float tx, ty, tz;
+    vec_t tx, ty, tz;
     shr(cx, tx, ty, tz);
-    const float VXp = cx.ux - tx;
+    const vec_t VXp = cx.ux - tx;
     shr(cy, tx, ty, tz);
-    const float VYp = cy.uy - ty;
-    float t = VXp - Vx0;
+    const vec_t VYp = cy.uy - ty;
+    vec_t t = VXp - Vx0;
     if (!z3) {
       t = t + VYp;
       t = t - Vy0;
     } else {
       if (w.v >= w.q && (w.y > 0 || !exact)) {
-        float Yb = VYp;
+        vec_t Yb = VYp;
         if (w.y == 0) {
           shr(cj, tx, ty, tz);
           Yb = cj.uy - ty;
@@ -588,24 +605,24 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
         t = t - Vy0;
       }
       shr(cz, tx, ty, tz);
-      const float VZp = cz.uz - tz;
+      const vec_t VZp = cz.uz - tz;
       t = t + VZp;
       t = t - Vz0;
     }
-    float dn = 0.0f;
+    vec_t dn = 0.0f;
     if (!INIT) {
-      const float uu = uc - up;
+      const vec_t uu = uc - up;
       acc[7] += (double)up * (double)up;
       acc[8] += (double)uu * (double)uu;
-      const float mdb = (float)S.muDbeta;
-      dn = fmaf(mdb, gc, dts + t);
+      const vec_t mdb = (vec_t)S.muDbeta;
+      dn = vfma(mdb, gc, dts + t);
       // the next iteration's Barzilai-Borwein step (get_tau, 2d:443-450):
       // s = U - Up, y = (g2 - g2p) + (mu/beta) (g - gp).  No kernel between
       // here and the next get_tau changes these vectors or mu/beta of an
       // instance that is not restarting its inner loop (which takes the
       // first-iteration formula instead), so the dot products are formed
       // here; slot 8 holds <s, s>.
-      const float y = fmaf(mdb, gc - gpv, t - g2pv);
+      const vec_t y = vfma(mdb, gc - gpv, t - g2pv);
       acc[9] += (double)uu * (double)y;
     }
     // ---- stores
@@ -625,14 +642,14 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_grad(KA K, Arrs a, int scale) {
   NLOOP_BEGIN(W)
-  float dts = a.DtsAtd[o];
-  const float dc = scale ? a.d[o] : 0.0f;
-  const float gc = a.g[o], g2c = a.g2[o];
+  vec_t dts = a.DtsAtd[o];
+  const vec_t dc = scale ? a.d[o] : 0.0f;
+  const vec_t gc = a.g[o], g2c = a.g2[o];
   if (scale) {
-    dts = (float)S.ratio * dc;
+    dts = (vec_t)S.ratio * dc;
     a.DtsAtd[o] = dts;
   }
-  a.d[o] = fmaf((float)S.muDbeta, gc, dts + g2c);
+  a.d[o] = vfma((vec_t)S.muDbeta, gc, dts + g2c);
   LOOP_END
 }
 
@@ -644,8 +661,8 @@ __global__ __launch_bounds__(kBlock) void k_outer1(KA K, Arrs a) {
   double acc[1] = {0};
   {
     NLOOP_BEGIN(W)
-    const float u = a.U[o];
-    const float df = u - a.rcdU[o];
+    const vec_t u = a.U[o];
+    const vec_t df = u - a.rcdU[o];
     acc[0] += (double)df * (double)df;
     a.rcdU[o] = u;
     LOOP_END
@@ -659,14 +676,14 @@ __global__ __launch_bounds__(kBlock) void k_mlp_n(KA K, Arrs a) {
   double acc[3] = {0, 0, 0};
   {
     NLOOP_BEGIN(W)
-    const float nb = (float)(-1 * S.beta);
+    const vec_t nb = (vec_t)(-1 * S.beta);
     const bool z3 = gd.r > 1;
-    const float Vx = a.Ux[o] - a.Wx[o], Vy = a.Uy[o] - a.Wy[o];
-    const float sx = fmaf(nb, Vx, a.sx[o]), sy = fmaf(nb, Vy, a.sy[o]);
-    float Vz = 0.0f, sz = 0.0f;
+    const vec_t Vx = a.Ux[o] - a.Wx[o], Vy = a.Uy[o] - a.Wy[o];
+    const vec_t sx = vfma(nb, Vx, a.sx[o]), sy = vfma(nb, Vy, a.sy[o]);
+    vec_t Vz = 0.0f, sz = 0.0f;
     if (z3) {
       Vz = a.Uz[o] - a.Wz[o];
-      sz = fmaf(nb, Vz, a.sz[o]);
+      sz = vfma(nb, Vz, a.sz[o]);
     }
     a.sx[o] = sx;
     a.sy[o] = sy;
@@ -687,9 +704,9 @@ __global__ __launch_bounds__(kBlock) void k_mlp_m(KA K, Arrs a) {
   double acc[1] = {0};
   {
     MLOOP_BEGIN(W)
-    const float nm = (float)(-1 * S.mu);
-    const float aub = a.Au[o] - a.b[o];
-    const float dl = fmaf(nm, aub, a.delta[o]);
+    const vec_t nm = (vec_t)(-1 * S.mu);
+    const vec_t aub = a.Au[o] - a.b[o];
+    const vec_t dl = vfma(nm, aub, a.delta[o]);
     a.delta[o] = dl;
     acc[0] += (double)dl * (double)aub;
     LOOP_END
@@ -702,19 +719,19 @@ __global__ __launch_bounds__(kBlock) void k_mlp_m(KA K, Arrs a) {
 template <int W, class TS>
 __global__ __launch_bounds__(kBlock) void k_finish(KA K, Arrs a) {
   NLOOP_BEGIN(W)
-  a.Uout[o] = (float)((TS)a.U[o] / (TS)S.scl);
+  a.Uout[o] = (vec_t)((TS)a.U[o] / (TS)S.scl);
   LOOP_END
 }
 
 // |Ut - U|^2 and |Ut|^2 for info.rel_error (2d:869-874)
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_relerr(KA K, const float *Ut,
-                                                   const float *U) {
+__global__ __launch_bounds__(kBlock) void k_relerr(KA K, const vec_t *Ut,
+                                                   const vec_t *U) {
   double acc[2] = {0, 0};
   {
     NLOOP_BEGIN(W)
-    const float t = Ut[o];
-    const float df = t - U[o];
+    const vec_t t = Ut[o];
+    const vec_t df = t - U[o];
     acc[0] += (double)df * (double)df;
     acc[1] += (double)t * (double)t;
     LOOP_END
@@ -727,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_relerr(KA K, const float *Ut,
 // scaleb (2d:228-266): first index of min |b| and of max |b|
 // ---------------------------------------------------------------------------
 struct MinMax {
-  float mn, mx;
+  vec_t mn, mx;
   int imn, imx;
 };
 
@@ -737,7 +754,7 @@ __device__ __forceinline__ void mm_merge(MinMax &a, const MinMax &b) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const float *bv,
+__global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const vec_t *bv,
                                                       MinMax *mmpart) {
   const GroupDesc &gd = K.G[blockIdx.y];
   const int lb = threadIdx.x % W;
@@ -746,7 +763,7 @@ __global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const float *bv,
   const long long E = (long long)gd.m * W;
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
        e += (long long)K.nblk * kBlock) {
-    const float t = fabsf(bv[gd.off_m + e]);
+    const vec_t t = vabs(bv[gd.off_m + e]);
     const int i = (int)(e / W);
     MinMax c{t, t, i, i};
     mm_merge(r, c);
@@ -762,11 +779,11 @@ __global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const float *bv,
     mmpart[(long long)inst * kNblkMax + blockIdx.x] = red[threadIdx.x];
 }
 
-// b = (float)scl * b (cblas_scal takes a float alpha)
+// b = (vec_t)scl * b (cblas_scal takes a vec_t alpha)
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_scale_m(KA K, float *x) {
+__global__ __launch_bounds__(kBlock) void k_scale_m(KA K, vec_t *x) {
   MLOOP_BEGIN(W)
-  x[o] = (float)S.scl * x[o];
+  x[o] = (vec_t)S.scl * x[o];
   LOOP_END
 }
 
@@ -774,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_scale_m(KA K, float *x) {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_init_U(KA K, Arrs a) {
   NLOOP_BEGIN(W)
-  const float t = a.Atb[o];
+  const vec_t t = a.Atb[o];
   a.U[o] = t;
   a.rcdU[o] = t;
   LOOP_END
@@ -784,8 +801,8 @@ __global__ __launch_bounds__(kBlock) void k_init_U(KA K, Arrs a) {
 // lane-interleaved layout, and back
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_pack(const GroupDesc *G,
-                                                 const float *const *src,
-                                                 float *dst, int use_m) {
+                                                 const vec_t *const *src,
+                                                 vec_t *dst, int use_m) {
   const GroupDesc &gd = G[blockIdx.y];
   const long long len = use_m ? gd.m : gd.n;
   const long long off = use_m ? gd.off_m : gd.off_n;
@@ -793,21 +810,21 @@ __global__ __launch_bounds__(kBlock) void k_pack(const GroupDesc *G,
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
        e += (long long)gridDim.x * kBlock) {
     const int lb = (int)(e % W);
-    const float *p = src[blockIdx.y * W + lb];
+    const vec_t *p = src[blockIdx.y * W + lb];
     dst[off + e] = p ? p[e / W] : 0.0f;
   }
 }
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_unpack(const GroupDesc *G,
-                                                   const float *src,
-                                                   float *const *dst) {
+                                                   const vec_t *src,
+                                                   vec_t *const *dst) {
   const GroupDesc &gd = G[blockIdx.y];
   const long long E = (long long)gd.n * W;
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
        e += (long long)gridDim.x * kBlock) {
     const int lb = (int)(e % W);
-    float *p = dst[blockIdx.y * W + lb];
+    vec_t *p = dst[blockIdx.y * W + lb];
     if (p) p[e / W] = src[gd.off_n + e];
   }
 }
@@ -816,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack(const GroupDesc *G,
 // per-instance scalar control (one 256-thread block per instance)
 // ===========================================================================
 struct FA {
-  InstScal *S;
+  Scal *S;
   const GroupDesc *G;
   const double *part;
   int nblk_n, nblk_m;
@@ -873,22 +890,22 @@ __device__ __forceinline__ void put_flags(const FA &F, int inst, uint32_t f) {
 
 // f = lam1 + beta/2*lam2 + mu/2*lam3 - lam4 - lam5 (2d:422, 523, 842)
 template <class TS>
-__device__ __forceinline__ float f_full(const InstScal &s) {
+__device__ __forceinline__ vec_t f_full(const Scal &s) {
   const TS lam1 = (TS)s.lam1, lam2 = (TS)s.lam2, lam3 = (TS)s.lam3;
   const TS beta = (TS)s.beta, mu = (TS)s.mu;
-  return (float)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)s.lam4) -
+  return (vec_t)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)s.lam4) -
                  (TS)s.lam5);
 }
 
 // the Armijo test of min_u (2d:548, 558): |f| > |C - alpha*c_armij|
 template <class TS>
-__device__ __forceinline__ bool armijo_fails(const InstScal &s) {
-  return (TS)fabsf(s.f) > fabs((TS)s.C - (TS)s.alpha * (TS)s.c_armij);
+__device__ __forceinline__ bool armijo_fails(const Scal &s) {
+  return (TS)vabs(s.f) > fabs((TS)s.C - (TS)s.alpha * (TS)s.c_armij);
 }
 
 #define FIN_BEGIN                                                      \
   const int inst = blockIdx.x;                                         \
-  InstScal &s = F.S[inst];                                             \
+  Scal &s = F.S[inst];                                             \
   const uint32_t fl0 = s.flags;                                        \
   const bool on = sel(fl0, F.req, F.forbid);                           \
   if (!on) {                                                           \
@@ -904,15 +921,15 @@ __device__ __forceinline__ bool armijo_fails(const InstScal &s) {
 // lam2 and lam4 from three direction dot products (get_g / update_g /
 // update_W: 2d:408-411)
 template <class TS>
-__device__ __forceinline__ void set_lam24(InstScal &s, bool z3, double x2,
+__device__ __forceinline__ void set_lam24(Scal &s, bool z3, double x2,
                                           double y2, double z2, double x4,
                                           double y4, double z4) {
-  TS lam2 = (TS)(float)x2;
-  lam2 += (TS)(float)y2;
-  if (z3) lam2 += (TS)(float)z2;
-  float lam4 = (float)x4;
-  lam4 += (float)y4;
-  if (z3) lam4 += (float)z4;
+  TS lam2 = (TS)(vec_t)x2;
+  lam2 += (TS)(vec_t)y2;
+  if (z3) lam2 += (TS)(vec_t)z2;
+  vec_t lam4 = (vec_t)x4;
+  lam4 += (vec_t)y4;
+  if (z3) lam4 += (vec_t)z4;
   s.lam2 = (double)lam2;
   s.lam4 = lam4;
 }
@@ -926,7 +943,7 @@ constexpr uint32_t kStepFlags =
 template <class TS>
 __global__ __launch_bounds__(kFin) void fin_tau(FA F) {
   const int inst = blockIdx.x;
-  InstScal &s = F.S[inst];
+  Scal &s = F.S[inst];
   const uint32_t fl = s.flags;
   const bool fb = (F.req & F_FB) != 0;
   if (fb ? !(fl & F_FB) : !sel(fl, F.req, F.forbid)) {
@@ -953,13 +970,13 @@ __global__ __launch_bounds__(kFin) void fin_tau(FA F) {
   if (threadIdx.x == 0) {
     TS tau;
     if (first) {
-      float dDd = (float)r[0] + (float)r[1];
-      if (gd.r > 1) dDd = dDd + (float)r[2];
-      const float dd = (float)r[3], Add = (float)r[4];
+      vec_t dDd = (vec_t)r[0] + (vec_t)r[1];
+      if (gd.r > 1) dDd = dDd + (vec_t)r[2];
+      const vec_t dd = (vec_t)r[3], Add = (vec_t)r[4];
       tau = fabs((TS)dd / ((TS)dDd + (TS)s.muDbeta * (TS)Add));
     } else {
-      const float ss = (float)r[0], sy = (float)r[1];
-      tau = (TS)fabsf(ss / sy);
+      const vec_t ss = (vec_t)r[0], sy = (vec_t)r[1];
+      tau = (TS)vabs(ss / sy);
     }
     s.tau = (double)tau;
     if (!fb) {
@@ -981,7 +998,7 @@ __global__ __launch_bounds__(kFin) void fin_skip(FA F, int force) {
   double r[2];
   fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    const TS nrm_d = (TS)(float)sqrt(r[0]), nrm_a = (TS)(float)sqrt(r[1]);
+    const TS nrm_d = (TS)(vec_t)sqrt(r[0]), nrm_a = (TS)(vec_t)sqrt(r[1]);
     const TS relChange = nrm_d / nrm_a;
     uint32_t fl = s.flags & ~(F_BWD | F_SKIPG);
     TS csv = (TS)s.currSkipValue;
@@ -1029,11 +1046,11 @@ __global__ __launch_bounds__(kFin) void fin_descend(FA F, int mode) {
   }
   if (threadIdx.x == 0) {
     set_lam24<TS>(s, z3, r[0], r[1], r[2], r[3], r[4], r[5]);
-    s.lam3 = (double)(TS)(float)r[12];
-    s.lam5 = (float)r[13];
+    s.lam3 = (double)(TS)(vec_t)r[12];
+    s.lam5 = (vec_t)r[13];
     s.f = f_full<TS>(s);
     if (mode == 0) {
-      s.c_armij = (double)((TS)(float)r[6] * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
+      s.c_armij = (double)((TS)(vec_t)r[6] * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
       s.alpha = 1.0;
       if (armijo_fails<TS>(s)) {
         s.cnt = 0;
@@ -1087,8 +1104,8 @@ __global__ __launch_bounds__(kFin) void fin_lams(FA F, int mode) {
   fin_sums<8>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     set_lam24<TS>(s, z3, r[0], r[1], r[2], r[3], r[4], r[5]);
-    s.lam3 = (double)(TS)(float)r[6];
-    s.lam5 = (float)r[7];
+    s.lam3 = (double)(TS)(vec_t)r[6];
+    s.lam5 = (vec_t)r[7];
     s.f = f_full<TS>(s);
     if (mode == 1) {
       s.flags &= ~F_BT;
@@ -1123,18 +1140,18 @@ __global__ __launch_bounds__(kFin) void fin_wstep(FA F, int init) {
       s.lam1 = (double)(TS)r[0];
       set_lam24<TS>(s, z3, r[1], r[2], r[3], r[4], r[5], r[6]);
     } else {
-      s.f = (float)((TS)s.f - (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
+      s.f = (vec_t)((TS)s.f - (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
       s.lam1 = (double)(TS)r[0];
       set_lam24<TS>(s, z3, r[1], r[2], r[3], r[4], r[5], r[6]);
-      s.f = (float)((TS)s.f + (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
+      s.f = (vec_t)((TS)s.f + (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
       const TS gam = (TS)s.gam;
       const TS Qp = (TS)s.Q;
       const TS Q = gam * Qp + 1;
       s.Q = (double)Q;
-      s.C = (float)((gam * Qp * (TS)s.C + (TS)s.f) / Q);
+      s.C = (vec_t)((gam * Qp * (TS)s.C + (TS)s.f) / Q);
       s.count_total++;
-      const TS nrmup = (TS)(float)sqrt(r[7]);
-      const TS relchg = (TS)(float)sqrt(r[8]) / nrmup;
+      const TS nrmup = (TS)(vec_t)sqrt(r[7]);
+      const TS relchg = (TS)(vec_t)sqrt(r[8]) / nrmup;
       s.nrmup = (double)nrmup;
       s.RelChg = (double)relchg;
       if (F.trace && s.count_total <= F.trace_cap) {
@@ -1164,8 +1181,8 @@ __global__ __launch_bounds__(kFin) void fin_init2(FA F) {
   double r[2];
   fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    s.lam3 = (double)(TS)(float)r[0];
-    s.lam5 = (float)r[1];
+    s.lam3 = (double)(TS)(vec_t)r[0];
+    s.lam5 = (vec_t)r[1];
     s.f = f_full<TS>(s);
     s.Q = 1.0;
     s.C = s.f;
@@ -1178,7 +1195,7 @@ __global__ __launch_bounds__(kFin) void fin_init2(FA F) {
 // scaleb (2d:228-266) and the penalty clamps (2d:776-778)
 template <class TS>
 __global__ __launch_bounds__(kBlock) void fin_scaleb(FA F, const MinMax *mmp,
-                                                     const float *bv) {
+                                                     const vec_t *bv) {
   FIN_BEGIN
   __shared__ MinMax red[kBlock];
   MinMax r{INFINITY, -INFINITY, 0x7fffffff, 0x7fffffff};
@@ -1200,9 +1217,9 @@ __global__ __launch_bounds__(kBlock) void fin_scaleb(FA F, const MinMax *mmp,
       // scan starts from it
       const int imn = (mm.imn >= 0 && mm.imn < m) ? mm.imn : 0;
       const int imx = (mm.imx >= 0 && mm.imx < m) ? mm.imx : 0;
-      const float bmin = bv[gd.off_m + (long long)imn * F.W + lbx];
-      const float bmax = bv[gd.off_m + (long long)imx * F.W + lbx];
-      const TS b_dif = fabsf(bmax - bmin);
+      const vec_t bmin = bv[gd.off_m + (long long)imn * F.W + lbx];
+      const vec_t bmax = bv[gd.off_m + (long long)imx * F.W + lbx];
+      const TS b_dif = vabs(bmax - bmin);
       const TS th1 = (TS)0.5, th2 = (TS)1.5;
       if (b_dif < th1)
         scl = th1 / b_dif;
@@ -1231,7 +1248,7 @@ __global__ __launch_bounds__(kFin) void fin_outer1(FA F) {
   fin_sums<1>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     s.count_outer++;
-    const TS rr = (TS)(float)sqrt(r[0]) / (TS)s.nrmup;
+    const TS rr = (TS)(vec_t)sqrt(r[0]) / (TS)s.nrmup;
     s.RelChgOut = (double)rr;
     uint32_t fl = s.flags & ~F_INNER_DONE;
     if (rr < (TS)F.O.tol)
@@ -1253,11 +1270,11 @@ __global__ __launch_bounds__(kFin) void fin_mlp(FA F) {
   fin_sums<4>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     s.f = s.f + (s.lam4 + s.lam5);
-    float lam4 = (float)r[0];
-    lam4 += (float)r[1];
-    if (z3) lam4 += (float)r[2];
+    vec_t lam4 = (vec_t)r[0];
+    lam4 += (vec_t)r[1];
+    if (z3) lam4 += (vec_t)r[2];
     s.lam4 = lam4;
-    s.lam5 = (float)r[3];
+    s.lam5 = (vec_t)r[3];
     s.f = s.f - (s.lam4 + s.lam5);
     const TS beta0 = (TS)s.beta;
     TS beta = beta0 * (TS)F.O.rate_cnt;
@@ -1269,7 +1286,7 @@ __global__ __launch_bounds__(kFin) void fin_mlp(FA F) {
     s.mu = (double)mu;
     s.muDbeta = (double)(mu / beta);
     s.f = f_full<TS>(s);
-    s.ratio = (double)(float)(beta0 / beta);
+    s.ratio = (double)(vec_t)(beta0 / beta);
     s.gam = (double)(TS)F.O.gam;
     s.Q = 1.0;
     s.C = s.f;
@@ -1285,7 +1302,7 @@ __global__ __launch_bounds__(kFin) void fin_mlp(FA F) {
 // the flags
 __global__ __launch_bounds__(kFin) void fin_refresh(FA F) {
   if (threadIdx.x != 0) return;
-  InstScal &s = F.S[blockIdx.x];
+  Scal &s = F.S[blockIdx.x];
   s.flags &= ~kStepFlags;
   put_flags(F, blockIdx.x, s.flags);
 }
@@ -1297,14 +1314,14 @@ __global__ __launch_bounds__(kFin) void fin_relerr(FA F) {
   double r[2];
   fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0)
-    s.rel_error = (double)((TS)(float)sqrt(r[0]) / (TS)(float)sqrt(r[1]));
+    s.rel_error = (double)((TS)(vec_t)sqrt(r[0]) / (TS)(vec_t)sqrt(r[1]));
   FIN_END
 }
 
-__global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
+__global__ void k_init_scal(Scal *S, int B, int Bpad, uint32_t *fout) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Bpad) return;
-  InstScal z;
+  Scal z;
   memset(&z, 0, sizeof(z));
   z.flags = i < B ? (F_LIVE | F_ACTIVE | F_FST) : 0u;
   z.alpha = 1.0;
@@ -1324,7 +1341,7 @@ __global__ void k_init_scal(InstScal *S, int B, int Bpad, uint32_t *fout) {
 // masked A / A^T product selects only a few lanes (the 5-strike fallback
 // usually does), their vectors are gathered out of the interleaved layout,
 // multiplied by the W = 1 kernels and scattered back.  Every walk sum is an
-// FP64 sum of float values in both layouts, so the results are the same.
+// FP64 sum of vec_t values in both layouts, so the results are the same.
 // Up to lane_limit() lanes (default 16, TV3_LANE_MAX=k sets another up to
 // 64, 0 turns it off) go to the W = 1 kernels.  More lanes, when they fit in
 // fewer 64-lane groups than the batch has, are packed into compact 64-lane
@@ -1350,8 +1367,8 @@ bool compact64_on() {
 }
 
 // xc[k * pitch + e] = x[off + e * 64 + lane] for the k-th listed instance
-__global__ __launch_bounds__(kBlock) void k_lane_gather(const GroupDesc *G, const float *x,
-                                                        float *xc, LaneList L, int len,
+__global__ __launch_bounds__(kBlock) void k_lane_gather(const GroupDesc *G, const vec_t *x,
+                                                        vec_t *xc, LaneList L, int len,
                                                         int use_m, long long pitch) {
   const int k = blockIdx.y;
   if (k >= L.n) return;
@@ -1365,10 +1382,10 @@ __global__ __launch_bounds__(kBlock) void k_lane_gather(const GroupDesc *G, cons
 
 // y[off + e * 64 + lane] = yc[k * pitch + e] (- sub) for listed instances
 // whose flags pass the launch's mask
-__global__ __launch_bounds__(kBlock) void k_lane_scatter(const GroupDesc *G, const InstScal *S,
-                                                         const float *yc, float *y, LaneList L,
+__global__ __launch_bounds__(kBlock) void k_lane_scatter(const GroupDesc *G, const Scal *S,
+                                                         const vec_t *yc, vec_t *y, LaneList L,
                                                          int len, int use_m, long long pitch,
-                                                         uint32_t mask, const float *sub) {
+                                                         uint32_t mask, const vec_t *sub) {
   const int k = blockIdx.y;
   if (k >= L.n) return;
   const int inst = L.inst[k];
@@ -1378,22 +1395,22 @@ __global__ __launch_bounds__(kBlock) void k_lane_scatter(const GroupDesc *G, con
   const long long off = use_m ? gd.off_m : gd.off_n;
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < len; e += gridDim.x * kBlock) {
     const long long o = off + (long long)e * 64 + b;
-    const float v = yc[k * pitch + e];
+    const vec_t v = yc[k * pitch + e];
     y[o] = sub ? v - sub[o] : v;
   }
 }
 
 // compact 64-lane layout: xc[g * pitch + v * 64 + l] = x of listed instance
 // 64 g + l at voxel (row) v; unlisted lanes of the last group get 0
-__global__ __launch_bounds__(kBlock) void k_lane_gather64(const GroupDesc *G, const float *x,
-                                                          float *xc, LaneList L, int len,
+__global__ __launch_bounds__(kBlock) void k_lane_gather64(const GroupDesc *G, const vec_t *x,
+                                                          vec_t *xc, LaneList L, int len,
                                                           int use_m, long long pitch) {
   const int g = blockIdx.y;
   const long long E = (long long)len * 64;
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
        e += (long long)gridDim.x * kBlock) {
     const int k = g * 64 + (int)(e & 63);
-    float v = 0.0f;
+    vec_t v = 0.0f;
     if (k < L.n) {
       const int inst = L.inst[k];
       const GroupDesc &gd = G[inst / 64];
@@ -1404,10 +1421,10 @@ __global__ __launch_bounds__(kBlock) void k_lane_gather64(const GroupDesc *G, co
 }
 
 // y of listed instances whose flags pass the mask = compact result (- sub)
-__global__ __launch_bounds__(kBlock) void k_lane_scatter64(const GroupDesc *G, const InstScal *S,
-                                                           const float *yc, float *y, LaneList L,
+__global__ __launch_bounds__(kBlock) void k_lane_scatter64(const GroupDesc *G, const Scal *S,
+                                                           const vec_t *yc, vec_t *y, LaneList L,
                                                            int len, int use_m, long long pitch,
-                                                           uint32_t mask, const float *sub) {
+                                                           uint32_t mask, const vec_t *sub) {
   const int g = blockIdx.y;
   const long long E = (long long)len * 64;
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
@@ -1418,7 +1435,7 @@ __global__ __launch_bounds__(kBlock) void k_lane_scatter64(const GroupDesc *G, c
     if (mask && (S[inst].flags & mask) != mask) continue;
     const GroupDesc &gd = G[inst / 64];
     const long long o = (use_m ? gd.off_m : gd.off_n) + (e & ~63LL) + inst % 64;
-    const float v = yc[g * pitch + e];
+    const vec_t v = yc[g * pitch + e];
     y[o] = sub ? v - sub[o] : v;
   }
 }
@@ -1495,13 +1512,13 @@ class Engine {
     const int nn_arrays = bt_.z3 ? 30 : 25;
     const int nm_arrays = 7;
     size_t need = 0;
-    need += align_up(sizeof(float) * bt_.Nn) * nn_arrays;
-    need += align_up(sizeof(float) * bt_.Nm) * nm_arrays;
+    need += align_up(sizeof(vec_t) * bt_.Nn) * nn_arrays;
+    need += align_up(sizeof(vec_t) * bt_.Nm) * nm_arrays;
     need += align_up(sizeof(double) * (size_t)bt_.Bpad * kNslot * kNblkMax);
     need += align_up(sizeof(MinMax) * (size_t)bt_.Bpad * kNblkMax);
-    need += align_up(sizeof(InstScal) * bt_.Bpad);
+    need += align_up(sizeof(Scal) * bt_.Bpad);
     need += align_up(sizeof(GroupDesc) * bt_.G);
-    need += align_up(sizeof(float *) * bt_.Bpad) * 2;
+    need += align_up(sizeof(vec_t *) * bt_.Bpad) * 2;
     if (bt_.kind == kDense && W == 1)
       need += align_up(sizeof(double) * scratch_doubles());
     lane_max_ = lane_limit();
@@ -1511,9 +1528,9 @@ class Engine {
     pitch64_ = (long long)std::max(bt_.max_n, bt_.max_m) * 64;
     if (compact_ok_) {
       need += align_up(sizeof(GroupDesc) * std::max(lane_max_, 1));
-      need += align_up(sizeof(float) * (size_t)lane_max_ * pitch_) * 2;
+      need += align_up(sizeof(vec_t) * (size_t)lane_max_ * pitch_) * 2;
       need += align_up(sizeof(GroupDesc) * std::max(gc64_max_, 1));
-      need += align_up(sizeof(float) * (size_t)gc64_max_ * pitch64_) * 2;
+      need += align_up(sizeof(vec_t) * (size_t)gc64_max_ * pitch64_) * 2;
     }
     if (ctx_->ws_bytes < need) {
       if (ctx_->ws) {
@@ -1528,42 +1545,42 @@ class Engine {
     }
     char *p = (char *)ctx_->ws;
     auto take = [&](size_t bytes) { char *r = p; p += align_up(bytes); return r; };
-    const size_t fn = sizeof(float) * bt_.Nn, fm = sizeof(float) * bt_.Nm;
-    float **nptrs[] = {&a_.U, &a_.Up, &a_.Ux, &a_.Uxp, &a_.Uy, &a_.Uyp,
+    const size_t fn = sizeof(vec_t) * bt_.Nn, fm = sizeof(vec_t) * bt_.Nm;
+    vec_t **nptrs[] = {&a_.U, &a_.Up, &a_.Ux, &a_.Uxp, &a_.Uy, &a_.Uyp,
                        &a_.g, &a_.gp, &a_.g2, &a_.g2p, &a_.dU, &a_.dUx,
                        &a_.dUy, &a_.dg, &a_.dg2, &a_.d, &a_.DtsAtd, &a_.Wx,
                        &a_.Wy, &a_.sx, &a_.sy, &a_.Atb};
-    for (float **q : nptrs) *q = (float *)take(fn);
+    for (vec_t **q : nptrs) *q = (vec_t *)take(fn);
     if (bt_.z3) {
-      float **zp[] = {&a_.Uz, &a_.Uzp, &a_.dUz, &a_.Wz, &a_.sz};
-      for (float **q : zp) *q = (float *)take(fn);
+      vec_t **zp[] = {&a_.Uz, &a_.Uzp, &a_.dUz, &a_.Wz, &a_.sz};
+      for (vec_t **q : zp) *q = (vec_t *)take(fn);
     } else {
       a_.Uz = a_.Uzp = a_.dUz = a_.Wz = a_.sz = nullptr;
     }
-    a_.tmpn = (float *)take(fn);
-    a_.rcdU = (float *)take(fn);
-    a_.Uout = (float *)take(fn);
-    float **mptrs[] = {&a_.b, &a_.Au, &a_.Aup, &a_.dAu, &a_.delta, &a_.lastAu, &a_.tmpm};
-    for (float **q : mptrs) *q = (float *)take(fm);
+    a_.tmpn = (vec_t *)take(fn);
+    a_.rcdU = (vec_t *)take(fn);
+    a_.Uout = (vec_t *)take(fn);
+    vec_t **mptrs[] = {&a_.b, &a_.Au, &a_.Aup, &a_.dAu, &a_.delta, &a_.lastAu, &a_.tmpm};
+    for (vec_t **q : mptrs) *q = (vec_t *)take(fm);
     part_ = (double *)take(sizeof(double) * (size_t)bt_.Bpad * kNslot * kNblkMax);
     mm_ = (MinMax *)take(sizeof(MinMax) * (size_t)bt_.Bpad * kNblkMax);
-    S_ = (InstScal *)take(sizeof(InstScal) * bt_.Bpad);
+    S_ = (Scal *)take(sizeof(Scal) * bt_.Bpad);
     G_ = (GroupDesc *)take(sizeof(GroupDesc) * bt_.G);
-    ptrs_in_ = (const float **)take(sizeof(float *) * bt_.Bpad);
-    ptrs_out_ = (float **)take(sizeof(float *) * bt_.Bpad);
-    scratch_ = (bt_.kind == kDense && W == 1) ? (float *)take(sizeof(double) * scratch_doubles()) : nullptr;
+    ptrs_in_ = (const vec_t **)take(sizeof(vec_t *) * bt_.Bpad);
+    ptrs_out_ = (vec_t **)take(sizeof(vec_t *) * bt_.Bpad);
+    scratch_ = (bt_.kind == kDense && W == 1) ? (vec_t *)take(sizeof(double) * scratch_doubles()) : nullptr;
     if (compact_ok_) {
       Gc_ = (GroupDesc *)take(sizeof(GroupDesc) * std::max(lane_max_, 1));
-      xc_ = (float *)take(sizeof(float) * (size_t)lane_max_ * pitch_);
-      yc_ = (float *)take(sizeof(float) * (size_t)lane_max_ * pitch_);
+      xc_ = (vec_t *)take(sizeof(vec_t) * (size_t)lane_max_ * pitch_);
+      yc_ = (vec_t *)take(sizeof(vec_t) * (size_t)lane_max_ * pitch_);
       gc_host_.assign(lane_max_, bt_.groups[0]);
       for (int k = 0; k < lane_max_; ++k) {
         gc_host_[k].off_n = k * pitch_;
         gc_host_[k].off_m = k * pitch_;
       }
       G64c_ = (GroupDesc *)take(sizeof(GroupDesc) * std::max(gc64_max_, 1));
-      x64c_ = (float *)take(sizeof(float) * (size_t)gc64_max_ * pitch64_);
-      y64c_ = (float *)take(sizeof(float) * (size_t)gc64_max_ * pitch64_);
+      x64c_ = (vec_t *)take(sizeof(vec_t) * (size_t)gc64_max_ * pitch64_);
+      y64c_ = (vec_t *)take(sizeof(vec_t) * (size_t)gc64_max_ * pitch64_);
       g64_host_.assign(gc64_max_, bt_.groups[0]);
       for (int k = 0; k < gc64_max_; ++k) {
         g64_host_[k].off_n = k * pitch64_;
@@ -1620,7 +1637,8 @@ class Engine {
 
   // picks the dense launch shape for this batch (dense, W = 1 only)
   int tune_dense() {
-    if (!(W == 1 && bt_.kind == kDense)) return TV3_OK;
+    // the FP64 kernels have one launch shape
+    if (!(W == 1 && bt_.kind == kDense) || kVecF64) return TV3_OK;
     DenseTune best{2, 2, 8};
     if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
     long long work = 0;
@@ -1635,7 +1653,7 @@ class Engine {
     for (int r : {1, 2, 4}) {
       dense_tune_set(DenseTune{r, best.unroll, best.cols});
       const float t = time_launches([&] {
-        launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, a_.Atb, a_.tmpm, S_, 0);
+        launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, a_.Atb, a_.tmpm, flags_of(S_), 0);
       });
       if (t < tbest) { tbest = t; best.rows = r; }
     }
@@ -1644,7 +1662,7 @@ class Engine {
       for (int u : {1, 2, 4, 8}) {
         dense_tune_set(DenseTune{best.rows, u, c});
         const float t = time_launches([&] {
-          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, S_, 0, nullptr, scratch_,
+          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, flags_of(S_), 0, nullptr, scratch_,
                         bt_.layered ? 0 : submat_mb_);
         });
         if (t < tbest) { tbest = t; best.unroll = u; best.cols = c; }
@@ -1658,7 +1676,7 @@ class Engine {
     return TV3_OK;
   }
 
-  int run(const float *const *b, float *const *U, const float *const *Ut,
+  int run(const vec_t *const *b, vec_t *const *U, const vec_t *const *Ut,
           tv3_info *infos, tv3_trace_rec *trace, int trace_cap) {
     int rc = setup();
     if (rc) return rc;
@@ -1690,11 +1708,11 @@ class Engine {
   Arrs a_{};
   double *part_ = nullptr;
   MinMax *mm_ = nullptr;
-  InstScal *S_ = nullptr;
+  Scal *S_ = nullptr;
   GroupDesc *G_ = nullptr;
-  const float **ptrs_in_ = nullptr;
-  float **ptrs_out_ = nullptr;
-  float *scratch_ = nullptr;
+  const vec_t **ptrs_in_ = nullptr;
+  vec_t **ptrs_out_ = nullptr;
+  vec_t *scratch_ = nullptr;
   // graph mode (tv3_options::graph) and capture state
   bool graph_ = false, capturing_ = false;
   int submat_mb_ = 5;  // row blocks of the dense get_g product
@@ -1704,7 +1722,7 @@ class Engine {
   int ncg_ = 0, gc64_max_ = 0;  // compact 64-lane groups in use / available
   long long pitch64_ = 0;
   GroupDesc *G64c_ = nullptr;
-  float *x64c_ = nullptr, *y64c_ = nullptr;
+  vec_t *x64c_ = nullptr, *y64c_ = nullptr;
   std::vector<GroupDesc> g64_host_;
   int lane_max_ = 0;
   // the host copy of the flags is current: read_summary ran after the last
@@ -1712,7 +1730,7 @@ class Engine {
   bool flags_fresh_ = false;
   LaneList lanes_{};
   GroupDesc *Gc_ = nullptr;
-  float *xc_ = nullptr, *yc_ = nullptr;
+  vec_t *xc_ = nullptr, *yc_ = nullptr;
   long long pitch_ = 0;
   std::vector<GroupDesc> gc_host_;
   tv3_trace_rec *trace_ = nullptr;
@@ -1738,66 +1756,71 @@ class Engine {
   }
   dim3 gi() const { return dim3(bt_.Bpad); }
 
-  void fwd(const float *x, float *y, uint32_t mask) {
-    if (cmode_ == 2) {
-      const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
-      const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
-      hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_n, 0, pitch64_);
-      prof_begin(ctx_, 0);
-      launch_fwd<64>(s_, G64c_, ncg_, bt_.max_m, bt_.kind, bt_.dtype, x64c_, y64c_, S_, 0);
-      prof_end(ctx_);
-      prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 0);
-      hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const float *)nullptr);
-      return;
-    }
-    if (cmode_ == 1) {
-      const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
-      const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
-      hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
-      launch_fwd<1>(s_, Gc_, lanes_.n, bt_.max_m, bt_.kind, bt_.dtype, xc_, yc_, S_, 0);
-      hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const float *)nullptr);
-      return;
+  void fwd(const vec_t *x, vec_t *y, uint32_t mask) {
+    // lane compaction exists only in 64-lane engines (cmode_ stays 0 else)
+    if constexpr (W == 64) {
+      if (cmode_ == 2) {
+        const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
+        const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
+        hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_n, 0, pitch64_);
+        prof_begin(ctx_, 0);
+        launch_fwd<64>(s_, G64c_, ncg_, bt_.max_m, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0);
+        prof_end(ctx_);
+        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
+        hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const vec_t *)nullptr);
+        return;
+      }
+      if (cmode_ == 1) {
+        const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
+        const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
+        hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
+        launch_fwd<1>(s_, Gc_, lanes_.n, bt_.max_m, bt_.kind, bt_.dtype, xc_, yc_, flags_of(S_), 0);
+        hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const vec_t *)nullptr);
+        return;
+      }
     }
     if (capturing_) {
-      launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
+      launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask);
       return;
     }
     prof_begin(ctx_, 0);
-    launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, S_, mask);
+    launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask);
     prof_end(ctx_);
-    prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 0);
+    prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
   }
   // blocked: the get_g product (dense A: row blocks of submat_mb)
-  void bwd(const float *x, float *y, uint32_t mask, const float *sub, bool blocked = false) {
+  void bwd(const vec_t *x, vec_t *y, uint32_t mask, const vec_t *sub, bool blocked = false) {
     // several layers: two plain products (2d/tval3_cpu_2d.cpp:365, 390-393)
     const int smb = (blocked && !bt_.layered) ? submat_mb_ : 0;
-    if (cmode_ == 2) {
-      const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
-      const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
-      hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_m, 1, pitch64_);
-      prof_begin(ctx_, 1);
-      launch_bwd<64>(s_, G64c_, ncg_, bt_.max_n, bt_.kind, bt_.dtype, x64c_, y64c_, S_, 0, nullptr, nullptr, 0);
-      prof_end(ctx_);
-      prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 1);
-      hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
-      return;
-    }
-    if (cmode_ == 1) {
-      const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
-      const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
-      hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
-      launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, S_, 0, nullptr, nullptr, 0);
-      hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
-      return;
+    if constexpr (W == 64) {
+      if (cmode_ == 2) {
+        const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
+        const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
+        hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_m, 1, pitch64_);
+        prof_begin(ctx_, 1);
+        launch_bwd<64>(s_, G64c_, ncg_, bt_.max_n, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0, nullptr, nullptr, 0);
+        prof_end(ctx_);
+        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
+        hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
+        return;
+      }
+      if (cmode_ == 1) {
+        const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
+        const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
+        hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
+        launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, flags_of(S_), 0, nullptr, nullptr, 0);
+        hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
+        return;
+      }
     }
     if (capturing_) {
-      launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_, smb);
+      launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask, sub, scratch_, smb);
       return;
     }
     prof_begin(ctx_, 1);
-    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, S_, mask, sub, scratch_, smb);
+    launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask, sub, scratch_, smb);
     prof_end(ctx_);
-    prof_count(ctx_, G_, W, bt_.Bpad, S_, mask, 1);
+    prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
   }
 
   // instances whose published flags (last read_summary) contain all bits of
@@ -1995,16 +2018,16 @@ class Engine {
     return rc;
   }
 
-  int solve(const float *const *b, float *const *U, const float *const *Ut,
+  int solve(const vec_t *const *b, vec_t *const *U, const vec_t *const *Ut,
             tv3_info *infos) {
     int rc;
     const auto t0 = std::chrono::steady_clock::now();
     flags_fresh_ = false;
     // instance scalars and caller pointers
     hipLaunchKernelGGL(k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad, ctx_->dflags);
-    std::vector<const float *> hin(bt_.Bpad, nullptr);
+    std::vector<const vec_t *> hin(bt_.Bpad, nullptr);
     for (int i = 0; i < bt_.B; ++i) hin[i] = b[i];
-    rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hin.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
+    rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hin.data(), sizeof(vec_t *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
     if (rc) return rc;
     const dim3 gp_(std::max(1, std::min(nblk_m_, 1024)), bt_.G);
     hipLaunchKernelGGL(k_pack<W>, gp_, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.b, 1);
@@ -2034,26 +2057,26 @@ class Engine {
       if ((rc = run_graph(sum))) return rc;
     }
     // outputs, rel_error
-    std::vector<float *> hout(bt_.Bpad, nullptr);
+    std::vector<vec_t *> hout(bt_.Bpad, nullptr);
     for (int i = 0; i < bt_.B; ++i) hout[i] = U[i];
-    rc = hip_check(ctx_, hipMemcpyAsync(ptrs_out_, hout.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
+    rc = hip_check(ctx_, hipMemcpyAsync(ptrs_out_, hout.data(), sizeof(vec_t *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
     if (rc) return rc;
     const dim3 gpn(std::max(1, std::min(nblk_n_, 1024)), bt_.G);
     hipLaunchKernelGGL(k_unpack<W>, gpn, dim3(kBlock), 0, s_, G_, a_.Uout, ptrs_out_);
     bool any_ut = false;
-    std::vector<const float *> hut(bt_.Bpad, nullptr);  // alive until the sync below
+    std::vector<const vec_t *> hut(bt_.Bpad, nullptr);  // alive until the sync below
     if (Ut) {
       for (int i = 0; i < bt_.B; ++i) { hut[i] = Ut[i]; any_ut |= Ut[i] != nullptr; }
       if (any_ut) {
-        rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hut.data(), sizeof(float *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
+        rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hut.data(), sizeof(vec_t *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
         if (rc) return rc;
         hipLaunchKernelGGL(k_pack<W>, gpn, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.rcdU, 0);
         hipLaunchKernelGGL(k_relerr<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.rcdU, a_.Uout);
         hipLaunchKernelGGL(fin_relerr<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
       }
     }
-    std::vector<InstScal> hs(bt_.Bpad);
-    rc = hip_check(ctx_, hipMemcpyAsync(hs.data(), S_, sizeof(InstScal) * bt_.Bpad, hipMemcpyDeviceToHost, s_), "scalar copy");
+    std::vector<Scal> hs(bt_.Bpad);
+    rc = hip_check(ctx_, hipMemcpyAsync(hs.data(), S_, sizeof(Scal) * bt_.Bpad, hipMemcpyDeviceToHost, s_), "scalar copy");
     if (!rc) rc = hip_check(ctx_, hipStreamSynchronize(s_), "solve sync");
     if (rc) return rc;
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2078,8 +2101,8 @@ int check_opts(tv3_ctx *ctx, const tv3_options *o) {
 
 int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
                      const tv3_options *o, const int *dims,
-                     const float *const *b, float *const *U,
-                     const float *const *Ut, tv3_info *infos,
+                     const vec_t *const *b, vec_t *const *U,
+                     const vec_t *const *Ut, tv3_info *infos,
                      tv3_trace_rec *trace, int trace_cap) {
   if (!ctx) return TV3_ERR_ARG;
   if (B <= 0 || !A || !dims || !b || !U || !infos)
@@ -2107,6 +2130,10 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
       layers[i] = (int)(nu / na);
       bt.layered = true;
     }
+    if (!(kVecF64 ? plan_takes_f64(A[i]->dev) : plan_takes_f32(A[i]->dev)))
+      return set_err(ctx, TV3_ERR_ARG,
+                     kVecF64 ? "Argument error: tv3_solve_f64 needs an f64 dense, double sparse or geometry plan"
+                             : "Argument error: an f64 / double-valued plan needs tv3_solve_f64");
     if (A[i]->dev.kind != A[0]->dev.kind ||
         (A[i]->dev.kind == kDense && A[i]->dev.dtype != A[0]->dev.dtype))
       return set_err(ctx, TV3_ERR_ARG, "Argument error: one batch needs one operator kind and element type");
@@ -2124,8 +2151,9 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
   // gather per voxel for 64 instances).  TV3_GROUP_WIDTH=16 selects 16-lane
   // groups dealt one per XCD (measured slower on the 256x256 ring).
   // Multi-layer instances run in W = 1 groups.
-  bt.W = (shared && B > 1 && !bt.layered) ? 64 : 1;
-  if (const char *ev = getenv("TV3_GROUP_WIDTH")) {
+  // FP64 vectors: W = 1 groups only.
+  bt.W = (shared && B > 1 && !bt.layered && !kVecF64) ? 64 : 1;
+  if (const char *ev = kVecF64 ? nullptr : getenv("TV3_GROUP_WIDTH")) {
     const int w = atoi(ev);
     if (shared && B > 1 && !bt.layered && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense))
       bt.W = w;
@@ -2151,7 +2179,7 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
       // blocked get_g product: groups with up to 64 row blocks split each
       // block into row slices (about 16 or more chunks in all)
       const int smb = o->submat_mb > 0 ? o->submat_mb : 5;
-      gd.dblk_rows = dense_block_rows(smb, gd.n);
+      gd.dblk_rows = dense_block_rows(smb, gd.n, (int)sizeof(vec_t));
       const long long nb = ((long long)gd.m + gd.dblk_rows - 1) / gd.dblk_rows;
       if (nb <= 64) {
         gd.dblk_S = (int)std::max<long long>(1, 16 / std::max<long long>(nb, 1));
@@ -2163,12 +2191,17 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
   }
   bt.Nn = std::max<long long>(on, 1);
   bt.Nm = std::max<long long>(om, 1);
+#ifdef TV3_SOLVER_F64
+  (void)W;
+  Engine<1, double> e(ctx, bt, *o);
+  return e.run(b, U, Ut, infos, trace, trace_cap);
+#else
   if (W == 64) {
     if (o->scal_double) {
       Engine<64, double> e(ctx, bt, *o);
       return e.run(b, U, Ut, infos, trace, trace_cap);
     }
-    Engine<64, float> e(ctx, bt, *o);
+    Engine<64, vec_t> e(ctx, bt, *o);
     return e.run(b, U, Ut, infos, trace, trace_cap);
   }
   if (W == 16) {
@@ -2176,17 +2209,19 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
       Engine<16, double> e(ctx, bt, *o);
       return e.run(b, U, Ut, infos, trace, trace_cap);
     }
-    Engine<16, float> e(ctx, bt, *o);
+    Engine<16, vec_t> e(ctx, bt, *o);
     return e.run(b, U, Ut, infos, trace, trace_cap);
   }
   if (o->scal_double) {
     Engine<1, double> e(ctx, bt, *o);
     return e.run(b, U, Ut, infos, trace, trace_cap);
   }
-  Engine<1, float> e(ctx, bt, *o);
+  Engine<1, vec_t> e(ctx, bt, *o);
   return e.run(b, U, Ut, infos, trace, trace_cap);
+#endif
 }
 
+#ifndef TV3_SOLVER_F64
 // ---------------------------------------------------------------------------
 // standalone operators (single volume, W = 1)
 // ---------------------------------------------------------------------------
@@ -2198,7 +2233,7 @@ __global__ __launch_bounds__(kBlock) void k_shrink_only(KA K, Arrs a, int iso) {
     NLOOP_BEGIN(1)
     const TS beta = (TS)S.beta, ib = (TS)1 / beta;
     const bool z3 = gd.r > 1;
-    float wx, wy, wz = 0.0f;
+    vec_t wx, wy, wz = 0.0f;
     TS t;
     if (iso) {
       t = shrink_iso<TS>(a.Ux[o], a.Uy[o], z3 ? a.Uz[o] : 0.0f, a.sx[o], a.sy[o],
@@ -2222,9 +2257,9 @@ __global__ __launch_bounds__(kBlock) void k_shrink_only(KA K, Arrs a, int iso) {
   block_partials<1, 1>(acc, sl, K.part, 0, blockIdx.x, K.S, K.req, K.forbid);
 }
 
-__global__ __launch_bounds__(kBlock) void k_Dt_only(KA K, const float *X,
-                                                    const float *Y,
-                                                    const float *Z, float *res,
+__global__ __launch_bounds__(kBlock) void k_Dt_only(KA K, const vec_t *X,
+                                                    const vec_t *Y,
+                                                    const vec_t *Z, vec_t *res,
                                                     int exact) {
   constexpr int W = 1;
   NLOOP_BEGIN(1)
@@ -2235,8 +2270,8 @@ __global__ __launch_bounds__(kBlock) void k_Dt_only(KA K, const float *X,
   LOOP_END
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot_only(KA K, const float *x,
-                                                     const float *y) {
+__global__ __launch_bounds__(kBlock) void k_dot_only(KA K, const vec_t *x,
+                                                     const vec_t *y) {
   double acc[1] = {0};
   {
     constexpr int W = 1;
@@ -2249,17 +2284,17 @@ __global__ __launch_bounds__(kBlock) void k_dot_only(KA K, const float *x,
   block_partials<1, 1>(acc, sl, K.part, 0, blockIdx.x, K.S, K.req, K.forbid);
 }
 
-// small single-volume scratch: one GroupDesc, one InstScal, partials
+// small single-volume scratch: one GroupDesc, one Scal, partials
 struct OpScratch {
   GroupDesc *G;
-  InstScal *S;
+  Scal *S;
   double *part;
   double *dense;
 };
 
 int op_scratch(tv3_ctx *ctx, const GroupDesc &gd, double beta, OpScratch &sc,
                long long dense_doubles) {
-  const size_t need = align_up(sizeof(GroupDesc)) + align_up(sizeof(InstScal)) +
+  const size_t need = align_up(sizeof(GroupDesc)) + align_up(sizeof(Scal)) +
                       align_up(sizeof(double) * kNslot * kNblkMax) +
                       align_up(sizeof(double) * std::max<long long>(dense_doubles, 1));
   if (ctx->ws_bytes < need) {
@@ -2270,15 +2305,15 @@ int op_scratch(tv3_ctx *ctx, const GroupDesc &gd, double beta, OpScratch &sc,
   }
   char *p = (char *)ctx->ws;
   sc.G = (GroupDesc *)p; p += align_up(sizeof(GroupDesc));
-  sc.S = (InstScal *)p; p += align_up(sizeof(InstScal));
+  sc.S = (Scal *)p; p += align_up(sizeof(Scal));
   sc.part = (double *)p; p += align_up(sizeof(double) * kNslot * kNblkMax);
   sc.dense = (double *)p;
-  InstScal s{};
+  Scal s{};
   s.flags = F_LIVE | F_ACTIVE;
   s.beta = beta;
   // the sources are stack variables: wait for the copies to finish
   int rc = hip_check(ctx, hipMemcpyAsync(sc.G, &gd, sizeof(GroupDesc), hipMemcpyHostToDevice, ctx->stream), "copy");
-  if (!rc) rc = hip_check(ctx, hipMemcpyAsync(sc.S, &s, sizeof(InstScal), hipMemcpyHostToDevice, ctx->stream), "copy");
+  if (!rc) rc = hip_check(ctx, hipMemcpyAsync(sc.S, &s, sizeof(Scal), hipMemcpyHostToDevice, ctx->stream), "copy");
   if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "copy sync");
   return rc;
 }
@@ -2304,12 +2339,75 @@ GroupDesc vol_desc(int q, int p, int r) {
   gd.n = q * p * r;
   return gd;
 }
+#endif  // !TV3_SOLVER_F64
 
 }  // namespace
 
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+#ifdef TV3_SOLVER_F64
+extern "C" {
+
+int tv3_solve_f64(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+                  int dim_x, int dim_y, int dim_z, const double *b, double *U,
+                  const double *Ut, tv3_info *info, tv3_trace_rec *trace,
+                  int trace_cap) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (!A || !b || !U || !info) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  const int dims[3] = {dim_x, dim_y, dim_z};
+  const tv3_plan *As[1] = {A};
+  const double *bs[1] = {b};
+  double *Us[1] = {U};
+  const double *Uts[1] = {Ut};
+  return solve_batch_impl(ctx, 1, As, opts, dims, bs, Us, Ut ? Uts : nullptr,
+                          info, trace, trace_cap);
+}
+
+int tv3_solve_batch_f64(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                        const tv3_options *opts, const int *dims,
+                        const double *const *b, double *const *U,
+                        const double *const *Ut, tv3_info *infos) {
+  return solve_batch_impl(ctx, B, A, opts, dims, b, U, Ut, infos, nullptr, 0);
+}
+
+int tv3_gemv_f64(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
+                 double *y) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (!A || !x || !y) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  if (!plan_takes_f64(A->dev))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: tv3_gemv_f64 needs an f64 dense, double sparse or geometry plan");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  GroupDesc gd{};
+  gd.q = A->dev.n; gd.p = 1; gd.r = 1;
+  gd.n = A->dev.n;
+  gd.m = A->dev.m;
+  gd.layers = 1;
+  gd.plan = A->dev;
+  // one group descriptor and, for dense A, the chunk sums of A^T x
+  const long long dense = A->dev.kind == kDense ? 16LL * A->dev.n : 1;
+  const size_t need = align_up(sizeof(GroupDesc)) + align_up(sizeof(double) * dense);
+  if (ctx->ws_bytes < need) {
+    if (ctx->ws) { (void)hipStreamSynchronize(ctx->stream); (void)hipFree(ctx->ws); ctx->ws = nullptr; ctx->ws_bytes = 0; }
+    if ((rc = hip_check(ctx, hipMalloc(&ctx->ws, need), "workspace hipMalloc"))) return rc;
+    ctx->ws_bytes = need;
+  }
+  GroupDesc *G = (GroupDesc *)ctx->ws;
+  double *scratch = (double *)((char *)ctx->ws + align_up(sizeof(GroupDesc)));
+  rc = hip_check(ctx, hipMemcpyAsync(G, &gd, sizeof(GroupDesc), hipMemcpyHostToDevice, ctx->stream), "copy");
+  // gd is a stack variable: wait for the copy
+  if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "copy sync");
+  if (rc) return rc;
+  if (trans)
+    launch_bwd<1>(ctx->stream, G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, nullptr, 0, nullptr, scratch, 0);
+  else
+    launch_fwd<1>(ctx->stream, G, 1, gd.m, A->dev.kind, A->dev.dtype, x, y, nullptr, 0);
+  return hip_check(ctx, hipGetLastError(), "gemv launch");
+}
+
+}  // extern "C"
+#else
 extern "C" {
 
 int tv3_solve(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
@@ -2338,6 +2436,8 @@ int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
              float *y) {
   if (!ctx) return TV3_ERR_ARG;
   if (!A || !x || !y) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  if (!plan_takes_f32(A->dev))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: an f64 / double-valued plan needs tv3_gemv_f64");
   int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if (rc) return rc;
   GroupDesc gd = vol_desc(A->dev.n, 1, 1);
@@ -2439,3 +2539,4 @@ int tv3_nrm2(tv3_ctx *ctx, long long n, const float *x, double *out) {
 }
 
 }  // extern "C"
+#endif  // TV3_SOLVER_F64

@@ -77,6 +77,10 @@ def lib():
         "tv3_nrm2": (_I, [_VP, _LL, _VP, C.POINTER(C.c_double)]),
         "tv3_solve": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
         "tv3_solve_batch": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+        "tv3_plan_sparse_f64": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
+        "tv3_gemv_f64": (_I, [_VP, _VP, _I, _VP, _VP]),
+        "tv3_solve_f64": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
+        "tv3_solve_batch_f64": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
         "tv3_mem_alloc": (_I, [_VP, C.c_size_t, C.POINTER(_VP)]),
         "tv3_mem_free": (_I, [_VP, _VP]),
         "tv3_mem_upload": (_I, [_VP, _VP, _VP, C.c_size_t]),
@@ -188,24 +192,31 @@ class Context:
         return x.contiguous()
 
     # ---- plans -----------------------------------------------------------
-    def plan_dense(self, A, bf16=False):
+    def plan_dense(self, A, bf16=False, f64=False):
+        """Dense A on the device: float32 (default), bfloat16 (bf16=True) or,
+        for the double-vector entry points, float64 (f64=True, mat<double>)."""
         t = self.torch
-        At = self.tensor(A, t.bfloat16 if bf16 else t.float32)
+        if bf16 and f64:
+            raise Tval3Error(1, "Argument error: bf16 and f64 exclude each other")
+        At = self.tensor(A, t.bfloat16 if bf16 else t.float64 if f64 else t.float32)
         m, n = At.shape
         h = _VP()
-        self._call(lib().tv3_plan_dense, self.h, m, n, 1 if bf16 else 0,
+        self._call(lib().tv3_plan_dense, self.h, m, n, 1 if bf16 else 2 if f64 else 0,
                                          _ptr(At), C.byref(h))
         return Plan(self, h, [At])
 
     def plan_sparse(self, m, n, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind,
-                    csc_val):
+                    csc_val, f64=False):
+        """CSR + CSC copies of one m x n matrix; f64=True: double values
+        (sparse_mat<double>, for the double-vector entry points)."""
         t = self.torch
+        vt = t.float64 if f64 else t.float32
         arrs = [self.tensor(csr_ptr, t.int32), self.tensor(csr_ind, t.int32),
-                self.tensor(csr_val, t.float32), self.tensor(csc_ptr, t.int32),
-                self.tensor(csc_ind, t.int32), self.tensor(csc_val, t.float32)]
+                self.tensor(csr_val, vt), self.tensor(csc_ptr, t.int32),
+                self.tensor(csc_ind, t.int32), self.tensor(csc_val, vt)]
         h = _VP()
-        self._call(lib().tv3_plan_sparse, self.h, m, n, int(arrs[1].numel()),
-                                          *[_ptr(a) for a in arrs], C.byref(h))
+        fn = lib().tv3_plan_sparse_f64 if f64 else lib().tv3_plan_sparse
+        self._call(fn, self.h, m, n, int(arrs[1].numel()), *[_ptr(a) for a in arrs], C.byref(h))
         return Plan(self, h, arrs)
 
     def plan_geometry(self, geo):
@@ -215,13 +226,15 @@ class Context:
         return Plan(self, h, [])
 
     # ---- operators ------------------------------------------------------
-    def gemv(self, plan, x, trans=False):
+    def gemv(self, plan, x, trans=False, f64=False):
+        """y = A x (A^T x with trans); f64=True: double vectors
+        (tv3_gemv_f64)."""
         t = self.torch
-        x = self.tensor(x, t.float32)
-        y = t.empty(plan.n if trans else plan.m, dtype=t.float32,
-                    device=self.device)
-        self._call(lib().tv3_gemv, self.h, plan.h, int(bool(trans)), _ptr(x),
-                                   _ptr(y))
+        vt = t.float64 if f64 else t.float32
+        x = self.tensor(x, vt)
+        y = t.empty(plan.n if trans else plan.m, dtype=vt, device=self.device)
+        fn = lib().tv3_gemv_f64 if f64 else lib().tv3_gemv
+        self._call(fn, self.h, plan.h, int(bool(trans)), _ptr(x), _ptr(y))
         return y
 
     def D(self, U, dims):
@@ -265,22 +278,25 @@ class Context:
         return out.value
 
     # ---- solver -----------------------------------------------------------
-    def solve(self, plan, b, dims, opts=None, Ut=None, trace_cap=0):
-        """tval3_cpu_2d/3d on the device: returns (U tensor, Info, [TraceRec])."""
+    def solve(self, plan, b, dims, opts=None, Ut=None, trace_cap=0, f64=False):
+        """tval3_cpu_2d/3d on the device: returns (U tensor, Info, [TraceRec]).
+        f64=True: the double/double solve (tv3_solve_f64, float64 tensors)."""
         t = self.torch
         if opts is None:
             opts = default_options()
         q, p, r = dims
-        b = self.tensor(b, t.float32)
+        vt = t.float64 if f64 else t.float32
+        b = self.tensor(b, vt)
         if b.numel() != plan.m * _layers(plan, dims):
             raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y * layers!")
-        U = t.empty(q * p * r, dtype=t.float32, device=self.device)
-        Utt = self.tensor(Ut, t.float32) if Ut is not None else None
+        U = t.empty(q * p * r, dtype=vt, device=self.device)
+        Utt = self.tensor(Ut, vt) if Ut is not None else None
         if Utt is not None and Utt.numel() != q * p * r:
             raise Tval3Error(1, "Argument error: U and Ut must have the same size!")
         info = Info()
         trace = (TraceRec * trace_cap)() if trace_cap else None
-        self._call(lib().tv3_solve, self.h, plan.h, C.byref(opts), q, p, r,
+        fn = lib().tv3_solve_f64 if f64 else lib().tv3_solve
+        self._call(fn, self.h, plan.h, C.byref(opts), q, p, r,
                                     _ptr(b), _ptr(U), _ptr(Utt), C.byref(info),
                                     trace, int(trace_cap))
         n_tr = min(info.total_iters, trace_cap)
@@ -316,16 +332,17 @@ class Context:
                    C.byref(dmu))
         return db, dmu.value
 
-    def solve_batch(self, plans, bs, dims, opts=None, Uts=None):
+    def solve_batch(self, plans, bs, dims, opts=None, Uts=None, f64=False):
         """B reconstructions in one batched launch sequence.  Returns
-        (list of U tensors, list of Info)."""
+        (list of U tensors, list of Info).  f64=True: tv3_solve_batch_f64."""
         t = self.torch
+        vt = t.float64 if f64 else t.float32
         if opts is None:
             opts = default_options()
         B = len(plans)
         if len(bs) != B or len(dims) != B:
             raise Tval3Error(1, "Argument error: plans, b and dims differ in length")
-        bts = [self.tensor(b, t.float32) for b in bs]
+        bts = [self.tensor(b, vt) for b in bs]
         for pl, x, d in zip(plans, bts, dims):
             if x.numel() != pl.m * _layers(pl, d):
                 raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y * layers!")
@@ -333,9 +350,9 @@ class Context:
             for d, u in zip(dims, Uts):
                 if u is not None and np.size(u) != d[0] * d[1] * d[2]:
                     raise Tval3Error(1, "Argument error: U and Ut must have the same size!")
-        Us = [t.empty(d[0] * d[1] * d[2], dtype=t.float32, device=self.device)
+        Us = [t.empty(d[0] * d[1] * d[2], dtype=vt, device=self.device)
               for d in dims]
-        Utts = ([self.tensor(u, t.float32) if u is not None else None for u in Uts]
+        Utts = ([self.tensor(u, vt) if u is not None else None for u in Uts]
                 if Uts is not None else None)
         parr = (_VP * B)(*[p.h for p in plans])
         dims_arr = (C.c_int * (3 * B))(*[int(v) for d in dims for v in d])
@@ -344,8 +361,8 @@ class Context:
         utarr = ((_VP * B)(*[(x.data_ptr() if x is not None else None) for x in Utts])
                  if Utts is not None else None)
         infos = (Info * B)()
-        self._call(lib().tv3_solve_batch, self.h, B, parr, C.byref(opts),
-                                          dims_arr, barr, uarr, utarr, infos)
+        fn = lib().tv3_solve_batch_f64 if f64 else lib().tv3_solve_batch
+        self._call(fn, self.h, B, parr, C.byref(opts), dims_arr, barr, uarr, utarr, infos)
         return Us, [infos[i] for i in range(B)]
 
 
