@@ -8,8 +8,10 @@
 // virtual_procs_per_core (MKL threads) is accepted and ignored.
 //
 // Supported element types: float vectors with float or double scalars
-// (T_comp = float, T_scal = float | double).  The double/double and complex
-// overloads exist so that callers link, and throw tval3_exception.
+// (T_comp = float, T_scal = float | double) and double/double (FP64 vectors
+// and scalars, through tv3_solve_f64; results agree with the reference's
+// double solve to FP64 rounding).  The complex overloads exist so that
+// callers link, and throw tval3_exception.
 // Multi-layer calls (b.len = layers * A.dim_y, U.len = layers * A.dim_x:
 // U's pixels hold the layers one after the other, every product applies A
 // to each layer, get_g takes two plain products) run for dense and sparse A,
@@ -68,9 +70,9 @@ inline void check_2d(const mat<Tc> &U, const geometry &A, const mat<Tc> &b,
     throw tval3_exception(std::string("Argument error: b.len must be equal to A.num_emitters * A.num_receivers!"));
 }
 
-template <class Ts>
-inline tval3_info<Ts> run_2d(mat<float> &U, const mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+template <class Ts, class Tc>
+inline tval3_info<Ts> run_2d(mat<Tc> &U, const mat<Tc> &A, const mat<Tc> &b,
+                             const tval3_options<Ts> &opts, const mat<Tc> &Ut,
                              int submat_mb) {
   const int layers = A.dim_y ? (int)(b.len / A.dim_y) : 0;
   check_2d(U, A, b, Ut, layers);
@@ -81,9 +83,9 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const mat<float> &A, const mat<float
       });
 }
 
-template <class Ts>
-inline tval3_info<Ts> run_2d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+template <class Ts, class Tc>
+inline tval3_info<Ts> run_2d(mat<Tc> &U, const sparse_mat<Tc> &A, const mat<Tc> &b,
+                             const tval3_options<Ts> &opts, const mat<Tc> &Ut,
                              int submat_mb) {
   const int layers = A.dim_y ? (int)(b.len / A.dim_y) : 0;
   check_2d(U, A, b, Ut, layers);
@@ -94,9 +96,9 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const sparse_mat<float> &A, const ma
       });
 }
 
-template <class Ts>
-inline tval3_info<Ts> run_2d(mat<float> &U, const geometry &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+template <class Ts, class Tc>
+inline tval3_info<Ts> run_2d(mat<Tc> &U, const geometry &A, const mat<Tc> &b,
+                             const tval3_options<Ts> &opts, const mat<Tc> &Ut,
                              int submat_mb) {
   check_2d(U, A, b, Ut, 1);
   const int E = (int)A.num_emitters, R = (int)A.num_receivers;
@@ -135,8 +137,8 @@ inline tval3_info<Ts> run_2d(mat<float> &U, const geometry &A, const mat<float> 
 
 [[noreturn]] inline void unsupported_types() {
   throw tval3_exception(std::string(
-      "Internal error: the GPU engine computes with float vectors (float or double scalars); "
-      "double and complex data are not supported"));
+      "Internal error: the GPU engine computes with float vectors (float or double scalars) "
+      "or double vectors; complex data are not supported"));
 }
 
 }  // namespace tval3_gpu_detail
@@ -180,13 +182,26 @@ inline const tval3_info<double> tval3_gpu_2d(mat<float> &U, const geometry &A, c
   (void)virtual_procs_per_core;
   return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
-// double / double, complex<float> / float, complex<double> / double
-template <class A_t>
-inline const tval3_info<double> tval3_gpu_2d(mat<double> &, const A_t &, const mat<double> &,
-                                             const tval3_options<double> &, const mat<double> &,
-                                             int = 5, int = 2) {
-  tval3_gpu_detail::unsupported_types();
+// double / double
+inline const tval3_info<double> tval3_gpu_2d(mat<double> &U, const mat<double> &A, const mat<double> &b,
+                                             const tval3_options<double> &opts, const mat<double> &Ut,
+                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
+inline const tval3_info<double> tval3_gpu_2d(mat<double> &U, const sparse_mat<double> &A, const mat<double> &b,
+                                             const tval3_options<double> &opts, const mat<double> &Ut,
+                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_2d(mat<double> &U, const geometry &A, const mat<double> &b,
+                                             const tval3_options<double> &opts, const mat<double> &Ut,
+                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
+}
+// complex<float> / float, complex<double> / double
 template <class A_t>
 inline const tval3_info<float> tval3_gpu_2d(mat<std::complex<float> > &, const A_t &,
                                             const mat<std::complex<float> > &,

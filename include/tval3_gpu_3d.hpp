@@ -9,8 +9,9 @@
 // U's dim_z selects the 3-D stencil; dim_z == 1 is the 2-D one.  The Dt
 // y-term of the reference (not the adjoint of D for dim_z >= 2) is kept.
 //
-// Supported element types: float vectors with float or double scalars.  The
-// double/double and complex overloads exist so that callers link, and throw.
+// Supported element types: float vectors with float or double scalars, and
+// double/double (FP64 vectors and scalars, through tv3_solve_f64).  The
+// complex overloads exist so that callers link, and throw.
 #ifndef TVAL3_GPU_3D_HPP_
 #define TVAL3_GPU_3D_HPP_
 
@@ -55,9 +56,9 @@ inline void check_3d(const mat<Tc> &U, const geometry &A, const mat<Tc> &b, cons
     throw tval3_exception(std::string("Argument error: b.len must be equal to A.num_emitters * A.num_receivers!"));
 }
 
-template <class Ts>
-inline tval3_info<Ts> run_3d(mat<float> &U, const mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+template <class Ts, class Tc>
+inline tval3_info<Ts> run_3d(mat<Tc> &U, const mat<Tc> &A, const mat<Tc> &b,
+                             const tval3_options<Ts> &opts, const mat<Tc> &Ut,
                              int submat_mb) {
   check_3d(U, A, b, Ut);
   return solve_host<tval3_exception, tval3_info<Ts>>(
@@ -67,9 +68,9 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const mat<float> &A, const mat<float
       });
 }
 
-template <class Ts>
-inline tval3_info<Ts> run_3d(mat<float> &U, const sparse_mat<float> &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+template <class Ts, class Tc>
+inline tval3_info<Ts> run_3d(mat<Tc> &U, const sparse_mat<Tc> &A, const mat<Tc> &b,
+                             const tval3_options<Ts> &opts, const mat<Tc> &Ut,
                              int submat_mb) {
   check_3d(U, A, b, Ut);
   return solve_host<tval3_exception, tval3_info<Ts>>(
@@ -79,9 +80,9 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const sparse_mat<float> &A, const ma
       });
 }
 
-template <class Ts>
-inline tval3_info<Ts> run_3d(mat<float> &U, const geometry &A, const mat<float> &b,
-                             const tval3_options<Ts> &opts, const mat<float> &Ut,
+template <class Ts, class Tc>
+inline tval3_info<Ts> run_3d(mat<Tc> &U, const geometry &A, const mat<Tc> &b,
+                             const tval3_options<Ts> &opts, const mat<Tc> &Ut,
                              int submat_mb) {
   check_3d(U, A, b, Ut);
   const int P = (int)A.numMPs, E = (int)A.num_emitters, R = (int)A.num_receivers;
@@ -125,8 +126,8 @@ inline tval3_info<Ts> run_3d(mat<float> &U, const geometry &A, const mat<float> 
 
 [[noreturn]] inline void unsupported_types_3d() {
   throw tval3_exception(std::string(
-      "Internal error: the GPU engine computes with float vectors (float or double scalars); "
-      "double and complex data are not supported"));
+      "Internal error: the GPU engine computes with float vectors (float or double scalars) "
+      "or double vectors; complex data are not supported"));
 }
 
 }  // namespace tval3_gpu_detail
@@ -168,12 +169,26 @@ inline const tval3_info<double> tval3_gpu_3d(mat<float> &U, const geometry &A, c
   (void)virtual_procs_per_core;
   return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
-template <class A_t>
-inline const tval3_info<double> tval3_gpu_3d(mat<double> &, const A_t &, const mat<double> &,
-                                             const tval3_options<double> &, const mat<double> &,
-                                             int = 5, int = 2) {
-  tval3_gpu_detail::unsupported_types_3d();
+// double / double
+inline const tval3_info<double> tval3_gpu_3d(mat<double> &U, const mat<double> &A, const mat<double> &b,
+                                             const tval3_options<double> &opts, const mat<double> &Ut,
+                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
+inline const tval3_info<double> tval3_gpu_3d(mat<double> &U, const sparse_mat<double> &A, const mat<double> &b,
+                                             const tval3_options<double> &opts, const mat<double> &Ut,
+                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_3d(mat<double> &U, const geometry &A, const mat<double> &b,
+                                             const tval3_options<double> &opts, const mat<double> &Ut,
+                                             int submat_mb = 5, int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
+}
+// complex<float> / float, complex<double> / double
 template <class A_t>
 inline const tval3_info<float> tval3_gpu_3d(mat<std::complex<float> > &, const A_t &,
                                             const mat<std::complex<float> > &,

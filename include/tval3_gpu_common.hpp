@@ -7,8 +7,9 @@
 // from tval3_types.h at 2d/tval3_cpu_2d.cpp:1-14 and 3d/tval3_cpu_3d.cpp:1-30).
 //
 // The overload layer takes the reference's host containers, stages them in
-// device memory through tv3_mem_*, builds a plan for A, runs tv3_solve on a
-// process-wide context and copies U back.  Status codes become
+// device memory through tv3_mem_*, builds a plan for A, runs tv3_solve (float
+// vectors) or tv3_solve_f64 (double/double) on a process-wide context and
+// copies U back.  Status codes become
 // tval3_exception with the reference's "Argument error: ..." /
 // "Internal error: ..." prefixes (2d/tval3_cpu_2d.cpp:696-750, 878-884).
 //
@@ -25,6 +26,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "tval3_amd.h"
@@ -164,11 +166,22 @@ struct KeepAll {
   }
 };
 
-// Runs one solve for host float data.  make_plan(ctx, PlanHolder&, KeepAll&)
-// builds the operator plan, keeping its device arrays in the KeepAll; dims
-// are U's (dim_x contiguous).
-template <class Exc, class Info, class Opts, class MatU, class MakePlan>
-inline Info solve_host(MatU &U, const MatU &b, const Opts &opts, const MatU &Ut,
+// the solve entry point for the vector type: float vectors (tv3_solve) or
+// the double/double solver (tv3_solve_f64)
+inline int solve_call(tv3_ctx *c, const tv3_plan *A, const tv3_options *o, int dx, int dy,
+                      int dz, const float *b, float *U, const float *Ut, tv3_info *info) {
+  return tv3_solve(c, A, o, dx, dy, dz, b, U, Ut, info, nullptr, 0);
+}
+inline int solve_call(tv3_ctx *c, const tv3_plan *A, const tv3_options *o, int dx, int dy,
+                      int dz, const double *b, double *U, const double *Ut, tv3_info *info) {
+  return tv3_solve_f64(c, A, o, dx, dy, dz, b, U, Ut, info, nullptr, 0);
+}
+
+// Runs one solve for host data of element type T (float or double).
+// make_plan(ctx, PlanHolder&, KeepAll&) builds the operator plan, keeping
+// its device arrays in the KeepAll; dims are U's (dim_x contiguous).
+template <class Exc, class Info, class Opts, class T, class MakePlan>
+inline Info solve_host(mat<T> &U, const mat<T> &b, const Opts &opts, const mat<T> &Ut,
                        int dx, int dy, int dz, int scal_double, int submat_mb,
                        MakePlan make_plan) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -179,24 +192,23 @@ inline Info solve_host(MatU &U, const MatU &b, const Opts &opts, const MatU &Ut,
   PlanHolder plan;
   make_plan(c, plan, keep);
   const size_t n = (size_t)U.len, m = (size_t)b.len;
-  DevBuf<Exc> *db = keep.put(c, &b.data()[0], m * sizeof(float));
-  DevBuf<Exc> dU(c, n * sizeof(float));
+  DevBuf<Exc> *db = keep.put(c, &b.data()[0], m * sizeof(T));
+  DevBuf<Exc> dU(c, n * sizeof(T));
   const bool has_ut = Ut.len > 0 && (size_t)Ut.len == n;
-  DevBuf<Exc> *dUt = has_ut ? keep.put(c, &Ut.data()[0], n * sizeof(float)) : nullptr;
+  DevBuf<Exc> *dUt = has_ut ? keep.put(c, &Ut.data()[0], n * sizeof(T)) : nullptr;
   const tv3_options co = to_c(opts, scal_double, s.ls_poly, submat_mb);
   tv3_info ci{};
-  const int rc = tv3_solve(c, plan.p, &co, dx, dy, dz, db->template as<float>(),
-                           dU.template as<float>(),
-                           dUt ? dUt->template as<float>() : nullptr, &ci, nullptr, 0);
+  const int rc = solve_call(c, plan.p, &co, dx, dy, dz, db->template as<T>(), dU.template as<T>(),
+                            dUt ? dUt->template as<T>() : nullptr, &ci);
   if (rc) fail<Exc>(c, rc);
-  dU.download(&U.data()[0], n * sizeof(float));
+  dU.download(&U.data()[0], n * sizeof(T));
   Info info;
   from_c(ci, info);
   info.secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return info;
 }
 
-// dense A (host, row-major float) -> device copy + plan
+// dense A (host, row-major float or double) -> device copy + plan
 template <class Exc>
 inline void dense_plan(tv3_ctx *c, PlanHolder &ph, KeepAll<Exc> &keep,
                        const float *A, int m, int n) {
@@ -204,21 +216,40 @@ inline void dense_plan(tv3_ctx *c, PlanHolder &ph, KeepAll<Exc> &keep,
   const int rc = tv3_plan_dense(c, m, n, TV3_F32, a->template as<float>(), &ph.p);
   if (rc) fail<Exc>(c, rc);
 }
+template <class Exc>
+inline void dense_plan(tv3_ctx *c, PlanHolder &ph, KeepAll<Exc> &keep,
+                       const double *A, int m, int n) {
+  DevBuf<Exc> *a = keep.put(c, A, (size_t)m * n * sizeof(double));
+  const int rc = tv3_plan_dense(c, m, n, TV3_F64, a->template as<double>(), &ph.p);
+  if (rc) fail<Exc>(c, rc);
+}
 
-// CSR + CSC (host, zero-based) -> device copies + plan
+inline int sparse_call(tv3_ctx *c, int m, int n, int nnz, const int *rp, const int *ri,
+                       const float *rv, const int *cp, const int *ci, const float *cv,
+                       tv3_plan **out) {
+  return tv3_plan_sparse(c, m, n, nnz, rp, ri, rv, cp, ci, cv, out);
+}
+inline int sparse_call(tv3_ctx *c, int m, int n, int nnz, const int *rp, const int *ri,
+                       const double *rv, const int *cp, const int *ci, const double *cv,
+                       tv3_plan **out) {
+  return tv3_plan_sparse_f64(c, m, n, nnz, rp, ri, rv, cp, ci, cv, out);
+}
+
+// CSR + CSC (host, zero-based, float or double values) -> device copies + plan
 template <class Exc, class SM>
 inline void sparse_plan(tv3_ctx *c, PlanHolder &ph, KeepAll<Exc> &keep, const SM &A) {
+  typedef typename std::decay<decltype(A.csr_val[0])>::type V;
   const int m = (int)A.dim_y, n = (int)A.dim_x;
   const int nnz = (int)A.csr_ptr[m];
   DevBuf<Exc> *rp = keep.put(c, &A.csr_ptr[0], sizeof(int) * (size_t)(m + 1));
   DevBuf<Exc> *ri = keep.put(c, &A.csr_ind[0], sizeof(int) * (size_t)nnz);
-  DevBuf<Exc> *rv = keep.put(c, &A.csr_val[0], sizeof(float) * (size_t)nnz);
+  DevBuf<Exc> *rv = keep.put(c, &A.csr_val[0], sizeof(V) * (size_t)nnz);
   DevBuf<Exc> *cp = keep.put(c, &A.csc_ptr[0], sizeof(int) * (size_t)(n + 1));
   DevBuf<Exc> *ci = keep.put(c, &A.csc_ind[0], sizeof(int) * (size_t)nnz);
-  DevBuf<Exc> *cv = keep.put(c, &A.csc_val[0], sizeof(float) * (size_t)nnz);
-  const int rc = tv3_plan_sparse(c, m, n, nnz, rp->template as<int>(), ri->template as<int>(),
-                                 rv->template as<float>(), cp->template as<int>(),
-                                 ci->template as<int>(), cv->template as<float>(), &ph.p);
+  DevBuf<Exc> *cv = keep.put(c, &A.csc_val[0], sizeof(V) * (size_t)nnz);
+  const int rc = sparse_call(c, m, n, nnz, rp->template as<int>(), ri->template as<int>(),
+                             rv->template as<V>(), cp->template as<int>(), ci->template as<int>(),
+                             cv->template as<V>(), &ph.p);
   if (rc) fail<Exc>(c, rc);
 }
 

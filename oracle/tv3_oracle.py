@@ -57,6 +57,7 @@ def lib():
         L.tv3o_gemv_f64.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
         L.tv3o_AtA_blocked.argtypes = [C.c_void_p, _FP, C.c_int, _FP, _FP]
         L.tv3o_set_threads.argtypes = [C.c_int]
+        L.tv3o_set_sum_chunk.argtypes = [C.c_longlong]
         L.tv3o_get_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -80,6 +81,12 @@ def set_threads(t):
 
 def get_threads():
     return lib().tv3o_get_threads()
+
+
+def set_sum_chunk(c):
+    """Chunk length of the FP64 reductions (default 4096): only the
+    summation order of dots and norms changes."""
+    lib().tv3o_set_sum_chunk(int(c))
 
 
 class Op:

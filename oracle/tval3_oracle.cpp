@@ -40,7 +40,9 @@
  *     with no contraction (build with -ffp-contract=off);
  *   - the shrinkage sum lam1 accumulates in FP64 and rounds to T_scal.
  * Reductions use fixed 4096-element chunks summed in chunk order, so results
- * do not depend on the OpenMP thread count.
+ * do not depend on the OpenMP thread count (tv3o_set_sum_chunk changes the
+ * chunk length: the tests use it to see how far a different FP64 summation
+ * order moves a double/double solve).
  *
  * How each part is pinned (tests/test_oracle.py, tests/test_walk_closed_form.py):
  *   Bresenham walk   golden walks from an independent transcription of the
@@ -84,13 +86,16 @@
 namespace {
 
 typedef long long i64;
-const i64 kChunk = 4096;
+// chunk length of the FP64 reductions (4096; tests change it only to measure
+// how much a different summation order moves a double/double solve)
+i64 g_chunk = 4096;
 
 /* ---------------------------------------------------------------------- */
 /* reductions                                                              */
 /* ---------------------------------------------------------------------- */
 template <class F>
 double chunked_sum(i64 n, F term) {
+  const i64 kChunk = g_chunk;
   const i64 nc = (n + kChunk - 1) / kChunk;
   std::vector<double> part(nc > 0 ? nc : 1, 0.0);
 #pragma omp parallel for schedule(static)
@@ -866,6 +871,10 @@ void tv3o_set_threads(int t) {
   (void)t;
 #endif
 }
+
+/* Chunk length of the FP64 reductions (default 4096; >= 1).  Changing it
+ * changes only the summation order of dot products and norms. */
+void tv3o_set_sum_chunk(long long c) { g_chunk = c >= 1 ? c : 1; }
 
 int tv3o_get_threads(void) {
 #ifdef _OPENMP

@@ -1,12 +1,14 @@
 // Test driver of tval3_gpu_2d (include/tval3_gpu_2d.hpp).
 //   shim_2d args                 argument checks only (no GPU use)
 //   shim_2d dense|sparse|geometry DIR   solve the problem stored in DIR with
-//                                float and double scalars; writes U_f.bin,
-//                                U_d.bin, info_f.txt, info_d.txt into DIR
+//                                float and double scalars and with the
+//                                double/double overload; writes U_f.bin,
+//                                U_d.bin, U_dd.bin and info_*.txt into DIR
 #include "ref_types_2d.h"
 #include "tval3_gpu_2d.hpp"
 #include "shim_io.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -36,9 +38,11 @@ static int args_mode() {
   expect_error("geometry format", [&] { tval3_gpu_2d(U, g, b, of, Ut); });
   mat<float> Ug(4, 4), Utg(4, 4);
   expect_error("geometry b length", [&] { tval3_gpu_2d(Ug, g, b, of, Utg); });
-  mat<double> Ud(4, 4), Ad(6, 16), bd(6), Utd(4, 4);
+  // the double/double overloads run the same checks
+  mat<double> Ud(4, 4), Ad(6, 16), bd(7), Utd(4, 4);
+  Ud.format = Utd.format = mat_col_major;
   tval3_options<double> od;
-  expect_error("double data", [&] { tval3_gpu_2d(Ud, Ad, bd, od, Utd); });
+  expect_error("double b length", [&] { tval3_gpu_2d(Ud, Ad, bd, od, Utd); });
   mat<std::complex<float> > Uc(4, 4), Ac(6, 16), bc(6), Utc(4, 4);
   expect_error("complex data", [&] { tval3_gpu_2d(Uc, Ac, bc, of, Utc); });
   return 0;
@@ -54,6 +58,38 @@ static void run(const std::string &dir, const char *tag, const A_t &A, const mat
   const tval3_info<Ts> info = tval3_gpu_2d(U, A, b, o, Ut);
   save(dir + "/U_" + tag + ".bin", U.data(), (size_t)U.len);
   save_info(dir + "/info_" + tag + ".txt", info);
+}
+
+static mat<double> to_double(const mat<float> &a) {
+  mat<double> d(a.dim_y, a.dim_x, a.dim_z);
+  d.format = a.format;
+  std::copy(a.buf.begin(), a.buf.end(), d.buf.begin());
+  return d;
+}
+
+static sparse_mat<double> to_double(const sparse_mat<float> &a) {
+  sparse_mat<double> d;
+  d.dim_x = a.dim_x; d.dim_y = a.dim_y; d.format = a.format;
+  d.csr_ptr = a.csr_ptr; d.csr_ind = a.csr_ind; d.csc_ptr = a.csc_ptr; d.csc_ind = a.csc_ind;
+  d.csr_val.assign(a.csr_val.begin(), a.csr_val.end());
+  d.csc_val.assign(a.csc_val.begin(), a.csc_val.end());
+  return d;
+}
+
+static const geometry &to_double(const geometry &g) { return g; }
+
+// the double/double overload: U_dd.bin holds doubles
+template <class A_t>
+static void run_dd(const std::string &dir, const A_t &A, const mat<float> &b, int dx, int dy,
+                   bool col_major, int maxit) {
+  mat<double> U(dy, dx), Ut(0);
+  U.format = Ut.format = col_major ? mat_col_major : mat_row_major;
+  const mat<double> bd = to_double(b);
+  tval3_options<double> o;
+  o.maxit = maxit;
+  const tval3_info<double> info = tval3_gpu_2d(U, to_double(A), bd, o, Ut);
+  save(dir + "/U_dd.bin", U.data(), (size_t)U.len);
+  save_info(dir + "/info_dd.txt", info);
 }
 
 int main(int argc, char **argv) {
@@ -75,6 +111,7 @@ int main(int argc, char **argv) {
       std::copy(Av.begin(), Av.end(), A.data());
       run<float>(dir, "f", A, b, dx, dy, true, maxit);
       run<double>(dir, "d", A, b, dx, dy, true, maxit);
+      run_dd(dir, A, b, dx, dy, true, maxit);
     } else if (kind == "sparse") {
       sparse_mat<float> S;
       S.dim_y = (int)meta["m"]; S.dim_x = (int)meta["n"];
@@ -83,6 +120,7 @@ int main(int argc, char **argv) {
       S.csc_ind = load<int>(dir + "/csc_ind.bin"); S.csc_val = load<float>(dir + "/csc_val.bin");
       run<float>(dir, "f", S, b, dx, dy, true, maxit);
       run<double>(dir, "d", S, b, dx, dy, true, maxit);
+      run_dd(dir, S, b, dx, dy, true, maxit);
     } else if (kind == "geometry") {
       std::vector<int> up = load<int>(dir + "/use_path.bin");
       std::vector<int> xe = load<int>(dir + "/xe.bin"), ye = load<int>(dir + "/ye.bin");
@@ -98,6 +136,7 @@ int main(int argc, char **argv) {
       g.scale_factor = (float)meta["scale"];
       run<float>(dir, "f", g, b, dx, dy, false, maxit);
       run<double>(dir, "d", g, b, dx, dy, false, maxit);
+      run_dd(dir, g, b, dx, dy, false, maxit);
     } else {
       std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
       return 2;

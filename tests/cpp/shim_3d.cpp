@@ -5,6 +5,7 @@
 #include "tval3_gpu_3d.hpp"
 #include "shim_io.h"
 
+#include <algorithm>
 #include <cstring>
 
 static int args_mode() {
@@ -27,9 +28,10 @@ static int args_mode() {
   geometry g;
   g.numMPs = 1; g.numPaths = 5;
   expect_error("geometry b length", [&] { tval3_gpu_3d(U, g, b, of, Ut); });
-  mat<double> Ud(4, 4, 4), Ad(6, 64), bd(6), Utd(4, 4, 4);
+  // the double/double overloads run the same checks
+  mat<double> Ud(4, 4, 4), Ad(6, 64), bd(7), Utd(4, 4, 4);
   tval3_options<double> od;
-  expect_error("double data", [&] { tval3_gpu_3d(Ud, Ad, bd, od, Utd); });
+  expect_error("double b length", [&] { tval3_gpu_3d(Ud, Ad, bd, od, Utd); });
   return 0;
 }
 
@@ -43,6 +45,38 @@ static void run(const std::string &dir, const char *tag, const A_t &A, const mat
   const tval3_info<Ts> info = tval3_gpu_3d(U, A, b, o, Ut);
   save(dir + "/U_" + tag + ".bin", U.data(), (size_t)U.len);
   save_info(dir + "/info_" + tag + ".txt", info);
+}
+
+static mat<double> to_double(const mat<float> &a) {
+  mat<double> d(a.dim_y, a.dim_x, a.dim_z);
+  d.format = a.format;
+  std::copy(a.buf.begin(), a.buf.end(), d.buf.begin());
+  return d;
+}
+
+static sparse_mat<double> to_double(const sparse_mat<float> &a) {
+  sparse_mat<double> d;
+  d.dim_x = a.dim_x; d.dim_y = a.dim_y; d.format = a.format;
+  d.csr_ptr = a.csr_ptr; d.csr_ind = a.csr_ind; d.csc_ptr = a.csc_ptr; d.csc_ind = a.csc_ind;
+  d.csr_val.assign(a.csr_val.begin(), a.csr_val.end());
+  d.csc_val.assign(a.csc_val.begin(), a.csc_val.end());
+  return d;
+}
+
+static const geometry &to_double(const geometry &g) { return g; }
+
+// the double/double overload: U_dd.bin holds doubles
+template <class A_t>
+static void run_dd(const std::string &dir, const A_t &A, const mat<float> &b, int dx, int dy,
+                   int dz, bool col_major, int maxit) {
+  mat<double> U(dy, dx, dz), Ut(0);
+  U.format = Ut.format = col_major ? mat_col_major : mat_row_major;
+  const mat<double> bd = to_double(b);
+  tval3_options<double> o;
+  o.maxit = maxit;
+  const tval3_info<double> info = tval3_gpu_3d(U, to_double(A), bd, o, Ut);
+  save(dir + "/U_dd.bin", U.data(), (size_t)U.len);
+  save_info(dir + "/info_dd.txt", info);
 }
 
 int main(int argc, char **argv) {
@@ -65,6 +99,7 @@ int main(int argc, char **argv) {
       std::copy(Av.begin(), Av.end(), A.data());
       run<float>(dir, "f", A, b, dx, dy, dz, false, maxit);
       run<double>(dir, "d", A, b, dx, dy, dz, false, maxit);
+      run_dd(dir, A, b, dx, dy, dz, false, maxit);
     } else if (kind == "sparse") {
       sparse_mat<float> S;
       S.dim_y = (int)meta["m"]; S.dim_x = (int)meta["n"];
@@ -73,6 +108,7 @@ int main(int argc, char **argv) {
       S.csc_ind = load<int>(dir + "/csc_ind.bin"); S.csc_val = load<float>(dir + "/csc_val.bin");
       run<float>(dir, "f", S, b, dx, dy, dz, true, maxit);
       run<double>(dir, "d", S, b, dx, dy, dz, true, maxit);
+      run_dd(dir, S, b, dx, dy, dz, true, maxit);
     } else if (kind == "geometry") {
       const int P = (int)meta["mps"], E = (int)meta["E"], R = (int)meta["R"];
       std::vector<int> up = load<int>(dir + "/use_path.bin");
@@ -94,6 +130,7 @@ int main(int argc, char **argv) {
       g.scale_factor = (float)meta["scale"];
       run<float>(dir, "f", g, b, dx, dy, dz, false, maxit);
       run<double>(dir, "d", g, b, dx, dy, dz, false, maxit);
+      run_dd(dir, g, b, dx, dy, dz, false, maxit);
     } else {
       std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
       return 2;

@@ -359,6 +359,34 @@ def test_fp64_solve_close_to_fp32():
     assert i64.rel_error < 0.05
 
 
+def test_f64_bar_separates_reduction_order_from_float_rounding():
+    """The FP64 agreement bar (tests/f64_bar.py) admits another summation
+    order of the oracle's own reductions and rejects a plausible mistake of a
+    double/double path: one input rounded through float (here b).  The chunk
+    length changes only the order of dots and norms: the default restores
+    bit-identical results and the products do not depend on it."""
+    import f64_bar as F
+    A, x, b = _dense(20, 20, 0.5, 17)
+    op = O.Op.dense(A)
+    rng = np.random.default_rng(3)
+    bd = b.astype(np.float64) + 1e-3 * rng.standard_normal(b.size)  # not float-valued
+    opts = default_options(maxit=60, ls_exact=1)
+    Uo, io, _ = O.solve_f64(op, bd, (20, 20, 1), opts)
+    bar = F.bound(op, bd, (20, 20, 1), opts, Uo)
+    O.set_sum_chunk(100)
+    try:
+        Uc, ic, _ = O.solve_f64(op, bd, (20, 20, 1), opts)
+        y = op.gemv_f64(bd, trans=True)
+    finally:
+        O.set_sum_chunk(4096)
+    assert F.rel(Uc, Uo) <= bar and ic.total_iters == io.total_iters
+    np.testing.assert_array_equal(y, op.gemv_f64(bd, trans=True))
+    U2, _, _ = O.solve_f64(op, bd, (20, 20, 1), opts)
+    assert F.rel(U2, Uo) == 0.0
+    Uf, _, _ = O.solve_f64(op, bd.astype(np.float32).astype(np.float64), (20, 20, 1), opts)
+    assert F.rel(Uf, Uo) > 10 * bar, (F.rel(Uf, Uo), bar)
+
+
 # ---------------------------------------------------------------------------
 # step size, line search and multipliers, pinned by the mathematics of the
 # augmented Lagrangian (no reference formula retyped):

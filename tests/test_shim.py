@@ -4,8 +4,10 @@ reference's tval3_cpu_2d / tval3_cpu_3d call surface over the C-ABI.
 The drivers under tests/cpp are built by build().  The argument checks run
 without a GPU and must give the reference's messages (check_params,
 2d/tval3_cpu_2d.cpp:696-750, 3d/tval3_cpu_3d.cpp:1028-1165).  The solves run
-float and double scalars and are compared with the oracle given the same
-options (the layer selects the reference's round-by-round backtracking).
+float and double scalars and the double/double overload, and are compared with
+the oracle given the same options (the layer selects the reference's
+round-by-round backtracking); double/double results are held to the FP64 bar
+of f64_bar.
 """
 import os
 import subprocess
@@ -14,6 +16,7 @@ import numpy as np
 import pytest
 
 import tv3_oracle as O
+import f64_bar as F
 import tval3_amd
 from tval3_amd import synth
 
@@ -54,8 +57,9 @@ def test_argument_checks(name, expect):
     got = dict(line.split(": ", 1) for line in out.strip().splitlines())
     for case, msg in expect.items():
         assert got[case] == msg, case
-    # the engine's own limits are argument or internal errors, never silent
-    assert got["double data"].startswith("Internal error: ")
+    # the double/double overloads run the reference's checks; complex data
+    # (not supported by the engine) is an internal error, never silent
+    assert got["double b length"] == expect["dense b length"]
     if name == "shim_2d":
         assert got["dense layers b length"] == "Argument error: b.len must be equal to A.dim_y * layers!"
         assert got["complex data"].startswith("Internal error: ")
@@ -85,6 +89,14 @@ def _run_and_compare(name, kind, d, op, b, dims, maxit):
         rel = float(np.linalg.norm(Ug - Uo) / np.linalg.norm(Uo))
         assert int(total) == info_o.total_iters and int(outer) == info_o.outer_iters
         assert rel <= 1e-6, (tag, rel)
+    # the double/double overload against the oracle's double/double solve
+    Ud = np.fromfile(os.path.join(d, "U_dd.bin"), np.float64)
+    assert Ud.size == n
+    outer, total, _, _ = open(os.path.join(d, "info_dd.txt")).read().split()
+    opts = tval3_amd.default_options(maxit=maxit, ls_exact=1)
+    Uo, info_o, _ = O.solve_f64(op, b, dims, opts)
+    assert int(total) == info_o.total_iters and int(outer) == info_o.outer_iters
+    assert F.rel(Ud, Uo) <= F.bound(op, b, dims, opts, Uo), ("dd", F.rel(Ud, Uo))
 
 
 @pytest.mark.gpu
