@@ -24,6 +24,12 @@
  *   tv3_gemv_f64,                   geometry A (3d/tval3_cpu_3d.cpp:47, 88,
  *   tv3_plan_sparse_f64,            124; 2d/tval3_cpu_2d.cpp:26-29) and gemv
  *   tv3_plan_dense(TV3_F64)         on double vectors (3d/mat_mult.h:17-139)
+ *   tv3_solve_c64 / _c128,       <- the complex overloads: mat<complex<float>>
+ *   tv3_solve_batch_c64 / _c128,    / mat<complex<double>> U with complex
+ *   tv3_gemv_c64 / _c128,           dense, complex sparse or geometry A
+ *   tv3_plan_sparse_c64 / _c128,    (2d/tval3_cpu_2d.cpp:31-41, 58-68, 85-95;
+ *   tv3_plan_dense(TV3_C64/C128)    3d/tval3_cpu_3d.cpp:56-65, 97-103,
+ *                                   132-141), gemv with CblasConjTrans
  *   tv3_D / tv3_Dt               <- D / Dt (2d/tval3_cpu_2d.cpp:268-293,
  *                                   3d/tval3_cpu_3d.cpp:365-425)
  *   tv3_shrink                   <- shrinkage (2d/tval3_cpu_2d.cpp:588-611,
@@ -61,6 +67,8 @@ extern "C" {
 #define TV3_F32 0
 #define TV3_BF16 1
 #define TV3_F64 2 /* mat<double>: for tv3_solve_f64 / tv3_gemv_f64 only */
+#define TV3_C64 3 /* mat<complex<float>>: for the _c64 entry points only */
+#define TV3_C128 4 /* mat<complex<double>>: for the _c128 entry points only */
 
 /* ---- gemv transpose flags (CBLAS_TRANSPOSE) ----------------------------- */
 #define TV3_NOTRANS 0
@@ -90,8 +98,10 @@ typedef struct tv3_options {
   int scal_double;            /* 0: T_scal = float, 1: T_scal = double */
   int tv_iso;                 /* 0: anisotropic TV (reference), 1: isotropic */
   int dt_exact;               /* 3-D only. 0: the reference's Dt (its y-term
-                                 is not the adjoint of D for dim_z >= 2),
-                                 1: the exact adjoint of D */
+                                 is not the adjoint of D for dim_z >= 2) and,
+                                 for complex data, its Wz from the y
+                                 direction; 1: the exact adjoint of D and Wz
+                                 from the z direction */
   int ls_exact;               /* Armijo backtracking.  1: the reference's
                                  rounds (each step re-interpolates every
                                  vector and re-evaluates f from rounded
@@ -203,6 +213,17 @@ int tv3_plan_sparse_f64(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
                         const int *csr_ind, const double *csr_val,
                         const int *csc_ptr, const int *csc_ind,
                         const double *csc_val, tv3_plan **out);
+/* sparse_mat<complex<float>> / sparse_mat<complex<double>>: values as
+ * interleaved (re, im) pairs (2 * nnz components, aligned to the complex
+ * element), for the _c64 / _c128 entry points only. */
+int tv3_plan_sparse_c64(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                        const int *csr_ind, const float *csr_val,
+                        const int *csc_ptr, const int *csc_ind,
+                        const float *csc_val, tv3_plan **out);
+int tv3_plan_sparse_c128(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                         const int *csr_ind, const double *csr_val,
+                         const int *csc_ptr, const int *csc_ind,
+                         const double *csc_val, tv3_plan **out);
 /* Frees the plan's own device arrays; valid after its context is destroyed. */
 void tv3_plan_destroy(tv3_plan *plan);
 /* m = rows (measurements), n = columns (pixels), nnz = stored or walked
@@ -302,6 +323,37 @@ int tv3_solve_batch_f64(tv3_ctx *ctx, int B, const tv3_plan *const *A,
 /* y = A x or A^T x on double vectors (plans as for tv3_solve_f64). */
 int tv3_gemv_f64(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
                  double *y);
+
+/* The complex solves: vectors of interleaved (re, im) pairs (2 * len
+ * components, aligned to the complex element), T_scal = the component type
+ * (opts->scal_double has no effect; tv_iso = 1 is rejected: the reference
+ * has no complex isotropic TV).  A is a TV3_C64 / TV3_C128 dense plan, a
+ * tv3_plan_sparse_c64 / _c128 plan or a geometry plan (real weights).  Dots
+ * are real(cblas_?dotc), shrinkage is per direction on |ubar| (2d:614-649),
+ * nonneg clamps the real part and zeroes the imaginary one, opts->isreal = 1
+ * keeps the real part of the result, and A^T is the conjugate transpose.
+ * The reference's 3-D Wz takes the y direction's ubar (3d:934-937);
+ * dt_exact = 1 takes the z direction's. */
+int tv3_solve_c64(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+                  int dim_x, int dim_y, int dim_z, const float *b, float *U,
+                  const float *Ut, tv3_info *info, tv3_trace_rec *trace,
+                  int trace_cap);
+int tv3_solve_batch_c64(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                        const tv3_options *opts, const int *dims,
+                        const float *const *b, float *const *U,
+                        const float *const *Ut, tv3_info *infos);
+int tv3_gemv_c64(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
+                 float *y);
+int tv3_solve_c128(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
+                   int dim_x, int dim_y, int dim_z, const double *b, double *U,
+                   const double *Ut, tv3_info *info, tv3_trace_rec *trace,
+                   int trace_cap);
+int tv3_solve_batch_c128(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                         const tv3_options *opts, const int *dims,
+                         const double *const *b, double *const *U,
+                         const double *const *Ut, tv3_info *infos);
+int tv3_gemv_c128(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
+                  double *y);
 
 /* ---- differentiation: K unrolled inner iterations and their adjoint ------- */
 /* No single reference counterpart: the map is K inner iterations of the first

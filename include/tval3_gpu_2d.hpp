@@ -7,11 +7,11 @@
 // the row blocks of the dense A^T A product (as in the reference);
 // virtual_procs_per_core (MKL threads) is accepted and ignored.
 //
-// Supported element types: float vectors with float or double scalars
-// (T_comp = float, T_scal = float | double) and double/double (FP64 vectors
-// and scalars, through tv3_solve_f64; results agree with the reference's
-// double solve to FP64 rounding).  The complex overloads exist so that
-// callers link, and throw tval3_exception.
+// Supported element types: all of the reference's: float vectors with float
+// or double scalars (T_comp = float, T_scal = float | double), double/double
+// (through tv3_solve_f64; results agree with the reference's double solve to
+// FP64 rounding) and complex<float>/float, complex<double>/double (through
+// tv3_solve_c64 / _c128).
 // Multi-layer calls (b.len = layers * A.dim_y, U.len = layers * A.dim_x:
 // U's pixels hold the layers one after the other, every product applies A
 // to each layer, get_g takes two plain products) run for dense and sparse A,
@@ -135,12 +135,6 @@ inline tval3_info<Ts> run_2d(mat<Tc> &U, const geometry &A, const mat<Tc> &b,
       });
 }
 
-[[noreturn]] inline void unsupported_types() {
-  throw tval3_exception(std::string(
-      "Internal error: the GPU engine computes with float vectors (float or double scalars) "
-      "or double vectors; complex data are not supported"));
-}
-
 }  // namespace tval3_gpu_detail
 
 // ---- the overload set ---------------------------------------------------------
@@ -202,19 +196,47 @@ inline const tval3_info<double> tval3_gpu_2d(mat<double> &U, const geometry &A, 
   return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 // complex<float> / float, complex<double> / double
-template <class A_t>
-inline const tval3_info<float> tval3_gpu_2d(mat<std::complex<float> > &, const A_t &,
-                                            const mat<std::complex<float> > &,
-                                            const tval3_options<float> &,
-                                            const mat<std::complex<float> > &, int = 5, int = 2) {
-  tval3_gpu_detail::unsupported_types();
+inline const tval3_info<float> tval3_gpu_2d(mat<std::complex<float> > &U, const mat<std::complex<float> > &A,
+                                   const mat<std::complex<float> > &b, const tval3_options<float> &opts,
+                                   const mat<std::complex<float> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
-template <class A_t>
-inline const tval3_info<double> tval3_gpu_2d(mat<std::complex<double> > &, const A_t &,
-                                             const mat<std::complex<double> > &,
-                                             const tval3_options<double> &,
-                                             const mat<std::complex<double> > &, int = 5, int = 2) {
-  tval3_gpu_detail::unsupported_types();
+inline const tval3_info<float> tval3_gpu_2d(mat<std::complex<float> > &U, const sparse_mat<std::complex<float> > &A,
+                                   const mat<std::complex<float> > &b, const tval3_options<float> &opts,
+                                   const mat<std::complex<float> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<float> tval3_gpu_2d(mat<std::complex<float> > &U, const geometry &A,
+                                   const mat<std::complex<float> > &b, const tval3_options<float> &opts,
+                                   const mat<std::complex<float> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_2d(mat<std::complex<double> > &U, const mat<std::complex<double> > &A,
+                                   const mat<std::complex<double> > &b, const tval3_options<double> &opts,
+                                   const mat<std::complex<double> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_2d(mat<std::complex<double> > &U, const sparse_mat<std::complex<double> > &A,
+                                   const mat<std::complex<double> > &b, const tval3_options<double> &opts,
+                                   const mat<std::complex<double> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_2d(mat<std::complex<double> > &U, const geometry &A,
+                                   const mat<std::complex<double> > &b, const tval3_options<double> &opts,
+                                   const mat<std::complex<double> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_2d(U, A, b, opts, Ut, submat_mb);
 }
 
 #endif  // TVAL3_GPU_2D_HPP_

@@ -9,9 +9,10 @@
 // U's dim_z selects the 3-D stencil; dim_z == 1 is the 2-D one.  The Dt
 // y-term of the reference (not the adjoint of D for dim_z >= 2) is kept.
 //
-// Supported element types: float vectors with float or double scalars, and
-// double/double (FP64 vectors and scalars, through tv3_solve_f64).  The
-// complex overloads exist so that callers link, and throw.
+// Supported element types: all of the reference's: float vectors with float
+// or double scalars, double/double (through tv3_solve_f64) and
+// complex<float>/float, complex<double>/double (through tv3_solve_c64 /
+// _c128; Wz from the y direction as the reference computes it, 3d:934-937).
 #ifndef TVAL3_GPU_3D_HPP_
 #define TVAL3_GPU_3D_HPP_
 
@@ -124,12 +125,6 @@ inline tval3_info<Ts> run_3d(mat<Tc> &U, const geometry &A, const mat<Tc> &b,
       });
 }
 
-[[noreturn]] inline void unsupported_types_3d() {
-  throw tval3_exception(std::string(
-      "Internal error: the GPU engine computes with float vectors (float or double scalars) "
-      "or double vectors; complex data are not supported"));
-}
-
 }  // namespace tval3_gpu_detail
 
 // ---- the overload set ---------------------------------------------------------
@@ -189,19 +184,47 @@ inline const tval3_info<double> tval3_gpu_3d(mat<double> &U, const geometry &A, 
   return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 // complex<float> / float, complex<double> / double
-template <class A_t>
-inline const tval3_info<float> tval3_gpu_3d(mat<std::complex<float> > &, const A_t &,
-                                            const mat<std::complex<float> > &,
-                                            const tval3_options<float> &,
-                                            const mat<std::complex<float> > &, int = 5, int = 2) {
-  tval3_gpu_detail::unsupported_types_3d();
+inline const tval3_info<float> tval3_gpu_3d(mat<std::complex<float> > &U, const mat<std::complex<float> > &A,
+                                   const mat<std::complex<float> > &b, const tval3_options<float> &opts,
+                                   const mat<std::complex<float> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
-template <class A_t>
-inline const tval3_info<double> tval3_gpu_3d(mat<std::complex<double> > &, const A_t &,
-                                             const mat<std::complex<double> > &,
-                                             const tval3_options<double> &,
-                                             const mat<std::complex<double> > &, int = 5, int = 2) {
-  tval3_gpu_detail::unsupported_types_3d();
+inline const tval3_info<float> tval3_gpu_3d(mat<std::complex<float> > &U, const sparse_mat<std::complex<float> > &A,
+                                   const mat<std::complex<float> > &b, const tval3_options<float> &opts,
+                                   const mat<std::complex<float> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<float> tval3_gpu_3d(mat<std::complex<float> > &U, const geometry &A,
+                                   const mat<std::complex<float> > &b, const tval3_options<float> &opts,
+                                   const mat<std::complex<float> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_3d(mat<std::complex<double> > &U, const mat<std::complex<double> > &A,
+                                   const mat<std::complex<double> > &b, const tval3_options<double> &opts,
+                                   const mat<std::complex<double> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_3d(mat<std::complex<double> > &U, const sparse_mat<std::complex<double> > &A,
+                                   const mat<std::complex<double> > &b, const tval3_options<double> &opts,
+                                   const mat<std::complex<double> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
+}
+inline const tval3_info<double> tval3_gpu_3d(mat<std::complex<double> > &U, const geometry &A,
+                                   const mat<std::complex<double> > &b, const tval3_options<double> &opts,
+                                   const mat<std::complex<double> > &Ut, int submat_mb = 5,
+                                   int virtual_procs_per_core = 2) {
+  (void)virtual_procs_per_core;
+  return tval3_gpu_detail::run_3d(U, A, b, opts, Ut, submat_mb);
 }
 
 #endif  // TVAL3_GPU_3D_HPP_

@@ -176,8 +176,24 @@ inline int solve_call(tv3_ctx *c, const tv3_plan *A, const tv3_options *o, int d
                       int dz, const double *b, double *U, const double *Ut, tv3_info *info) {
   return tv3_solve_f64(c, A, o, dx, dy, dz, b, U, Ut, info, nullptr, 0);
 }
+// complex vectors: std::complex<T> is an interleaved (re, im) pair
+inline int solve_call(tv3_ctx *c, const tv3_plan *A, const tv3_options *o, int dx, int dy,
+                      int dz, const std::complex<float> *b, std::complex<float> *U,
+                      const std::complex<float> *Ut, tv3_info *info) {
+  return tv3_solve_c64(c, A, o, dx, dy, dz, reinterpret_cast<const float *>(b),
+                       reinterpret_cast<float *>(U), reinterpret_cast<const float *>(Ut), info,
+                       nullptr, 0);
+}
+inline int solve_call(tv3_ctx *c, const tv3_plan *A, const tv3_options *o, int dx, int dy,
+                      int dz, const std::complex<double> *b, std::complex<double> *U,
+                      const std::complex<double> *Ut, tv3_info *info) {
+  return tv3_solve_c128(c, A, o, dx, dy, dz, reinterpret_cast<const double *>(b),
+                        reinterpret_cast<double *>(U), reinterpret_cast<const double *>(Ut), info,
+                        nullptr, 0);
+}
 
-// Runs one solve for host data of element type T (float or double).
+// Runs one solve for host data of element type T (float, double,
+// std::complex<float>, std::complex<double>).
 // make_plan(ctx, PlanHolder&, KeepAll&) builds the operator plan, keeping
 // its device arrays in the KeepAll; dims are U's (dim_x contiguous).
 template <class Exc, class Info, class Opts, class T, class MakePlan>
@@ -223,6 +239,13 @@ inline void dense_plan(tv3_ctx *c, PlanHolder &ph, KeepAll<Exc> &keep,
   const int rc = tv3_plan_dense(c, m, n, TV3_F64, a->template as<double>(), &ph.p);
   if (rc) fail<Exc>(c, rc);
 }
+template <class Exc, class R>
+inline void dense_plan(tv3_ctx *c, PlanHolder &ph, KeepAll<Exc> &keep,
+                       const std::complex<R> *A, int m, int n) {
+  DevBuf<Exc> *a = keep.put(c, A, (size_t)m * n * sizeof(std::complex<R>));
+  const int rc = tv3_plan_dense(c, m, n, sizeof(R) == 4 ? TV3_C64 : TV3_C128, a->template as<void>(), &ph.p);
+  if (rc) fail<Exc>(c, rc);
+}
 
 inline int sparse_call(tv3_ctx *c, int m, int n, int nnz, const int *rp, const int *ri,
                        const float *rv, const int *cp, const int *ci, const float *cv,
@@ -233,6 +256,18 @@ inline int sparse_call(tv3_ctx *c, int m, int n, int nnz, const int *rp, const i
                        const double *rv, const int *cp, const int *ci, const double *cv,
                        tv3_plan **out) {
   return tv3_plan_sparse_f64(c, m, n, nnz, rp, ri, rv, cp, ci, cv, out);
+}
+inline int sparse_call(tv3_ctx *c, int m, int n, int nnz, const int *rp, const int *ri,
+                       const std::complex<float> *rv, const int *cp, const int *ci,
+                       const std::complex<float> *cv, tv3_plan **out) {
+  return tv3_plan_sparse_c64(c, m, n, nnz, rp, ri, reinterpret_cast<const float *>(rv), cp, ci,
+                             reinterpret_cast<const float *>(cv), out);
+}
+inline int sparse_call(tv3_ctx *c, int m, int n, int nnz, const int *rp, const int *ri,
+                       const std::complex<double> *rv, const int *cp, const int *ci,
+                       const std::complex<double> *cv, tv3_plan **out) {
+  return tv3_plan_sparse_c128(c, m, n, nnz, rp, ri, reinterpret_cast<const double *>(rv), cp, ci,
+                              reinterpret_cast<const double *>(cv), out);
 }
 
 // CSR + CSC (host, zero-based, float or double values) -> device copies + plan

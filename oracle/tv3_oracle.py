@@ -58,6 +58,18 @@ def lib():
         L.tv3o_AtA_blocked.argtypes = [C.c_void_p, _FP, C.c_int, _FP, _FP]
         L.tv3o_set_threads.argtypes = [C.c_int]
         L.tv3o_set_sum_chunk.argtypes = [C.c_longlong]
+        L.tv3o_op_dense_c.restype = C.c_void_p
+        L.tv3o_op_dense_c.argtypes = [C.c_int, C.c_int, _DP]
+        L.tv3o_op_sparse_c.restype = C.c_void_p
+        L.tv3o_op_sparse_c.argtypes = [C.c_int, C.c_int, C.c_int, _IP, _IP, _DP,
+                                       _IP, _IP, _DP]
+        for nm, P_ in (("c64", _FP), ("c128", _DP)):
+            getattr(L, "tv3o_solve_" + nm).argtypes = [
+                C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, P_, P_, P_,
+                C.c_void_p, C.c_void_p, C.c_int]
+            getattr(L, "tv3o_gemv_" + nm).argtypes = [C.c_void_p, C.c_int, P_, P_]
+        L.tv3o_shrink_c64.restype = C.c_double
+        L.tv3o_shrink_c64.argtypes = [C.c_int] * 3 + [C.c_double, C.c_int] + [_FP] * 9
         L.tv3o_get_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -119,6 +131,28 @@ class Op:
         return cls(h, m, n, arrs)
 
     @classmethod
+    def dense_c(cls, A):
+        """Complex dense A (the complex overloads): entries kept as complex128
+        (complex64 entries widen exactly)."""
+        A = np.ascontiguousarray(A, np.complex128)
+        m, n = A.shape
+        return cls(lib().tv3o_op_dense_c(m, n, A.view(np.float64).ctypes.data_as(
+            C.POINTER(C.c_double))), m, n, [A])
+
+    @classmethod
+    def sparse_c(cls, m, n, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind, csc_val):
+        """Complex CSR + CSC (values complex64 or complex128)."""
+        arrs = [np.ascontiguousarray(a, t) for a, t in
+                ((csr_ptr, np.int32), (csr_ind, np.int32), (csr_val, np.complex128),
+                 (csc_ptr, np.int32), (csc_ind, np.int32), (csc_val, np.complex128))]
+        dp = C.POINTER(C.c_double)
+        h = lib().tv3o_op_sparse_c(m, n, int(arrs[1].size), _i(arrs[0]), _i(arrs[1]),
+                                   arrs[2].view(np.float64).ctypes.data_as(dp),
+                                   _i(arrs[3]), _i(arrs[4]),
+                                   arrs[5].view(np.float64).ctypes.data_as(dp))
+        return cls(h, m, n, arrs)
+
+    @classmethod
     def geometry(cls, geo):
         s = geo.struct()
         h = lib().tv3o_op_geometry(C.byref(s))
@@ -142,6 +176,19 @@ class Op:
         dp = C.POINTER(C.c_double)
         lib().tv3o_gemv_f64(self.h, int(bool(trans)), x.ctypes.data_as(dp),
                             y.ctypes.data_as(dp))
+        return y
+
+    def gemv_c(self, x, trans=False):
+        """y = A x or A^H x on complex64 / complex128 vectors (by x's type)."""
+        x = np.asarray(x)
+        t = np.complex128 if x.dtype == np.complex128 else np.complex64
+        x = np.ascontiguousarray(x, t)
+        y = np.zeros(self.n if trans else self.m, t)
+        c128 = t == np.complex128
+        fn = lib().tv3o_gemv_c128 if c128 else lib().tv3o_gemv_c64
+        rt = np.float64 if c128 else np.float32
+        P_ = C.POINTER(C.c_double if c128 else C.c_float)
+        fn(self.h, int(bool(trans)), x.view(rt).ctypes.data_as(P_), y.view(rt).ctypes.data_as(P_))
         return y
 
     def AtA_blocked(self, U, submat_mb=5):
@@ -220,6 +267,48 @@ def solve(op, b, dims, opts, Ut=None, trace_cap=0):
         raise ValueError("oracle solve: dimension mismatch")
     n_tr = min(info.total_iters, trace_cap)
     return U, info, ([trace[i] for i in range(n_tr)] if trace else [])
+
+
+def solve_c(op, b, dims, opts, Ut=None, trace_cap=0):
+    """The complex instantiations of the restated solver: complex64 b gives
+    the complex<float>/float solve, complex128 the complex<double>/double
+    one.  Returns (U, info, trace list)."""
+    from tval3_amd.abi import Info, TraceRec
+    q, p, r = dims
+    b = np.asarray(b)
+    t = np.complex128 if b.dtype == np.complex128 else np.complex64
+    rt = np.float64 if t == np.complex128 else np.float32
+    P_ = C.POINTER(C.c_double if t == np.complex128 else C.c_float)
+    b = np.ascontiguousarray(b, t)
+    _check_layers(op, q * p * r, b.size)
+    U = np.zeros(q * p * r, t)
+    info = Info()
+    trace = (TraceRec * trace_cap)() if trace_cap else None
+    Utc = np.ascontiguousarray(Ut, t) if Ut is not None else None
+    fn = lib().tv3o_solve_c128 if t == np.complex128 else lib().tv3o_solve_c64
+    rc = fn(op.h, C.byref(opts), q, p, r, b.view(rt).ctypes.data_as(P_),
+            U.view(rt).ctypes.data_as(P_),
+            Utc.view(rt).ctypes.data_as(P_) if Utc is not None else None,
+            C.byref(info), trace, int(trace_cap))
+    if rc:
+        raise ValueError({1: "oracle solve: dimension mismatch",
+                          2: "oracle solve: isotropic TV takes real data",
+                          3: "oracle solve: the operator has no complex entries"}[rc])
+    n_tr = min(info.total_iters, trace_cap)
+    return U, info, ([trace[i] for i in range(n_tr)] if trace else [])
+
+
+def shrink_c(Ux, Uy, Uz, sx, sy, sz, dims, beta, wz_fix=False):
+    """Complex shrinkage (complex64): (Wx, Wy, Wz, lam1)."""
+    q, p, r = dims
+    c = [np.ascontiguousarray(a, np.complex64) if a is not None else None
+         for a in (Ux, Uy, Uz, sx, sy, sz)]
+    outs = [np.zeros_like(c[0]), np.zeros_like(c[0]),
+            np.zeros_like(c[0]) if r > 1 else None]
+    fp = lambda a: a.view(np.float32).ctypes.data_as(_FP) if a is not None else None  # noqa: E731
+    lam1 = lib().tv3o_shrink_c64(q, p, r, float(beta), int(bool(wz_fix)),
+                                 *[fp(a) for a in c + outs])
+    return outs[0], outs[1], outs[2], lam1
 
 
 def solve_f64(op, b, dims, opts, Ut=None, trace_cap=0):

@@ -74,6 +74,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -110,22 +111,88 @@ double chunked_sum(i64 n, F term) {
   return s;
 }
 
+/* Vector element types: float, double, std::complex<float>,
+ * std::complex<double> (the reference's T_comp).  R is the real type of a
+ * component. */
+template <class T>
+struct VT {
+  typedef T R;
+  static const bool cplx = false;
+};
+template <class R_>
+struct VT<std::complex<R_> > {
+  typedef R_ R;
+  static const bool cplx = true;
+};
+
+/* Terms of the FP64 dot products.  Real: the exact product.  Complex
+ * (cblas_?dotc, conj(a) b): Re = a.re b.re + a.im b.im and Im = a.re b.im -
+ * a.im b.re, each one FP64 rounding of two FP64 products. */
+inline double dre(float a, float b) { return (double)a * (double)b; }
+inline double dre(double a, double b) { return a * b; }
+template <class R>
+double dre(const std::complex<R> &a, const std::complex<R> &b) {
+  return (double)a.real() * (double)b.real() + (double)a.imag() * (double)b.imag();
+}
+template <class R>
+double dim_(const std::complex<R> &a, const std::complex<R> &b) {
+  return (double)a.real() * (double)b.imag() - (double)a.imag() * (double)b.real();
+}
+
 template <class T>
 double ddot(i64 n, const T *a, const T *b) {
-  return chunked_sum(n, [&](i64 i) { return (double)a[i] * (double)b[i]; });
+  return chunked_sum(n, [&](i64 i) { return dre(a[i], b[i]); });
 }
-/* cblas_?dot / cblas_?nrm2: result in the vector type */
+/* cblas_?dot / cblas_?nrm2 (complex: the real part of cblas_?dotc, which is
+ * all the solver uses but for one BB quotient): result in the component
+ * type */
 template <class T>
-T sdot(i64 n, const T *a, const T *b) { return (T)ddot(n, a, b); }
+typename VT<T>::R sdot(i64 n, const T *a, const T *b) {
+  return (typename VT<T>::R)ddot(n, a, b);
+}
 template <class T>
-T snrm2(i64 n, const T *a) { return (T)std::sqrt(ddot(n, a, a)); }
+typename VT<T>::R sdot_im(i64 n, const T *a, const T *b) {
+  return (typename VT<T>::R)chunked_sum(n, [&](i64 i) { return dim_(a[i], b[i]); });
+}
+template <class T>
+typename VT<T>::R snrm2(i64 n, const T *a) {
+  return (typename VT<T>::R)std::sqrt(ddot(n, a, a));
+}
 
-/* z = x + alpha*y with one rounding, computed in precision TS */
+/* |z| of a complex value: FP64 square root of the FP64 sum of squares,
+ * rounded once; a value on an axis gives its one component's magnitude */
+inline float modulus(float a) { return std::abs(a); }
+inline double modulus(double a) { return std::abs(a); }
+template <class R>
+R modulus(const std::complex<R> &z) {
+  if (z.imag() == 0) return std::abs(z.real());
+  if (z.real() == 0) return std::abs(z.imag());
+  const double a = z.real(), b = z.imag();
+  return (R)std::sqrt(a * a + b * b);
+}
+
+/* real scalar times a vector element, per component */
+template <class R>
+R mul_r(R s, R x) { return s * x; }
+template <class R>
+std::complex<R> mul_r(R s, const std::complex<R> &x) {
+  return std::complex<R>(s * x.real(), s * x.imag());
+}
+/* fma(alpha, y, x) per component in precision TS, rounded to the component */
+template <class TS, class R>
+R fma_s(TS alpha, R y, R x) { return (R)std::fma(alpha, (TS)y, (TS)x); }
+template <class TS, class R>
+std::complex<R> fma_s(TS alpha, const std::complex<R> &y, const std::complex<R> &x) {
+  return std::complex<R>((R)std::fma(alpha, (TS)y.real(), (TS)x.real()),
+                         (R)std::fma(alpha, (TS)y.imag(), (TS)x.imag()));
+}
+
+/* z = x + alpha*y with one rounding, computed in precision TS (complex: per
+ * component, as the reference's complex vec_add, 2d/tval3_cpu_2d.cpp:210-226) */
 template <class T, class TS>
 void vec_add(i64 n, const T *x, TS alpha, const T *y, T *z) {
 #pragma omp parallel for schedule(static)
-  for (i64 i = 0; i < n; ++i)
-    z[i] = (T)std::fma(alpha, (TS)y[i], (TS)x[i]);
+  for (i64 i = 0; i < n; ++i) z[i] = fma_s(alpha, y[i], x[i]);
 }
 /* z = x - y (vec_add with alpha = -1: exact negation, one rounding) */
 template <class T>
@@ -133,16 +200,18 @@ void vec_sub(i64 n, const T *x, const T *y, T *z) {
 #pragma omp parallel for schedule(static)
   for (i64 i = 0; i < n; ++i) z[i] = x[i] - y[i];
 }
-/* y += alpha*x, alpha in the vector type (cblas_saxpy / cblas_daxpy) */
+/* y += alpha*x, alpha a real value in the component type (cblas_saxpy /
+ * cblas_daxpy; the complex solver's alphas are real values, applied per
+ * component) */
 template <class T>
-void axpy(i64 n, T alpha, const T *x, T *y) {
+void axpy(i64 n, typename VT<T>::R alpha, const T *x, T *y) {
 #pragma omp parallel for schedule(static)
-  for (i64 i = 0; i < n; ++i) y[i] = std::fma(alpha, x[i], y[i]);
+  for (i64 i = 0; i < n; ++i) y[i] = fma_s(alpha, x[i], y[i]);
 }
 template <class T>
-void scal(i64 n, T alpha, T *x) {
+void scal(i64 n, typename VT<T>::R alpha, T *x) {
 #pragma omp parallel for schedule(static)
-  for (i64 i = 0; i < n; ++i) x[i] = alpha * x[i];
+  for (i64 i = 0; i < n; ++i) x[i] = mul_r(alpha, x[i]);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -156,6 +225,11 @@ struct Op {
   // sparse (borrowed)
   const int *csr_ptr, *csr_ind, *csc_ptr, *csc_ind;
   const float *csr_val, *csc_val;
+  // complex entries (the complex solver's mat<complex<T>> / sparse_mat
+  // values), re and im interleaved, as doubles (complex<float> entries widen
+  // exactly): dense m*n, sparse nnz
+  const double *Az;
+  const double *csr_valz, *csc_valz;
   i64 nnz;
   // geometry: walks in reference loop order
   float scale;
@@ -361,6 +435,136 @@ void gemv(const Op &op, bool trans, const T *x, T *y) {
   }
 }
 
+/* Complex products (the complex instantiations of gemv, 2d/mat_mult.h:15-91
+ * with complex A, and of get_g_mult's blocked product): y = A x takes the
+ * terms a x, y = A^H x (CblasConjTrans, 2d:308, 392, 774) the terms conj(a)
+ * x, each component of a term one FP64 rounding of two FP64 products, all
+ * sums in FP64 and rounded once per component.  The geometry projector has
+ * real weights: each component is multiplied by the scale factor in the
+ * component type. */
+struct CAcc {
+  double re = 0.0, im = 0.0;
+};
+inline void cadd(CAcc &s, double ar, double ai, double xr, double xi, bool conj) {
+  if (!conj) {
+    s.re += ar * xr - ai * xi;
+    s.im += ar * xi + ai * xr;
+  } else {
+    s.re += ar * xr + ai * xi;
+    s.im += ar * xi - ai * xr;
+  }
+}
+
+template <class R>
+void dense_AtA_blocked(const Op &op, const std::complex<R> *U, std::complex<R> *Au,
+                       std::complex<R> *g, int submat_mb) {
+  typedef std::complex<R> T;
+  const i64 m = op.m, n = op.n;
+  const i64 elems = (i64)submat_mb * 1024 * 1024 / (i64)sizeof(T);
+  const i64 r = std::max<i64>(elems / n, 1);
+  const double *A = op.Az;
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < m; ++i) {
+    CAcc s;
+    const double *row = A + 2 * i * n;
+    for (i64 j = 0; j < n; ++j)
+      cadd(s, row[2 * j], row[2 * j + 1], (double)U[j].real(), (double)U[j].imag(), false);
+    Au[i] = T((R)s.re, (R)s.im);
+  }
+  const i64 cb = 256;
+#pragma omp parallel for schedule(static)
+  for (i64 j0 = 0; j0 < n; j0 += cb) {
+    const i64 j1 = std::min(n, j0 + cb);
+    T gacc[256];
+    CAcc acc[256];
+    for (i64 j = j0; j < j1; ++j) gacc[j - j0] = T(0, 0);
+    for (i64 y0 = 0; y0 < m; y0 += r) {
+      const i64 y1 = std::min(m, y0 + r);
+      for (i64 j = j0; j < j1; ++j) acc[j - j0] = CAcc();
+      for (i64 i = y0; i < y1; ++i) {
+        const double xr = (double)Au[i].real(), xi = (double)Au[i].imag();
+        const double *row = A + 2 * i * n;
+        for (i64 j = j0; j < j1; ++j) cadd(acc[j - j0], row[2 * j], row[2 * j + 1], xr, xi, true);
+      }
+      for (i64 j = j0; j < j1; ++j)
+        gacc[j - j0] = T(gacc[j - j0].real() + (R)acc[j - j0].re, gacc[j - j0].imag() + (R)acc[j - j0].im);
+    }
+    for (i64 j = j0; j < j1; ++j) g[j] = gacc[j - j0];
+  }
+}
+
+template <class R>
+void gemv(const Op &op, bool trans, const std::complex<R> *x, std::complex<R> *y) {
+  typedef std::complex<R> T;
+  const i64 m = op.m, n = op.n;
+  if (op.kind == 0) {
+    const double *A = op.Az;
+    if (!trans) {
+#pragma omp parallel for schedule(static)
+      for (i64 i = 0; i < m; ++i) {
+        CAcc s;
+        const double *row = A + 2 * i * n;
+        for (i64 j = 0; j < n; ++j)
+          cadd(s, row[2 * j], row[2 * j + 1], (double)x[j].real(), (double)x[j].imag(), false);
+        y[i] = T((R)s.re, (R)s.im);
+      }
+    } else {
+      const i64 cb = 256;
+#pragma omp parallel for schedule(static)
+      for (i64 j0 = 0; j0 < n; j0 += cb) {
+        const i64 j1 = std::min(n, j0 + cb);
+        CAcc acc[256];
+        for (i64 i = 0; i < m; ++i) {
+          const double xr = (double)x[i].real(), xi = (double)x[i].imag();
+          const double *row = A + 2 * i * n;
+          for (i64 j = j0; j < j1; ++j) cadd(acc[j - j0], row[2 * j], row[2 * j + 1], xr, xi, true);
+        }
+        for (i64 j = j0; j < j1; ++j) y[j] = T((R)acc[j - j0].re, (R)acc[j - j0].im);
+      }
+    }
+  } else if (op.kind == 1) {
+    const int *ptr = trans ? op.csc_ptr : op.csr_ptr;
+    const int *ind = trans ? op.csc_ind : op.csr_ind;
+    const double *val = trans ? op.csc_valz : op.csr_valz;
+    const i64 rows = trans ? n : m;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i64 r = 0; r < rows; ++r) {
+      CAcc s;
+      for (int k = ptr[r]; k < ptr[r + 1]; ++k)
+        cadd(s, val[2 * k], val[2 * k + 1], (double)x[ind[k]].real(), (double)x[ind[k]].imag(), trans);
+      y[r] = T((R)s.re, (R)s.im);
+    }
+  } else {
+    const R sc = (R)op.scale;
+    if (!trans) {
+#pragma omp parallel for schedule(dynamic, 64)
+      for (i64 r = 0; r < m; ++r) {
+        double sr = 0.0, si = 0.0;
+        for (i64 t = op.row_ptr[r]; t < op.row_ptr[r + 1]; ++t) {
+          const int w = op.row_walk[t];
+          for (i64 k = op.walk_ptr[w]; k < op.walk_ptr[w + 1]; ++k) {
+            const T v = x[op.walk_lin[k]];
+            sr += (double)(v.real() * sc);
+            si += (double)(v.imag() * sc);
+          }
+        }
+        y[r] = T((R)sr, (R)si);
+      }
+    } else {
+#pragma omp parallel for schedule(dynamic, 256)
+      for (i64 v = 0; v < n; ++v) {
+        double sr = 0.0, si = 0.0;
+        for (i64 k = op.col_ptr[v]; k < op.col_ptr[v + 1]; ++k) {
+          const T u = x[op.col_ray[k]];
+          sr += (double)(u.real() * sc);
+          si += (double)(u.imag() * sc);
+        }
+        y[v] = T((R)sr, (R)si);
+      }
+    }
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* finite differences (x is the contiguous axis; periodic boundaries)      */
 /* ---------------------------------------------------------------------- */
@@ -467,12 +671,62 @@ double shrink(const Dims &d, TS beta, bool iso, const T *Ux, const T *Uy,
   });
 }
 
+/* complex shrinkage: 2d/tval3_cpu_2d.cpp:614-649, 3d/tval3_cpu_3d.cpp:876-947.
+ * Per direction: ubar = u - sigma/beta per component (rounded to R),
+ * |ubar| = sqrt(re*re + im*im) in R, w = max(|ubar| - 1/beta, 0) in R and
+ * W = w * ubar / |ubar| per component in R (0 where |ubar| = 0).  The
+ * reference's 3-D version takes Wz's components from the y direction's ubar
+ * (3d:934-937); wz_fix = 1 takes the z direction's.  Returns the FP64 sum of
+ * the per-voxel sums of w (each in R). */
+template <class R, class TS>
+double shrink_c(const Dims &d, TS beta, bool wz_fix, const std::complex<R> *Ux,
+                const std::complex<R> *Uy, const std::complex<R> *Uz,
+                const std::complex<R> *sx, const std::complex<R> *sy,
+                const std::complex<R> *sz, std::complex<R> *Wx,
+                std::complex<R> *Wy, std::complex<R> *Wz) {
+  typedef std::complex<R> T;
+  const i64 n = d.n();
+  const bool z3 = d.r > 1;
+  const TS ib = (TS)1 / beta;
+  auto bar = [&](const T &u, const T &sg) {
+    return T((R)((TS)u.real() - (TS)sg.real() / beta), (R)((TS)u.imag() - (TS)sg.imag() / beta));
+  };
+  auto mag = [](const T &b) -> R { return std::sqrt(b.real() * b.real() + b.imag() * b.imag()); };
+  auto thr = [&](R a) -> R {
+    R t = (R)((TS)a - ib);
+    return (t >= 0) ? t : (R)0;
+  };
+  auto dirW = [](R w, const T &b, R a) {
+    return (a > 0) ? T(w * b.real() / a, w * b.imag() / a) : T(0, 0);
+  };
+  return chunked_sum(n, [&](i64 i) -> double {
+    const T bx = bar(Ux[i], sx[i]), by = bar(Uy[i], sy[i]);
+    const R ax = mag(bx), ay = mag(by);
+    const R wx = thr(ax), wy = thr(ay);
+    Wx[i] = dirW(wx, bx, ax);
+    Wy[i] = dirW(wy, by, ay);
+    R t = wx + wy;
+    if (z3) {
+      const T bz = bar(Uz[i], sz[i]);
+      const R az = mag(bz), wz = thr(az);
+      Wz[i] = dirW(wz, wz_fix ? bz : by, az);
+      t = t + wz;
+    }
+    return (double)t;
+  });
+}
+
 /* ---------------------------------------------------------------------- */
 /* the solver                                                              */
 /* ---------------------------------------------------------------------- */
 template <class TC, class TS>
 struct Solver {
   typedef std::vector<TC> V;
+  // component type: f, C, lam4, lam5 are T_comp in the reference; with
+  // complex T_comp they are real-valued (imaginary parts stay 0), so they
+  // are kept as their real component
+  typedef typename VT<TC>::R TR;
+  static const bool kCplx = VT<TC>::cplx;
   const Op &A;
   const tv3_options &o;
   Dims dm;
@@ -488,9 +742,9 @@ struct Solver {
   // penalty parameters, multipliers, Lagrangian terms (tval3_data)
   TS mu, beta, muDbeta;
   V delta, sx, sy, sz;
-  TC f;
+  TR f;
   TS lam1, lam2, lam3;
-  TC lam4, lam5;
+  TR lam4, lam5;
   V Up, dU, uup, Ux, Uxp, dUx, Uy, Uyp, dUy, Uz, Uzp, dUz, Wx, Wy, Wz, Atb,
       Au, Aup, dAu, d, g, gp, dg, g2, g2p, dg2;
   unsigned currInitIteration;
@@ -533,20 +787,20 @@ struct Solver {
     if (m > 0) {
       const TS th1 = (TS)0.5, th2 = (TS)1.5;
       TC bmin = b[0], bmax = bmin;
-      TS bmin_abs = std::abs(bmin), bmax_abs = bmin_abs;
+      TS bmin_abs = modulus(bmin), bmax_abs = bmin_abs;
       for (i64 i = 0; i < m; ++i) {
         const TC val = b[i];
-        const TS va = std::abs(val);
+        const TS va = modulus(val);
         if (va < bmin_abs) { bmin = val; bmin_abs = va; }
         if (va > bmax_abs) { bmax = val; bmax_abs = va; }
       }
-      const TS b_dif = std::abs(bmax - bmin);
+      const TS b_dif = modulus(bmax - bmin);
       if (b_dif < th1) {
         scl = th1 / b_dif;
-        scal(m, (TC)scl, b.data());
+        scal(m, (TR)scl, b.data());
       } else if (b_dif > th2) {
         scl = th2 / b_dif;
-        scal(m, (TC)scl, b.data());
+        scal(m, (TR)scl, b.data());
       }
     }
     return scl;
@@ -570,8 +824,8 @@ struct Solver {
     lam3 = (TS)sdot(m, Aub.data(), Aub.data());
     lam5 = sdot(m, delta.data(), Aub.data());
   }
-  TC f_full() {
-    return (TC)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)lam4) -
+  TR f_full() {
+    return (TR)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)lam4) -
                    (TS)lam5);
   }
 
@@ -589,7 +843,7 @@ struct Solver {
   void get_g_mult(const V &U, bool force_mul) {
     if (A.kind == 0) {
       if (layers == 1) {
-        dense_AtA_blocked<TC>(A, U.data(), Au.data(), g.data(), o.submat_mb > 0 ? o.submat_mb : 5);
+        dense_AtA_blocked(A, U.data(), Au.data(), g.data(), o.submat_mb > 0 ? o.submat_mb : 5);
       } else {
         gemvL(false, U.data(), Au.data());
         gemvL(true, Au.data(), g.data());
@@ -602,7 +856,7 @@ struct Solver {
       return;
     }
     V deltaAu(Au);
-    axpy(m, (TC)-1, last_Au.data(), deltaAu.data());
+    axpy(m, (TR)-1, last_Au.data(), deltaAu.data());
     const TS nrm_deltaAu = snrm2(m, deltaAu.data());
     const TS nrm_Au = snrm2(m, Au.data());
     const TS relChange = nrm_deltaAu / nrm_Au;
@@ -625,7 +879,7 @@ struct Solver {
   /* get_g: 2d/tval3_cpu_2d.cpp:396-423 */
   void get_g(const V &U, const V &b, bool force_mul) {
     get_g_mult(U, force_mul);
-    axpy(n, (TC)-1, Atb.data(), g.data());
+    axpy(n, (TR)-1, Atb.data(), g.data());
     V Vx(n), Vy(n), Vz(z3 ? n : 0);
     vec_sub(n, Ux.data(), Wx.data(), Vx.data());
     vec_sub(n, Uy.data(), Wy.data(), Vy.data());
@@ -642,33 +896,54 @@ struct Solver {
     if (fst_iter) {
       V dx(n), dy(n), dz(z3 ? n : 0);
       D_(d, dx, dy, dz);
-      TC dDd = sdot(n, dx.data(), dx.data()) + sdot(n, dy.data(), dy.data());
+      TR dDd = sdot(n, dx.data(), dx.data()) + sdot(n, dy.data(), dy.data());
       if (z3) dDd = dDd + sdot(n, dz.data(), dz.data());
-      const TC dd = sdot(n, d.data(), d.data());
+      const TR dd = sdot(n, d.data(), d.data());
       V Ad(m);
       gemvL(false, d.data(), Ad.data());
-      const TC Add = sdot(m, Ad.data(), Ad.data());
+      const TR Add = sdot(m, Ad.data(), Ad.data());
       tau = std::abs((TS)dd / ((TS)dDd + muDbeta * (TS)Add));
     } else {
       vec_sub(n, g.data(), gp.data(), dg.data());
       vec_sub(n, g2.data(), g2p.data(), dg2.data());
-      const TC ss = sdot(n, uup.data(), uup.data());
+      const TR ss = sdot(n, uup.data(), uup.data());
       V tmp(dg2);
-      axpy(n, (TC)muDbeta, dg.data(), tmp.data());
-      const TC sy_ = sdot(n, uup.data(), tmp.data());
-      tau = (TS)std::abs(ss / sy_);
+      axpy(n, (TR)muDbeta, dg.data(), tmp.data());
+      const TR sy_ = sdot(n, uup.data(), tmp.data());
+      if constexpr (kCplx) {
+        // abs(ss / sy) of complex values (ss real-valued): |ss| / |sy| in
+        // FP64, rounded to the component type
+        const TR sy_i = sdot_im(n, uup.data(), tmp.data());
+        const double den = modulus(std::complex<double>((double)sy_, (double)sy_i));
+        tau = (TS)(TR)(std::abs((double)ss) / den);
+      } else {
+        tau = (TS)std::abs(ss / sy_);
+      }
     }
     return tau;
   }
 
+  /* nonneg: 2d/tval3_cpu_2d.cpp:455-479 (complex: the real part clamped,
+   * the imaginary part set to 0) */
+  static void clamp0(float &u) { u = (u < 0) ? 0.0f : u; }
+  static void clamp0(double &u) { u = (u < 0) ? 0.0 : u; }
+  template <class R>
+  static void clamp0(std::complex<R> &u) { u = std::complex<R>(u.real() < 0 ? (R)0 : u.real(), (R)0); }
   void nonneg(V &U) {
 #pragma omp parallel for schedule(static)
-    for (i64 i = 0; i < n; ++i) U[i] = (U[i] < 0) ? 0.0f : U[i];
+    for (i64 i = 0; i < n; ++i) clamp0(U[i]);
+  }
+  TC finish(float u, TS scl) const { return (TC)((TS)u / scl); }
+  TC finish(double u, TS scl) const { return (TC)((TS)u / scl); }
+  template <class R>
+  TC finish(const std::complex<R> &u, TS scl) const {
+    const R im = o.isreal ? (R)0 : u.imag();
+    return TC((R)((TS)u.real() / scl), (R)((TS)im / scl));
   }
 
   /* descend: 2d/tval3_cpu_2d.cpp:481-493 */
   void descend(V &U, const V &b, TS tau, bool force_mul) {
-    axpy(n, (TC)(-1 * tau), d.data(), U.data());
+    axpy(n, (TR)(-1 * tau), d.data(), U.data());
     if (o.nonneg) nonneg(U);
     D_(U, Ux, Uy, Uz);
     get_g(U, b, force_mul);
@@ -690,7 +965,7 @@ struct Solver {
   }
 
   /* min_u: 2d/tval3_cpu_2d.cpp:526-576 */
-  void min_u(V &U, TS &gam, const V &b, TC C, bool fst_iter) {
+  void min_u(V &U, TS &gam, const V &b, TR C, bool fst_iter) {
     TS tau = get_tau(fst_iter);
     Up = U; gp = g; g2p = g2; Aup = Au; Uxp = Ux; Uyp = Uy;
     if (z3) Uzp = Uz;
@@ -729,19 +1004,27 @@ struct Solver {
   /* get_gradient: 2d/tval3_cpu_2d.cpp:578-585 */
   void get_gradient(const V &DtsAtd) {
     d = DtsAtd;
-    axpy(n, (TC)1, g2.data(), d.data());
-    axpy(n, (TC)muDbeta, g.data(), d.data());
+    axpy(n, (TR)1, g2.data(), d.data());
+    axpy(n, (TR)muDbeta, g.data(), d.data());
   }
 
-  void shrinkage() {
+  void shrinkage() { shrinkage_(Ux.data()); }
+  template <class T>
+  void shrinkage_(const T *) {
     lam1 = (TS)shrink<TC, TS>(dm, beta, o.tv_iso != 0, Ux.data(), Uy.data(),
                           P(Uz), sx.data(), sy.data(), P(sz), Wx.data(),
                           Wy.data(), P(Wz));
   }
+  template <class R>
+  void shrinkage_(const std::complex<R> *) {
+    // the complex sum is T_mat (2d:617): rounded to the component type
+    lam1 = (TS)(TR)shrink_c<R, TS>(dm, beta, o.dt_exact != 0, Ux.data(), Uy.data(), P(Uz),
+                                   sx.data(), sy.data(), P(sz), Wx.data(), Wy.data(), P(Wz));
+  }
 
   /* update_W: 2d/tval3_cpu_2d.cpp:651-671 */
   void update_W() {
-    f = (TC)((TS)f - ((lam1 + beta / 2 * lam2) - (TS)lam4));
+    f = (TR)((TS)f - ((lam1 + beta / 2 * lam2) - (TS)lam4));
     shrinkage();
     V Vx(n), Vy(n), Vz(z3 ? n : 0);
     vec_sub(n, Ux.data(), Wx.data(), Vx.data());
@@ -749,7 +1032,7 @@ struct Solver {
     if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
     Dt_(g2, Vx, Vy, Vz);
     lam24(Vx, Vy, Vz);
-    f = (TC)((TS)f + ((lam1 + beta / 2 * lam2) - (TS)lam4));
+    f = (TR)((TS)f + ((lam1 + beta / 2 * lam2) - (TS)lam4));
   }
 
   /* update_mlp: 2d/tval3_cpu_2d.cpp:673-694 */
@@ -759,7 +1042,7 @@ struct Solver {
     vec_sub(n, Ux.data(), Wx.data(), Vx.data());
     vec_sub(n, Uy.data(), Wy.data(), Vy.data());
     if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
-    const TC nb = (TC)(-1 * beta);
+    const TR nb = (TR)(-1 * beta);
     axpy(n, nb, Vx.data(), sx.data());
     axpy(n, nb, Vy.data(), sy.data());
     if (z3) axpy(n, nb, Vz.data(), sz.data());
@@ -768,7 +1051,7 @@ struct Solver {
     if (z3) lam4 += sdot(n, sz.data(), Vz.data());
     V Aub(m);
     vec_sub(m, Au.data(), b.data(), Aub.data());
-    axpy(m, (TC)(-1 * mu), Aub.data(), delta.data());
+    axpy(m, (TR)(-1 * mu), Aub.data(), delta.data());
     lam5 = sdot(m, delta.data(), Aub.data());
     f = f - (lam4 + lam5);
   }
@@ -793,7 +1076,7 @@ struct Solver {
     get_g(U, bs, true);
     get_gradient(DtsAtd);
     TS Q = 1, Qp;
-    TC C = f;
+    TR C = f;
     int count_outer = 0, count_total = 0;
     bool fst_iter;
     TS RelChg, RelChgOut = 0, nrmup = 0;
@@ -803,7 +1086,7 @@ struct Solver {
         min_u(U, gam, bs, C, fst_iter);
         update_W();
         Qp = Q, Q = gam * Qp + 1;
-        C = (TC)((gam * Qp * (TS)C + (TS)f) / Q);
+        C = (TR)((gam * Qp * (TS)C + (TS)f) / Q);
         vec_sub(n, U.data(), Up.data(), uup.data());
         get_gradient(DtsAtd);
         count_total++;
@@ -834,11 +1117,12 @@ struct Solver {
       muDbeta = mu / beta;
       f = f_full();
       DtsAtd = d;
-      scal(n, (TC)(beta0 / beta), DtsAtd.data());
+      scal(n, (TR)(beta0 / beta), DtsAtd.data());
       get_gradient(DtsAtd);
       gam = o_gam; Q = 1; C = f;
     }
-    for (i64 i = 0; i < n; ++i) U[i] = (TC)((TS)U[i] / scl);
+    // real part only with isreal (complex data, 2d:855-860), then U / scl
+    for (i64 i = 0; i < n; ++i) U[i] = finish(U[i], scl);
     std::memcpy(U_out, U.data(), sizeof(TC) * n);
     info->secs = 0;
     info->outer_iters = count_outer;
@@ -846,7 +1130,7 @@ struct Solver {
     info->rel_chg = RelChgOut;
     if (Ut) {
       V dlt(Ut, Ut + n);
-      axpy(n, (TC)-1, U.data(), dlt.data());
+      axpy(n, (TR)-1, U.data(), dlt.data());
       const TS a = snrm2(n, dlt.data()), bb = snrm2(n, Ut);
       info->rel_error = a / bb;
     } else {
@@ -900,6 +1184,27 @@ tv3o_op *tv3o_op_sparse(int m, int n, int nnz, const int *csr_ptr,
   op->kind = 1; op->m = m; op->n = n; op->nnz = nnz;
   op->csr_ptr = csr_ptr; op->csr_ind = csr_ind; op->csr_val = csr_val;
   op->csc_ptr = csc_ptr; op->csc_ind = csc_ind; op->csc_val = csc_val;
+  return (tv3o_op *)op;
+}
+
+/* Complex operators (the complex overloads): entries re, im interleaved as
+ * doubles (complex<float> entries widen exactly).  Borrowed. */
+tv3o_op *tv3o_op_dense_c(int m, int n, const double *Az) {
+  Op *op = new (std::nothrow) Op();
+  if (!op) return nullptr;
+  op->kind = 0; op->m = m; op->n = n; op->Az = Az; op->nnz = (i64)m * n;
+  return (tv3o_op *)op;
+}
+
+tv3o_op *tv3o_op_sparse_c(int m, int n, int nnz, const int *csr_ptr,
+                          const int *csr_ind, const double *csr_valz,
+                          const int *csc_ptr, const int *csc_ind,
+                          const double *csc_valz) {
+  Op *op = new (std::nothrow) Op();
+  if (!op) return nullptr;
+  op->kind = 1; op->m = m; op->n = n; op->nnz = nnz;
+  op->csr_ptr = csr_ptr; op->csr_ind = csr_ind; op->csr_valz = csr_valz;
+  op->csc_ptr = csc_ptr; op->csc_ind = csc_ind; op->csc_valz = csc_valz;
   return (tv3o_op *)op;
 }
 
@@ -1013,6 +1318,68 @@ int tv3o_AtA_blocked(const tv3o_op *h, const float *U, int submat_mb, float *Au,
 int tv3o_gemv_f64(const tv3o_op *h, int trans, const double *x, double *y) {
   gemv(*(const Op *)h, trans != 0, x, y);
   return 0;
+}
+
+}  // extern "C"
+
+namespace {
+/* The complex instantiations (tval3_cpu_2d/3d with mat<complex<float>> /
+ * mat<complex<double>>, 2d/tval3_cpu_2d.cpp:31-41, 58-68, 85-95): vectors of
+ * interleaved re, im; T_scal = the component type.  Dense and sparse
+ * operators need complex entries (tv3o_op_dense_c / _sparse_c); geometry has
+ * real weights.  Returns 2 for isotropic TV (no complex form), 3 for an
+ * operator without complex entries. */
+template <class R>
+int solve_c(const tv3o_op *h, const tv3_options *o, int q, int p, int r,
+            const R *b, R *U, const R *Ut, tv3_info *info,
+            tv3_trace_rec *trace, int trace_cap) {
+  typedef std::complex<R> T;
+  const Op &op = *(const Op *)h;
+  const Dims dm{q, p, r};
+  if (!layers_ok(op, dm)) return 1;
+  if (o->tv_iso) return 2;
+  if (op.kind != 2 && ((op.kind == 0 && !op.Az) || (op.kind == 1 && !op.csr_valz))) return 3;
+  Solver<T, R> s(op, *o, dm);
+  s.run((const T *)b, (T *)U, (const T *)Ut, info, trace, trace_cap);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tv3o_solve_c64(const tv3o_op *h, const tv3_options *o, int q, int p,
+                   int r, const float *b, float *U, const float *Ut,
+                   tv3_info *info, tv3_trace_rec *trace, int trace_cap) {
+  return solve_c<float>(h, o, q, p, r, b, U, Ut, info, trace, trace_cap);
+}
+
+int tv3o_solve_c128(const tv3o_op *h, const tv3_options *o, int q, int p,
+                    int r, const double *b, double *U, const double *Ut,
+                    tv3_info *info, tv3_trace_rec *trace, int trace_cap) {
+  return solve_c<double>(h, o, q, p, r, b, U, Ut, info, trace, trace_cap);
+}
+
+int tv3o_gemv_c64(const tv3o_op *h, int trans, const float *x, float *y) {
+  gemv(*(const Op *)h, trans != 0, (const std::complex<float> *)x, (std::complex<float> *)y);
+  return 0;
+}
+
+int tv3o_gemv_c128(const tv3o_op *h, int trans, const double *x, double *y) {
+  gemv(*(const Op *)h, trans != 0, (const std::complex<double> *)x, (std::complex<double> *)y);
+  return 0;
+}
+
+/* complex shrinkage (test entry point): returns lam1 (FP64 sum rounded to
+ * float) */
+double tv3o_shrink_c64(int q, int p, int r, double beta, int wz_fix,
+                       const float *Ux, const float *Uy, const float *Uz,
+                       const float *sx, const float *sy, const float *sz,
+                       float *Wx, float *Wy, float *Wz) {
+  typedef std::complex<float> T;
+  return (double)(float)shrink_c<float, float>(
+      Dims{q, p, r}, (float)beta, wz_fix != 0, (const T *)Ux, (const T *)Uy, (const T *)Uz,
+      (const T *)sx, (const T *)sy, (const T *)sz, (T *)Wx, (T *)Wy, (T *)Wz);
 }
 
 }  // extern "C"

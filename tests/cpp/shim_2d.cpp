@@ -43,8 +43,10 @@ static int args_mode() {
   Ud.format = Utd.format = mat_col_major;
   tval3_options<double> od;
   expect_error("double b length", [&] { tval3_gpu_2d(Ud, Ad, bd, od, Utd); });
-  mat<std::complex<float> > Uc(4, 4), Ac(6, 16), bc(6), Utc(4, 4);
-  expect_error("complex data", [&] { tval3_gpu_2d(Uc, Ac, bc, of, Utc); });
+  // the complex overloads run the same checks
+  mat<std::complex<float> > Uc(4, 4), Ac(6, 16), bc(7), Utc(4, 4);
+  Uc.format = Utc.format = mat_col_major;
+  expect_error("complex b length", [&] { tval3_gpu_2d(Uc, Ac, bc, of, Utc); });
   return 0;
 }
 
@@ -92,6 +94,45 @@ static void run_dd(const std::string &dir, const A_t &A, const mat<float> &b, in
   save_info(dir + "/info_dd.txt", info);
 }
 
+template <class R>
+static mat<std::complex<R> > to_complex(const mat<float> &a) {
+  mat<std::complex<R> > d(a.dim_y, a.dim_x, a.dim_z);
+  d.format = a.format;
+  for (size_t k = 0; k < a.buf.size(); ++k) d.buf[k] = std::complex<R>((R)a.buf[k], (R)0);
+  return d;
+}
+
+template <class R>
+static sparse_mat<std::complex<R> > to_complex(const sparse_mat<float> &a) {
+  sparse_mat<std::complex<R> > d;
+  d.dim_x = a.dim_x; d.dim_y = a.dim_y; d.format = a.format;
+  d.csr_ptr = a.csr_ptr; d.csr_ind = a.csr_ind; d.csc_ptr = a.csc_ptr; d.csc_ind = a.csc_ind;
+  for (float v : a.csr_val) d.csr_val.push_back(std::complex<R>((R)v, (R)0));
+  for (float v : a.csc_val) d.csc_val.push_back(std::complex<R>((R)v, (R)0));
+  return d;
+}
+
+template <class R>
+static const geometry &to_complex(const geometry &g) { return g; }
+
+// the complex<R>/R overload on b rotated by 0.6 + 0.8i (computed in double),
+// imaginary part kept: U_c<tag>.bin holds (re, im) pairs
+template <class R, class A_t>
+static void run_cx(const std::string &dir, const char *tag, const A_t &A, const mat<float> &b,
+                   int dx, int dy, bool col_major, int maxit) {
+  mat<std::complex<R> > U(dy, dx), Ut(0);
+  U.format = Ut.format = col_major ? mat_col_major : mat_row_major;
+  mat<std::complex<R> > bc(b.len);
+  for (int k = 0; k < b.len; ++k)
+    bc.buf[k] = std::complex<R>((R)(0.6 * (double)b.buf[k]), (R)(0.8 * (double)b.buf[k]));
+  tval3_options<R> o;
+  o.maxit = maxit;
+  o.isreal = false;
+  const tval3_info<R> info = tval3_gpu_2d(U, to_complex<R>(A), bc, o, Ut);
+  save(dir + "/U_c" + tag + ".bin", U.data(), (size_t)U.len);
+  save_info(dir + "/info_c" + tag + ".txt", info);
+}
+
 int main(int argc, char **argv) {
   try {
     if (argc >= 2 && !std::strcmp(argv[1], "args")) return args_mode();
@@ -112,6 +153,8 @@ int main(int argc, char **argv) {
       run<float>(dir, "f", A, b, dx, dy, true, maxit);
       run<double>(dir, "d", A, b, dx, dy, true, maxit);
       run_dd(dir, A, b, dx, dy, true, maxit);
+      run_cx<float>(dir, "f", A, b, dx, dy, true, maxit);
+      run_cx<double>(dir, "d", A, b, dx, dy, true, maxit);
     } else if (kind == "sparse") {
       sparse_mat<float> S;
       S.dim_y = (int)meta["m"]; S.dim_x = (int)meta["n"];
@@ -121,6 +164,8 @@ int main(int argc, char **argv) {
       run<float>(dir, "f", S, b, dx, dy, true, maxit);
       run<double>(dir, "d", S, b, dx, dy, true, maxit);
       run_dd(dir, S, b, dx, dy, true, maxit);
+      run_cx<float>(dir, "f", S, b, dx, dy, true, maxit);
+      run_cx<double>(dir, "d", S, b, dx, dy, true, maxit);
     } else if (kind == "geometry") {
       std::vector<int> up = load<int>(dir + "/use_path.bin");
       std::vector<int> xe = load<int>(dir + "/xe.bin"), ye = load<int>(dir + "/ye.bin");
@@ -137,6 +182,8 @@ int main(int argc, char **argv) {
       run<float>(dir, "f", g, b, dx, dy, false, maxit);
       run<double>(dir, "d", g, b, dx, dy, false, maxit);
       run_dd(dir, g, b, dx, dy, false, maxit);
+      run_cx<float>(dir, "f", g, b, dx, dy, false, maxit);
+      run_cx<double>(dir, "d", g, b, dx, dy, false, maxit);
     } else {
       std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
       return 2;

@@ -32,6 +32,9 @@ static int args_mode() {
   mat<double> Ud(4, 4, 4), Ad(6, 64), bd(7), Utd(4, 4, 4);
   tval3_options<double> od;
   expect_error("double b length", [&] { tval3_gpu_3d(Ud, Ad, bd, od, Utd); });
+  // the complex overloads run the same checks
+  mat<std::complex<float> > Ucx(4, 4, 4), Acx(6, 64), bcx(7), Utcx(4, 4, 4);
+  expect_error("complex b length", [&] { tval3_gpu_3d(Ucx, Acx, bcx, of, Utcx); });
   return 0;
 }
 
@@ -79,6 +82,45 @@ static void run_dd(const std::string &dir, const A_t &A, const mat<float> &b, in
   save_info(dir + "/info_dd.txt", info);
 }
 
+template <class R>
+static mat<std::complex<R> > to_complex(const mat<float> &a) {
+  mat<std::complex<R> > d(a.dim_y, a.dim_x, a.dim_z);
+  d.format = a.format;
+  for (size_t k = 0; k < a.buf.size(); ++k) d.buf[k] = std::complex<R>((R)a.buf[k], (R)0);
+  return d;
+}
+
+template <class R>
+static sparse_mat<std::complex<R> > to_complex(const sparse_mat<float> &a) {
+  sparse_mat<std::complex<R> > d;
+  d.dim_x = a.dim_x; d.dim_y = a.dim_y; d.format = a.format;
+  d.csr_ptr = a.csr_ptr; d.csr_ind = a.csr_ind; d.csc_ptr = a.csc_ptr; d.csc_ind = a.csc_ind;
+  for (float v : a.csr_val) d.csr_val.push_back(std::complex<R>((R)v, (R)0));
+  for (float v : a.csc_val) d.csc_val.push_back(std::complex<R>((R)v, (R)0));
+  return d;
+}
+
+template <class R>
+static const geometry &to_complex(const geometry &g) { return g; }
+
+// the complex<R>/R overload on b rotated by 0.6 + 0.8i (computed in double),
+// imaginary part kept: U_c<tag>.bin holds (re, im) pairs
+template <class R, class A_t>
+static void run_cx(const std::string &dir, const char *tag, const A_t &A, const mat<float> &b,
+                   int dx, int dy, int dz, bool col_major, int maxit) {
+  mat<std::complex<R> > U(dy, dx, dz), Ut(0);
+  U.format = Ut.format = col_major ? mat_col_major : mat_row_major;
+  mat<std::complex<R> > bc(b.len);
+  for (int k = 0; k < b.len; ++k)
+    bc.buf[k] = std::complex<R>((R)(0.6 * (double)b.buf[k]), (R)(0.8 * (double)b.buf[k]));
+  tval3_options<R> o;
+  o.maxit = maxit;
+  o.isreal = false;
+  const tval3_info<R> info = tval3_gpu_3d(U, to_complex<R>(A), bc, o, Ut);
+  save(dir + "/U_c" + tag + ".bin", U.data(), (size_t)U.len);
+  save_info(dir + "/info_c" + tag + ".txt", info);
+}
+
 int main(int argc, char **argv) {
   try {
     if (argc >= 2 && !std::strcmp(argv[1], "args")) return args_mode();
@@ -100,6 +142,8 @@ int main(int argc, char **argv) {
       run<float>(dir, "f", A, b, dx, dy, dz, false, maxit);
       run<double>(dir, "d", A, b, dx, dy, dz, false, maxit);
       run_dd(dir, A, b, dx, dy, dz, false, maxit);
+      run_cx<float>(dir, "f", A, b, dx, dy, dz, false, maxit);
+      run_cx<double>(dir, "d", A, b, dx, dy, dz, false, maxit);
     } else if (kind == "sparse") {
       sparse_mat<float> S;
       S.dim_y = (int)meta["m"]; S.dim_x = (int)meta["n"];
@@ -109,6 +153,8 @@ int main(int argc, char **argv) {
       run<float>(dir, "f", S, b, dx, dy, dz, true, maxit);
       run<double>(dir, "d", S, b, dx, dy, dz, true, maxit);
       run_dd(dir, S, b, dx, dy, dz, true, maxit);
+      run_cx<float>(dir, "f", S, b, dx, dy, dz, true, maxit);
+      run_cx<double>(dir, "d", S, b, dx, dy, dz, true, maxit);
     } else if (kind == "geometry") {
       const int P = (int)meta["mps"], E = (int)meta["E"], R = (int)meta["R"];
       std::vector<int> up = load<int>(dir + "/use_path.bin");
@@ -131,6 +177,8 @@ int main(int argc, char **argv) {
       run<float>(dir, "f", g, b, dx, dy, dz, false, maxit);
       run<double>(dir, "d", g, b, dx, dy, dz, false, maxit);
       run_dd(dir, g, b, dx, dy, dz, false, maxit);
+      run_cx<float>(dir, "f", g, b, dx, dy, dz, false, maxit);
+      run_cx<double>(dir, "d", g, b, dx, dy, dz, false, maxit);
     } else {
       std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
       return 2;

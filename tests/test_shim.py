@@ -4,10 +4,10 @@ reference's tval3_cpu_2d / tval3_cpu_3d call surface over the C-ABI.
 The drivers under tests/cpp are built by build().  The argument checks run
 without a GPU and must give the reference's messages (check_params,
 2d/tval3_cpu_2d.cpp:696-750, 3d/tval3_cpu_3d.cpp:1028-1165).  The solves run
-float and double scalars and the double/double overload, and are compared with
-the oracle given the same options (the layer selects the reference's
-round-by-round backtracking); double/double results are held to the FP64 bar
-of f64_bar.
+float and double scalars, the double/double overload and both complex
+overloads, and are compared with the oracle given the same options (the layer
+selects the reference's round-by-round backtracking); double/double and
+complex<double> results are held to the FP64 bar of f64_bar.
 """
 import os
 import subprocess
@@ -57,12 +57,11 @@ def test_argument_checks(name, expect):
     got = dict(line.split(": ", 1) for line in out.strip().splitlines())
     for case, msg in expect.items():
         assert got[case] == msg, case
-    # the double/double overloads run the reference's checks; complex data
-    # (not supported by the engine) is an internal error, never silent
+    # the double/double and complex overloads run the reference's checks
     assert got["double b length"] == expect["dense b length"]
+    assert got["complex b length"] == expect["dense b length"]
     if name == "shim_2d":
         assert got["dense layers b length"] == "Argument error: b.len must be equal to A.dim_y * layers!"
-        assert got["complex data"].startswith("Internal error: ")
 
 
 # ---- GPU solves through the overload layer ------------------------------------
@@ -73,6 +72,21 @@ def _write(d, meta, **arrays):
             f.write(f"{k} {v}\n")
     for k, a in arrays.items():
         a.tofile(os.path.join(d, k + ".bin"))
+
+
+def _complex_op(op, kind, d):
+    """The oracle operator of the complex runs: the same entries as complex
+    values (the projector is the same operator)."""
+    if kind == "dense":
+        m, n = op.m, op.n
+        A = np.fromfile(os.path.join(d, "A.bin"), np.float32).reshape(m, n)
+        return O.Op.dense_c(A.astype(np.complex64))
+    if kind == "sparse":
+        ld = lambda k, t: np.fromfile(os.path.join(d, k + ".bin"), t)  # noqa: E731
+        return O.Op.sparse_c(op.m, op.n, ld("csr_ptr", np.int32), ld("csr_ind", np.int32),
+                             ld("csr_val", np.float32).astype(np.complex64), ld("csc_ptr", np.int32),
+                             ld("csc_ind", np.int32), ld("csc_val", np.float32).astype(np.complex64))
+    return op
 
 
 def _run_and_compare(name, kind, d, op, b, dims, maxit):
@@ -97,6 +111,21 @@ def _run_and_compare(name, kind, d, op, b, dims, maxit):
     Uo, info_o, _ = O.solve_f64(op, b, dims, opts)
     assert int(total) == info_o.total_iters and int(outer) == info_o.outer_iters
     assert F.rel(Ud, Uo) <= F.bound(op, b, dims, opts, Uo), ("dd", F.rel(Ud, Uo))
+    # the complex overloads on b rotated by 0.6 + 0.8i, imaginary part kept:
+    # complex<float> as the float path (1e-6 here), complex<double> to the
+    # FP64 bar
+    cop = _complex_op(op, kind, d)
+    opts = tval3_amd.default_options(maxit=maxit, ls_exact=1, isreal=0)
+    for tag, t in (("f", np.complex64), ("d", np.complex128)):
+        bc = (b.astype(np.float64) * (0.6 + 0.8j)).astype(t)
+        Uc = np.fromfile(os.path.join(d, f"U_c{tag}.bin"), t)
+        assert Uc.size == n
+        outer, total, _, _ = open(os.path.join(d, f"info_c{tag}.txt")).read().split()
+        Uo, info_o, _ = O.solve_c(cop, bc, dims, opts)
+        assert int(total) == info_o.total_iters and int(outer) == info_o.outer_iters, tag
+        bar = 1e-6 if tag == "f" else F.bound(cop, bc, dims, opts, Uo, O.solve_c)
+        assert F.rel(Uc, Uo) <= bar, ("c" + tag, F.rel(Uc, Uo))
+        assert np.abs(Uc.imag).max() > 0
 
 
 @pytest.mark.gpu

@@ -9,7 +9,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = ["csrc/tv3_plan.hip", "csrc/tv3_solver.hip", "csrc/tv3_solver_f64.hip", "csrc/tv3_adjoint.hip",
+SRCS = ["csrc/tv3_plan.hip", "csrc/tv3_solver.hip", "csrc/tv3_solver_f64.hip",
+        "csrc/tv3_solver_c64.hip", "csrc/tv3_solver_c128.hip", "csrc/tv3_adjoint.hip",
         "csrc/tv3_io.hip"]
 OUT = os.path.join(HERE, "libtval3_amd.so")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

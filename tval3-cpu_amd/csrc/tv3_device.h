@@ -172,4 +172,29 @@ __device__ __forceinline__ TS shrink_iso(VC ux, VC uy, VC uz, VC sx, VC sy, VC s
   return w;
 }
 
+// Complex anisotropic shrinkage of one direction (2d/tval3_cpu_2d.cpp:614-649,
+// 3d/tval3_cpu_3d.cpp:876-947): ubar = u - sigma/beta per component, rounded
+// to the component type R; |ubar| = sqrt(re*re + im*im) in R; w = max(|ubar|
+// - 1/beta, 0) in R.  cshrink_w gives W = w * b / |ubar| per component in R
+// (0 where |ubar| = 0); the reference's 3-D Wz takes b from the y direction.
+template <class C>
+struct CShr {
+  C bar;
+  decltype(C::x) mag, w;
+};
+template <class TS, class C>
+__device__ __forceinline__ CShr<C> cshrink_bar(C u, C sg, TS beta, TS ib) {
+  typedef decltype(C::x) R;
+  CShr<C> r;
+  r.bar = C((R)((TS)u.x - (TS)sg.x / beta), (R)((TS)u.y - (TS)sg.y / beta));
+  r.mag = sqrt(r.bar.x * r.bar.x + r.bar.y * r.bar.y);
+  const R t = (R)((TS)r.mag - ib);
+  r.w = (t >= (R)0) ? t : (R)0;
+  return r;
+}
+template <class C, class R>
+__device__ __forceinline__ C cshrink_w(R w, C b, R mag) {
+  return (mag > (R)0) ? C(w * b.x / mag, w * b.y / mag) : C((R)0, (R)0);
+}
+
 }  // namespace tv3

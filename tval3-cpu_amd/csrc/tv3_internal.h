@@ -24,6 +24,55 @@ constexpr int kBlock = 256;       // threads per block of every vector kernel
 constexpr int kNblkMax = 512;     // max blocks per group of a reduction kernel
 constexpr int kNslot = 24;        // partial-sum slots per instance
 constexpr int kSlotM = 12;        // first slot written by m-vector kernels
+// Im <s, y> of the complex Barzilai-Borwein step: written by the same pass as
+// <s, s> (slot 8) and Re <s, y> (slot 9), read by the next get_tau; no kernel
+// between the two writes slot 10
+constexpr int kSlotSyIm = 10;
+
+// ---- vector element types ----------------------------------------------------
+// Complex elements of the complex solvers (the reference's complex<float> /
+// complex<double> T_comp): x = real part, y = imaginary part.  Only the
+// operations the solver needs are defined: sums and differences, negation and
+// scaling by a real value, each per component.  A real value converts to
+// (value, 0).
+struct __align__(8) cfloat {
+  float x, y;
+  cfloat() = default;
+  __host__ __device__ constexpr cfloat(float r) : x(r), y(0.0f) {}
+  __host__ __device__ constexpr cfloat(float r, float i) : x(r), y(i) {}
+};
+struct __align__(16) cdouble {
+  double x, y;
+  cdouble() = default;
+  __host__ __device__ constexpr cdouble(double r) : x(r), y(0.0) {}
+  __host__ __device__ constexpr cdouble(double r, double i) : x(r), y(i) {}
+};
+#define TV3_CPLX_OPS(C, R)                                                                  \
+  __host__ __device__ inline C operator+(C a, C b) { return C(a.x + b.x, a.y + b.y); }    \
+  __host__ __device__ inline C operator-(C a, C b) { return C(a.x - b.x, a.y - b.y); }    \
+  __host__ __device__ inline C operator-(C a) { return C(-a.x, -a.y); }                   \
+  __host__ __device__ inline C operator*(R s, C a) { return C(s * a.x, s * a.y); }         \
+  __host__ __device__ inline C operator*(C a, R s) { return C(a.x * s, a.y * s); }
+TV3_CPLX_OPS(cfloat, float)
+TV3_CPLX_OPS(cdouble, double)
+#undef TV3_CPLX_OPS
+
+// component type of a vector element
+template <class T>
+struct vreal {
+  typedef T type;
+  static constexpr bool cplx = false;
+};
+template <>
+struct vreal<cfloat> {
+  typedef float type;
+  static constexpr bool cplx = true;
+};
+template <>
+struct vreal<cdouble> {
+  typedef double type;
+  static constexpr bool cplx = true;
+};
 
 // ---- plans ------------------------------------------------------------------
 enum PlanKind { kDense = 0, kSparse = 1, kGeometry = 2 };
@@ -57,6 +106,10 @@ struct PlanDev {
   const float *c_val;
   // double values of a sparse_mat<double> plan (then r_val / c_val are null)
   const double *r_vald, *c_vald;
+  // complex values of a sparse_mat<complex<T>> plan: cfloat (cval = 2) or
+  // cdouble (cval = 3) elements; 0: none
+  const void *r_valx, *c_valx;
+  int cval;
   // geometry
   const RayDesc *rays;
   const int *g_cptr;  // voxel-major transpose: rays through voxel v,
@@ -74,10 +127,18 @@ struct PlanDev {
 // sparse) multiply float vectors, double-valued ones (f64 dense, double
 // sparse) double vectors; the geometry projector serves both.
 __host__ __device__ inline bool plan_takes_f32(const PlanDev &p) {
-  return p.kind == kDense ? p.dtype != TV3_F64 : p.kind == kSparse ? p.r_val != nullptr || p.nnz == 0 : true;
+  return p.kind == kDense ? (p.dtype == TV3_F32 || p.dtype == TV3_BF16)
+                          : p.kind == kSparse ? (p.r_val != nullptr || (p.nnz == 0 && p.cval == 0)) : true;
 }
 __host__ __device__ inline bool plan_takes_f64(const PlanDev &p) {
   return p.kind == kDense ? p.dtype == TV3_F64 : p.kind == kSparse ? p.r_vald != nullptr || p.nnz == 0 : true;
+}
+// complex vectors: complex entries of the same precision, or the projector
+__host__ __device__ inline bool plan_takes_c64(const PlanDev &p) {
+  return p.kind == kDense ? p.dtype == TV3_C64 : p.kind == kSparse ? p.cval == 2 : true;
+}
+__host__ __device__ inline bool plan_takes_c128(const PlanDev &p) {
+  return p.kind == kDense ? p.dtype == TV3_C128 : p.kind == kSparse ? p.cval == 3 : true;
 }
 
 // ---- batch description ------------------------------------------------------
@@ -260,6 +321,28 @@ void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                 int kind, int dtype, const double *x, double *y,
                 const InstScal *S, uint32_t mask, const double *sub,
                 double *scratch, int submat_mb);
+
+// Complex vectors (the complex solvers): W = 1 only.  submat_mb > 0 (dense
+// A): the blocked A^H product of get_g, block sums rounded to the component
+// type and added per component in block order.
+template <int W>
+void launch_fwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                int kind, int dtype, const cfloat *x, cfloat *y,
+                const InstScal *S, uint32_t mask);
+template <int W>
+void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                int kind, int dtype, const cfloat *x, cfloat *y,
+                const InstScal *S, uint32_t mask, const cfloat *sub,
+                cfloat *scratch, int submat_mb);
+template <int W>
+void launch_fwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
+                int kind, int dtype, const cdouble *x, cdouble *y,
+                const InstScal *S, uint32_t mask);
+template <int W>
+void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
+                int kind, int dtype, const cdouble *x, cdouble *y,
+                const InstScal *S, uint32_t mask, const cdouble *sub,
+                cdouble *scratch, int submat_mb);
 
 // rows per block of the blocked dense product (2d/tval3_cpu_2d.cpp:366-369):
 // submat_mb MiB of elem-byte entries

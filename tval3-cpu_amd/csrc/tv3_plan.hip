@@ -75,7 +75,10 @@ __global__ void k_prof_bytes(const GroupDesc *G, int W, int Bpad, const InstScal
   const unsigned long long vec = 4ull * gd.n + 4ull * gd.m;
   unsigned long long b;
   if (pd.kind == kDense) {
-    b = (pd.dtype == TV3_BF16 ? 2ull : pd.dtype == TV3_F64 ? 8ull : 4ull) * m * n + vec;
+    const unsigned long long es = pd.dtype == TV3_BF16 ? 2ull
+                                  : pd.dtype == TV3_C128 ? 16ull
+                                  : (pd.dtype == TV3_F64 || pd.dtype == TV3_C64) ? 8ull : 4ull;
+    b = es * m * n + vec;
   } else {  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
     b = 8 * nnz + 4 * ((trans ? n : m) + 1) + vec;
   }
@@ -1499,6 +1502,238 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
 }
 
 // ---------------------------------------------------------------------------
+// Complex vectors, W = 1: the complex overloads (mat<complex<T>> U with
+// mat<complex<T>>, sparse_mat<complex<T>> or geometry A, 2d/tval3_cpu_2d.cpp:
+// 31-41, 58-68, 85-95).  y = A x takes the terms a x and y = A^H x
+// (CblasConjTrans, 2d:308, 392, 774) the terms conj(a) x; each component of
+// a term is one FP64 rounding of two FP64 products (exact for float
+// components), summed in FP64 and rounded once.  The projector's weights are
+// real: each component times the scale factor in the component type.
+// ---------------------------------------------------------------------------
+template <class C>
+__device__ __forceinline__ void cterm(double &sr, double &si, C a, C v, bool conj) {
+  if (!conj) {
+    sr += (double)a.x * (double)v.x - (double)a.y * (double)v.y;
+    si += (double)a.x * (double)v.y + (double)a.y * (double)v.x;
+  } else {
+    sr += (double)a.x * (double)v.x + (double)a.y * (double)v.y;
+    si += (double)a.x * (double)v.y - (double)a.y * (double)v.x;
+  }
+}
+
+template <int L, class C>
+__global__ __launch_bounds__(kBlock) void k_fwd_w1c(const GroupDesc *G, const C *x, C *y,
+                                                    const InstScal *S, uint32_t mask) {
+  typedef decltype(C::x) R;
+  const GroupDesc &gd = G[blockIdx.y];
+  const int sl = threadIdx.x & (L - 1);
+  const int row = blockIdx.x * (kBlock / L) + (threadIdx.x / L);
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const int ar = row % P.m;
+  const C *xg = x + gd.off_n + (long long)(row / P.m) * P.n;
+  double sr = 0.0, si = 0.0;
+  if (row < gd.m) {
+    if (P.kind == kGeometry) {
+      const RayDesc rd = P.rays[ar];
+      const R sc = (R)P.scale;
+      for (int j = sl; j <= rd.L; j += L) {
+        const C v = xg[ray_lin(rd, j)];
+        sr += (double)(v.x * sc);
+        si += (double)(v.y * sc);
+      }
+    } else if (P.kind == kSparse) {
+      const C *val = (const C *)P.r_valx;
+      const int k1 = P.r_ptr[ar + 1];
+      for (int k = P.r_ptr[ar] + sl; k < k1; k += L) cterm(sr, si, val[k], xg[P.r_ind[k]], false);
+    } else {
+      const C *A = (const C *)P.A + (long long)ar * P.n;
+      for (int j = sl; j < P.n; j += L) cterm(sr, si, A[j], xg[j], false);
+    }
+  }
+  sr = lane_sum<L>(sr);
+  si = lane_sum<L>(si);
+  if (sl == 0 && row < gd.m) y[gd.off_m + (long long)(row / P.m) * P.m + ar] = C((R)sr, (R)si);
+}
+
+template <int L, class C>
+__global__ __launch_bounds__(kBlock) void k_bwd_w1c(const GroupDesc *G, const C *x, C *y,
+                                                    const InstScal *S, uint32_t mask, const C *sub) {
+  typedef decltype(C::x) R;
+  const GroupDesc &gd = G[blockIdx.y];
+  const int sl = threadIdx.x & (L - 1);
+  const int col = blockIdx.x * (kBlock / L) + (threadIdx.x / L);
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const bool geo = P.kind == kGeometry;
+  const bool tiled = geo && P.t_col;
+  int ac = col % P.n;
+  const C *xg = x + gd.off_m + (long long)(col / P.n) * P.m;
+  double sr = 0.0, si = 0.0;
+  if (col < gd.n) {
+    const int *ptr = tiled ? P.t_ptr : geo ? P.g_cptr : P.c_ptr;
+    const int *ind = tiled ? P.t_ray : geo ? P.g_cray : P.c_ind;
+    const int k0 = ptr[ac], k1 = ptr[ac + 1];
+    if (tiled) ac = P.t_col[col];
+    const R sc = (R)P.scale;
+    const C *val = (const C *)P.c_valx;
+    for (int k = k0 + sl; k < k1; k += L) {
+      const C v = xg[ind[k]];
+      if (geo) {
+        sr += (double)(v.x * sc);
+        si += (double)(v.y * sc);
+      } else {
+        cterm(sr, si, val[k], v, true);
+      }
+    }
+  }
+  sr = lane_sum<L>(sr);
+  si = lane_sum<L>(si);
+  if (sl == 0 && col < gd.n) {
+    const long long o = gd.off_n + (long long)(col / P.n) * P.n + ac;
+    const C v((R)sr, (R)si);
+    y[o] = sub ? v - sub[o] : v;
+  }
+}
+
+// dense A^H x: chunk c of a group covers rows [c*R, min(m, (c+1)*R)); a
+// thread owns one column and writes its FP64 chunk sums (re, im)
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_cdense_bwd_part(const GroupDesc *G, const C *x,
+                                                            double *scratch, const InstScal *S,
+                                                            uint32_t mask, long long sstride) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m;
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (pm + 63) / 64);
+  const int c = blockIdx.y;
+  if (c >= nch) return;
+  const int Rr = (pm + nch - 1) / nch;
+  const int i0 = c * Rr, i1 = min(pm, i0 + Rr);
+  const C *A = (const C *)gd.plan.A + col % n;
+  const C *xg = x + gd.off_m + (long long)(col / n) * pm;
+  double sr = 0.0, si = 0.0;
+  for (int i = i0; i < i1; ++i) cterm(sr, si, A[(long long)i * n], xg[i], true);
+  double *out = scratch + blockIdx.z * sstride + 2 * ((long long)c * gd.n + col);
+  out[0] = sr;
+  out[1] = si;
+}
+
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_cdense_bwd_reduce(const GroupDesc *G, const double *scratch,
+                                                              C *y, const InstScal *S, uint32_t mask,
+                                                              const C *sub, long long sstride) {
+  typedef decltype(C::x) R;
+  const GroupDesc &gd = G[blockIdx.y];
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (gd.plan.m + 63) / 64);
+  const double *sp = scratch + blockIdx.y * sstride + 2 * (long long)col;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nch; ++c) {
+    sr += sp[2 * (long long)c * gd.n];
+    si += sp[2 * (long long)c * gd.n + 1];
+  }
+  const long long o = gd.off_n + col;
+  const C v((R)sr, (R)si);
+  y[o] = sub ? v - sub[o] : v;
+}
+
+// get_g's blocked A^H (A U) (2d/tval3_cpu_2d.cpp:361-394) with complex
+// entries, one layer: a thread owns one column and walks the rows in order;
+// FP64 within a block of rows, each block sum rounded per component and
+// added in the component type in block order.  y = g - sub.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_cdense_bwd_blk(const GroupDesc *G, const C *x, C *y,
+                                                           const InstScal *S, uint32_t mask,
+                                                           const C *sub, int submat_mb) {
+  typedef decltype(C::x) R;
+  const GroupDesc &gd = G[blockIdx.y];
+  const int col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= gd.n) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int n = gd.plan.n, m = gd.plan.m;
+  const int rb = dense_block_rows(submat_mb, n, (int)sizeof(C));
+  const C *A = (const C *)gd.plan.A + col;
+  const C *xg = x + gd.off_m;
+  R gr = 0, gi = 0;
+  for (int y0 = 0; y0 < m; y0 += rb) {
+    const int y1 = min(m, y0 + rb);
+    double sr = 0.0, si = 0.0;
+    for (int i = y0; i < y1; ++i) cterm(sr, si, A[(long long)i * n], xg[i], true);
+    gr = gr + (R)sr;
+    gi = gi + (R)si;
+  }
+  const long long o = gd.off_n + col;
+  const C v(gr, gi);
+  y[o] = sub ? v - sub[o] : v;
+}
+
+template <class C>
+static void launch_fwd_c(hipStream_t s, const GroupDesc *G_dev, int G, int max_m, int kind,
+                         const C *x, C *y, const InstScal *S, uint32_t mask) {
+  if (kind == kDense) {
+    hipLaunchKernelGGL((k_fwd_w1c<64, C>), dim3((max_m + kBlock / 64 - 1) / (kBlock / 64), G), dim3(kBlock), 0,
+                       s, G_dev, x, y, S, mask);
+    return;
+  }
+  hipLaunchKernelGGL((k_fwd_w1c<kSubF, C>), dim3((max_m + kBlock / kSubF - 1) / (kBlock / kSubF), G),
+                     dim3(kBlock), 0, s, G_dev, x, y, S, mask);
+}
+
+template <class C>
+static void launch_bwd_c(hipStream_t s, const GroupDesc *G_dev, int G, int max_n, int kind,
+                         const C *x, C *y, const InstScal *S, uint32_t mask, const C *sub,
+                         C *scratch, int submat_mb) {
+  if (kind == kDense && submat_mb > 0) {
+    hipLaunchKernelGGL((k_cdense_bwd_blk<C>), dim3((max_n + kBlock - 1) / kBlock, G), dim3(kBlock), 0, s,
+                       G_dev, x, y, S, mask, sub, submat_mb);
+    return;
+  }
+  if (kind == kDense) {
+    // scratch holds 2 * kDenseChunks * max_n doubles per group
+    const long long sstride = 2LL * kDenseChunks * max_n;
+    hipLaunchKernelGGL((k_cdense_bwd_part<C>), dim3((max_n + kBlock - 1) / kBlock, kDenseChunks, G),
+                       dim3(kBlock), 0, s, G_dev, x, (double *)scratch, S, mask, sstride);
+    hipLaunchKernelGGL((k_cdense_bwd_reduce<C>), dim3((max_n + kBlock - 1) / kBlock, G), dim3(kBlock), 0, s,
+                       G_dev, (const double *)scratch, y, S, mask, sub, sstride);
+    return;
+  }
+  hipLaunchKernelGGL((k_bwd_w1c<kSubT, C>), dim3((max_n + kBlock / kSubT - 1) / (kBlock / kSubT), G),
+                     dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
+}
+
+template <>
+void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m, int kind, int dtype,
+                   const cfloat *x, cfloat *y, const InstScal *S, uint32_t mask) {
+  (void)dtype;
+  launch_fwd_c(s, G_dev, G, max_m, kind, x, y, S, mask);
+}
+template <>
+void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m, int kind, int dtype,
+                   const cdouble *x, cdouble *y, const InstScal *S, uint32_t mask) {
+  (void)dtype;
+  launch_fwd_c(s, G_dev, G, max_m, kind, x, y, S, mask);
+}
+template <>
+void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n, int kind, int dtype,
+                   const cfloat *x, cfloat *y, const InstScal *S, uint32_t mask, const cfloat *sub,
+                   cfloat *scratch, int submat_mb) {
+  (void)dtype;
+  launch_bwd_c(s, G_dev, G, max_n, kind, x, y, S, mask, sub, scratch, submat_mb);
+}
+template <>
+void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n, int kind, int dtype,
+                   const cdouble *x, cdouble *y, const InstScal *S, uint32_t mask, const cdouble *sub,
+                   cdouble *scratch, int submat_mb) {
+  (void)dtype;
+  launch_bwd_c(s, G_dev, G, max_n, kind, x, y, S, mask, sub, scratch, submat_mb);
+}
+
+// ---------------------------------------------------------------------------
 // geometry plan construction
 // ---------------------------------------------------------------------------
 
@@ -1664,8 +1899,10 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
 int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,
                    tv3_plan **out) {
   if (!ctx || !out) return TV3_ERR_ARG;
-  if (m <= 0 || n <= 0 || !A || (dtype != TV3_F32 && dtype != TV3_BF16 && dtype != TV3_F64))
-    return set_err(ctx, TV3_ERR_ARG, "Argument error: dense plan needs m, n > 0, A and dtype f32/bf16/f64");
+  if (m <= 0 || n <= 0 || !A || dtype < TV3_F32 || dtype > TV3_C128)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: dense plan needs m, n > 0, A and dtype f32/bf16/f64/c64/c128");
+  if ((dtype == TV3_C64 && (uintptr_t)A % 8) || (dtype == TV3_C128 && (uintptr_t)A % 16))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: complex A must be aligned to its element size");
   tv3_plan *p = new tv3_plan();
   p->ctx = ctx;
   p->dev.kind = kDense; p->dev.m = m; p->dev.n = n; p->dev.A = A;
@@ -1717,6 +1954,42 @@ int tv3_plan_sparse_f64(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
   p->dev.nnz = nnz;
   *out = p;
   return TV3_OK;
+}
+
+static int plan_sparse_c(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                         const int *csr_ind, const void *csr_val, const int *csc_ptr,
+                         const int *csc_ind, const void *csc_val, int cval, tv3_plan **out) {
+  if (!ctx || !out) return TV3_ERR_ARG;
+  if (m <= 0 || n <= 0 || nnz < 0)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: sparse plan needs m, n > 0");
+  if (!csr_ptr || !csc_ptr || (nnz > 0 && (!csr_ind || !csr_val || !csc_ind || !csc_val)))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: A must be available in CSR and CSC format!");
+  const uintptr_t al = cval == 2 ? 8 : 16;
+  if ((uintptr_t)csr_val % al || (uintptr_t)csc_val % al)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: complex values must be aligned to their element size");
+  tv3_plan *p = new tv3_plan();
+  p->ctx = ctx;
+  PlanDev &d = p->dev;
+  d.kind = kSparse; d.m = m; d.n = n;
+  d.r_ptr = csr_ptr; d.r_ind = csr_ind; d.r_valx = csr_val;
+  d.c_ptr = csc_ptr; d.c_ind = csc_ind; d.c_valx = csc_val;
+  d.cval = cval;
+  p->nnz = nnz;
+  p->dev.nnz = nnz;
+  *out = p;
+  return TV3_OK;
+}
+
+int tv3_plan_sparse_c64(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                        const int *csr_ind, const float *csr_val, const int *csc_ptr,
+                        const int *csc_ind, const float *csc_val, tv3_plan **out) {
+  return plan_sparse_c(ctx, m, n, nnz, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind, csc_val, 2, out);
+}
+
+int tv3_plan_sparse_c128(tv3_ctx *ctx, int m, int n, int nnz, const int *csr_ptr,
+                         const int *csr_ind, const double *csr_val, const int *csc_ptr,
+                         const int *csc_ind, const double *csc_val, tv3_plan **out) {
+  return plan_sparse_c(ctx, m, n, nnz, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind, csc_val, 3, out);
 }
 
 int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {

@@ -44,20 +44,115 @@ using namespace tv3;
 #ifndef TV3_VEC
 #define TV3_VEC float
 #endif
+// 0: float vectors (this unit); 1: double (tv3_solver_f64.hip); 2: complex
+// float (tv3_solver_c64.hip); 3: complex double (tv3_solver_c128.hip)
+#ifndef TV3_SOLVER_VARIANT
+#define TV3_SOLVER_VARIANT 0
+#endif
 
 namespace {
 
 // Vector (T_comp) type of this translation unit: float here; the FP64
 // engine (tv3_solver_f64.hip) compiles this file again with double.
 typedef TV3_VEC vec_t;
+// component type (the type of every T_comp scalar: with complex T_comp those
+// stay real-valued)
+typedef typename vreal<vec_t>::type vr_t;
+constexpr bool kCplx = vreal<vec_t>::cplx;
 __device__ __forceinline__ float vfma(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double vfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ cfloat vfma(float a, cfloat b, cfloat c) {
+  return cfloat(fmaf(a, b.x, c.x), fmaf(a, b.y, c.y));
+}
+__device__ __forceinline__ cdouble vfma(double a, cdouble b, cdouble c) {
+  return cdouble(fma(a, b.x, c.x), fma(a, b.y, c.y));
+}
 __device__ __forceinline__ float vabs(float a) { return fabsf(a); }
 __device__ __forceinline__ double vabs(double a) { return fabs(a); }
+// |z|: a value on an axis gives its one component's magnitude, else the FP64
+// root of the FP64 sum of squares, rounded once
+__device__ __forceinline__ double cmod(double a, double b) {
+  if (b == 0) return fabs(a);
+  if (a == 0) return fabs(b);
+  return sqrt(a * a + b * b);
+}
+__device__ __forceinline__ float vmod(float a) { return fabsf(a); }
+__device__ __forceinline__ double vmod(double a) { return fabs(a); }
+__device__ __forceinline__ float vmod(cfloat a) { return (float)cmod((double)a.x, (double)a.y); }
+__device__ __forceinline__ double vmod(cdouble a) { return cmod(a.x, a.y); }
+// terms of the FP64 dot products: real, the exact product; complex
+// (conj(a) b), Re = a.x b.x + a.y b.y and Im = a.x b.y - a.y b.x, one FP64
+// rounding each
+__device__ __forceinline__ double dre(float a, float b) { return (double)a * (double)b; }
+__device__ __forceinline__ double dre(double a, double b) { return a * b; }
+__device__ __forceinline__ double dre(cfloat a, cfloat b) {
+  return (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+}
+__device__ __forceinline__ double dre(cdouble a, cdouble b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ double dimag(float, float) { return 0.0; }
+__device__ __forceinline__ double dimag(double, double) { return 0.0; }
+__device__ __forceinline__ double dimag(cfloat a, cfloat b) {
+  return (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+}
+__device__ __forceinline__ double dimag(cdouble a, cdouble b) { return a.x * b.y - a.y * b.x; }
+// vector values widened to FP64 (the quadratic line-search coefficients)
+__device__ __forceinline__ double to_vd(float a) { return (double)a; }
+__device__ __forceinline__ double to_vd(double a) { return a; }
+__device__ __forceinline__ cdouble to_vd(cfloat a) { return cdouble((double)a.x, (double)a.y); }
+__device__ __forceinline__ cdouble to_vd(cdouble a) { return a; }
+// nonneg (2d:455-479): complex keeps the clamped real part only
+__device__ __forceinline__ float clamp0(float u) { return (u < 0.0f) ? 0.0f : u; }
+__device__ __forceinline__ double clamp0(double u) { return (u < 0.0f) ? 0.0f : u; }
+__device__ __forceinline__ cfloat clamp0(cfloat u) { return cfloat((u.x < 0.0f) ? 0.0f : u.x, 0.0f); }
+__device__ __forceinline__ cdouble clamp0(cdouble u) { return cdouble((u.x < 0.0) ? 0.0 : u.x, 0.0); }
+// fma(alpha, y, x) in precision TS, per component, rounded to the component
+template <class TS>
+__device__ __forceinline__ float vfma_s(TS alpha, float y, float x) { return (float)fma(alpha, (TS)y, (TS)x); }
+template <class TS>
+__device__ __forceinline__ double vfma_s(TS alpha, double y, double x) { return (double)fma(alpha, (TS)y, (TS)x); }
+template <class TS>
+__device__ __forceinline__ cfloat vfma_s(TS alpha, cfloat y, cfloat x) {
+  return cfloat((float)fma(alpha, (TS)y.x, (TS)x.x), (float)fma(alpha, (TS)y.y, (TS)x.y));
+}
+template <class TS>
+__device__ __forceinline__ cdouble vfma_s(TS alpha, cdouble y, cdouble x) {
+  return cdouble((double)fma(alpha, (TS)y.x, (TS)x.x), (double)fma(alpha, (TS)y.y, (TS)x.y));
+}
+// U / scl, complex with isreal: the real part only (driver, 2d:855-860)
+template <class TS>
+__device__ __forceinline__ float vdiv_s(float u, TS scl, int) { return (float)((TS)u / scl); }
+template <class TS>
+__device__ __forceinline__ double vdiv_s(double u, TS scl, int) { return (double)((TS)u / scl); }
+template <class TS>
+__device__ __forceinline__ cfloat vdiv_s(cfloat u, TS scl, int isreal) {
+  return cfloat((float)((TS)u.x / scl), (float)((TS)(isreal ? 0.0f : u.y) / scl));
+}
+template <class TS>
+__device__ __forceinline__ cdouble vdiv_s(cdouble u, TS scl, int isreal) {
+  return cdouble((double)((TS)u.x / scl), (double)((TS)(isreal ? 0.0 : u.y) / scl));
+}
 // per-instance scalars with T_comp fields of vec_t; the operator launches
 // read only their flags and take them through InstScal (same layout)
-constexpr bool kVecF64 = sizeof(vec_t) == sizeof(double);
-typedef InstScalT<vec_t> Scal;
+// the double and complex solvers run one instance per group (W = 1)
+constexpr bool kOnlyW1 = TV3_SOLVER_VARIANT != 0;
+// the plans the vector type takes, and the entry point named in errors
+inline bool plan_ok(const PlanDev &p) {
+  return TV3_SOLVER_VARIANT == 1   ? plan_takes_f64(p)
+         : TV3_SOLVER_VARIANT == 2 ? plan_takes_c64(p)
+         : TV3_SOLVER_VARIANT == 3 ? plan_takes_c128(p)
+                                   : plan_takes_f32(p);
+}
+constexpr const char *kPlanErr =
+    TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_solve_f64 needs an f64 dense, double sparse or geometry plan"
+    : TV3_SOLVER_VARIANT == 2 ? "Argument error: tv3_solve_c64 needs a c64 dense, complex-float sparse or geometry plan"
+    : TV3_SOLVER_VARIANT == 3 ? "Argument error: tv3_solve_c128 needs a c128 dense, complex-double sparse or geometry plan"
+                              : "Argument error: an f64 / double-valued or complex plan needs its own entry point "
+                                "(tv3_solve_f64, tv3_solve_c64, tv3_solve_c128)";
+constexpr const char *kGemvErr =
+    TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_gemv_f64 needs an f64 dense, double sparse or geometry plan"
+    : TV3_SOLVER_VARIANT == 2 ? "Argument error: tv3_gemv_c64 needs a c64 dense, complex-float sparse or geometry plan"
+                              : "Argument error: tv3_gemv_c128 needs a c128 dense, complex-double sparse or geometry plan";
+typedef InstScalT<vr_t> Scal;
 inline const InstScal *flags_of(const Scal *s) { return reinterpret_cast<const InstScal *>(s); }
 
 struct Arrs {
@@ -132,13 +227,13 @@ __global__ __launch_bounds__(kBlock) void k_tau_first(KA K, Arrs a) {
     const vec_t dzn = z3 ? AT(a.d, w.zn()) : 0.0f;
     const vec_t dx = dxn - d0;
     const vec_t dy = dyn - d0;
-    acc[0] += (double)dx * (double)dx;
-    acc[1] += (double)dy * (double)dy;
+    acc[0] += dre(dx, dx);
+    acc[1] += dre(dy, dy);
     if (z3) {
       const vec_t dz = dzn - d0;
-      acc[2] += (double)dz * (double)dz;
+      acc[2] += dre(dz, dz);
     }
-    acc[3] += (double)d0 * (double)d0;
+    acc[3] += dre(d0, d0);
     LOOP_END
   }
   const int sl[4] = {0, 1, 2, 3};
@@ -151,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_msq(KA K, const vec_t *x) {
   {
     MLOOP_BEGIN(W)
     const vec_t t = x[o];
-    acc[0] += (double)t * (double)t;
+    acc[0] += dre(t, t);
     LOOP_END
   }
   const int sl[1] = {kSlotM};
@@ -175,10 +270,10 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
   double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     NLOOP_BEGIN(W)
-    const vec_t tf = (vec_t)(-1 * (TS)S.tau);
+    const vr_t tf = (vr_t)(-1 * (TS)S.tau);
     auto un = [&](vec_t dd, vec_t pp) -> vec_t {
       vec_t u = vfma(tf, dd, pp);
-      if (nonneg) u = (u < 0.0f) ? 0.0f : u;
+      if (nonneg) u = clamp0(u);
       return u;
     };
     auto unj = [&](int j) -> vec_t { return un(AT(a.d, j), AT(a.Up, j)); };
@@ -214,25 +309,25 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
     const vec_t Vx0 = ux - wx;
     const vec_t Vy0 = uy - wy;
     const vec_t Vz0 = z3 ? uz - wz : 0.0f;
-    acc[0] += (double)Vx0 * (double)Vx0;
-    acc[1] += (double)Vy0 * (double)Vy0;
-    acc[3] += (double)sx * (double)Vx0;
-    acc[4] += (double)sy * (double)Vy0;
+    acc[0] += dre(Vx0, Vx0);
+    acc[1] += dre(Vy0, Vy0);
+    acc[3] += dre(sx, Vx0);
+    acc[4] += dre(sy, Vy0);
     if (z3) {
-      acc[2] += (double)Vz0 * (double)Vz0;
-      acc[5] += (double)sz * (double)Vz0;
+      acc[2] += dre(Vz0, Vz0);
+      acc[5] += dre(sz, Vz0);
     }
-    acc[6] += (double)d0 * (double)d0;
+    acc[6] += dre(d0, d0);
     if (POLY) {
-      const double vx = (double)qx - (double)wx, dx = (double)(ux - qx);
-      const double vy = (double)qy - (double)wy, dy = (double)(uy - qy);
-      double A2 = vx * vx + vy * vy, B2 = vx * dx + vy * dy, C2 = dx * dx + dy * dy;
-      double A4 = (double)sx * vx + (double)sy * vy;
-      double B4 = (double)sx * dx + (double)sy * dy;
+      const auto vx = to_vd(qx) - to_vd(wx), dx = to_vd(ux - qx);
+      const auto vy = to_vd(qy) - to_vd(wy), dy = to_vd(uy - qy);
+      double A2 = dre(vx, vx) + dre(vy, vy), B2 = dre(vx, dx) + dre(vy, dy), C2 = dre(dx, dx) + dre(dy, dy);
+      double A4 = dre(to_vd(sx), vx) + dre(to_vd(sy), vy);
+      double B4 = dre(to_vd(sx), dx) + dre(to_vd(sy), dy);
       if (z3) {
-        const double vz = (double)qz - (double)wz, dz = (double)(uz - qz);
-        A2 += vz * vz; B2 += vz * dz; C2 += dz * dz;
-        A4 += (double)sz * vz; B4 += (double)sz * dz;
+        const auto vz = to_vd(qz) - to_vd(wz), dz = to_vd(uz - qz);
+        A2 += dre(vz, vz); B2 += dre(vz, dz); C2 += dre(dz, dz);
+        A4 += dre(to_vd(sz), vz); B4 += dre(to_vd(sz), dz);
       }
       acc[7] += A2; acc[8] += B2; acc[9] += C2; acc[10] += A4; acc[11] += B4;
     }
@@ -290,20 +385,20 @@ __global__ __launch_bounds__(kBlock) void k_mpost(KA K, Arrs a) {
     const vec_t la = SKIP ? a.lastAu[o] : 0.0f;
     const vec_t aup = POLY ? a.Aup[o] : 0.0f;
     const vec_t aub = au - bb;
-    acc[0] += (double)aub * (double)aub;
-    acc[1] += (double)dl * (double)aub;
+    acc[0] += dre(aub, aub);
+    acc[1] += dre(dl, aub);
     if (SKIP) {
       const vec_t da = au - la;
-      acc[2] += (double)da * (double)da;
-      acc[3] += (double)au * (double)au;
+      acc[2] += dre(da, da);
+      acc[3] += dre(au, au);
     }
     if (POLY) {
-      const double ap = (double)aup - (double)bb, da = (double)(au - aup);
-      acc[4] += ap * ap;
-      acc[5] += ap * da;
-      acc[6] += da * da;
-      acc[7] += (double)dl * ap;
-      acc[8] += (double)dl * da;
+      const auto ap = to_vd(aup) - to_vd(bb), da = to_vd(au - aup);
+      acc[4] += dre(ap, ap);
+      acc[5] += dre(ap, da);
+      acc[6] += dre(da, da);
+      acc[7] += dre(to_vd(dl), ap);
+      acc[8] += dre(to_vd(dl), da);
     }
     LOOP_END
   }
@@ -357,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_diff_m(KA K, Arrs a) {
 
 template <class TS>
 __device__ __forceinline__ vec_t interp(vec_t x, TS alpha, vec_t y) {
-  return (vec_t)fma(alpha, (TS)y, (TS)x);
+  return vfma_s(alpha, y, x);
 }
 
 template <int W, class TS>
@@ -383,16 +478,16 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
     const vec_t ux = interp<TS>(uxp, al, dux);
     const vec_t uy = interp<TS>(uyp, al, duy);
     const vec_t Vx = ux - wx, Vy = uy - wy;
-    acc[0] += (double)Vx * (double)Vx;
-    acc[1] += (double)Vy * (double)Vy;
-    acc[3] += (double)sx * (double)Vx;
-    acc[4] += (double)sy * (double)Vy;
+    acc[0] += dre(Vx, Vx);
+    acc[1] += dre(Vy, Vy);
+    acc[3] += dre(sx, Vx);
+    acc[4] += dre(sy, Vy);
     vec_t uz = 0.0f;
     if (z3) {
       uz = interp<TS>(uzp, al, duz);
       const vec_t Vz = uz - wz;
-      acc[2] += (double)Vz * (double)Vz;
-      acc[5] += (double)sz * (double)Vz;
+      acc[2] += dre(Vz, Vz);
+      acc[5] += dre(sz, Vz);
     }
     // ---- stores
     a.g[o] = interp<TS>(gp, al, dg);
@@ -416,8 +511,8 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_m(KA K, Arrs a) {
     const vec_t au = interp<TS>(a.Aup[o], al, a.dAu[o]);
     a.Au[o] = au;
     const vec_t aub = au - a.b[o];
-    acc[0] += (double)aub * (double)aub;
-    acc[1] += (double)a.delta[o] * (double)aub;
+    acc[0] += dre(aub, aub);
+    acc[1] += dre(a.delta[o], aub);
     LOOP_END
   }
   const int sl[2] = {kSlotM, kSlotM + 1};
@@ -451,16 +546,16 @@ __global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
     const vec_t ux = interp<TS>(uxp, al, uxc - uxp);
     const vec_t uy = interp<TS>(uyp, al, uyc - uyp);
     const vec_t Vx = ux - wx, Vy = uy - wy;
-    acc[0] += (double)Vx * (double)Vx;
-    acc[1] += (double)Vy * (double)Vy;
-    acc[3] += (double)sx * (double)Vx;
-    acc[4] += (double)sy * (double)Vy;
+    acc[0] += dre(Vx, Vx);
+    acc[1] += dre(Vy, Vy);
+    acc[3] += dre(sx, Vx);
+    acc[4] += dre(sy, Vy);
     vec_t uz = 0.0f;
     if (z3) {
       uz = interp<TS>(uzp, al, uzc - uzp);
       const vec_t Vz = uz - wz;
-      acc[2] += (double)Vz * (double)Vz;
-      acc[5] += (double)sz * (double)Vz;
+      acc[2] += dre(Vz, Vz);
+      acc[5] += dre(sz, Vz);
     }
     // ---- stores
     a.g[o] = interp<TS>(gp, al, gc - gp);
@@ -485,8 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_interp_m(KA K, Arrs a) {
     const vec_t au = interp<TS>(aup, al, a.Au[o] - aup);
     a.Au[o] = au;
     const vec_t aub = au - a.b[o];
-    acc[0] += (double)aub * (double)aub;
-    acc[1] += (double)a.delta[o] * (double)aub;
+    acc[0] += dre(aub, aub);
+    acc[1] += dre(a.delta[o], aub);
     LOOP_END
   }
   const int sl[2] = {kSlotM, kSlotM + 1};
@@ -514,7 +609,9 @@ __global__ __launch_bounds__(kBlock) void k_D(KA K, const vec_t *U, vec_t *Ux,
 // ---------------------------------------------------------------------------
 template <int W, class TS, bool INIT, bool D3, bool ISO>
 __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
-  double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // complex: slot kSlotSyIm also takes Im <s, y>
+  constexpr int NW = kCplx ? 11 : 10;
+  double acc[NW] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   {
     NLOOP_BEGIN(W)
     const TS beta = (TS)S.beta;
@@ -530,21 +627,36 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
       const long long oj = gd.off_n + (long long)j * W + lb;
       Cell c{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
       if (ISO || dir <= 0) { c.ux = a.Ux[oj]; c.sx = a.sx[oj]; }
-      if (ISO || dir < 0 || dir == 1) { c.uy = a.Uy[oj]; c.sy = a.sy[oj]; }
+      // the reference's complex 3-D Wz takes the y direction's ubar
+      if (ISO || dir < 0 || dir == 1 || (kCplx && dir == 2 && !exact)) { c.uy = a.Uy[oj]; c.sy = a.sy[oj]; }
       if (z3 && (ISO || dir < 0 || dir == 2)) { c.uz = a.Uz[oj]; c.sz = a.sz[oj]; }
       return c;
     };
     // shrinkage of one voxel; returns the T_scal lam1 term
     auto shr = [&](const Cell &c, vec_t &wx, vec_t &wy, vec_t &wz) -> TS {
-      if (ISO)
-        return shrink_iso<TS>(c.ux, c.uy, c.uz, c.sx, c.sy, c.sz, z3, beta, ib, wx, wy, wz);
-      TS tx, ty, tz = 0;
-      wx = shrink1<TS>(c.ux, c.sx, beta, ib, tx);
-      wy = shrink1<TS>(c.uy, c.sy, beta, ib, ty);
-      wz = z3 ? shrink1<TS>(c.uz, c.sz, beta, ib, tz) : 0.0f;
-      TS t = tx + ty;
-      if (z3) t = t + tz;
-      return t;
+      if constexpr (kCplx) {
+        const auto bx = cshrink_bar(c.ux, c.sx, beta, ib), by = cshrink_bar(c.uy, c.sy, beta, ib);
+        wx = cshrink_w(bx.w, bx.bar, bx.mag);
+        wy = cshrink_w(by.w, by.bar, by.mag);
+        TS t = bx.w + by.w;
+        wz = 0.0f;
+        if (z3) {
+          const auto bz = cshrink_bar(c.uz, c.sz, beta, ib);
+          wz = cshrink_w(bz.w, exact ? bz.bar : by.bar, bz.mag);
+          t = t + bz.w;
+        }
+        return t;
+      } else {
+        if (ISO)
+          return shrink_iso<TS>(c.ux, c.uy, c.uz, c.sx, c.sy, c.sz, z3, beta, ib, wx, wy, wz);
+        TS tx, ty, tz = 0;
+        wx = shrink1<TS>(c.ux, c.sx, beta, ib, tx);
+        wy = shrink1<TS>(c.uy, c.sy, beta, ib, ty);
+        wz = z3 ? shrink1<TS>(c.uz, c.sz, beta, ib, tz) : 0.0f;
+        TS t = tx + ty;
+        if (z3) t = t + tz;
+        return t;
+      }
     };
     // ---- loads
     const int xp = w.xp(), yp = w.yp();
@@ -572,13 +684,13 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
     acc[0] += (double)l1;
     const vec_t Vx0 = c0.ux - wx0, Vy0 = c0.uy - wy0;
     const vec_t Vz0 = z3 ? c0.uz - wz0 : 0.0f;
-    acc[1] += (double)Vx0 * (double)Vx0;
-    acc[2] += (double)Vy0 * (double)Vy0;
-    acc[4] += (double)c0.sx * (double)Vx0;
-    acc[5] += (double)c0.sy * (double)Vy0;
+    acc[1] += dre(Vx0, Vx0);
+    acc[2] += dre(Vy0, Vy0);
+    acc[4] += dre(c0.sx, Vx0);
+    acc[5] += dre(c0.sy, Vy0);
     if (z3) {
-      acc[3] += (double)Vz0 * (double)Vz0;
-      acc[6] += (double)c0.sz * (double)Vz0;
+      acc[3] += dre(Vz0, Vz0);
+      acc[6] += dre(c0.sz, Vz0);
     }
     // ---- g2 = Dt(V) at v
     vec_t tx, ty, tz;
@@ -612,9 +724,9 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
     vec_t dn = 0.0f;
     if (!INIT) {
       const vec_t uu = uc - up;
-      acc[7] += (double)up * (double)up;
-      acc[8] += (double)uu * (double)uu;
-      const vec_t mdb = (vec_t)S.muDbeta;
+      acc[7] += dre(up, up);
+      acc[8] += dre(uu, uu);
+      const vr_t mdb = (vr_t)S.muDbeta;
       dn = vfma(mdb, gc, dts + t);
       // the next iteration's Barzilai-Borwein step (get_tau, 2d:443-450):
       // s = U - Up, y = (g2 - g2p) + (mu/beta) (g - gp).  No kernel between
@@ -623,7 +735,8 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
       // first-iteration formula instead), so the dot products are formed
       // here; slot 8 holds <s, s>.
       const vec_t y = vfma(mdb, gc - gpv, t - g2pv);
-      acc[9] += (double)uu * (double)y;
+      acc[9] += dre(uu, y);
+      if constexpr (kCplx) acc[NW - 1] += dimag(uu, y);
     }
     // ---- stores
     a.Wx[o] = wx0;
@@ -633,8 +746,13 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
     if (!INIT) a.d[o] = dn;
     LOOP_END
   }
-  const int sl[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  block_partials<W, 10>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+  if constexpr (kCplx) {
+    const int sl[11] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, kSlotSyIm};
+    block_partials<W, 11>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+  } else {
+    const int sl[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+    block_partials<W, 10>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
+  }
 }
 
 // d = DtsAtd + g2 + muDbeta*g, with DtsAtd = (beta0/beta) d first when
@@ -646,10 +764,10 @@ __global__ __launch_bounds__(kBlock) void k_grad(KA K, Arrs a, int scale) {
   const vec_t dc = scale ? a.d[o] : 0.0f;
   const vec_t gc = a.g[o], g2c = a.g2[o];
   if (scale) {
-    dts = (vec_t)S.ratio * dc;
+    dts = (vr_t)S.ratio * dc;
     a.DtsAtd[o] = dts;
   }
-  a.d[o] = vfma((vec_t)S.muDbeta, gc, dts + g2c);
+  a.d[o] = vfma((vr_t)S.muDbeta, gc, dts + g2c);
   LOOP_END
 }
 
@@ -663,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_outer1(KA K, Arrs a) {
     NLOOP_BEGIN(W)
     const vec_t u = a.U[o];
     const vec_t df = u - a.rcdU[o];
-    acc[0] += (double)df * (double)df;
+    acc[0] += dre(df, df);
     a.rcdU[o] = u;
     LOOP_END
   }
@@ -676,7 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_n(KA K, Arrs a) {
   double acc[3] = {0, 0, 0};
   {
     NLOOP_BEGIN(W)
-    const vec_t nb = (vec_t)(-1 * S.beta);
+    const vr_t nb = (vr_t)(-1 * S.beta);
     const bool z3 = gd.r > 1;
     const vec_t Vx = a.Ux[o] - a.Wx[o], Vy = a.Uy[o] - a.Wy[o];
     const vec_t sx = vfma(nb, Vx, a.sx[o]), sy = vfma(nb, Vy, a.sy[o]);
@@ -687,11 +805,11 @@ __global__ __launch_bounds__(kBlock) void k_mlp_n(KA K, Arrs a) {
     }
     a.sx[o] = sx;
     a.sy[o] = sy;
-    acc[0] += (double)sx * (double)Vx;
-    acc[1] += (double)sy * (double)Vy;
+    acc[0] += dre(sx, Vx);
+    acc[1] += dre(sy, Vy);
     if (z3) {
       a.sz[o] = sz;
-      acc[2] += (double)sz * (double)Vz;
+      acc[2] += dre(sz, Vz);
     }
     LOOP_END
   }
@@ -704,11 +822,11 @@ __global__ __launch_bounds__(kBlock) void k_mlp_m(KA K, Arrs a) {
   double acc[1] = {0};
   {
     MLOOP_BEGIN(W)
-    const vec_t nm = (vec_t)(-1 * S.mu);
+    const vr_t nm = (vr_t)(-1 * S.mu);
     const vec_t aub = a.Au[o] - a.b[o];
     const vec_t dl = vfma(nm, aub, a.delta[o]);
     a.delta[o] = dl;
-    acc[0] += (double)dl * (double)aub;
+    acc[0] += dre(dl, aub);
     LOOP_END
   }
   const int sl[1] = {kSlotM};
@@ -717,9 +835,9 @@ __global__ __launch_bounds__(kBlock) void k_mlp_m(KA K, Arrs a) {
 
 // U_out = U / scl (driver, 2d:856-860)
 template <int W, class TS>
-__global__ __launch_bounds__(kBlock) void k_finish(KA K, Arrs a) {
+__global__ __launch_bounds__(kBlock) void k_finish(KA K, Arrs a, int isreal) {
   NLOOP_BEGIN(W)
-  a.Uout[o] = (vec_t)((TS)a.U[o] / (TS)S.scl);
+  a.Uout[o] = vdiv_s(a.U[o], (TS)S.scl, isreal);
   LOOP_END
 }
 
@@ -732,8 +850,8 @@ __global__ __launch_bounds__(kBlock) void k_relerr(KA K, const vec_t *Ut,
     NLOOP_BEGIN(W)
     const vec_t t = Ut[o];
     const vec_t df = t - U[o];
-    acc[0] += (double)df * (double)df;
-    acc[1] += (double)t * (double)t;
+    acc[0] += dre(df, df);
+    acc[1] += dre(t, t);
     LOOP_END
   }
   const int sl[2] = {0, 1};
@@ -744,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_relerr(KA K, const vec_t *Ut,
 // scaleb (2d:228-266): first index of min |b| and of max |b|
 // ---------------------------------------------------------------------------
 struct MinMax {
-  vec_t mn, mx;
+  vr_t mn, mx;
   int imn, imx;
 };
 
@@ -763,7 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const vec_t *bv,
   const long long E = (long long)gd.m * W;
   for (long long e = (long long)blockIdx.x * kBlock + threadIdx.x; e < E;
        e += (long long)K.nblk * kBlock) {
-    const vec_t t = vabs(bv[gd.off_m + e]);
+    const vr_t t = vmod(bv[gd.off_m + e]);
     const int i = (int)(e / W);
     MinMax c{t, t, i, i};
     mm_merge(r, c);
@@ -783,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_absminmax(KA K, const vec_t *bv,
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_scale_m(KA K, vec_t *x) {
   MLOOP_BEGIN(W)
-  x[o] = (vec_t)S.scl * x[o];
+  x[o] = (vr_t)S.scl * x[o];
   LOOP_END
 }
 
@@ -890,10 +1008,10 @@ __device__ __forceinline__ void put_flags(const FA &F, int inst, uint32_t f) {
 
 // f = lam1 + beta/2*lam2 + mu/2*lam3 - lam4 - lam5 (2d:422, 523, 842)
 template <class TS>
-__device__ __forceinline__ vec_t f_full(const Scal &s) {
+__device__ __forceinline__ vr_t f_full(const Scal &s) {
   const TS lam1 = (TS)s.lam1, lam2 = (TS)s.lam2, lam3 = (TS)s.lam3;
   const TS beta = (TS)s.beta, mu = (TS)s.mu;
-  return (vec_t)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)s.lam4) -
+  return (vr_t)((((lam1 + beta / 2 * lam2) + mu / 2 * lam3) - (TS)s.lam4) -
                  (TS)s.lam5);
 }
 
@@ -924,12 +1042,12 @@ template <class TS>
 __device__ __forceinline__ void set_lam24(Scal &s, bool z3, double x2,
                                           double y2, double z2, double x4,
                                           double y4, double z4) {
-  TS lam2 = (TS)(vec_t)x2;
-  lam2 += (TS)(vec_t)y2;
-  if (z3) lam2 += (TS)(vec_t)z2;
-  vec_t lam4 = (vec_t)x4;
-  lam4 += (vec_t)y4;
-  if (z3) lam4 += (vec_t)z4;
+  TS lam2 = (TS)(vr_t)x2;
+  lam2 += (TS)(vr_t)y2;
+  if (z3) lam2 += (TS)(vr_t)z2;
+  vr_t lam4 = (vr_t)x4;
+  lam4 += (vr_t)y4;
+  if (z3) lam4 += (vr_t)z4;
   s.lam2 = (double)lam2;
   s.lam4 = lam4;
 }
@@ -960,23 +1078,40 @@ __global__ __launch_bounds__(kFin) void fin_tau(FA F) {
     const int sl[5] = {0, 1, 2, 3, kSlotM};
     fin_sums<5>(F, inst, sl, r);
   } else {
-    // <s, s> and <s, y>, formed by the previous update_W pass
-    const int sl[2] = {8, 9};
-    double r2[2];
-    fin_sums<2>(F, inst, sl, r2);
-    r[0] = r2[0];
-    r[1] = r2[1];
+    // <s, s> and <s, y> (complex: also Im <s, y>), formed by the previous
+    // update_W pass
+    if constexpr (kCplx) {
+      const int sl[3] = {8, 9, kSlotSyIm};
+      double r2[3];
+      fin_sums<3>(F, inst, sl, r2);
+      r[0] = r2[0];
+      r[1] = r2[1];
+      r[2] = r2[2];
+    } else {
+      const int sl[2] = {8, 9};
+      double r2[2];
+      fin_sums<2>(F, inst, sl, r2);
+      r[0] = r2[0];
+      r[1] = r2[1];
+    }
   }
   if (threadIdx.x == 0) {
     TS tau;
     if (first) {
-      vec_t dDd = (vec_t)r[0] + (vec_t)r[1];
-      if (gd.r > 1) dDd = dDd + (vec_t)r[2];
-      const vec_t dd = (vec_t)r[3], Add = (vec_t)r[4];
+      vr_t dDd = (vr_t)r[0] + (vr_t)r[1];
+      if (gd.r > 1) dDd = dDd + (vr_t)r[2];
+      const vr_t dd = (vr_t)r[3], Add = (vr_t)r[4];
       tau = fabs((TS)dd / ((TS)dDd + (TS)s.muDbeta * (TS)Add));
     } else {
-      const vec_t ss = (vec_t)r[0], sy = (vec_t)r[1];
-      tau = (TS)vabs(ss / sy);
+      const vr_t ss = (vr_t)r[0], sy = (vr_t)r[1];
+      if constexpr (kCplx) {
+        // abs(ss / sy) of complex values (ss real-valued): |ss| / |sy| in
+        // FP64, rounded to the component type
+        const double den = cmod((double)sy, (double)(vr_t)r[2]);
+        tau = (TS)(vr_t)(fabs((double)ss) / den);
+      } else {
+        tau = (TS)vabs(ss / sy);
+      }
     }
     s.tau = (double)tau;
     if (!fb) {
@@ -998,7 +1133,7 @@ __global__ __launch_bounds__(kFin) void fin_skip(FA F, int force) {
   double r[2];
   fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    const TS nrm_d = (TS)(vec_t)sqrt(r[0]), nrm_a = (TS)(vec_t)sqrt(r[1]);
+    const TS nrm_d = (TS)(vr_t)sqrt(r[0]), nrm_a = (TS)(vr_t)sqrt(r[1]);
     const TS relChange = nrm_d / nrm_a;
     uint32_t fl = s.flags & ~(F_BWD | F_SKIPG);
     TS csv = (TS)s.currSkipValue;
@@ -1046,11 +1181,11 @@ __global__ __launch_bounds__(kFin) void fin_descend(FA F, int mode) {
   }
   if (threadIdx.x == 0) {
     set_lam24<TS>(s, z3, r[0], r[1], r[2], r[3], r[4], r[5]);
-    s.lam3 = (double)(TS)(vec_t)r[12];
-    s.lam5 = (vec_t)r[13];
+    s.lam3 = (double)(TS)(vr_t)r[12];
+    s.lam5 = (vr_t)r[13];
     s.f = f_full<TS>(s);
     if (mode == 0) {
-      s.c_armij = (double)((TS)(vec_t)r[6] * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
+      s.c_armij = (double)((TS)(vr_t)r[6] * (TS)s.tau * (TS)F.O.c * (TS)s.beta);
       s.alpha = 1.0;
       if (armijo_fails<TS>(s)) {
         s.cnt = 0;
@@ -1104,8 +1239,8 @@ __global__ __launch_bounds__(kFin) void fin_lams(FA F, int mode) {
   fin_sums<8>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     set_lam24<TS>(s, z3, r[0], r[1], r[2], r[3], r[4], r[5]);
-    s.lam3 = (double)(TS)(vec_t)r[6];
-    s.lam5 = (vec_t)r[7];
+    s.lam3 = (double)(TS)(vr_t)r[6];
+    s.lam5 = (vr_t)r[7];
     s.f = f_full<TS>(s);
     if (mode == 1) {
       s.flags &= ~F_BT;
@@ -1140,18 +1275,18 @@ __global__ __launch_bounds__(kFin) void fin_wstep(FA F, int init) {
       s.lam1 = (double)(TS)r[0];
       set_lam24<TS>(s, z3, r[1], r[2], r[3], r[4], r[5], r[6]);
     } else {
-      s.f = (vec_t)((TS)s.f - (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
+      s.f = (vr_t)((TS)s.f - (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
       s.lam1 = (double)(TS)r[0];
       set_lam24<TS>(s, z3, r[1], r[2], r[3], r[4], r[5], r[6]);
-      s.f = (vec_t)((TS)s.f + (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
+      s.f = (vr_t)((TS)s.f + (((TS)s.lam1 + beta / 2 * (TS)s.lam2) - (TS)s.lam4));
       const TS gam = (TS)s.gam;
       const TS Qp = (TS)s.Q;
       const TS Q = gam * Qp + 1;
       s.Q = (double)Q;
-      s.C = (vec_t)((gam * Qp * (TS)s.C + (TS)s.f) / Q);
+      s.C = (vr_t)((gam * Qp * (TS)s.C + (TS)s.f) / Q);
       s.count_total++;
-      const TS nrmup = (TS)(vec_t)sqrt(r[7]);
-      const TS relchg = (TS)(vec_t)sqrt(r[8]) / nrmup;
+      const TS nrmup = (TS)(vr_t)sqrt(r[7]);
+      const TS relchg = (TS)(vr_t)sqrt(r[8]) / nrmup;
       s.nrmup = (double)nrmup;
       s.RelChg = (double)relchg;
       if (F.trace && s.count_total <= F.trace_cap) {
@@ -1181,8 +1316,8 @@ __global__ __launch_bounds__(kFin) void fin_init2(FA F) {
   double r[2];
   fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0) {
-    s.lam3 = (double)(TS)(vec_t)r[0];
-    s.lam5 = (vec_t)r[1];
+    s.lam3 = (double)(TS)(vr_t)r[0];
+    s.lam5 = (vr_t)r[1];
     s.f = f_full<TS>(s);
     s.Q = 1.0;
     s.C = s.f;
@@ -1219,7 +1354,7 @@ __global__ __launch_bounds__(kBlock) void fin_scaleb(FA F, const MinMax *mmp,
       const int imx = (mm.imx >= 0 && mm.imx < m) ? mm.imx : 0;
       const vec_t bmin = bv[gd.off_m + (long long)imn * F.W + lbx];
       const vec_t bmax = bv[gd.off_m + (long long)imx * F.W + lbx];
-      const TS b_dif = vabs(bmax - bmin);
+      const TS b_dif = vmod(bmax - bmin);
       const TS th1 = (TS)0.5, th2 = (TS)1.5;
       if (b_dif < th1)
         scl = th1 / b_dif;
@@ -1248,7 +1383,7 @@ __global__ __launch_bounds__(kFin) void fin_outer1(FA F) {
   fin_sums<1>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     s.count_outer++;
-    const TS rr = (TS)(vec_t)sqrt(r[0]) / (TS)s.nrmup;
+    const TS rr = (TS)(vr_t)sqrt(r[0]) / (TS)s.nrmup;
     s.RelChgOut = (double)rr;
     uint32_t fl = s.flags & ~F_INNER_DONE;
     if (rr < (TS)F.O.tol)
@@ -1270,11 +1405,11 @@ __global__ __launch_bounds__(kFin) void fin_mlp(FA F) {
   fin_sums<4>(F, inst, sl, r);
   if (threadIdx.x == 0) {
     s.f = s.f + (s.lam4 + s.lam5);
-    vec_t lam4 = (vec_t)r[0];
-    lam4 += (vec_t)r[1];
-    if (z3) lam4 += (vec_t)r[2];
+    vr_t lam4 = (vr_t)r[0];
+    lam4 += (vr_t)r[1];
+    if (z3) lam4 += (vr_t)r[2];
     s.lam4 = lam4;
-    s.lam5 = (vec_t)r[3];
+    s.lam5 = (vr_t)r[3];
     s.f = s.f - (s.lam4 + s.lam5);
     const TS beta0 = (TS)s.beta;
     TS beta = beta0 * (TS)F.O.rate_cnt;
@@ -1286,7 +1421,7 @@ __global__ __launch_bounds__(kFin) void fin_mlp(FA F) {
     s.mu = (double)mu;
     s.muDbeta = (double)(mu / beta);
     s.f = f_full<TS>(s);
-    s.ratio = (double)(vec_t)(beta0 / beta);
+    s.ratio = (double)(vr_t)(beta0 / beta);
     s.gam = (double)(TS)F.O.gam;
     s.Q = 1.0;
     s.C = s.f;
@@ -1314,7 +1449,7 @@ __global__ __launch_bounds__(kFin) void fin_relerr(FA F) {
   double r[2];
   fin_sums<2>(F, inst, sl, r);
   if (threadIdx.x == 0)
-    s.rel_error = (double)((TS)(vec_t)sqrt(r[0]) / (TS)(vec_t)sqrt(r[1]));
+    s.rel_error = (double)((TS)(vr_t)sqrt(r[0]) / (TS)(vr_t)sqrt(r[1]));
   FIN_END
 }
 
@@ -1489,6 +1624,7 @@ class Engine {
     O_.skip = (o.skipMulRatio != 0 && bt.kind != kDense) ? 1 : 0;
     O_.ls_exact = o.ls_exact;
     graph_ = o.graph != 0;
+    isreal_ = o.isreal != 0;
     submat_mb_ = o.submat_mb > 0 ? o.submat_mb : 5;
     // blocks per group of the vector passes: at most kNblkMax (512 measured
     // best on the 256 ring at B = 128 and 256: 1024 gave 503 / 559 recons/s,
@@ -1504,7 +1640,7 @@ class Engine {
   // dense W = 1 scratch: split-K partials of the plain A^T product or slice
   // partials of the blocked one (never live at the same time)
   size_t scratch_doubles() const {
-    return (size_t)std::max<long long>((long long)bt_.G * 16 * bt_.max_n, bt_.dblk_doubles);
+    return (size_t)std::max<long long>((long long)bt_.G * (kCplx ? 32 : 16) * bt_.max_n, bt_.dblk_doubles);
   }
 
   int setup() {
@@ -1638,7 +1774,7 @@ class Engine {
   // picks the dense launch shape for this batch (dense, W = 1 only)
   int tune_dense() {
     // the FP64 kernels have one launch shape
-    if (!(W == 1 && bt_.kind == kDense) || kVecF64) return TV3_OK;
+    if (!(W == 1 && bt_.kind == kDense) || kOnlyW1) return TV3_OK;
     DenseTune best{2, 2, 8};
     if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
     long long work = 0;
@@ -1718,6 +1854,7 @@ class Engine {
   int submat_mb_ = 5;  // row blocks of the dense get_g product
   // lane compaction (W = 64, on-the-fly projector)
   bool compact_ok_ = false;
+  int isreal_ = 1;  // complex data: keep the real part of the result
   int cmode_ = 0;  // lane compaction of the next products: 0 off, 1 W = 1, 2 compact W = 64
   int ncg_ = 0, gc64_max_ = 0;  // compact 64-lane groups in use / available
   long long pitch64_ = 0;
@@ -1891,11 +2028,15 @@ class Engine {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.dt_exact);
     };
-    const bool iso = O_.tv_iso != 0;
-    if (bt_.z3)
-      iso ? go(k_wstep<W, TS, INIT, true, true>) : go(k_wstep<W, TS, INIT, true, false>);
-    else
-      iso ? go(k_wstep<W, TS, INIT, false, true>) : go(k_wstep<W, TS, INIT, false, false>);
+    if constexpr (kCplx) {  // anisotropic only (check_opts)
+      bt_.z3 ? go(k_wstep<W, TS, INIT, true, false>) : go(k_wstep<W, TS, INIT, false, false>);
+    } else {
+      const bool iso = O_.tv_iso != 0;
+      if (bt_.z3)
+        iso ? go(k_wstep<W, TS, INIT, true, true>) : go(k_wstep<W, TS, INIT, true, false>);
+      else
+        iso ? go(k_wstep<W, TS, INIT, false, true>) : go(k_wstep<W, TS, INIT, false, false>);
+    }
   }
 
   // tau of the first inner iteration / fallback: D(d), |d|^2, A d
@@ -1977,7 +2118,7 @@ class Engine {
       hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
       hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_MLP));
       hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
-      hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
+      hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_, isreal_);
       hipLaunchKernelGGL(fin_refresh, gi(), dim3(kFin), 0, s_, fa(0));
       if (!uniform && (rc = read_summary(sum))) return rc;
     }
@@ -2045,7 +2186,7 @@ class Engine {
     get_g_products(F_LIVE, 0, 1);
     hipLaunchKernelGGL(fin_init2<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
     hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, 0);
-    hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_);
+    hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_, isreal_);
     rc = hip_check(ctx_, hipGetLastError(), "init kernels");
     if (rc) return rc;
 
@@ -2096,6 +2237,8 @@ int check_opts(tv3_ctx *ctx, const tv3_options *o) {
   if (!o) return set_err(ctx, TV3_ERR_ARG, "Argument error: options missing");
   if (o->maxit < 0 || o->maxcnt < 0)
     return set_err(ctx, TV3_ERR_ARG, "Argument error: maxit and maxcnt must be >= 0");
+  if (kCplx && o->tv_iso)
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: isotropic TV takes real data");
   return TV3_OK;
 }
 
@@ -2130,10 +2273,7 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
       layers[i] = (int)(nu / na);
       bt.layered = true;
     }
-    if (!(kVecF64 ? plan_takes_f64(A[i]->dev) : plan_takes_f32(A[i]->dev)))
-      return set_err(ctx, TV3_ERR_ARG,
-                     kVecF64 ? "Argument error: tv3_solve_f64 needs an f64 dense, double sparse or geometry plan"
-                             : "Argument error: an f64 / double-valued plan needs tv3_solve_f64");
+    if (!plan_ok(A[i]->dev)) return set_err(ctx, TV3_ERR_ARG, kPlanErr);
     if (A[i]->dev.kind != A[0]->dev.kind ||
         (A[i]->dev.kind == kDense && A[i]->dev.dtype != A[0]->dev.dtype))
       return set_err(ctx, TV3_ERR_ARG, "Argument error: one batch needs one operator kind and element type");
@@ -2152,8 +2292,8 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
   // groups dealt one per XCD (measured slower on the 256x256 ring).
   // Multi-layer instances run in W = 1 groups.
   // FP64 vectors: W = 1 groups only.
-  bt.W = (shared && B > 1 && !bt.layered && !kVecF64) ? 64 : 1;
-  if (const char *ev = kVecF64 ? nullptr : getenv("TV3_GROUP_WIDTH")) {
+  bt.W = (shared && B > 1 && !bt.layered && !kOnlyW1) ? 64 : 1;
+  if (const char *ev = kOnlyW1 ? nullptr : getenv("TV3_GROUP_WIDTH")) {
     const int w = atoi(ev);
     if (shared && B > 1 && !bt.layered && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense))
       bt.W = w;
@@ -2191,9 +2331,11 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
   }
   bt.Nn = std::max<long long>(on, 1);
   bt.Nm = std::max<long long>(om, 1);
-#ifdef TV3_SOLVER_F64
+#if TV3_SOLVER_VARIANT != 0
+  // T_scal = the component type (the reference's double and complex
+  // overloads); scal_double has no effect
   (void)W;
-  Engine<1, double> e(ctx, bt, *o);
+  Engine<1, vr_t> e(ctx, bt, *o);
   return e.run(b, U, Ut, infos, trace, trace_cap);
 #else
   if (W == 64) {
@@ -2221,7 +2363,7 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
 #endif
 }
 
-#ifndef TV3_SOLVER_F64
+#if TV3_SOLVER_VARIANT == 0
 // ---------------------------------------------------------------------------
 // standalone operators (single volume, W = 1)
 // ---------------------------------------------------------------------------
@@ -2276,7 +2418,7 @@ __global__ __launch_bounds__(kBlock) void k_dot_only(KA K, const vec_t *x,
   {
     constexpr int W = 1;
     NLOOP_BEGIN(1)
-    acc[0] += (double)x[o] * (double)y[o];
+    acc[0] += dre(x[o], y[o]);
     (void)W;
     LOOP_END
   }
@@ -2339,44 +2481,63 @@ GroupDesc vol_desc(int q, int p, int r) {
   gd.n = q * p * r;
   return gd;
 }
-#endif  // !TV3_SOLVER_F64
+#endif  // TV3_SOLVER_VARIANT == 0
 
 }  // namespace
 
 // ===========================================================================
 // C-ABI
 // ===========================================================================
-#ifdef TV3_SOLVER_F64
+#if TV3_SOLVER_VARIANT != 0
+#if TV3_SOLVER_VARIANT == 1
+#define TV3_ENTRY(name) name##_f64
+#elif TV3_SOLVER_VARIANT == 2
+#define TV3_ENTRY(name) name##_c64
+#else
+#define TV3_ENTRY(name) name##_c128
+#endif
+namespace {
+// caller vectors: vr_t components, vec_t elements (complex: re, im pairs)
+bool vec_aligned(const void *p) { return p == nullptr || (uintptr_t)p % alignof(vec_t) == 0; }
+}  // namespace
+
 extern "C" {
 
-int tv3_solve_f64(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts,
-                  int dim_x, int dim_y, int dim_z, const double *b, double *U,
-                  const double *Ut, tv3_info *info, tv3_trace_rec *trace,
-                  int trace_cap) {
+int TV3_ENTRY(tv3_solve)(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opts, int dim_x,
+                         int dim_y, int dim_z, const vr_t *b, vr_t *U, const vr_t *Ut,
+                         tv3_info *info, tv3_trace_rec *trace, int trace_cap) {
   if (!ctx) return TV3_ERR_ARG;
   if (!A || !b || !U || !info) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  if (!vec_aligned(b) || !vec_aligned(U) || !vec_aligned(Ut))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: vectors must be aligned to their element size");
   const int dims[3] = {dim_x, dim_y, dim_z};
   const tv3_plan *As[1] = {A};
-  const double *bs[1] = {b};
-  double *Us[1] = {U};
-  const double *Uts[1] = {Ut};
+  const vec_t *bs[1] = {(const vec_t *)b};
+  vec_t *Us[1] = {(vec_t *)U};
+  const vec_t *Uts[1] = {(const vec_t *)Ut};
   return solve_batch_impl(ctx, 1, As, opts, dims, bs, Us, Ut ? Uts : nullptr,
                           info, trace, trace_cap);
 }
 
-int tv3_solve_batch_f64(tv3_ctx *ctx, int B, const tv3_plan *const *A,
-                        const tv3_options *opts, const int *dims,
-                        const double *const *b, double *const *U,
-                        const double *const *Ut, tv3_info *infos) {
-  return solve_batch_impl(ctx, B, A, opts, dims, b, U, Ut, infos, nullptr, 0);
+int TV3_ENTRY(tv3_solve_batch)(tv3_ctx *ctx, int B, const tv3_plan *const *A,
+                               const tv3_options *opts, const int *dims,
+                               const vr_t *const *b, vr_t *const *U,
+                               const vr_t *const *Ut, tv3_info *infos) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (B > 0 && b && U)
+    for (int i = 0; i < B; ++i)
+      if (!vec_aligned(b[i]) || !vec_aligned(U[i]) || (Ut && !vec_aligned(Ut[i])))
+        return set_err(ctx, TV3_ERR_ARG, "Argument error: vectors must be aligned to their element size");
+  return solve_batch_impl(ctx, B, A, opts, dims, (const vec_t *const *)b, (vec_t *const *)U,
+                          (const vec_t *const *)Ut, infos, nullptr, 0);
 }
 
-int tv3_gemv_f64(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
-                 double *y) {
+int TV3_ENTRY(tv3_gemv)(tv3_ctx *ctx, const tv3_plan *A, int trans, const vr_t *x, vr_t *y) {
   if (!ctx) return TV3_ERR_ARG;
   if (!A || !x || !y) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
-  if (!plan_takes_f64(A->dev))
-    return set_err(ctx, TV3_ERR_ARG, "Argument error: tv3_gemv_f64 needs an f64 dense, double sparse or geometry plan");
+  if (!vec_aligned(x) || !vec_aligned(y))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: vectors must be aligned to their element size");
+  if (!plan_ok(A->dev)) return set_err(ctx, TV3_ERR_ARG, kGemvErr);
   int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if (rc) return rc;
   GroupDesc gd{};
@@ -2385,8 +2546,9 @@ int tv3_gemv_f64(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
   gd.m = A->dev.m;
   gd.layers = 1;
   gd.plan = A->dev;
-  // one group descriptor and, for dense A, the chunk sums of A^T x
-  const long long dense = A->dev.kind == kDense ? 16LL * A->dev.n : 1;
+  // one group descriptor and, for dense A, the chunk sums of A^T x (two per
+  // complex element)
+  const long long dense = A->dev.kind == kDense ? (kCplx ? 32LL : 16LL) * A->dev.n : 1;
   const size_t need = align_up(sizeof(GroupDesc)) + align_up(sizeof(double) * dense);
   if (ctx->ws_bytes < need) {
     if (ctx->ws) { (void)hipStreamSynchronize(ctx->stream); (void)hipFree(ctx->ws); ctx->ws = nullptr; ctx->ws_bytes = 0; }
@@ -2394,15 +2556,17 @@ int tv3_gemv_f64(tv3_ctx *ctx, const tv3_plan *A, int trans, const double *x,
     ctx->ws_bytes = need;
   }
   GroupDesc *G = (GroupDesc *)ctx->ws;
-  double *scratch = (double *)((char *)ctx->ws + align_up(sizeof(GroupDesc)));
+  vec_t *scratch = (vec_t *)((char *)ctx->ws + align_up(sizeof(GroupDesc)));
   rc = hip_check(ctx, hipMemcpyAsync(G, &gd, sizeof(GroupDesc), hipMemcpyHostToDevice, ctx->stream), "copy");
   // gd is a stack variable: wait for the copy
   if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "copy sync");
   if (rc) return rc;
+  const vec_t *xv = (const vec_t *)x;
+  vec_t *yv = (vec_t *)y;
   if (trans)
-    launch_bwd<1>(ctx->stream, G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, nullptr, 0, nullptr, scratch, 0);
+    launch_bwd<1>(ctx->stream, G, 1, gd.n, A->dev.kind, A->dev.dtype, xv, yv, nullptr, 0, nullptr, scratch, 0);
   else
-    launch_fwd<1>(ctx->stream, G, 1, gd.m, A->dev.kind, A->dev.dtype, x, y, nullptr, 0);
+    launch_fwd<1>(ctx->stream, G, 1, gd.m, A->dev.kind, A->dev.dtype, xv, yv, nullptr, 0);
   return hip_check(ctx, hipGetLastError(), "gemv launch");
 }
 
@@ -2437,7 +2601,7 @@ int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
   if (!ctx) return TV3_ERR_ARG;
   if (!A || !x || !y) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
   if (!plan_takes_f32(A->dev))
-    return set_err(ctx, TV3_ERR_ARG, "Argument error: an f64 / double-valued plan needs tv3_gemv_f64");
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: an f64 / double-valued or complex plan needs its own entry point (tv3_gemv_f64, tv3_gemv_c64, tv3_gemv_c128)");
   int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if (rc) return rc;
   GroupDesc gd = vol_desc(A->dev.n, 1, 1);
@@ -2539,4 +2703,4 @@ int tv3_nrm2(tv3_ctx *ctx, long long n, const float *x, double *out) {
 }
 
 }  // extern "C"
-#endif  // TV3_SOLVER_F64
+#endif  // TV3_SOLVER_VARIANT

@@ -5,5 +5,5 @@
 // double; this unit exports tv3_solve_f64, tv3_solve_batch_f64 and
 // tv3_gemv_f64 and runs W = 1 groups only.
 #define TV3_VEC double
-#define TV3_SOLVER_F64 1
+#define TV3_SOLVER_VARIANT 1
 #include "tv3_solver.hip"

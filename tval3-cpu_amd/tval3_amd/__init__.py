@@ -81,6 +81,14 @@ def lib():
         "tv3_gemv_f64": (_I, [_VP, _VP, _I, _VP, _VP]),
         "tv3_solve_f64": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
         "tv3_solve_batch_f64": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+        "tv3_plan_sparse_c64": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
+        "tv3_plan_sparse_c128": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
+        "tv3_gemv_c64": (_I, [_VP, _VP, _I, _VP, _VP]),
+        "tv3_gemv_c128": (_I, [_VP, _VP, _I, _VP, _VP]),
+        "tv3_solve_c64": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
+        "tv3_solve_c128": (_I, [_VP, _VP, _VP, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
+        "tv3_solve_batch_c64": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+        "tv3_solve_batch_c128": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
         "tv3_mem_alloc": (_I, [_VP, C.c_size_t, C.POINTER(_VP)]),
         "tv3_mem_free": (_I, [_VP, _VP]),
         "tv3_mem_upload": (_I, [_VP, _VP, _VP, C.c_size_t]),
@@ -107,6 +115,19 @@ def lib():
 
 def _ptr(t):
     return None if t is None else _VP(t.data_ptr())
+
+
+def _vkind(x, f64):
+    """Vector kind of an entry point: 'f32' / 'f64' for real data, 'c64' /
+    'c128' for complex data (complex128 data or f64=True: 'c128')."""
+    cplx = x is not None and (np.iscomplexobj(x) if not hasattr(x, "is_complex") else x.is_complex())
+    wide = f64 or (x is not None and str(getattr(x, "dtype", "")).endswith("complex128"))
+    if cplx:
+        return "c128" if wide else "c64"
+    return "f64" if f64 else "f32"
+
+
+_SUFFIX = {"f32": "", "f64": "_f64", "c64": "_c64", "c128": "_c128"}
 
 
 class Context:
@@ -194,28 +215,36 @@ class Context:
     # ---- plans -----------------------------------------------------------
     def plan_dense(self, A, bf16=False, f64=False):
         """Dense A on the device: float32 (default), bfloat16 (bf16=True) or,
-        for the double-vector entry points, float64 (f64=True, mat<double>)."""
+        for the double-vector entry points, float64 (f64=True, mat<double>).
+        Complex A gives a complex plan (complex64; complex128 when A is
+        complex128 or f64=True) for the complex entry points."""
         t = self.torch
         if bf16 and f64:
             raise Tval3Error(1, "Argument error: bf16 and f64 exclude each other")
-        At = self.tensor(A, t.bfloat16 if bf16 else t.float64 if f64 else t.float32)
+        k = _vkind(A, f64)
+        if k in ("c64", "c128") and bf16:
+            raise Tval3Error(1, "Argument error: bf16 takes real A")
+        dt = {"c64": (t.complex64, 3), "c128": (t.complex128, 4), "f64": (t.float64, 2),
+              "f32": (t.bfloat16, 1) if bf16 else (t.float32, 0)}[k]
+        At = self.tensor(A, dt[0])
         m, n = At.shape
         h = _VP()
-        self._call(lib().tv3_plan_dense, self.h, m, n, 1 if bf16 else 2 if f64 else 0,
-                                         _ptr(At), C.byref(h))
+        self._call(lib().tv3_plan_dense, self.h, m, n, dt[1], _ptr(At), C.byref(h))
         return Plan(self, h, [At])
 
     def plan_sparse(self, m, n, csr_ptr, csr_ind, csr_val, csc_ptr, csc_ind,
                     csc_val, f64=False):
         """CSR + CSC copies of one m x n matrix; f64=True: double values
-        (sparse_mat<double>, for the double-vector entry points)."""
+        (sparse_mat<double>, for the double-vector entry points); complex
+        values: a complex plan (complex128 values or f64=True: complex128)."""
         t = self.torch
-        vt = t.float64 if f64 else t.float32
+        k = _vkind(csr_val, f64)
+        vt = {"f32": t.float32, "f64": t.float64, "c64": t.complex64, "c128": t.complex128}[k]
         arrs = [self.tensor(csr_ptr, t.int32), self.tensor(csr_ind, t.int32),
                 self.tensor(csr_val, vt), self.tensor(csc_ptr, t.int32),
                 self.tensor(csc_ind, t.int32), self.tensor(csc_val, vt)]
         h = _VP()
-        fn = lib().tv3_plan_sparse_f64 if f64 else lib().tv3_plan_sparse
+        fn = getattr(lib(), "tv3_plan_sparse" + _SUFFIX[k])
         self._call(fn, self.h, m, n, int(arrs[1].numel()), *[_ptr(a) for a in arrs], C.byref(h))
         return Plan(self, h, arrs)
 
@@ -227,13 +256,14 @@ class Context:
 
     # ---- operators ------------------------------------------------------
     def gemv(self, plan, x, trans=False, f64=False):
-        """y = A x (A^T x with trans); f64=True: double vectors
-        (tv3_gemv_f64)."""
+        """y = A x (A^T x, complex: A^H x, with trans); f64=True: double
+        vectors (tv3_gemv_f64); complex x: tv3_gemv_c64 / _c128."""
         t = self.torch
-        vt = t.float64 if f64 else t.float32
+        k = _vkind(x, f64)
+        vt = {"f32": t.float32, "f64": t.float64, "c64": t.complex64, "c128": t.complex128}[k]
         x = self.tensor(x, vt)
         y = t.empty(plan.n if trans else plan.m, dtype=vt, device=self.device)
-        fn = lib().tv3_gemv_f64 if f64 else lib().tv3_gemv
+        fn = getattr(lib(), "tv3_gemv" + _SUFFIX[k])
         self._call(fn, self.h, plan.h, int(bool(trans)), _ptr(x), _ptr(y))
         return y
 
@@ -280,12 +310,15 @@ class Context:
     # ---- solver -----------------------------------------------------------
     def solve(self, plan, b, dims, opts=None, Ut=None, trace_cap=0, f64=False):
         """tval3_cpu_2d/3d on the device: returns (U tensor, Info, [TraceRec]).
-        f64=True: the double/double solve (tv3_solve_f64, float64 tensors)."""
+        f64=True: the double/double solve (tv3_solve_f64, float64 tensors);
+        complex b: the complex solve (tv3_solve_c64, or _c128 for complex128
+        b or f64=True)."""
         t = self.torch
         if opts is None:
             opts = default_options()
         q, p, r = dims
-        vt = t.float64 if f64 else t.float32
+        k = _vkind(b, f64)
+        vt = {"f32": t.float32, "f64": t.float64, "c64": t.complex64, "c128": t.complex128}[k]
         b = self.tensor(b, vt)
         if b.numel() != plan.m * _layers(plan, dims):
             raise Tval3Error(1, "Argument error: b.len must be equal to A.dim_y * layers!")
@@ -295,7 +328,7 @@ class Context:
             raise Tval3Error(1, "Argument error: U and Ut must have the same size!")
         info = Info()
         trace = (TraceRec * trace_cap)() if trace_cap else None
-        fn = lib().tv3_solve_f64 if f64 else lib().tv3_solve
+        fn = getattr(lib(), "tv3_solve" + _SUFFIX[k])
         self._call(fn, self.h, plan.h, C.byref(opts), q, p, r,
                                     _ptr(b), _ptr(U), _ptr(Utt), C.byref(info),
                                     trace, int(trace_cap))
@@ -334,9 +367,11 @@ class Context:
 
     def solve_batch(self, plans, bs, dims, opts=None, Uts=None, f64=False):
         """B reconstructions in one batched launch sequence.  Returns
-        (list of U tensors, list of Info).  f64=True: tv3_solve_batch_f64."""
+        (list of U tensors, list of Info).  f64=True: tv3_solve_batch_f64;
+        complex b: tv3_solve_batch_c64 / _c128."""
         t = self.torch
-        vt = t.float64 if f64 else t.float32
+        k = _vkind(bs[0] if len(bs) else None, f64)
+        vt = {"f32": t.float32, "f64": t.float64, "c64": t.complex64, "c128": t.complex128}[k]
         if opts is None:
             opts = default_options()
         B = len(plans)
@@ -361,7 +396,7 @@ class Context:
         utarr = ((_VP * B)(*[(x.data_ptr() if x is not None else None) for x in Utts])
                  if Utts is not None else None)
         infos = (Info * B)()
-        fn = lib().tv3_solve_batch_f64 if f64 else lib().tv3_solve_batch
+        fn = getattr(lib(), "tv3_solve_batch" + _SUFFIX[k])
         self._call(fn, self.h, B, parr, C.byref(opts), dims_arr, barr, uarr, utarr, infos)
         return Us, [infos[i] for i in range(B)]
 
