@@ -60,6 +60,13 @@
  *                    are Barzilai-Borwein, accepted steps satisfy the
  *                    nonmonotone Armijo rule and rejected ones do not, f
  *                    equals the Lagrangian, multipliers follow the ALM update
+ *   complex instantiations (Solver<std::complex<R>, R>)
+ *                    products vs numpy (A^H by the adjoint identity); the
+ *                    complex shrinkage closed form (both 3-D Wz forms); an
+ *                    FP64 model of the complex augmented Lagrangian (first
+ *                    step, BB step); exact covariance under b -> i b; phase
+ *                    covariance and the real solve on real data
+ *                    (tests/test_oracle_complex.py)
  *   ls_exact = 0     same steps as the round-by-round search (f(alpha) is
  *                    an exact quadratic)
  *   layers > 1       bit-identical to the single-layer solve with the
