@@ -193,6 +193,27 @@ int tv3_mem_download(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tv3_ctx_profile(tv3_ctx *ctx, int enable);
 int tv3_ctx_profile_read(tv3_ctx *ctx, double *ms, long long *count, double *bytes);
 
+/* Per-slot timing report, the GPU counterpart of the reference's PROFILING
+ * table (profile_info[slot]: name, time, runs; printed at 3d/tval3_cpu_3d.cpp:
+ * 1324-1351).  With tv3_ctx_profile(ctx, 2) every launch of the solver is
+ * timed by an event pair on the context stream (this slows a solve; level 1
+ * times the two product slots only).  A slot number is the reference's
+ * profile_info index where one pass implements that function (1 scaleb,
+ * 3 shrinkage, 10 / 11 gemv NoTrans / ConjTrans), and 30 and up for fused
+ * passes with no single counterpart (vec_add, nonneg, D and Dt, slots 0 and
+ * 2 of the reference, run inside them; the names list what each contains).
+ * Launches inside a captured graph (tv3_options.graph = 1) are not timed.
+ * Sets *n to the number of slots that ran and fills out[0 .. min(cap, *n))
+ * with them, in slot order (cap = 0 and out = NULL ask for the count);
+ * synchronizes the stream. */
+typedef struct tv3_profile_slot {
+  int slot;
+  long long runs;     /* launches */
+  double ms;          /* summed milliseconds */
+  char name[96];
+} tv3_profile_slot;
+int tv3_ctx_profile_slots(tv3_ctx *ctx, tv3_profile_slot *out, int cap, int *n);
+
 /* ---- measurement-operator plans ------------------------------------------ */
 /* Dense A, row-major m x n, on the device (f32 or bf16).  Borrowed. */
 int tv3_plan_dense(tv3_ctx *ctx, int m, int n, int dtype, const void *A,

@@ -38,7 +38,8 @@ def test_python_binding_loads():
 
 @pytest.mark.parametrize("name,pycls", [("tv3_options", abi.Options), ("tv3_info", abi.Info),
                                         ("tv3_trace_rec", abi.TraceRec),
-                                        ("tv3_geometry", abi.Geometry)])
+                                        ("tv3_geometry", abi.Geometry),
+                                        ("tv3_profile_slot", abi.ProfileSlot)])
 def test_struct_layout(tmp_path, name, pycls):
     fields = [f for f, _ in pycls._fields_]
     src = tmp_path / "probe.c"

@@ -1,0 +1,99 @@
+"""The launch-timing report (tv3_ctx_profile levels 1 and 2 and
+tv3_ctx_profile_slots), the counterpart of the reference's PROFILING table
+(profile_info[slot], 3d/tval3_cpu_3d.cpp:1337-1348; 3d/mat_mult.h:19-44).
+
+Timing must not change a result, the product slots must agree with
+tv3_ctx_profile_read, and the slots carry the reference's numbers for the
+passes that implement one reference function."""
+import numpy as np
+import pytest
+
+import tval3_amd
+from tval3_amd import synth
+import tv3_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _ring(grid=48, seed=3):
+    geo = synth.ring_geometry(grid, grid)
+    op = O.Op.geometry(geo)
+    b = synth.add_noise(op.gemv(synth.disk_phantom(grid, seed=seed)), 0.005, seed)
+    return geo, b
+
+
+@pytest.mark.parametrize("ls", [1, 0])
+def test_profile_slots_every_launch(ctx, ls):
+    geo, b = _ring()
+    plan = ctx.plan_geometry(geo)
+    dims = (48, 48, 1)
+    opts = tval3_amd.default_options(maxit=40, ls_exact=ls)
+    U0, info0, _ = ctx.solve(plan, b, dims, opts)
+    ref = U0.cpu().numpy()
+
+    ctx.profile(2)
+    U, info, _ = ctx.solve(plan, b, dims, opts)
+    slots = ctx.profile_slots()
+    read = ctx.profile_read()
+    report = ctx.profile_report()
+    ctx.profile(False)
+    print(report)
+
+    assert np.array_equal(U.cpu().numpy(), ref)
+    assert info.total_iters == info0.total_iters
+    d = {s: (name, runs, ms) for s, name, runs, ms in slots}
+    assert [s for s, *_ in slots] == sorted(d)
+    # scaleb, shrinkage, both products, descend, the setup / finish passes
+    assert {1, 3, 10, 11, 30, 37} <= set(d), sorted(d)
+    assert all(0 <= s < 40 and runs > 0 and ms > 0 and name for s, (name, runs, ms) in d.items())
+    assert d[10][0].startswith("gemv CblasNoTrans") and d[11][0].startswith("gemv CblasConjTrans")
+    # the product slots are the figures tv3_ctx_profile_read gives
+    assert d[10][1] == read["A"][1] and d[11][1] == read["At"][1]
+    assert d[10][2] == pytest.approx(read["A"][0], rel=1e-12)
+    assert d[11][2] == pytest.approx(read["At"][0], rel=1e-12)
+    # one shrinkage pass per inner iteration at least
+    assert d[3][1] >= info.total_iters
+    lines = report.splitlines()
+    assert len(lines) == len(slots)
+    assert lines[0].startswith(d[min(d)][0] + ":\t ") and "(runs: " in lines[0]
+
+
+def test_profile_levels(ctx):
+    geo, b = _ring(32, 4)
+    plan = ctx.plan_geometry(geo)
+    opts = tval3_amd.default_options(maxit=20, ls_exact=1)
+    ctx.profile(1)
+    ctx.solve(plan, b, (32, 32, 1), opts)
+    s1 = ctx.profile_slots()
+    ctx.profile(0)
+    ctx.solve(plan, b, (32, 32, 1), opts)
+    s0 = ctx.profile_slots()
+    assert [s for s, *_ in s1] == [10, 11]
+    assert s0 == []
+    with pytest.raises(ValueError):
+        ctx.profile(3)
+
+
+def test_profile_batch_long_solve(ctx):
+    """A batched solve long enough to fold the event pool in mid-solve: the
+    per-slot counts match a level-1 run of the same solve."""
+    geo, b = _ring(32, 5)
+    plan = ctx.plan_geometry(geo)
+    B = 8
+    bs = [b * (1 + 0.01 * i) for i in range(B)]
+    # tolerances no iterate meets: every instance runs all 800 iterations (about 8 launches each)
+    opts = tval3_amd.default_options(maxit=800, ls_exact=1, tol=1e-30, tol_inn=1e-30)
+    counts = {}
+    for level in (1, 2):
+        ctx.profile(level)
+        Us, infos = ctx.solve_batch([plan] * B, bs, [(32, 32, 1)] * B, opts)
+        counts[level] = {s: runs for s, _, runs, _ in ctx.profile_slots()}
+        res = np.stack([u.cpu().numpy() for u in Us])
+        ctx.profile(False)
+        if level == 1:
+            ref = res
+        else:
+            assert np.array_equal(res, ref)
+    assert all(i.total_iters == 800 for i in infos)
+    assert counts[2][10] == counts[1][10] and counts[2][11] == counts[1][11]
+    assert sum(counts[2].values()) > 4096, counts[2]

@@ -217,13 +217,13 @@ struct tv3_tape {
   }
 
   void fwdA(const float *x, float *y) {
-    prof_begin(ctx, 0);
+    prof_begin(ctx, kProfGemvN);
     launch_fwd<1>(st, G, 1, m, A->dev.kind, A->dev.dtype, x, y, S, 0);
     prof_end(ctx);
     prof_count(ctx, G, 1, 1, S, 0, 0);
   }
   void bwdA(const float *x, float *y) {
-    prof_begin(ctx, 1);
+    prof_begin(ctx, kProfGemvT);
     launch_bwd<1>(st, G, 1, n, A->dev.kind, A->dev.dtype, x, y, S, 0, nullptr, (float *)dense, 0);
     prof_end(ctx);
     prof_count(ctx, G, 1, 1, S, 0, 1);

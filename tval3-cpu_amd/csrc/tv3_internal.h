@@ -246,12 +246,15 @@ struct tv3_ctx {
   size_t hflags_n = 0;
   // timing events for the benchmark (kernel-level, on this stream)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // optional per-launch timing of the A / A^T products (tv3_ctx_profile)
+  // optional per-launch timing (tv3_ctx_profile): level 1 times the A / A^T
+  // products, level 2 every engine launch, into the slots of tv3::kProf*
   bool prof = false;
+  int prof_level = 0;
+  bool prof_open = false;      // a begin event of the current launch is recorded
   std::vector<hipEvent_t> evpool;
-  std::vector<int> evkind;     // kind of each recorded pair: 0 = A, 1 = A^T
-  double prof_ms[2] = {0, 0};
-  long long prof_n[2] = {0, 0};
+  std::vector<int> evkind;     // slot of each recorded pair
+  double prof_ms[40] = {};     // per slot (tv3::kProfSlots)
+  long long prof_n[40] = {};
   // device: algorithmic bytes of the instances each profiled launch computed
   unsigned long long *prof_bytes = nullptr;
 };
@@ -270,7 +273,18 @@ struct tv3_plan {
 namespace tv3 {
 // error helpers (tv3_plan.hip)
 int set_err(tv3_ctx *ctx, int code, const std::string &msg);
-// event pair around one operator launch when profiling is on
+// Profile slots: the reference's profile_info indices where one pass
+// implements that reference function (3d/tval3_cpu_3d.cpp:255, 314, 837;
+// 3d/mat_mult.h:21-27), 30 and up for fused passes with no single
+// counterpart.
+constexpr int kProfSlots = 40;
+static_assert(sizeof(tv3_ctx::prof_ms) == kProfSlots * sizeof(double), "slot arrays");
+constexpr int kProfScaleb = 1, kProfShrink = 3, kProfGemvN = 10, kProfGemvT = 11;
+constexpr int kProfDescend = 30, kProfLineSearch = 31, kProfTau = 32, kProfGetG = 33,
+              kProfMlp = 34, kProfScalars = 35, kProfLanes = 36, kProfSetup = 37;
+const char *prof_slot_name(int slot);
+// event pair around one launch when profiling is on (level 1: the A / A^T
+// slots only)
 void prof_begin(tv3_ctx *ctx, int kind);
 void prof_end(tv3_ctx *ctx);
 void prof_collect(tv3_ctx *ctx);

@@ -33,8 +33,35 @@ int set_err(tv3_ctx *ctx, int code, const std::string &msg) {
   return code;
 }
 
+const char *prof_slot_name(int slot) {
+  switch (slot) {
+    case kProfScaleb: return "scaleb (k_absminmax, fin_scaleb, k_scale_m)";
+    case kProfShrink: return "shrinkage with update_W, Dt, get_gradient and the BB dots (k_wstep)";
+    case kProfGemvN: return "gemv CblasNoTrans: A x";
+    case kProfGemvT: return "gemv CblasConjTrans: A^T x";
+    case kProfDescend: return "descend with nonneg, D, get_g's Dt, lam2, lam4 (k_descend)";
+    case kProfLineSearch: return "min_u backtracking, update_g (k_interp_*, k_bt_*)";
+    case kProfTau: return "get_tau first step: D(d), |d|^2, |Ad|^2 (k_tau_first, k_msq)";
+    case kProfGetG: return "get_g m-part: Au - b, lam3, lam5, skip norms (k_mpost)";
+    case kProfMlp: return "update_mlp, outer step, get_gradient (k_mlp_*, k_outer1, k_grad)";
+    case kProfScalars: return "per-instance scalar control (fin_*)";
+    case kProfLanes: return "lane compaction of masked products (k_lane_*)";
+    case kProfSetup: return "setup, copies and finish (k_pack, k_init_U, k_D, k_finish, ...)";
+    default: return "";
+  }
+}
+
 void prof_begin(tv3_ctx *ctx, int kind) {
-  if (!ctx->prof) return;
+  ctx->prof_open = false;
+  if (!ctx->prof || kind < 0 || kind >= kProfSlots) return;
+  if (ctx->prof_level < 2 && kind != kProfGemvN && kind != kProfGemvT) return;
+  if (ctx->evkind.size() >= 4096) {
+    // fold the finished pairs in, so that long solves reuse the pool
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusActive;
+    if (hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone &&
+        hipStreamSynchronize(ctx->stream) == hipSuccess)
+      prof_collect(ctx);
+  }
   const size_t i = ctx->evkind.size() * 2;
   while (ctx->evpool.size() < i + 2) {
     hipEvent_t e;
@@ -43,11 +70,13 @@ void prof_begin(tv3_ctx *ctx, int kind) {
   }
   ctx->evkind.push_back(kind);
   (void)hipEventRecord(ctx->evpool[i], ctx->stream);
+  ctx->prof_open = true;
 }
 
 void prof_end(tv3_ctx *ctx) {
-  if (!ctx->prof || ctx->evkind.empty()) return;
+  if (!ctx->prof || !ctx->prof_open || ctx->evkind.empty()) return;
   (void)hipEventRecord(ctx->evpool[ctx->evkind.size() * 2 - 1], ctx->stream);
+  ctx->prof_open = false;
 }
 
 // folds the recorded pairs into the per-kind totals (after a stream sync)
@@ -1865,8 +1894,12 @@ int tv3_ctx_profile(tv3_ctx *c, int enable) {
   (void)hipStreamSynchronize(c->stream);
   c->evkind.clear();
   c->prof = enable != 0;
-  c->prof_ms[0] = c->prof_ms[1] = 0;
-  c->prof_n[0] = c->prof_n[1] = 0;
+  c->prof_level = enable;
+  c->prof_open = false;
+  for (int k = 0; k < kProfSlots; ++k) {
+    c->prof_ms[k] = 0;
+    c->prof_n[k] = 0;
+  }
   if (c->prof) {
     if (!c->prof_bytes) {
       int rc = hip_check(c, hipMalloc(&c->prof_bytes, 2 * sizeof(unsigned long long)), "profile hipMalloc");
@@ -1883,7 +1916,8 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
   int rc = hip_check(c, hipStreamSynchronize(c->stream), "profile sync");
   if (rc) return rc;
   prof_collect(c);
-  for (int k = 0; k < 2; ++k) { ms[k] = c->prof_ms[k]; count[k] = c->prof_n[k]; }
+  const int slot[2] = {kProfGemvN, kProfGemvT};
+  for (int k = 0; k < 2; ++k) { ms[k] = c->prof_ms[slot[k]]; count[k] = c->prof_n[slot[k]]; }
   if (bytes) {
     unsigned long long hb[2] = {0, 0};
     if (c->prof_bytes) {
@@ -1893,6 +1927,27 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
     bytes[0] = (double)hb[0];
     bytes[1] = (double)hb[1];
   }
+  return TV3_OK;
+}
+
+int tv3_ctx_profile_slots(tv3_ctx *c, tv3_profile_slot *out, int cap, int *n) {
+  if (!c || !n || (cap > 0 && !out)) return TV3_ERR_ARG;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "profile sync");
+  if (rc) return rc;
+  prof_collect(c);
+  int k = 0;
+  for (int sl = 0; sl < kProfSlots; ++sl) {
+    if (c->prof_n[sl] == 0) continue;
+    if (k < cap) {
+      tv3_profile_slot &o = out[k];
+      o.slot = sl;
+      o.runs = c->prof_n[sl];
+      o.ms = c->prof_ms[sl];
+      std::snprintf(o.name, sizeof(o.name), "%s", prof_slot_name(sl));
+    }
+    ++k;
+  }
+  *n = k;
   return TV3_OK;
 }
 

@@ -1608,6 +1608,15 @@ bool autotune_enabled() {
   return !(e && e[0] == '0');
 }
 
+// one kernel launch of the engine; with profiling on (and outside graph
+// capture) an event pair around it adds its time to profile slot SLOT
+#define TV3_GO(SLOT, ...)                      \
+  do {                                         \
+    if (!capturing_) prof_begin(ctx_, SLOT);   \
+    hipLaunchKernelGGL(__VA_ARGS__);           \
+    if (!capturing_) prof_end(ctx_);           \
+  } while (0)
+
 template <int W, class TS>
 class Engine {
  public:
@@ -1899,20 +1908,23 @@ class Engine {
       if (cmode_ == 2) {
         const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
         const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
-        hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_n, 0, pitch64_);
-        prof_begin(ctx_, 0);
+        TV3_GO(kProfLanes, k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_n, 0, pitch64_);
+        prof_begin(ctx_, kProfGemvN);
         launch_fwd<64>(s_, G64c_, ncg_, bt_.max_m, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0);
         prof_end(ctx_);
         prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
-        hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const vec_t *)nullptr);
+        TV3_GO(kProfLanes, k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const vec_t *)nullptr);
         return;
       }
       if (cmode_ == 1) {
         const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
         const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
-        hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
+        TV3_GO(kProfLanes, k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
+        prof_begin(ctx_, kProfGemvN);
         launch_fwd<1>(s_, Gc_, lanes_.n, bt_.max_m, bt_.kind, bt_.dtype, xc_, yc_, flags_of(S_), 0);
-        hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const vec_t *)nullptr);
+        prof_end(ctx_);
+        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
+        TV3_GO(kProfLanes, k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const vec_t *)nullptr);
         return;
       }
     }
@@ -1920,7 +1932,7 @@ class Engine {
       launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask);
       return;
     }
-    prof_begin(ctx_, 0);
+    prof_begin(ctx_, kProfGemvN);
     launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask);
     prof_end(ctx_);
     prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
@@ -1933,20 +1945,23 @@ class Engine {
       if (cmode_ == 2) {
         const dim3 gg(std::max(1, std::min(1024, (int)(((long long)bt_.max_m * 64 + kBlock - 1) / kBlock))), ncg_);
         const dim3 gs(std::max(1, std::min(1024, (int)(((long long)bt_.max_n * 64 + kBlock - 1) / kBlock))), ncg_);
-        hipLaunchKernelGGL(k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_m, 1, pitch64_);
-        prof_begin(ctx_, 1);
+        TV3_GO(kProfLanes, k_lane_gather64, gg, dim3(kBlock), 0, s_, G_, x, x64c_, lanes_, bt_.max_m, 1, pitch64_);
+        prof_begin(ctx_, kProfGemvT);
         launch_bwd<64>(s_, G64c_, ncg_, bt_.max_n, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0, nullptr, nullptr, 0);
         prof_end(ctx_);
         prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
-        hipLaunchKernelGGL(k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
+        TV3_GO(kProfLanes, k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
         return;
       }
       if (cmode_ == 1) {
         const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
         const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
-        hipLaunchKernelGGL(k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
+        TV3_GO(kProfLanes, k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
+        prof_begin(ctx_, kProfGemvT);
         launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, flags_of(S_), 0, nullptr, nullptr, 0);
-        hipLaunchKernelGGL(k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
+        prof_end(ctx_);
+        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
+        TV3_GO(kProfLanes, k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
         return;
       }
     }
@@ -1954,7 +1969,7 @@ class Engine {
       launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask, sub, scratch_, smb);
       return;
     }
-    prof_begin(ctx_, 1);
+    prof_begin(ctx_, kProfGemvT);
     launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask, sub, scratch_, smb);
     prof_end(ctx_);
     prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
@@ -1993,7 +2008,7 @@ class Engine {
   void get_g_products(uint32_t req, uint32_t forbid, int force) {
     fwd(a_.U, a_.Au, req);
     {
-      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_); };
+      auto go = [&](auto kern) { TV3_GO(kProfGetG, kern, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_); };
       const bool poly = !force && !O_.ls_exact;
       if (O_.skip)
         poly ? go(k_mpost<W, true, true>) : go(k_mpost<W, true, false>);
@@ -2001,10 +2016,10 @@ class Engine {
         poly ? go(k_mpost<W, false, true>) : go(k_mpost<W, false, false>);
     }
     if (O_.skip) {
-      hipLaunchKernelGGL(fin_skip<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(req, forbid), force);
-      hipLaunchKernelGGL(k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
+      TV3_GO(kProfScalars, fin_skip<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(req, forbid), force);
+      TV3_GO(kProfSetup, k_copy_m<W>, gm(), dim3(kBlock), 0, s_, km(req | F_BWD, forbid), a_.Au, a_.lastAu);
       bwd(a_.Au, a_.g, req | F_BWD, a_.Atb, true);
-      hipLaunchKernelGGL(k_skipg<W>, gn(), dim3(kBlock), 0, s_, kn(req | F_SKIPG, forbid), a_);
+      TV3_GO(kProfSetup, k_skipg<W>, gn(), dim3(kBlock), 0, s_, kn(req | F_SKIPG, forbid), a_);
     } else {
       bwd(a_.Au, a_.g, req, a_.Atb, true);
     }
@@ -2014,7 +2029,7 @@ class Engine {
   // dimensionality and line-search mode
   void descend(uint32_t req, bool poly) {
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg, O_.dt_exact);
+      TV3_GO(kProfDescend, kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg, O_.dt_exact);
     };
     if (bt_.z3)
       poly ? go(k_descend<W, TS, true, true>) : go(k_descend<W, TS, true, false>);
@@ -2026,7 +2041,7 @@ class Engine {
   template <bool INIT>
   void wstep(uint32_t req) {
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.dt_exact);
+      TV3_GO(kProfShrink, kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.dt_exact);
     };
     if constexpr (kCplx) {  // anisotropic only (check_opts)
       bt_.z3 ? go(k_wstep<W, TS, INIT, true, false>) : go(k_wstep<W, TS, INIT, false, false>);
@@ -2042,11 +2057,11 @@ class Engine {
   // tau of the first inner iteration / fallback: D(d), |d|^2, A d
   void tau_first(uint32_t req, uint32_t forbid) {
     if (bt_.z3)
-      hipLaunchKernelGGL((k_tau_first<W, true>), gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
+      TV3_GO(kProfTau, (k_tau_first<W, true>), gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
     else
-      hipLaunchKernelGGL((k_tau_first<W, false>), gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
+      TV3_GO(kProfTau, (k_tau_first<W, false>), gn(), dim3(kBlock), 0, s_, kn(req, forbid), a_);
     fwd(a_.d, a_.tmpm, req);
-    hipLaunchKernelGGL(k_msq<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_.tmpm);
+    TV3_GO(kProfTau, k_msq<W>, gm(), dim3(kBlock), 0, s_, km(req, forbid), a_.tmpm);
   }
 
   // One inner iteration of every active instance (min_u, update_W, the stop
@@ -2067,7 +2082,7 @@ class Engine {
       tau_first(F_ACTIVE | F_FST, 0);
       cmode_ = 0;
     }
-    hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_ACTIVE));
+    TV3_GO(kProfScalars, fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_ACTIVE));
     // keep previous values (2d:536-537) by swapping buffers
     std::swap(a_.U, a_.Up); std::swap(a_.Ux, a_.Uxp); std::swap(a_.Uy, a_.Uyp);
     std::swap(a_.Uz, a_.Uzp); std::swap(a_.g, a_.gp); std::swap(a_.g2, a_.g2p);
@@ -2075,23 +2090,23 @@ class Engine {
     // descend (2d:540)
     descend(F_ACTIVE, !O_.ls_exact);
     get_g_products(F_ACTIVE, 0, 0);
-    hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE), 0);
+    TV3_GO(kProfScalars, fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE), 0);
     if (!uniform && (rc = read_summary(sum))) return rc;
     // Armijo backtracking (2d:548-575)
     if (uniform || (sum & F_BT)) {
       if (!O_.ls_exact) {
         // step size already chosen on the device: rebuild the state once
-        hipLaunchKernelGGL((k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-        hipLaunchKernelGGL((k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-        hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_BT), 1);
+        TV3_GO(kProfLineSearch, (k_interp_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+        TV3_GO(kProfLineSearch, (k_interp_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+        TV3_GO(kProfScalars, fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_BT), 1);
       } else {
-        hipLaunchKernelGGL(k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-        hipLaunchKernelGGL(k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+        TV3_GO(kProfLineSearch, k_bt_diff_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+        TV3_GO(kProfLineSearch, k_bt_diff_m<W>, gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
         // at most five rounds: the fifth failure sets the fallback
         for (int round = 0; round < 5 && (uniform || (sum & F_BT)); ++round) {
-          hipLaunchKernelGGL((k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
-          hipLaunchKernelGGL((k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
-          hipLaunchKernelGGL(fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa(F_BT), 0);
+          TV3_GO(kProfLineSearch, (k_bt_upd_n<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_BT), a_);
+          TV3_GO(kProfLineSearch, (k_bt_upd_m<W, TS>), gm(), dim3(kBlock), 0, s_, km(F_BT), a_);
+          TV3_GO(kProfScalars, fin_lams<TS>, gi(), dim3(kFin), 0, s_, fa(F_BT), 0);
           if (!uniform && (rc = read_summary(sum))) return rc;
         }
       }
@@ -2101,25 +2116,25 @@ class Engine {
       // formula, U = Up, descend(force)
       cmode_ = (!uniform && compact_ok_) ? pick_compact(F_FB) : 0;
       tau_first(F_FB, 0);
-      hipLaunchKernelGGL(fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB));
+      TV3_GO(kProfScalars, fin_tau<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB));
       descend(F_FB, false);
       get_g_products(F_FB, 0, 1);
       cmode_ = 0;
-      hipLaunchKernelGGL(fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB), 1);
+      TV3_GO(kProfScalars, fin_descend<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_FB), 1);
     }
     // ---- update_W, reference values, gradient, stop test ----
     wstep<false>(F_ACTIVE);
-    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE), 0);
+    TV3_GO(kProfScalars, fin_wstep<TS>, gi(), dim3(kFin), 0, s_, fa(F_ACTIVE), 0);
     if (!uniform && (rc = read_summary(sum))) return rc;
     if (uniform || (sum & F_INNER_DONE)) {
-      hipLaunchKernelGGL(k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
-      hipLaunchKernelGGL(fin_outer1<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_INNER_DONE));
-      hipLaunchKernelGGL(k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
-      hipLaunchKernelGGL(k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
-      hipLaunchKernelGGL(fin_mlp<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_MLP));
-      hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
-      hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_, isreal_);
-      hipLaunchKernelGGL(fin_refresh, gi(), dim3(kFin), 0, s_, fa(0));
+      TV3_GO(kProfMlp, k_outer1<W>, gn(), dim3(kBlock), 0, s_, kn(F_INNER_DONE), a_);
+      TV3_GO(kProfScalars, fin_outer1<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_INNER_DONE));
+      TV3_GO(kProfMlp, k_mlp_n<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_);
+      TV3_GO(kProfMlp, k_mlp_m<W>, gm(), dim3(kBlock), 0, s_, km(F_MLP), a_);
+      TV3_GO(kProfScalars, fin_mlp<TS>, gi(), dim3(kFin), 0, s_, fa_quiet(F_MLP));
+      TV3_GO(kProfMlp, k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_MLP), a_, 1);
+      TV3_GO(kProfSetup, (k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_, isreal_);
+      TV3_GO(kProfScalars, fin_refresh, gi(), dim3(kFin), 0, s_, fa(0));
       if (!uniform && (rc = read_summary(sum))) return rc;
     }
     rc = hip_check(ctx_, hipGetLastError(), "step kernels");
@@ -2165,28 +2180,28 @@ class Engine {
     const auto t0 = std::chrono::steady_clock::now();
     flags_fresh_ = false;
     // instance scalars and caller pointers
-    hipLaunchKernelGGL(k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad, ctx_->dflags);
+    TV3_GO(kProfScalars, k_init_scal, dim3((bt_.Bpad + 255) / 256), dim3(256), 0, s_, S_, bt_.B, bt_.Bpad, ctx_->dflags);
     std::vector<const vec_t *> hin(bt_.Bpad, nullptr);
     for (int i = 0; i < bt_.B; ++i) hin[i] = b[i];
     rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hin.data(), sizeof(vec_t *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
     if (rc) return rc;
     const dim3 gp_(std::max(1, std::min(nblk_m_, 1024)), bt_.G);
-    hipLaunchKernelGGL(k_pack<W>, gp_, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.b, 1);
+    TV3_GO(kProfSetup, k_pack<W>, gp_, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.b, 1);
     // scaleb
-    hipLaunchKernelGGL(k_absminmax<W>, gm(), dim3(kBlock), 0, s_, km(F_LIVE), a_.b, mm_);
-    hipLaunchKernelGGL(fin_scaleb<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE), mm_, a_.b);
-    hipLaunchKernelGGL(k_scale_m<W>, gm(), dim3(kBlock), 0, s_, km(F_LIVE), a_.b);
+    TV3_GO(kProfScaleb, k_absminmax<W>, gm(), dim3(kBlock), 0, s_, km(F_LIVE), a_.b, mm_);
+    TV3_GO(kProfScaleb, fin_scaleb<TS>, gi(), dim3(kBlock), 0, s_, fa(F_LIVE), mm_, a_.b);
+    TV3_GO(kProfScaleb, k_scale_m<W>, gm(), dim3(kBlock), 0, s_, km(F_LIVE), a_.b);
     // Atb = A^T bs; U = Atb
     bwd(a_.b, a_.Atb, F_LIVE, nullptr);
-    hipLaunchKernelGGL(k_init_U<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_);
+    TV3_GO(kProfSetup, k_init_U<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_);
     // D(U), shrinkage, get_g(force), get_gradient
-    hipLaunchKernelGGL(k_D<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.U, a_.Ux, a_.Uy, a_.Uz);
+    TV3_GO(kProfSetup, k_D<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.U, a_.Ux, a_.Uy, a_.Uz);
     wstep<true>(F_LIVE);
-    hipLaunchKernelGGL(fin_wstep<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE), 1);
+    TV3_GO(kProfScalars, fin_wstep<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE), 1);
     get_g_products(F_LIVE, 0, 1);
-    hipLaunchKernelGGL(fin_init2<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
-    hipLaunchKernelGGL(k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, 0);
-    hipLaunchKernelGGL((k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_, isreal_);
+    TV3_GO(kProfScalars, fin_init2<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
+    TV3_GO(kProfMlp, k_grad<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_, 0);
+    TV3_GO(kProfSetup, (k_finish<W, TS>), gn(), dim3(kBlock), 0, s_, kn(F_JUST_DONE), a_, isreal_);
     rc = hip_check(ctx_, hipGetLastError(), "init kernels");
     if (rc) return rc;
 
@@ -2203,7 +2218,7 @@ class Engine {
     rc = hip_check(ctx_, hipMemcpyAsync(ptrs_out_, hout.data(), sizeof(vec_t *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
     if (rc) return rc;
     const dim3 gpn(std::max(1, std::min(nblk_n_, 1024)), bt_.G);
-    hipLaunchKernelGGL(k_unpack<W>, gpn, dim3(kBlock), 0, s_, G_, a_.Uout, ptrs_out_);
+    TV3_GO(kProfSetup, k_unpack<W>, gpn, dim3(kBlock), 0, s_, G_, a_.Uout, ptrs_out_);
     bool any_ut = false;
     std::vector<const vec_t *> hut(bt_.Bpad, nullptr);  // alive until the sync below
     if (Ut) {
@@ -2211,9 +2226,9 @@ class Engine {
       if (any_ut) {
         rc = hip_check(ctx_, hipMemcpyAsync(ptrs_in_, hut.data(), sizeof(vec_t *) * bt_.Bpad, hipMemcpyHostToDevice, s_), "ptr copy");
         if (rc) return rc;
-        hipLaunchKernelGGL(k_pack<W>, gpn, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.rcdU, 0);
-        hipLaunchKernelGGL(k_relerr<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.rcdU, a_.Uout);
-        hipLaunchKernelGGL(fin_relerr<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
+        TV3_GO(kProfSetup, k_pack<W>, gpn, dim3(kBlock), 0, s_, G_, ptrs_in_, a_.rcdU, 0);
+        TV3_GO(kProfSetup, k_relerr<W>, gn(), dim3(kBlock), 0, s_, kn(F_LIVE), a_.rcdU, a_.Uout);
+        TV3_GO(kProfScalars, fin_relerr<TS>, gi(), dim3(kFin), 0, s_, fa(F_LIVE));
       }
     }
     std::vector<Scal> hs(bt_.Bpad);

@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from .abi import Options, Info, TraceRec, Geometry, default_options  # noqa: F401
+from .abi import Options, Info, TraceRec, Geometry, ProfileSlot, default_options  # noqa: F401
 
 __all__ = ["Tval3Error", "lib", "lib_path", "Context", "Plan", "Options",
            "Info", "TraceRec", "default_options"]
@@ -62,6 +62,7 @@ def lib():
         "tv3_ctx_profile": (_I, [_VP, _I]),
         "tv3_ctx_profile_read": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_LL),
                                       C.POINTER(C.c_double)]),
+        "tv3_ctx_profile_slots": (_I, [_VP, C.POINTER(ProfileSlot), _I, C.POINTER(_I)]),
         "tv3_plan_dense": (_I, [_VP, _I, _I, _I, _VP, C.POINTER(_VP)]),
         "tv3_plan_sparse": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
         "tv3_plan_geometry": (_I, [_VP, _VP, C.POINTER(_VP)]),
@@ -187,8 +188,30 @@ class Context:
         self._check(lib().tv3_ctx_sync(self.h))
 
     def profile(self, enable=True):
-        """Start (and reset) or stop per-launch timing of the A / A^T products."""
-        self._check(lib().tv3_ctx_profile(self.h, int(bool(enable))))
+        """Start (and reset) or stop per-launch timing: False / 0 off, True / 1
+        the A / A^T products only, 2 every launch of the solver (slower)."""
+        level = int(enable)
+        if level not in (0, 1, 2):
+            raise ValueError(f"profile level {enable!r}: 0, 1 or 2")
+        self._check(lib().tv3_ctx_profile(self.h, level))
+
+    def profile_slots(self):
+        """[(slot, name, runs, ms_total)] of the slots that ran since
+        profile() was switched on, in slot order (slot numbers as in the
+        reference's profile_info table where one pass implements that
+        function, 30 and up for fused passes)."""
+        n = C.c_int(0)
+        self._check(lib().tv3_ctx_profile_slots(self.h, None, 0, C.byref(n)))
+        buf = (ProfileSlot * max(1, n.value))()
+        self._check(lib().tv3_ctx_profile_slots(self.h, buf, n.value, C.byref(n)))
+        return [(r.slot, r.name.decode(), r.runs, r.ms) for r in buf[:n.value]]
+
+    def profile_report(self):
+        """The slot table as text, one line per slot in the reference's
+        PROFILING print format (name, ms per run, runs, total ms;
+        3d/tval3_cpu_3d.cpp:1337-1348)."""
+        return "".join(f"{name}:\t {ms / runs:.2f} ms (runs: {runs}, total: {ms:.1f} ms)\n"
+                       for _, name, runs, ms in self.profile_slots() if runs)
 
     def profile_read(self):
         """{'A': (ms_total, launches, bytes), 'At': (...)}: summed launch

@@ -5,7 +5,7 @@ made here too (tests/test_abi.py checks the sizes against the compiled library).
 """
 import ctypes as C
 
-__all__ = ["Options", "Info", "TraceRec", "Geometry", "default_options"]
+__all__ = ["Options", "Info", "TraceRec", "Geometry", "ProfileSlot", "default_options"]
 
 
 class Options(C.Structure):
@@ -46,6 +46,12 @@ class TraceRec(C.Structure):
                 ("lam3", C.c_double), ("lam4", C.c_double),
                 ("lam5", C.c_double), ("mu", C.c_double),
                 ("beta", C.c_double)]
+
+
+class ProfileSlot(C.Structure):
+    """tv3_profile_slot: one slot of the launch-timing report."""
+    _fields_ = [("slot", C.c_int), ("runs", C.c_longlong), ("ms", C.c_double),
+                ("name", C.c_char * 96)]
 
 
 _IP = C.POINTER(C.c_int)
