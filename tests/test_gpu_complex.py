@@ -174,3 +174,33 @@ def test_complex_batch_and_errors(ctx):
         ctx.solve(p1, b1.real.copy(), (16, 16, 1), opts)
     with pytest.raises(tval3_amd.Tval3Error, match="isotropic"):
         ctx.solve(p1, b1, (16, 16, 1), tval3_amd.default_options(maxit=5, tv_iso=1))
+
+
+@pytest.mark.parametrize("t", [np.complex64, np.complex128])
+def test_complex_graph_mode(ctx, t):
+    """Steps replayed from a hipGraph give the host-driven complex result bit
+    for bit, for a single solve (twice, a fresh capture each time), a batch
+    and a ring geometry."""
+    A, b = _dense(18, t=t)
+    plan = ctx.plan_dense(A)
+    kw = dict(maxit=60, ls_exact=1, isreal=0)
+    U0, i0, _ = ctx.solve(plan, b, (16, 16, 1), tval3_amd.default_options(**kw))
+    for rep in range(2):
+        U1, i1, _ = ctx.solve(plan, b, (16, 16, 1), tval3_amd.default_options(graph=1, **kw))
+        assert np.array_equal(U0.cpu().numpy(), U1.cpu().numpy()), rep
+        assert (i1.total_iters, i1.outer_iters) == (i0.total_iters, i0.outer_iters)
+    # a batch shares one operator kind: two dense instances of different size
+    A2, b2 = _dense(19, m=120, n=256, t=t)
+    plans, bs, dims = [plan, ctx.plan_dense(A2)], [b, b2], [(16, 16, 1), (16, 16, 1)]
+    res = [ctx.solve_batch(plans, bs, dims, tval3_amd.default_options(graph=g, **kw))
+           for g in (0, 1)]
+    for k in range(2):
+        assert np.array_equal(res[0][0][k].cpu().numpy(), res[1][0][k].cpu().numpy()), k
+        assert res[0][1][k].total_iters == res[1][1][k].total_iters
+    geo = synth.ring_geometry(32, 32)
+    op = O.Op.geometry(geo)
+    bg = (op.gemv(synth.disk_phantom(32, seed=7)) * (0.8 - 0.6j)).astype(t)
+    gp = ctx.plan_geometry(geo)
+    Ug = [ctx.solve(gp, bg, (32, 32, 1), tval3_amd.default_options(graph=g, **kw))[0].cpu().numpy()
+          for g in (0, 1)]
+    assert np.array_equal(Ug[0], Ug[1])

@@ -195,3 +195,35 @@ def test_f64_plan_errors(ctx):
         ctx.gemv(p64, x.ravel())
     with pytest.raises(tval3_amd.Tval3Error, match="tv3_gemv_f64"):
         ctx.gemv(p32, x.ravel(), f64=True)
+
+
+@pytest.mark.parametrize("ls", [1, 0])
+def test_graph_mode_f64(ctx, ls):
+    """Steps replayed from a hipGraph give the host-driven FP64 result bit
+    for bit: one ring solve twice in a row (a fresh capture each time) and a
+    batch of two operators."""
+    geo = synth.ring_geometry(48, 48)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(48, seed=6)
+    b = synth.add_noise(op.gemv(x), 0.005, 6).astype(np.float64)
+    plan = ctx.plan_geometry(geo)
+    kw = dict(maxit=100, ls_exact=ls)
+    U0, i0, t0 = ctx.solve(plan, b, (48, 48, 1), tval3_amd.default_options(**kw), Ut=x,
+                           trace_cap=100, f64=True)
+    for rep in range(2):
+        U1, i1, t1 = ctx.solve(plan, b, (48, 48, 1), tval3_amd.default_options(graph=1, **kw),
+                               Ut=x, trace_cap=100, f64=True)
+        assert np.array_equal(U0.cpu().numpy(), U1.cpu().numpy()), rep
+        assert (i1.total_iters, i1.outer_iters) == (i0.total_iters, i0.outer_iters)
+        assert [(t.backtracks, t.fallback, t.tau) for t in t1] == \
+               [(t.backtracks, t.fallback, t.tau) for t in t0]
+    # a batch shares one operator kind: two dense instances of different size
+    A1, _, b1 = _dense_case(16, 16, 0.4, 11)
+    A2, _, b2 = _dense_case(20, 12, 0.5, 12)
+    plans = [ctx.plan_dense(A1, f64=True), ctx.plan_dense(A2, f64=True)]
+    bs, dims = [b1, b2], [(16, 16, 1), (20, 12, 1)]
+    res = [ctx.solve_batch(plans, bs, dims, tval3_amd.default_options(graph=g, **kw), f64=True)
+           for g in (0, 1)]
+    for k in range(2):
+        assert np.array_equal(res[0][0][k].cpu().numpy(), res[1][0][k].cpu().numpy()), k
+        assert res[0][1][k].total_iters == res[1][1][k].total_iters
