@@ -148,6 +148,7 @@ def run_ring(args, ctx):
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
+    args.one_step = lambda: ctx.solve_batch(plans, bs, dl, opts)
     value = B * args.steps / t_steps
     rel_b0 = float(torch.linalg.norm(Us[0] - U0) / torch.linalg.norm(U0))
     iters = [inf.total_iters for inf in infos]
@@ -246,6 +247,7 @@ def run_dense_batch(args, ctx):
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
+    args.one_step = lambda: ctx.solve_batch(plans, bs, dims, opts)
     value = B * args.steps / t_steps
     ops, roof = roofline(prof, {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_blk"},
                          "dense-batch")
@@ -332,6 +334,7 @@ def run_diff(args, ctx):
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
+    args.one_step = grad_once
     ops, roof = roofline(prof, {"A": "k_fwd_w1", "At": "k_bwd_w1"}, "diff")
     roof["cache_resident"] = True
     roof["csr_equivalent"] = True
@@ -443,6 +446,7 @@ def run_single(args, ctx, workload):
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
+    args.one_step = lambda: ctx.solve(plan, bt, dims, opts, Ut=x, f64=f64)
     ops, roof = roofline(prof, kern, workload)
     if workload in ("hemisphere", "ring-f64"):
         roof["csr_equivalent"] = True
@@ -486,6 +490,9 @@ def main():
     ap.add_argument("--f64-sample", type=int, default=4)
     ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
                     help="1: replay each pair of solver steps from a hipGraph")
+    ap.add_argument("--slots", default=None, metavar="FILE",
+                    help="after the timed steps, run one more step with every launch "
+                         "timed and write the per-slot report to FILE")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -515,6 +522,16 @@ def main():
         out = run_diff(args, ctx)
     else:
         out = run_single(args, ctx, args.workload)
+    if args.slots:
+        # outside the timed region: an event pair per launch slows the step
+        ctx.profile(2)
+        args.one_step()
+        report = ctx.profile_report()
+        ctx.profile(False)
+        with open(args.slots, "w") as f:
+            f.write(f"per-slot launch times of one step of --workload {args.workload} "
+                    f"({out['config']['workload']})\n{report}")
+        log(report)
     if args.copy_mb:
         src = torch.ones(args.copy_mb * 262144, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

@@ -118,8 +118,10 @@ struct PlanDev {
   // neighbouring emitters and neighbouring receivers come together
   const int *ray_order;
   // the transpose lists again in the 64-lane transpose's walk order: slot s
-  // is voxel t_col[s], its rays t_ray[t_ptr[s] .. t_ptr[s+1])
+  // is voxel t_col[s], its rays t_ray[t_ptr[s] .. t_ptr[s+1]); the voxels no
+  // ray crosses come last, from slot t_nz on
   const int *t_ptr, *t_ray, *t_col;
+  int t_nz;
   float scale;
 };
 

@@ -136,7 +136,10 @@ int hip_check(tv3_ctx *ctx, hipError_t e, const char *what) {
 // butterfly.  Geometry and CSR rows issue kW1Unroll index and gather loads
 // before they add.  kSubF / kSubT: lanes per row of A x / A^T y (8 and 8
 // measured best on the 128^3 hemisphere and the 256 ring's plain path;
-// four loads ahead took A^T y there from 0.250 to 0.231 ms).
+// four loads ahead took A^T y there from 0.250 to 0.231 ms; with the
+// uncrossed voxels split off, A^T y of the hemisphere takes 0.186 ms at
+// 8 lanes x 4 loads, 0.187 at 16 x 4, 0.193 at 16 x 2, 0.203 at 8 x 2,
+// 0.235 at 4 x 4 and 0.251 at 32 x 2).
 constexpr int kSubF = 8;
 constexpr int kSubT = 8;
 // geometry columns of A^T y in the plan's tiled voxel order (t_*): 0.224 ->
@@ -365,6 +368,28 @@ int vox_tile() {
   return v;
 }
 
+// Brick of the transpose's column order for 3-D volumes (TV3_VOX_BRICK=b:
+// b x b x b bricks in raster order of bricks; 0 keeps the squares of
+// TV3_VOX_TILE in each slice).
+int vox_brick() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_VOX_BRICK");
+    return e ? std::max(0, std::atoi(e)) : 0;
+  }();
+  return v;
+}
+
+// TV3_VOX_NZFIRST=0 keeps the voxels no ray crosses in their tile places of
+// the transpose order (default: they go last, and the W = 1 transpose
+// writes them one thread per voxel)
+bool vox_nz_first() {
+  static const bool v = [] {
+    const char *e = std::getenv("TV3_VOX_NZFIRST");
+    return !(e && e[0] == '0');
+  }();
+  return v;
+}
+
 // Tile of the forward walk order: rays from te consecutive emitters to tr
 // consecutive receivers (TV3_RAY_TILE=te,tr; 0,0 keeps the measurement
 // order).  Default 16 x 32, the best of a sweep on the 256-transducer ring
@@ -556,6 +581,10 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
 // transposed product: y[col] = sum_k A[k, col] x[k] (- sub[col])
 // ---------------------------------------------------------------------------
 
+// A geometry plan of one layer sends the voxels no ray crosses (slots
+// t_nz .. n of its transpose order) to blocks of their own, one thread per
+// voxel, after the ceil(t_nz / (kBlock / L)) blocks of L lanes per voxel; the
+// grid has one block more than ceil(n / (kBlock / L)), which covers both.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
                                                    const float *x, float *y,
@@ -571,10 +600,25 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
   // itself; a geometry column slot walks the plan's tiled voxel order)
   const bool geo = P.kind == kGeometry;
   const bool tiled = kW1TiledT && geo && P.t_col;
+  const bool split = tiled && gd.n == P.n;
+  if (split) {
+    const int nbA = (P.t_nz + kBlock / L - 1) / (kBlock / L);
+    if ((int)blockIdx.x >= nbA) {
+      const long long k = P.t_nz + (long long)(blockIdx.x - nbA) * kBlock + threadIdx.x;
+      if (k < P.n) {
+        const long long o = gd.off_n + P.t_col[k];
+        float v = 0.0f;
+        if (sub) v = v - sub[o];
+        y[o] = v;
+      }
+      return;
+    }
+  }
+  const int colEnd = split ? P.t_nz : gd.n;
   int ac = col % P.n;
   const float *xg = x + gd.off_m + (long long)(col / P.n) * P.m;
   double acc = 0.0;
-  if (col < gd.n) {
+  if (col < colEnd) {
     constexpr int U = kW1Unroll;
     const int *ptr = tiled ? P.t_ptr : geo ? P.g_cptr : P.c_ptr;
     const int *ind = tiled ? P.t_ray : geo ? P.g_cray : P.c_ind;
@@ -598,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
     }
   }
   acc = lane_sum<L>(acc);
-  if (sl == 0 && col < gd.n) {
+  if (sl == 0 && col < colEnd) {
     const long long o = gd.off_n + (long long)(col / P.n) * P.n + ac;
     float v = (float)acc;
     if (sub) v = v - sub[o];
@@ -1277,7 +1321,9 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                        (const double *)scratch, y, S, mask, sub, sstride);
     return;
   }
-  dim3 grid((max_n + kBlock / kSubT - 1) / (kBlock / kSubT), G);
+  // one block more than the columns need: the voxels no ray crosses take
+  // blocks of their own (k_bwd_w1)
+  dim3 grid((max_n + kBlock / kSubT - 1) / (kBlock / kSubT) + 1, G);
   hipLaunchKernelGGL(k_bwd_w1<kSubT>, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask,
                      sub);
 }
@@ -2125,7 +2171,16 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
   // transpose walk order: t x t tiles of each slice (raster order for t = 0)
   std::vector<int> corder;
   corder.reserve(nvox);
-  {
+  if (g->rv_z > 1 && vox_brick() > 0) {
+    const int b = vox_brick();
+    for (int z0 = 0; z0 < g->rv_z; z0 += b)
+      for (int y0 = 0; y0 < g->rv_y; y0 += b)
+        for (int x0 = 0; x0 < g->rv_x; x0 += b)
+          for (int z = z0; z < std::min(g->rv_z, z0 + b); ++z)
+            for (int y = y0; y < std::min(g->rv_y, y0 + b); ++y)
+              for (int x = x0; x < std::min(g->rv_x, x0 + b); ++x)
+                corder.push_back(x * sx + y * sy + z * sz);
+  } else {
     const int t = vox_tile() > 0 ? vox_tile() : std::max(g->rv_x, g->rv_y);
     for (int z = 0; z < g->rv_z; ++z)
       for (int y0 = 0; y0 < g->rv_y; y0 += t)
@@ -2178,6 +2233,14 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
     TV3_TRY(hipMalloc(&p->cray, sizeof(int) * cray.size()), "hipMalloc cray");
     TV3_TRY(hipMemcpyAsync(p->cptr, cptr.data(), sizeof(int) * (nvox + 1), hipMemcpyHostToDevice, s), "copy cptr");
     TV3_TRY(hipMemcpyAsync(p->cray, cray.data(), sizeof(int) * cray.size(), hipMemcpyHostToDevice, s), "copy cray");
+    // voxels no ray crosses go to the end of the transpose order (stable)
+    int t_nz = (int)nvox;
+    if (vox_nz_first()) {
+      auto mid = std::stable_partition(corder.begin(), corder.end(),
+                                       [&](int v) { return cptr[v + 1] > cptr[v]; });
+      t_nz = (int)(mid - corder.begin());
+    }
+    dv.t_nz = t_nz;
     // the same lists in the walk order of the 64-lane transpose
     std::vector<int> tptr(nvox + 1, 0), tray(cray.size());
     for (long long k = 0; k < nvox; ++k) {
