@@ -32,16 +32,45 @@ __device__ __forceinline__ int ray_lin(const RayDesc &r, int j) {
 }
 
 // ---- wavefront / block reductions (fixed order) ------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// One DPP lane move of a double (both 32-bit halves, same control); lanes
+// whose source is outside the row, or whose row is not in RM, read 0.
+template <int CTRL, int RM = 0xf>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, RM, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, RM, 0xf, true);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, left in lane 63, by register moves (no LDS round
+// trips): an inclusive scan of each 16-lane row (row_shr 1, 2, 4, 8), then
+// the row totals (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2
+// and 3).  The tree is fixed, so the result does not depend on timing.
+__device__ __forceinline__ double wave_total63(double v) {
+  v += dpp_f64<0x111>(v);
+  v += dpp_f64<0x112>(v);
+  v += dpp_f64<0x114>(v);
+  v += dpp_f64<0x118>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
   return v;
 }
 
+__device__ __forceinline__ double lane63(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// the wavefront sum, returned to every lane
+__device__ __forceinline__ double wave_sum(double v) { return lane63(wave_total63(v)); }
+
 // Sums NS per-thread values over the threads of one lane class (tid % W) of a
 // 256-thread block and writes the block partial of instance inst0 + b to
-// part[((inst0 + b) * kNslot + slot[s]) * kNblkMax + blk].  Order: butterfly
-// inside each wavefront, then wavefronts 0..3 in order.
+// part[((inst0 + b) * kNslot + slot[s]) * kNblkMax + blk].  Order: inside
+// each wavefront the DPP tree of wave_total63 (W = 1) or a butterfly over
+// the lanes of one class, then wavefronts 0..3 in order.
 __device__ __forceinline__ bool sel(uint32_t f, uint32_t req, uint32_t forbid) {
   return (f & req) == req && !(f & forbid);
 }
@@ -60,9 +89,14 @@ __device__ __forceinline__ void block_partials(const double (&v)[NS],
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     double x = v[s];
+    if constexpr (W == 1) {
+      x = wave_total63(x);
+      if (lane == 63) red[wv][s][0] = x;
+    } else {
 #pragma unroll
-    for (int off = 32; off >= W; off >>= 1) x += __shfl_xor(x, off, 64);
-    if (lane < W) red[wv][s][lane] = x;
+      for (int off = 32; off >= W; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (lane < W) red[wv][s][lane] = x;
+    }
   }
   __syncthreads();
   for (int t = threadIdx.x; t < NS * W; t += kBlock) {

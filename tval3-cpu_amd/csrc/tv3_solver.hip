@@ -963,9 +963,9 @@ struct FA {
   OptDev O;
 };
 
-// Sums of NS partial-sum slots of one instance by a 256-thread block, in one
-// pass: thread t adds partials t, t+256, ...; wavefront butterfly; the four
-// wavefronts in order.  Every thread receives the results.
+// Sums of NS partial-sum slots of one instance by a kFin-thread block, in one
+// pass: thread t adds partials t, t+kFin, ...; the DPP tree of wave_sum;
+// the wavefronts in order.  Every thread receives the results.
 // (one-block-per-instance kernels run with kFin threads)
 template <int NS>
 __device__ __forceinline__ void fin_sums(const FA &F, int inst,
@@ -1762,7 +1762,8 @@ class Engine {
     return rc;
   }
 
-  // times fn() (two launches after a warm-up) on the solver stream, in ms
+  // times fn() (kTuneReps launches after a warm-up) on the solver stream, in ms
+  static constexpr int kTuneReps = 8;
   template <class F>
   float time_launches(F fn) {
     hipEvent_t e0, e1;
@@ -1770,8 +1771,7 @@ class Engine {
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 1e30f; }
     fn();
     (void)hipEventRecord(e0, s_);
-    fn();
-    fn();
+    for (int k = 0; k < kTuneReps; ++k) fn();
     (void)hipEventRecord(e1, s_);
     float ms = 1e30f;
     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
@@ -1784,35 +1784,53 @@ class Engine {
   int tune_dense() {
     // the FP64 kernels have one launch shape
     if (!(W == 1 && bt_.kind == kDense) || kOnlyW1) return TV3_OK;
-    DenseTune best{2, 2, 8};
+    // Below kTuneMinBytes of stored A a product takes a few microseconds,
+    // and event timings of such launches measure launch overhead more than
+    // the kernel: fresh processes picked R = 1, 2 or 4 at random, and R = 4
+    // ran 14 us inside the solve where R = 1 runs 5.5 us (64x64, m = 1229).
+    // Small batches take the shape that was fastest inside the solve there.
+    constexpr long long kTuneMinBytes = 256ll << 20;
+    DenseTune best{1, 4, 8};
     if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
     long long work = 0;
     for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
+    if (work * (bt_.dtype == TV3_BF16 ? 2 : 4) < kTuneMinBytes) { dense_tune_set(best); return TV3_OK; }
     const TuneKey key{ctx_->device, bt_.dtype, bt_.G, bt_.max_n, bt_.max_m, work, bt_.layered ? 1 : 0};
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
       if (it != g_tune_cache.end()) { dense_tune_set(it->second); return TV3_OK; }
     }
-    float tbest = 1e30f;
-    for (int r : {1, 2, 4}) {
-      dense_tune_set(DenseTune{r, best.unroll, best.cols});
-      const float t = time_launches([&] {
-        launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, a_.Atb, a_.tmpm, flags_of(S_), 0);
-      });
-      if (t < tbest) { tbest = t; best.rows = r; }
-    }
-    tbest = 1e30f;
-    for (int c : {8, 4}) {
-      for (int u : {1, 2, 4, 8}) {
-        dense_tune_set(DenseTune{best.rows, u, c});
-        const float t = time_launches([&] {
-          launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, flags_of(S_), 0, nullptr, scratch_,
-                        bt_.layered ? 0 : submat_mb_);
-        });
-        if (t < tbest) { tbest = t; best.unroll = u; best.cols = c; }
+    // every candidate is timed in kTuneRounds interleaved rounds and keeps
+    // its fastest round
+    constexpr int kTuneRounds = 3;
+    const int rows_c[3] = {1, 2, 4};
+    float tr[3] = {1e30f, 1e30f, 1e30f};
+    for (int rd = 0; rd < kTuneRounds; ++rd)
+      for (int i = 0; i < 3; ++i) {
+        dense_tune_set(DenseTune{rows_c[i], best.unroll, best.cols});
+        tr[i] = std::min(tr[i], time_launches([&] {
+          launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, a_.Atb, a_.tmpm, flags_of(S_), 0);
+        }));
       }
-    }
+    best.rows = rows_c[std::min_element(tr, tr + 3) - tr];
+    const int cols_c[2] = {8, 4}, unr_c[4] = {1, 2, 4, 8};
+    float tb[2][4];
+    for (auto &row : tb)
+      for (float &t : row) t = 1e30f;
+    for (int rd = 0; rd < kTuneRounds; ++rd)
+      for (int ci = 0; ci < 2; ++ci)
+        for (int ui = 0; ui < 4; ++ui) {
+          dense_tune_set(DenseTune{best.rows, unr_c[ui], cols_c[ci]});
+          tb[ci][ui] = std::min(tb[ci][ui], time_launches([&] {
+            launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, a_.tmpm, a_.tmpn, flags_of(S_), 0, nullptr, scratch_,
+                          bt_.layered ? 0 : submat_mb_);
+          }));
+        }
+    float tbest = 1e30f;
+    for (int ci = 0; ci < 2; ++ci)
+      for (int ui = 0; ui < 4; ++ui)
+        if (tb[ci][ui] < tbest) { tbest = tb[ci][ui]; best.unroll = unr_c[ui]; best.cols = cols_c[ci]; }
     dense_tune_set(best);
     int rc = hip_check(ctx_, hipGetLastError(), "autotune launches");
     if (rc) return rc;
