@@ -86,6 +86,35 @@ def test_geometry_gemv(ctx, name):
     assert abs(lhs - rhs) <= 1e-5 * (np.abs(fa) @ np.abs(y) + 1)
 
 
+@pytest.mark.parametrize("case", ["one_voxel", "no_ray"])
+def test_geometry_mostly_uncrossed(ctx, case):
+    """3-D volumes where (almost) no voxel is crossed: one ray whose emitter
+    and receiver coincide (a walk of one voxel) and two rows no pair maps to,
+    or no mapped pair at all.  The transpose writes the uncrossed voxels
+    apart from the crossed ones (one thread each), so this covers a crossed
+    part of one voxel and of none."""
+    q = 16
+    up = np.array([1 if case == "one_voxel" else 0], np.int32)
+    geo = synth.GeometryArrays(1, up, [5], [7], [3], [5], [7], [3], (q, q, q), 3, scale=1.25)
+    plan = ctx.plan_geometry(geo)
+    op = O.Op.geometry(geo)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(geo.n).astype(np.float32)
+    y = rng.standard_normal(geo.m).astype(np.float32)
+    g_f = ctx.gemv(plan, x).cpu().numpy()
+    g_b = ctx.gemv(plan, y, trans=True).cpu().numpy()
+    np.testing.assert_array_equal(g_f, op.gemv(x))
+    np.testing.assert_array_equal(g_b, op.gemv(y, trans=True))
+    v = 5 + 7 * q + 3 * q * q
+    want_b = np.zeros(geo.n, np.float32)
+    want_f = np.zeros(geo.m, np.float32)
+    if case == "one_voxel":
+        want_b[v] = y[0] * np.float32(1.25)
+        want_f[0] = x[v] * np.float32(1.25)
+    np.testing.assert_array_equal(g_b, want_b)
+    np.testing.assert_array_equal(g_f, want_f)
+
+
 def test_sparse_gemv(ctx):
     geo = synth.ring_geometry(64, 64, scale=1.0)
     op = O.Op.geometry(geo)

@@ -2218,7 +2218,8 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
   TV3_TRY(hipMemcpyAsync(d_ptr, p->walk_ptr.data(), sizeof(long long) * (m + 1), hipMemcpyHostToDevice, s), "copy walk ptr");
   hipLaunchKernelGGL(k_walk_export, dim3((unsigned)((m + 3) / 4)), dim3(kBlock), 0, s, p->rays, (int)m, d_ptr, d_lin);
   TV3_TRY(hipGetLastError(), "k_walk_export");
-  TV3_TRY(hipMemcpyAsync(lin.data(), d_lin, sizeof(int) * nnz, hipMemcpyDeviceToHost, s), "copy walk");
+  if (nnz > 0)
+    TV3_TRY(hipMemcpyAsync(lin.data(), d_lin, sizeof(int) * nnz, hipMemcpyDeviceToHost, s), "copy walk");
   TV3_TRY(hipStreamSynchronize(s), "walk sync");
   {
     // voxel-major transpose, rays ascending within each voxel (counting sort)
@@ -2317,7 +2318,7 @@ int tv3_plan_walk(tv3_ctx *ctx, const tv3_plan *p, long long *ptr, int *lin) {
     hipLaunchKernelGGL(k_walk_export, dim3((unsigned)((m + 3) / 4)), dim3(kBlock), 0, ctx->stream, p->dev.rays, (int)m, d_ptr, d_lin);
     rc = hip_check(ctx, hipGetLastError(), "k_walk_export");
   }
-  if (!rc) rc = hip_check(ctx, hipMemcpyAsync(lin, d_lin, sizeof(int) * nnz, hipMemcpyDeviceToHost, ctx->stream), "copy");
+  if (!rc && nnz > 0) rc = hip_check(ctx, hipMemcpyAsync(lin, d_lin, sizeof(int) * nnz, hipMemcpyDeviceToHost, ctx->stream), "copy");
   if (!rc) rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
   if (d_ptr) (void)hipFree(d_ptr);
   if (d_lin) (void)hipFree(d_lin);
