@@ -18,11 +18,16 @@
 //   TVAL3_GPU_LS_POLY   1: quadratic-model Armijo search (one interpolation
 //                       pass per step); default 0: the reference's
 //                       round-by-round backtracking
+//   TVAL3_GPU_PROFILE   1: time every launch of each solve and print the
+//                       per-slot table to stdout afterwards, one line per
+//                       slot in the reference's PROFILING print format
+//                       (3d/tval3_cpu_3d.cpp:1337-1348); default 0
 #ifndef TVAL3_GPU_COMMON_HPP_
 #define TVAL3_GPU_COMMON_HPP_
 
 #include <chrono>
 #include <complex>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <string>
@@ -37,6 +42,7 @@ struct Session {
   std::mutex mu;
   tv3_ctx *ctx = nullptr;
   int ls_poly = 0;
+  int profile = 0;
 };
 
 inline Session &session() {
@@ -58,6 +64,8 @@ inline tv3_ctx *context(Session &s) {
     const char *dv = std::getenv("TVAL3_GPU_DEVICE");
     const char *lp = std::getenv("TVAL3_GPU_LS_POLY");
     s.ls_poly = (lp && std::atoi(lp) != 0) ? 1 : 0;
+    const char *pr = std::getenv("TVAL3_GPU_PROFILE");
+    s.profile = (pr && std::atoi(pr) != 0) ? 1 : 0;
     tv3_ctx *c = nullptr;
     const int rc = tv3_ctx_create(dv ? std::atoi(dv) : 0, nullptr, &c);
     if (rc) {
@@ -192,6 +200,22 @@ inline int solve_call(tv3_ctx *c, const tv3_plan *A, const tv3_options *o, int d
                         nullptr, 0);
 }
 
+// The slot table of the last solve, as the reference prints profile_info
+// (name: ms per run, runs, total ms); profiling is switched off after it.
+template <class Exc>
+inline void print_profile(tv3_ctx *c) {
+  tv3_profile_slot sl[64];
+  int n = 0;
+  int rc = tv3_ctx_profile_slots(c, sl, 64, &n);
+  if (rc) fail<Exc>(c, rc);
+  for (int i = 0; i < n && i < 64; ++i)
+    std::printf("%s:\t %.2f ms (runs: %lld, total: %.1f ms)\n", sl[i].name,
+                sl[i].ms / (double)sl[i].runs, sl[i].runs, sl[i].ms);
+  std::fflush(stdout);
+  rc = tv3_ctx_profile(c, 0);
+  if (rc) fail<Exc>(c, rc);
+}
+
 // Runs one solve for host data of element type T (float, double,
 // std::complex<float>, std::complex<double>).
 // make_plan(ctx, PlanHolder&, KeepAll&) builds the operator plan, keeping
@@ -214,9 +238,11 @@ inline Info solve_host(mat<T> &U, const mat<T> &b, const Opts &opts, const mat<T
   DevBuf<Exc> *dUt = has_ut ? keep.put(c, &Ut.data()[0], n * sizeof(T)) : nullptr;
   const tv3_options co = to_c(opts, scal_double, s.ls_poly, submat_mb);
   tv3_info ci{};
+  if (s.profile && tv3_ctx_profile(c, 2)) fail<Exc>(c, TV3_ERR_HIP);
   const int rc = solve_call(c, plan.p, &co, dx, dy, dz, db->template as<T>(), dU.template as<T>(),
                             dUt ? dUt->template as<T>() : nullptr, &ci);
   if (rc) fail<Exc>(c, rc);
+  if (s.profile) print_profile<Exc>(c);
   dU.download(&U.data()[0], n * sizeof(T));
   Info info;
   from_c(ci, info);

@@ -10,6 +10,7 @@ selects the reference's round-by-round backtracking); double/double and
 complex<double> results are held to the FP64 bar of f64_bar.
 """
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -89,10 +90,21 @@ def _complex_op(op, kind, d):
     return op
 
 
-def _run_and_compare(name, kind, d, op, b, dims, maxit):
-    print(f"shim case {name} {kind} dims={dims} m={b.size}", flush=True)
-    r = subprocess.run([_driver(name), kind, d], capture_output=True, text=True, timeout=120)
+def _run_and_compare(name, kind, d, op, b, dims, maxit, profile=False):
+    print(f"shim case {name} {kind} dims={dims} m={b.size} profile={profile}", flush=True)
+    env = dict(os.environ, TVAL3_GPU_PROFILE="1" if profile else "0")
+    r = subprocess.run([_driver(name), kind, d], capture_output=True, text=True, timeout=120,
+                       env=env)
     assert r.returncode == 0, r.stderr
+    # TVAL3_GPU_PROFILE=1: one slot table per solve in the reference's
+    # PROFILING print format; it changes no result (checked below)
+    lines = [l for l in r.stdout.splitlines() if "(runs: " in l]
+    if profile:
+        pat = re.compile(r"^.+:\t \d+\.\d\d ms \(runs: \d+, total: \d+\.\d ms\)$")
+        assert lines and all(pat.match(l) for l in lines), r.stdout[-2000:]
+        assert sum(l.startswith("gemv CblasNoTrans: A x:") for l in lines) >= 5
+    else:
+        assert not lines
     n = int(np.prod(dims))
     for tag, sd in (("f", 0), ("d", 1)):
         Ug = np.fromfile(os.path.join(d, f"U_{tag}.bin"), np.float32)
@@ -206,7 +218,7 @@ def test_shim_3d_geometry(tmp_path):
                "rv_z": geo.rv[2], "scale": geo.scale, "maxit": 40},
            b=b, use_path=geo.use_path, xe=geo.xe, ye=geo.ye, ze=geo.ze,
            xr=geo.xr, yr=geo.yr, zr=geo.zr)
-    _run_and_compare("shim_3d", "geometry", d, op, b, (grid, grid, grid), 40)
+    _run_and_compare("shim_3d", "geometry", d, op, b, (grid, grid, grid), 40, profile=True)
 
 
 @pytest.mark.gpu
