@@ -97,3 +97,24 @@ def test_profile_batch_long_solve(ctx):
     assert all(i.total_iters == 800 for i in infos)
     assert counts[2][10] == counts[1][10] and counts[2][11] == counts[1][11]
     assert sum(counts[2].values()) > 4096, counts[2]
+
+
+@pytest.mark.parametrize("kind", ["f64", "c64", "c128"])
+def test_profile_slots_variants(ctx, kind):
+    """The double/double and complex solvers tag their launches with the
+    same slots, and timing changes none of their results either."""
+    geo, b = _ring(32, 6)
+    plan = ctx.plan_geometry(geo)
+    f64 = kind == "f64"
+    bb = {"f64": b.astype(np.float64), "c64": (b * (0.6 + 0.8j)).astype(np.complex64),
+          "c128": (b * (0.6 + 0.8j)).astype(np.complex128)}[kind]
+    opts = tval3_amd.default_options(maxit=30, ls_exact=1, isreal=0 if kind != "f64" else 1)
+    U0, i0, _ = ctx.solve(plan, bb, (32, 32, 1), opts, f64=f64)
+    ctx.profile(2)
+    U1, i1, _ = ctx.solve(plan, bb, (32, 32, 1), opts, f64=f64)
+    d = {s: runs for s, _, runs, _ in ctx.profile_slots()}
+    ctx.profile(False)
+    assert np.array_equal(U0.cpu().numpy(), U1.cpu().numpy())
+    assert i0.total_iters == i1.total_iters
+    assert {1, 3, 10, 11, 30, 35, 37} <= set(d), sorted(d)
+    assert d[3] >= i1.total_iters
