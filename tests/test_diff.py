@@ -95,7 +95,7 @@ def _gpu_loss_grads(ctx, plan, b, mu, x, alpha, first, ckpt=5):
                       alpha=alpha, first=first)
     L = ((U.double() - ctx.tensor(x).double()) ** 2).sum()
     L.backward()
-    return float(L), bt.grad.double().cpu().numpy(), float(mt.grad), U.detach().cpu().numpy()
+    return float(L.detach()), bt.grad.double().cpu().numpy(), float(mt.grad), U.detach().cpu().numpy()
 
 
 @pytest.mark.gpu
