@@ -201,7 +201,9 @@ int tv3_ctx_profile_read(tv3_ctx *ctx, double *ms, long long *count, double *byt
  * profile_info index where one pass implements that function (1 scaleb,
  * 3 shrinkage, 10 / 11 gemv NoTrans / ConjTrans), and 30 and up for fused
  * passes with no single counterpart (vec_add, nonneg, D and Dt, slots 0 and
- * 2 of the reference, run inside them; the names list what each contains).
+ * 2 of the reference, run inside them; the names list what each contains;
+ * 38 / 39 are the products of lanes gathered out of a 64-lane batch into
+ * single-instance kernels, kept apart so that 10 / 11 time one kernel each).
  * Launches inside a captured graph (tv3_options.graph = 1) are not timed.
  * Sets *n to the number of slots that ran and fills out[0 .. min(cap, *n))
  * with them, in slot order (cap = 0 and out = NULL ask for the count);

@@ -283,7 +283,8 @@ constexpr int kProfSlots = 40;
 static_assert(sizeof(tv3_ctx::prof_ms) == kProfSlots * sizeof(double), "slot arrays");
 constexpr int kProfScaleb = 1, kProfShrink = 3, kProfGemvN = 10, kProfGemvT = 11;
 constexpr int kProfDescend = 30, kProfLineSearch = 31, kProfTau = 32, kProfGetG = 33,
-              kProfMlp = 34, kProfScalars = 35, kProfLanes = 36, kProfSetup = 37;
+              kProfMlp = 34, kProfScalars = 35, kProfLanes = 36, kProfSetup = 37,
+              kProfGemvNLanes = 38, kProfGemvTLanes = 39;
 const char *prof_slot_name(int slot);
 // event pair around one launch when profiling is on (level 1: the A / A^T
 // slots only)

@@ -47,6 +47,8 @@ const char *prof_slot_name(int slot) {
     case kProfScalars: return "per-instance scalar control (fin_*)";
     case kProfLanes: return "lane compaction of masked products (k_lane_*)";
     case kProfSetup: return "setup, copies and finish (k_pack, k_init_U, k_D, k_finish, ...)";
+    case kProfGemvNLanes: return "gemv CblasNoTrans on compacted lanes (W = 1 kernels): A x";
+    case kProfGemvTLanes: return "gemv CblasConjTrans on compacted lanes (W = 1 kernels): A^T x";
     default: return "";
   }
 }

@@ -1938,10 +1938,11 @@ class Engine {
         const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
         const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
         TV3_GO(kProfLanes, k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_n, 0, pitch_);
-        prof_begin(ctx_, kProfGemvN);
+        // own slot: slots 10 / 11 (and their byte counts) keep the 64-lane
+        // kernel's launches only
+        prof_begin(ctx_, kProfGemvNLanes);
         launch_fwd<1>(s_, Gc_, lanes_.n, bt_.max_m, bt_.kind, bt_.dtype, xc_, yc_, flags_of(S_), 0);
         prof_end(ctx_);
-        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
         TV3_GO(kProfLanes, k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_m, 1, pitch_, mask, (const vec_t *)nullptr);
         return;
       }
@@ -1975,10 +1976,9 @@ class Engine {
         const dim3 gg(std::max(1, std::min(64, (int)((bt_.max_m + kBlock - 1) / kBlock))), lanes_.n);
         const dim3 gs(std::max(1, std::min(64, (int)((bt_.max_n + kBlock - 1) / kBlock))), lanes_.n);
         TV3_GO(kProfLanes, k_lane_gather, gg, dim3(kBlock), 0, s_, G_, x, xc_, lanes_, bt_.max_m, 1, pitch_);
-        prof_begin(ctx_, kProfGemvT);
+        prof_begin(ctx_, kProfGemvTLanes);
         launch_bwd<1>(s_, Gc_, lanes_.n, bt_.max_n, bt_.kind, bt_.dtype, xc_, yc_, flags_of(S_), 0, nullptr, nullptr, 0);
         prof_end(ctx_);
-        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
         TV3_GO(kProfLanes, k_lane_scatter, gs, dim3(kBlock), 0, s_, G_, S_, yc_, y, lanes_, bt_.max_n, 0, pitch_, mask, sub);
         return;
       }
