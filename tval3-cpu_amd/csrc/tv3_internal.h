@@ -300,6 +300,9 @@ struct DenseTune {
   int rows;
   int unroll;  // 1, 2, 4, 8
   int cols;    // columns per thread of the blocked A^T product: 4 or 8
+  // no group of the batch runs the whole-row blocked kernel (1) or the
+  // row-slice kernels (1): their launches are skipped
+  int no_blk = 0, no_slice = 0;
 };
 void dense_tune_set(DenseTune t);
 DenseTune dense_tune_cur();

@@ -1275,13 +1275,16 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
   else if (Ur == 4) TV3_DBLK(TT, 4, CC); \
   else if (Ur == 2) TV3_DBLK(TT, 2, CC); \
   else TV3_DBLK(TT, 1, CC);
-    if (dtype == TV3_BF16) {
+    if (tn.no_blk) {
+      // no group of this batch takes the whole-row kernel
+    } else if (dtype == TV3_BF16) {
       if (Cc == 4) { TV3_DBLK_U(__hip_bfloat16, 4) } else { TV3_DBLK_U(__hip_bfloat16, 8) }
     } else {
       if (Cc == 4) { TV3_DBLK_U(float, 4) } else { TV3_DBLK_U(float, 8) }
     }
 #undef TV3_DBLK_U
 #undef TV3_DBLK
+    if (tn.no_slice) return;
     // groups in slice mode
     dim3 g1((max_n + 8 * kBlock - 1) / (8 * kBlock), kDenseBlkChunks, G);
 #define TV3_DBS(TT, UU) hipLaunchKernelGGL((k_dense_bwd_bs<TT, UU>), g1, dim3(kBlock), 0, s, G_dev, x, \

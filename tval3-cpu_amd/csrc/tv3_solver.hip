@@ -1791,15 +1791,24 @@ class Engine {
     // Small batches take the shape that was fastest inside the solve there.
     constexpr long long kTuneMinBytes = 256ll << 20;
     DenseTune best{1, 4, 8};
-    if (!autotune_enabled()) { dense_tune_set(best); return TV3_OK; }
+    // which blocked-A^T kernels the batch needs (per group: dblk_S > 0 takes
+    // the row-slice pair, else the whole-row kernel); set with every choice
+    bool any_slice = false, any_blk = false;
+    for (const GroupDesc &g : bt_.groups) (g.dblk_S > 0 ? any_slice : any_blk) = true;
+    auto use = [&](DenseTune t) {
+      t.no_blk = !any_blk;
+      t.no_slice = !any_slice;
+      dense_tune_set(t);
+    };
+    if (!autotune_enabled()) { use(best); return TV3_OK; }
     long long work = 0;
     for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
-    if (work * (bt_.dtype == TV3_BF16 ? 2 : 4) < kTuneMinBytes) { dense_tune_set(best); return TV3_OK; }
+    if (work * (bt_.dtype == TV3_BF16 ? 2 : 4) < kTuneMinBytes) { use(best); return TV3_OK; }
     const TuneKey key{ctx_->device, bt_.dtype, bt_.G, bt_.max_n, bt_.max_m, work, bt_.layered ? 1 : 0};
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
-      if (it != g_tune_cache.end()) { dense_tune_set(it->second); return TV3_OK; }
+      if (it != g_tune_cache.end()) { use(it->second); return TV3_OK; }
     }
     // every candidate is timed in kTuneRounds interleaved rounds and keeps
     // its fastest round
@@ -1831,7 +1840,7 @@ class Engine {
     for (int ci = 0; ci < 2; ++ci)
       for (int ui = 0; ui < 4; ++ui)
         if (tb[ci][ui] < tbest) { tbest = tb[ci][ui]; best.unroll = unr_c[ui]; best.cols = cols_c[ci]; }
-    dense_tune_set(best);
+    use(best);
     int rc = hip_check(ctx_, hipGetLastError(), "autotune launches");
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(g_tune_mu);
