@@ -118,3 +118,28 @@ def test_profile_slots_variants(ctx, kind):
     assert i0.total_iters == i1.total_iters
     assert {1, 3, 10, 11, 30, 35, 37} <= set(d), sorted(d)
     assert d[3] >= i1.total_iters
+
+
+@pytest.mark.parametrize("skip", [0, 1])
+def test_profile_bytes_single_instance(ctx, skip):
+    """A single-instance solve counts each product's algorithmic bytes on the
+    host (the A^T-skip launches on the device): every timed A launch carries
+    8 nnz + 4 (m + 1) + 4 n + 4 m bytes, and with skipping the A^T launches
+    that skipped carry none."""
+    geo, b = _ring(48, 7)
+    plan = ctx.plan_geometry(geo)
+    kw = dict(skipMulRatio=0.5, numInitIterations=3, maxRelativeChange=0.05) if skip else {}
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1, **kw)
+    ctx.profile(1)
+    ctx.solve(plan, b, (48, 48, 1), opts)
+    r = ctx.profile_read()
+    ctx.profile(False)
+    m, n, nnz = plan.m, plan.n, plan.nnz
+    per_a = 8 * nnz + 4 * (m + 1) + 4 * n + 4 * m
+    per_t = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * m
+    assert r["A"][1] > 0 and r["A"][2] == r["A"][1] * per_a
+    if skip:
+        assert 0 < r["At"][2] <= r["At"][1] * per_t and r["At"][2] % per_t == 0
+        print("A^T launches", r["At"][1], "that computed", r["At"][2] // per_t)
+    else:
+        assert r["At"][2] == r["At"][1] * per_t

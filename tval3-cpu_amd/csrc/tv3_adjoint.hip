@@ -179,6 +179,7 @@ struct tv3_tape {
   // device memory
   void *mem = nullptr;
   GroupDesc *G = nullptr;
+  GroupDesc gh{};  // host copy of *G (the instance is always live here)
   InstScal *S = nullptr;
   double *part = nullptr, *dense = nullptr;
   float *bs = nullptr, *ckp = nullptr;      // ckp: checkpoints, 4 n-vectors each
@@ -220,13 +221,13 @@ struct tv3_tape {
     prof_begin(ctx, kProfGemvN);
     launch_fwd<1>(st, G, 1, m, A->dev.kind, A->dev.dtype, x, y, S, 0);
     prof_end(ctx);
-    prof_count(ctx, G, 1, 1, S, 0, 0);
+    prof_count_host(ctx, gh, 0);
   }
   void bwdA(const float *x, float *y) {
     prof_begin(ctx, kProfGemvT);
     launch_bwd<1>(st, G, 1, n, A->dev.kind, A->dev.dtype, x, y, S, 0, nullptr, (float *)dense, 0);
     prof_end(ctx);
-    prof_count(ctx, G, 1, 1, S, 0, 1);
+    prof_count_host(ctx, gh, 1);
   }
   int host_sum(int slots, double *out) {
     std::vector<double> h((size_t)slots * kNblkMax);
@@ -335,6 +336,7 @@ int make_tape(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *o, int q, int 
   GroupDesc gd{};
   gd.q = q; gd.p = p; gd.r = 1; gd.n = t->n; gd.m = t->m;
   gd.plan = A->dev;
+  t->gh = gd;
   InstScal s{};
   s.flags = F_LIVE | F_ACTIVE;
   rc = hip_check(ctx, hipMemcpyAsync(t->G, &gd, sizeof(gd), hipMemcpyHostToDevice, t->st), "copy");

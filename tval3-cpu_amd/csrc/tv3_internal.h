@@ -259,6 +259,8 @@ struct tv3_ctx {
   long long prof_n[40] = {};
   // device: algorithmic bytes of the instances each profiled launch computed
   unsigned long long *prof_bytes = nullptr;
+  // host: the same for launches whose one instance the host knows computes
+  unsigned long long prof_hbytes[2] = {0, 0};
 };
 
 struct tv3_plan {
@@ -309,6 +311,7 @@ DenseTune dense_tune_cur();
 
 // adds the algorithmic bytes of the lanes a masked A (trans 0) / A^T (trans 1)
 // launch computed to ctx->prof_bytes[trans]; W lanes per group, Bpad lanes
+void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans);
 void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
                 const InstScal *S, uint32_t mask, int trans);
 int hip_check(tv3_ctx *ctx, hipError_t e, const char *what);

@@ -93,27 +93,33 @@ void prof_collect(tv3_ctx *ctx) {
   ctx->evkind.clear();
 }
 
+// algorithmic bytes of one instance's product
+__host__ __device__ inline unsigned long long prof_bytes_of(const GroupDesc &gd, int trans) {
+  const PlanDev &pd = gd.plan;
+  const unsigned long long m = pd.m, n = pd.n, nnz = pd.nnz;
+  // the operator once, the vectors of every layer
+  const unsigned long long vec = 4ull * gd.n + 4ull * gd.m;
+  if (pd.kind == kDense) {
+    const unsigned long long es = pd.dtype == TV3_BF16 ? 2ull
+                                  : pd.dtype == TV3_C128 ? 16ull
+                                  : (pd.dtype == TV3_F64 || pd.dtype == TV3_C64) ? 8ull : 4ull;
+    return es * m * n + vec;
+  }
+  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
+  return 8 * nnz + 4 * ((trans ? n : m) + 1) + vec;
+}
+
 __global__ void k_prof_bytes(const GroupDesc *G, int W, int Bpad, const InstScal *S,
                              uint32_t mask, int trans, unsigned long long *acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Bpad) return;
   const uint32_t f = S[i].flags;
   if (!(f & F_LIVE) || (mask && (f & mask) != mask)) return;
-  const GroupDesc &gd = G[i / W];
-  const PlanDev &pd = gd.plan;
-  const unsigned long long m = pd.m, n = pd.n, nnz = pd.nnz;
-  // the operator once, the vectors of every layer
-  const unsigned long long vec = 4ull * gd.n + 4ull * gd.m;
-  unsigned long long b;
-  if (pd.kind == kDense) {
-    const unsigned long long es = pd.dtype == TV3_BF16 ? 2ull
-                                  : pd.dtype == TV3_C128 ? 16ull
-                                  : (pd.dtype == TV3_F64 || pd.dtype == TV3_C64) ? 8ull : 4ull;
-    b = es * m * n + vec;
-  } else {  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
-    b = 8 * nnz + 4 * ((trans ? n : m) + 1) + vec;
-  }
-  atomicAdd(&acc[trans], b);
+  atomicAdd(&acc[trans], prof_bytes_of(G[i / W], trans));
+}
+
+void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans) {
+  if (ctx->prof) ctx->prof_hbytes[trans] += prof_bytes_of(g, trans);
 }
 
 void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
@@ -1951,6 +1957,7 @@ int tv3_ctx_profile(tv3_ctx *c, int enable) {
     c->prof_ms[k] = 0;
     c->prof_n[k] = 0;
   }
+  c->prof_hbytes[0] = c->prof_hbytes[1] = 0;
   if (c->prof) {
     if (!c->prof_bytes) {
       int rc = hip_check(c, hipMalloc(&c->prof_bytes, 2 * sizeof(unsigned long long)), "profile hipMalloc");
@@ -1975,8 +1982,8 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
       rc = hip_check(c, hipMemcpy(hb, c->prof_bytes, sizeof(hb), hipMemcpyDeviceToHost), "profile read");
       if (rc) return rc;
     }
-    bytes[0] = (double)hb[0];
-    bytes[1] = (double)hb[1];
+    bytes[0] = (double)(hb[0] + c->prof_hbytes[0]);
+    bytes[1] = (double)(hb[1] + c->prof_hbytes[1]);
   }
   return TV3_OK;
 }

@@ -1929,6 +1929,17 @@ class Engine {
   }
   dim3 gi() const { return dim3(bt_.Bpad); }
 
+  // algorithmic bytes of a profiled product launch: a host-driven solve of
+  // one instance launches a product only while that instance needs it, so
+  // the host adds its bytes and no counting kernel joins the timed stream.
+  // F_BWD (the A^T-skip decision) is set on the device after the host's
+  // last look, so those launches are counted on the device.
+  void count_bytes(uint32_t mask, int trans) {
+    if (!ctx_->prof) return;
+    if (bt_.Bpad == 1 && !graph_ && !(mask & F_BWD)) prof_count_host(ctx_, bt_.groups[0], trans);
+    else prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, trans);
+  }
+
   void fwd(const vec_t *x, vec_t *y, uint32_t mask) {
     // lane compaction exists only in 64-lane engines (cmode_ stays 0 else)
     if constexpr (W == 64) {
@@ -1939,7 +1950,7 @@ class Engine {
         prof_begin(ctx_, kProfGemvN);
         launch_fwd<64>(s_, G64c_, ncg_, bt_.max_m, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0);
         prof_end(ctx_);
-        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
+        count_bytes(mask, 0);
         TV3_GO(kProfLanes, k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const vec_t *)nullptr);
         return;
       }
@@ -1963,7 +1974,7 @@ class Engine {
     prof_begin(ctx_, kProfGemvN);
     launch_fwd<W>(s_, G_, bt_.G, bt_.max_m, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask);
     prof_end(ctx_);
-    prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 0);
+    count_bytes(mask, 0);
   }
   // blocked: the get_g product (dense A: row blocks of submat_mb)
   void bwd(const vec_t *x, vec_t *y, uint32_t mask, const vec_t *sub, bool blocked = false) {
@@ -1977,7 +1988,7 @@ class Engine {
         prof_begin(ctx_, kProfGemvT);
         launch_bwd<64>(s_, G64c_, ncg_, bt_.max_n, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0, nullptr, nullptr, 0);
         prof_end(ctx_);
-        prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
+        count_bytes(mask, 1);
         TV3_GO(kProfLanes, k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
         return;
       }
@@ -1999,7 +2010,7 @@ class Engine {
     prof_begin(ctx_, kProfGemvT);
     launch_bwd<W>(s_, G_, bt_.G, bt_.max_n, bt_.kind, bt_.dtype, x, y, flags_of(S_), mask, sub, scratch_, smb);
     prof_end(ctx_);
-    prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, 1);
+    count_bytes(mask, 1);
   }
 
   // instances whose published flags (last read_summary) contain all bits of
