@@ -182,10 +182,11 @@ int tv3_mem_alloc(tv3_ctx *ctx, size_t bytes, void **out);
 int tv3_mem_free(tv3_ctx *ctx, void *p);
 int tv3_mem_upload(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
 int tv3_mem_download(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
-/* Per-launch HIP-event timing of the A and A^T products issued by the solver
- * on the context stream (no reference counterpart: it replaces the
- * reference's PROFILING gemv slots 10/11, 3d/mat_mult.h:19-44).  enable
- * resets the totals.  profile_read returns, for [A, A^T], the summed
+/* Per-launch HIP-event timing of the solver's launches on the context stream
+ * (it replaces the reference's PROFILING table, whose gemv slots 10/11 are
+ * at 3d/mat_mult.h:19-44).  enable: 0 off, 1 the 64-lane or single-instance
+ * A and A^T products (slots 10/11), 2 every launch (tv3_ctx_profile_slots);
+ * each call resets the totals.  profile_read returns, for [A, A^T], the summed
  * milliseconds, the number of launches and (bytes, may be NULL) the summed
  * algorithmic bytes of the instances those launches computed: dense
  * s_A*m*n + 4n + 4m, CSR / CSC 8*nnz + 4(rows+1) + 4n + 4m, geometry the
