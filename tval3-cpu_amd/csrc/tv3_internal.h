@@ -367,6 +367,10 @@ void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                 const InstScal *S, uint32_t mask, const cdouble *sub,
                 cdouble *scratch, int submat_mb);
 
+// chunks of the split-K dense A^T x (plain products: Atb, tv3_gemv): its
+// scratch holds kDenseChunks * n FP64 partial sums per group
+constexpr int kDenseChunks = 16;
+
 // rows per block of the blocked dense product (2d/tval3_cpu_2d.cpp:366-369):
 // submat_mb MiB of elem-byte entries
 __host__ __device__ inline int dense_block_rows(int submat_mb, int n, int elem = 4) {

@@ -956,12 +956,25 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
   const int n = gd.plan.n, pm = gd.plan.m;
   const int nr = min(R, gd.m - row0);
   if (R > 1 && row0 / pm != (row0 + nr - 1) / pm) {
+    // rows one at a time, each with the lane partition of the main path
+    // below (8 consecutive entries per lane with vector rows, else one per
+    // lane), so that every row is summed in the same order for every R
     for (int r = 0; r < nr; ++r) {
       const int row = row0 + r;
       const T *Ar = (const T *)gd.plan.A + (long long)(row % pm) * n;
       const float *xr = x + gd.off_n + (long long)(row / pm) * n;
       double a = 0.0;
-      for (int j = lane; j < n; j += 64) a += (double)a_val(Ar, j) * (double)xr[j];
+      if (gd.plan.a_vec) {
+        for (int j = lane * 8; j < n; j += 512) {
+          float v[8], av[8];
+          load8(xr + j, v);
+          load8(Ar + j, av);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) a += (double)av[u] * (double)v[u];
+        }
+      } else {
+        for (int j = lane; j < n; j += 64) a += (double)a_val(Ar, j) * (double)xr[j];
+      }
       a = wave_sum(a);
       if (lane == 0) y[gd.off_m + row] = (float)a;
     }
@@ -1005,8 +1018,8 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
 
 // A^T x, split-K: chunk c of a group covers rows [c*R, min(m, (c+1)*R)); a
 // thread owns 8 consecutive columns (one 16/32-byte load per row) and writes
-// FP64 partial sums; k_dense_bwd_reduce adds the chunks in order.
-constexpr int kDenseChunks = 16;
+// FP64 partial sums; k_dense_bwd_reduce adds the chunks in order
+// (kDenseChunks chunks at most, tv3_internal.h).
 
 template <class T, int U>
 __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -148,7 +149,7 @@ constexpr const char *kPlanErr =
     : TV3_SOLVER_VARIANT == 3 ? "Argument error: tv3_solve_c128 needs a c128 dense, complex-double sparse or geometry plan"
                               : "Argument error: an f64 / double-valued or complex plan needs its own entry point "
                                 "(tv3_solve_f64, tv3_solve_c64, tv3_solve_c128)";
-constexpr const char *kGemvErr =
+[[maybe_unused]] constexpr const char *kGemvErr =
     TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_gemv_f64 needs an f64 dense, double sparse or geometry plan"
     : TV3_SOLVER_VARIANT == 2 ? "Argument error: tv3_gemv_c64 needs a c64 dense, complex-float sparse or geometry plan"
                               : "Argument error: tv3_gemv_c128 needs a c128 dense, complex-double sparse or geometry plan";
@@ -1608,6 +1609,23 @@ bool autotune_enabled() {
   return !(e && e[0] == '0');
 }
 
+// TV3_DENSE_SHAPE=R,U,C forces the dense launch shape (R in {1,2,4} rows per
+// wavefront of A x, U in {1,2,4,8} rows ahead, C in {4,8} columns per thread
+// of the blocked A^T x), so that tests can run every candidate; false when
+// unset or malformed
+bool forced_dense_shape(DenseTune &t) {
+  const char *e = std::getenv("TV3_DENSE_SHAPE");
+  if (!e || !*e) return false;
+  int r = 0, u = 0, c = 0;
+  if (std::sscanf(e, "%d,%d,%d", &r, &u, &c) != 3) return false;
+  if (!(r == 1 || r == 2 || r == 4) || !(u == 1 || u == 2 || u == 4 || u == 8) || !(c == 4 || c == 8))
+    return false;
+  t.rows = r;
+  t.unroll = u;
+  t.cols = c;
+  return true;
+}
+
 // one kernel launch of the engine; with profiling on (and outside graph
 // capture) an event pair around it adds its time to profile slot SLOT
 #define TV3_GO(SLOT, ...)                      \
@@ -1800,7 +1818,7 @@ class Engine {
       t.no_slice = !any_slice;
       dense_tune_set(t);
     };
-    if (!autotune_enabled()) { use(best); return TV3_OK; }
+    if (forced_dense_shape(best) || !autotune_enabled()) { use(best); return TV3_OK; }
     long long work = 0;
     for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
     if (work * (bt_.dtype == TV3_BF16 ? 2 : 4) < kTuneMinBytes) { use(best); return TV3_OK; }
