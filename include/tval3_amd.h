@@ -289,6 +289,19 @@ const char *tv3_io_error(void);
 /* y = A x (trans = TV3_NOTRANS) or y = A^T x (TV3_TRANS); overwrites y. */
 int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
              float *y);
+/* tv3_gemv that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as tv3_gemv; it calls no hipMalloc / hipFree, no
+ * memset and no host copy.  Its group descriptor, instance record and, for
+ * a dense A^T x, the 16 * n FP64 split-K partial sums live in one fixed
+ * device array per GPU (512 B + 8 MiB, held until the process ends), so a
+ * dense A^T x takes at most n = 65536 columns (TV3_ERR_ARG beyond that);
+ * A x of any dense plan and sparse or geometry plans of any size take no
+ * partials.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads, other contexts) race. */
+int tv3_gemv_noalloc(tv3_ctx *ctx, const tv3_plan *A, int trans,
+                     const float *x, float *y);
 /* Periodic forward differences of U (dim_x contiguous).  Uz may be NULL
  * when dim_z == 1. */
 int tv3_D(tv3_ctx *ctx, int dim_x, int dim_y, int dim_z, const float *U,

@@ -371,6 +371,12 @@ void launch_bwd(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
 // scratch holds kDenseChunks * n FP64 partial sums per group
 constexpr int kDenseChunks = 16;
 
+// The no-allocation gemv (tv3_gemv_noalloc, tv3_solver.hip) keeps its group
+// descriptor, instance record and dense partial sums in one file-scope
+// device array per GPU; noalloc_attach looks up that array's address on
+// `device` (current) once and keeps it.  tv3_ctx_create calls it.
+int noalloc_attach(int device);
+
 // rows per block of the blocked dense product (2d/tval3_cpu_2d.cpp:366-369):
 // submat_mb MiB of elem-byte entries
 __host__ __device__ inline int dense_block_rows(int submat_mb, int n, int elem = 4) {

@@ -1885,6 +1885,12 @@ int tv3_ctx_create(int device, void *stream, tv3_ctx **out) {
     }
     c->own_stream = true;
   }
+  // the no-allocation gemv's device scratch of this GPU (once per device)
+  if (tv3::noalloc_attach(device) != TV3_OK) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TV3_ERR_HIP;
+  }
   if (hipHostMalloc(&c->pinned, 1 << 16, hipHostMallocDefault) != hipSuccess) {
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -2552,9 +2552,46 @@ GroupDesc vol_desc(int q, int p, int r) {
   gd.n = q * p * r;
   return gd;
 }
+
+// ---- the no-allocation gemv (tv3_gemv_noalloc) --------------------------------
+// Its kernels take their group descriptor, instance record and, for dense
+// A^T x, the kDenseChunks * n split-K partial sums from `scratch`, one array
+// per GPU that lives as long as the process.  In host code `scratch` names a
+// host-side stand-in: only the per-device address from hipGetSymbolAddress
+// (g_noalloc_ptr) may reach a kernel or a copy.
+constexpr int kNoAllocMaxN = 65536;                 // columns of a dense A^T x
+constexpr size_t kNoAllocHead = 512;                // GroupDesc, then the instance record
+constexpr size_t kNoAllocBytes = kNoAllocHead + sizeof(double) * kDenseChunks * (size_t)kNoAllocMaxN;
+static_assert(sizeof(GroupDesc) <= 256 && sizeof(Scal) <= 256, "no-alloc head layout");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocDevices] = {};
+
+// writes the call's descriptor and record (kernel arguments, so the call
+// needs no host buffer, copy or stream synchronization)
+__global__ void k_noalloc_head(GroupDesc gd, Scal s, GroupDesc *G, Scal *S) {
+  if (threadIdx.x == 0) {
+    *G = gd;
+    *S = s;
+  }
+}
 #endif  // TV3_SOLVER_VARIANT == 0
 
 }  // namespace
+
+#if TV3_SOLVER_VARIANT == 0
+int tv3::noalloc_attach(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return TV3_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return TV3_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TV3_ERR_HIP;
+  g_noalloc_ptr[device] = p;
+  return TV3_OK;
+}
+#endif
 
 // ===========================================================================
 // C-ABI
@@ -2685,6 +2722,38 @@ int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
     launch_bwd<1>(ctx->stream, sc.G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, sc.S, 0, nullptr, (float *)sc.dense, 0);
   else
     launch_fwd<1>(ctx->stream, sc.G, 1, gd.m, A->dev.kind, A->dev.dtype, x, y, sc.S, 0);
+  return hip_check(ctx, hipGetLastError(), "gemv launch");
+}
+
+int tv3_gemv_noalloc(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
+                     float *y) {
+  if (!ctx) return TV3_ERR_ARG;
+  if (!A || !x || !y) return set_err(ctx, TV3_ERR_ARG, "Argument error: null argument");
+  if (!plan_takes_f32(A->dev))
+    return set_err(ctx, TV3_ERR_ARG, "Argument error: tv3_gemv_noalloc takes f32 / bf16 dense, float sparse or geometry plans");
+  if (A->dev.kind == kDense && trans && A->dev.n > kNoAllocMaxN)
+    return set_err(ctx, TV3_ERR_ARG,
+                   "Argument error: tv3_gemv_noalloc: a dense A^T x needs n <= 65536 columns (its fixed scratch "
+                   "holds 16 * 65536 FP64 partial sums); call tv3_gemv");
+  if (ctx->device < 0 || ctx->device >= kNoAllocDevices || !g_noalloc_ptr[ctx->device])
+    return set_err(ctx, TV3_ERR_HIP, "tv3_gemv_noalloc: the device scratch was not attached (tv3_ctx_create)");
+  int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (rc) return rc;
+  char *base = (char *)g_noalloc_ptr[ctx->device];
+  GroupDesc *G = (GroupDesc *)base;
+  Scal *S = (Scal *)(base + 256);
+  double *part = (double *)(base + kNoAllocHead);
+  GroupDesc gd = vol_desc(A->dev.n, 1, 1);
+  gd.m = A->dev.m;
+  gd.layers = 1;
+  gd.plan = A->dev;
+  Scal sc{};
+  sc.flags = F_LIVE | F_ACTIVE;
+  hipLaunchKernelGGL(k_noalloc_head, dim3(1), dim3(64), 0, ctx->stream, gd, sc, G, S);
+  if (trans)
+    launch_bwd<1>(ctx->stream, G, 1, gd.n, A->dev.kind, A->dev.dtype, x, y, flags_of(S), 0, nullptr, (float *)part, 0);
+  else
+    launch_fwd<1>(ctx->stream, G, 1, gd.m, A->dev.kind, A->dev.dtype, x, y, flags_of(S), 0);
   return hip_check(ctx, hipGetLastError(), "gemv launch");
 }
 

@@ -70,6 +70,7 @@ def lib():
         "tv3_plan_shape": (_I, [_VP, C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL)]),
         "tv3_plan_walk": (_I, [_VP, _VP, _VP, _VP]),
         "tv3_gemv": (_I, [_VP, _VP, _I, _VP, _VP]),
+        "tv3_gemv_noalloc": (_I, [_VP, _VP, _I, _VP, _VP]),
         "tv3_D": (_I, [_VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
         "tv3_Dt": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _VP]),
         "tv3_shrink": (_I, [_VP, _I, _I, _I, C.c_double, _I, _I] + [_VP] * 9
