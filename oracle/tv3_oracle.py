@@ -71,6 +71,8 @@ def lib():
         L.tv3o_shrink_c64.restype = C.c_double
         L.tv3o_shrink_c64.argtypes = [C.c_int] * 3 + [C.c_double, C.c_int] + [_FP] * 9
         L.tv3o_get_threads.restype = C.c_int
+        L.tv3o_plant.argtypes = [C.c_int]
+        L.tv3o_counts.argtypes = [_LP, _LP]
         _lib = L
     return _lib
 
@@ -99,6 +101,23 @@ def set_sum_chunk(c):
     """Chunk length of the FP64 reductions (default 4096): only the
     summation order of dots and norms changes."""
     lib().tv3o_set_sum_chunk(int(c))
+
+
+def plant(k):
+    """Plants oracle mistake k (0: none); for the tests that show each one
+    fails a pinning check."""
+    lib().tv3o_plant(int(k))
+
+
+def reset_counts():
+    lib().tv3o_reset_counts()
+
+
+def counts():
+    """(A x, A^T x) products the oracle's solver issued since reset_counts()."""
+    f, b = C.c_longlong(0), C.c_longlong(0)
+    lib().tv3o_counts(C.byref(f), C.byref(b))
+    return f.value, b.value
 
 
 class Op:

@@ -72,10 +72,18 @@
  *   layers > 1       bit-identical to the single-layer solve with the
  *                    block-diagonal operator I (x) A (dense and sparse), and
  *                    independent of submat_mb (two plain products, 2d:365)
- * Parity unpinned: the A^T-skipping state machine of get_g_mult
- * (2d/tval3_cpu_2d.cpp:300-357, skipMulRatio > 0).  No closed form or
- * invariant fixes its skip decisions; it follows the reference's control
- * flow line by line and is checked only as the GPU path's reference.
+ *   3-D steps (z terms of get_tau, lam2 / lam4, shrinkage, Dt, update_mlp)
+ *                    the same FP64 model of the 3-D augmented Lagrangian
+ *                    with the exact Dt (dt_exact = 1): first-step line
+ *                    minimiser, BB step, Armijo acceptance, f = L, the
+ *                    multiplier step and the 5-strike fallback
+ *   A^T skipping (get_g_mult, 2d/tval3_cpu_2d.cpp:300-357)
+ *                    maxRelativeChange = 0 gives the skipMulRatio = 0 solve
+ *                    bit for bit; maxRelativeChange = +inf gives the A^T
+ *                    product count of the counter schedule (periodic
+ *                    multiply / skip patterns for ratios 1/2, 1/4 and 2, a
+ *                    reset at every forced product)
+ *   planted mistakes (tv3o_plant) each fail one of these checks
  */
 #include "../include/tval3_amd.h"
 
@@ -97,6 +105,18 @@ typedef long long i64;
 // chunk length of the FP64 reductions (4096; tests change it only to measure
 // how much a different summation order moves a double/double solve)
 i64 g_chunk = 4096;
+// Planted mistakes (tv3o_plant; 0 = none, the only setting outside the
+// tests that show each one fails a pinning check): 1 first-step tau without
+// the z part of |Dd|^2, 2 lam2 without |Vz|^2, 3 lam4 without <sz, Vz>,
+// 4 no z multiplier step, 5 exact Dt without its z terms, 6 BB step without
+// (mu/beta) dg, 7 Armijo without its slope term, 8 no gam decay in the
+// 5-strike fallback, 9 no U reset in the fallback, 10 the skip counter not
+// raised by skipMulRatio, 11 forced products not resetting the skip
+// counter, 12 lam1 without the z shrinkage, 13 delta step with the wrong
+// sign, 14 first-step tau without the (mu/beta) |Ad|^2 term.
+int g_plant = 0;
+// products A x / A^T x issued by the solver since tv3o_reset_counts
+i64 g_nfwd = 0, g_nbwd = 0;
 
 /* ---------------------------------------------------------------------- */
 /* reductions                                                              */
@@ -625,7 +645,8 @@ void Dtop(const Dims &d, bool exact, const T *X, const T *Y, const T *Z,
         t = t + Y[i + (p - 1) * q];
         t = t - Y[i];
       }
-      if (i >= s) {
+      if (g_plant == 5 && exact) {
+      } else if (i >= s) {
         t = t + Z[i - s];
         t = t - Z[i];
       } else {
@@ -662,7 +683,7 @@ double shrink(const Dims &d, TS beta, bool iso, const T *Ux, const T *Uy,
         TS wz = std::abs((TS)ubz) - ib;
         wz = (wz >= 0) ? wz : (TS)0;
         Wz[i] = (T)((ubz >= 0) ? wz : -wz);
-        t = t + wz;
+        if (g_plant != 12) t = t + wz;
       }
       return (double)t;
     }
@@ -820,10 +841,10 @@ struct Solver {
     if (z3) vec_sub(n, Uz.data(), Wz.data(), Vz.data());
     lam2 = (TS)sdot(n, Vx.data(), Vx.data());
     lam2 += (TS)sdot(n, Vy.data(), Vy.data());
-    if (z3) lam2 += (TS)sdot(n, Vz.data(), Vz.data());
+    if (z3 && g_plant != 2) lam2 += (TS)sdot(n, Vz.data(), Vz.data());
     lam4 = sdot(n, sx.data(), Vx.data());
     lam4 += sdot(n, sy.data(), Vy.data());
-    if (z3) lam4 += sdot(n, sz.data(), Vz.data());
+    if (z3 && g_plant != 3) lam4 += sdot(n, sz.data(), Vz.data());
   }
   void lam35(const V &b) {
     V Aub(m);
@@ -840,6 +861,7 @@ struct Solver {
    * cscmm computes the same products; each output is an FP64 sum rounded
    * once, as for one layer) */
   void gemvL(bool trans, const TC *x, TC *y) {
+    ++(trans ? g_nbwd : g_nfwd);
     const i64 in = trans ? A.m : A.n, out = trans ? A.n : A.m;
     for (int l = 0; l < layers; ++l) gemv(A, trans, x + l * in, y + l * out);
   }
@@ -850,6 +872,8 @@ struct Solver {
   void get_g_mult(const V &U, bool force_mul) {
     if (A.kind == 0) {
       if (layers == 1) {
+        ++g_nfwd;
+        ++g_nbwd;
         dense_AtA_blocked(A, U.data(), Au.data(), g.data(), o.submat_mb > 0 ? o.submat_mb : 5);
       } else {
         gemvL(false, U.data(), Au.data());
@@ -869,13 +893,14 @@ struct Solver {
     const TS relChange = nrm_deltaAu / nrm_Au;
     const TS ratio = (TS)o.skipMulRatio, maxrel = (TS)o.maxRelativeChange;
     if (force_mul) {
-      last_Au = Au; gemvL(true, Au.data(), g.data()); currSkipValue = 0;
+      last_Au = Au; gemvL(true, Au.data(), g.data());
+      if (g_plant != 11) currSkipValue = 0;
     } else if (currInitIteration < o.numInitIterations) {
       last_Au = Au; gemvL(true, Au.data(), g.data());
       currInitIteration++; currSkipValue = 0;
     } else if (currSkipValue < 1) {
       last_Au = Au; gemvL(true, Au.data(), g.data());
-      currSkipValue += ratio;
+      if (g_plant != 10) currSkipValue += ratio;
     } else if (relChange > maxrel) {
       last_Au = Au; gemvL(true, Au.data(), g.data());
     } else {
@@ -904,18 +929,18 @@ struct Solver {
       V dx(n), dy(n), dz(z3 ? n : 0);
       D_(d, dx, dy, dz);
       TR dDd = sdot(n, dx.data(), dx.data()) + sdot(n, dy.data(), dy.data());
-      if (z3) dDd = dDd + sdot(n, dz.data(), dz.data());
+      if (z3 && g_plant != 1) dDd = dDd + sdot(n, dz.data(), dz.data());
       const TR dd = sdot(n, d.data(), d.data());
       V Ad(m);
       gemvL(false, d.data(), Ad.data());
-      const TR Add = sdot(m, Ad.data(), Ad.data());
+      const TR Add = g_plant == 14 ? (TR)0 : sdot(m, Ad.data(), Ad.data());
       tau = std::abs((TS)dd / ((TS)dDd + muDbeta * (TS)Add));
     } else {
       vec_sub(n, g.data(), gp.data(), dg.data());
       vec_sub(n, g2.data(), g2p.data(), dg2.data());
       const TR ss = sdot(n, uup.data(), uup.data());
       V tmp(dg2);
-      axpy(n, (TR)muDbeta, dg.data(), tmp.data());
+      if (g_plant != 6) axpy(n, (TR)muDbeta, dg.data(), tmp.data());
       const TR sy_ = sdot(n, uup.data(), tmp.data());
       if constexpr (kCplx) {
         // abs(ss / sy) of complex values (ss real-valued): |ss| / |sy| in
@@ -979,7 +1004,7 @@ struct Solver {
     descend(U, b, tau, false);
     TS alpha = 1;
     vec_sub(n, U.data(), Up.data(), dU.data());
-    const TS c_armij = (TS)sdot(n, d.data(), d.data()) * tau * o_c * beta;
+    const TS c_armij = g_plant == 7 ? (TS)0 : (TS)sdot(n, d.data(), d.data()) * tau * o_c * beta;
     bt_cnt = 0; bt_fallback = 0;
     if ((TS)std::abs(f) > std::abs((TS)C - alpha * c_armij)) {
       vec_sub(n, g.data(), gp.data(), dg.data());
@@ -991,9 +1016,9 @@ struct Solver {
       int cnt = 0;
       while ((TS)std::abs(f) > std::abs((TS)C - alpha * c_armij)) {
         if (cnt == 5) {
-          gam *= o_rate_gam;
+          if (g_plant != 8) gam *= o_rate_gam;
           tau = get_tau(true);
-          U = Up;
+          if (g_plant != 9) U = Up;
           descend(U, b, tau, true);
           bt_fallback = 1;
           break;
@@ -1052,13 +1077,13 @@ struct Solver {
     const TR nb = (TR)(-1 * beta);
     axpy(n, nb, Vx.data(), sx.data());
     axpy(n, nb, Vy.data(), sy.data());
-    if (z3) axpy(n, nb, Vz.data(), sz.data());
+    if (z3 && g_plant != 4) axpy(n, nb, Vz.data(), sz.data());
     lam4 = sdot(n, sx.data(), Vx.data());
     lam4 += sdot(n, sy.data(), Vy.data());
     if (z3) lam4 += sdot(n, sz.data(), Vz.data());
     V Aub(m);
     vec_sub(m, Au.data(), b.data(), Aub.data());
-    axpy(m, (TR)(-1 * mu), Aub.data(), delta.data());
+    axpy(m, (TR)(g_plant == 13 ? mu : -1 * mu), Aub.data(), delta.data());
     lam5 = sdot(m, delta.data(), Aub.data());
     f = f - (lam4 + lam5);
   }
@@ -1166,6 +1191,17 @@ void tv3o_set_threads(int t) {
 /* Chunk length of the FP64 reductions (default 4096; >= 1).  Changing it
  * changes only the summation order of dot products and norms. */
 void tv3o_set_sum_chunk(long long c) { g_chunk = c >= 1 ? c : 1; }
+
+/* Planted mistake k (see g_plant; 0 restores the faithful oracle). */
+void tv3o_plant(int k) { g_plant = k; }
+
+/* A x and A^T x products the solver issued since the last reset (the blocked
+ * dense A^T A of get_g counts one of each). */
+void tv3o_reset_counts(void) { g_nfwd = g_nbwd = 0; }
+void tv3o_counts(long long *fwd, long long *bwd) {
+  if (fwd) *fwd = g_nfwd;
+  if (bwd) *bwd = g_nbwd;
+}
 
 int tv3o_get_threads(void) {
 #ifdef _OPENMP

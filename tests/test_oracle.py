@@ -704,3 +704,24 @@ def test_multi_mp_rows_are_per_position_operators(skip, extra):
         geo.xr[i * K:(i + 1) * K], geo.yr[i * K:(i + 1) * K], geo.zr[i * K:(i + 1) * K], geo.rv,
         npaths)).gemv(yb[i * npaths:(i + 1) * npaths], trans=True).astype(np.float64) for i in range(P))
     np.testing.assert_allclose(At, ref, rtol=1e-6, atol=1e-5)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_line_search_modes_agree_on_ring256(seed):
+    """The 256x256 ring workload (256 transducers, maxit 200) in FP32: the
+    quadratic-model line search (ls_exact = 0, the engine default) and the
+    reference's round-by-round backtracking (ls_exact = 1) give the same
+    reconstruction bit for bit and the same trajectory."""
+    geo = synth.ring_geometry(256, 256)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(256, seed=0)
+    b = synth.add_noise(op.gemv(x), 0.005, seed=seed)
+    res = []
+    for ls in (0, 1):
+        opts = default_options(maxit=200, ls_exact=ls)
+        U, info, tr = O.solve(op, b, (256, 256, 1), opts, trace_cap=200)
+        res.append((U, info, [(t.backtracks, t.fallback) for t in tr]))
+    (U0, i0, t0), (U1, i1, t1) = res
+    assert t0 == t1 and i0.total_iters == i1.total_iters
+    assert sum(fb for _, fb in t0) > 0
+    np.testing.assert_array_equal(U0.view(np.uint32), U1.view(np.uint32))
