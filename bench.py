@@ -30,12 +30,19 @@ Workloads (--workload):
 value = reconstructions per second of the timed steps (inputs already in HBM).
 The CPU oracle (the C++/OpenMP restatement of the reference) is timed on the
 host cores on a bounded sample of the same workload.  The roofline object
-reports the dominant measurement-operator kernel: algorithmic bytes per launch
-(CSR-equivalent for the projector: 4-byte index + 4-byte value per entry, row
-pointers, x and y; dense: the stored A, x and y) over its mean launch time,
-taken live with HIP events on the solver's stream.  "traffic" is the HBM bytes
-per launch measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE, calibrated on a
-copy kernel) when a committed profile summary for the workload exists.
+reports the dominant measurement-operator kernel: its compulsory bytes per
+launch over its mean launch time, taken live with HIP events on the solver's
+stream.  Compulsory bytes: the operator once for every group of instances
+that computed (dense: the stored A; CSR / CSC: indices, values and row
+pointers; the on-the-fly projector: its 32-byte ray descriptors for A x, its
+voxel-major ray lists for A^T) plus the input and output vectors of every
+instance that computed.  The per-instance CSR-equivalent rate (8 bytes per
+entry for every instance, as if each instance streamed its own CSR matrix)
+and, for the projector, the rate of the 4-byte gathers per entry and
+instance (served by the caches) are reported beside it, labelled.  "traffic"
+is the HBM bytes per launch measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE,
+calibrated on a copy kernel) when a committed profile summary for the
+workload exists.
 
 Only one GPU is used; --gpus above 1 exits with a message.
 """
@@ -55,10 +62,6 @@ HBM_PEAK_GBPS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def csr_equiv_bytes(nnz, m, n):
-    return 8 * nnz + 4 * (m + 1) + 4 * n + 4 * m
 
 
 def traffic_from_profiles(workload, kernel):
@@ -85,24 +88,46 @@ def cpu_time(fn, budget_s, max_runs=20):
             return runs, el, O.get_threads()
 
 
-def roofline(prof, kernels, workload):
-    """Per-operator figures from the library's launch profile: each launch is
-    charged the algorithmic bytes of the instances it computed (masked-off
-    instances of a batch read nothing), so achieved = bytes / time."""
+def roofline(prof, kernels, workload, plan=None, t_steps=None, steps=None):
+    """Per-operator figures from the library's launch profile.  Each launch
+    is charged its compulsory bytes (the operator once per computing group,
+    the vectors of each computing instance), so achieved = bytes / time is a
+    physical rate.  With a plan, the per-instance CSR-equivalent rate and,
+    for the projector, the cache-served gather rate are added (labelled).
+    With t_steps, the compulsory bytes of all products per step over the
+    step's wall time are checked against the HBM peak."""
     ops = {}
-    for k, (ms, cnt, by) in prof.items():
+    for k, (ms, cnt, by, lanes) in prof.items():
         ops[k] = {"launches": cnt, "ms_per_launch": ms / max(cnt, 1),
                   "bytes_per_launch": by / max(cnt, 1),
+                  "instances_per_launch": lanes / max(cnt, 1),
                   "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None}
+        if plan is not None and ms > 0 and workload in ("ring", "hemisphere", "diff", "ring-f64"):
+            rows = plan.m if k == "A" else plan.n
+            csr = 8 * plan.nnz + 4 * (rows + 1) + 4 * (plan.m + plan.n)
+            ops[k]["csr_equivalent_GBps_per_instance_operator"] = lanes * csr / (ms * 1e-3) / 1e9
+            es = 8 if workload == "ring-f64" else 4
+            ops[k]["cache_gather_GBps"] = lanes * es * plan.nnz / (ms * 1e-3) / 1e9
     dom = max(ops, key=lambda k: prof[k][0])
     ach = ops[dom]["GBps"]
     kern = kernels[dom]
-    return ops, {"bound": "hbm", "kernel": kern, "achieved": ach,
-                 "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                 "frac": ach / HBM_PEAK_GBPS if ach else None,
-                 "traffic": traffic_from_profiles(workload, kern),
-                 "bytes_per_launch": ops[dom]["bytes_per_launch"],
-                 "ms_per_launch": ops[dom]["ms_per_launch"]}
+    traffic = traffic_from_profiles(workload, kern)
+    roof = {"bound": "hbm", "kernel": kern, "achieved": ach,
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBPS if ach else None,
+            "traffic": traffic,
+            "bytes_per_launch": ops[dom]["bytes_per_launch"],
+            "ms_per_launch": ops[dom]["ms_per_launch"],
+            "traffic_over_bytes": (traffic / ops[dom]["bytes_per_launch"]
+                                   if traffic and ops[dom]["bytes_per_launch"] else None)}
+    if t_steps:
+        tot = sum(prof[k][2] for k in prof) / steps
+        rate = tot / (t_steps / steps) / 1e9
+        roof["products_GBps_over_wall"] = rate
+        if rate > HBM_PEAK_GBPS:
+            raise RuntimeError(f"compulsory product bytes per step over the step's wall time "
+                               f"({rate:.0f} GB/s) exceed the HBM peak: byte accounting is wrong")
+    return ops, roof
 
 
 # ---------------------------------------------------------------------------
@@ -152,33 +177,20 @@ def run_ring(args, ctx):
     value = B * args.steps / t_steps
     rel_b0 = float(torch.linalg.norm(Us[0] - U0) / torch.linalg.norm(U0))
     iters = [inf.total_iters for inf in infos]
-    ops, roof = roofline(prof, {"A": "k_fwd_w64q", "At": "k_bwd_w64q"}, "ring")
-    # the projector walks its indices on the fly: its bytes are the CSR
-    # equivalent (8 B per entry), and x / y stay cache-resident
-    roof["cache_resident"] = True
-    roof["csr_equivalent"] = True
-    roof["voxel_visits_per_s"] = (ops["A"]["bytes_per_launch"] / csr_equiv_bytes(
-        plan.nnz, plan.m, plan.n) * plan.nnz / (ops["A"]["ms_per_launch"] * 1e-3))
+    ops, roof = roofline(prof, {"A": "k_fwd_w64q", "At": "k_bwd_w64q"}, "ring", plan,
+                         t_steps, args.steps)
+    # the walk's gathers (one 256-byte row of 64 instances per entry) are
+    # served by L2 and the last-level cache
+    roof["cache_gather_GBps"] = ops["A"]["cache_gather_GBps"]
+    roof["voxel_visits_per_s"] = (ops["A"]["instances_per_launch"] * plan.nnz
+                                  / (ops["A"]["ms_per_launch"] * 1e-3))
     import numpy as np
     import tv3_oracle as O
     op = O.Op.geometry(geo)
     fp64 = None
     if not args.no_fp64:
-        U64, _, _ = O.solve_f64(op, bs_host[0], dims, opts)
-        U32, _, _ = O.solve(op, bs_host[0], dims, opts)
-        nrm = np.linalg.norm(U64)
-        fp64 = {"instance": 0,
-                "batched_vs_fp64": float(np.linalg.norm(Us[0].cpu().numpy() - U64) / nrm),
-                "plain_vs_fp64": float(np.linalg.norm(U0.cpu().numpy() - U64) / nrm),
-                "oracle_fp32_vs_fp64": float(np.linalg.norm(U32 - U64) / nrm),
-                "batched_vs_oracle_fp32": float(np.linalg.norm(Us[0].cpu().numpy() - U32)
-                                                / np.linalg.norm(U32))}
-        # the FP64 solve's own sensitivity: b perturbed by 1e-12 relative
-        rng = np.random.default_rng(5)
-        bp = bs_host[0].astype(np.float64) * (1 + 1e-12 * rng.standard_normal(plan.m))
-        Up, _, _ = O.solve_f64(op, bp, dims, opts)
-        fp64["fp64_vs_fp64_b_perturbed_1e-12"] = float(np.linalg.norm(Up - U64) / nrm)
-        log(f"fp64: {fp64}")
+        fp64 = ring_precision(args, ctx, plan, op, bs_host, Us, dims, opts)
+        log(f"fp64: {json.dumps(fp64['summary'])}")
     cpu = None
     if not args.no_cpu:
         runs, el, thr = cpu_time(lambda: O.solve(op, bs_host[0], dims, opts), args.cpu_seconds)
@@ -198,10 +210,81 @@ def run_ring(args, ctx):
         "batched_over_plain": value * t_plain,
         "batch_vs_plain_rel_l2": rel_b0,
         "iters_min_max": [min(iters), max(iters)],
-        "rel_l2_vs_fp64": fp64,
+        "measured": ("batched FP32 reconstructions (FP32 vectors, T_scal float) with the "
+                     "on-the-fly weight-1 projector; bf16 storage applies to dense A "
+                     "(--workload dense-batch); error vs FP64 over the sample in "
+                     "precision_vs_fp64"),
+        "precision_vs_fp64": fp64,
         "operators": ops, "roofline": roof, "cpu_baseline": cpu,
     }
     return out
+
+
+def ring_precision(args, ctx, plan, op, bs_host, Us, dims, opts):
+    """FP32 batched results against the FP64 solve (tv3_solve_batch_f64 on
+    the GPU) over a sample of instances, and, for each, the first iteration
+    whose line-search decision (backtracks, fallback) differs between the
+    FP32 and FP64 trajectories, with the two iterates' distance one
+    iteration before it.  Instance 0 is also checked against the CPU oracle
+    (the FP32 reference arithmetic) bit for bit."""
+    import numpy as np
+    import torch
+    K = min(args.fp64_sample_ring, len(bs_host))
+    b64 = [ctx.tensor(bs_host[i], torch.float64) for i in range(K)]
+    U64, _ = ctx.solve_batch([plan] * K, b64, [dims] * K, opts, f64=True)
+    per = []
+    for i in range(K):
+        u32 = Us[i].cpu().numpy().astype(np.float64)
+        u64 = U64[i].cpu().numpy()
+        nrm = np.linalg.norm(u64)
+        _, _, t32 = ctx.solve(plan, ctx.tensor(bs_host[i]), dims, opts, trace_cap=opts.maxit)
+        _, _, t64 = ctx.solve(plan, b64[i], dims, opts, trace_cap=opts.maxit, f64=True)
+        k = next((a.iter for a, c in zip(t32, t64)
+                  if (a.backtracks, a.fallback) != (c.backtracks, c.fallback)), None)
+        rec = {"instance": i, "rel_l2": float(np.linalg.norm(u32 - u64) / nrm),
+               "fallbacks_fp32": sum(t.fallback for t in t32),
+               "fallbacks_fp64": sum(t.fallback for t in t64),
+               "first_decision_diff_iter": k}
+        if k is not None and k > 1:
+            o2 = tval3_amd_options(opts, maxit=k - 1)
+            a32, _, _ = ctx.solve(plan, ctx.tensor(bs_host[i]), dims, o2)
+            a64, _, _ = ctx.solve(plan, b64[i], dims, o2, f64=True)
+            a64 = a64.cpu().numpy()
+            rec["rel_l2_one_iter_before_diff"] = float(
+                np.linalg.norm(a32.cpu().numpy().astype(np.float64) - a64) / np.linalg.norm(a64))
+        per.append(rec)
+    rels = np.array([r["rel_l2"] for r in per])
+    before = [r["rel_l2_one_iter_before_diff"] for r in per if "rel_l2_one_iter_before_diff" in r]
+    # the FP64 solve's own sensitivity: b perturbed by 1e-12 relative
+    rng = np.random.default_rng(5)
+    bp = ctx.tensor(bs_host[0].astype(np.float64) * (1 + 1e-12 * rng.standard_normal(plan.m)),
+                    torch.float64)
+    Up, _, _ = ctx.solve(plan, bp, dims, opts, f64=True)
+    u640 = U64[0].cpu().numpy()
+    U32o, _, _ = O_solve_oracle(op, bs_host[0], dims, opts)
+    summary = {"instances": K,
+               "fp32_vs_fp64_rel_l2_min_median_max": [float(rels.min()), float(np.median(rels)),
+                                                      float(rels.max())],
+               "instances_within_1e-3": int((rels <= 1e-3).sum()),
+               "rel_l2_one_iter_before_first_decision_diff_max": (float(max(before))
+                                                                  if before else None),
+               "fp64_vs_fp64_b_perturbed_1e-12": float(np.linalg.norm(
+                   Up.cpu().numpy() - u640) / np.linalg.norm(u640)),
+               "batched_vs_oracle_fp32_instance0": float(np.linalg.norm(
+                   Us[0].cpu().numpy() - U32o) / np.linalg.norm(U32o))}
+    return {"summary": summary, "per_instance": per}
+
+
+def tval3_amd_options(opts, **kw):
+    import tval3_amd
+    d = opts.as_dict()
+    d.update(kw)
+    return tval3_amd.default_options(**d)
+
+
+def O_solve_oracle(op, b, dims, opts):
+    import tv3_oracle as O
+    return O.solve(op, b, dims, opts)
 
 
 def run_dense_batch(args, ctx):
@@ -250,28 +333,47 @@ def run_dense_batch(args, ctx):
     args.one_step = lambda: ctx.solve_batch(plans, bs, dims, opts)
     value = B * args.steps / t_steps
     ops, roof = roofline(prof, {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_blk"},
-                         "dense-batch")
-    # error against the FP64 solve of the same bf16-rounded matrices, on the
-    # smallest instances (CPU oracle, double/double), beside the FP32 oracle's
-    # own distance to that FP64 solve
+                         "dense-batch", None, t_steps, args.steps)
+    # every instance against the FP64 solve of the same bf16-rounded
+    # matrices (tv3_solve_batch_f64 on the GPU, in chunks of FP64 copies of
+    # A), and the smallest ones against the CPU oracle's FP32 solve (the
+    # reference arithmetic, bit for bit)
     import tv3_oracle as O
-    order = sorted(range(B), key=lambda k: sizes[k][0] ** 2 * sizes[k][1])[:args.f64_sample]
     errs = {}
-    for k in order:
-        Ab, x = keep[k]
-        op = O.Op.dense(Ab.float().cpu().numpy())
-        bk = bs[k].cpu().numpy()
-        U64, _, _ = O.solve_f64(op, bk, dims[k], opts)
-        U32, _, _ = O.solve(op, bk, dims[k], opts)
-        Ug = Us[k].cpu().numpy().astype(np.float64)
-        nrm = np.linalg.norm(U64)
-        errs[k] = {"gpu_vs_fp64": float(np.linalg.norm(Ug - U64) / nrm),
-                   "oracle_fp32_vs_fp64": float(np.linalg.norm(U32 - U64) / nrm),
-                   "gpu_vs_oracle_fp32": float(np.linalg.norm(Ug - U32) / np.linalg.norm(U32)),
-                   "grid": sizes[k][0], "m": sizes[k][1]}
-    log("rel L2 vs FP64: " + ", ".join(f"{k}: {v['gpu_vs_fp64']:.2e} "
-                                      f"(oracle {v['oracle_fp32_vs_fp64']:.2e})"
-                                      for k, v in errs.items()))
+    if not args.no_fp64:
+        chunk, used = [], 0.0
+        for k in list(range(B)) + [None]:
+            if k is not None:
+                g, m = sizes[k]
+                need = 8.0 * m * g * g / 2 ** 30
+            if chunk and (k is None or used + need > args.fp64_chunk_gib):
+                p64 = [ctx.plan_dense(keep[j][0].to(torch.float64), f64=True) for j in chunk]
+                b64 = [bs[j].to(torch.float64) for j in chunk]
+                U64, _ = ctx.solve_batch(p64, b64, [dims[j] for j in chunk], opts, f64=True)
+                for j, u in zip(chunk, U64):
+                    u = u.cpu().numpy()
+                    errs[j] = {"grid": sizes[j][0], "m": sizes[j][1], "gpu_vs_fp64": float(
+                        np.linalg.norm(Us[j].cpu().numpy().astype(np.float64) - u) / np.linalg.norm(u))}
+                del p64, b64, U64
+                torch.cuda.empty_cache()
+                chunk, used = [], 0.0
+            if k is not None:
+                chunk.append(k)
+                used += need
+    order = sorted(range(B), key=lambda k: sizes[k][0] ** 2 * sizes[k][1])
+    for k in order[:args.f64_sample]:
+        op = O.Op.dense(keep[k][0].float().cpu().numpy())
+        U32, _, _ = O.solve(op, bs[k].cpu().numpy(), dims[k], opts)
+        errs.setdefault(k, {"grid": sizes[k][0], "m": sizes[k][1]})["gpu_vs_oracle_fp32"] = float(
+            np.linalg.norm(Us[k].cpu().numpy() - U32) / np.linalg.norm(U32))
+    e64 = np.array([v["gpu_vs_fp64"] for v in errs.values() if "gpu_vs_fp64" in v])
+    prec = {"instances_vs_fp64": int(e64.size),
+            "rel_l2_min_median_max": ([float(e64.min()), float(np.median(e64)), float(e64.max())]
+                                      if e64.size else None),
+            "instances_within_1e-3": int((e64 <= 1e-3).sum()),
+            "instances_within_1e-2": int((e64 <= 1e-2).sum()),
+            "per_instance": errs}
+    log(f"rel L2 vs FP64 over {e64.size} instances: {prec['rel_l2_min_median_max']}")
     cpu = None
     if not args.no_cpu:
         k = order[0]
@@ -293,7 +395,7 @@ def run_dense_batch(args, ctx):
         "data": "synthetic: A ~ N(0,1/m) rounded to bf16, 8-ellipse phantoms, 1% noise",
         "config": {"workload": f"{B} uneven dense 2D reconstructions (bf16 A), maxit {args.maxit}",
                    "batch": B, "bf16_entries": tot_elems, "maxit": args.maxit},
-        "rel_l2_vs_fp64": errs, "iters": [i.total_iters for i in infos],
+        "precision_vs_fp64": prec, "iters": [i.total_iters for i in infos],
         "operators": ops, "roofline": roof, "cpu_baseline": cpu,
     }
 
@@ -335,9 +437,9 @@ def run_diff(args, ctx):
     prof = ctx.profile_read()
     ctx.profile(False)
     args.one_step = grad_once
-    ops, roof = roofline(prof, {"A": "k_fwd_w1", "At": "k_bwd_w1"}, "diff")
+    ops, roof = roofline(prof, {"A": "k_fwd_w1", "At": "k_bwd_w1"}, "diff", plan,
+                         t_steps, args.steps)
     roof["cache_resident"] = True
-    roof["csr_equivalent"] = True
     cpu = None
     if not args.no_cpu:
         import diff_ref as R
@@ -447,11 +549,9 @@ def run_single(args, ctx, workload):
     prof = ctx.profile_read()
     ctx.profile(False)
     args.one_step = lambda: ctx.solve(plan, bt, dims, opts, Ut=x, f64=f64)
-    ops, roof = roofline(prof, kern, workload)
-    if workload in ("hemisphere", "ring-f64"):
-        roof["csr_equivalent"] = True
-    else:
-        roof["cache_resident"] = True  # 20 MB A stays in the 256 MB last-level cache
+    ops, roof = roofline(prof, kern, workload, plan, t_steps, args.steps)
+    if workload != "hemisphere":
+        roof["cache_resident"] = True  # operator and vectors fit the 256 MB last-level cache
     cpu = None
     if not args.no_cpu:
         op = op_fn()
@@ -487,7 +587,12 @@ def main():
                     help="skip the FP64 oracle solve used for the error report")
     ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--maxit", type=int, default=None)
-    ap.add_argument("--f64-sample", type=int, default=4)
+    ap.add_argument("--f64-sample", type=int, default=3,
+                    help="dense-batch: smallest instances also checked against the CPU oracle")
+    ap.add_argument("--fp64-sample-ring", type=int, default=16,
+                    help="ring: instances compared with their FP64 (GPU) solves")
+    ap.add_argument("--fp64-chunk-gib", type=float, default=48.0,
+                    help="dense-batch: FP64 copies of A per chunk of the FP64 batch solves")
     ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
                     help="1: replay each pair of solver steps from a hipGraph")
     ap.add_argument("--slots", default=None, metavar="FILE",

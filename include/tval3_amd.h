@@ -188,11 +188,17 @@ int tv3_mem_download(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
  * A and A^T products (slots 10/11), 2 every launch (tv3_ctx_profile_slots);
  * each call resets the totals.  profile_read returns, for [A, A^T], the summed
  * milliseconds, the number of launches and (bytes, may be NULL) the summed
- * algorithmic bytes of the instances those launches computed: dense
- * s_A*m*n + 4n + 4m, CSR / CSC 8*nnz + 4(rows+1) + 4n + 4m, geometry the
- * CSR-equivalent of its walk. */
+ * compulsory bytes of those launches: the operator once for every group of
+ * instances that computed (dense s_A*m*n; CSR / CSC (4 + s_v)*nnz +
+ * 4(rows+1); geometry A x the 32-byte ray descriptors (and the 4-byte walk
+ * order of the 64-lane kernel), A^T the 4*nnz + 8n + 4 bytes of the
+ * voxel-major lists) plus s_vec*(n + m) of vectors for every instance that
+ * computed.  profile_lanes returns, for [A, A^T], the number of instance
+ * products those launches computed (a 64-lane launch with 40 computing
+ * lanes counts 40). */
 int tv3_ctx_profile(tv3_ctx *ctx, int enable);
 int tv3_ctx_profile_read(tv3_ctx *ctx, double *ms, long long *count, double *bytes);
+int tv3_ctx_profile_lanes(tv3_ctx *ctx, long long *lanes);
 
 /* Per-slot timing report, the GPU counterpart of the reference's PROFILING
  * table (profile_info[slot]: name, time, runs; printed at 3d/tval3_cpu_3d.cpp:

@@ -122,10 +122,12 @@ def test_profile_slots_variants(ctx, kind):
 
 @pytest.mark.parametrize("skip", [0, 1])
 def test_profile_bytes_single_instance(ctx, skip):
-    """A single-instance solve counts each product's algorithmic bytes on the
-    host (the A^T-skip launches on the device): every timed A launch carries
-    8 nnz + 4 (m + 1) + 4 n + 4 m bytes, and with skipping the A^T launches
-    that skipped carry none."""
+    """A single-instance solve counts each product's compulsory bytes on the
+    host (the A^T-skip launches on the device): every timed A launch of the
+    projector carries its 32-byte ray descriptors and x and y (32 m + 4 n +
+    4 m bytes), every A^T launch its voxel-major lists and vectors (4 nnz +
+    8 n + 4 + 4 n + 4 m), and with skipping the A^T launches that skipped
+    carry none."""
     geo, b = _ring(48, 7)
     plan = ctx.plan_geometry(geo)
     kw = dict(skipMulRatio=0.5, numInitIterations=3, maxRelativeChange=0.05) if skip else {}
@@ -135,11 +137,35 @@ def test_profile_bytes_single_instance(ctx, skip):
     r = ctx.profile_read()
     ctx.profile(False)
     m, n, nnz = plan.m, plan.n, plan.nnz
-    per_a = 8 * nnz + 4 * (m + 1) + 4 * n + 4 * m
-    per_t = 8 * nnz + 4 * (n + 1) + 4 * n + 4 * m
-    assert r["A"][1] > 0 and r["A"][2] == r["A"][1] * per_a
+    per_a = 32 * m + 4 * n + 4 * m
+    per_t = 4 * nnz + 8 * n + 4 + 4 * n + 4 * m
+    assert r["A"][1] > 0 and r["A"][2] == r["A"][1] * per_a and r["A"][3] == r["A"][1]
     if skip:
         assert 0 < r["At"][2] <= r["At"][1] * per_t and r["At"][2] % per_t == 0
+        assert r["At"][3] == r["At"][2] // per_t
         print("A^T launches", r["At"][1], "that computed", r["At"][2] // per_t)
     else:
-        assert r["At"][2] == r["At"][1] * per_t
+        assert r["At"][2] == r["At"][1] * per_t and r["At"][3] == r["At"][1]
+
+
+def test_profile_bytes_shared_batch(ctx):
+    """70 instances sharing the ring geometry (two 64-lane groups): every A
+    launch is charged the operator (ray descriptors and the 64-lane walk
+    order, 36 m bytes) once per group that computed, and x and y once per
+    computing instance, so the bytes minus the instances' vector bytes are
+    a whole number of operator copies, at most two per launch."""
+    geo, b = _ring(32, 8)
+    plan = ctx.plan_geometry(geo)
+    B = 70
+    bs = [b * (1 + 0.01 * i) for i in range(B)]
+    opts = tval3_amd.default_options(maxit=30, ls_exact=1)
+    ctx.profile(1)
+    ctx.solve_batch([plan] * B, bs, [(32, 32, 1)] * B, opts)
+    r = ctx.profile_read()
+    ctx.profile(False)
+    m, n, nnz = plan.m, plan.n, plan.nnz
+    for key, op in (("A", 36 * m), ("At", 4 * nnz + 8 * n + 4)):
+        ms, cnt, by, lanes = r[key]
+        assert cnt > 0 and 0 < lanes <= cnt * B
+        rest = by - lanes * 4 * (n + m)
+        assert rest % op == 0 and cnt <= rest // op <= 2 * cnt, (key, rest / op, cnt)

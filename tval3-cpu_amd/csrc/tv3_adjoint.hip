@@ -221,13 +221,13 @@ struct tv3_tape {
     prof_begin(ctx, kProfGemvN);
     launch_fwd<1>(st, G, 1, m, A->dev.kind, A->dev.dtype, x, y, S, 0);
     prof_end(ctx);
-    prof_count_host(ctx, gh, 0);
+    prof_count_host(ctx, gh, 0, 4);
   }
   void bwdA(const float *x, float *y) {
     prof_begin(ctx, kProfGemvT);
     launch_bwd<1>(st, G, 1, n, A->dev.kind, A->dev.dtype, x, y, S, 0, nullptr, (float *)dense, 0);
     prof_end(ctx);
-    prof_count_host(ctx, gh, 1);
+    prof_count_host(ctx, gh, 1, 4);
   }
   int host_sum(int slots, double *out) {
     std::vector<double> h((size_t)slots * kNblkMax);

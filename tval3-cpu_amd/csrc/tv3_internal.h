@@ -260,7 +260,7 @@ struct tv3_ctx {
   // device: algorithmic bytes of the instances each profiled launch computed
   unsigned long long *prof_bytes = nullptr;
   // host: the same for launches whose one instance the host knows computes
-  unsigned long long prof_hbytes[2] = {0, 0};
+  unsigned long long prof_hbytes[4] = {0, 0, 0, 0};  // bytes A, A^T; lanes A, A^T
 };
 
 struct tv3_plan {
@@ -311,9 +311,12 @@ DenseTune dense_tune_cur();
 
 // adds the algorithmic bytes of the lanes a masked A (trans 0) / A^T (trans 1)
 // launch computed to ctx->prof_bytes[trans]; W lanes per group, Bpad lanes
-void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans);
+// (host-known launches: `groups` groups of width W read the operator,
+// `lanes` lanes computed)
+void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans, int vbytes, int W = 1,
+                     int groups = 1, int lanes = 1);
 void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
-                const InstScal *S, uint32_t mask, int trans);
+                const InstScal *S, uint32_t mask, int trans, int vbytes);
 int hip_check(tv3_ctx *ctx, hipError_t e, const char *what);
 
 // operator launches (tv3_plan.hip).  x and y are batch base pointers;

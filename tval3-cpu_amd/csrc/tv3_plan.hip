@@ -93,40 +93,62 @@ void prof_collect(tv3_ctx *ctx) {
   ctx->evkind.clear();
 }
 
-// algorithmic bytes of one instance's product
-__host__ __device__ inline unsigned long long prof_bytes_of(const GroupDesc &gd, int trans) {
-  const PlanDev &pd = gd.plan;
+// Compulsory bytes of a product launch: the operator's stored or walked
+// description once for every group that computes (dense A; CSR or CSC
+// indices, values and row pointers; the projector's ray descriptors and
+// walk order, or its voxel-major transpose lists), and each computing
+// lane's input and output vectors (vbytes per element, every layer).
+__host__ __device__ inline unsigned long long prof_op_bytes(const PlanDev &pd, int trans, int W) {
   const unsigned long long m = pd.m, n = pd.n, nnz = pd.nnz;
-  // the operator once, the vectors of every layer
-  const unsigned long long vec = 4ull * gd.n + 4ull * gd.m;
   if (pd.kind == kDense) {
     const unsigned long long es = pd.dtype == TV3_BF16 ? 2ull
                                   : pd.dtype == TV3_C128 ? 16ull
                                   : (pd.dtype == TV3_F64 || pd.dtype == TV3_C64) ? 8ull : 4ull;
-    return es * m * n + vec;
+    return es * m * n;
   }
-  // CSR / CSC (geometry: the CSR-equivalent with 4-byte values)
-  return 8 * nnz + 4 * ((trans ? n : m) + 1) + vec;
+  if (pd.kind == kSparse) {
+    const unsigned long long vs = pd.cval == 3 ? 16ull : (pd.cval == 2 || pd.r_vald) ? 8ull : 4ull;
+    return (4 + vs) * nnz + 4 * ((trans ? n : m) + 1);
+  }
+  // geometry: 32-byte RayDesc per ray (and the 64-lane kernel's ray order),
+  // or t_ptr, t_col and t_ray of the transpose
+  return trans ? 4ull * nnz + 8ull * n + 4ull : 32ull * m + (W == 64 ? 4ull * m : 0ull);
+}
+__host__ __device__ inline unsigned long long prof_vec_bytes(const GroupDesc &gd, int vbytes) {
+  return (unsigned long long)vbytes * ((unsigned long long)gd.n + (unsigned long long)gd.m);
 }
 
-__global__ void k_prof_bytes(const GroupDesc *G, int W, int Bpad, const InstScal *S,
-                             uint32_t mask, int trans, unsigned long long *acc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Bpad) return;
-  const uint32_t f = S[i].flags;
-  if (!(f & F_LIVE) || (mask && (f & mask) != mask)) return;
-  atomicAdd(&acc[trans], prof_bytes_of(G[i / W], trans));
+// one thread per group: acc[trans] += compulsory bytes, acc[2 + trans] +=
+// lanes that computed
+__global__ void k_prof_bytes(const GroupDesc *G, int W, int nG, const InstScal *S,
+                             uint32_t mask, int trans, int vbytes, unsigned long long *acc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nG) return;
+  unsigned long long cnt = 0;
+  for (int l = 0; l < W; ++l) {
+    const uint32_t f = S[g * W + l].flags;
+    if ((f & F_LIVE) && (!mask || (f & mask) == mask)) ++cnt;
+  }
+  if (!cnt) return;
+  const GroupDesc &gd = G[g];
+  atomicAdd(&acc[trans], prof_op_bytes(gd.plan, trans, W) + cnt * prof_vec_bytes(gd, vbytes));
+  atomicAdd(&acc[2 + trans], cnt);
 }
 
-void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans) {
-  if (ctx->prof) ctx->prof_hbytes[trans] += prof_bytes_of(g, trans);
+void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans, int vbytes, int W, int groups,
+                     int lanes) {
+  if (!ctx->prof) return;
+  ctx->prof_hbytes[trans] += (unsigned long long)groups * prof_op_bytes(g.plan, trans, W) +
+                             (unsigned long long)lanes * prof_vec_bytes(g, vbytes);
+  ctx->prof_hbytes[2 + trans] += (unsigned long long)lanes;
 }
 
 void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
-                const InstScal *S, uint32_t mask, int trans) {
+                const InstScal *S, uint32_t mask, int trans, int vbytes) {
   if (!ctx->prof || !ctx->prof_bytes) return;
-  hipLaunchKernelGGL(k_prof_bytes, dim3((Bpad + 255) / 256), dim3(256), 0, ctx->stream,
-                     G_dev, W, Bpad, S, mask, trans, ctx->prof_bytes);
+  const int nG = Bpad / W;
+  hipLaunchKernelGGL(k_prof_bytes, dim3((nG + 255) / 256), dim3(256), 0, ctx->stream,
+                     G_dev, W, nG, S, mask, trans, vbytes, ctx->prof_bytes);
 }
 
 int hip_check(tv3_ctx *ctx, hipError_t e, const char *what) {
@@ -1976,13 +1998,13 @@ int tv3_ctx_profile(tv3_ctx *c, int enable) {
     c->prof_ms[k] = 0;
     c->prof_n[k] = 0;
   }
-  c->prof_hbytes[0] = c->prof_hbytes[1] = 0;
+  for (unsigned long long &v : c->prof_hbytes) v = 0;
   if (c->prof) {
     if (!c->prof_bytes) {
-      int rc = hip_check(c, hipMalloc(&c->prof_bytes, 2 * sizeof(unsigned long long)), "profile hipMalloc");
+      int rc = hip_check(c, hipMalloc(&c->prof_bytes, 4 * sizeof(unsigned long long)), "profile hipMalloc");
       if (rc) { c->prof = false; c->prof_bytes = nullptr; return rc; }
     }
-    int rc = hip_check(c, hipMemsetAsync(c->prof_bytes, 0, 2 * sizeof(unsigned long long), c->stream), "profile clear");
+    int rc = hip_check(c, hipMemsetAsync(c->prof_bytes, 0, 4 * sizeof(unsigned long long), c->stream), "profile clear");
     if (rc) { c->prof = false; return rc; }
   }
   return TV3_OK;
@@ -1996,7 +2018,7 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
   const int slot[2] = {kProfGemvN, kProfGemvT};
   for (int k = 0; k < 2; ++k) { ms[k] = c->prof_ms[slot[k]]; count[k] = c->prof_n[slot[k]]; }
   if (bytes) {
-    unsigned long long hb[2] = {0, 0};
+    unsigned long long hb[4] = {0, 0, 0, 0};
     if (c->prof_bytes) {
       rc = hip_check(c, hipMemcpy(hb, c->prof_bytes, sizeof(hb), hipMemcpyDeviceToHost), "profile read");
       if (rc) return rc;
@@ -2004,6 +2026,20 @@ int tv3_ctx_profile_read(tv3_ctx *c, double *ms, long long *count, double *bytes
     bytes[0] = (double)(hb[0] + c->prof_hbytes[0]);
     bytes[1] = (double)(hb[1] + c->prof_hbytes[1]);
   }
+  return TV3_OK;
+}
+
+int tv3_ctx_profile_lanes(tv3_ctx *c, long long *lanes) {
+  if (!c || !lanes) return TV3_ERR_ARG;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "profile sync");
+  if (rc) return rc;
+  unsigned long long hb[4] = {0, 0, 0, 0};
+  if (c->prof_bytes) {
+    rc = hip_check(c, hipMemcpy(hb, c->prof_bytes, sizeof(hb), hipMemcpyDeviceToHost), "profile read");
+    if (rc) return rc;
+  }
+  lanes[0] = (long long)(hb[2] + c->prof_hbytes[2]);
+  lanes[1] = (long long)(hb[3] + c->prof_hbytes[3]);
   return TV3_OK;
 }
 

@@ -1954,8 +1954,8 @@ class Engine {
   // last look, so those launches are counted on the device.
   void count_bytes(uint32_t mask, int trans) {
     if (!ctx_->prof) return;
-    if (bt_.Bpad == 1 && !graph_ && !(mask & F_BWD)) prof_count_host(ctx_, bt_.groups[0], trans);
-    else prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, trans);
+    if (bt_.Bpad == 1 && !graph_ && !(mask & F_BWD)) prof_count_host(ctx_, bt_.groups[0], trans, (int)sizeof(vec_t));
+    else prof_count(ctx_, G_, W, bt_.Bpad, flags_of(S_), mask, trans, (int)sizeof(vec_t));
   }
 
   void fwd(const vec_t *x, vec_t *y, uint32_t mask) {
@@ -1968,7 +1968,7 @@ class Engine {
         prof_begin(ctx_, kProfGemvN);
         launch_fwd<64>(s_, G64c_, ncg_, bt_.max_m, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0);
         prof_end(ctx_);
-        count_bytes(mask, 0);
+        prof_count_host(ctx_, bt_.groups[0], 0, (int)sizeof(vec_t), 64, ncg_, lanes_.n);
         TV3_GO(kProfLanes, k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_m, 1, pitch64_, mask, (const vec_t *)nullptr);
         return;
       }
@@ -2006,7 +2006,7 @@ class Engine {
         prof_begin(ctx_, kProfGemvT);
         launch_bwd<64>(s_, G64c_, ncg_, bt_.max_n, bt_.kind, bt_.dtype, x64c_, y64c_, flags_of(S_), 0, nullptr, nullptr, 0);
         prof_end(ctx_);
-        count_bytes(mask, 1);
+        prof_count_host(ctx_, bt_.groups[0], 1, (int)sizeof(vec_t), 64, ncg_, lanes_.n);
         TV3_GO(kProfLanes, k_lane_scatter64, gs, dim3(kBlock), 0, s_, G_, S_, y64c_, y, lanes_, bt_.max_n, 0, pitch64_, mask, sub);
         return;
       }

@@ -63,6 +63,7 @@ def lib():
         "tv3_ctx_profile_read": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_LL),
                                       C.POINTER(C.c_double)]),
         "tv3_ctx_profile_slots": (_I, [_VP, C.POINTER(ProfileSlot), _I, C.POINTER(_I)]),
+        "tv3_ctx_profile_lanes": (_I, [_VP, C.POINTER(_LL)]),
         "tv3_plan_dense": (_I, [_VP, _I, _I, _I, _VP, C.POINTER(_VP)]),
         "tv3_plan_sparse": (_I, [_VP, _I, _I, _I] + [_VP] * 6 + [C.POINTER(_VP)]),
         "tv3_plan_geometry": (_I, [_VP, _VP, C.POINTER(_VP)]),
@@ -215,14 +216,17 @@ class Context:
                        for _, name, runs, ms in self.profile_slots() if runs)
 
     def profile_read(self):
-        """{'A': (ms_total, launches, bytes), 'At': (...)}: summed launch
-        milliseconds, launch count and summed algorithmic bytes of the
-        instances the launches computed."""
+        """{'A': (ms_total, launches, bytes, lanes), 'At': (...)}: summed
+        launch milliseconds, launch count, summed compulsory bytes (operator
+        once per computing group, vectors per computing instance) and the
+        number of instance products the launches computed."""
         ms = (C.c_double * 2)()
         cnt = (_LL * 2)()
         by = (C.c_double * 2)()
         self._check(lib().tv3_ctx_profile_read(self.h, ms, cnt, by))
-        return {"A": (ms[0], cnt[0], by[0]), "At": (ms[1], cnt[1], by[1])}
+        ln = (_LL * 2)()
+        self._check(lib().tv3_ctx_profile_lanes(self.h, ln))
+        return {"A": (ms[0], cnt[0], by[0], ln[0]), "At": (ms[1], cnt[1], by[1], ln[1])}
 
     def tensor(self, a, dtype=None):
         """Host array -> contiguous device tensor (float32 unless given)."""
