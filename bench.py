@@ -18,9 +18,10 @@ Workloads (--workload):
   hemisphere   1024 Fibonacci transducers on the lower hemisphere of a 128^3
                volume, 523,776 rays, 3-D anisotropic TV, maxit 100, one solve.
   dense        64x64 image, m = 1229, dense FP32 A, isotropic TV, maxit 300.
-  dense-layers the reference's 2-D multi-layer mode: 8 layers of 64x64 stacked
-               along y sharing one A (m = 1229 per layer), isotropic TV,
-               maxit 300; value = layered solves/s (images/s = 8x).
+  dense-layers the reference's 2-D multi-layer mode: --layers (64) images of
+               64x64 stacked along y sharing one A (m = 1229 per layer),
+               isotropic TV, maxit 300; both products are FP64-MFMA GEMMs
+               over the layers; value = layered solves/s (images/s = layers x).
   ring-f64     the 256x256 ring with the double/double solver (FP64 vectors
                and scalars, tv3_solve_f64), one reconstruction at a time,
                maxit 200; CPU baseline: the oracle's double/double solve.
@@ -58,6 +59,7 @@ sys.path[:0] = [os.path.join(ROOT, "tval3-cpu_amd"), os.path.join(ROOT, "oracle"
 METRIC = ("TVAL3 recons/s, 256² ring (plain FP32 vs batched BF16, ≤1e-3 rel err); "
           "A/Aᵀ GB/s")
 HBM_PEAK_GBPS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
@@ -510,7 +512,7 @@ def run_single(args, ctx, workload):
         desc = f"2D USCT ring 256 transducers on {grid}x{grid}, FP64 vectors, maxit {maxit}"
     elif workload == "dense-layers":
         q = p = 64
-        m, L = 1229, 8
+        m, L = 1229, args.layers
         A = synth.gaussian_A(m, q * p, 0)
         x = np.concatenate([synth.ellipse_phantom(p, q, 8, l) for l in range(L)])
         b = np.concatenate([synth.add_noise(A @ x[l * q * p:(l + 1) * q * p], 0.01, l)
@@ -519,9 +521,10 @@ def run_single(args, ctx, workload):
         dims = (q, L * p, 1)
         maxit = args.maxit if args.maxit_set else 300
         opts = tval3_amd.default_options(maxit=maxit, tv_iso=1, graph=args.graph)
-        kern = {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_part"}
+        kern = {"A": "k_dense_fwd_mfma", "At": "k_dense_bwd_mfma"}
         op_fn = lambda: O.Op.dense(A)  # noqa: E731
-        desc = f"2D dense multi-layer: {L} layers of 64x64, m={m} per layer, isotropic TV, maxit {maxit}"
+        desc = (f"2D dense multi-layer: {L} layers of 64x64 sharing one A (m={m} per layer), "
+                f"isotropic TV, maxit {maxit}")
     else:
         q = p = 64
         m = 1229
@@ -552,13 +555,38 @@ def run_single(args, ctx, workload):
     ops, roof = roofline(prof, kern, workload, plan, t_steps, args.steps)
     if workload != "hemisphere":
         roof["cache_resident"] = True  # operator and vectors fit the 256 MB last-level cache
+    if workload == "dense-layers":
+        # GEMMs over the layers on FP64 MFMA: 2 L m n flops per product
+        fl = {k: prof[k][3] * 2.0 * L * m * (q * p) for k in prof}
+        for k in ops:
+            ops[k]["TFLOPs"] = fl[k] / (prof[k][0] * 1e-3) / 1e12 if prof[k][0] > 0 else None
+        dom = max(ops, key=lambda k: prof[k][0])
+        hbm = dict(roof)
+        roof = {"bound": "mfma", "kernel": kern[dom], "achieved": ops[dom]["TFLOPs"],
+                "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ops[dom]["TFLOPs"] / FP64_MFMA_PEAK_TFLOPS,
+                "traffic": hbm.get("traffic"), "flops_per_launch": fl[dom] / max(prof[dom][1], 1),
+                "ms_per_launch": ops[dom]["ms_per_launch"], "precision": "fp64 MFMA (exact products)",
+                "hbm": {"achieved": hbm["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": hbm["frac"], "bytes_per_launch": hbm["bytes_per_launch"]},
+                "products_GBps_over_wall": hbm.get("products_GBps_over_wall"),
+                "cache_resident": True}
     cpu = None
     if not args.no_cpu:
         op = op_fn()
         solve = O.solve_f64 if f64 else O.solve
-        runs, el, thr = cpu_time(lambda: solve(op, b, dims, opts), args.cpu_seconds, 5)
-        cpu = {"value": runs / el, "unit": "recons/s", "cores": thr, "kind": "port",
-               "sample": f"{runs} full reconstructions ({el:.1f} s, C++/OpenMP oracle)"}
+        if workload == "dense-layers" and L > 8:
+            # the CPU solve is linear in the layer count: time 8 layers, scale
+            Ls = 8
+            bs8 = b[:Ls * m]
+            runs, el, thr = cpu_time(lambda: solve(op, bs8, (q, Ls * p, 1), opts), args.cpu_seconds, 5)
+            cpu = {"value": runs / el * Ls / L, "unit": "recons/s", "cores": thr, "kind": "port",
+                   "sample": f"{runs} full reconstructions of {Ls} of the {L} layers ({el:.1f} s, "
+                             f"C++/OpenMP oracle), scaled by {Ls}/{L}"}
+        else:
+            runs, el, thr = cpu_time(lambda: solve(op, b, dims, opts), args.cpu_seconds, 5)
+            cpu = {"value": runs / el, "unit": "recons/s", "cores": thr, "kind": "port",
+                   "sample": f"{runs} full reconstructions ({el:.1f} s, C++/OpenMP oracle)"}
     extra = {"images_per_s": L * args.steps / t_steps} if workload == "dense-layers" else {}
     return {
         **extra,
@@ -580,6 +608,8 @@ def main():
                     choices=["ring", "dense-batch", "hemisphere", "dense", "dense-layers", "ring-f64",
                              "diff"])
     ap.add_argument("--batch", type=int, default=None)
+    ap.add_argument("--layers", type=int, default=64,
+                    help="dense-layers: images sharing the one A")
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")

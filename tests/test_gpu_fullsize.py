@@ -167,3 +167,49 @@ def test_layered_rows_across_a_tile_boundary(ctx, bf16, monkeypatch):
         monkeypatch.setenv("TV3_DENSE_SHAPE", f"{R},4,8")
         U, _, _ = ctx.solve(plan, b, dims, opts)
         assert _bits_equal(U.cpu().numpy(), Uo), R
+
+
+def _many_layers(L, q, p, m, seed, bf16):
+    import torch
+    N = q * p
+    A = synth.gaussian_A(m, N, seed)
+    At = torch.from_numpy(A).to(torch.bfloat16) if bf16 else A
+    Ar = At.to(torch.float32).numpy() if bf16 else A
+    x = np.concatenate([synth.ellipse_phantom(p, q, 4, seed + l).ravel() for l in range(L)])
+    b = np.concatenate([synth.add_noise(Ar @ x[l * N:(l + 1) * N], 0.01, seed + l)
+                        for l in range(L)]).astype(np.float32)
+    return At, Ar, b, (q, p * L, 1)
+
+
+# (layers, q, p, m, bf16): 64 layers with vector A rows and scalar m-vector
+# loads (m % 4 = 1); 70 layers (a partial layer tile) of bf16 with vector
+# loads on both sides; 20 layers of 15 x 13 (rows of 195 entries: scalar
+# loads) with a row tile that ends short
+MANY = [(64, 24, 24, 173, False), (70, 16, 16, 100, True), (20, 15, 13, 59, False)]
+
+
+@pytest.mark.parametrize("L,q,p,m,bf16", MANY)
+def test_layers_on_matrix_cores(ctx, L, q, p, m, bf16, monkeypatch):
+    """The multi-layer products as FP64-MFMA GEMMs over the layers: the solve
+    equals the oracle's bit for bit, with its trajectory, and equals the
+    per-layer W = 1 kernels' solve (TV3_LAYER_MFMA=0) bit for bit."""
+    At, Ar, b, dims = _many_layers(L, q, p, m, 90 + L, bf16)
+    plan = ctx.plan_dense(At, bf16=bf16)
+    opts = tval3_amd.default_options(maxit=60, ls_exact=1)
+    U1, i1, t1 = ctx.solve(plan, b, dims, opts, trace_cap=60)
+    Uo, io, to = O.solve(O.Op.dense(Ar), b, dims, opts, trace_cap=60)
+    assert _bits_equal(U1.cpu().numpy(), Uo)
+    assert _traj(t1) == _traj(to)
+    monkeypatch.setenv("TV3_LAYER_MFMA", "0")
+    U0, i0, _ = ctx.solve(plan, b, dims, opts)
+    assert _bits_equal(U0.cpu().numpy(), U1.cpu().numpy())
+    # maxit = 0 returns the scaled A^T b alone (the transposed GEMM)
+    monkeypatch.delenv("TV3_LAYER_MFMA")
+    o0 = tval3_amd.default_options(maxit=0)
+    Z1, _, _ = ctx.solve(plan, b, dims, o0)
+    Zo, _, _ = O.solve(O.Op.dense(Ar), b, dims, o0)
+    assert _bits_equal(Z1.cpu().numpy(), Zo)
+    # a single-layer product on the same thread afterwards takes the W = 1
+    # kernels again
+    xl = synth.ellipse_phantom(p, q, 4, 5).ravel()
+    np.testing.assert_array_equal(ctx.gemv(plan, xl).cpu().numpy(), O.Op.dense(Ar).gemv(xl))

@@ -305,6 +305,13 @@ struct DenseTune {
   // no group of the batch runs the whole-row blocked kernel (1) or the
   // row-slice kernels (1): their launches are skipped
   int no_blk = 0, no_slice = 0;
+  // multi-layer batch (every group holds layers >= 2 of one shared A): the
+  // products run as FP64-MFMA GEMMs over the layers (k_dense_*_mfma).  The
+  // largest layer count, A row and column counts of the batch size the
+  // grids; vec_f: every group's A takes vector row loads (a_vec), vec_b:
+  // every group's m is a multiple of 4 (vector loads of the layers'
+  // m-vectors)
+  int mfma = 0, mfma_layers = 0, mfma_m = 0, mfma_n = 0, mfma_vec_f = 0, mfma_vec_b = 0;
 };
 void dense_tune_set(DenseTune t);
 DenseTune dense_tune_cur();

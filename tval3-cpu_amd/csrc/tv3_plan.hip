@@ -1263,10 +1263,233 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_bred(
   y[o] = sub ? gf - sub[o] : gf;
 }
 
+// ---------------------------------------------------------------------------
+// Multi-layer dense products on the matrix cores (FP64 MFMA)
+// ---------------------------------------------------------------------------
+// The reference's multi-layer mode applies one A to L images
+// (2d/mat_mult.h:22-35: cblas_gemm over the layers).  Here both products are
+// GEMMs over the layers on v_mfma_f64_16x16x4f64: float and bf16 entries and
+// float vector values widen to FP64 exactly, so every product term is exact
+// and the sums accumulate in FP64, rounded once to float as in the W = 1
+// kernels (a different FP64 summation order, as between those kernels and
+// the oracle).  A is read once per 16 layers instead of once per layer.
+//
+// Operand layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l
+// passes A-operand entry (row l % 16, k l / 16) and B-operand entry (k l / 16,
+// column l % 16); result (i, j) is register i / 4 of lane 16 (i % 4) + j.
+// A workgroup of NW wavefronts computes one 16 x 16 output tile; wave w sums
+// its share of K (whole 16-wide K blocks), and wave 0 adds the NW partial
+// tiles in wave order through LDS.
+typedef double tv3_d4 __attribute__((ext_vector_type(4)));
+
+// wavefronts per output tile (TV3_MFMA_WAVES = 4, 8 or 16 overrides).  8
+// measured best on 64 x 64 layers sharing a 1229 x 4096 A: A X / A^T Y took
+// 23.0 / 15.2 us at 8 layers and 38.6 / 42.9 us at 64 (4 waves: 31.8 / 15.9
+// and 43.2 / 41.9; 16 waves: 21.8 / 15.7 and 43.2 / 58.8).  The kernels
+// load one value per lane per MFMA (4 flops per loaded byte), so L2-to-CU
+// bandwidth, not the matrix cores, bounds them.
+int mfma_waves() {
+  static const int w = [] {
+    const char *e = std::getenv("TV3_MFMA_WAVES");
+    const int v = e ? std::atoi(e) : 8;
+    return (v == 4 || v == 16) ? v : 8;
+  }();
+  return w;
+}
+
+__device__ __forceinline__ float mf_load(const float *p) { return *p; }
+__device__ __forceinline__ float mf_load(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+// four consecutive entries, 16-byte (f32) / 8-byte (bf16) aligned
+__device__ __forceinline__ void mf_load4(const float *p, float (&v)[4]) {
+  const float4 q = *(const float4 *)p;
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void mf_load4(const __hip_bfloat16 *p, float (&v)[4]) { load4(p, v); }
+
+// wave w's K range of NW: whole 16-wide blocks, an equal share each
+template <int NW>
+__device__ __forceinline__ void mf_krange(int K, int w, int &k0, int &k1) {
+  const int nb = (K + 15) / 16, per = (nb + NW - 1) / NW;
+  k0 = min(K, w * per * 16);
+  k1 = min(K, (w + 1) * per * 16);
+}
+
+// adds the NW waves' partial tiles in wave order; true in wave 0
+template <int NW>
+__device__ __forceinline__ bool mf_reduce(tv3_d4 &acc) {
+  __shared__ double red[NW - 1][4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w > 0)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[w - 1][r][lane] = acc[r];
+  __syncthreads();
+  if (w > 0) return false;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double t = acc[r];
+    for (int v = 0; v < NW - 1; ++v) t += red[v][r][lane];
+    acc[r] = t;
+  }
+  return true;
+}
+
+// Each wave walks its K range in groups of kMfU 16-wide blocks: the next
+// group's operands are loaded while the current group's 4 kMfU MFMAs run,
+// and substep s of every block accumulates into acc[s], so four MFMA chains
+// are in flight; the four accumulators are added in order at the end.
+constexpr int kMfU = 4;
+
+// loads one group of kMfU blocks starting at kb: the lane's four entries
+// k = kb' + 4 q + s of every block kb' < k1 (zero beyond the operand's K and
+// the wave's range).  FWD: both operands run along K contiguously; else the
+// B operand is column `col` of a row-major A with row stride n.
+template <class T, bool VEC, bool FWD>
+__device__ __forceinline__ void mf_group(const T *Ab, const float *xl, bool aok, bool xok, int kb,
+                                         int k1, int K, int q, long long n, float (&av)[kMfU][4],
+                                         float (&xv)[kMfU][4]) {
+#pragma unroll
+  for (int u = 0; u < kMfU; ++u) {
+    const int k = kb + 16 * u + 4 * q;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) { av[u][s] = 0.0f; xv[u][s] = 0.0f; }
+    if (kb + 16 * u >= k1) continue;
+    if (VEC && k + 4 <= K) {
+      if (xok) mf_load4(xl + k, xv[u]);
+      if (FWD && aok) mf_load4(Ab + k, av[u]);
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (k + s < K) {
+          if (xok) xv[u][s] = xl[k + s];
+          if (FWD && aok) av[u][s] = mf_load(Ab + k + s);
+        }
+    }
+    if (!FWD)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (aok && k + s < K) av[u][s] = mf_load(Ab + (long long)(k + s) * n);
+  }
+}
+
+// the four-accumulator MFMA walk of one wave's K range [k0, k1)
+template <class T, bool VEC, bool FWD>
+__device__ __forceinline__ tv3_d4 mf_walk(const T *Ab, const float *xl, bool aok, bool xok, int k0,
+                                          int k1, int K, int q, long long n) {
+  tv3_d4 acc[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) acc[s] = tv3_d4{0.0, 0.0, 0.0, 0.0};
+  float av[kMfU][4], xv[kMfU][4];
+  mf_group<T, VEC, FWD>(Ab, xl, aok, xok, k0, k1, K, q, n, av, xv);
+  for (int kb = k0; kb < k1; kb += 16 * kMfU) {
+    float an[kMfU][4], xn[kMfU][4];
+    const bool more = kb + 16 * kMfU < k1;
+    if (more) mf_group<T, VEC, FWD>(Ab, xl, aok, xok, kb + 16 * kMfU, k1, K, q, n, an, xn);
+#pragma unroll
+    for (int u = 0; u < kMfU; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xv[u][s], (double)av[u][s], acc[s], 0, 0, 0);
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < kMfU; ++u)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) { av[u][s] = an[u][s]; xv[u][s] = xn[u][s]; }
+    }
+  }
+  tv3_d4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r[i] = ((acc[0][i] + acc[1][i]) + acc[2][i]) + acc[3][i];
+  return r;
+}
+
+// Y (L x m) = X (L x n) A^T: layer l's A x at y[off_m + l m + i].  Grid:
+// (row tiles of 16, layer tiles of 16, groups); VEC: A rows and the layers'
+// n-vectors take 16-byte vector loads (a_vec).
+template <class T, bool VEC, int NW>
+__global__ __launch_bounds__(64 * NW) void k_dense_fwd_mfma(const GroupDesc *G, const float *x,
+                                                            float *y, const InstScal *S,
+                                                            uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m, L = gd.layers;
+  const int i0 = blockIdx.x * 16, l0 = blockIdx.y * 16;
+  if (i0 >= pm || l0 >= L) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, q = lane >> 4;
+  const int row = i0 + c, lyr = l0 + c;
+  const bool rok = row < pm, lok = lyr < L;
+  const T *Ar = (const T *)gd.plan.A + (long long)(rok ? row : 0) * n;
+  const float *xl = x + gd.off_n + (long long)(lok ? lyr : 0) * n;
+  int k0, k1;
+  mf_krange<NW>(n, w, k0, k1);
+  tv3_d4 acc = mf_walk<T, VEC, true>(Ar, xl, rok, lok, k0, k1, n, q, n);
+  if (!mf_reduce<NW>(acc)) return;
+  // register r of this lane: layer l0 + 4 r + q, row i0 + c
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int l = l0 + 4 * r + q;
+    if (l < L && rok) y[gd.off_m + (long long)l * pm + row] = (float)acc[r];
+  }
+}
+
+// G (L x n) = Y (L x m) A: layer l's A^T y at y[off_n + l n + j], minus sub.
+// Grid: (column tiles of 16, layer tiles of 16, groups); VEC: the layers'
+// m-vectors take 16-byte vector loads (m % 4 == 0).
+template <class T, bool VEC, int NW>
+__global__ __launch_bounds__(64 * NW) void k_dense_bwd_mfma(const GroupDesc *G, const float *x,
+                                                            float *y, const InstScal *S,
+                                                            uint32_t mask, const float *sub) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m, L = gd.layers;
+  const int j0 = blockIdx.x * 16, l0 = blockIdx.y * 16;
+  if (j0 >= n || l0 >= L) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, q = lane >> 4;
+  const int col = j0 + c, lyr = l0 + c;
+  const bool cok = col < n, lok = lyr < L;
+  const T *Ac = (const T *)gd.plan.A + (cok ? col : 0);
+  const float *xl = x + gd.off_m + (long long)(lok ? lyr : 0) * pm;
+  int k0, k1;
+  mf_krange<NW>(pm, w, k0, k1);
+  tv3_d4 acc = mf_walk<T, VEC, false>(Ac, xl, cok, lok, k0, k1, pm, q, n);
+  if (!mf_reduce<NW>(acc)) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int l = l0 + 4 * r + q;
+    if (l < L && cok) {
+      const long long o = gd.off_n + (long long)l * n + col;
+      float v = (float)acc[r];
+      if (sub) v = v - sub[o];
+      y[o] = v;
+    }
+  }
+}
+
 template <>
 void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                    int kind, int dtype, const float *x, float *y,
                    const InstScal *S, uint32_t mask) {
+  if (kind == kDense && dense_tune_cur().mfma) {
+    // tiles of the largest A and layer count; smaller groups exit early
+    const DenseTune t = dense_tune_cur();
+    const dim3 g((t.mfma_m + 15) / 16, (t.mfma_layers + 15) / 16, G);
+#define TV3_MF(TT, VV, NW) \
+  hipLaunchKernelGGL((k_dense_fwd_mfma<TT, VV, NW>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask)
+#define TV3_MFW(TT, VV)                           \
+  if (mfma_waves() == 4) TV3_MF(TT, VV, 4);       \
+  else if (mfma_waves() == 16) TV3_MF(TT, VV, 16); \
+  else TV3_MF(TT, VV, 8);
+    if (dtype == TV3_BF16) {
+      if (t.mfma_vec_f) { TV3_MFW(__hip_bfloat16, true) } else { TV3_MFW(__hip_bfloat16, false) }
+    } else {
+      if (t.mfma_vec_f) { TV3_MFW(float, true) } else { TV3_MFW(float, false) }
+    }
+#undef TV3_MFW
+#undef TV3_MF
+    (void)max_m;
+    return;
+  }
   if (kind == kDense) {
     const int R = dense_tune_cur().rows;
     dim3 g((max_m + 4 * R - 1) / (4 * R), G);
@@ -1305,6 +1528,27 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                    int kind, int dtype, const float *x, float *y,
                    const InstScal *S, uint32_t mask, const float *sub,
                    float *scratch, int submat_mb) {
+  if (kind == kDense && dense_tune_cur().mfma) {
+    const DenseTune t = dense_tune_cur();
+    const dim3 g((t.mfma_n + 15) / 16, (t.mfma_layers + 15) / 16, G);
+#define TV3_MB(TT, VV, NW) \
+  hipLaunchKernelGGL((k_dense_bwd_mfma<TT, VV, NW>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask, sub)
+#define TV3_MBW(TT, VV)                           \
+  if (mfma_waves() == 4) TV3_MB(TT, VV, 4);       \
+  else if (mfma_waves() == 16) TV3_MB(TT, VV, 16); \
+  else TV3_MB(TT, VV, 8);
+    if (dtype == TV3_BF16) {
+      if (t.mfma_vec_b) { TV3_MBW(__hip_bfloat16, true) } else { TV3_MBW(__hip_bfloat16, false) }
+    } else {
+      if (t.mfma_vec_b) { TV3_MBW(float, true) } else { TV3_MBW(float, false) }
+    }
+#undef TV3_MBW
+#undef TV3_MB
+    (void)max_n;
+    (void)scratch;
+    (void)submat_mb;
+    return;
+  }
   if (kind == kDense && submat_mb > 0) {
     const DenseTune tn = dense_tune_cur();
     const int Ur = tn.unroll, Cc = tn.cols;

@@ -1609,6 +1609,11 @@ bool autotune_enabled() {
   return !(e && e[0] == '0');
 }
 
+bool layer_mfma_enabled() {
+  const char *e = std::getenv("TV3_LAYER_MFMA");
+  return !(e && e[0] == '0');
+}
+
 // TV3_DENSE_SHAPE=R,U,C forces the dense launch shape (R in {1,2,4} rows per
 // wavefront of A x, U in {1,2,4,8} rows ahead, C in {4,8} columns per thread
 // of the blocked A^T x), so that tests can run every candidate; false when
@@ -1813,11 +1818,33 @@ class Engine {
     // the row-slice pair, else the whole-row kernel); set with every choice
     bool any_slice = false, any_blk = false;
     for (const GroupDesc &g : bt_.groups) (g.dblk_S > 0 ? any_slice : any_blk) = true;
+    // multi-layer batches: both products on the matrix cores
+    // (TV3_LAYER_MFMA=0 keeps the per-layer W = 1 kernels)
+    DenseTune mf{};
+    if (bt_.layered && layer_mfma_enabled()) {
+      mf.mfma = 1;
+      mf.mfma_vec_f = mf.mfma_vec_b = 1;
+      for (const GroupDesc &g : bt_.groups) {
+        mf.mfma_layers = std::max(mf.mfma_layers, g.layers);
+        mf.mfma_m = std::max(mf.mfma_m, g.plan.m);
+        mf.mfma_n = std::max(mf.mfma_n, g.plan.n);
+        if (!g.plan.a_vec) mf.mfma_vec_f = 0;
+        if (g.plan.m % 4) mf.mfma_vec_b = 0;
+      }
+    }
     auto use = [&](DenseTune t) {
       t.no_blk = !any_blk;
       t.no_slice = !any_slice;
+      t.mfma = mf.mfma;
+      t.mfma_layers = mf.mfma_layers;
+      t.mfma_m = mf.mfma_m;
+      t.mfma_n = mf.mfma_n;
+      t.mfma_vec_f = mf.mfma_vec_f;
+      t.mfma_vec_b = mf.mfma_vec_b;
       dense_tune_set(t);
     };
+    // the MFMA products have one launch shape
+    if (mf.mfma) { use(best); return TV3_OK; }
     if (forced_dense_shape(best) || !autotune_enabled()) { use(best); return TV3_OK; }
     long long work = 0;
     for (const GroupDesc &g : bt_.groups) work += (long long)g.m * g.n;
@@ -1868,6 +1895,15 @@ class Engine {
 
   int run(const vec_t *const *b, vec_t *const *U, const vec_t *const *Ut,
           tv3_info *infos, tv3_trace_rec *trace, int trace_cap) {
+    // the MFMA layer products are this solve's choice only: later products
+    // on this thread (tv3_gemv, single-layer solves) must not take them
+    struct MfmaReset {
+      ~MfmaReset() {
+        DenseTune t = dense_tune_cur();
+        t.mfma = 0;
+        dense_tune_set(t);
+      }
+    } mfma_reset;
     int rc = setup();
     if (rc) return rc;
     if ((rc = tune_dense())) return rc;
