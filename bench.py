@@ -318,24 +318,31 @@ def run_dense_batch(args, ctx):
     torch.cuda.synchronize()
     log(f"dense-batch: {B} instances, {tot_elems / 1e9:.2f} G bf16 entries "
         f"({2 * tot_elems / 2**30:.1f} GiB)")
-    # FP32 vectors, FP64 step-size / line-search scalars
+    # FP32 vectors, FP64 step-size / line-search scalars; --vectors f64: FP64
+    # vectors and scalars on the same bf16 A (the FP64 solve of the rounded
+    # matrices)
     opts = tval3_amd.default_options(maxit=args.maxit, scal_double=1, graph=args.graph)
+    f64v = args.vectors == "f64"
+    if f64v:
+        bs = [b.to(torch.float64) for b in bs]
     for _ in range(args.warmup):
-        ctx.solve_batch(plans, bs, dims, opts)
+        ctx.solve_batch(plans, bs, dims, opts, f64=f64v)
     torch.cuda.synchronize()
     ctx.profile(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        Us, infos = ctx.solve_batch(plans, bs, dims, opts)
+        Us, infos = ctx.solve_batch(plans, bs, dims, opts, f64=f64v)
         log(f"step {k + 1}/{args.steps} done")
     torch.cuda.synchronize()
     t_steps = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
-    args.one_step = lambda: ctx.solve_batch(plans, bs, dims, opts)
+    args.one_step = lambda: ctx.solve_batch(plans, bs, dims, opts, f64=f64v)
     value = B * args.steps / t_steps
-    ops, roof = roofline(prof, {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_blk"},
-                         "dense-batch", None, t_steps, args.steps)
+    kern = ({"A": "k_dense_fwd_db", "At": "k_dense_bwd_part_db"} if f64v
+            else {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_blk"})
+    ops, roof = roofline(prof, kern, "dense-batch" + ("-f64" if f64v else ""), None, t_steps,
+                         args.steps)
     # every instance against the FP64 solve of the same bf16-rounded
     # matrices (tv3_solve_batch_f64 on the GPU, in chunks of FP64 copies of
     # A), and the smallest ones against the CPU oracle's FP32 solve (the
@@ -351,6 +358,7 @@ def run_dense_batch(args, ctx):
             if chunk and (k is None or used + need > args.fp64_chunk_gib):
                 p64 = [ctx.plan_dense(keep[j][0].to(torch.float64), f64=True) for j in chunk]
                 b64 = [bs[j].to(torch.float64) for j in chunk]
+                # the reference: FP64 copies of the bf16-rounded A
                 U64, _ = ctx.solve_batch(p64, b64, [dims[j] for j in chunk], opts, f64=True)
                 for j, u in zip(chunk, U64):
                     u = u.cpu().numpy()
@@ -363,7 +371,7 @@ def run_dense_batch(args, ctx):
                 chunk.append(k)
                 used += need
     order = sorted(range(B), key=lambda k: sizes[k][0] ** 2 * sizes[k][1])
-    for k in order[:args.f64_sample]:
+    for k in (order[:args.f64_sample] if not f64v else []):
         op = O.Op.dense(keep[k][0].float().cpu().numpy())
         U32, _, _ = O.solve(op, bs[k].cpu().numpy(), dims[k], opts)
         errs.setdefault(k, {"grid": sizes[k][0], "m": sizes[k][1]})["gpu_vs_oracle_fp32"] = float(
@@ -393,7 +401,8 @@ def run_dense_batch(args, ctx):
                          f"average instance"}
     return {
         "value": value, "unit": "recons/s", "steps": args.steps,
-        "ms_per_step": 1e3 * t_steps / args.steps, "dtype": "bf16 A, f32 vectors",
+        "ms_per_step": 1e3 * t_steps / args.steps,
+        "dtype": "bf16 A, f64 vectors" if f64v else "bf16 A, f32 vectors",
         "data": "synthetic: A ~ N(0,1/m) rounded to bf16, 8-ellipse phantoms, 1% noise",
         "config": {"workload": f"{B} uneven dense 2D reconstructions (bf16 A), maxit {args.maxit}",
                    "batch": B, "bf16_entries": tot_elems, "maxit": args.maxit},
@@ -623,6 +632,8 @@ def main():
                     help="ring: instances compared with their FP64 (GPU) solves")
     ap.add_argument("--fp64-chunk-gib", type=float, default=48.0,
                     help="dense-batch: FP64 copies of A per chunk of the FP64 batch solves")
+    ap.add_argument("--vectors", default="f32", choices=["f32", "f64"],
+                    help="dense-batch: vector and scalar precision on the bf16 A")
     ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
                     help="1: replay each pair of solver steps from a hipGraph")
     ap.add_argument("--slots", default=None, metavar="FILE",

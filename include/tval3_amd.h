@@ -351,7 +351,9 @@ int tv3_solve_batch(tv3_ctx *ctx, int B, const tv3_plan *const *A,
                     const float *const *Ut, tv3_info *infos);
 /* The same with FP64 vectors and scalars (T_comp = T_scal = double;
  * opts->scal_double is ignored): A is a TV3_F64 dense, a double sparse or a
- * geometry plan.  Every product, dot and norm is an FP64 sum, in a different
+ * geometry plan, or a TV3_BF16 dense plan (no reference counterpart: its
+ * entries widen exactly, so this is the FP64 solve of the bf16-rounded
+ * matrix at the storage cost of bf16).  Every product, dot and norm is an FP64 sum, in a different
  * order than the reference's loops, so results agree with the reference's
  * double solve to FP64 rounding.  Instances run one per group (no 64-lane
  * interleaving); multi-layer mode as in tv3_solve. */

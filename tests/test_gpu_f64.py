@@ -227,3 +227,42 @@ def test_graph_mode_f64(ctx, ls):
     for k in range(2):
         assert np.array_equal(res[0][0][k].cpu().numpy(), res[1][0][k].cpu().numpy()), k
         assert res[0][1][k].total_iters == res[1][1][k].total_iters
+
+
+def test_bf16_A_with_f64_vectors(ctx):
+    """A bf16-stored A with FP64 vectors and scalars: the FP64 solve of the
+    bf16-rounded matrix.  An uneven batch (vector-aligned rows and one 15 x 13
+    instance with rows of 195 entries) agrees with the same solves on FP64
+    copies of the rounded matrices to FP64 rounding (their products sum in a
+    different order), and each solve meets the FP64 bar against the
+    oracle's double/double solve of the rounded matrix."""
+    import torch
+    sizes = [(24, 24, 0.3), (32, 32, 0.25), (20, 28, 0.4), (15, 13, 0.5)]
+    p16, p64, bs, dims, mats = [], [], [], [], []
+    for k, (q, p, fr) in enumerate(sizes):
+        A, x, _ = _dense_case(q, p, fr, 70 + k)
+        Ab = torch.from_numpy(A).to(torch.bfloat16)
+        Ar = Ab.to(torch.float64).numpy()
+        b = synth.add_noise((Ar @ x.ravel()).astype(np.float32), 0.01, 70 + k).astype(np.float64)
+        p16.append(ctx.plan_dense(Ab, bf16=True))
+        p64.append(ctx.plan_dense(Ar, f64=True))
+        bs.append(b)
+        dims.append((q, p, 1))
+        mats.append(Ar)
+    opts = tval3_amd.default_options(maxit=80, ls_exact=1)
+    U16, i16 = ctx.solve_batch(p16, bs, dims, opts, f64=True)
+    U64, i64 = ctx.solve_batch(p64, bs, dims, opts, f64=True)
+    for k in range(len(sizes)):
+        u16, u64 = U16[k].cpu().numpy(), U64[k].cpu().numpy()
+        op = O.Op.dense(mats[k].astype(np.float32))
+        Uo, info_o, _ = O.solve_f64(op, bs[k], dims[k], opts)
+        bar = _bound(op, bs[k], dims[k], opts, Uo)
+        assert i16[k].total_iters == i64[k].total_iters == info_o.total_iters, k
+        assert _rel(u16, Uo) <= bar and _rel(u64, Uo) <= bar, (k, _rel(u16, Uo), bar)
+        print(f"instance {k}: bf16-A FP64 vs f64-A FP64 {_rel(u16, u64):.2e} (bar {bar:.2e})")
+    # the product itself
+    xk = np.random.default_rng(1).standard_normal(mats[0].shape[1])
+    y16 = ctx.gemv(p16[0], xk, f64=True).cpu().numpy()
+    np.testing.assert_allclose(y16, mats[0] @ xk, rtol=1e-13, atol=1e-13 * np.abs(mats[0]).sum(1).max())
+    yt = ctx.gemv(p16[3], bs[3], trans=True, f64=True).cpu().numpy()
+    np.testing.assert_allclose(yt, mats[3].T @ bs[3], rtol=1e-13, atol=1e-13 * np.abs(mats[3]).sum(0).max())

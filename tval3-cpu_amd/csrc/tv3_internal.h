@@ -132,8 +132,11 @@ __host__ __device__ inline bool plan_takes_f32(const PlanDev &p) {
   return p.kind == kDense ? (p.dtype == TV3_F32 || p.dtype == TV3_BF16)
                           : p.kind == kSparse ? (p.r_val != nullptr || (p.nnz == 0 && p.cval == 0)) : true;
 }
+// (a bf16 dense A also serves double vectors: widened exactly, it gives the
+// FP64 solve of the bf16-rounded matrix)
 __host__ __device__ inline bool plan_takes_f64(const PlanDev &p) {
-  return p.kind == kDense ? p.dtype == TV3_F64 : p.kind == kSparse ? p.r_vald != nullptr || p.nnz == 0 : true;
+  return p.kind == kDense ? (p.dtype == TV3_F64 || p.dtype == TV3_BF16)
+                          : p.kind == kSparse ? p.r_vald != nullptr || p.nnz == 0 : true;
 }
 // complex vectors: complex entries of the same precision, or the projector
 __host__ __device__ inline bool plan_takes_c64(const PlanDev &p) {

@@ -1832,11 +1832,136 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_reduce_d(
   y[o] = sub ? acc - sub[o] : acc;
 }
 
+// Double vectors with a bf16-stored A: the FP64 solve of the bf16-rounded
+// matrix (entries widen exactly).  y = A x: four rows per wavefront share
+// the x loads; with vector rows (a_vec) lane l sums entries 8 l + 512 t + u
+// (u = 0..7) in order, else entries l + 64 t; FP64 lane sums and a fixed
+// butterfly, so each row's order is the same whether or not its tile
+// crosses a layer boundary.
+__global__ __launch_bounds__(kBlock) void k_dense_fwd_db(const GroupDesc *G, const double *x, double *y,
+                                                         const InstScal *S, uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int row0 = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 4;
+  if (row0 >= gd.m) return;
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const int n = gd.plan.n, pm = gd.plan.m;
+  const int nr = min(4, gd.m - row0);
+  const __hip_bfloat16 *A = (const __hip_bfloat16 *)gd.plan.A;
+  const bool vec = gd.plan.a_vec != 0;
+  auto row_sum = [&](const __hip_bfloat16 *Ar, const double *xr) -> double {
+    double a = 0.0;
+    if (vec) {
+      for (int j = lane * 8; j < n; j += 512) {
+        float av[8];
+        load8(Ar + j, av);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += (double)av[u] * xr[j + u];
+      }
+    } else {
+      for (int j = lane; j < n; j += 64) a += (double)a_val(Ar, j) * xr[j];
+    }
+    return a;
+  };
+  if (row0 / pm != (row0 + nr - 1) / pm || !vec) {
+    for (int r = 0; r < nr; ++r) {
+      const int row = row0 + r;
+      const double t = wave_sum(row_sum(A + (long long)(row % pm) * n, x + gd.off_n + (long long)(row / pm) * n));
+      if (lane == 0) y[gd.off_m + row] = t;
+    }
+    return;
+  }
+  const __hip_bfloat16 *Ab = A + (long long)(row0 % pm) * n;
+  const double *xg = x + gd.off_n + (long long)(row0 / pm) * n;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = lane * 8; j < n; j += 512) {
+    double xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = xg[j + u];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < nr) {
+        float av[8];
+        load8(Ab + (long long)r * n + j, av);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[r] += (double)av[u] * xv[u];
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double t = wave_sum(acc[r]);
+    if (lane == 0 && r < nr) y[gd.off_m + row0 + r] = t;
+  }
+}
+
+// A^T x for a bf16 A and double vectors: the chunks of k_dense_bwd_part_d
+// (the same rows per chunk, each column's rows in order), eight consecutive
+// columns per thread, from one 16-byte load per row when the rows are vector
+// aligned (a_vec: the 8 columns then lie in one layer), else column by
+// column; k_dense_bwd_reduce_d adds the chunks.
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_part_db(
+    const GroupDesc *G, const double *x, double *scratch, const InstScal *S,
+    uint32_t mask, long long sstride) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m;
+  const int c0 = (blockIdx.x * kBlock + threadIdx.x) * 8;
+  if (c0 >= gd.n) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int nch = min(kDenseChunks, (pm + 63) / 64);
+  const int c = blockIdx.y;
+  if (c >= nch) return;
+  const int R = (pm + nch - 1) / nch;
+  const int i0 = c * R, i1 = min(pm, i0 + R);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int w = min(8, gd.n - c0);
+  double *out = scratch + blockIdx.z * sstride + (long long)c * gd.n + c0;
+  if (!(gd.plan.a_vec && w == 8)) {
+    for (int u = 0; u < w; ++u) {
+      const int col = c0 + u;
+      const __hip_bfloat16 *Ac = (const __hip_bfloat16 *)gd.plan.A + col % n;
+      const double *xc = x + gd.off_m + (long long)(col / n) * pm;
+      double a = 0.0;
+      for (int i = i0; i < i1; ++i) a += (double)a_val(Ac, (long long)i * n) * xc[i];
+      out[u] = a;
+    }
+    return;
+  }
+  const __hip_bfloat16 *A = (const __hip_bfloat16 *)gd.plan.A + c0 % n;
+  const double *xg = x + gd.off_m + (long long)(c0 / n) * pm;
+  int i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    float a[4][8];
+    double xi[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      load8(A + (long long)(i + r) * n, a[r]);
+      xi[r] = xg[i + r];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+  }
+  for (; i < i1; ++i) {
+    float a[8];
+    load8(A + (long long)i * n, a);
+    const double xi = xg[i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += (double)a[u] * xi;
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) out[u] = acc[u];
+}
+
 template <>
 void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                    int kind, int dtype, const double *x, double *y,
                    const InstScal *S, uint32_t mask) {
-  (void)dtype;
+  if (kind == kDense && dtype == TV3_BF16) {
+    hipLaunchKernelGGL(k_dense_fwd_db, dim3((max_m + 4 * (kBlock / 64) - 1) / (4 * (kBlock / 64)), G), dim3(kBlock), 0,
+                       s, G_dev, x, y, S, mask);
+    return;
+  }
   if (kind == kDense) {
     hipLaunchKernelGGL(k_fwd_w1d<64>, dim3((max_m + kBlock / 64 - 1) / (kBlock / 64), G), dim3(kBlock), 0, s,
                        G_dev, x, y, S, mask);
@@ -1851,13 +1976,16 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                    int kind, int dtype, const double *x, double *y,
                    const InstScal *S, uint32_t mask, const double *sub,
                    double *scratch, int submat_mb) {
-  (void)dtype;
   (void)submat_mb;
   if (kind == kDense) {
     // scratch holds kDenseChunks * max_n doubles per group
     const long long sstride = (long long)kDenseChunks * max_n;
-    hipLaunchKernelGGL(k_dense_bwd_part_d, dim3((max_n + kBlock - 1) / kBlock, kDenseChunks, G), dim3(kBlock), 0,
-                       s, G_dev, x, scratch, S, mask, sstride);
+    if (dtype == TV3_BF16)
+      hipLaunchKernelGGL(k_dense_bwd_part_db, dim3((max_n + 8 * kBlock - 1) / (8 * kBlock), kDenseChunks, G),
+                         dim3(kBlock), 0, s, G_dev, x, scratch, S, mask, sstride);
+    else
+      hipLaunchKernelGGL(k_dense_bwd_part_d, dim3((max_n + kBlock - 1) / kBlock, kDenseChunks, G), dim3(kBlock), 0,
+                         s, G_dev, x, scratch, S, mask, sstride);
     hipLaunchKernelGGL(k_dense_bwd_reduce_d, dim3((max_n + kBlock - 1) / kBlock, G), dim3(kBlock), 0, s, G_dev,
                        (const double *)scratch, y, S, mask, sub, sstride);
     return;

@@ -144,13 +144,13 @@ inline bool plan_ok(const PlanDev &p) {
                                    : plan_takes_f32(p);
 }
 constexpr const char *kPlanErr =
-    TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_solve_f64 needs an f64 dense, double sparse or geometry plan"
+    TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_solve_f64 needs an f64 or bf16 dense, double sparse or geometry plan"
     : TV3_SOLVER_VARIANT == 2 ? "Argument error: tv3_solve_c64 needs a c64 dense, complex-float sparse or geometry plan"
     : TV3_SOLVER_VARIANT == 3 ? "Argument error: tv3_solve_c128 needs a c128 dense, complex-double sparse or geometry plan"
                               : "Argument error: an f64 / double-valued or complex plan needs its own entry point "
                                 "(tv3_solve_f64, tv3_solve_c64, tv3_solve_c128)";
 [[maybe_unused]] constexpr const char *kGemvErr =
-    TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_gemv_f64 needs an f64 dense, double sparse or geometry plan"
+    TV3_SOLVER_VARIANT == 1   ? "Argument error: tv3_gemv_f64 needs an f64 or bf16 dense, double sparse or geometry plan"
     : TV3_SOLVER_VARIANT == 2 ? "Argument error: tv3_gemv_c64 needs a c64 dense, complex-float sparse or geometry plan"
                               : "Argument error: tv3_gemv_c128 needs a c128 dense, complex-double sparse or geometry plan";
 typedef InstScalT<vr_t> Scal;
