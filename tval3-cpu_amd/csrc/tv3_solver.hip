@@ -363,7 +363,6 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
     a.Ux[o] = ux;
     a.Uy[o] = uy;
     if (z3) a.Uz[o] = uz;
-    a.dU[o] = u0 - p0;
     a.g2[o] = t;
     LOOP_END
   }
@@ -426,18 +425,22 @@ __global__ __launch_bounds__(kBlock) void k_skipg(KA K, Arrs a) {
 }
 
 // ---------------------------------------------------------------------------
-// min_u backtracking (2d:550-575)
+// min_u backtracking (2d:550-575).  The reference interpolates g2 too
+// (2d:509); update_W recomputes g2 from the new W before anything reads it
+// (2d:664), so that interpolation is not carried out here.  dU = U - Up is
+// formed here, once, from the descent's U (the same float subtraction as
+// 2d:542).
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bt_diff_n(KA K, Arrs a) {
   NLOOP_BEGIN(W)
   const vec_t dg = a.g[o] - a.gp[o];
-  const vec_t dg2 = a.g2[o] - a.g2p[o];
+  const vec_t du = a.U[o] - a.Up[o];
   const vec_t dux = a.Ux[o] - a.Uxp[o];
   const vec_t duy = a.Uy[o] - a.Uyp[o];
   const vec_t duz = gd.r > 1 ? a.Uz[o] - a.Uzp[o] : 0.0f;
   a.dg[o] = dg;
-  a.dg2[o] = dg2;
+  a.dU[o] = du;
   a.dUx[o] = dux;
   a.dUy[o] = duy;
   if (gd.r > 1) a.dUz[o] = duz;
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
     const TS al = (TS)S.alpha;
     const bool z3 = gd.r > 1;
     // ---- loads
-    const vec_t gp = a.gp[o], dg = a.dg[o], g2p = a.g2p[o], dg2 = a.dg2[o];
+    const vec_t gp = a.gp[o], dg = a.dg[o];
     const vec_t up = a.Up[o], du = a.dU[o];
     const vec_t uxp = a.Uxp[o], dux = a.dUx[o], uyp = a.Uyp[o], duy = a.dUy[o];
     const vec_t wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
@@ -492,7 +495,6 @@ __global__ __launch_bounds__(kBlock) void k_bt_upd_n(KA K, Arrs a) {
     }
     // ---- stores
     a.g[o] = interp<TS>(gp, al, dg);
-    a.g2[o] = interp<TS>(g2p, al, dg2);
     a.U[o] = interp<TS>(up, al, du);
     a.Ux[o] = ux;
     a.Uy[o] = uy;
@@ -532,8 +534,8 @@ __global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
     const TS al = (TS)S.alpha;
     const bool z3 = gd.r > 1;
     // ---- loads
-    const vec_t gp = a.gp[o], g2p = a.g2p[o], gc = a.g[o], g2c = a.g2[o];
-    const vec_t up = a.Up[o], du = a.dU[o];
+    const vec_t gp = a.gp[o], gc = a.g[o];
+    const vec_t up = a.Up[o], du = a.U[o] - up;
     const vec_t uxp = a.Uxp[o], uyp = a.Uyp[o], uxc = a.Ux[o], uyc = a.Uy[o];
     const vec_t wx = a.Wx[o], wy = a.Wy[o], sx = a.sx[o], sy = a.sy[o];
     vec_t uzp = 0.0f, uzc = 0.0f, wz = 0.0f, sz = 0.0f;
@@ -560,7 +562,6 @@ __global__ __launch_bounds__(kBlock) void k_interp_n(KA K, Arrs a) {
     }
     // ---- stores
     a.g[o] = interp<TS>(gp, al, gc - gp);
-    a.g2[o] = interp<TS>(g2p, al, g2c - g2p);
     a.U[o] = interp<TS>(up, al, du);
     a.Ux[o] = ux;
     a.Uy[o] = uy;
