@@ -264,6 +264,26 @@ def test_maxit_zero_is_scaled_Atb(amp):
     assert info.total_iters == 0 and info.outer_iters == 0
 
 
+@pytest.mark.parametrize("kind", ["zero", "constant", "constant magnitude"])
+def test_scaleb_degenerate_is_nan(kind):
+    """b_dif = 0 in scaleb (2d:256-258): scl = 0.5 / 0 = inf, and the scaled
+    b is 0 * inf = NaN (b = 0) or +-inf, whose A^T product mixes signs into
+    NaN; the unscaled U = NaN / inf stays NaN.  With iterations the NaN
+    comparisons all fail: no backtracking, no inner or outer stop, so the
+    solve runs to maxit in one outer iteration."""
+    A, x, b = _dense(12, 10, 0.5, 2)
+    b = {"zero": np.zeros_like(b), "constant": np.full_like(b, 3.0),
+         "constant magnitude": np.where(np.arange(b.size) % 2, 0.4, -0.4).astype(np.float32)}[kind]
+    a = np.abs(b)
+    assert b[np.argmax(a)] == b[np.argmin(a)]  # b_dif = 0
+    for maxit in (0, 25):
+        opts = default_options(maxit=maxit, scal_double=1)
+        U, info, tr = O.solve(O.Op.dense(A), b, (12, 10, 1), opts, trace_cap=32)
+        assert np.isnan(U).all()
+        assert info.total_iters == maxit and info.outer_iters == (1 if maxit else 0)
+        assert [(t.backtracks, t.fallback) for t in tr] == [(0, 0)] * maxit
+
+
 @pytest.mark.parametrize("sd", [0, 1])
 def test_objective_bookkeeping(sd):
     """f is maintained incrementally (update_W subtracts and re-adds its terms,
