@@ -301,7 +301,7 @@ int tv3_gemv(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
  * kernels and results as tv3_gemv; it calls no hipMalloc / hipFree, no
  * memset and no host copy.  Its group descriptor, instance record and, for
  * a dense A^T x, the 16 * n FP64 split-K partial sums live in one fixed
- * device array per GPU (512 B + 8 MiB, held until the process ends), so a
+ * device array per GPU (1 KiB + 8 MiB, held until the process ends), so a
  * dense A^T x takes at most n = 65536 columns (TV3_ERR_ARG beyond that);
  * A x of any dense plan and sparse or geometry plans of any size take no
  * partials.  Every call on one GPU shares that array: calls that overlap

@@ -7,6 +7,16 @@
 
 namespace tv3 {
 
+// Load through a pointer that a kernel read from a descriptor (a plan's
+// arrays).  The compiler cannot tell that such a pointer addresses global
+// memory and emits a flat load, which also counts against the LDS counter:
+// every later LDS access or cross-lane shuffle then waits for it.  Through
+// the global address space it is an ordinary vector load.
+template <class T>
+__device__ __forceinline__ T ldg(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+
 // ---- Bresenham closed form --------------------------------------------------
 // Number of minor-axis steps after j major steps of the reference's error
 // recurrence (3d/mat_mult.h:241-359: error starts at 2*m - L, a minor step

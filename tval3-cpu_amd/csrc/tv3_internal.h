@@ -122,6 +122,19 @@ struct PlanDev {
   // ray crosses come last, from slot t_nz on
   const int *t_ptr, *t_ray, *t_col;
   int t_nz;
+  // the staged transpose of the single-instance kernels (k_bwd_w1s): slot
+  // s < s_nz is voxel s_col[s] (bricks of the volume), the voxels no ray
+  // crosses follow from s_nz on.  Tile t is slots [t * s_V, (t + 1) * s_V);
+  // its rays s_ray[s_rptr[t] .. s_rptr[t + 1]) (ascending) are staged in
+  // LDS.  Slot s's list is s_loc[s_ptr[s] .. s_ptr[s + 1]): the voxel's
+  // g_cray list in order, each ray given by its position among the tile's
+  // rays.  A tile crossed by more than kStageR rays stages none
+  // (s_rptr[t] == s_rptr[t + 1]) and reads g_cray.  s_col null: no staging.
+  const int *s_col, *s_ptr, *s_rptr, *s_ray;
+  const unsigned short *s_loc;
+  int s_nz, s_V;
+  long long s_rays;  // s_rptr[tiles]
+  long long s_dnnz;  // list entries of the tiles that stage no rays
   float scale;
 };
 
@@ -274,6 +287,8 @@ struct tv3_plan {
   tv3::RayDesc *rays = nullptr;
   int *cptr = nullptr, *cray = nullptr, *order = nullptr;
   int *tptr = nullptr, *tray = nullptr, *tcol = nullptr;
+  int *scol = nullptr, *sptr = nullptr, *srptr = nullptr, *sray = nullptr;
+  unsigned short *sloc = nullptr;
   std::vector<long long> walk_ptr;  // host: start of each row's walk
 };
 

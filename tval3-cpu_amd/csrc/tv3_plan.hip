@@ -111,7 +111,15 @@ __host__ __device__ inline unsigned long long prof_op_bytes(const PlanDev &pd, i
     return (4 + vs) * nnz + 4 * ((trans ? n : m) + 1);
   }
   // geometry: 32-byte RayDesc per ray (and the 64-lane kernel's ray order),
-  // or t_ptr, t_col and t_ray of the transpose
+  // or t_ptr, t_col and t_ray of the transpose; the staged single-instance
+  // transpose reads s_col, s_ptr, 2-byte s_loc entries (4-byte g_cray
+  // entries in tiles that stage nothing), the tiles' ray lists and their
+  // pointers instead
+  if (trans && W == 1 && pd.s_col) {
+    const unsigned long long tiles = (pd.s_nz + pd.s_V - 1) / pd.s_V;
+    return 2ull * nnz + 2ull * (unsigned long long)pd.s_dnnz + 4ull * n + 4ull * (pd.s_nz + 1) +
+           4ull * (tiles + 1) + 4ull * (unsigned long long)pd.s_rays;
+  }
   return trans ? 4ull * nnz + 8ull * n + 4ull : 32ull * m + (W == 64 ? 4ull * m : 0ull);
 }
 __host__ __device__ inline unsigned long long prof_vec_bytes(const GroupDesc &gd, int vbytes) {
@@ -678,6 +686,172 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
     if (sub) v = v - sub[o];
     y[o] = v;
   }
+}
+
+// Staged transpose of a geometry plan (PlanDev::s_*).  A block takes one tile
+// of kStageV voxel slots, bricks of the volume.  It gathers the y entries of
+// the tile's rays into LDS, one load per ray instead of one per list entry
+// (a ray through a brick crosses several of its voxels: on the 128^3
+// hemisphere 8.8 M loads replace 32.4 M), kStageLd loads in flight per
+// thread.  L lanes per voxel then sum the voxel's list from LDS exactly as
+// k_bwd_w1 sums it (lane sl takes entries sl, sl + L, ... in order, the same
+// butterfly adds the lanes), so the two kernels give the same bits.  The
+// list bounds of all kStageP passes are loaded up front, and the first
+// kStageI entries per lane of the next pass's voxel load while the current
+// pass sums (longer lists load the rest as they go), so a pass waits for
+// memory about once.  A tile crossed by more than kStageR rays gathers from
+// y directly (its voxels' lists in g_cray).  The blocks after a group's
+// tiles write the voxels no ray crosses, kBlock per block, so
+// ceil(max_n / kBlock) + 1 blocks per group cover tiles and those blocks.
+// (Geometry plans take no layers: gd.n == P.n.)
+constexpr int kStageR = 8192;                          // rays staged per tile: 32 KiB of LDS
+constexpr int kStageLd = 8;                            // staging loads in flight per thread
+constexpr int kStageV = kBlock;                        // voxel slots per tile
+constexpr int kStageP = kStageV / (kBlock / kSubT);    // passes per tile
+constexpr int kStageI = 16;                            // entries per lane loaded ahead
+
+// list bounds of the kStageP passes and the first entries of pass 0 (issued
+// before the staging loads, so they are in flight together)
+struct StagePre {
+  int vv[kStageP], kb[kStageP], ke[kStageP];
+  int cur[kStageI];
+};
+
+template <int L, bool ST, class I>
+__device__ __forceinline__ void stage_pre(StagePre &q, const int *scol, const int *ptr, const I *ind,
+                                          int s_beg, int s_end) {
+  const int sl = threadIdx.x & (L - 1);
+  const int s_me = s_beg + (int)threadIdx.x / L;
+#pragma unroll
+  for (int p = 0; p < kStageP; ++p) {
+    const int s = s_me + p * (kBlock / L);
+    q.vv[p] = s < s_end ? ldg(scol + s) : 0;
+  }
+#pragma unroll
+  for (int p = 0; p < kStageP; ++p) {
+    const int s = s_me + p * (kBlock / L);
+    const int i = ST ? s : q.vv[p];
+    q.kb[p] = s < s_end ? ldg(ptr + i) : 0;
+    q.ke[p] = s < s_end ? ldg(ptr + i + 1) : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kStageI; ++j) {
+    const int k = q.kb[0] + sl + j * L;
+    q.cur[j] = k < q.ke[0] ? (int)ldg(ind + k) : -1;
+  }
+}
+
+template <int L, class I, class F>
+__device__ __forceinline__ void stage_sums(StagePre &q, const I *ind, F val, int s_beg, int s_end,
+                                           float sc, float *yg, const float *subg) {
+  static_assert(L == kSubT, "tile passes assume kSubT lanes per voxel");
+  constexpr int U = kW1Unroll;
+  const int sl = threadIdx.x & (L - 1);
+  const int s_me = s_beg + (int)threadIdx.x / L;
+  int nxt[kStageI];
+#pragma unroll
+  for (int p = 0; p < kStageP; ++p) {
+    if (s_beg + p * (kBlock / L) >= s_end) break;  // uniform: the tile's last pass
+    if (p + 1 < kStageP) {
+#pragma unroll
+      for (int j = 0; j < kStageI; ++j) {
+        const int k = q.kb[p + 1] + sl + j * L;
+        nxt[j] = k < q.ke[p + 1] ? (int)ldg(ind + k) : -1;
+      }
+    }
+    double acc = 0.0;
+    {
+      float w[kStageI];
+#pragma unroll
+      for (int j = 0; j < kStageI; ++j) w[j] = q.cur[j] >= 0 ? val(q.cur[j]) : 0.0f;
+#pragma unroll
+      for (int j = 0; j < kStageI; ++j)
+        if (q.cur[j] >= 0) acc += (double)(w[j] * sc);
+    }
+    for (int k = q.kb[p] + sl + kStageI * L; k < q.ke[p]; k += L * U) {
+      int id[U];
+      float w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) id[u] = k + u * L < q.ke[p] ? (int)ldg(ind + k + u * L) : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = id[u] >= 0 ? val(id[u]) : 0.0f;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (id[u] >= 0) acc += (double)(w[u] * sc);
+    }
+    acc = lane_sum<L>(acc);
+    if (sl == 0 && s_me + p * (kBlock / L) < s_end) {
+      const int v = q.vv[p];
+      float r = (float)acc;
+      if (subg) r = r - ldg(subg + v);
+      yg[v] = r;
+    }
+    if (p + 1 < kStageP) {
+#pragma unroll
+      for (int j = 0; j < kStageI; ++j) q.cur[j] = nxt[j];
+    }
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_bwd_w1s(const GroupDesc *G,
+                                                    const float *x, float *y,
+                                                    const InstScal *S,
+                                                    uint32_t mask,
+                                                    const float *sub) {
+  const GroupDesc &gd = G[blockIdx.y];
+  if (mask && (S[blockIdx.y].flags & mask) != mask) return;
+  const PlanDev &P = gd.plan;
+  const int n = P.n;
+  const int s_nz = P.s_nz;
+  const int T = (s_nz + kStageV - 1) / kStageV;
+  const int t = blockIdx.x;
+  float *yg = y + gd.off_n;
+  const float *subg = sub ? sub + gd.off_n : nullptr;
+  if (t >= T) {
+    const long long k = s_nz + (long long)(t - T) * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int v = ldg(P.s_col + k);
+    float r = 0.0f;
+    if (subg) r = r - ldg(subg + v);
+    yg[v] = r;
+    return;
+  }
+  __shared__ float ys[kStageR];
+  const int r0 = ldg(P.s_rptr + t), r1 = ldg(P.s_rptr + t + 1);
+  const int s_beg = t * kStageV, s_end = min(s_nz, s_beg + kStageV);
+  const float sc = P.scale;
+  const float *xg = x + gd.off_m;
+  StagePre q;
+  if (r1 > r0) {
+    stage_pre<L, true>(q, P.s_col, P.s_ptr, P.s_loc, s_beg, s_end);
+    for (int i0 = r0 + (int)threadIdx.x; i0 < r1; i0 += kBlock * kStageLd) {
+      int id[kStageLd];
+      float w[kStageLd];
+#pragma unroll
+      for (int u = 0; u < kStageLd; ++u) id[u] = i0 + u * kBlock < r1 ? ldg(P.s_ray + i0 + u * kBlock) : -1;
+#pragma unroll
+      for (int u = 0; u < kStageLd; ++u) w[u] = id[u] >= 0 ? ldg(xg + id[u]) : 0.0f;
+#pragma unroll
+      for (int u = 0; u < kStageLd; ++u)
+        if (id[u] >= 0) ys[i0 + u * kBlock - r0] = w[u];
+    }
+    __syncthreads();
+    stage_sums<L>(q, P.s_loc, [&](int i) { return ys[i]; }, s_beg, s_end, sc, yg, subg);
+  } else {
+    stage_pre<L, false>(q, P.s_col, P.g_cptr, P.g_cray, s_beg, s_end);
+    stage_sums<L>(q, P.g_cray, [&](int i) { return ldg(xg + i); }, s_beg, s_end, sc, yg, subg);
+  }
+}
+
+// TV3_STAGE=0 keeps the unstaged single-instance transpose (k_bwd_w1) for
+// geometry plans; both give the same results
+bool stage_enabled() {
+  static const bool v = [] {
+    const char *e = std::getenv("TV3_STAGE");
+    return !(e && e[0] == '0');
+  }();
+  return v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_bwd_w64(const GroupDesc *G,
@@ -1609,6 +1783,11 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
     dim3 g2((max_n + kBlock - 1) / kBlock, G);
     hipLaunchKernelGGL(k_dense_bwd_reduce, g2, dim3(kBlock), 0, s, G_dev,
                        (const double *)scratch, y, S, mask, sub, sstride);
+    return;
+  }
+  if (kind == kGeometry && stage_enabled()) {
+    dim3 grid((max_n + kBlock - 1) / kBlock + 1, G);
+    hipLaunchKernelGGL(k_bwd_w1s<kSubT>, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
     return;
   }
   // one block more than the columns need: the voxels no ray crosses take
@@ -2698,12 +2877,85 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
     TV3_TRY(hipMemcpyAsync(p->tray, tray.data(), sizeof(int) * tray.size(), hipMemcpyHostToDevice, s), "copy tray");
     TV3_TRY(hipMemcpyAsync(p->tcol, corder.data(), sizeof(int) * nvox, hipMemcpyHostToDevice, s), "copy tcol");
     TV3_TRY(hipStreamSynchronize(s), "plan sync");
+    // staged transpose (k_bwd_w1s): the crossed voxels brick by brick (8^3
+    // bricks of a volume, 16^2 squares of a slice), then the uncrossed ones;
+    // tiles of V slots, each with its ascending ray list, and the slots'
+    // lists in slot order, each entry the ray's position in its tile's list
+    {
+      const bool vol = g->rv_z > 1;
+      const int b = vol ? 8 : 16, bz = vol ? 8 : 1;
+      const int V = kStageV;
+      std::vector<int> scol;
+      scol.reserve(nvox);
+      for (int z0 = 0; z0 < g->rv_z; z0 += bz)
+        for (int y0 = 0; y0 < g->rv_y; y0 += b)
+          for (int x0 = 0; x0 < g->rv_x; x0 += b)
+            for (int z = z0; z < std::min(g->rv_z, z0 + bz); ++z)
+              for (int y = y0; y < std::min(g->rv_y, y0 + b); ++y)
+                for (int x = x0; x < std::min(g->rv_x, x0 + b); ++x) {
+                  const int v = x * sx + y * sy + z * sz;
+                  if (cptr[v + 1] > cptr[v]) scol.push_back(v);
+                }
+      const int s_nz = (int)scol.size();
+      for (long long v = 0; v < nvox; ++v)
+        if (cptr[v + 1] == cptr[v]) scol.push_back((int)v);
+      const int T = (s_nz + V - 1) / V;
+      std::vector<int> sptr(s_nz + 1, 0), srptr(T + 1, 0), sray, tl, pos(m, 0), seen(m, -1);
+      std::vector<unsigned short> sloc;
+      sloc.reserve(std::max<long long>(nnz, 1));
+      long long dnnz = 0;
+      for (int t = 0; t < T; ++t) {
+        tl.clear();
+        const int s1 = std::min(s_nz, (t + 1) * V);
+        for (int k = t * V; k < s1; ++k) {
+          const int v = scol[k];
+          for (int e = cptr[v]; e < cptr[v + 1]; ++e)
+            if (seen[cray[e]] != t) {
+              seen[cray[e]] = t;
+              tl.push_back(cray[e]);
+            }
+        }
+        const bool st = (int)tl.size() <= kStageR;
+        if (st) {
+          std::sort(tl.begin(), tl.end());
+          for (int i = 0; i < (int)tl.size(); ++i) pos[tl[i]] = i;
+          sray.insert(sray.end(), tl.begin(), tl.end());
+        }
+        srptr[t + 1] = (int)sray.size();
+        for (int k = t * V; k < s1; ++k) {
+          const int v = scol[k];
+          for (int e = cptr[v]; e < cptr[v + 1]; ++e)
+            sloc.push_back(st ? (unsigned short)pos[cray[e]] : (unsigned short)0);
+          if (!st) dnnz += cptr[v + 1] - cptr[v];
+          sptr[k + 1] = (int)sloc.size();
+        }
+      }
+      if (sray.empty()) sray.push_back(0);
+      if (sloc.empty()) sloc.push_back(0);
+      dv.s_nz = s_nz;
+      dv.s_V = V;
+      dv.s_rays = srptr[T];
+      dv.s_dnnz = dnnz;
+      TV3_TRY(hipMalloc(&p->scol, sizeof(int) * nvox), "hipMalloc scol");
+      TV3_TRY(hipMalloc(&p->sptr, sizeof(int) * (s_nz + 1)), "hipMalloc sptr");
+      TV3_TRY(hipMalloc(&p->srptr, sizeof(int) * (T + 1)), "hipMalloc srptr");
+      TV3_TRY(hipMalloc(&p->sray, sizeof(int) * sray.size()), "hipMalloc sray");
+      TV3_TRY(hipMalloc(&p->sloc, sizeof(unsigned short) * sloc.size()), "hipMalloc sloc");
+      TV3_TRY(hipMemcpyAsync(p->scol, scol.data(), sizeof(int) * nvox, hipMemcpyHostToDevice, s), "copy scol");
+      TV3_TRY(hipMemcpyAsync(p->sptr, sptr.data(), sizeof(int) * (s_nz + 1), hipMemcpyHostToDevice, s), "copy sptr");
+      TV3_TRY(hipMemcpyAsync(p->srptr, srptr.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice, s), "copy srptr");
+      TV3_TRY(hipMemcpyAsync(p->sray, sray.data(), sizeof(int) * sray.size(), hipMemcpyHostToDevice, s), "copy sray");
+      TV3_TRY(hipMemcpyAsync(p->sloc, sloc.data(), sizeof(unsigned short) * sloc.size(), hipMemcpyHostToDevice, s),
+              "copy sloc");
+      TV3_TRY(hipStreamSynchronize(s), "plan sync");
+    }
   }
 #undef TV3_TRY
   (void)hipFree(d_ptr);
   (void)hipFree(d_lin);
   dv.rays = p->rays; dv.g_cptr = p->cptr; dv.g_cray = p->cray; dv.ray_order = p->order;
   dv.t_ptr = p->tptr; dv.t_ray = p->tray; dv.t_col = p->tcol;
+  dv.s_col = p->scol; dv.s_ptr = p->sptr; dv.s_rptr = p->srptr; dv.s_ray = p->sray; dv.s_loc = p->sloc;
   *out = p;
   return TV3_OK;
 fail:
@@ -2714,6 +2966,11 @@ fail:
   if (p->tptr) (void)hipFree(p->tptr);
   if (p->tray) (void)hipFree(p->tray);
   if (p->tcol) (void)hipFree(p->tcol);
+  if (p->scol) (void)hipFree(p->scol);
+  if (p->sptr) (void)hipFree(p->sptr);
+  if (p->srptr) (void)hipFree(p->srptr);
+  if (p->sray) (void)hipFree(p->sray);
+  if (p->sloc) (void)hipFree(p->sloc);
   if (p->cptr) (void)hipFree(p->cptr);
   if (p->cray) (void)hipFree(p->cray);
   delete p;
@@ -2728,6 +2985,11 @@ void tv3_plan_destroy(tv3_plan *p) {
   if (p->tptr) (void)hipFree(p->tptr);
   if (p->tray) (void)hipFree(p->tray);
   if (p->tcol) (void)hipFree(p->tcol);
+  if (p->scol) (void)hipFree(p->scol);
+  if (p->sptr) (void)hipFree(p->sptr);
+  if (p->srptr) (void)hipFree(p->srptr);
+  if (p->sray) (void)hipFree(p->sray);
+  if (p->sloc) (void)hipFree(p->sloc);
   if (p->cptr) (void)hipFree(p->cptr);
   if (p->cray) (void)hipFree(p->cray);
   delete p;

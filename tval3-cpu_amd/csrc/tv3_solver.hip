@@ -2597,9 +2597,9 @@ GroupDesc vol_desc(int q, int p, int r) {
 // host-side stand-in: only the per-device address from hipGetSymbolAddress
 // (g_noalloc_ptr) may reach a kernel or a copy.
 constexpr int kNoAllocMaxN = 65536;                 // columns of a dense A^T x
-constexpr size_t kNoAllocHead = 512;                // GroupDesc, then the instance record
+constexpr size_t kNoAllocHead = 1024;               // GroupDesc, then the instance record
 constexpr size_t kNoAllocBytes = kNoAllocHead + sizeof(double) * kDenseChunks * (size_t)kNoAllocMaxN;
-static_assert(sizeof(GroupDesc) <= 256 && sizeof(Scal) <= 256, "no-alloc head layout");
+static_assert(sizeof(GroupDesc) <= kNoAllocHead / 2 && sizeof(Scal) <= kNoAllocHead / 2, "no-alloc head layout");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kNoAllocDevices = 64;
@@ -2778,7 +2778,7 @@ int tv3_gemv_noalloc(tv3_ctx *ctx, const tv3_plan *A, int trans, const float *x,
   if (rc) return rc;
   char *base = (char *)g_noalloc_ptr[ctx->device];
   GroupDesc *G = (GroupDesc *)base;
-  Scal *S = (Scal *)(base + 256);
+  Scal *S = (Scal *)(base + kNoAllocHead / 2);
   double *part = (double *)(base + kNoAllocHead);
   GroupDesc gd = vol_desc(A->dev.n, 1, 1);
   gd.m = A->dev.m;
