@@ -24,34 +24,38 @@ def _ring(grid=48, seed=3):
 
 def _staged_transpose_bytes(geo, dims):
     """Operator bytes of the single-instance staged transpose: per tile of
-    256 slots (16 x 16 squares of a slice, 8^3 bricks of a volume, crossed
-    voxels only) its ascending ray list when at most 8192 rays cross it, the
-    slots' lists as 2-byte positions in those lists (4-byte ray ids in tiles
-    that stage none), s_col, s_ptr and the tile pointers.  Restated here
-    from the plan's definition, so the byte count also checks the tiles the
-    library built."""
+    256 slots (16 x 16 squares of a slice, 8^3 bricks of a volume, voxels
+    with 1 to 128 list entries) its ascending ray list when at most 8192
+    rays cross it, the slots' lists as 2-byte positions in those lists
+    (4-byte ray ids in tiles that stage none and in the longer lists, whose
+    voxels also read their two list bounds), s_col, s_ptr and the tile
+    pointers.  Restated here from the plan's definition, so the byte count
+    also checks the tiles the library built."""
     op = O.Op.geometry(geo)
     ptr, lin = op.walk()
     m, nnz = len(ptr) - 1, len(lin)
     X, Y, Z = dims
     n = X * Y * Z
-    b, bz, V = (8, 8, 256) if Z > 1 else (16, 1, 256)
+    b, bz, V, H = (8, 8, 256, 128) if Z > 1 else (16, 1, 256, 128)
     cnt = np.bincount(lin, minlength=n)
     v = np.arange(n)
     x, y, z = v % X, (v // X) % Y, v // (X * Y)
     key = np.lexsort((x, y, z, x // b, y // b, z // bz))
-    nz = key[cnt[key] > 0]
-    slot = np.empty(n, np.int64)
+    nz = key[(cnt[key] > 0) & (cnt[key] <= H)]
+    heavy = cnt > H
+    slot = np.full(n, -1, np.int64)
     slot[nz] = np.arange(len(nz))
     ray = np.repeat(np.arange(m, dtype=np.int64), np.diff(ptr))
-    tile = slot[lin] // V
+    sel = slot[lin] >= 0
+    tile = slot[lin][sel] // V
     T = (len(nz) + V - 1) // V
-    per_tile = np.bincount(np.unique(tile * m + ray) // m, minlength=T)
+    per_tile = np.bincount(np.unique(tile * m + ray[sel]) // m, minlength=T)
     ent = np.bincount(tile, minlength=T)
     staged = per_tile <= 8192
     s_rays = int(per_tile[staged].sum())
-    dnnz = int(ent[~staged].sum())
-    return 2 * nnz + 2 * dnnz + 4 * n + 4 * (len(nz) + 1) + 4 * (T + 1) + 4 * s_rays
+    dnnz = int(ent[~staged].sum()) + int(cnt[heavy].sum())
+    return (2 * nnz + 2 * dnnz + 4 * n + 4 * (len(nz) + 1) + 8 * int(heavy.sum()) + 4 * (T + 1)
+            + 4 * s_rays)
 
 
 @pytest.mark.parametrize("ls", [1, 0])

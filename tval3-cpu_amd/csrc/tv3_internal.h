@@ -123,8 +123,10 @@ struct PlanDev {
   const int *t_ptr, *t_ray, *t_col;
   int t_nz;
   // the staged transpose of the single-instance kernels (k_bwd_w1s): slot
-  // s < s_nz is voxel s_col[s] (bricks of the volume), the voxels no ray
-  // crosses follow from s_nz on.  Tile t is slots [t * s_V, (t + 1) * s_V);
+  // s < s_nz is voxel s_col[s] (bricks of the volume; lists of at most
+  // kStageH entries), the s_nh voxels with longer lists follow (longest
+  // first), then the voxels no ray crosses.  Tile t is slots
+  // [t * s_V, (t + 1) * s_V);
   // its rays s_ray[s_rptr[t] .. s_rptr[t + 1]) (ascending) are staged in
   // LDS.  Slot s's list is s_loc[s_ptr[s] .. s_ptr[s + 1]): the voxel's
   // g_cray list in order, each ray given by its position among the tile's
@@ -132,9 +134,10 @@ struct PlanDev {
   // (s_rptr[t] == s_rptr[t + 1]) and reads g_cray.  s_col null: no staging.
   const int *s_col, *s_ptr, *s_rptr, *s_ray;
   const unsigned short *s_loc;
-  int s_nz, s_V;
+  int s_nz, s_nh, s_V;
   long long s_rays;  // s_rptr[tiles]
-  long long s_dnnz;  // list entries of the tiles that stage no rays
+  long long s_dnnz;  // list entries read as ray ids: the longer lists and
+                     // the tiles that stage no rays
   float scale;
 };
 

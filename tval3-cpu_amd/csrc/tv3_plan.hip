@@ -118,7 +118,7 @@ __host__ __device__ inline unsigned long long prof_op_bytes(const PlanDev &pd, i
   if (trans && W == 1 && pd.s_col) {
     const unsigned long long tiles = (pd.s_nz + pd.s_V - 1) / pd.s_V;
     return 2ull * nnz + 2ull * (unsigned long long)pd.s_dnnz + 4ull * n + 4ull * (pd.s_nz + 1) +
-           4ull * (tiles + 1) + 4ull * (unsigned long long)pd.s_rays;
+           8ull * pd.s_nh + 4ull * (tiles + 1) + 4ull * (unsigned long long)pd.s_rays;
   }
   return trans ? 4ull * nnz + 8ull * n + 4ull : 32ull * m + (W == 64 ? 4ull * m : 0ull);
 }
@@ -692,33 +692,39 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
 // of kStageV voxel slots, bricks of the volume.  It gathers the y entries of
 // the tile's rays into LDS, one load per ray instead of one per list entry
 // (a ray through a brick crosses several of its voxels: on the 128^3
-// hemisphere 8.8 M loads replace 32.4 M), kStageLd loads in flight per
-// thread.  L lanes per voxel then sum the voxel's list from LDS exactly as
-// k_bwd_w1 sums it (lane sl takes entries sl, sl + L, ... in order, the same
-// butterfly adds the lanes), so the two kernels give the same bits.  The
-// list bounds of all kStageP passes are loaded up front, and the first
-// kStageI entries per lane of the next pass's voxel load while the current
-// pass sums (longer lists load the rest as they go), so a pass waits for
-// memory about once.  A tile crossed by more than kStageR rays gathers from
-// y directly (its voxels' lists in g_cray).  The blocks after a group's
-// tiles write the voxels no ray crosses, kBlock per block, so
-// ceil(max_n / kBlock) + 1 blocks per group cover tiles and those blocks.
+// hemisphere 8.5 M loads replace 32.4 M), kStageLd loads in flight per
+// thread.  L lanes per voxel then sum the voxel's list from LDS as k_bwd_w1
+// sums it (lane sl takes entries sl, sl + L, ... in order, the same
+// butterfly adds the lanes).  The list bounds of all kStageP passes are
+// loaded up front, and the first NI entries per lane of the next pass's
+// voxel load while the current pass sums, so a pass waits for memory about
+// once.  A tile crossed by more than NR rays gathers from y directly (its
+// voxels' lists in g_cray).  Lists longer than kStageH entries (the voxels
+// at the transducers: about a thousand rays each on the hemisphere) would
+// hold their wavefront for dozens of dependent loads; they go to
+// kHeavyBlocks blocks instead, a wavefront per voxel with 64 lanes on its
+// list.  The blocks after those write the voxels no ray crosses, kBlock per
+// block, so ceil(max_n / kBlock) + 1 + kHeavyBlocks blocks per group cover
+// everything.  Sums are FP64 and rounded once, as in every other transpose.
 // (Geometry plans take no layers: gd.n == P.n.)
-constexpr int kStageR = 8192;                          // rays staged per tile: 32 KiB of LDS
+constexpr int kStageR = 8192;                          // most rays staged per tile (32 KiB of LDS)
 constexpr int kStageLd = 8;                            // staging loads in flight per thread
 constexpr int kStageV = kBlock;                        // voxel slots per tile
 constexpr int kStageP = kStageV / (kBlock / kSubT);    // passes per tile
-constexpr int kStageI = 16;                            // entries per lane loaded ahead
+constexpr int kStageI = 16;                            // most entries per lane loaded ahead
+constexpr int kStageH = kStageI * kSubT;               // longest list of a tile slot
+constexpr int kHeavyBlocks = 512;                      // blocks that take the longer lists
 
 // list bounds of the kStageP passes and the first entries of pass 0 (issued
 // before the staging loads, so they are in flight together)
+template <int NI>
 struct StagePre {
   int vv[kStageP], kb[kStageP], ke[kStageP];
-  int cur[kStageI];
+  int cur[NI];
 };
 
-template <int L, bool ST, class I>
-__device__ __forceinline__ void stage_pre(StagePre &q, const int *scol, const int *ptr, const I *ind,
+template <int L, int NI, bool ST, class I>
+__device__ __forceinline__ void stage_pre(StagePre<NI> &q, const int *scol, const int *ptr, const I *ind,
                                           int s_beg, int s_end) {
   const int sl = threadIdx.x & (L - 1);
   const int s_me = s_beg + (int)threadIdx.x / L;
@@ -735,40 +741,40 @@ __device__ __forceinline__ void stage_pre(StagePre &q, const int *scol, const in
     q.ke[p] = s < s_end ? ldg(ptr + i + 1) : 0;
   }
 #pragma unroll
-  for (int j = 0; j < kStageI; ++j) {
+  for (int j = 0; j < NI; ++j) {
     const int k = q.kb[0] + sl + j * L;
     q.cur[j] = k < q.ke[0] ? (int)ldg(ind + k) : -1;
   }
 }
 
-template <int L, class I, class F>
-__device__ __forceinline__ void stage_sums(StagePre &q, const I *ind, F val, int s_beg, int s_end,
+template <int L, int NI, class I, class F>
+__device__ __forceinline__ void stage_sums(StagePre<NI> &q, const I *ind, F val, int s_beg, int s_end,
                                            float sc, float *yg, const float *subg) {
   static_assert(L == kSubT, "tile passes assume kSubT lanes per voxel");
   constexpr int U = kW1Unroll;
   const int sl = threadIdx.x & (L - 1);
   const int s_me = s_beg + (int)threadIdx.x / L;
-  int nxt[kStageI];
+  int nxt[NI];
 #pragma unroll
   for (int p = 0; p < kStageP; ++p) {
     if (s_beg + p * (kBlock / L) >= s_end) break;  // uniform: the tile's last pass
     if (p + 1 < kStageP) {
 #pragma unroll
-      for (int j = 0; j < kStageI; ++j) {
+      for (int j = 0; j < NI; ++j) {
         const int k = q.kb[p + 1] + sl + j * L;
         nxt[j] = k < q.ke[p + 1] ? (int)ldg(ind + k) : -1;
       }
     }
     double acc = 0.0;
     {
-      float w[kStageI];
+      float w[NI];
 #pragma unroll
-      for (int j = 0; j < kStageI; ++j) w[j] = q.cur[j] >= 0 ? val(q.cur[j]) : 0.0f;
+      for (int j = 0; j < NI; ++j) w[j] = q.cur[j] >= 0 ? val(q.cur[j]) : 0.0f;
 #pragma unroll
-      for (int j = 0; j < kStageI; ++j)
+      for (int j = 0; j < NI; ++j)
         if (q.cur[j] >= 0) acc += (double)(w[j] * sc);
     }
-    for (int k = q.kb[p] + sl + kStageI * L; k < q.ke[p]; k += L * U) {
+    for (int k = q.kb[p] + sl + NI * L; k < q.ke[p]; k += L * U) {
       int id[U];
       float w[U];
 #pragma unroll
@@ -788,12 +794,12 @@ __device__ __forceinline__ void stage_sums(StagePre &q, const I *ind, F val, int
     }
     if (p + 1 < kStageP) {
 #pragma unroll
-      for (int j = 0; j < kStageI; ++j) q.cur[j] = nxt[j];
+      for (int j = 0; j < NI; ++j) q.cur[j] = nxt[j];
     }
   }
 }
 
-template <int L>
+template <int L, int NI, int NR>
 __global__ __launch_bounds__(kBlock) void k_bwd_w1s(const GroupDesc *G,
                                                     const float *x, float *y,
                                                     const InstScal *S,
@@ -808,8 +814,10 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1s(const GroupDesc *G,
   const int t = blockIdx.x;
   float *yg = y + gd.off_n;
   const float *subg = sub ? sub + gd.off_n : nullptr;
-  if (t >= T) {
-    const long long k = s_nz + (long long)(t - T) * kBlock + threadIdx.x;
+  const float sc = P.scale;
+  const float *xg = x + gd.off_m;
+  if (t >= T + kHeavyBlocks) {
+    const long long k = s_nz + P.s_nh + (long long)(t - T - kHeavyBlocks) * kBlock + threadIdx.x;
     if (k >= n) return;
     const int v = ldg(P.s_col + k);
     float r = 0.0f;
@@ -817,14 +825,42 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1s(const GroupDesc *G,
     yg[v] = r;
     return;
   }
-  __shared__ float ys[kStageR];
+  if (t >= T) {
+    // the voxels with lists longer than kStageH, longest first: a wavefront
+    // per voxel, lane l takes entries l, l + 64, ..., the wave's lanes are
+    // added by a butterfly
+    const int lane = threadIdx.x & 63;
+    for (int h = (t - T) * (kBlock / 64) + (int)threadIdx.x / 64; h < P.s_nh;
+         h += kHeavyBlocks * (kBlock / 64)) {
+      const int v = ldg(P.s_col + s_nz + h);
+      const int k0 = ldg(P.g_cptr + v), k1 = ldg(P.g_cptr + v + 1);
+      double acc = 0.0;
+      for (int k = k0 + lane; k < k1; k += 64 * kStageI) {
+        int id[kStageI];
+        float w[kStageI];
+#pragma unroll
+        for (int u = 0; u < kStageI; ++u) id[u] = k + 64 * u < k1 ? ldg(P.g_cray + k + 64 * u) : -1;
+#pragma unroll
+        for (int u = 0; u < kStageI; ++u) w[u] = id[u] >= 0 ? ldg(xg + id[u]) : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kStageI; ++u)
+          if (id[u] >= 0) acc += (double)(w[u] * sc);
+      }
+      acc = lane_sum<64>(acc);
+      if (lane == 0) {
+        float r = (float)acc;
+        if (subg) r = r - ldg(subg + v);
+        yg[v] = r;
+      }
+    }
+    return;
+  }
+  __shared__ float ys[NR];
   const int r0 = ldg(P.s_rptr + t), r1 = ldg(P.s_rptr + t + 1);
   const int s_beg = t * kStageV, s_end = min(s_nz, s_beg + kStageV);
-  const float sc = P.scale;
-  const float *xg = x + gd.off_m;
-  StagePre q;
+  StagePre<NI> q;
   if (r1 > r0) {
-    stage_pre<L, true>(q, P.s_col, P.s_ptr, P.s_loc, s_beg, s_end);
+    stage_pre<L, NI, true>(q, P.s_col, P.s_ptr, P.s_loc, s_beg, s_end);
     for (int i0 = r0 + (int)threadIdx.x; i0 < r1; i0 += kBlock * kStageLd) {
       int id[kStageLd];
       float w[kStageLd];
@@ -839,9 +875,30 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1s(const GroupDesc *G,
     __syncthreads();
     stage_sums<L>(q, P.s_loc, [&](int i) { return ys[i]; }, s_beg, s_end, sc, yg, subg);
   } else {
-    stage_pre<L, false>(q, P.s_col, P.g_cptr, P.g_cray, s_beg, s_end);
+    stage_pre<L, NI, false>(q, P.s_col, P.g_cptr, P.g_cray, s_beg, s_end);
     stage_sums<L>(q, P.g_cray, [&](int i) { return ldg(xg + i); }, s_beg, s_end, sc, yg, subg);
   }
+}
+
+// staged transpose shape: list entries per lane loaded ahead
+// (TV3_STAGE_DEPTH = 8, 12, 16) and the most rays a tile stages
+// (TV3_STAGE_RAYS = 6144 or 8192, read when a plan is built); neither
+// changes a result
+int stage_depth() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_STAGE_DEPTH");
+    const int d = e ? std::atoi(e) : 0;
+    return d > 0 ? d : kStageI;
+  }();
+  return v;
+}
+int stage_rays() {
+  static const int v = [] {
+    const char *e = std::getenv("TV3_STAGE_RAYS");
+    const int r = e ? std::atoi(e) : 0;
+    return r > 0 && r <= 6144 ? 6144 : kStageR;
+  }();
+  return v;
 }
 
 // TV3_STAGE=0 keeps the unstaged single-instance transpose (k_bwd_w1) for
@@ -1786,8 +1843,15 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
     return;
   }
   if (kind == kGeometry && stage_enabled()) {
-    dim3 grid((max_n + kBlock - 1) / kBlock + 1, G);
-    hipLaunchKernelGGL(k_bwd_w1s<kSubT>, grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub);
+    dim3 grid((max_n + kBlock - 1) / kBlock + 1 + kHeavyBlocks, G);
+    const int ni = stage_depth();
+#define TV3_W1S(NI_, NR_) hipLaunchKernelGGL((k_bwd_w1s<kSubT, NI_, NR_>), grid, dim3(kBlock), 0, s, G_dev, x, y, S, mask, sub)
+    if (stage_rays() <= 6144) {
+      if (ni <= 8) TV3_W1S(8, 6144); else if (ni <= 12) TV3_W1S(12, 6144); else TV3_W1S(16, 6144);
+    } else {
+      if (ni <= 8) TV3_W1S(8, 8192); else if (ni <= 12) TV3_W1S(12, 8192); else TV3_W1S(16, 8192);
+    }
+#undef TV3_W1S
     return;
   }
   // one block more than the columns need: the voxels no ray crosses take
@@ -2894,9 +2958,21 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
               for (int y = y0; y < std::min(g->rv_y, y0 + b); ++y)
                 for (int x = x0; x < std::min(g->rv_x, x0 + b); ++x) {
                   const int v = x * sx + y * sy + z * sz;
-                  if (cptr[v + 1] > cptr[v]) scol.push_back(v);
+                  const int len = cptr[v + 1] - cptr[v];
+                  if (len > 0 && len <= kStageH) scol.push_back(v);
                 }
       const int s_nz = (int)scol.size();
+      // the voxels with longer lists, longest first, then the uncrossed ones
+      std::vector<int> heavy;
+      long long hnnz = 0;
+      for (long long v = 0; v < nvox; ++v)
+        if (cptr[v + 1] - cptr[v] > kStageH) {
+          heavy.push_back((int)v);
+          hnnz += cptr[v + 1] - cptr[v];
+        }
+      std::stable_sort(heavy.begin(), heavy.end(),
+                       [&](int a, int c) { return cptr[a + 1] - cptr[a] > cptr[c + 1] - cptr[c]; });
+      scol.insert(scol.end(), heavy.begin(), heavy.end());
       for (long long v = 0; v < nvox; ++v)
         if (cptr[v + 1] == cptr[v]) scol.push_back((int)v);
       const int T = (s_nz + V - 1) / V;
@@ -2915,7 +2991,7 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
               tl.push_back(cray[e]);
             }
         }
-        const bool st = (int)tl.size() <= kStageR;
+        const bool st = (int)tl.size() <= stage_rays();
         if (st) {
           std::sort(tl.begin(), tl.end());
           for (int i = 0; i < (int)tl.size(); ++i) pos[tl[i]] = i;
@@ -2933,9 +3009,10 @@ int tv3_plan_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_plan **out) {
       if (sray.empty()) sray.push_back(0);
       if (sloc.empty()) sloc.push_back(0);
       dv.s_nz = s_nz;
+      dv.s_nh = (int)heavy.size();
       dv.s_V = V;
       dv.s_rays = srptr[T];
-      dv.s_dnnz = dnnz;
+      dv.s_dnnz = dnnz + hnnz;
       TV3_TRY(hipMalloc(&p->scol, sizeof(int) * nvox), "hipMalloc scol");
       TV3_TRY(hipMalloc(&p->sptr, sizeof(int) * (s_nz + 1)), "hipMalloc sptr");
       TV3_TRY(hipMalloc(&p->srptr, sizeof(int) * (T + 1)), "hipMalloc srptr");
