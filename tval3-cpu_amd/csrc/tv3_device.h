@@ -16,6 +16,19 @@ template <class T>
 __device__ __forceinline__ T ldg(const T *p) {
   return *(const __attribute__((address_space(1))) T *)p;
 }
+__device__ __forceinline__ RayDesc ldg(const RayDesc *p) {
+  const int *q = (const int *)p;
+  RayDesc r;
+  r.lin0 = ldg(q);
+  r.L = ldg(q + 1);
+  r.s_major = ldg(q + 2);
+  r.s_a = ldg(q + 3);
+  r.s_b = ldg(q + 4);
+  r.ma = ldg(q + 5);
+  r.mb = ldg(q + 6);
+  r.inv2L = __int_as_float(ldg(q + 7));
+  return r;
+}
 
 // ---- Bresenham closed form --------------------------------------------------
 // Number of minor-axis steps after j major steps of the reference's error

@@ -207,13 +207,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w1(const GroupDesc *G,
   // row of A and layer of this output row (one layer: the row itself; a
   // geometry row slot walks the plan's tiled ray order)
   int ar = row % P.m;
-  if (kW1TiledF && P.kind == kGeometry && P.ray_order && row < gd.m) ar = P.ray_order[row];
+  if (kW1TiledF && P.kind == kGeometry && P.ray_order && row < gd.m) ar = ldg(P.ray_order + row);
   const float *xg = x + gd.off_n + (long long)(row / P.m) * P.n;
   double acc = 0.0;
   if (row < gd.m) {
     constexpr int U = kW1Unroll;
     if (P.kind == kGeometry) {
-      const RayDesc rd = P.rays[ar];
+      const RayDesc rd = ldg(P.rays + ar);
       const float sc = P.scale;
       for (int j = sl; j <= rd.L; j += L * U) {
         float v[U];
@@ -227,15 +227,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w1(const GroupDesc *G,
           if (j + u * L <= rd.L) acc += (double)(v[u] * sc);
       }
     } else if (P.kind == kSparse) {
-      const int k0 = P.r_ptr[ar], k1 = P.r_ptr[ar + 1];
+      const int k0 = ldg(P.r_ptr + ar), k1 = ldg(P.r_ptr + ar + 1);
       for (int k = k0 + sl; k < k1; k += L * U) {
         int id[U];
         float w[U], v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int kk = k + u * L;
-          id[u] = kk < k1 ? P.r_ind[kk] : -1;
-          w[u] = kk < k1 ? P.r_val[kk] : 0.0f;
+          id[u] = kk < k1 ? ldg(P.r_ind + kk) : -1;
+          w[u] = kk < k1 ? ldg(P.r_val + kk) : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? xg[id[u]] : 0.0f;
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
   const int q = lane >> 4, c = lane & 15;
   const int slot = uni_i(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
   if (slot >= gd.m) return;
-  const int row = gd.plan.ray_order ? uni_i(gd.plan.ray_order[slot]) : slot;
+  const int row = gd.plan.ray_order ? uni_i(ldg(gd.plan.ray_order + slot)) : slot;
   const InstScal *Sg = S + blockIdx.y * 64 + 4 * c;
   bool on[4];
   bool any = false;
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
   }
   if (!__any(any)) return;
   const PlanDev &P = gd.plan;
-  const RayDesc rd = P.rays[row];
+  const RayDesc rd = ldg(P.rays + row);
   const int L = uni_i(rd.L);
   const float sc = P.scale;
   const bool unit = sc == 1.0f;
@@ -553,27 +553,27 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
   const bool unit = sc == 1.0f;
   const float *xb = x + gd.off_m + 4 * c;
   // voxel A: current, B: next (bounds known), C: after next
-  int kA0 = uni_i(P.t_ptr[s0]), kA1 = uni_i(P.t_ptr[s0 + 1]);
-  int colA = uni_i(P.t_col[s0]);
-  int idA = lane < min(64, kA1 - kA0) ? P.t_ray[kA0 + lane] : 0;
+  int kA0 = uni_i(ldg(P.t_ptr + (s0))), kA1 = uni_i(ldg(P.t_ptr + (s0 + 1)));
+  int colA = uni_i(ldg(P.t_col + (s0)));
+  int idA = lane < min(64, kA1 - kA0) ? ldg(P.t_ray + (kA0 + lane)) : 0;
   int kB0 = 0, kB1 = 0, colB = 0;
   if (ns > 1) {
-    kB0 = uni_i(P.t_ptr[s0 + 1]);
-    kB1 = uni_i(P.t_ptr[s0 + 2]);
-    colB = uni_i(P.t_col[s0 + 1]);
+    kB0 = uni_i(ldg(P.t_ptr + (s0 + 1)));
+    kB1 = uni_i(ldg(P.t_ptr + (s0 + 2)));
+    colB = uni_i(ldg(P.t_col + (s0 + 1)));
   }
   for (int i = 0; i < ns; ++i) {
-    const int idB = (i + 1 < ns && lane < min(64, kB1 - kB0)) ? P.t_ray[kB0 + lane] : 0;
+    const int idB = (i + 1 < ns && lane < min(64, kB1 - kB0)) ? ldg(P.t_ray + (kB0 + lane)) : 0;
     int kC0 = 0, kC1 = 0, colC = 0;
     if (i + 2 < ns) {
-      kC0 = uni_i(P.t_ptr[s0 + i + 2]);
-      kC1 = uni_i(P.t_ptr[s0 + i + 3]);
-      colC = uni_i(P.t_col[s0 + i + 2]);
+      kC0 = uni_i(ldg(P.t_ptr + (s0 + i + 2)));
+      kC1 = uni_i(ldg(P.t_ptr + (s0 + i + 3)));
+      colC = uni_i(ldg(P.t_col + (s0 + i + 2)));
     }
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     for (int kb = kA0; kb < kA1; kb += 64) {
       const int cnt = min(64, kA1 - kb);
-      const int id = kb == kA0 ? idA : (lane < cnt ? P.t_ray[kb + lane] : 0);
+      const int id = kb == kA0 ? idA : (lane < cnt ? ldg(P.t_ray + (kb + lane)) : 0);
       const int tmax = (cnt + 3) >> 2;
       for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
         float4 v[kPipeQ];
@@ -644,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
     if ((int)blockIdx.x >= nbA) {
       const long long k = P.t_nz + (long long)(blockIdx.x - nbA) * kBlock + threadIdx.x;
       if (k < P.n) {
-        const long long o = gd.off_n + P.t_col[k];
+        const long long o = gd.off_n + ldg(P.t_col + k);
         float v = 0.0f;
         if (sub) v = v - sub[o];
         y[o] = v;
@@ -660,8 +660,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
     constexpr int U = kW1Unroll;
     const int *ptr = tiled ? P.t_ptr : geo ? P.g_cptr : P.c_ptr;
     const int *ind = tiled ? P.t_ray : geo ? P.g_cray : P.c_ind;
-    const int k0 = ptr[ac], k1 = ptr[ac + 1];
-    if (tiled) ac = P.t_col[col];
+    const int k0 = ldg(ptr + ac), k1 = ldg(ptr + ac + 1);
+    if (tiled) ac = ldg(P.t_col + col);
     const float sc = P.scale;
     for (int k = k0 + sl; k < k1; k += L * U) {
       int id[U];
@@ -669,8 +669,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w1(const GroupDesc *G,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int kk = k + u * L;
-        id[u] = kk < k1 ? ind[kk] : -1;
-        w[u] = (!geo && kk < k1) ? P.c_val[kk] : 0.0f;
+        id[u] = kk < k1 ? ldg(ind + kk) : -1;
+        w[u] = (!geo && kk < k1) ? ldg(P.c_val + kk) : 0.0f;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? xg[id[u]] : 0.0f;
