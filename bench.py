@@ -66,6 +66,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# the single-instance projector transpose: the staged kernel unless
+# TV3_STAGE=0 selects the unstaged one
+W1_TRANSPOSE = "k_bwd_w1" if os.environ.get("TV3_STAGE", "1") == "0" else "k_bwd_w1s"
+
+
 def traffic_from_profiles(workload, kernel):
     """HBM bytes per launch of `kernel` from profiles/traffic_<workload>.json
     (written from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), or None."""
@@ -448,7 +453,7 @@ def run_diff(args, ctx):
     prof = ctx.profile_read()
     ctx.profile(False)
     args.one_step = grad_once
-    ops, roof = roofline(prof, {"A": "k_fwd_w1", "At": "k_bwd_w1"}, "diff", plan,
+    ops, roof = roofline(prof, {"A": "k_fwd_w1", "At": W1_TRANSPOSE}, "diff", plan,
                          t_steps, args.steps)
     roof["cache_resident"] = True
     cpu = None
@@ -503,7 +508,7 @@ def run_single(args, ctx, workload):
         dims = (grid, grid, grid)
         maxit = args.maxit if args.maxit_set else 100
         opts = tval3_amd.default_options(maxit=maxit, graph=args.graph)
-        kern = {"A": "k_fwd_w1", "At": "k_bwd_w1"}
+        kern = {"A": "k_fwd_w1", "At": W1_TRANSPOSE}
         op_fn = lambda: O.Op.geometry(geo)  # noqa: E731
         desc = f"3D USCT hemisphere {grid}^3, {plan.m} rays, maxit {maxit}"
     elif workload == "ring-f64":
