@@ -101,32 +101,54 @@ __device__ __forceinline__ bool sel(uint32_t f, uint32_t req, uint32_t forbid) {
 // Only lanes whose instance passes the (req, forbid) flag test write, so a
 // masked kernel never overwrites the partials another kernel left for the
 // instances it skips.
+//
+// LDS holds one value per (wavefront, slot) for W = 1; for W = 64 the four
+// wavefronts add into one array in turn (the same left-to-right order), so
+// the kernels' occupancy is not bounded by this scratch (NS * 512 bytes
+// instead of NS * 2 KiB).
 template <int W, int NS, class SC>
 __device__ __forceinline__ void block_partials(const double (&v)[NS],
                                                const int (&slot)[NS],
                                                double *part, int inst0,
                                                int blk, const SC *S,
                                                uint32_t req, uint32_t forbid) {
-  __shared__ double red[4][NS][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if constexpr (W == 64) {
+    __shared__ double acc[NS][64];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    double x = v[s];
-    if constexpr (W == 1) {
-      x = wave_total63(x);
-      if (lane == 63) red[wv][s][0] = x;
-    } else {
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (wv == w) {
 #pragma unroll
-      for (int off = 32; off >= W; off >>= 1) x += __shfl_xor(x, off, 64);
-      if (lane < W) red[wv][s][lane] = x;
+        for (int s = 0; s < NS; ++s) acc[s][lane] = w == 0 ? v[s] : acc[s][lane] + v[s];
+      }
+      __syncthreads();
     }
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < NS * W; t += kBlock) {
-    const int s = t / W, b = t % W;
-    if (!sel(S[inst0 + b].flags, req, forbid)) continue;
-    const double x = ((red[0][s][b] + red[1][s][b]) + red[2][s][b]) + red[3][s][b];
-    part[((long long)(inst0 + b) * kNslot + slot[s]) * kNblkMax + blk] = x;
+    for (int t = threadIdx.x; t < NS * W; t += kBlock) {
+      const int s = t / W, b = t % W;
+      if (!sel(S[inst0 + b].flags, req, forbid)) continue;
+      part[((long long)(inst0 + b) * kNslot + slot[s]) * kNblkMax + blk] = acc[s][b];
+    }
+  } else {
+    __shared__ double red[4][NS][W];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      double x = v[s];
+      if constexpr (W == 1) {
+        x = wave_total63(x);
+        if (lane == 63) red[wv][s][0] = x;
+      } else {
+#pragma unroll
+        for (int off = 32; off >= W; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane < W) red[wv][s][lane] = x;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NS * W; t += kBlock) {
+      const int s = t / W, b = t % W;
+      if (!sel(S[inst0 + b].flags, req, forbid)) continue;
+      const double x = ((red[0][s][b] + red[1][s][b]) + red[2][s][b]) + red[3][s][b];
+      part[((long long)(inst0 + b) * kNslot + slot[s]) * kNblkMax + blk] = x;
+    }
   }
 }
 
