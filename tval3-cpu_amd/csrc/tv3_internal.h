@@ -263,8 +263,13 @@ struct tv3_ctx {
   size_t pinned_bytes = 0;
   // per-instance control flags, written by the device straight into mapped
   // pinned host memory (read after a stream synchronization)
-  uint32_t *hflags = nullptr, *dflags = nullptr;
+  // published instance flags in mapped host memory (host and device views):
+  // one 64-bit word per instance, the flags in the low half and the number
+  // of the launch that wrote them in the high half; pub_epoch is the last
+  // number given out
+  unsigned long long *hflags = nullptr, *dflags = nullptr;
   size_t hflags_n = 0;
+  uint32_t pub_epoch = 0;
   // timing events for the benchmark (kernel-level, on this stream)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-launch timing (tv3_ctx_profile): level 1 times the A / A^T

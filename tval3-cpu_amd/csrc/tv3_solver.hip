@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -959,7 +960,8 @@ struct FA {
   int nblk_n, nblk_m;
   int W;
   uint32_t req, forbid;
-  uint32_t *fout;  // device view of the mapped host flag array (or NULL)
+  unsigned long long *fout;  // device view of the mapped host flag words (or NULL)
+  uint32_t epoch;             // this launch's number, written with the flags
   tv3_trace_rec *trace;
   int trace_cap;
   OptDev O;
@@ -1003,7 +1005,8 @@ __device__ __forceinline__ void fin_sums(const FA &F, int inst,
 
 __device__ __forceinline__ void put_flags(const FA &F, int inst, uint32_t f) {
   if (F.fout) {
-    F.fout[inst] = f;
+    // one 64-bit store: a host that reads the launch's number reads its flags
+    F.fout[inst] = ((unsigned long long)F.epoch << 32) | f;
     __threadfence_system();
   }
 }
@@ -1455,7 +1458,7 @@ __global__ __launch_bounds__(kFin) void fin_relerr(FA F) {
   FIN_END
 }
 
-__global__ void k_init_scal(Scal *S, int B, int Bpad, uint32_t *fout) {
+__global__ void k_init_scal(Scal *S, int B, int Bpad, unsigned long long *fout) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Bpad) return;
   Scal z;
@@ -1603,6 +1606,16 @@ long long vec_blocks() {
   const char *e = std::getenv("TV3_VEC_BLOCKS");
   const long long v = e ? std::atoll(e) : 0;
   return v > 0 ? v : 2048;
+}
+
+// TV3_SPIN=0: host-driven solves read the flags after a stream
+// synchronization instead of polling the published launch numbers
+bool spin_wait() {
+  static const bool v = [] {
+    const char *e = std::getenv("TV3_SPIN");
+    return !(e && e[0] == '0');
+  }();
+  return v;
 }
 
 bool autotune_enabled() {
@@ -1764,13 +1777,14 @@ class Engine {
         ctx_->hflags_n = 0;
       }
       void *h = nullptr;
-      int rc0 = hip_check(ctx_, hipHostMalloc(&h, sizeof(uint32_t) * bt_.Bpad, hipHostMallocMapped | hipHostMallocCoherent), "flag hipHostMalloc");
+      int rc0 = hip_check(ctx_, hipHostMalloc(&h, sizeof(unsigned long long) * bt_.Bpad, hipHostMallocMapped | hipHostMallocCoherent), "flag hipHostMalloc");
       if (rc0) return rc0;
       void *d = nullptr;
       rc0 = hip_check(ctx_, hipHostGetDevicePointer(&d, h, 0), "flag device pointer");
       if (rc0) { (void)hipHostFree(h); return rc0; }
-      ctx_->hflags = (uint32_t *)h;
-      ctx_->dflags = (uint32_t *)d;
+      std::memset(h, 0, sizeof(unsigned long long) * bt_.Bpad);
+      ctx_->hflags = (unsigned long long *)h;
+      ctx_->dflags = (unsigned long long *)d;
       ctx_->hflags_n = bt_.Bpad;
     }
     if ((size_t)(p - (char *)ctx_->ws) > need)
@@ -1968,18 +1982,21 @@ class Engine {
   KA km(uint32_t req, uint32_t forbid = 0) const { return KA{G_, S_, part_, nblk_m_, req, forbid}; }
   dim3 gn() const { return dim3(nblk_n_, bt_.G); }
   dim3 gm() const { return dim3(nblk_m_, bt_.G); }
+  // arguments of a fin_* kernel that publishes the flags (numbered: see
+  // wait_published)
   FA fa(uint32_t req, uint32_t forbid = 0) const {
-    FA f;
-    f.S = S_; f.G = G_; f.part = part_; f.nblk_n = nblk_n_; f.nblk_m = nblk_m_;
-    f.W = W; f.req = req; f.forbid = forbid; f.fout = ctx_->dflags;
-    f.trace = trace_; f.trace_cap = trace_cap_; f.O = O_;
+    FA f = fa_quiet(req, forbid);
+    f.fout = ctx_->dflags;
+    f.epoch = ++ctx_->pub_epoch;
     return f;
   }
   // fin_* kernels after which the host reads no flags need not publish them
   // to host memory
   FA fa_quiet(uint32_t req, uint32_t forbid = 0) const {
-    FA f = fa(req, forbid);
-    f.fout = nullptr;
+    FA f;
+    f.S = S_; f.G = G_; f.part = part_; f.nblk_n = nblk_n_; f.nblk_m = nblk_m_;
+    f.W = W; f.req = req; f.forbid = forbid; f.fout = nullptr; f.epoch = 0;
+    f.trace = trace_; f.trace_cap = trace_cap_; f.O = O_;
     return f;
   }
   dim3 gi() const { return dim3(bt_.Bpad); }
@@ -2073,10 +2090,10 @@ class Engine {
   // to lane_max_ lanes, 2 (compact 64-lane groups) when they fill fewer
   // groups than the batch, else 0
   int pick_compact(uint32_t req) {
-    const volatile uint32_t *h = ctx_->hflags;
+    const volatile unsigned long long *h = ctx_->hflags;
     lanes_.n = 0;
     for (int i = 0; i < bt_.Bpad; ++i) {
-      if ((h[i] & req) != req) continue;
+      if (((uint32_t)h[i] & req) != req) continue;
       if (lanes_.n == kLaneCap) return 0;
       lanes_.inst[lanes_.n++] = i;
     }
@@ -2086,14 +2103,40 @@ class Engine {
     return ncg_ <= gc64_max_ ? 2 : 0;
   }
 
+  // Waits until every instance's flags of the last publishing launch are in
+  // host memory.  Host-driven solves poll the launch number the fin kernel
+  // stores with the flags instead of synchronizing the stream: the host
+  // wakes as soon as the flags land, and later launches need no drained
+  // stream.  A captured graph replays its launch numbers, so graph mode (and
+  // TV3_SPIN=0) synchronizes the stream; so does a poll that has not seen
+  // the number after two seconds (a failed launch then reports its error).
+  int wait_published() {
+    if (graph_ || !spin_wait() || !ctx_->hflags)
+      return hip_check(ctx_, hipStreamSynchronize(s_), "flag sync");
+    const uint32_t e = ctx_->pub_epoch;
+    const volatile unsigned long long *q = ctx_->hflags;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < bt_.Bpad; ++i) {
+      unsigned spins = 0;
+      while ((uint32_t)(q[i] >> 32) != e) {
+        if ((++spins & 1023) == 0 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2))
+          return hip_check(ctx_, hipStreamSynchronize(s_), "flag sync");
+        __builtin_ia32_pause();
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return TV3_OK;
+  }
+
   // OR of the instance flags the fin kernels published to mapped host memory
   int read_summary(uint32_t &out) {
-    int rc = hip_check(ctx_, hipStreamSynchronize(s_), "flag sync");
+    int rc = wait_published();
     out = 0;
     if (rc) return rc;
     flags_fresh_ = true;
-    const volatile uint32_t *h = ctx_->hflags;
-    for (int i = 0; i < bt_.Bpad; ++i) out |= h[i];
+    const volatile unsigned long long *h = ctx_->hflags;
+    for (int i = 0; i < bt_.Bpad; ++i) out |= (uint32_t)h[i];
     return TV3_OK;
   }
 
