@@ -1389,12 +1389,25 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_blk(
 #pragma unroll
           for (int u = 0; u < C; ++u) acc[u] += (double)a[r][u] * xi[r];
       }
-      for (; i < y1; ++i) {
-        const double xi = (double)xg[i];
-        float a[C];
-        loadc<C>(A + (long long)i * n + c0, a);
+      if (i < y1) {
+        // the block's last rows in one round, their loads in flight together
+        // (row by row they took a memory latency each: 11.5 -> 10.7 ms per
+        // product of the 64-instance bf16 batch)
+        float a[U][C];
+        double xi[U];
 #pragma unroll
-        for (int u = 0; u < C; ++u) acc[u] += (double)a[u] * xi;
+        for (int r = 0; r < U; ++r) {
+          if (i + r < y1) {
+            loadc<C>(A + (long long)(i + r) * n + c0, a[r]);
+            xi[r] = (double)xg[i + r];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < U; ++r)
+          if (i + r < y1) {
+#pragma unroll
+            for (int u = 0; u < C; ++u) acc[u] += (double)a[r][u] * xi[r];
+          }
       }
     } else {
       for (int i = y0; i < y1; ++i) {
@@ -1456,12 +1469,23 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_bs(
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
     }
-    for (; i < i1; ++i) {
-      const double xi = (double)xg[i];
-      float a[8];
-      load8(A + (long long)i * n + c0, a);
+    if (i < i1) {
+      // the slice's last rows in one round
+      float a[U][8];
+      double xi[U];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc[u] += (double)a[u] * xi;
+      for (int r = 0; r < U; ++r) {
+        if (i + r < i1) {
+          load8(A + (long long)(i + r) * n + c0, a[r]);
+          xi[r] = (double)xg[i + r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < U; ++r)
+        if (i + r < i1) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+        }
     }
   } else {
     for (int i = i0; i < i1; ++i) {
