@@ -139,7 +139,7 @@ def test_every_dense_launch_shape(ctx, bf16, monkeypatch):
     want = [O.solve(op, b, d, opts)[0] for op, b, d in zip(ops, bs, dims)]
     for R in (1, 2, 4):
         for U in (1, 2, 4, 8):
-            for Cc in (4, 8):
+            for Cc in (2, 4, 8):
                 monkeypatch.setenv("TV3_DENSE_SHAPE", f"{R},{U},{Cc}")
                 Us, _ = ctx.solve_batch(plans, bs, dims, opts)
                 for k in range(3):

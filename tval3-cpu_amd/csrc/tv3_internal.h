@@ -327,7 +327,7 @@ void prof_collect(tv3_ctx *ctx);
 struct DenseTune {
   int rows;
   int unroll;  // 1, 2, 4, 8
-  int cols;    // columns per thread of the blocked A^T product: 4 or 8
+  int cols;    // columns per thread of the blocked A^T product: 2, 4 or 8
   // no group of the batch runs the whole-row blocked kernel (1) or the
   // row-slice kernels (1): their launches are skipped
   int no_blk = 0, no_slice = 0;

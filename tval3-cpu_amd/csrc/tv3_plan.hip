@@ -1183,9 +1183,19 @@ __device__ __forceinline__ void load4(const __hip_bfloat16 *p, float (&v)[4]) {
   v[2] = __uint_as_float(q.y << 16);
   v[3] = __uint_as_float(q.y & 0xffff0000u);
 }
+// two consecutive entries (4 bytes of bf16, 8 of f32)
+__device__ __forceinline__ void load2(const float *p, float (&v)[2]) {
+  const float2 a = *(const float2 *)p;
+  v[0] = a.x; v[1] = a.y;
+}
+__device__ __forceinline__ void load2(const __hip_bfloat16 *p, float (&v)[2]) {
+  const uint32_t q = *(const uint32_t *)p;
+  v[0] = __uint_as_float(q << 16);
+  v[1] = __uint_as_float(q & 0xffff0000u);
+}
 template <int C, class T>
 __device__ __forceinline__ void loadc(const T *p, float (&v)[C]) {
-  if constexpr (C == 8) load8(p, v); else load4(p, v);
+  if constexpr (C == 8) load8(p, v); else if constexpr (C == 4) load4(p, v); else load2(p, v);
 }
 
 // y = A x: one wavefront per R consecutive rows; with n a multiple of 8 every
@@ -1818,9 +1828,9 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
     if (tn.no_blk) {
       // no group of this batch takes the whole-row kernel
     } else if (dtype == TV3_BF16) {
-      if (Cc == 4) { TV3_DBLK_U(__hip_bfloat16, 4) } else { TV3_DBLK_U(__hip_bfloat16, 8) }
+      if (Cc == 2) { TV3_DBLK_U(__hip_bfloat16, 2) } else if (Cc == 4) { TV3_DBLK_U(__hip_bfloat16, 4) } else { TV3_DBLK_U(__hip_bfloat16, 8) }
     } else {
-      if (Cc == 4) { TV3_DBLK_U(float, 4) } else { TV3_DBLK_U(float, 8) }
+      if (Cc == 2) { TV3_DBLK_U(float, 2) } else if (Cc == 4) { TV3_DBLK_U(float, 4) } else { TV3_DBLK_U(float, 8) }
     }
 #undef TV3_DBLK_U
 #undef TV3_DBLK

@@ -1637,7 +1637,7 @@ bool forced_dense_shape(DenseTune &t) {
   if (!e || !*e) return false;
   int r = 0, u = 0, c = 0;
   if (std::sscanf(e, "%d,%d,%d", &r, &u, &c) != 3) return false;
-  if (!(r == 1 || r == 2 || r == 4) || !(u == 1 || u == 2 || u == 4 || u == 8) || !(c == 4 || c == 8))
+  if (!(r == 1 || r == 2 || r == 4) || !(u == 1 || u == 2 || u == 4 || u == 8) || !(c == 2 || c == 4 || c == 8))
     return false;
   t.rows = r;
   t.unroll = u;
@@ -1883,12 +1883,12 @@ class Engine {
         }));
       }
     best.rows = rows_c[std::min_element(tr, tr + 3) - tr];
-    const int cols_c[2] = {8, 4}, unr_c[4] = {1, 2, 4, 8};
-    float tb[2][4];
+    const int cols_c[3] = {8, 4, 2}, unr_c[4] = {1, 2, 4, 8};
+    float tb[3][4];
     for (auto &row : tb)
       for (float &t : row) t = 1e30f;
     for (int rd = 0; rd < kTuneRounds; ++rd)
-      for (int ci = 0; ci < 2; ++ci)
+      for (int ci = 0; ci < 3; ++ci)
         for (int ui = 0; ui < 4; ++ui) {
           dense_tune_set(DenseTune{best.rows, unr_c[ui], cols_c[ci]});
           tb[ci][ui] = std::min(tb[ci][ui], time_launches([&] {
@@ -1897,7 +1897,7 @@ class Engine {
           }));
         }
     float tbest = 1e30f;
-    for (int ci = 0; ci < 2; ++ci)
+    for (int ci = 0; ci < 3; ++ci)
       for (int ui = 0; ui < 4; ++ui)
         if (tb[ci][ui] < tbest) { tbest = tb[ci][ui]; best.unroll = unr_c[ui]; best.cols = cols_c[ci]; }
     use(best);
