@@ -73,8 +73,13 @@ def traffic(fetch_dir, write_dir, copy_mib=512, kernels=None):
     fr = counter_means(fetch_dir, "FETCH_SIZE")
     wr = counter_means(write_dir, "WRITE_SIZE")
     copy_kib = copy_mib * 1024.0
-    cf = [v for n, vs in fr.items() if "copyBuffer" in n for v in vs if v > 0.4 * copy_kib]
-    cw = [v for n, vs in wr.items() if "copyBuffer" in n for v in vs if v > 0.9 * copy_kib]
+    # copies of exactly copy_mib MiB (a run may hold other copies of about
+    # that size, e.g. the last chunk of a tensor): FETCH_SIZE counts half
+    # their bytes and WRITE_SIZE all of them, within 0.2 %
+    cf = [v for n, vs in fr.items() if "copyBuffer" in n for v in vs
+          if abs(v - 0.5 * copy_kib) <= 0.002 * 0.5 * copy_kib]
+    cw = [v for n, vs in wr.items() if "copyBuffer" in n for v in vs
+          if abs(v - copy_kib) <= 0.002 * copy_kib]
     if not cf or not cw:
         raise SystemExit("no calibration copies of the given size in the runs")
     f_fac = copy_kib / (sum(cf) / len(cf))
@@ -119,9 +124,12 @@ def main(argv=None):
         return
     ks = set(a.kernels.split(",")) if a.kernels else None
     res, ff, wf, cf, cw = traffic(a.fetch, a.write, a.copy_mib, ks)
-    print(f"calibration: {a.copy_mib} MiB device copies; FETCH_SIZE "
-          f"{', '.join(f'{v:.1f}' for v in cf)} KiB -> factor {ff:.4f}; WRITE_SIZE "
-          f"{', '.join(f'{v:.1f}' for v in cw)} KiB -> factor {wf:.4f}")
+    def show(vs):
+        if len(vs) <= 6:
+            return ", ".join(f"{v:.1f}" for v in vs)
+        return f"{len(vs)} copies, mean {sum(vs) / len(vs):.1f} (min {min(vs):.1f}, max {max(vs):.1f})"
+    print(f"calibration: {a.copy_mib} MiB device copies; FETCH_SIZE {show(cf)} KiB -> factor {ff:.4f}; "
+          f"WRITE_SIZE {show(cw)} KiB -> factor {wf:.4f}")
     print(f"{'kernel':40s} {'n':>6s} {'FETCH KiB':>14s} {'WRITE KiB':>14s} {'HBM bytes (cal.)':>17s}")
     for name, d in sorted(res.items(), key=lambda kv: -kv[1]["hbm_bytes_per_launch"]):
         print(f"{name[:40]:40s} {d['dispatches']:6d} {d['fetch_kib']:14.1f} {d['write_kib']:14.1f} "
