@@ -451,6 +451,61 @@ __device__ __forceinline__ double quarter_sum(double v) {
   return v;
 }
 
+// Row gathers of the 64-lane projector kernels go through a buffer resource
+// over the group's vector: a lane's 16 bytes of row r are at byte offset
+// 256 r + 16 c.  A step past the end of a walk or list takes the row one
+// past the vector (offset >= the resource's size), which the hardware's range
+// check returns as zeros: adding +0.0 leaves every sum (which starts at +0.0)
+// unchanged, so the rounds need no bounds tests and no masked branches (a
+// non-finite scale factor makes the sums non-finite either way).  Byte
+// sizes and offsets are 32-bit: batches of geometries with 2^24 or more
+// voxels or rays run in W = 1 groups.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const float *base, int rows) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)((unsigned)rows * 256u), 0x00020000);
+}
+__device__ __forceinline__ float4 row_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return make_float4(__int_as_float(v[0]), __int_as_float(v[1]), __int_as_float(v[2]), __int_as_float(v[3]));
+}
+
+// the four instance sums of one lane over rounds t of 64 list or walk steps:
+// step 4t + q of the chunk is held, as a row byte offset, by lane 4t + q
+template <bool UNIT>
+__device__ __forceinline__ void chunk_sums(__amdgpu_buffer_rsrc_t r, int myoff, unsigned coff, int q,
+                                           int tmax, float sc, double &a0, double &a1, double &a2,
+                                           double &a3) {
+  for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
+    float4 v[kPipeQ];
+#pragma unroll
+    for (int u = 0; u < kPipeQ; ++u) {
+      const int off = __builtin_amdgcn_ds_bpermute(16 * (t0 + u) + 4 * q, myoff);
+      v[u] = row_load(r, (unsigned)off + coff);
+    }
+#pragma unroll
+    for (int u = 0; u < kPipeQ; ++u) {
+      if (UNIT) {
+        a0 += (double)v[u].x; a1 += (double)v[u].y;
+        a2 += (double)v[u].z; a3 += (double)v[u].w;
+      } else {
+        a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
+        a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
+      }
+    }
+  }
+}
+
+template <bool UNIT>
+__device__ __forceinline__ void fwd_ray64(const RayDesc &rd, int L, __amdgpu_buffer_rsrc_t r, int n,
+                                          unsigned coff, int q, int lane, float sc, double &a0, double &a1,
+                                          double &a2, double &a3) {
+  for (int j0 = 0; j0 <= L; j0 += 64) {
+    const int jl = j0 + lane;
+    const int myoff = (int)((unsigned)(jl <= L ? ray_lin(rd, jl) : n) * 256u);
+    const int cnt = min(64, L + 1 - j0);
+    chunk_sums<UNIT>(r, myoff, coff, q, (cnt + 3) >> 2, sc, a0, a1, a2, a3);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
                                                      const float *x, float *y,
                                                      const InstScal *S,
@@ -474,36 +529,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
   const RayDesc rd = ldg(P.rays + row);
   const int L = uni_i(rd.L);
   const float sc = P.scale;
-  const bool unit = sc == 1.0f;
-  const float *xb = x + gd.off_n + 4 * c;
+  const int n = gd.n;
+  const __amdgpu_buffer_rsrc_t xr = rows_rsrc(x + gd.off_n, n);
+  const unsigned coff = 16u * c;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  for (int j0 = 0; j0 <= L; j0 += 64) {
-    const int jl = j0 + lane;
-    const int myidx = jl <= L ? ray_lin(rd, jl) : 0;
-    const int cnt = min(64, L + 1 - j0);
-    const int tmax = (cnt + 3) >> 2;
-    for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
-      float4 v[kPipeQ];
-#pragma unroll
-      for (int u = 0; u < kPipeQ; ++u) {
-        const int st = 4 * (t0 + u) + q;
-        const int iu = __shfl(myidx, st & 63, 64);
-        v[u] = st < cnt ? *(const float4 *)(xb + (long long)iu * 64) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-#pragma unroll
-      for (int u = 0; u < kPipeQ; ++u) {
-        if (4 * (t0 + u) + q < cnt) {
-          if (unit) {
-            a0 += (double)v[u].x; a1 += (double)v[u].y;
-            a2 += (double)v[u].z; a3 += (double)v[u].w;
-          } else {
-            a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
-            a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
-          }
-        }
-      }
-    }
-  }
+  if (sc == 1.0f)
+    fwd_ray64<true>(rd, L, xr, n, coff, q, lane, sc, a0, a1, a2, a3);
+  else
+    fwd_ray64<false>(rd, L, xr, n, coff, q, lane, sc, a0, a1, a2, a3);
   a0 = quarter_sum(a0); a1 = quarter_sum(a1);
   a2 = quarter_sum(a2); a3 = quarter_sum(a3);
   if (q == 0) {
@@ -528,6 +561,19 @@ __global__ __launch_bounds__(kBlock) void k_fwd_w64q(const GroupDesc *G,
 // (two voxels per wavefront measured best: 0.186 -> 0.173 ms for two 64-lane
 // groups of the 256 ring; four 0.186, eight 0.237).
 constexpr int kBwdCols = 2;
+
+template <bool UNIT>
+__device__ __forceinline__ void bwd_list64(const int *t_ray, int kA0, int kA1, int idA,
+                                           __amdgpu_buffer_rsrc_t r, int m, unsigned coff, int q,
+                                           int lane, float sc, double &a0, double &a1, double &a2,
+                                           double &a3) {
+  for (int kb = kA0; kb < kA1; kb += 64) {
+    const int cnt = min(64, kA1 - kb);
+    const int id = kb == kA0 ? idA : (lane < cnt ? ldg(t_ray + kb + lane) : m);
+    chunk_sums<UNIT>(r, (int)((unsigned)id * 256u), coff, q, (cnt + 3) >> 2, sc, a0, a1, a2, a3);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
                                                      const float *x, float *y,
                                                      const InstScal *S,
@@ -551,11 +597,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
   const PlanDev &P = gd.plan;
   const float sc = P.scale;
   const bool unit = sc == 1.0f;
-  const float *xb = x + gd.off_m + 4 * c;
+  const int m = gd.m;
+  const __amdgpu_buffer_rsrc_t xr = rows_rsrc(x + gd.off_m, m);
+  const unsigned coff = 16u * c;
   // voxel A: current, B: next (bounds known), C: after next
   int kA0 = uni_i(ldg(P.t_ptr + (s0))), kA1 = uni_i(ldg(P.t_ptr + (s0 + 1)));
   int colA = uni_i(ldg(P.t_col + (s0)));
-  int idA = lane < min(64, kA1 - kA0) ? ldg(P.t_ray + (kA0 + lane)) : 0;
+  int idA = lane < min(64, kA1 - kA0) ? ldg(P.t_ray + (kA0 + lane)) : m;
   int kB0 = 0, kB1 = 0, colB = 0;
   if (ns > 1) {
     kB0 = uni_i(ldg(P.t_ptr + (s0 + 1)));
@@ -563,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
     colB = uni_i(ldg(P.t_col + (s0 + 1)));
   }
   for (int i = 0; i < ns; ++i) {
-    const int idB = (i + 1 < ns && lane < min(64, kB1 - kB0)) ? ldg(P.t_ray + (kB0 + lane)) : 0;
+    const int idB = (i + 1 < ns && lane < min(64, kB1 - kB0)) ? ldg(P.t_ray + (kB0 + lane)) : m;
     int kC0 = 0, kC1 = 0, colC = 0;
     if (i + 2 < ns) {
       kC0 = uni_i(ldg(P.t_ptr + (s0 + i + 2)));
@@ -571,32 +619,10 @@ __global__ __launch_bounds__(kBlock) void k_bwd_w64q(const GroupDesc *G,
       colC = uni_i(ldg(P.t_col + (s0 + i + 2)));
     }
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int kb = kA0; kb < kA1; kb += 64) {
-      const int cnt = min(64, kA1 - kb);
-      const int id = kb == kA0 ? idA : (lane < cnt ? ldg(P.t_ray + (kb + lane)) : 0);
-      const int tmax = (cnt + 3) >> 2;
-      for (int t0 = 0; t0 < tmax; t0 += kPipeQ) {
-        float4 v[kPipeQ];
-#pragma unroll
-        for (int u = 0; u < kPipeQ; ++u) {
-          const int st = 4 * (t0 + u) + q;
-          const int iu = __shfl(id, st & 63, 64);
-          v[u] = st < cnt ? *(const float4 *)(xb + (long long)iu * 64) : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int u = 0; u < kPipeQ; ++u) {
-          if (4 * (t0 + u) + q < cnt) {
-            if (unit) {
-              a0 += (double)v[u].x; a1 += (double)v[u].y;
-              a2 += (double)v[u].z; a3 += (double)v[u].w;
-            } else {
-              a0 += (double)(v[u].x * sc); a1 += (double)(v[u].y * sc);
-              a2 += (double)(v[u].z * sc); a3 += (double)(v[u].w * sc);
-            }
-          }
-        }
-      }
-    }
+    if (unit)
+      bwd_list64<true>(P.t_ray, kA0, kA1, idA, xr, m, coff, q, lane, sc, a0, a1, a2, a3);
+    else
+      bwd_list64<false>(P.t_ray, kA0, kA1, idA, xr, m, coff, q, lane, sc, a0, a1, a2, a3);
     a0 = quarter_sum(a0); a1 = quarter_sum(a1);
     a2 = quarter_sum(a2); a3 = quarter_sum(a3);
     if (q == 0) {

@@ -2442,11 +2442,14 @@ int solve_batch_impl(tv3_ctx *ctx, int B, const tv3_plan *const *A,
   // gather per voxel for 64 instances).  TV3_GROUP_WIDTH=16 selects 16-lane
   // groups dealt one per XCD (measured slower on the 256x256 ring).
   // Multi-layer instances run in W = 1 groups.
-  // FP64 vectors: W = 1 groups only.
-  bt.W = (shared && B > 1 && !bt.layered && !kOnlyW1) ? 64 : 1;
+  // FP64 vectors: W = 1 groups only.  The 64-lane projector kernels address
+  // vectors by 32-bit byte offsets of 256-byte rows: fewer than 2^24 voxels
+  // and rays.
+  const bool rows_ok = A[0]->dev.kind != kGeometry || (A[0]->dev.n < (1 << 24) && A[0]->dev.m < (1 << 24));
+  bt.W = (shared && B > 1 && !bt.layered && !kOnlyW1 && rows_ok) ? 64 : 1;
   if (const char *ev = kOnlyW1 ? nullptr : getenv("TV3_GROUP_WIDTH")) {
     const int w = atoi(ev);
-    if (shared && B > 1 && !bt.layered && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense))
+    if (shared && B > 1 && !bt.layered && rows_ok && (w == 16 || w == 64) && !(w == 16 && A[0]->dev.kind == kDense))
       bt.W = w;
   }
   const int W = bt.W;
