@@ -170,16 +170,29 @@ def run_ring(args, ctx):
     plans, dl = [plan] * B, [dims] * B
     for _ in range(args.warmup):
         Us, infos = ctx.solve_batch(plans, bs, dl, opts)
+    # one untimed step counts the products' compulsory bytes and lanes on the
+    # device; the timed steps (the same deterministic launches) time the
+    # products with events only, so no counting kernel runs in them
+    ctx.profile(1)
+    ctx.solve_batch(plans, bs, dl, opts)
+    counted = ctx.profile_read()
     torch.cuda.synchronize()
-    ctx.profile(True)
+    ctx.profile(3)
     t0 = time.perf_counter()
     for k in range(args.steps):
         Us, infos = ctx.solve_batch(plans, bs, dl, opts)
         log(f"step {k + 1}/{args.steps} done")
     torch.cuda.synchronize()
     t_steps = time.perf_counter() - t0
-    prof = ctx.profile_read()
+    timed = ctx.profile_read()
     ctx.profile(False)
+    prof = {}
+    for k in timed:
+        ms, cnt = timed[k][0], timed[k][1]
+        _, cnt1, by1, ln1 = counted[k]
+        if cnt != cnt1 * args.steps:
+            raise RuntimeError(f"{k}: {cnt} timed launches, {cnt1} per counted step")
+        prof[k] = (ms, cnt, by1 * args.steps, ln1 * args.steps)
     args.one_step = lambda: ctx.solve_batch(plans, bs, dl, opts)
     value = B * args.steps / t_steps
     rel_b0 = float(torch.linalg.norm(Us[0] - U0) / torch.linalg.norm(U0))
@@ -348,6 +361,26 @@ def run_dense_batch(args, ctx):
             else {"A": "k_dense_fwd_w1", "At": "k_dense_bwd_blk"})
     ops, roof = roofline(prof, kern, "dense-batch" + ("-f64" if f64v else ""), None, t_steps,
                          args.steps)
+    # the same batch replayed from a hipGraph (tv3_options.graph = 1: every
+    # pair of iterations is one graph launch, the host reads the stop flags
+    # once per launch); timed apart, since launches inside a graph carry no
+    # events, and held to the host-driven result bit for bit
+    graph = None
+    if not args.graph:
+        gopts = tval3_amd.default_options(maxit=args.maxit, scal_double=1, graph=1)
+        ctx.solve_batch(plans, bs, dims, gopts, f64=f64v)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Ug, _ = ctx.solve_batch(plans, bs, dims, gopts, f64=f64v)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t0
+        same = all(torch.equal(a, b) for a, b in zip(Ug, Us))
+        graph = {"recons_per_s": B / tg, "ms_per_step": 1e3 * tg,
+                 "bit_identical_to_host_driven": bool(same)}
+        log(f"graph mode: {B / tg:.2f} recons/s, bit-identical: {same}")
+        if not same:
+            raise RuntimeError("graph-mode batch differs from the host-driven batch")
+        del Ug
     # every instance against the FP64 solve of the same bf16-rounded
     # matrices (tv3_solve_batch_f64 on the GPU, in chunks of FP64 copies of
     # A), and the smallest ones against the CPU oracle's FP32 solve (the
@@ -412,6 +445,7 @@ def run_dense_batch(args, ctx):
         "config": {"workload": f"{B} uneven dense 2D reconstructions (bf16 A), maxit {args.maxit}",
                    "batch": B, "bf16_entries": tot_elems, "maxit": args.maxit},
         "precision_vs_fp64": prec, "iters": [i.total_iters for i in infos],
+        "graph_replay": graph,
         "operators": ops, "roofline": roof, "cpu_baseline": cpu,
     }
 

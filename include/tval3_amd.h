@@ -185,8 +185,10 @@ int tv3_mem_download(tv3_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Per-launch HIP-event timing of the solver's launches on the context stream
  * (it replaces the reference's PROFILING table, whose gemv slots 10/11 are
  * at 3d/mat_mult.h:19-44).  enable: 0 off, 1 the 64-lane or single-instance
- * A and A^T products (slots 10/11), 2 every launch (tv3_ctx_profile_slots);
- * each call resets the totals.  profile_read returns, for [A, A^T], the summed
+ * A and A^T products (slots 10/11), 2 every launch (tv3_ctx_profile_slots),
+ * 3 as 1 without the byte and lane counts of batched launches (no counting
+ * kernel runs on the stream; profile_read then reports only the bytes of
+ * launches the host counts); each call resets the totals.  profile_read returns, for [A, A^T], the summed
  * milliseconds, the number of launches and (bytes, may be NULL) the summed
  * compulsory bytes of those launches: the operator once for every group of
  * instances that computed (dense s_A*m*n; CSR / CSC (4 + s_v)*nnz +

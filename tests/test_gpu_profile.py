@@ -1,4 +1,4 @@
-"""The launch-timing report (tv3_ctx_profile levels 1 and 2 and
+"""The launch-timing report (tv3_ctx_profile levels 1 to 3 and
 tv3_ctx_profile_slots), the counterpart of the reference's PROFILING table
 (profile_info[slot], 3d/tval3_cpu_3d.cpp:1337-1348; 3d/mat_mult.h:19-44).
 
@@ -107,7 +107,31 @@ def test_profile_levels(ctx):
     assert [s for s, *_ in s1] == [10, 11]
     assert s0 == []
     with pytest.raises(ValueError):
-        ctx.profile(3)
+        ctx.profile(4)
+
+
+def test_profile_level3_times_without_counting(ctx):
+    """Level 3 times the same product launches as level 1 in a 64-lane batch
+    but runs no counting kernel: only the launches the host counts (lanes
+    gathered into single-instance kernels) add bytes and lanes."""
+    geo, b = _ring(32, 4)
+    plan = ctx.plan_geometry(geo)
+    B = 70
+    bs = [b * (1 + 0.01 * i) for i in range(B)]
+    opts = tval3_amd.default_options(maxit=20, ls_exact=1)
+    got = {}
+    for level in (1, 3):
+        ctx.profile(level)
+        Us, _ = ctx.solve_batch([plan] * B, bs, [(32, 32, 1)] * B, opts)
+        got[level] = (ctx.profile_read(), np.stack([u.cpu().numpy() for u in Us]))
+        ctx.profile(0)
+    (p1, U1), (p3, U3) = got[1], got[3]
+    assert np.array_equal(U1, U3)
+    for k in ("A", "At"):
+        assert p3[k][1] == p1[k][1] > 0  # launches
+        assert p1[k][2] > 0 and p1[k][3] > 0
+    # the full 64-lane launches are counted on the device only
+    assert p3["A"][2] < p1["A"][2] and p3["A"][3] < p1["A"][3]
 
 
 def test_profile_batch_long_solve(ctx):

@@ -56,7 +56,7 @@ const char *prof_slot_name(int slot) {
 void prof_begin(tv3_ctx *ctx, int kind) {
   ctx->prof_open = false;
   if (!ctx->prof || kind < 0 || kind >= kProfSlots) return;
-  if (ctx->prof_level < 2 && kind != kProfGemvN && kind != kProfGemvT) return;
+  if (ctx->prof_level != 2 && kind != kProfGemvN && kind != kProfGemvT) return;
   if (ctx->evkind.size() >= 4096) {
     // fold the finished pairs in, so that long solves reuse the pool
     hipStreamCaptureStatus cs = hipStreamCaptureStatusActive;
@@ -153,7 +153,7 @@ void prof_count_host(tv3_ctx *ctx, const GroupDesc &g, int trans, int vbytes, in
 
 void prof_count(tv3_ctx *ctx, const GroupDesc *G_dev, int W, int Bpad,
                 const InstScal *S, uint32_t mask, int trans, int vbytes) {
-  if (!ctx->prof || !ctx->prof_bytes) return;
+  if (!ctx->prof || !ctx->prof_bytes || ctx->prof_level == 3) return;
   const int nG = Bpad / W;
   hipLaunchKernelGGL(k_prof_bytes, dim3((nG + 255) / 256), dim3(256), 0, ctx->stream,
                      G_dev, W, nG, S, mask, trans, vbytes, ctx->prof_bytes);

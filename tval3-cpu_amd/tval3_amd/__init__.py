@@ -191,10 +191,12 @@ class Context:
 
     def profile(self, enable=True):
         """Start (and reset) or stop per-launch timing: False / 0 off, True / 1
-        the A / A^T products only, 2 every launch of the solver (slower)."""
+        the A / A^T products only, 2 every launch of the solver (slower), 3 the
+        products only without the device-side byte and lane counts of batched
+        launches."""
         level = int(enable)
-        if level not in (0, 1, 2):
-            raise ValueError(f"profile level {enable!r}: 0, 1 or 2")
+        if level not in (0, 1, 2, 3):
+            raise ValueError(f"profile level {enable!r}: 0, 1, 2 or 3")
         self._check(lib().tv3_ctx_profile(self.h, level))
 
     def profile_slots(self):
