@@ -192,7 +192,8 @@ MANY = [(64, 24, 24, 173, False), (70, 16, 16, 100, True), (20, 15, 13, 59, Fals
 def test_layers_on_matrix_cores(ctx, L, q, p, m, bf16, monkeypatch):
     """The multi-layer products as FP64-MFMA GEMMs over the layers: the solve
     equals the oracle's bit for bit, with its trajectory, and equals the
-    per-layer W = 1 kernels' solve (TV3_LAYER_MFMA=0) bit for bit."""
+    per-layer W = 1 kernels' solve (TV3_LAYER_MFMA=0) and the solves with
+    one or 2 x 2 output tiles per workgroup in each product bit for bit."""
     At, Ar, b, dims = _many_layers(L, q, p, m, 90 + L, bf16)
     plan = ctx.plan_dense(At, bf16=bf16)
     opts = tval3_amd.default_options(maxit=60, ls_exact=1)
@@ -203,8 +204,14 @@ def test_layers_on_matrix_cores(ctx, L, q, p, m, bf16, monkeypatch):
     monkeypatch.setenv("TV3_LAYER_MFMA", "0")
     U0, i0, _ = ctx.solve(plan, b, dims, opts)
     assert _bits_equal(U0.cpu().numpy(), U1.cpu().numpy())
-    # maxit = 0 returns the scaled A^T b alone (the transposed GEMM)
     monkeypatch.delenv("TV3_LAYER_MFMA")
+    # one 16 x 16 tile, 2 x 2 or (A X) 2 x 1 tiles per workgroup
+    for blk in ("1,1", "2,2", "3,1", "2,1"):
+        monkeypatch.setenv("TV3_MFMA_BLOCK", blk)
+        Ub, _, _ = ctx.solve(plan, b, dims, opts)
+        assert _bits_equal(Ub.cpu().numpy(), U1.cpu().numpy()), blk
+    monkeypatch.delenv("TV3_MFMA_BLOCK")
+    # maxit = 0 returns the scaled A^T b alone (the transposed GEMM)
     o0 = tval3_amd.default_options(maxit=0)
     Z1, _, _ = ctx.solve(plan, b, dims, o0)
     Zo, _, _ = O.solve(O.Op.dense(Ar), b, dims, o0)

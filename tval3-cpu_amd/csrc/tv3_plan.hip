@@ -1573,8 +1573,9 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_bred(
 // tiles in wave order through LDS.
 typedef double tv3_d4 __attribute__((ext_vector_type(4)));
 
-// wavefronts per output tile (TV3_MFMA_WAVES = 4, 8 or 16 overrides).  8
-// measured best on 64 x 64 layers sharing a 1229 x 4096 A: A X / A^T Y took
+// wavefronts per output tile group (TV3_MFMA_WAVES = 4, 8 or 16
+// overrides).  8 measured best for one-tile workgroups on 64 x 64 layers
+// sharing a 1229 x 4096 A: A X / A^T Y took
 // 23.0 / 15.2 us at 8 layers and 38.6 / 42.9 us at 64 (4 waves: 31.8 / 15.9
 // and 43.2 / 41.9; 16 waves: 21.8 / 15.7 and 43.2 / 58.8).  The kernels
 // load one value per lane per MFMA (4 flops per loaded byte), so L2-to-CU
@@ -1757,6 +1758,242 @@ __global__ __launch_bounds__(64 * NW) void k_dense_bwd_mfma(const GroupDesc *G, 
   }
 }
 
+// Register-blocked variants: a workgroup computes TL x TR output tiles (TL
+// layer tiles by TR row or column tiles), so each operand a lane loads feeds
+// TR (layer operands) or TL (A operands) MFMAs instead of one.  Every tile
+// is summed exactly as in the one-tile kernels above (the same wave split of
+// K, the same four accumulator chains per 16-wide block, the same final and
+// wave-order additions), so both give the same bits.
+constexpr int kMfUb = 2;  // 16-wide blocks per load group
+
+template <class T, bool VEC, bool FWD, int TL, int TR>
+__device__ __forceinline__ void mfb_group(const T *const (&Ab)[TR], const float *const (&xl)[TL],
+                                          const bool (&aok)[TR], const bool (&xok)[TL], int kb, int k1,
+                                          int K, int q, long long n, float (&av)[kMfUb][TR][4],
+                                          float (&xv)[kMfUb][TL][4]) {
+#pragma unroll
+  for (int u = 0; u < kMfUb; ++u) {
+    const int k = kb + 16 * u + 4 * q;
+    const bool live = kb + 16 * u < k1;
+#pragma unroll
+    for (int t = 0; t < TL; ++t) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) xv[u][t][s] = 0.0f;
+      if (!live || !xok[t]) continue;
+      if (VEC && k + 4 <= K) {
+        mf_load4(xl[t] + k, xv[u][t]);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          if (k + s < K) xv[u][t][s] = xl[t][k + s];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TR; ++t) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) av[u][t][s] = 0.0f;
+      if (!live || !aok[t]) continue;
+      if (FWD) {
+        if (VEC && k + 4 <= K) {
+          mf_load4(Ab[t] + k, av[u][t]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; ++s)
+            if (k + s < K) av[u][t][s] = mf_load(Ab[t] + k + s);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          if (k + s < K) av[u][t][s] = mf_load(Ab[t] + (long long)(k + s) * n);
+      }
+    }
+  }
+}
+
+template <class T, bool VEC, bool FWD, int TL, int TR>
+__device__ __forceinline__ void mfb_walk(const T *const (&Ab)[TR], const float *const (&xl)[TL],
+                                         const bool (&aok)[TR], const bool (&xok)[TL], int k0, int k1,
+                                         int K, int q, long long n, tv3_d4 (&out)[TL][TR]) {
+  tv3_d4 acc[TL][TR][4];
+#pragma unroll
+  for (int a = 0; a < TL; ++a)
+#pragma unroll
+    for (int b = 0; b < TR; ++b)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc[a][b][s] = tv3_d4{0.0, 0.0, 0.0, 0.0};
+  float av[kMfUb][TR][4], xv[kMfUb][TL][4];
+  mfb_group<T, VEC, FWD, TL, TR>(Ab, xl, aok, xok, k0, k1, K, q, n, av, xv);
+  for (int kb = k0; kb < k1; kb += 16 * kMfUb) {
+    float an[kMfUb][TR][4], xn[kMfUb][TL][4];
+    const bool more = kb + 16 * kMfUb < k1;
+    if (more) mfb_group<T, VEC, FWD, TL, TR>(Ab, xl, aok, xok, kb + 16 * kMfUb, k1, K, q, n, an, xn);
+#pragma unroll
+    for (int u = 0; u < kMfUb; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int a = 0; a < TL; ++a)
+#pragma unroll
+          for (int b = 0; b < TR; ++b)
+            acc[a][b][s] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xv[u][a][s], (double)av[u][b][s],
+                                                                acc[a][b][s], 0, 0, 0);
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < kMfUb; ++u) {
+#pragma unroll
+        for (int a = 0; a < TL; ++a)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) xv[u][a][s] = xn[u][a][s];
+#pragma unroll
+        for (int b = 0; b < TR; ++b)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) av[u][b][s] = an[u][b][s];
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < TL; ++a)
+#pragma unroll
+    for (int b = 0; b < TR; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        out[a][b][i] = ((acc[a][b][0][i] + acc[a][b][1][i]) + acc[a][b][2][i]) + acc[a][b][3][i];
+}
+
+// adds the NW waves' partial tiles in wave order (tile by tile, as
+// mf_reduce); true in wave 0
+template <int NW, int NT>
+__device__ __forceinline__ bool mfb_reduce(tv3_d4 *acc) {
+  __shared__ double red[NW - 1][NT][4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w > 0)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[w - 1][t][r][lane] = acc[t][r];
+  __syncthreads();
+  if (w > 0) return false;
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double v = acc[t][r];
+      for (int k = 0; k < NW - 1; ++k) v += red[k][t][r][lane];
+      acc[t][r] = v;
+    }
+  return true;
+}
+
+// Y (L x m) = X (L x n) A^T over TL x TR tiles of 16 layers x 16 rows.
+// Grid: (row tile groups, layer tile groups, groups)
+template <class T, bool VEC, int NW, int TL, int TR>
+__global__ __launch_bounds__(64 * NW) void k_dense_fwd_mfmab(const GroupDesc *G, const float *x,
+                                                             float *y, const InstScal *S,
+                                                             uint32_t mask) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m, L = gd.layers;
+  const int i0 = blockIdx.x * 16 * TR, l0 = blockIdx.y * 16 * TL;
+  if (i0 >= pm || l0 >= L) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, q = lane >> 4;
+  const T *Ar[TR];
+  bool rok[TR];
+#pragma unroll
+  for (int b = 0; b < TR; ++b) {
+    const int row = i0 + 16 * b + c;
+    rok[b] = row < pm;
+    Ar[b] = (const T *)gd.plan.A + (long long)(rok[b] ? row : 0) * n;
+  }
+  const float *xl[TL];
+  bool lok[TL];
+#pragma unroll
+  for (int a = 0; a < TL; ++a) {
+    const int lyr = l0 + 16 * a + c;
+    lok[a] = lyr < L;
+    xl[a] = x + gd.off_n + (long long)(lok[a] ? lyr : 0) * n;
+  }
+  int k0, k1;
+  mf_krange<NW>(n, w, k0, k1);
+  tv3_d4 acc[TL][TR];
+  mfb_walk<T, VEC, true, TL, TR>(Ar, xl, rok, lok, k0, k1, n, q, n, acc);
+  if (!mfb_reduce<NW, TL * TR>(&acc[0][0])) return;
+#pragma unroll
+  for (int a = 0; a < TL; ++a)
+#pragma unroll
+    for (int b = 0; b < TR; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int l = l0 + 16 * a + 4 * r + q, row = i0 + 16 * b + c;
+        if (l < L && row < pm) y[gd.off_m + (long long)l * pm + row] = (float)acc[a][b][r];
+      }
+}
+
+// G (L x n) = Y (L x m) A over TL x TR tiles of 16 layers x 16 columns,
+// minus sub.  Grid: (column tile groups, layer tile groups, groups)
+template <class T, bool VEC, int NW, int TL, int TR>
+__global__ __launch_bounds__(64 * NW) void k_dense_bwd_mfmab(const GroupDesc *G, const float *x,
+                                                             float *y, const InstScal *S,
+                                                             uint32_t mask, const float *sub) {
+  const GroupDesc &gd = G[blockIdx.z];
+  const int n = gd.plan.n, pm = gd.plan.m, L = gd.layers;
+  const int j0 = blockIdx.x * 16 * TR, l0 = blockIdx.y * 16 * TL;
+  if (j0 >= n || l0 >= L) return;
+  if (mask && (S[blockIdx.z].flags & mask) != mask) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane & 15, q = lane >> 4;
+  const T *Ac[TR];
+  bool cok[TR];
+#pragma unroll
+  for (int b = 0; b < TR; ++b) {
+    const int col = j0 + 16 * b + c;
+    cok[b] = col < n;
+    Ac[b] = (const T *)gd.plan.A + (cok[b] ? col : 0);
+  }
+  const float *xl[TL];
+  bool lok[TL];
+#pragma unroll
+  for (int a = 0; a < TL; ++a) {
+    const int lyr = l0 + 16 * a + c;
+    lok[a] = lyr < L;
+    xl[a] = x + gd.off_m + (long long)(lok[a] ? lyr : 0) * pm;
+  }
+  int k0, k1;
+  mf_krange<NW>(pm, w, k0, k1);
+  tv3_d4 acc[TL][TR];
+  mfb_walk<T, VEC, false, TL, TR>(Ac, xl, cok, lok, k0, k1, pm, q, n, acc);
+  if (!mfb_reduce<NW, TL * TR>(&acc[0][0])) return;
+#pragma unroll
+  for (int a = 0; a < TL; ++a)
+#pragma unroll
+    for (int b = 0; b < TR; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int l = l0 + 16 * a + 4 * r + q, col = j0 + 16 * b + c;
+        if (l < L && col < n) {
+          const long long o = gd.off_n + (long long)l * n + col;
+          float v = (float)acc[a][b][r];
+          if (sub) v = v - sub[o];
+          y[o] = v;
+        }
+      }
+}
+
+// output tiles per workgroup of the multi-layer GEMMs, for A X (f) and
+// A^T Y (b): 1 one 16 x 16 tile (k_dense_*_mfma), 2 two by two tiles, 3 (A X
+// only) two layer tiles by one row tile (k_dense_*_mfmab); TV3_MFMA_BLOCK=f,b
+// overrides.  Measured on 64 layers of 64 x 64 sharing a 1229 x 4096 A
+// (eight wavefronts per workgroup): A^T Y 42.4 -> 34.5 us with 2 x 2 tiles
+// (256 workgroups); A X 38.8 -> 57.7 us with 2 x 2 (78 workgroups: too few
+// to fill the chip) and 36.8 us with 2 x 1; sixteen wavefronts per
+// workgroup leave the 2 x 2 kernels 128 registers and spill them.  (Read
+// at every launch, so tests can switch it; the kernels give the same bits.)
+int mfma_block(bool fwd) {
+  int f = 3, b = 2;
+  if (const char *e = std::getenv("TV3_MFMA_BLOCK")) (void)std::sscanf(e, "%d,%d", &f, &b);
+  return fwd ? (f == 2 || f == 3 ? f : 1) : (b == 1 ? 1 : 2);
+}
+
 template <>
 void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
                    int kind, int dtype, const float *x, float *y,
@@ -1764,9 +2001,16 @@ void launch_fwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_m,
   if (kind == kDense && dense_tune_cur().mfma) {
     // tiles of the largest A and layer count; smaller groups exit early
     const DenseTune t = dense_tune_cur();
-    const dim3 g((t.mfma_m + 15) / 16, (t.mfma_layers + 15) / 16, G);
-#define TV3_MF(TT, VV, NW) \
-  hipLaunchKernelGGL((k_dense_fwd_mfma<TT, VV, NW>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask)
+    const int tb = mfma_block(true);
+    const int tl = tb == 1 ? 1 : 2, tr = tb == 2 ? 2 : 1;
+    const dim3 g((t.mfma_m + 16 * tr - 1) / (16 * tr), (t.mfma_layers + 16 * tl - 1) / (16 * tl), G);
+#define TV3_MF(TT, VV, NW)                                                                                    \
+  if (tb == 2)                                                                                              \
+    hipLaunchKernelGGL((k_dense_fwd_mfmab<TT, VV, NW, 2, 2>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask); \
+  else if (tb == 3)                                                                                         \
+    hipLaunchKernelGGL((k_dense_fwd_mfmab<TT, VV, NW, 2, 1>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask); \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_dense_fwd_mfma<TT, VV, NW>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask)
 #define TV3_MFW(TT, VV)                           \
   if (mfma_waves() == 4) TV3_MF(TT, VV, 4);       \
   else if (mfma_waves() == 16) TV3_MF(TT, VV, 16); \
@@ -1821,9 +2065,14 @@ void launch_bwd<1>(hipStream_t s, const GroupDesc *G_dev, int G, int max_n,
                    float *scratch, int submat_mb) {
   if (kind == kDense && dense_tune_cur().mfma) {
     const DenseTune t = dense_tune_cur();
-    const dim3 g((t.mfma_n + 15) / 16, (t.mfma_layers + 15) / 16, G);
-#define TV3_MB(TT, VV, NW) \
-  hipLaunchKernelGGL((k_dense_bwd_mfma<TT, VV, NW>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask, sub)
+    const int tb = mfma_block(false);
+    const dim3 g((t.mfma_n + 16 * tb - 1) / (16 * tb), (t.mfma_layers + 16 * tb - 1) / (16 * tb), G);
+#define TV3_MB(TT, VV, NW)                                                                                     \
+  if (tb == 2)                                                                                               \
+    hipLaunchKernelGGL((k_dense_bwd_mfmab<TT, VV, NW, 2, 2>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask, \
+                       sub);                                                                                 \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_dense_bwd_mfma<TT, VV, NW>), g, dim3(64 * NW), 0, s, G_dev, x, y, S, mask, sub)
 #define TV3_MBW(TT, VV)                           \
   if (mfma_waves() == 4) TV3_MB(TT, VV, 4);       \
   else if (mfma_waves() == 16) TV3_MB(TT, VV, 16); \
