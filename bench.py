@@ -569,7 +569,11 @@ def run_single(args, ctx, workload):
         dims = (q, L * p, 1)
         maxit = args.maxit if args.maxit_set else 300
         opts = tval3_amd.default_options(maxit=maxit, tv_iso=1, graph=args.graph)
-        kern = {"A": "k_dense_fwd_mfma", "At": "k_dense_bwd_mfma"}
+        # the register-blocked kernels (one-tile k_dense_*_mfma with
+        # TV3_MFMA_BLOCK=1,1)
+        blk = os.environ.get("TV3_MFMA_BLOCK", "3,2").split(",")
+        kern = {"A": "k_dense_fwd_mfma" + ("" if blk[0] == "1" else "b"),
+                "At": "k_dense_bwd_mfma" + ("" if len(blk) > 1 and blk[1] == "1" else "b")}
         op_fn = lambda: O.Op.dense(A)  # noqa: E731
         desc = (f"2D dense multi-layer: {L} layers of 64x64 sharing one A (m={m} per layer), "
                 f"isotropic TV, maxit {maxit}")
