@@ -1224,6 +1224,12 @@ __device__ __forceinline__ void loadc(const T *p, float (&v)[C]) {
   if constexpr (C == 8) load8(p, v); else if constexpr (C == 4) load4(p, v); else load2(p, v);
 }
 
+// acc + x y for float-valued x and y (float or bf16 entries, float vector
+// values widened to FP64): their product has at most 48 significant bits and
+// is exact in FP64, so one fused rounding gives the bits of the product
+// rounded, then added (the oracle's order), in one instruction
+__device__ __forceinline__ double prod_add(double acc, double x, double y) { return __builtin_fma(x, y, acc); }
+
 // y = A x: one wavefront per R consecutive rows; with n a multiple of 8 every
 // lane loads 8 consecutive entries of each row (16 bytes of bf16, 32 of f32)
 // and the matching 8 x values once for all R rows per step; FP64 lane sums,
@@ -1259,7 +1265,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
           load8(xr + j, v);
           load8(Ar + j, av);
 #pragma unroll
-          for (int u = 0; u < 8; ++u) a += (double)av[u] * (double)v[u];
+          for (int u = 0; u < 8; ++u) a = prod_add(a, (double)av[u], (double)v[u]);
         }
       } else {
         for (int j = lane; j < n; j += 64) a += (double)a_val(Ar, j) * (double)xr[j];
@@ -1286,7 +1292,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_fwd_w1(const GroupDesc *G,
           float a[8];
           load8(A + (long long)r * n + j, a);
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc[r] += (double)a[u] * (double)v[u];
+          for (int u = 0; u < 8; ++u) acc[r] = prod_add(acc[r], (double)a[u], (double)v[u]);
         }
       }
     }
@@ -1346,7 +1352,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_part(
 #pragma unroll
       for (int r = 0; r < U; ++r)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+        for (int u = 0; u < 8; ++u) acc[u] = prod_add(acc[u], (double)a[r][u], xi[r]);
     }
     for (; i < i1; ++i) {
       const double xi = (double)xg[i];
@@ -1423,7 +1429,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_blk(
 #pragma unroll
         for (int r = 0; r < U; ++r)
 #pragma unroll
-          for (int u = 0; u < C; ++u) acc[u] += (double)a[r][u] * xi[r];
+          for (int u = 0; u < C; ++u) acc[u] = prod_add(acc[u], (double)a[r][u], xi[r]);
       }
       if (i < y1) {
         // the block's last rows in one round, their loads in flight together
@@ -1442,7 +1448,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_blk(
         for (int r = 0; r < U; ++r)
           if (i + r < y1) {
 #pragma unroll
-            for (int u = 0; u < C; ++u) acc[u] += (double)a[r][u] * xi[r];
+            for (int u = 0; u < C; ++u) acc[u] = prod_add(acc[u], (double)a[r][u], xi[r]);
           }
       }
     } else {
@@ -1503,7 +1509,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_bs(
 #pragma unroll
       for (int r = 0; r < U; ++r)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+        for (int u = 0; u < 8; ++u) acc[u] = prod_add(acc[u], (double)a[r][u], xi[r]);
     }
     if (i < i1) {
       // the slice's last rows in one round
@@ -1520,7 +1526,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_bwd_bs(
       for (int r = 0; r < U; ++r)
         if (i + r < i1) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc[u] += (double)a[r][u] * xi[r];
+          for (int u = 0; u < 8; ++u) acc[u] = prod_add(acc[u], (double)a[r][u], xi[r]);
         }
     }
   } else {
