@@ -980,12 +980,19 @@ __device__ __forceinline__ void fin_sums(const FA &F, int inst,
   double x[NS];
 #pragma unroll
   for (int k = 0; k < NS; ++k) x[k] = 0.0;
-  // one partial per thread and slot: all loads are issued before any reduction
+  // one partial per thread and slot: all loads are issued before any
+  // reduction.  The loads are unconditional (every slot holds kNblkMax
+  // entries; those past the launch's block count are stale and not added):
+  // a load under a per-slot branch would wait for its value inside the
+  // branch, one memory round trip per slot.
   for (int i = threadIdx.x; i < kNblkMax; i += kFin) {
+    double t[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) t[k] = F.part[((long long)inst * kNslot + slots[k]) * kNblkMax + i];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
       const int nb = slots[k] >= kSlotM ? F.nblk_m : F.nblk_n;
-      if (i < nb) x[k] += F.part[((long long)inst * kNslot + slots[k]) * kNblkMax + i];
+      x[k] = i < nb ? x[k] + t[k] : x[k];
     }
   }
 #pragma unroll
