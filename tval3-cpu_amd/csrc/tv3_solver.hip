@@ -1012,9 +1012,12 @@ __device__ __forceinline__ void fin_sums(const FA &F, int inst,
 
 __device__ __forceinline__ void put_flags(const FA &F, int inst, uint32_t f) {
   if (F.fout) {
-    // one 64-bit store: a host that reads the launch's number reads its flags
-    F.fout[inst] = ((unsigned long long)F.epoch << 32) | f;
-    __threadfence_system();
+    // one 64-bit store: a host that reads the launch's number reads its
+    // flags.  A system-scope store goes to host memory by itself; the host
+    // reads nothing else the kernel wrote, so no fence (a write-back of the
+    // whole L2 that the block would wait for) orders other stores before it.
+    __hip_atomic_store(F.fout + inst, ((unsigned long long)F.epoch << 32) | f,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
