@@ -69,3 +69,35 @@ def test_hemisphere_batch_solves(ctx):
         Uo, info_o, _ = O.solve(op, bs[i], dims, opts, Ut=x)
         assert np.array_equal(Us[i].cpu().numpy(), Uo), i
         assert infos[i].total_iters == info_o.total_iters, i
+
+
+@pytest.mark.parametrize("kw", [dict(ls_exact=0), dict(ls_exact=1),
+                                dict(ls_exact=0, nonneg=1, scal_double=1),
+                                dict(ls_exact=1, tv_iso=1)])
+def test_four_instances_per_lane_passes(ctx, kw, monkeypatch):
+    """The 2-D float vector passes of 64-lane groups with four instances per
+    lane (16-byte accesses) against one instance per lane: every lane of a
+    130-instance ring batch (three groups, the last padded, lanes finishing
+    at different iterations) gives the same bits and iteration counts, and
+    sampled lanes agree with the oracle."""
+    geo, op, x, bs = _ring_batch(48, 48, 130, 300)
+    plan = ctx.plan_geometry(geo)
+    opts = tval3_amd.default_options(maxit=80, **kw)
+    res = {}
+    for v4 in ("0", "1"):
+        monkeypatch.setenv("TV3_VEC4", v4)
+        Us, infos = ctx.solve_batch([plan] * len(bs), bs, [(48, 48, 1)] * len(bs), opts)
+        res[v4] = (np.stack([u.cpu().numpy() for u in Us]), [i.total_iters for i in infos])
+    assert res["0"][1] == res["1"][1]
+    for k in range(len(bs)):
+        assert np.array_equal(res["0"][0][k], res["1"][0][k]), k
+    for i in (0, 77, 129):
+        Uo, info_o, _ = O.solve(op, bs[i], (48, 48, 1), opts)
+        if kw["ls_exact"]:
+            assert np.array_equal(res["1"][0][i], Uo), i
+            assert res["1"][1][i] == info_o.total_iters, i
+        else:
+            # the quadratic line search (ls_exact = 0) is held to the
+            # tolerance the single-instance tests give it
+            rel = np.linalg.norm(res["1"][0][i].astype(np.float64) - Uo) / np.linalg.norm(Uo)
+            assert rel <= 1e-4, (i, rel)

@@ -371,6 +371,164 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
   block_partials<W, 12>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
 }
 
+// Block partials of the four-instances-per-lane kernels (lane l holds
+// instances 4 (l mod 16) .. +3 of voxel slot l / 16): each wavefront adds
+// its four slots in order into its own LDS rows, then the four wavefronts
+// are added in order; one partial per computing instance and slot.
+template <int NS>
+__device__ __forceinline__ void block_partials4(const double (&acc)[4][NS], const int (&slot)[NS],
+                                                const KA &K) {
+  __shared__ double red[kBlock / 64][NS][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int quad = lane & 15, vs = lane >> 4;
+#pragma unroll
+  for (int sl = 0; sl < 4; ++sl) {
+    if (vs == sl) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          double &r = red[wv][s][4 * quad + k];
+          r = sl == 0 ? acc[k][s] : r + acc[k][s];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NS * 64; t += kBlock) {
+    const int s = t / 64, b = t % 64;
+    if (!sel(K.S[blockIdx.y * 64 + b].flags, K.req, K.forbid)) continue;
+    const double x = ((red[0][s][b] + red[1][s][b]) + red[2][s][b]) + red[3][s][b];
+    K.part[((long long)(blockIdx.y * 64 + b) * kNslot + slot[s]) * kNblkMax + blockIdx.x] = x;
+  }
+}
+
+// One element of k_descend for a 2-D volume (the same operations in the same
+// order as its !z3 path): U, Ux, Uy and g2 = Dt(V) at the voxel, and the
+// instance's dot-product terms (slots 0, 1, 3, 4, 6 and, with POLY, 7-11).
+template <class TS, bool POLY>
+__device__ __forceinline__ void descend2d_elem(vr_t tf, int nonneg, vec_t d0, vec_t p0,
+                                               vec_t dxn, vec_t pxn, vec_t dyn, vec_t pyn,
+                                               vec_t dxp, vec_t pxp, vec_t dyp, vec_t pyp,
+                                               vec_t wx, vec_t wy, vec_t sx, vec_t sy,
+                                               vec_t wxp, vec_t wyp, vec_t qx, vec_t qy,
+                                               double (&acc)[12], vec_t &u0, vec_t &ux,
+                                               vec_t &uy, vec_t &t) {
+  auto un = [&](vec_t dd, vec_t pp) -> vec_t {
+    vec_t u = vfma(tf, dd, pp);
+    if (nonneg) u = clamp0(u);
+    return u;
+  };
+  u0 = un(d0, p0);
+  ux = un(dxn, pxn) - u0;
+  uy = un(dyn, pyn) - u0;
+  const vec_t Vx0 = ux - wx;
+  const vec_t Vy0 = uy - wy;
+  acc[0] += dre(Vx0, Vx0);
+  acc[1] += dre(Vy0, Vy0);
+  acc[3] += dre(sx, Vx0);
+  acc[4] += dre(sy, Vy0);
+  acc[6] += dre(d0, d0);
+  if (POLY) {
+    const auto vx = to_vd(qx) - to_vd(wx), dx = to_vd(ux - qx);
+    const auto vy = to_vd(qy) - to_vd(wy), dy = to_vd(uy - qy);
+    const double A2 = dre(vx, vx) + dre(vy, vy), B2 = dre(vx, dx) + dre(vy, dy), C2 = dre(dx, dx) + dre(dy, dy);
+    const double A4 = dre(to_vd(sx), vx) + dre(to_vd(sy), vy);
+    const double B4 = dre(to_vd(sx), dx) + dre(to_vd(sy), dy);
+    acc[7] += A2; acc[8] += B2; acc[9] += C2; acc[10] += A4; acc[11] += B4;
+  }
+  const vec_t VXp = (u0 - un(dxp, pxp)) - wxp;
+  const vec_t VYp = (u0 - un(dyp, pyp)) - wyp;
+  t = VXp - Vx0;
+  t = t + VYp;
+  t = t - Vy0;
+}
+
+// k_descend for 64-lane groups of 2-D float instances, four instances per
+// lane: lane l of a wavefront takes instances 4 (l mod 16) .. +3 of voxel
+// slot l / 16, so every array access is one 16-byte load or store (a
+// wavefront moves four voxels' 256-byte rows per instruction; with one
+// float per lane the stencil's 16-18 loads per voxel were issue-bound).
+// The element arithmetic is descend2d_elem's; each instance's per-block
+// FP64 partial sums its voxels in a fixed order (rounds, then the four
+// slots of a wavefront, then the four wavefronts: block_partials4).
+template <class TS, bool POLY>
+__global__ __launch_bounds__(kBlock) void k_descend4(KA K, Arrs a, int nonneg) {
+  const GroupDesc &gd = K.G[blockIdx.y];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int quad = lane & 15, slot = lane >> 4;
+  const int i0 = blockIdx.y * 64 + 4 * quad;
+  bool on[4];
+  vr_t tf[4];
+  bool any = false, all = true;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const Scal &S = K.S[i0 + k];
+    on[k] = sel(S.flags, K.req, K.forbid);
+    tf[k] = (vr_t)(-1 * (TS)S.tau);
+    any = any || on[k];
+    all = all && on[k];
+  }
+  double acc[4][12];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int s = 0; s < 12; ++s) acc[k][s] = 0.0;
+  if (any) {
+    const int n = gd.n, q = gd.q, p = gd.p;
+    // element (v, instance 4 quad + k) of a group vector: row v, float4 quad
+    auto ld = [&](const vec_t *X, int v) -> float4 {
+      return *reinterpret_cast<const float4 *>(X + gd.off_n + (long long)v * 64 + 4 * quad);
+    };
+    for (int v = blockIdx.x * 16 + wv * 4 + slot; v < n; v += K.nblk * 16) {
+      const Vox w(v, q, p, 1);
+      const int xn = w.xn(), yn = w.yn(), xp = w.xp(), yp = w.yp();
+      const float4 d0 = ld(a.d, v), p0 = ld(a.Up, v);
+      const float4 dxn = ld(a.d, xn), pxn = ld(a.Up, xn);
+      const float4 dyn = ld(a.d, yn), pyn = ld(a.Up, yn);
+      const float4 dxp = ld(a.d, xp), pxp = ld(a.Up, xp);
+      const float4 dyp = ld(a.d, yp), pyp = ld(a.Up, yp);
+      const float4 wx = ld(a.Wx, v), wy = ld(a.Wy, v), sx = ld(a.sx, v), sy = ld(a.sy, v);
+      const float4 wxp = ld(a.Wx, xp), wyp = ld(a.Wy, yp);
+      float4 qx = make_float4(0.0f, 0.0f, 0.0f, 0.0f), qy = qx;
+      if (POLY) {
+        qx = ld(a.Uxp, v);
+        qy = ld(a.Uyp, v);
+      }
+      float4 u0, ux, uy, t;
+#define TV3_D4(k, c)                                                                        \
+  descend2d_elem<TS, POLY>(tf[k], nonneg, d0.c, p0.c, dxn.c, pxn.c, dyn.c, pyn.c, dxp.c, pxp.c, \
+                           dyp.c, pyp.c, wx.c, wy.c, sx.c, sy.c, wxp.c, wyp.c, qx.c, qy.c,       \
+                           acc[k], u0.c, ux.c, uy.c, t.c)
+      TV3_D4(0, x);
+      TV3_D4(1, y);
+      TV3_D4(2, z);
+      TV3_D4(3, w);
+#undef TV3_D4
+      const long long o = gd.off_n + (long long)v * 64 + 4 * quad;
+      if (all) {
+        *reinterpret_cast<float4 *>(a.U + o) = u0;
+        *reinterpret_cast<float4 *>(a.Ux + o) = ux;
+        *reinterpret_cast<float4 *>(a.Uy + o) = uy;
+        *reinterpret_cast<float4 *>(a.g2 + o) = t;
+      } else {
+        const float cu[4] = {u0.x, u0.y, u0.z, u0.w}, cx[4] = {ux.x, ux.y, ux.z, ux.w};
+        const float cy[4] = {uy.x, uy.y, uy.z, uy.w}, ct[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (on[k]) {
+            a.U[o + k] = cu[k];
+            a.Ux[o + k] = cx[k];
+            a.Uy[o + k] = cy[k];
+            a.g2[o + k] = ct[k];
+          }
+      }
+    }
+  }
+  const int sl[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+  block_partials4<12>(acc, sl, K);
+}
+
 // get_g's m-vector part: Aub = Au - b, lam3 = |Aub|^2, lam5 = <delta, Aub>,
 // and the norms of the A^T-skip test (2d:313-316)
 // poly != 0: also the quadratic coefficients of lam3(alpha), lam5(alpha)
@@ -756,6 +914,130 @@ __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
     const int sl[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
     block_partials<W, 10>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
   }
+}
+
+// One element of k_wstep for a 2-D float volume after the first iteration
+// (the same operations in the same order as its !INIT, !z3 path): W at the
+// voxel, g2 = Dt(V), d, and the instance's terms (slots 0-2, 4, 5, 7-9).
+// c0, cx, cy: (Ux, sx, Uy, sy) at the voxel, at x - 1 and at y - 1 (the
+// anisotropic shrinkage reads one direction of cx and cy only).
+template <class TS, bool ISO>
+__device__ __forceinline__ void wstep2d_elem(TS beta, TS ib, float mdb, const float (&c0)[4],
+                                             const float (&cx)[4], const float (&cy)[4], float up,
+                                             float uc, float gc, float gpv, float g2pv, float dts,
+                                             double (&acc)[10], float &wx0, float &wy0, float &t,
+                                             float &dn) {
+  // shrinkage of one cell (ux, sx, uy, sy); returns the lam1 term
+  auto shr = [&](const float (&c)[4], float &wx, float &wy) -> TS {
+    float wz;
+    if (ISO) return shrink_iso<TS>(c[0], c[2], 0.0f, c[1], c[3], 0.0f, false, beta, ib, wx, wy, wz);
+    TS tx, ty;
+    wx = shrink1<TS>(c[0], c[1], beta, ib, tx);
+    wy = shrink1<TS>(c[2], c[3], beta, ib, ty);
+    return tx + ty;
+  };
+  const TS l1 = shr(c0, wx0, wy0);
+  acc[0] += (double)l1;
+  const float Vx0 = c0[0] - wx0, Vy0 = c0[2] - wy0;
+  acc[1] += dre(Vx0, Vx0);
+  acc[2] += dre(Vy0, Vy0);
+  acc[4] += dre(c0[1], Vx0);
+  acc[5] += dre(c0[3], Vy0);
+  float tx, ty;
+  shr(cx, tx, ty);
+  const float VXp = cx[0] - tx;
+  shr(cy, tx, ty);
+  const float VYp = cy[2] - ty;
+  t = VXp - Vx0;
+  t = t + VYp;
+  t = t - Vy0;
+  const float uu = uc - up;
+  acc[7] += dre(up, up);
+  acc[8] += dre(uu, uu);
+  dn = vfma(mdb, gc, dts + t);
+  const float y = vfma(mdb, gc - gpv, t - g2pv);
+  acc[9] += dre(uu, y);
+}
+
+// k_wstep after the first iteration for 64-lane groups of 2-D float
+// instances, four instances per lane (the layout of k_descend4).
+template <class TS, bool ISO>
+__global__ __launch_bounds__(kBlock) void k_wstep4(KA K, Arrs a) {
+  const GroupDesc &gd = K.G[blockIdx.y];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int quad = lane & 15, slot = lane >> 4;
+  const int i0 = blockIdx.y * 64 + 4 * quad;
+  bool on[4];
+  TS beta[4], ib[4];
+  float mdb[4];
+  bool any = false, all = true;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const Scal &S = K.S[i0 + k];
+    on[k] = sel(S.flags, K.req, K.forbid);
+    beta[k] = (TS)S.beta;
+    ib[k] = (TS)1 / beta[k];
+    mdb[k] = (float)(vr_t)S.muDbeta;
+    any = any || on[k];
+    all = all && on[k];
+  }
+  double acc[4][10];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int s = 0; s < 10; ++s) acc[k][s] = 0.0;
+  if (any) {
+    const int n = gd.n, q = gd.q, p = gd.p;
+    auto ld = [&](const vec_t *X, int v) -> float4 {
+      return *reinterpret_cast<const float4 *>(X + gd.off_n + (long long)v * 64 + 4 * quad);
+    };
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int v = blockIdx.x * 16 + wv * 4 + slot; v < n; v += K.nblk * 16) {
+      const Vox w(v, q, p, 1);
+      const int xp = w.xp(), yp = w.yp();
+      const float4 ux0 = ld(a.Ux, v), sx0 = ld(a.sx, v), uy0 = ld(a.Uy, v), sy0 = ld(a.sy, v);
+      const float4 uxx = ld(a.Ux, xp), sxx = ld(a.sx, xp);
+      const float4 uyx = ISO ? ld(a.Uy, xp) : z4, syx = ISO ? ld(a.sy, xp) : z4;
+      const float4 uxy = ISO ? ld(a.Ux, yp) : z4, sxy = ISO ? ld(a.sx, yp) : z4;
+      const float4 uyy = ld(a.Uy, yp), syy = ld(a.sy, yp);
+      const float4 up = ld(a.Up, v), uc = ld(a.U, v), gc = ld(a.g, v);
+      const float4 gpv = ld(a.gp, v), g2pv = ld(a.g2p, v), dts = ld(a.DtsAtd, v);
+      float4 wx, wy, t, dn;
+#define TV3_W4(k, c)                                                                        \
+  {                                                                                         \
+    const float c0[4] = {ux0.c, sx0.c, uy0.c, sy0.c};                                       \
+    const float cx[4] = {uxx.c, sxx.c, uyx.c, syx.c};                                       \
+    const float cy[4] = {uxy.c, sxy.c, uyy.c, syy.c};                                       \
+    wstep2d_elem<TS, ISO>(beta[k], ib[k], mdb[k], c0, cx, cy, up.c, uc.c, gc.c, gpv.c,      \
+                          g2pv.c, dts.c, acc[k], wx.c, wy.c, t.c, dn.c);                    \
+  }
+      TV3_W4(0, x);
+      TV3_W4(1, y);
+      TV3_W4(2, z);
+      TV3_W4(3, w);
+#undef TV3_W4
+      const long long o = gd.off_n + (long long)v * 64 + 4 * quad;
+      if (all) {
+        *reinterpret_cast<float4 *>(a.Wx + o) = wx;
+        *reinterpret_cast<float4 *>(a.Wy + o) = wy;
+        *reinterpret_cast<float4 *>(a.g2 + o) = t;
+        *reinterpret_cast<float4 *>(a.d + o) = dn;
+      } else {
+        const float cwx[4] = {wx.x, wx.y, wx.z, wx.w}, cwy[4] = {wy.x, wy.y, wy.z, wy.w};
+        const float ct[4] = {t.x, t.y, t.z, t.w}, cd[4] = {dn.x, dn.y, dn.z, dn.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (on[k]) {
+            a.Wx[o + k] = cwx[k];
+            a.Wy[o + k] = cwy[k];
+            a.g2[o + k] = ct[k];
+            a.d[o + k] = cd[k];
+          }
+      }
+    }
+  }
+  const int sl[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+  block_partials4<10>(acc, sl, K);
 }
 
 // d = DtsAtd + g2 + muDbeta*g, with DtsAtd = (beta0/beta) d first when
@@ -1618,6 +1900,13 @@ long long vec_blocks() {
   return v > 0 ? v : 2048;
 }
 
+// TV3_VEC4=0: 64-lane 2-D float batches run the vector passes with one
+// instance per lane instead of four (k_descend4)
+bool vec4_passes() {
+  const char *e = std::getenv("TV3_VEC4");
+  return !(e && e[0] == '0');
+}
+
 // TV3_SPIN=0: host-driven solves read the flags after a stream
 // synchronization instead of polling the published launch numbers
 bool spin_wait() {
@@ -2177,6 +2466,16 @@ class Engine {
     auto go = [&](auto kern) {
       TV3_GO(kProfDescend, kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg, O_.dt_exact);
     };
+    // four instances per lane for the step of every active instance (POLY);
+    // the masked fallback launches keep one instance per lane, where lanes
+    // that skip would split each 16-byte store into four (measured 393 ->
+    // 417 us per launch at B = 512, against 386 -> 332 us for POLY)
+    if constexpr (W == 64 && std::is_same<vec_t, float>::value) {
+      if (!bt_.z3 && poly && vec4_passes()) {
+        TV3_GO(kProfDescend, (k_descend4<TS, true>), gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg);
+        return;
+      }
+    }
     if (bt_.z3)
       poly ? go(k_descend<W, TS, true, true>) : go(k_descend<W, TS, true, false>);
     else
@@ -2189,6 +2488,15 @@ class Engine {
     auto go = [&](auto kern) {
       TV3_GO(kProfShrink, kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.dt_exact);
     };
+    if constexpr (W == 64 && !INIT && std::is_same<vec_t, float>::value) {
+      if (!bt_.z3 && vec4_passes()) {
+        auto go4 = [&](auto kern) {
+          TV3_GO(kProfShrink, kern, gn(), dim3(kBlock), 0, s_, kn(req), a_);
+        };
+        O_.tv_iso ? go4(k_wstep4<TS, true>) : go4(k_wstep4<TS, false>);
+        return;
+      }
+    }
     if constexpr (kCplx) {  // anisotropic only (check_opts)
       bt_.z3 ? go(k_wstep<W, TS, INIT, true, false>) : go(k_wstep<W, TS, INIT, false, false>);
     } else {
