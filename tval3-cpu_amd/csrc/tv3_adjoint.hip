@@ -249,7 +249,7 @@ struct tv3_tape {
   }
   int dotm(const float *x, const float *y, double *out) {
     const int nb = nblk;
-    nblk = (int)std::min<long long>(kNblkMax, (m + kBlock - 1) / kBlock);
+    nblk = (int)std::min<long long>(kNblkGroup, (m + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_ad_dot, grid(), dim3(kBlock), 0, st, m, x, y, part);
     int rc = err();
     if (!rc) rc = host_sum(1, out);
@@ -260,7 +260,7 @@ struct tv3_tape {
   // d = F(u); r = A u - bs kept in r
   int F(const float *u, float *d, float *r) {
     fwdA(u, r);
-    hipLaunchKernelGGL(k_ad_axpby, dim3(std::min(kNblkMax, (m + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL(k_ad_axpby, dim3(std::min(kNblkGroup, (m + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
                        m, 1.0f, r, -1.0f, bs, r);
     bwdA(r, tn1);
     hipLaunchKernelGGL(k_ad_F, grid(), dim3(kBlock), 0, st, q, p, u, tn1, (float)mr,
@@ -304,7 +304,7 @@ int make_tape(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *o, int q, int 
   t->ctx = ctx; t->A = A; t->q = q; t->p = p; t->n = q * p; t->m = A->dev.m;
   t->K = K; t->ck = ck; t->mu = o->mu; t->beta = o->beta; t->mr = o->mu / o->beta;
   t->st = ctx->stream;
-  t->nblk = (int)std::min<long long>(kNblkMax, (t->n + kBlock - 1) / kBlock);
+  t->nblk = (int)std::min<long long>(kNblkGroup, (t->n + kBlock - 1) / kBlock);
   const int ncp = K / ck + 1;
   const size_t n = t->n, m = t->m;
   const size_t dense = A->dev.kind == kDense ? 16 * n : 1;
@@ -390,7 +390,7 @@ int tv3_unrolled_forward(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opt
   t->imax = imax; t->imin = imin; t->bdif_signed = bd;
   if (m > 0 && bdif < 0.5) { t->scl_case = 1; t->th = 0.5; t->scl = 0.5 / bdif; }
   else if (m > 0 && bdif > 1.5) { t->scl_case = 2; t->th = 1.5; t->scl = 1.5 / bdif; }
-  hipLaunchKernelGGL(k_ad_axpby, dim3(std::min(kNblkMax, (m + kBlock - 1) / kBlock)), dim3(kBlock), 0, t->st,
+  hipLaunchKernelGGL(k_ad_axpby, dim3(std::min(kNblkGroup, (m + kBlock - 1) / kBlock)), dim3(kBlock), 0, t->st,
                      m, (float)t->scl, b, 0.0f, nullptr, t->bs);
   // u_0, d_0
   float *u_prev = t->tn2, *d_prev = nullptr;  // u_{k-2}, d_{k-2} (valid from k = 2)
@@ -435,7 +435,7 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
   int rc = hip_check(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if (rc) return rc;
   const int n = t->n, m = t->m, K = t->K, ck = t->ck;
-  const dim3 gm(std::min(kNblkMax, (m + kBlock - 1) / kBlock));
+  const dim3 gm(std::min(kNblkGroup, (m + kBlock - 1) / kBlock));
   const float ib = (float)(1.0 / t->beta);
   const float mr = (float)t->mr;
   double mbar = 0;

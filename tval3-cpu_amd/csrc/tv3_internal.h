@@ -21,7 +21,10 @@
 namespace tv3 {
 
 constexpr int kBlock = 256;       // threads per block of every vector kernel
-constexpr int kNblkMax = 512;     // max blocks per group of a reduction kernel
+constexpr int kNblkMax = 2048;    // partial-sum entries per slot: max blocks of a
+                                  // single-instance (W = 1) reduction kernel
+constexpr int kNblkGroup = 512;   // max blocks per group of a 16/64-lane reduction
+                                  // kernel, and of the stand-alone operations
 constexpr int kNslot = 24;        // partial-sum slots per instance
 constexpr int kSlotM = 12;        // first slot written by m-vector kernels
 // Im <s, y> of the complex Barzilai-Borwein step: written by the same pass as

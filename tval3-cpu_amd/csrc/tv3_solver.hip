@@ -1264,10 +1264,11 @@ __device__ __forceinline__ void fin_sums(const FA &F, int inst,
   for (int k = 0; k < NS; ++k) x[k] = 0.0;
   // one partial per thread and slot: all loads are issued before any
   // reduction.  The loads are unconditional (every slot holds kNblkMax
-  // entries; those past the launch's block count are stale and not added):
+  // entries; those past the slot's block count are stale and not added):
   // a load under a per-slot branch would wait for its value inside the
   // branch, one memory round trip per slot.
-  for (int i = threadIdx.x; i < kNblkMax; i += kFin) {
+  const int nbmax = F.nblk_n > F.nblk_m ? F.nblk_n : F.nblk_m;
+  for (int i = threadIdx.x; i < nbmax; i += kFin) {
     double t[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) t[k] = F.part[((long long)inst * kNslot + slots[k]) * kNblkMax + i];
@@ -1971,14 +1972,16 @@ class Engine {
     graph_ = o.graph != 0;
     isreal_ = o.isreal != 0;
     submat_mb_ = o.submat_mb > 0 ? o.submat_mb : 5;
-    // blocks per group of the vector passes: at most kNblkMax (512 measured
-    // best on the 256 ring at B = 128 and 256: 1024 gave 503 / 559 recons/s,
-    // 512 gave 511 / 579), and about vec_blocks() over the whole batch, at
-    // least 128 per group
+    // blocks per group of the vector passes: about vec_blocks() over the
+    // whole batch, at least 128 per group, at most kNblkGroup for 16/64-lane
+    // groups (512 measured best on the 256 ring at B = 128 and 256: 1024 gave
+    // 503 / 559 recons/s, 512 gave 511 / 579) and kNblkMax for one instance
+    // (a 128^3 volume then runs 8 wavefronts per SIMD instead of 2)
+    const long long cap = W == 1 ? kNblkMax : kNblkGroup;
     const long long per = std::max<long long>(128, vec_blocks() / std::max(bt.G, 1));
-    nblk_n_ = (int)std::min<long long>(std::min<long long>(kNblkMax, per),
+    nblk_n_ = (int)std::min<long long>(std::min<long long>(cap, per),
                                        std::max<long long>(1, ((long long)bt.max_n * W + kBlock - 1) / kBlock));
-    nblk_m_ = (int)std::min<long long>(std::min<long long>(kNblkMax, per),
+    nblk_m_ = (int)std::min<long long>(std::min<long long>(cap, per),
                                        std::max<long long>(1, ((long long)bt.max_m * W + kBlock - 1) / kBlock));
   }
 
@@ -2933,7 +2936,7 @@ int op_scratch(tv3_ctx *ctx, const GroupDesc &gd, double beta, OpScratch &sc,
 }
 
 int nblk_for(long long n) {
-  return (int)std::min<long long>(kNblkMax, std::max<long long>(1, (n + kBlock - 1) / kBlock));
+  return (int)std::min<long long>(kNblkGroup, std::max<long long>(1, (n + kBlock - 1) / kBlock));
 }
 
 int host_partial_sum(tv3_ctx *ctx, const double *part, int slot, int nblk, double *out) {
