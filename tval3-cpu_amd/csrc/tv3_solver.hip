@@ -262,6 +262,9 @@ __global__ __launch_bounds__(kBlock) void k_msq(KA K, const vec_t *x) {
 // quadratic coefficients of lam2(alpha) and lam4(alpha): V_dir(alpha) =
 // (U_dir_p - W_dir) + alpha dU_dir with dU_dir = U_dir - U_dir_p (slots 7-11).
 // D3: the batch holds 3-D volumes (each group still tests its own dim_z).
+// k_descend4 (64-lane 2-D float groups, four instances per lane) repeats the
+// 2-D arithmetic of this kernel in descend2d_elem; the GPU tests compare the
+// two bit for bit.
 //
 // The vector kernels read every value of an element before they write any:
 // the arrays are distinct, but the compiler cannot prove it and would keep
@@ -768,6 +771,9 @@ __global__ __launch_bounds__(kBlock) void k_D(KA K, const vec_t *U, vec_t *Ux,
 // update_W (2d:651-671) + get_gradient + stop-test norms + get_tau's
 // <s, s>, <s, y> of the next iteration
 // ---------------------------------------------------------------------------
+// k_wstep4 (64-lane 2-D float groups after the first iteration, four
+// instances per lane) repeats the 2-D arithmetic of this kernel in
+// wstep2d_elem; the GPU tests compare the two bit for bit.
 template <int W, class TS, bool INIT, bool D3, bool ISO>
 __global__ __launch_bounds__(kBlock) void k_wstep(KA K, Arrs a, int exact) {
   // complex: slot kSlotSyIm also takes Im <s, y>
