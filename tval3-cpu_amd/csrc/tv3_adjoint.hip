@@ -46,7 +46,8 @@ size_t ad_align(size_t x) { return (x + 255) & ~(size_t)255; }
   for (int v = blockIdx.x * kBlock + threadIdx.x; v < (n); v += gridDim.x * kBlock)
 
 // fixed-order block sum of NS values: wavefront butterfly, then the four
-// wavefronts in order; thread 0 writes part[s * kNblkMax + blockIdx.x]
+// wavefronts in order; thread 0 writes part[s * kNblkGroup + blockIdx.x]
+// (the tape's launches take at most kNblkGroup blocks)
 template <int NS>
 __device__ __forceinline__ void ad_block_sum(double (&acc)[NS], double *part) {
   __shared__ double sh[NS][kBlock / 64];
@@ -62,7 +63,7 @@ __device__ __forceinline__ void ad_block_sum(double (&acc)[NS], double *part) {
     for (int s = 0; s < NS; ++s) {
       double t = 0;
       for (int w = 0; w < kBlock / 64; ++w) t += sh[s][w];
-      part[s * kNblkMax + blockIdx.x] = t;
+      part[s * kNblkGroup + blockIdx.x] = t;
     }
   }
 }
@@ -230,14 +231,14 @@ struct tv3_tape {
     prof_count_host(ctx, gh, 1, 4);
   }
   int host_sum(int slots, double *out) {
-    std::vector<double> h((size_t)slots * kNblkMax);
-    int rc = hip_check(ctx, hipMemcpyAsync(h.data(), part, sizeof(double) * slots * kNblkMax,
+    std::vector<double> h((size_t)slots * kNblkGroup);
+    int rc = hip_check(ctx, hipMemcpyAsync(h.data(), part, sizeof(double) * slots * kNblkGroup,
                                            hipMemcpyDeviceToHost, st), "partials copy");
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(st), "partials sync");
     if (rc) return rc;
     for (int s = 0; s < slots; ++s) {
       double t = 0;
-      for (int i = 0; i < nblk; ++i) t += h[(size_t)s * kNblkMax + i];
+      for (int i = 0; i < nblk; ++i) t += h[(size_t)s * kNblkGroup + i];
       out[s] = t;
     }
     return TV3_OK;
@@ -309,7 +310,7 @@ int make_tape(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *o, int q, int 
   const size_t n = t->n, m = t->m;
   const size_t dense = A->dev.kind == kDense ? 16 * n : 1;
   const size_t bytes = ad_align(sizeof(GroupDesc)) + ad_align(sizeof(InstScal)) +
-                       ad_align(sizeof(double) * 2 * kNblkMax) + ad_align(sizeof(double) * dense) +
+                       ad_align(sizeof(double) * 2 * kNblkGroup) + ad_align(sizeof(double) * dense) +
                        ad_align(sizeof(float) * m) * 4 + ad_align(sizeof(float) * n * 4 * ncp) +
                        ad_align(sizeof(float) * n * (ck + 2)) * 2 + ad_align(sizeof(float) * m * (ck + 2)) +
                        ad_align(sizeof(float) * n) * 8;
@@ -319,7 +320,7 @@ int make_tape(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *o, int q, int 
   auto take = [&](size_t b) { char *r = c; c += ad_align(b); return (void *)r; };
   t->G = (GroupDesc *)take(sizeof(GroupDesc));
   t->S = (InstScal *)take(sizeof(InstScal));
-  t->part = (double *)take(sizeof(double) * 2 * kNblkMax);
+  t->part = (double *)take(sizeof(double) * 2 * kNblkGroup);
   t->dense = (double *)take(sizeof(double) * dense);
   t->bs = (float *)take(sizeof(float) * m);
   t->tm1 = (float *)take(sizeof(float) * m);
