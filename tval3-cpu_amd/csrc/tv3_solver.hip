@@ -374,6 +374,21 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
   block_partials<W, 12>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
 }
 
+// 16-byte store of a lane's four instances.  Where some of them skip, their
+// old values are read back and stored unchanged, so that the row is still
+// written whole (four 4-byte stores left partly written sectors, which the
+// memory completes with a read of its own).
+__device__ __forceinline__ void store4_sel(vec_t *p, float4 v, const bool (&on)[4], bool all) {
+  if (!all) {
+    const float4 old = *reinterpret_cast<const float4 *>(p);
+    v.x = on[0] ? v.x : old.x;
+    v.y = on[1] ? v.y : old.y;
+    v.z = on[2] ? v.z : old.z;
+    v.w = on[3] ? v.w : old.w;
+  }
+  *reinterpret_cast<float4 *>(p) = v;
+}
+
 // Block partials of the four-instances-per-lane kernels (lane l holds
 // instances 4 (l mod 16) .. +3 of voxel slot l / 16): each wavefront adds
 // its four slots in order into its own LDS rows, then the four wavefronts
@@ -509,23 +524,10 @@ __global__ __launch_bounds__(kBlock) void k_descend4(KA K, Arrs a, int nonneg) {
       TV3_D4(3, w);
 #undef TV3_D4
       const long long o = gd.off_n + (long long)v * 64 + 4 * quad;
-      if (all) {
-        *reinterpret_cast<float4 *>(a.U + o) = u0;
-        *reinterpret_cast<float4 *>(a.Ux + o) = ux;
-        *reinterpret_cast<float4 *>(a.Uy + o) = uy;
-        *reinterpret_cast<float4 *>(a.g2 + o) = t;
-      } else {
-        const float cu[4] = {u0.x, u0.y, u0.z, u0.w}, cx[4] = {ux.x, ux.y, ux.z, ux.w};
-        const float cy[4] = {uy.x, uy.y, uy.z, uy.w}, ct[4] = {t.x, t.y, t.z, t.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (on[k]) {
-            a.U[o + k] = cu[k];
-            a.Ux[o + k] = cx[k];
-            a.Uy[o + k] = cy[k];
-            a.g2[o + k] = ct[k];
-          }
-      }
+      store4_sel(a.U + o, u0, on, all);
+      store4_sel(a.Ux + o, ux, on, all);
+      store4_sel(a.Uy + o, uy, on, all);
+      store4_sel(a.g2 + o, t, on, all);
     }
   }
   const int sl[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
@@ -1023,23 +1025,10 @@ __global__ __launch_bounds__(kBlock) void k_wstep4(KA K, Arrs a) {
       TV3_W4(3, w);
 #undef TV3_W4
       const long long o = gd.off_n + (long long)v * 64 + 4 * quad;
-      if (all) {
-        *reinterpret_cast<float4 *>(a.Wx + o) = wx;
-        *reinterpret_cast<float4 *>(a.Wy + o) = wy;
-        *reinterpret_cast<float4 *>(a.g2 + o) = t;
-        *reinterpret_cast<float4 *>(a.d + o) = dn;
-      } else {
-        const float cwx[4] = {wx.x, wx.y, wx.z, wx.w}, cwy[4] = {wy.x, wy.y, wy.z, wy.w};
-        const float ct[4] = {t.x, t.y, t.z, t.w}, cd[4] = {dn.x, dn.y, dn.z, dn.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (on[k]) {
-            a.Wx[o + k] = cwx[k];
-            a.Wy[o + k] = cwy[k];
-            a.g2[o + k] = ct[k];
-            a.d[o + k] = cd[k];
-          }
-      }
+      store4_sel(a.Wx + o, wx, on, all);
+      store4_sel(a.Wy + o, wy, on, all);
+      store4_sel(a.g2 + o, t, on, all);
+      store4_sel(a.d + o, dn, on, all);
     }
   }
   const int sl[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
@@ -2476,9 +2465,9 @@ class Engine {
       TV3_GO(kProfDescend, kern, gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg, O_.dt_exact);
     };
     // four instances per lane for the step of every active instance (POLY);
-    // the masked fallback launches keep one instance per lane, where lanes
-    // that skip would split each 16-byte store into four (measured 393 ->
-    // 417 us per launch at B = 512, against 386 -> 332 us for POLY)
+    // the masked fallback launches, where most lanes skip, keep one instance
+    // per lane (four per lane measured 348 -> 354 us per launch at B = 448,
+    // against 386 -> 332 us for POLY at B = 512)
     if constexpr (W == 64 && std::is_same<vec_t, float>::value) {
       if (!bt_.z3 && poly && vec4_passes()) {
         TV3_GO(kProfDescend, (k_descend4<TS, true>), gn(), dim3(kBlock), 0, s_, kn(req), a_, O_.nonneg);
