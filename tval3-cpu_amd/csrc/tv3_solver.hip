@@ -374,8 +374,9 @@ __global__ __launch_bounds__(kBlock) void k_descend(KA K, Arrs a, int nonneg,
   block_partials<W, 12>(acc, sl, K.part, blockIdx.y * W, blockIdx.x, K.S, K.req, K.forbid);
 }
 
-// 16-byte store of a lane's four instances.  Where some of them skip, their
-// old values are read back and stored unchanged, so that the row is still
+// 16-byte store of a lane's four instances.  all: every instance either
+// computed or may be overwritten (finished).  Otherwise the old values of the
+// skipping ones are read back and stored unchanged, so that the row is still
 // written whole (four 4-byte stores left partly written sectors, which the
 // memory completes with a read of its own).
 __device__ __forceinline__ void store4_sel(vec_t *p, float4 v, const bool (&on)[4], bool all) {
@@ -485,7 +486,10 @@ __global__ __launch_bounds__(kBlock) void k_descend4(KA K, Arrs a, int nonneg) {
     on[k] = sel(S.flags, K.req, K.forbid);
     tf[k] = (vr_t)(-1 * (TS)S.tau);
     any = any || on[k];
-    all = all && on[k];
+    // a skipping instance that has finished (or pads the group) keeps no
+    // state here: its result went to Uout when it finished, and no later
+    // pass reads its working arrays, so its lane may be overwritten
+    all = all && (on[k] || !(S.flags & F_ACTIVE));
   }
   double acc[4][12];
 #pragma unroll
@@ -987,7 +991,10 @@ __global__ __launch_bounds__(kBlock) void k_wstep4(KA K, Arrs a) {
     ib[k] = (TS)1 / beta[k];
     mdb[k] = (float)(vr_t)S.muDbeta;
     any = any || on[k];
-    all = all && on[k];
+    // a skipping instance that has finished (or pads the group) keeps no
+    // state here: its result went to Uout when it finished, and no later
+    // pass reads its working arrays, so its lane may be overwritten
+    all = all && (on[k] || !(S.flags & F_ACTIVE));
   }
   double acc[4][10];
 #pragma unroll
