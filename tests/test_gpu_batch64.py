@@ -101,3 +101,34 @@ def test_four_instances_per_lane_passes(ctx, kw, monkeypatch):
             # tolerance the single-instance tests give it
             rel = np.linalg.norm(res["1"][0][i].astype(np.float64) - Uo) / np.linalg.norm(Uo)
             assert rel <= 1e-4, (i, rel)
+
+
+def test_finished_instances_in_four_instance_passes(ctx, monkeypatch):
+    """Instances that finish tens of iterations apart (noise from 0 to 5%):
+    once an instance has finished, the four-instances-per-lane passes
+    overwrite its lane of the working arrays.  Every lane's result and
+    iteration count equals the one-instance-per-lane kernels', and sampled
+    early and late finishers equal the oracle."""
+    geo = synth.ring_geometry(48, 48)
+    op = O.Op.geometry(geo)
+    x = synth.disk_phantom(48, seed=301)
+    y = op.gemv(x)
+    B = 130
+    bs = [synth.add_noise(y, 0.05 * (i % 13) / 12, 400 + i) if i % 13 else y.copy() for i in range(B)]
+    plan = ctx.plan_geometry(geo)
+    opts = tval3_amd.default_options(maxit=150, tol=1e-3, tol_inn=1e-3, ls_exact=1)
+    res = {}
+    for v4 in ("0", "1"):
+        monkeypatch.setenv("TV3_VEC4", v4)
+        Us, infos = ctx.solve_batch([plan] * B, bs, [(48, 48, 1)] * B, opts)
+        res[v4] = (np.stack([u.cpu().numpy() for u in Us]), [i.total_iters for i in infos])
+    iters = res["1"][1]
+    assert res["0"][1] == iters
+    assert max(iters) - min(iters) >= 30, (min(iters), max(iters))
+    for k in range(B):
+        assert np.array_equal(res["0"][0][k], res["1"][0][k]), k
+    early, late = int(np.argmin(iters)), int(np.argmax(iters))
+    for i in sorted({early, late, 64, 129}):
+        Uo, info_o, _ = O.solve(op, bs[i], (48, 48, 1), opts)
+        assert np.array_equal(res["1"][0][i], Uo), i
+        assert iters[i] == info_o.total_iters, i
