@@ -183,6 +183,7 @@ struct tv3_tape {
   GroupDesc gh{};  // host copy of *G (the instance is always live here)
   InstScal *S = nullptr;
   double *part = nullptr, *dense = nullptr;
+  double *hpart = nullptr;  // pinned host copy of part (allocated at the first sum)
   float *bs = nullptr, *ckp = nullptr;      // ckp: checkpoints, 4 n-vectors each
   float *su = nullptr, *sd = nullptr, *sr = nullptr;  // segment states (ck + 2)
   float *tn1 = nullptr, *tn2 = nullptr, *tm1 = nullptr, *tm2 = nullptr;
@@ -230,15 +231,25 @@ struct tv3_tape {
     prof_end(ctx);
     prof_count_host(ctx, gh, 1, 4);
   }
+  // the block partials of `slots` sums, added on the host in block order;
+  // the copy lands in pinned memory (a pageable target is staged through a
+  // bounce buffer on every call)
   int host_sum(int slots, double *out) {
-    std::vector<double> h((size_t)slots * kNblkGroup);
-    int rc = hip_check(ctx, hipMemcpyAsync(h.data(), part, sizeof(double) * slots * kNblkGroup,
-                                           hipMemcpyDeviceToHost, st), "partials copy");
+    if (!hpart) {
+      void *h = nullptr;
+      int rc0 = hip_check(ctx, hipHostMalloc(&h, sizeof(double) * 2 * kNblkGroup, hipHostMallocDefault),
+                          "partials hipHostMalloc");
+      if (rc0) return rc0;
+      hpart = (double *)h;
+    }
+    const size_t cnt = (size_t)(slots - 1) * kNblkGroup + nblk;
+    int rc = hip_check(ctx, hipMemcpyAsync(hpart, part, sizeof(double) * cnt, hipMemcpyDeviceToHost, st),
+                       "partials copy");
     if (!rc) rc = hip_check(ctx, hipStreamSynchronize(st), "partials sync");
     if (rc) return rc;
     for (int s = 0; s < slots; ++s) {
       double t = 0;
-      for (int i = 0; i < nblk; ++i) t += h[(size_t)s * kNblkGroup + i];
+      for (int i = 0; i < nblk; ++i) t += hpart[(size_t)s * kNblkGroup + i];
       out[s] = t;
     }
     return TV3_OK;
@@ -559,6 +570,7 @@ void tv3_tape_destroy(tv3_tape *t) {
     (void)hipStreamSynchronize(t->st);
     (void)hipFree(t->mem);
   }
+  if (t->hpart) (void)hipHostFree(t->hpart);
   delete t;
 }
 
