@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -184,6 +186,7 @@ struct tv3_tape {
   InstScal *S = nullptr;
   double *part = nullptr, *dense = nullptr;
   double *hpart = nullptr;  // pinned host copy of part (allocated at the first sum)
+  hipEvent_t hev = nullptr;  // marks the copy into hpart (polled, see wait_copy)
   float *bs = nullptr, *ckp = nullptr;      // ckp: checkpoints, 4 n-vectors each
   float *su = nullptr, *sd = nullptr, *sr = nullptr;  // segment states (ck + 2)
   float *tn1 = nullptr, *tn2 = nullptr, *tm1 = nullptr, *tm2 = nullptr;
@@ -231,6 +234,31 @@ struct tv3_tape {
     prof_end(ctx);
     prof_count_host(ctx, gh, 1, 4);
   }
+  // waits for the tape's stream to pass the copy into hpart: an event after
+  // it is polled (a stream synchronization parks the host thread and costs
+  // its wake-up on every dot product); TV3_SPIN=0, or no completion within
+  // two seconds, synchronizes the stream instead
+  int wait_copy() {
+    static const bool spin = [] {
+      const char *e = std::getenv("TV3_SPIN");
+      return !(e && e[0] == '0');
+    }();
+    if (!spin) return hip_check(ctx, hipStreamSynchronize(st), "partials sync");
+    int rc = TV3_OK;
+    if (!hev) rc = hip_check(ctx, hipEventCreateWithFlags(&hev, hipEventDisableTiming), "partials event");
+    if (!rc) rc = hip_check(ctx, hipEventRecord(hev, st), "partials event");
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spins = 1;; ++spins) {
+      const hipError_t e = hipEventQuery(hev);
+      if (e == hipSuccess) return TV3_OK;
+      if (e != hipErrorNotReady) return hip_check(ctx, e, "partials event");
+      if ((spins & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2))
+        return hip_check(ctx, hipStreamSynchronize(st), "partials sync");
+      __builtin_ia32_pause();
+    }
+  }
+
   // the block partials of `slots` sums, added on the host in block order;
   // the copy lands in pinned memory (a pageable target is staged through a
   // bounce buffer on every call)
@@ -245,7 +273,7 @@ struct tv3_tape {
     const size_t cnt = (size_t)(slots - 1) * kNblkGroup + nblk;
     int rc = hip_check(ctx, hipMemcpyAsync(hpart, part, sizeof(double) * cnt, hipMemcpyDeviceToHost, st),
                        "partials copy");
-    if (!rc) rc = hip_check(ctx, hipStreamSynchronize(st), "partials sync");
+    if (!rc) rc = wait_copy();
     if (rc) return rc;
     for (int s = 0; s < slots; ++s) {
       double t = 0;
@@ -571,6 +599,7 @@ void tv3_tape_destroy(tv3_tape *t) {
     (void)hipFree(t->mem);
   }
   if (t->hpart) (void)hipHostFree(t->hpart);
+  if (t->hev) (void)hipEventDestroy(t->hev);
   delete t;
 }
 
