@@ -87,6 +87,18 @@ __global__ __launch_bounds__(kBlock) void k_ad_axpby(int n, float a, const float
   AD_LOOP(n) out[v] = fmaf(a, x[v], y ? b * y[v] : 0.0f);
 }
 
+// one backward step's carries: ub1 += ubk, db1 += c ubk, then ubk = 0 (the
+// two k_ad_axpby launches and the fill they replace, same operations)
+__global__ __launch_bounds__(kBlock) void k_ad_carry(int n, float c, float *ubk, float *ub1,
+                                                     float *db1) {
+  AD_LOOP(n) {
+    const float x = ubk[v];
+    ub1[v] = fmaf(1.0f, x, 1.0f * ub1[v]);
+    db1[v] = fmaf(c, x, 1.0f * db1[v]);
+    ubk[v] = 0.0f;
+  }
+}
+
 // d = Dt(clamp(D u, 1/beta)) + m g
 __global__ __launch_bounds__(kBlock) void k_ad_F(int q, int p, const float *u, const float *g,
                                                  float m, float ib, float *d) {
@@ -534,11 +546,8 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
       double tbar;
       if ((rc = t->dot(d1, t->ubar(k), &tbar))) return rc;
       tbar *= -al;
-      hipLaunchKernelGGL(k_ad_axpby, t->grid(), dim3(kBlock), 0, t->st, n, 1.0f, t->ubar(k), 1.0f,
-                         t->ubar(k - 1), t->ubar(k - 1));
-      hipLaunchKernelGGL(k_ad_axpby, t->grid(), dim3(kBlock), 0, t->st, n, (float)(-al * tau),
-                         t->ubar(k), 1.0f, t->dbr(k - 1), t->dbr(k - 1));
-      t->zero(t->ubar(k), n);
+      hipLaunchKernelGGL(k_ad_carry, t->grid(), dim3(kBlock), 0, t->st, n, (float)(-al * tau),
+                         t->ubar(k), t->ubar(k - 1), t->dbr(k - 1));
       if (t->first[i]) {
         // tau = N / Dn: d-bar += tbar (2/Dn) (d - tau (Dt D d + m A^T A d)); m-bar -= tbar tau |Ad|^2 / Dn
         const double Dn = t->tDn[i];
