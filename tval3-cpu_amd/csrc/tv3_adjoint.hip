@@ -87,6 +87,20 @@ __global__ __launch_bounds__(kBlock) void k_ad_axpby(int n, float a, const float
   AD_LOOP(n) out[v] = fmaf(a, x[v], y ? b * y[v] : 0.0f);
 }
 
+// up to four n-vector copies in one launch (checkpoint saves and segment
+// restores); a null source skips its pair
+struct Copy4 {
+  float *dst[4];
+  const float *src[4];
+};
+__global__ __launch_bounds__(kBlock) void k_ad_copy4(int n, Copy4 c) {
+  AD_LOOP(n) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (c.src[i]) c.dst[i][v] = c.src[i][v];
+  }
+}
+
 // one backward step's carries: ub1 += ubk, db1 += c ubk, then ubk = 0 (the
 // two k_ad_axpby launches and the fill they replace, same operations)
 __global__ __launch_bounds__(kBlock) void k_ad_carry(int n, float c, float *ubk, float *ub1,
@@ -452,10 +466,8 @@ int tv3_unrolled_forward(tv3_ctx *ctx, const tv3_plan *A, const tv3_options *opt
   // rolling states in the segment buffers: slots 0, 1, 2 reused
   int slot = 0;
   auto save_ck = [&](int c, const float *u1, const float *d1, const float *u0, const float *d0) {
-    t->cp(t->ck_vec(c, 0), u1, n);
-    t->cp(t->ck_vec(c, 1), d1, n);
-    if (u0) t->cp(t->ck_vec(c, 2), u0, n);
-    if (d0) t->cp(t->ck_vec(c, 3), d0, n);
+    const Copy4 cc{{t->ck_vec(c, 0), t->ck_vec(c, 1), t->ck_vec(c, 2), t->ck_vec(c, 3)}, {u1, d1, u0, d0}};
+    hipLaunchKernelGGL(k_ad_copy4, t->grid(), dim3(kBlock), 0, t->st, n, cc);
   };
   save_ck(0, u_cur, d_cur, nullptr, nullptr);
   for (int k = 1; k <= K; ++k) {
@@ -520,11 +532,12 @@ int tv3_unrolled_backward(tv3_ctx *ctx, tv3_tape *t, const float *dU, float *db,
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int k0 = sg * ck, k1 = std::min(K, k0 + ck);
     // recompute u_{k0-1} .. u_{k1}: segment index j = k - k0 + 1
-    t->cp(t->seg_u(1), t->ck_vec(sg, 0), n);
-    t->cp(t->seg_d(1), t->ck_vec(sg, 1), n);
-    if (k0 > 0) {
-      t->cp(t->seg_u(0), t->ck_vec(sg, 2), n);
-      t->cp(t->seg_d(0), t->ck_vec(sg, 3), n);
+    {
+      const bool prev = k0 > 0;
+      const Copy4 cc{{t->seg_u(1), t->seg_d(1), t->seg_u(0), t->seg_d(0)},
+                     {t->ck_vec(sg, 0), t->ck_vec(sg, 1), prev ? t->ck_vec(sg, 2) : nullptr,
+                      prev ? t->ck_vec(sg, 3) : nullptr}};
+      hipLaunchKernelGGL(k_ad_copy4, t->grid(), dim3(kBlock), 0, t->st, n, cc);
     }
     // r_{k0} = A u_{k0} - bs
     t->fwdA(t->seg_u(1), t->seg_r(1));
